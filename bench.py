@@ -1,0 +1,208 @@
+"""Benchmark: MarchingCubesSearch corner evals/s + tris/s on one MI355X.
+
+A step is one MarchingCubesSearch (model3d/mc.go:45-49) of the workload on
+the GPU, results left in HBM (device_only); the solid is lowered and uploaded
+inside the step.  Default workload: the seeded 64-primitive CSG tree clipped
+to [-1, 1]^3 at delta 0.005 (402^3 ~ 65M corners), iters 8.
+
+Prints one JSON line (see DESIGN.md for every field).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MarchingCubesSearch corner evals/s + tris/s, FP32+guard vs FP64, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+
+def workload(name):
+    from model3d_amd import workloads
+
+    if name == "csg64":
+        c = workloads.CSG64_CONFIG
+        return workloads.csg64_solid(c["seed"], c["n_prims"]), c
+    if name == "sphere":
+        return workloads.sphere_solid(), workloads.SPHERE_CONFIG
+    raise SystemExit(f"unknown workload {name}")
+
+
+def run_gpu(solid, cfg, precision, cull, steps, warmup):
+    import model3d_amd as m3
+    from model3d_amd import _lib
+    from model3d_amd.solids import LibBackend
+
+    L = _lib.lib()
+    ctx = _lib.context(0)
+    b = LibBackend()
+    h = solid._build(b)
+    opts = _lib.mc3d_options()
+    opts.precision = precision
+    opts.cull = 1 if cull else 0
+    opts.device_only = 1
+    stats = []
+
+    def step(record):
+        out = ctypes.c_void_p()
+        _lib.check(
+            L.mc3d_marching_cubes_search(ctx, h, cfg["delta"], cfg["iters"], ctypes.byref(opts), ctypes.byref(out)),
+            "mc3d_marching_cubes_search",
+        )
+        if record:
+            st = _lib.mc3d_stats()
+            L.mc3d_mesh_stats(out, ctypes.byref(st))
+            stats.append(st.as_dict())
+        L.mc3d_mesh_free(out)
+
+    for _ in range(warmup):
+        step(False)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(True)
+    wall = time.perf_counter() - t0
+    b.release(h)
+    b.close()
+    return wall, stats
+
+
+def mean(stats, key):
+    return sum(s[key] for s in stats) / max(1, len(stats))
+
+
+def cpu_baseline(solid, cfg):
+    from oracle.binding import oracle_search
+
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count() or 1
+    threads = max(1, min(threads, 16))
+    t0 = time.perf_counter()
+    tris, info = oracle_search(solid, cfg["delta"], cfg["iters"], threads=threads)
+    wall = time.perf_counter() - t0
+    return {
+        "value": info["corners"] / wall,
+        "unit": "corner evals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"the full workload once ({info['corners']} corners, {len(tris)} triangles), C restatement of "
+        f"mc.go with its plane-pipeline/parallel-search structure, {wall:.2f} s",
+        "tris_per_s": len(tris) / wall,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="csg64", choices=["csg64", "sphere"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            "bench.py measures one GPU only: this build runs on single-GPU boxes and has no multi-GPU path "
+            "(z-slab / per-solid sharding is a design, see DESIGN.md)",
+            file=sys.stderr,
+        )
+        raise SystemExit(2)
+
+    from model3d_amd import _lib
+
+    solid, cfg = workload(args.workload)
+    wall, st = run_gpu(solid, cfg, _lib.MC3D_PREC_FP32_GUARD, True, args.steps, args.warmup)
+    corners, tris, verts = st[0]["corners"], st[0]["triangles"], st[0]["vertices"]
+    ms = 1e3 * wall / args.steps
+    result = {
+        "metric": METRIC,
+        "value": corners / (wall / args.steps),
+        "unit": "corner evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{cfg['name']}_iters{cfg['iters']}",
+            "corners": corners,
+            "vertices": verts,
+            "triangles": tris,
+            "bisection_evals": st[0]["bisection_evals"],
+            "precision": "fp32_guard",
+            "cull": True,
+        },
+        "tris_per_s": tris / (wall / args.steps),
+    }
+    stages = {k: mean(st, k) for k in ("ms_corners", "ms_classify", "ms_emit", "ms_bisect", "ms_device")}
+    result["stages_ms"] = stages
+    ge, gr = mean(st, "guard_evals"), mean(st, "guard_rechecks")
+    result["guard"] = {"fp32_evals": ge, "fp64_rechecks": gr, "fallback_fraction": (gr / ge) if ge else None}
+
+    if not args.no_fp64:
+        wall64, st64 = run_gpu(solid, cfg, _lib.MC3D_PREC_FP64, False, max(1, args.steps // 2), 1)
+        ms64 = 1e3 * wall64 / max(1, args.steps // 2)
+        result["fp64_plain"] = {
+            "corner_evals_per_s": corners / (ms64 / 1e3),
+            "tris_per_s": tris / (ms64 / 1e3),
+            "ms_per_step": ms64,
+            "stages_ms": {k: mean(st64, k) for k in stages},
+        }
+        result["speedup_vs_fp64_plain"] = ms64 / ms
+
+    # Roofline of the dominant stage (HIP events on the library's stream).
+    dom = max(("ms_corners", "ms_classify", "ms_emit", "ms_bisect"), key=lambda k: stages[k])
+    result["roofline"] = roofline(dom, stages[dom], st[0])
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(solid, cfg)
+    print(json.dumps(result), flush=True)
+
+
+def roofline(stage, ms, st):
+    """Algorithmic bytes of the dominant stage per launch / its duration.
+
+    corners : writes the occupancy grid, 1 bit per corner  -> N/8 bytes
+    classify: reads the grid ~4x (row + 3 neighbour rows), writes 8 B counts
+              per word, scans them                          -> ~N/2 + 4*N/64*8
+    emit    : reads grid + offsets, writes 8 B per vertex key and 12 B per
+              triangle                                     -> N/8*4 + 8V + 12T
+    bisect  : reads 8 B key, writes 24 B per vertex         -> 32 V
+    """
+    n, v, t = st["corners"], st["vertices"], st["triangles"]
+    words = n / 64
+    algo = {
+        "ms_corners": n / 8,
+        "ms_classify": 4 * n / 8 + words * 8 * 3,
+        "ms_emit": 4 * n / 8 + words * 8 + 8 * v + 12 * t,
+        "ms_bisect": 32 * v,
+    }[stage]
+    gbs = algo / (ms * 1e-3) / 1e9
+    return {
+        "bound": "hbm",
+        "kernel": {"ms_corners": "k_corners", "ms_classify": "k_classify", "ms_emit": "k_emit",
+                   "ms_bisect": "k_bisect"}[stage],
+        "achieved": gbs,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": gbs / HBM_PEAK_GBS,
+        "traffic": None,
+        "avg_ms": ms,
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,175 @@
+/*
+ * mc3d.h -- C-ABI of the MI355X marching-cubes library (libmc3d.so).
+ *
+ * Drop-in boundary for model3d's solid-to-mesh path:
+ *   func MarchingCubesSearch(s Solid, delta float64, iters int) *Mesh
+ *     (model3d/mc.go:45-49)
+ * A caller lowers its Solid tree through the constructors below (one per
+ * reference Solid type), then calls mc3d_marching_cubes_search, which runs
+ * the whole path on the GPU and returns vertices (float64 xyz) and
+ * triangles (uint32 vertex-index triples).  Triangle i of the reference mesh
+ * is {verts[tris[3i]], verts[tris[3i+1]], verts[tris[3i+2]]}.
+ *
+ * All entry points return an int status (MC3D_OK == 0) or a handle (NULL on
+ * failure); none throws or aborts across the ABI.  mc3d_last_error() gives
+ * the message of the calling thread's last failure.  A context is not
+ * thread-safe: use one per host thread (one per goroutine in Go).  Solid
+ * handles are immutable and may be shared between threads and contexts.
+ */
+#ifndef MC3D_H
+#define MC3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    MC3D_OK = 0,
+    MC3D_E_INVALID_ARG = 1,
+    /* panic("invalid bounds: min=%#v max=%#v"), model3d/bounder.go:48-52 */
+    MC3D_E_INVALID_BOUNDS = 2,
+    /* panic("solid is true outside of bounds"), model3d/mc.go:687-689 */
+    MC3D_E_INSIDE_BOUNDARY = 3,
+    /* panic("vertex not on edge"), model3d/mc.go:659 (unreachable: every
+     * vertex is produced on its own grid edge) */
+    MC3D_E_NOT_ON_EDGE = 4,
+    MC3D_E_HIP = 5,
+    MC3D_E_OOM = 6,
+    MC3D_E_UNSUPPORTED = 7,
+    /* a host Contains callback reported failure */
+    MC3D_E_CALLBACK = 8,
+};
+
+typedef struct mc3d_ctx mc3d_ctx;
+typedef struct mc3d_solid mc3d_solid;
+typedef struct mc3d_mesh mc3d_mesh;
+
+/* ---- context ----------------------------------------------------------- */
+/* Binds a HIP device and a stream; owns grow-only device work buffers. */
+int mc3d_ctx_create(int device, mc3d_ctx **out);
+void mc3d_ctx_destroy(mc3d_ctx *ctx);
+
+/* ---- solids (model3d's Solid implementations) -------------------------- */
+/* Each constructor returns a new handle with reference count 1.  Composite
+ * constructors take their own reference on each child, so callers may
+ * release children right after.  Vectors are double[3] {x, y, z}. */
+
+/* &Sphere{Center, Radius}, model3d/shapes.go:13-31 */
+mc3d_solid *mc3d_sphere(const double center[3], double radius);
+/* &Rect{MinVal, MaxVal}, model3d/shapes.go:145-173 */
+mc3d_solid *mc3d_rect(const double min[3], const double max[3]);
+/* &Capsule{P1, P2, Radius}, model3d/shapes.go:348-368 */
+mc3d_solid *mc3d_capsule(const double p1[3], const double p2[3], double radius);
+/* &Cylinder{P1, P2, Radius}, model3d/shapes.go:542-597 */
+mc3d_solid *mc3d_cylinder(const double p1[3], const double p2[3], double radius);
+/* &Cone{Tip, Base, Radius}, model3d/shapes.go:859-898 */
+mc3d_solid *mc3d_cone(const double tip[3], const double base[3], double radius);
+/* &ConeSlice{P1, P2, R1, R2}, model3d/shapes.go:1039-1084 */
+mc3d_solid *mc3d_cone_slice(const double p1[3], const double p2[3], double r1, double r2);
+/* &Torus{Center, Axis, OuterRadius, InnerRadius}, model3d/shapes.go:1333-1365 */
+mc3d_solid *mc3d_torus(const double center[3], const double axis[3], double outer, double inner);
+/* JoinedSolid{...}, model3d/solid.go:70-96 (n >= 1) */
+mc3d_solid *mc3d_joined(mc3d_solid *const *children, size_t n);
+/* &SubtractedSolid{Positive, Negative}, model3d/solid.go:118-140 */
+mc3d_solid *mc3d_subtracted(mc3d_solid *positive, mc3d_solid *negative);
+/* IntersectedSolid{...}, model3d/solid.go:142-170 (n >= 1) */
+mc3d_solid *mc3d_intersected(mc3d_solid *const *children, size_t n);
+
+/* Elementary transforms of model3d/transform.go.  A list of them is a
+ * JoinedTransform (transform.go:191-219), applied left to right. */
+enum {
+    MC3D_XF_TRANSLATE = 1, /* &Translate{Offset}: v[0..2] */
+    MC3D_XF_SCALE = 2,     /* &Scale{Scale}: v[0] */
+    MC3D_XF_VECSCALE = 3,  /* &VecScale{Scale}: v[0..2] */
+    MC3D_XF_MATRIX = 4,    /* &Matrix3Transform{Matrix} (also Rotation, Mirror): v[0..8] row-major */
+};
+typedef struct {
+    int32_t kind;
+    int32_t pad;
+    double v[9];
+} mc3d_xform;
+/* TransformSolid(JoinedTransform(xforms), s), model3d/transform.go:240-246 */
+mc3d_solid *mc3d_transform(mc3d_solid *child, const mc3d_xform *xforms, size_t n);
+/* ForceSolidBounds(s, min, max), model3d/solid.go:307-309; CacheSolidBounds
+ * (solid.go:316-318) is mc3d_force_bounds(s, s.Min(), s.Max()). */
+mc3d_solid *mc3d_force_bounds(mc3d_solid *child, const double min[3], const double max[3]);
+/* NewColliderSolid(MeshToCollider(mesh)), model3d/solid.go:256-258 and
+ * model3d/collisions.go:138-142: ray-parity Contains over the triangles
+ * (9 doubles each: three xyz vertices). */
+mc3d_solid *mc3d_collider_solid(const double *triangles, size_t n_triangles);
+
+/* A Solid the library cannot lower (a Go closure, a user type): Contains is
+ * asked of the host in batches.  fn must write out[i] = Contains(p_i) for
+ * the n points pts[3i..3i+2] and return 0, or nonzero to abort the search
+ * with MC3D_E_CALLBACK.  It is called only from the thread that called
+ * mc3d_marching_cubes_search / mc3d_contains. */
+typedef int (*mc3d_contains_fn)(void *user, const double *pts, size_t n, uint8_t *out);
+mc3d_solid *mc3d_host_solid(const double min[3], const double max[3], mc3d_contains_fn fn, void *user);
+
+mc3d_solid *mc3d_solid_retain(mc3d_solid *s);
+void mc3d_solid_release(mc3d_solid *s);
+/* Solid.Min() / Solid.Max() with the reference's float64 arithmetic. */
+int mc3d_solid_bounds(const mc3d_solid *s, double min[3], double max[3]);
+
+/* ---- meshing ----------------------------------------------------------- */
+enum {
+    MC3D_PREC_FP64 = 0,       /* every primitive evaluated in float64 */
+    MC3D_PREC_FP32_GUARD = 1, /* float32 with an error band; in-band points re-decided in float64 */
+};
+typedef struct {
+    int32_t precision;   /* MC3D_PREC_* */
+    int32_t cull;        /* 1: per-brick interval culling of primitives */
+    int32_t device_only; /* 1: leave results on the device; mc3d_mesh_copy fetches them */
+    int32_t count_mismatch; /* 1: also count float32 sign mismatches (slower; statistics only) */
+    int32_t brick;       /* brick edge in corners for culling (0: default) */
+    int32_t reserved[3];
+} mc3d_options;
+
+typedef struct {
+    uint64_t corners;          /* |Xs| * |Ys| * |Zs| */
+    uint64_t vertices;         /* V */
+    uint64_t triangles;        /* T */
+    uint64_t bisection_evals;  /* V * (iters + 1) when iters > 0, else 0 */
+    uint64_t guard_evals;      /* float32 primitive evaluations */
+    uint64_t guard_rechecks;   /* of those, re-decided in float64 */
+    uint64_t guard_mismatches; /* float32 decision != float64 decision (count_mismatch only) */
+    uint64_t bricks_total, bricks_active;
+    double ms_device;          /* GPU time of the whole pipeline (HIP events) */
+    double ms_corners, ms_classify, ms_emit, ms_bisect; /* per stage (HIP events) */
+    double ms_d2h;             /* copy of results to host */
+    double ms_host_callbacks;  /* time spent in mc3d_contains_fn */
+} mc3d_stats;
+
+/* MarchingCubesSearch(s, delta, iters), model3d/mc.go:45-49 (iters == 0 is
+ * MarchingCubes, mc.go:14-37).  opts may be NULL (FP32 guard + culling). */
+int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
+                               const mc3d_options *opts, mc3d_mesh **out);
+int mc3d_mesh_counts(const mc3d_mesh *m, size_t *n_vertices, size_t *n_triangles);
+/* Copies the results to host arrays of 3*V doubles and 3*T uint32. */
+int mc3d_mesh_copy(mc3d_mesh *m, double *verts, uint32_t *tris);
+/* Device pointers of the results (valid until the mesh is freed). */
+int mc3d_mesh_device_ptrs(const mc3d_mesh *m, const double **verts, const uint32_t **tris);
+int mc3d_mesh_stats(const mc3d_mesh *m, mc3d_stats *out);
+void mc3d_mesh_free(mc3d_mesh *m);
+
+/* Solid.Contains for n host points on the GPU (out[i] = 0/1). */
+int mc3d_contains(mc3d_ctx *ctx, const mc3d_solid *s, const double *pts, size_t n, int32_t precision,
+                  uint8_t *out);
+
+/* The 256-case marching cubes table the kernels use (mcLookupTable,
+ * model3d/mc.go:434-456): out[16*k] = triangle count of case k, then up to
+ * 15 cube-edge ids (3 per triangle, CCW from outside).  Edge id e: axis
+ * e/4 (0=x, 1=y, 2=z), and the two remaining corner bits in e%4. */
+int mc3d_case_table(uint8_t out[256 * 16]);
+
+const char *mc3d_last_error(void);
+const char *mc3d_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

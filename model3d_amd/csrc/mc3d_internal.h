@@ -1,0 +1,116 @@
+// mc3d_internal.h -- shared between the host-side solid lowering (solid.cpp)
+// and the device pipeline (mc3d.hip).
+#pragma once
+
+#include <stdint.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/mc3d.h"
+
+namespace mc3d {
+
+// ---------------------------------------------------------------------------
+// Device program: a postfix boolean program over "atoms".  An atom is one
+// primitive Contains (or a bounds check) evaluated at the query point pushed
+// through the atom's transform frame.  Ops are pure, so the evaluation order
+// is free; the host orders children Sethi-Ullman style to keep the boolean
+// stack shallow (it is a bit stack in one 32-bit register).
+// ---------------------------------------------------------------------------
+enum Op : uint32_t {
+    OP_ATOM = 1,    // push atom(kind, params@y, frame)
+    OP_OR = 2,      // a b -> a|b
+    OP_AND = 3,     // a b -> a&b
+    OP_ANDNOT = 4,  // a b -> a & !b      (SubtractedSolid: positive below negative)
+    OP_RANDNOT = 5, // a b -> !a & b      (SubtractedSolid with children swapped)
+    OP_NOT = 6,     // a -> !a
+    OP_TRUE = 7,
+    OP_FALSE = 8,
+};
+
+enum AtomKind : uint32_t {
+    A_SPHERE = 1,     // cx cy cz r S_r          (S_r: largest s with sqrt(s) <= r)
+    A_RECT = 2,       // minx miny minz maxx maxy maxz
+    A_CAPSULE = 3,    // s0(3) s1(3) v(3) norm r S_r
+    A_CYLINDER = 4,   // p2(3) dir(3) norm r S_r
+    A_CONE = 5,       // base(3) dir(3) norm r
+    A_CONESLICE = 6,  // p1(3) dir(3) norm r1 r2
+    A_TORUS = 7,      // center(3) b1(3) b2(3) outer inner
+    A_COLLIDER = 8,   // collider index (as double), min(3) max(3)
+    A_HOST = 9,       // host atom index (as double)
+};
+
+// Instruction: x = op | kind << 8 | frame << 16; y = param offset (doubles).
+struct Insn {
+    uint32_t x, y;
+};
+#if defined(__HIPCC__)
+#define MC3D_HD __host__ __device__
+#else
+#define MC3D_HD
+#endif
+MC3D_HD static inline uint32_t insn_op(uint32_t x) { return x & 0xffu; }
+MC3D_HD static inline uint32_t insn_kind(uint32_t x) { return (x >> 8) & 0xffu; }
+MC3D_HD static inline uint32_t insn_frame(uint32_t x) { return x >> 16; }
+
+// Frame: chain of inverse elementary transforms from world space, stored in
+// the double param array as records of 10 doubles {kind, v[9]}.
+struct Frame {
+    uint32_t offset;  // in doubles
+    uint32_t nops;
+};
+
+// Conservative world-space AABB of where an atom can be true (used only to
+// cull atoms that are certainly false in a brick; never changes a result).
+struct AtomBox {
+    double lo[3], hi[3];
+};
+
+struct ColliderData {
+    std::vector<double> tris;   // 9 doubles per triangle
+    double mn[3], mx[3];
+};
+
+struct HostAtom {
+    mc3d_contains_fn fn;
+    void *user;
+};
+
+struct Program {
+    std::vector<Insn> code;
+    std::vector<double> f64;
+    std::vector<float> f32;        // float copies of f64 (guard-band path)
+    std::vector<Frame> frames;
+    std::vector<uint32_t> atom_pc; // pc of each atom instruction
+    std::vector<AtomBox> atom_box; // per atom, world space
+    std::vector<const ColliderData *> colliders;
+    std::vector<HostAtom> host_atoms;
+    int max_stack = 0;
+};
+
+struct Solid;
+using SolidPtr = std::shared_ptr<const Solid>;
+
+// Lowers a solid tree to a device program.  Returns false (with msg) when a
+// node cannot be represented.
+bool lower_solid(const Solid &s, Program &out, std::string &msg);
+void solid_bounds(const Solid &s, double mn[3], double mx[3]);
+bool bounds_valid(const Solid &s);
+
+// 256-case table (mcLookupTable, model3d/mc.go:434-456), 16 bytes per case:
+// count, then up to 15 cube-edge ids.
+const uint8_t *case_table();
+
+// Square spacer axis values (newSquareSpacer, model3d/mc.go:594-608).
+std::vector<double> spacer_axis(double lo, double hi, double delta);
+
+void set_error(const std::string &msg);
+
+}  // namespace mc3d
+
+struct mc3d_solid {
+    mc3d::SolidPtr ptr;
+    int refs;
+};

@@ -1,0 +1,126 @@
+"""MarchingCubesSearch and its result mesh (model3d/mc.go:14-49).
+
+The reference returns a *Mesh, a set of *Triangle.  Here a mesh is the
+indexed form the GPU produces: float64 vertices (V, 3) and uint32 triangles
+(T, 3); ``triangles()`` expands it to the reference's (T, 3, 3) triangle
+list with vertex order unchanged.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .solids import LibBackend, Solid
+
+
+class Panic(RuntimeError):
+    """A reference panic (invalid bounds, inside-boundary solid, ...)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+@dataclass
+class Mesh:
+    vertices: np.ndarray  # (V, 3) float64
+    faces: np.ndarray  # (T, 3) uint32
+    stats: dict = field(default_factory=dict)
+
+    def triangles(self) -> np.ndarray:
+        """(T, 3, 3) float64: triangle i = vertices[faces[i]]."""
+        return self.vertices[self.faces.astype(np.int64)]
+
+    def NumTriangles(self) -> int:
+        return int(len(self.faces))
+
+    def Volume(self) -> float:
+        """Mesh.Volume (measurements.go:18-29): |sum det(t)/6|."""
+        t = self.triangles()
+        a, b, c = t[:, 0], t[:, 1], t[:, 2]
+        det = (
+            a[:, 0] * (b[:, 1] * c[:, 2] - b[:, 2] * c[:, 1])
+            - a[:, 1] * (b[:, 0] * c[:, 2] - b[:, 2] * c[:, 0])
+            + a[:, 2] * (b[:, 0] * c[:, 1] - b[:, 1] * c[:, 0])
+        )
+        return float(abs(np.sum(det / 6.0)))
+
+
+def _options(precision, cull, device_only=False, count_mismatch=False, brick=0):
+    o = _lib.mc3d_options()
+    o.precision = precision
+    o.cull = 1 if cull else 0
+    o.device_only = 1 if device_only else 0
+    o.count_mismatch = 1 if count_mismatch else 0
+    o.brick = brick
+    return o
+
+
+def MarchingCubesSearch(
+    s: Solid,
+    delta: float,
+    iters: int,
+    *,
+    precision: int = _lib.MC3D_PREC_FP32_GUARD,
+    cull: bool = True,
+    device: int = 0,
+    count_mismatch: bool = False,
+) -> Mesh:
+    """model3d.MarchingCubesSearch (mc.go:45-49) on the GPU."""
+    L = _lib.lib()
+    ctx = _lib.context(device)
+    b = LibBackend()
+    h = s._build(b)
+    out = ctypes.c_void_p()
+    try:
+        opts = _options(precision, cull, count_mismatch=count_mismatch)
+        rc = L.mc3d_marching_cubes_search(ctx, h, float(delta), int(iters), ctypes.byref(opts), ctypes.byref(out))
+        if rc != _lib.MC3D_OK:
+            msg = _lib.last_error()
+            if rc in (_lib.MC3D_E_INVALID_BOUNDS, _lib.MC3D_E_INSIDE_BOUNDARY, _lib.MC3D_E_NOT_ON_EDGE):
+                raise Panic(rc, msg)
+            raise _lib.MC3DError(rc, msg)
+        nv, nt = ctypes.c_size_t(), ctypes.c_size_t()
+        _lib.check(L.mc3d_mesh_counts(out, ctypes.byref(nv), ctypes.byref(nt)), "mc3d_mesh_counts")
+        verts = np.empty((nv.value, 3), dtype=np.float64)
+        faces = np.empty((nt.value, 3), dtype=np.uint32)
+        _lib.check(L.mc3d_mesh_copy(out, verts.ctypes.data, faces.ctypes.data), "mc3d_mesh_copy")
+        st = _lib.mc3d_stats()
+        _lib.check(L.mc3d_mesh_stats(out, ctypes.byref(st)), "mc3d_mesh_stats")
+        return Mesh(verts, faces, st.as_dict())
+    finally:
+        if out.value:
+            L.mc3d_mesh_free(out)
+        b.release(h)
+        b.close()
+
+
+def MarchingCubes(s: Solid, delta: float, **kw) -> Mesh:
+    """model3d.MarchingCubes (mc.go:14-37): MarchingCubesSearch with iters=0."""
+    return MarchingCubesSearch(s, delta, 0, **kw)
+
+
+def Contains(s: Solid, pts, *, precision: int = _lib.MC3D_PREC_FP64, device: int = 0) -> np.ndarray:
+    """Solid.Contains for an (n, 3) array of points, evaluated on the GPU."""
+    p = np.ascontiguousarray(np.asarray(pts, dtype=np.float64).reshape(-1, 3))
+    out = np.zeros(len(p), dtype=np.uint8)
+    L = _lib.lib()
+    ctx = _lib.context(device)
+    b = LibBackend()
+    h = s._build(b)
+    try:
+        _lib.check(L.mc3d_contains(ctx, h, p.ctypes.data, len(p), precision, out.ctypes.data), "mc3d_contains")
+    finally:
+        b.release(h)
+        b.close()
+    return out.astype(bool)
+
+
+def case_table() -> np.ndarray:
+    """The library's 256 x 16 case table (count, then cube-edge ids)."""
+    t = np.zeros(256 * 16, dtype=np.uint8)
+    _lib.check(_lib.lib().mc3d_case_table(t.ctypes.data), "mc3d_case_table")
+    return t.reshape(256, 16)

@@ -1,0 +1,117 @@
+"""Seeded synthetic solids for the benchmark and the parity tests.
+
+Go's math/rand stream is not available outside Go, so the generators use
+splitmix64 (Steele, Lea & Flood, "Fast splittable pseudorandom number
+generators", OOPSLA 2014): state += 0x9E3779B97F4A7C15, then the xor-shift
+multiply finaliser; a float64 in [0, 1) is the top 53 bits times 2^-53.
+"""
+from __future__ import annotations
+
+import math
+
+from .solids import (
+    Capsule,
+    ClipSolid,
+    Cylinder,
+    IntersectedSolid,
+    JoinedSolid,
+    Rect,
+    Sphere,
+    SubtractedSolid,
+)
+
+_MASK = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.state = seed & _MASK
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & _MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK
+        return z ^ (z >> 31)
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / 9007199254740992.0))
+
+    def randint(self, lo: int, hi: int) -> int:
+        """Uniform integer in [lo, hi]."""
+        return lo + self.next_u64() % (hi - lo + 1)
+
+    def unit(self):
+        while True:
+            v = [self.uniform(-1, 1) for _ in range(3)]
+            n = math.sqrt(sum(x * x for x in v))
+            if 1e-3 < n <= 1:
+                return [x / n for x in v]
+
+
+def random_primitive(rng: SplitMix64, box: float = 1.0):
+    """One of the four primitive kinds with a centre uniform in [-box, box]^3."""
+    c = [rng.uniform(-box, box) for _ in range(3)]
+    kind = rng.randint(0, 3)
+    if kind == 0:
+        return Sphere(c, rng.uniform(0.05, 0.3))
+    if kind == 1:
+        side = [rng.uniform(0.1, 0.5) for _ in range(3)]
+        return Rect([c[i] - side[i] / 2 for i in range(3)], [c[i] + side[i] / 2 for i in range(3)])
+    d = rng.unit()
+    half = rng.uniform(0.05, 0.3)
+    p1 = [c[i] - d[i] * half for i in range(3)]
+    p2 = [c[i] + d[i] * half for i in range(3)]
+    r = rng.uniform(0.03, 0.2)
+    return Capsule(p1, p2, r) if kind == 2 else Cylinder(p1, p2, r)
+
+
+def _centre(s) -> list:
+    if isinstance(s, Sphere):
+        return list(s.Center)
+    if isinstance(s, Rect):
+        return [(s.MinVal[i] + s.MaxVal[i]) / 2 for i in range(3)]
+    return [(s.P1[i] + s.P2[i]) / 2 for i in range(3)]
+
+
+def random_csg(seed: int, n_prims: int = 64, p_union: float = 0.7, p_sub: float = 0.2, clip: float | None = 1.0):
+    """A random binary CSG tree over ``n_prims`` primitives.
+
+    Bottom-up: pick a random node, pair it with the node whose centre is
+    nearest (so subtractions and intersections act on overlapping parts, as
+    in a modelled part), and join the pair by union (70%), subtraction (20%)
+    or intersection (10%).  With ``clip`` the root is ClipSolid'ed to
+    [-clip, clip]^3."""
+    rng = SplitMix64(seed)
+    prims = [random_primitive(rng) for _ in range(n_prims)]
+    nodes = [(p, _centre(p)) for p in prims]
+    while len(nodes) > 1:
+        i = rng.randint(0, len(nodes) - 1)
+        a, ca = nodes.pop(i)
+        j = min(range(len(nodes)), key=lambda k: sum((nodes[k][1][t] - ca[t]) ** 2 for t in range(3)))
+        b, cb = nodes.pop(j)
+        u = rng.uniform()
+        if u < p_union:
+            n = JoinedSolid([a, b])
+        elif u < p_union + p_sub:
+            n = SubtractedSolid(a, b)
+        else:
+            n = IntersectedSolid([a, b])
+        nodes.append((n, [(ca[t] + cb[t]) / 2 for t in range(3)]))
+    root = nodes[0][0]
+    if clip is not None:
+        root = ClipSolid(root, (-clip,) * 3, (clip,) * 3)
+    return root
+
+
+# The benchmark configurations, by name.
+SPHERE_CONFIG = dict(name="sphere_r1_d0.02", delta=0.02, iters=8)
+CSG64_CONFIG = dict(name="csg64_seed2_d0.005", seed=2, n_prims=64, delta=0.005, iters=8)
+
+
+def sphere_solid():
+    return Sphere((0.0, 0.0, 0.0), 1.0)
+
+
+def csg64_solid(seed: int = 2, n_prims: int = 64):
+    return random_csg(seed, n_prims)

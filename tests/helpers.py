@@ -1,0 +1,96 @@
+"""Shared test helpers: canonical triangle sets and a catalogue of solids."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+import model3d_amd as m3
+
+
+def canonical(tris) -> np.ndarray:
+    """Triangle multiset in a canonical order: rows of 9 float64 bit patterns
+    (vertex order inside a triangle is kept), sorted lexicographically."""
+    a = np.ascontiguousarray(np.asarray(tris, dtype=np.float64).reshape(-1, 9)).view(np.uint64)
+    if len(a) == 0:
+        return a
+    return a[np.lexsort(a.T[::-1])]
+
+
+def same_triangles(a, b) -> bool:
+    ca, cb = canonical(a), canonical(b)
+    return ca.shape == cb.shape and bool(np.array_equal(ca, cb))
+
+
+def edge_topology(tris: np.ndarray):
+    """(directed edge multiset ok, undirected edge count ok) for a closed,
+    consistently oriented manifold: every directed edge once, and its
+    reverse exactly once (Mesh.NeedsRepair, mesh_ops.go:343-380)."""
+    t = canonical_vertices(tris)
+    edges = {}
+    for tri in t:
+        for i in range(3):
+            a, b = tri[i], tri[(i + 1) % 3]
+            edges[(a, b)] = edges.get((a, b), 0) + 1
+    directed_once = all(c == 1 for c in edges.values())
+    paired = all(edges.get((b, a), 0) == 1 for (a, b) in edges)
+    return directed_once, paired
+
+
+def canonical_vertices(tris: np.ndarray):
+    v = np.ascontiguousarray(np.asarray(tris, dtype=np.float64).reshape(-1, 3)).view(np.uint64)
+    keys = [tuple(r) for r in v]
+    return [keys[3 * i : 3 * i + 3] for i in range(len(keys) // 3)]
+
+
+def volume(tris: np.ndarray) -> float:
+    t = np.asarray(tris, dtype=np.float64).reshape(-1, 3, 3)
+    a, b, c = t[:, 0], t[:, 1], t[:, 2]
+    det = (
+        a[:, 0] * (b[:, 1] * c[:, 2] - b[:, 2] * c[:, 1])
+        - a[:, 1] * (b[:, 0] * c[:, 2] - b[:, 2] * c[:, 0])
+        + a[:, 2] * (b[:, 0] * c[:, 1] - b[:, 1] * c[:, 0])
+    )
+    return float(np.sum(det / 6.0))
+
+
+def solid_catalogue():
+    """Named solids covering every lowered type (small enough for the oracle
+    at delta ~0.05)."""
+    sph = m3.Sphere((0.1, -0.2, 0.05), 0.7)
+    box = m3.Rect((-0.5, -0.4, -0.3), (0.6, 0.2, 0.45))
+    cap = m3.Capsule((-0.4, -0.3, 0.1), (0.5, 0.35, -0.2), 0.3)
+    cyl = m3.Cylinder((-0.3, 0.4, -0.5), (0.45, -0.3, 0.5), 0.35)
+    cone = m3.Cone((0.2, 0.6, 0.1), (-0.2, -0.5, -0.1), 0.45)
+    cslice = m3.ConeSlice((0.0, -0.5, 0.2), (0.1, 0.55, -0.1), 0.5, 0.2)
+    tor = m3.Torus((0.05, 0.0, -0.1), (0.3, 0.8, 0.5), 0.6, 0.2)
+    tor_axis = m3.Torus((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 0.6, 0.25)
+    csg = m3.SubtractedSolid(
+        m3.JoinedSolid([sph, m3.Capsule((0.4, 0.5, 0.0), (0.8, 0.6, 0.2), 0.2)]),
+        m3.IntersectedSolid([box, m3.Sphere((0.3, 0.0, 0.0), 0.5)]),
+    )
+    rot = m3.RotateSolid(cyl, (0.0, 0.6, 0.8), 0.7)
+    chain = m3.TransformSolid(
+        m3.JoinedTransform([m3.Scale(1.3), m3.Translate((0.1, 0.2, -0.1)), m3.VecScale((1.0, 0.8, 1.2))]), cap
+    )
+    nested = m3.TranslateSolid(m3.JoinedSolid([m3.ScaleSolid(box, 0.8), m3.RotateSolid(tor, (1.0, 0.0, 0.0), 0.4)]), (0.05, -0.1, 0.0))
+    mirrored = m3.TransformSolid(m3.Mirror((1.0, 1.0, 0.0)), cone)
+    forced = m3.ForceSolidBounds(sph, (-0.7, -1.0, -0.7), (0.9, 0.6, 0.9))
+    clipped = m3.ClipSolid(m3.JoinedSolid([sph, cyl]), (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+    return {
+        "sphere": sph,
+        "rect": box,
+        "capsule": cap,
+        "cylinder": cyl,
+        "cone": cone,
+        "cone_slice": cslice,
+        "torus": tor,
+        "torus_axis": tor_axis,
+        "csg": csg,
+        "rotated": rot,
+        "transform_chain": chain,
+        "nested_transforms": nested,
+        "mirrored": mirrored,
+        "force_bounds": forced,
+        "clipped": clipped,
+    }

@@ -1,0 +1,126 @@
+"""GPU parity: the HIP library against the oracle, through the C-ABI.
+
+Bit-exact comparisons throughout: Contains booleans, and triangle sets as
+multisets of 9 float64 bit patterns per triangle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import model3d_amd as m3
+from model3d_amd import workloads
+from oracle.binding import oracle_contains, oracle_search
+from tests.helpers import canonical, edge_topology, same_triangles, solid_catalogue, volume
+
+pytestmark = pytest.mark.gpu
+
+FP64 = m3.MC3D_PREC_FP64
+GUARD = m3.MC3D_PREC_FP32_GUARD
+
+
+def _near_surface_points(s, rng, n=4000):
+    """Random points plus points on the bisection paths of the oracle mesh,
+    which straddle the surface at all scales."""
+    mn, mx = np.array(s.Min()), np.array(s.Max())
+    pad = 0.1 * (mx - mn) + 1e-3
+    p = rng.uniform(mn - pad, mx + pad, size=(n, 3))
+    tris, _ = oracle_search(s, 0.05, 12, threads=8)
+    v = tris.reshape(-1, 3)[: 3 * n]
+    return np.concatenate([p, v, v + rng.normal(scale=1e-9, size=v.shape)])
+
+
+@pytest.mark.parametrize("name", list(solid_catalogue()))
+@pytest.mark.parametrize("precision", [FP64, GUARD])
+def test_contains_parity(name, precision):
+    s = solid_catalogue()[name]
+    rng = np.random.default_rng(abs(hash(name)) % 1000)
+    pts = _near_surface_points(s, rng)
+    print(f"contains parity {name} precision={precision} n={len(pts)}", flush=True)
+    got = m3.Contains(s, pts, precision=precision)
+    want = oracle_contains(s, pts)
+    assert np.array_equal(got, want), f"{np.count_nonzero(got != want)} mismatches"
+
+
+@pytest.mark.parametrize("name", list(solid_catalogue()))
+@pytest.mark.parametrize("precision", [FP64, GUARD])
+def test_mesh_parity_catalogue(name, precision):
+    s = solid_catalogue()[name]
+    print(f"mesh parity {name} precision={precision}", flush=True)
+    want, _ = oracle_search(s, 0.04, 8, threads=8)
+    got = m3.MarchingCubesSearch(s, 0.04, 8, precision=precision)
+    assert len(want) > 0
+    assert same_triangles(got.triangles(), want)
+
+
+@pytest.mark.parametrize("iters", [0, 1, 8])
+def test_sphere_config_bit_exact(iters):
+    s = workloads.sphere_solid()
+    print(f"sphere r=1 d=0.02 iters={iters}", flush=True)
+    want, info = oracle_search(s, 0.02, iters, threads=8)
+    for precision in (FP64, GUARD):
+        got = m3.MarchingCubesSearch(s, 0.02, iters, precision=precision)
+        assert got.stats["corners"] == 102**3
+        assert got.stats["triangles"] == 93992 and got.stats["vertices"] == 46998
+        assert same_triangles(got.triangles(), want)
+
+
+def test_csg64_coarse_bit_exact():
+    s = workloads.csg64_solid(2)
+    print("csg64 d=0.02", flush=True)
+    want, _ = oracle_search(s, 0.02, 8, threads=8)
+    for precision in (FP64, GUARD):
+        got = m3.MarchingCubesSearch(s, 0.02, 8, precision=precision)
+        assert same_triangles(got.triangles(), want)
+
+
+def test_csg64_full_size_bit_exact():
+    """The benchmark workload at its full size (delta 0.005, ~65M corners)."""
+    s = workloads.csg64_solid(2)
+    print("csg64 d=0.005 (full size)", flush=True)
+    want, info = oracle_search(s, 0.005, 8)
+    got = m3.MarchingCubesSearch(s, 0.005, 8, precision=GUARD)
+    assert got.stats["corners"] == info["corners"]
+    assert same_triangles(got.triangles(), want)
+    got64 = m3.MarchingCubesSearch(s, 0.005, 8, precision=FP64)
+    assert np.array_equal(canonical(got64.triangles()), canonical(got.triangles()))
+
+
+def test_mesh_topology_and_volume():
+    s = m3.Sphere((0.1, 0.3, -0.2), 1.0)
+    got = m3.MarchingCubesSearch(s, 0.05, 8)
+    directed_once, paired = edge_topology(got.triangles())
+    assert directed_once and paired
+    assert abs(volume(got.triangles()) - 4 / 3 * math.pi) < 2e-2
+    # every vertex is referenced, and indices are in range
+    assert got.faces.max() < len(got.vertices)
+    assert len(np.unique(got.faces)) == len(got.vertices)
+
+
+class _LyingBounds(m3.Solid):
+    """A host-evaluated solid that is true beyond the bounds it reports."""
+
+    def Min(self):
+        return (-0.3, -0.3, -0.3)
+
+    def Max(self):
+        return (0.3, 0.3, 0.3)
+
+    def ContainsBatch(self, pts):
+        return np.linalg.norm(np.asarray(pts), axis=1) <= 1.0
+
+
+@pytest.mark.skip(reason="host-evaluated solids need the batch host fallback")
+def test_inside_boundary_panics():
+    forced = m3.ForceSolidBounds(m3.Sphere((0, 0, 0), 0.5), (-0.4, -0.4, -0.4), (0.4, 0.4, 0.4))
+    ok = m3.MarchingCubesSearch(forced, 0.1, 2)
+    assert ok.NumTriangles() > 0
+    with pytest.raises(m3.Panic) as e:
+        m3.MarchingCubesSearch(_LyingBounds(), 0.1, 0)
+    assert "solid is true outside of bounds" in str(e.value)
+
+
+def test_invalid_bounds_panics():
+    with pytest.raises(m3.Panic) as e:
+        m3.MarchingCubesSearch(m3.Sphere((0, 0, 0), float("nan")), 0.1, 0)
+    assert "invalid bounds" in str(e.value)
