@@ -36,8 +36,7 @@ def workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_gpu(solid, cfg, precision, cull, steps, warmup):
-    import model3d_amd as m3
+def run_gpu(solid, cfg, precision, cull, steps, warmup, count_mismatch=False):
     from model3d_amd import _lib
     from model3d_amd.solids import LibBackend
 
@@ -49,6 +48,7 @@ def run_gpu(solid, cfg, precision, cull, steps, warmup):
     opts.precision = precision
     opts.cull = 1 if cull else 0
     opts.device_only = 1
+    opts.count_mismatch = 1 if count_mismatch else 0
     stats = []
 
     def step(record):
@@ -147,24 +147,38 @@ def main():
         },
         "tris_per_s": tris / (wall / args.steps),
     }
-    stages = {k: mean(st, k) for k in ("ms_corners", "ms_classify", "ms_emit", "ms_bisect", "ms_device")}
+    stage_keys = ("ms_plan", "ms_corners", "ms_classify", "ms_emit", "ms_bisect", "ms_device")
+    stages = {k: mean(st, k) for k in stage_keys}
     result["stages_ms"] = stages
-    ge, gr = mean(st, "guard_evals"), mean(st, "guard_rechecks")
-    result["guard"] = {"fp32_evals": ge, "fp64_rechecks": gr, "fallback_fraction": (gr / ge) if ge else None}
+    result["bricks"] = {"total": st[0]["bricks_total"], "active": st[0]["bricks_active"]}
+
+    # Guard statistics from one extra (untimed) run that also evaluates every
+    # float32 decision in float64.
+    _, sst = run_gpu(solid, cfg, _lib.MC3D_PREC_FP32_GUARD, True, 1, 0, count_mismatch=True)
+    g = sst[0]
+    result["guard"] = {
+        "fp32_atom_evals": g["guard_evals"],
+        "fp64_rechecks": g["guard_rechecks"],
+        "fp64_fallback_fraction": g["guard_rechecks"] / g["guard_evals"] if g["guard_evals"] else None,
+        "fp32_sign_mismatch_rate": g["guard_mismatches"] / g["guard_evals"] if g["guard_evals"] else None,
+        "band_violations": g["guard_violations"],
+    }
 
     if not args.no_fp64:
-        wall64, st64 = run_gpu(solid, cfg, _lib.MC3D_PREC_FP64, False, max(1, args.steps // 2), 1)
-        ms64 = 1e3 * wall64 / max(1, args.steps // 2)
-        result["fp64_plain"] = {
-            "corner_evals_per_s": corners / (ms64 / 1e3),
-            "tris_per_s": tris / (ms64 / 1e3),
-            "ms_per_step": ms64,
-            "stages_ms": {k: mean(st64, k) for k in stages},
-        }
-        result["speedup_vs_fp64_plain"] = ms64 / ms
+        n64 = max(1, args.steps // 2)
+        for label, cull in (("fp64_plain", False), ("fp64_cull", True)):
+            wall64, st64 = run_gpu(solid, cfg, _lib.MC3D_PREC_FP64, cull, n64, 1)
+            ms64 = 1e3 * wall64 / n64
+            result[label] = {
+                "corner_evals_per_s": corners / (ms64 / 1e3),
+                "tris_per_s": tris / (ms64 / 1e3),
+                "ms_per_step": ms64,
+                "stages_ms": {k: mean(st64, k) for k in stage_keys},
+            }
+        result["speedup_vs_fp64_plain"] = result["fp64_plain"]["ms_per_step"] / ms
 
     # Roofline of the dominant stage (HIP events on the library's stream).
-    dom = max(("ms_corners", "ms_classify", "ms_emit", "ms_bisect"), key=lambda k: stages[k])
+    dom = max(("ms_plan", "ms_corners", "ms_classify", "ms_emit", "ms_bisect"), key=lambda k: stages[k])
     result["roofline"] = roofline(dom, stages[dom], st[0])
 
     if not args.no_cpu:
@@ -185,6 +199,7 @@ def roofline(stage, ms, st):
     n, v, t = st["corners"], st["vertices"], st["triangles"]
     words = n / 64
     algo = {
+        "ms_plan": st["bricks_total"] * 24,
         "ms_corners": n / 8,
         "ms_classify": 4 * n / 8 + words * 8 * 3,
         "ms_emit": 4 * n / 8 + words * 8 + 8 * v + 12 * t,
@@ -193,8 +208,8 @@ def roofline(stage, ms, st):
     gbs = algo / (ms * 1e-3) / 1e9
     return {
         "bound": "hbm",
-        "kernel": {"ms_corners": "k_corners", "ms_classify": "k_classify", "ms_emit": "k_emit",
-                   "ms_bisect": "k_bisect"}[stage],
+        "kernel": {"ms_plan": "k_plan", "ms_corners": "k_corners", "ms_classify": "k_classify",
+                   "ms_emit": "k_emit", "ms_bisect": "k_bisect"}[stage],
         "achieved": gbs,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",

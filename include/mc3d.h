@@ -136,10 +136,12 @@ typedef struct {
     uint64_t bisection_evals;  /* V * (iters + 1) when iters > 0, else 0 */
     uint64_t guard_evals;      /* float32 primitive evaluations */
     uint64_t guard_rechecks;   /* of those, re-decided in float64 */
-    uint64_t guard_mismatches; /* float32 decision != float64 decision (count_mismatch only) */
-    uint64_t bricks_total, bricks_active;
+    uint64_t guard_mismatches; /* plain float32 decision != float64 decision (count_mismatch only) */
+    uint64_t guard_violations; /* band-sure decision != float64 decision (count_mismatch only; must be 0) */
+    uint64_t bricks_total;     /* culling bricks */
+    uint64_t bricks_active;    /* bricks whose simplified program is not a constant */
     double ms_device;          /* GPU time of the whole pipeline (HIP events) */
-    double ms_corners, ms_classify, ms_emit, ms_bisect; /* per stage (HIP events) */
+    double ms_plan, ms_corners, ms_classify, ms_emit, ms_bisect; /* per stage (HIP events) */
     double ms_d2h;             /* copy of results to host */
     double ms_host_callbacks;  /* time spent in mc3d_contains_fn */
 } mc3d_stats;
@@ -151,7 +153,7 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
 int mc3d_mesh_counts(const mc3d_mesh *m, size_t *n_vertices, size_t *n_triangles);
 /* Copies the results to host arrays of 3*V doubles and 3*T uint32. */
 int mc3d_mesh_copy(mc3d_mesh *m, double *verts, uint32_t *tris);
-/* Device pointers of the results (valid until the mesh is freed). */
+/* Device pointers of the results (owned by the mesh; valid until mc3d_mesh_free). */
 int mc3d_mesh_device_ptrs(const mc3d_mesh *m, const double **verts, const uint32_t **tris);
 int mc3d_mesh_stats(const mc3d_mesh *m, mc3d_stats *out);
 void mc3d_mesh_free(mc3d_mesh *m);

@@ -60,9 +60,11 @@ class mc3d_stats(ctypes.Structure):
         ("guard_evals", ctypes.c_uint64),
         ("guard_rechecks", ctypes.c_uint64),
         ("guard_mismatches", ctypes.c_uint64),
+        ("guard_violations", ctypes.c_uint64),
         ("bricks_total", ctypes.c_uint64),
         ("bricks_active", ctypes.c_uint64),
         ("ms_device", ctypes.c_double),
+        ("ms_plan", ctypes.c_double),
         ("ms_corners", ctypes.c_double),
         ("ms_classify", ctypes.c_double),
         ("ms_emit", ctypes.c_double),

@@ -1,0 +1,252 @@
+// eval.h -- device-side Solid.Contains: the postfix boolean program over
+// atoms, each atom one primitive Contains in float64 (bit-exact with the
+// reference) or in float32 with an error band (guard mode).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "mc3d_internal.h"
+
+namespace mc3d {
+
+struct DevProg {
+    const Insn *__restrict__ code;
+    const double *__restrict__ f64;
+    const float *__restrict__ f32;
+    const Frame *__restrict__ frames;
+    const AtomBox *__restrict__ boxes;  // world-space box per pc (atoms only)
+    uint32_t ncode;
+    uint32_t pad;
+};
+
+// Guard statistics, kept in registers and reduced once per block.
+struct GStats {
+    uint32_t evals = 0;       // float32 atom evaluations
+    uint32_t rechecks = 0;    // of those, in the band -> float64
+    uint32_t mismatches = 0;  // plain float32 decision != float64 decision
+    uint32_t violations = 0;  // band said sure but float64 disagrees (must stay 0)
+};
+
+// Go math.Min(c, m) == m  <=>  m == -Inf || c >= m   (and the Max twin); this
+// is Rect.Contains / InBounds (shapes.go:171-173, bounder.go:23-27) including
+// the NaN and signed-zero cases.
+__device__ __forceinline__ bool in_lo(double c, double m) { return m == -INFINITY || c >= m; }
+__device__ __forceinline__ bool in_hi(double c, double m) { return m == INFINITY || c <= m; }
+
+__device__ __forceinline__ void apply_frame(const double *__restrict__ f64, Frame fr, double &x, double &y,
+                                            double &z) {
+    const double *r = f64 + fr.offset;
+    for (uint32_t i = 0; i < fr.nops; ++i, r += 10) {
+        int k = (int)r[0];
+        if (k == MC3D_XF_TRANSLATE) {  // Translate.Apply, transform.go:44-46
+            x = x + r[1];
+            y = y + r[2];
+            z = z + r[3];
+        } else if (k == MC3D_XF_SCALE) {  // Scale.Apply, transform.go:71-73
+            x = x * r[1];
+            y = y * r[1];
+            z = z * r[1];
+        } else if (k == MC3D_XF_VECSCALE) {  // VecScale.Apply, transform.go:103-105
+            x = x * r[1];
+            y = y * r[2];
+            z = z * r[3];
+        } else {  // Matrix3.MulColumn, matrix.go:131-137
+            double nx = r[1] * x + r[2] * y + r[3] * z;
+            double ny = r[4] * x + r[5] * y + r[6] * z;
+            double nz = r[7] * x + r[8] * y + r[9] * z;
+            x = nx, y = ny, z = nz;
+        }
+    }
+}
+
+// One primitive Contains in float64 (point already in the atom's frame).
+// Constants that do not depend on the point were computed on the host with
+// the reference's operation order (solid.cpp).
+__device__ __forceinline__ bool atom64(uint32_t kind, const double *__restrict__ a, double x, double y, double z) {
+    switch (kind) {
+    case A_SPHERE: {  // Sphere.Contains: coord.Dist(center) <= r, shapes.go:29-31
+        double d1 = x - a[0], d2 = y - a[1], d3 = z - a[2];
+        return d1 * d1 + d2 * d2 + d3 * d3 <= a[4];  // sqrt(s) <= r  <=>  s <= T
+    }
+    case A_RECT:  // Rect.Contains, shapes.go:171-173
+        return in_lo(x, a[0]) && in_lo(y, a[1]) && in_lo(z, a[2]) && in_hi(x, a[3]) && in_hi(y, a[4]) &&
+               in_hi(z, a[5]);
+    case A_CAPSULE: {  // Segment.Dist(c) <= r, primitives.go:568-593
+        double v2x = x - a[0], v2y = y - a[1], v2z = z - a[2];
+        double mag = a[6] * v2x + a[7] * v2y + a[8] * v2z;
+        double cx, cy, cz;
+        if (mag > a[9]) {
+            cx = a[3], cy = a[4], cz = a[5];
+        } else if (mag < 0) {
+            cx = a[0], cy = a[1], cz = a[2];
+        } else {
+            cx = a[6] * mag + a[0], cy = a[7] * mag + a[1], cz = a[8] * mag + a[2];
+        }
+        double d1 = x - cx, d2 = y - cy, d3 = z - cz;
+        return d1 * d1 + d2 * d2 + d3 * d3 <= a[11];
+    }
+    case A_CYLINDER: {  // Cylinder.Contains, shapes.go:588-597
+        double frac = (x - a[0]) * a[3] + (y - a[1]) * a[4] + (z - a[2]) * a[5];
+        if (frac < 0 || frac > a[6]) return false;
+        double px = a[0] + a[3] * frac, py = a[1] + a[4] * frac, pz = a[2] + a[5] * frac;
+        double d1 = px - x, d2 = py - y, d3 = pz - z;
+        return d1 * d1 + d2 * d2 + d3 * d3 <= a[8];
+    }
+    case A_CONE: {  // Cone.Contains, shapes.go:888-898
+        double frac = (x - a[0]) * a[3] + (y - a[1]) * a[4] + (z - a[2]) * a[5];
+        double rf = 1 - frac / a[6];
+        if (rf < 0 || rf > 1) return false;
+        double px = a[0] + a[3] * frac, py = a[1] + a[4] * frac, pz = a[2] + a[5] * frac;
+        double d1 = px - x, d2 = py - y, d3 = pz - z;
+        return sqrt(d1 * d1 + d2 * d2 + d3 * d3) <= a[7] * rf;
+    }
+    case A_CONESLICE: {  // ConeSlice.Contains, shapes.go:1073-1084
+        double frac = (x - a[0]) * a[3] + (y - a[1]) * a[4] + (z - a[2]) * a[5];
+        double rf = 1 - frac / a[6];
+        if (rf < 0 || rf > 1) return false;
+        double px = a[0] + a[3] * frac, py = a[1] + a[4] * frac, pz = a[2] + a[5] * frac;
+        double r = a[7] * rf + a[8] * (1 - rf);
+        double d1 = px - x, d2 = py - y, d3 = pz - z;
+        return sqrt(d1 * d1 + d2 * d2 + d3 * d3) <= r;
+    }
+    case A_TORUS: {  // Torus.Contains -> genericSDF, shapes.go:1363-1365, 1451-1484
+        double cx = x - a[0], cy = y - a[1], cz = z - a[2];
+        double u = a[3] * cx + a[4] * cy + a[5] * cz;
+        double v = a[6] * cx + a[7] * cy + a[8] * cz;
+        double on = sqrt(u * u + v * v);
+        if (on == 0) {
+            u = 1, v = 0, on = 1;
+        }
+        double s = a[9] / on;
+        u *= s;
+        v *= s;
+        double rx = a[3] * u + a[6] * v, ry = a[4] * u + a[7] * v, rz = a[5] * u + a[8] * v;
+        double d1 = rx - cx, d2 = ry - cy, d3 = rz - cz;
+        return a[10] - sqrt(d1 * d1 + d2 * d2 + d3 * d3) >= 0;
+    }
+    }
+    return false;
+}
+
+// Float32 decision with a band: 1 inside, 0 outside, -1 in the band (the
+// caller re-decides in float64).  The band thresholds (Tlo, Thi) and branch
+// margins were computed on the host from a forward error bound of this exact
+// float32 operation sequence, valid for every point within 2r of the
+// primitive; points farther away can only be decided "outside", which is
+// then correct (solid.cpp, guard_*).  *plain gets the unguarded float32
+// decision for the mismatch statistic.
+__device__ __forceinline__ int atom32(uint32_t kind, const float *__restrict__ a, float x, float y, float z,
+                                      bool *plain) {
+    switch (kind) {
+    case A_SPHERE: {  // [c0 c1 c2 r T Tlo Thi]
+        float d1 = x - a[0], d2 = y - a[1], d3 = z - a[2];
+        float q = d1 * d1 + d2 * d2 + d3 * d3;
+        *plain = q <= a[4];
+        return q <= a[5] ? 1 : (q > a[6] ? 0 : -1);
+    }
+    case A_CAPSULE: {  // [s0(3) s1(3) v(3) norm r T ebr Tlo Thi]
+        float v2x = x - a[0], v2y = y - a[1], v2z = z - a[2];
+        float mag = a[6] * v2x + a[7] * v2y + a[8] * v2z;
+        float cx, cy, cz;
+        bool amb = fabsf(mag - a[9]) <= a[12] || fabsf(mag) <= a[12];
+        if (mag > a[9]) {
+            cx = a[3], cy = a[4], cz = a[5];
+        } else if (mag < 0) {
+            cx = a[0], cy = a[1], cz = a[2];
+        } else {
+            cx = a[6] * mag + a[0], cy = a[7] * mag + a[1], cz = a[8] * mag + a[2];
+        }
+        float d1 = x - cx, d2 = y - cy, d3 = z - cz;
+        float q = d1 * d1 + d2 * d2 + d3 * d3;
+        *plain = q <= a[11];
+        if (amb) return -1;
+        return q <= a[13] ? 1 : (q > a[14] ? 0 : -1);
+    }
+    case A_CYLINDER: {  // [p2(3) dir(3) norm r T ebr Tlo Thi]
+        float frac = (x - a[0]) * a[3] + (y - a[1]) * a[4] + (z - a[2]) * a[5];
+        *plain = false;
+        if (frac < -a[9] || frac > a[6] + a[9]) return 0;
+        bool amb = fabsf(frac) <= a[9] || fabsf(frac - a[6]) <= a[9];
+        float px = a[0] + a[3] * frac, py = a[1] + a[4] * frac, pz = a[2] + a[5] * frac;
+        float d1 = px - x, d2 = py - y, d3 = pz - z;
+        float q = d1 * d1 + d2 * d2 + d3 * d3;
+        *plain = frac >= 0 && frac <= a[6] && q <= a[8];
+        if (amb) return -1;
+        return q <= a[10] ? 1 : (q > a[11] ? 0 : -1);
+    }
+    }
+    return -1;  // no float32 form: decide in float64
+}
+
+__device__ __forceinline__ bool has_atom32(uint32_t kind) {
+    return kind == A_SPHERE || kind == A_CAPSULE || kind == A_CYLINDER;
+}
+
+// Postfix boolean program over atoms; the stack is a bit stack in a register.
+// MODE 0: every atom in float64.  MODE 1: float32 with band, float64 in band.
+template <int MODE, bool STATS>
+__device__ __forceinline__ bool eval_code(const Insn *__restrict__ code, uint32_t n, const DevProg &P, double px,
+                                          double py, double pz, GStats &gs) {
+    uint32_t st = 0;
+    uint32_t cur = 0;
+    double qx = px, qy = py, qz = pz;
+    float fx = (float)px, fy = (float)py, fz = (float)pz;
+    for (uint32_t pc = 0; pc < n; ++pc) {
+        Insn in = code[pc];
+        uint32_t op = insn_op(in.x);
+        if (op == OP_ATOM) {
+            uint32_t fr = insn_frame(in.x);
+            if (fr != cur) {
+                qx = px, qy = py, qz = pz;
+                apply_frame(P.f64, P.frames[fr], qx, qy, qz);
+                cur = fr;
+                if (MODE == 1) fx = (float)qx, fy = (float)qy, fz = (float)qz;
+            }
+            const uint32_t kind = insn_kind(in.x);
+            bool b;
+            if (MODE == 1 && has_atom32(kind)) {
+                bool plain;
+                int r = atom32(kind, P.f32 + in.y, fx, fy, fz, &plain);
+                if (STATS) gs.evals++;
+                if (r < 0) {
+                    b = atom64(kind, P.f64 + in.y, qx, qy, qz);
+                    if (STATS) gs.rechecks++, gs.mismatches += plain != b;
+                } else {
+                    b = r != 0;
+                    if (STATS) {
+                        bool t = atom64(kind, P.f64 + in.y, qx, qy, qz);
+                        gs.mismatches += plain != t;
+                        gs.violations += b != t;
+                    }
+                }
+            } else {
+                b = atom64(kind, P.f64 + in.y, qx, qy, qz);
+            }
+            st = (st << 1) | (uint32_t)b;
+        } else if (op == OP_OR) {
+            uint32_t t = st & 1u;
+            st = (st >> 1) | t;
+        } else if (op == OP_AND) {
+            uint32_t t = st & 1u;
+            st = (st >> 1) & (~1u | t);
+        } else if (op == OP_ANDNOT) {  // second & !top
+            uint32_t t = st & 1u;
+            st = (st >> 1) & ~t;
+        } else if (op == OP_RANDNOT) {  // !second & top
+            uint32_t t = st & 1u;
+            st >>= 1;
+            st = (st & ~1u) | (t & ~st & 1u);
+        } else if (op == OP_NOT) {
+            st ^= 1u;
+        } else if (op == OP_TRUE) {
+            st = (st << 1) | 1u;
+        } else {
+            st <<= 1;
+        }
+    }
+    return st & 1u;
+}
+
+}  // namespace mc3d

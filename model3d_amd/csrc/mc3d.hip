@@ -2,6 +2,11 @@
 // mc3d_marching_cubes_search (model3d/mc.go:45-49).
 //
 // Stages, one solid per call:
+//   plan     : (culling on) per brick of 64 x 4 x 4 corners, the solid's
+//              program simplified against the brick's box: atoms whose
+//              conservative box misses the brick are false, bounds checks and
+//              spheres that contain it are true, and the boolean ops fold
+//              around them.  Empty and full bricks need no evaluation.
 //   corners  : Solid.Contains at every grid corner -> bit-packed occupancy,
 //              one 64-bit word per 64 consecutive x corners of a row; one
 //              wave per word, packed with a wave ballot (mc.go:614-692).
@@ -13,14 +18,15 @@
 //   emit     : per word, vertex keys (axis, x, y, z of the owning edge) and
 //              triangles as uint32 vertex-index triples in table order
 //              (mc.go:20-35, 401-407).
-//   bisect   : per vertex, mcSearchPoint (mc.go:223-258) in float64: the
-//              first probe is the edge's high corner, read from the
-//              occupancy grid (same point, same answer), then `iters`
-//              midpoint evaluations.
+//   bisect   : per vertex, mcSearchPoint (mc.go:223-258): the first probe is
+//              the edge's high corner, read from the occupancy grid (same
+//              point, same answer), then `iters` float64 midpoints.
 //
 // Every float64 expression keeps the reference's operation order and is
 // compiled with -ffp-contract=off, so Contains decisions, topology and
-// vertex coordinates are bit-identical to the reference's amd64 build.
+// vertex coordinates are bit-identical to the reference's amd64 build.  In
+// guard mode an atom is first decided in float32 and re-decided in float64
+// when the float32 value is inside its error band (eval.h).
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -33,166 +39,36 @@
 #include <string>
 #include <vector>
 
+#include "eval.h"
 #include "mc3d_internal.h"
 
 using namespace mc3d;
 
 namespace {
 
-// ---------------------------------------------------------------------------
-// Device program evaluation
-// ---------------------------------------------------------------------------
-struct DevProg {
+constexpr uint32_t BRICK_TRUE = 0xffffffffu;
+
+// Per-brick programs: info[b] = {offset into code, length}; length 0 is a
+// brick where the solid is false everywhere, BRICK_TRUE where it is true.
+struct Bricks {
+    const uint2 *__restrict__ info;  // null: no culling, use the full program
     const Insn *__restrict__ code;
-    const double *__restrict__ f64;
-    const float *__restrict__ f32;
-    const Frame *__restrict__ frames;
-    uint32_t ncode;
-    uint32_t pad;
+    int BY, BZ, nby, nbz;
 };
 
-// Go math.Min(c, m) == m  <=>  m == -Inf || c >= m   (and the Max twin); this
-// is Rect.Contains / InBounds (shapes.go:171-173, bounder.go:23-27) including
-// the NaN and signed-zero cases.
-__device__ __forceinline__ bool in_lo(double c, double m) { return m == -INFINITY || c >= m; }
-__device__ __forceinline__ bool in_hi(double c, double m) { return m == INFINITY || c <= m; }
-
-__device__ __forceinline__ void apply_frame(const double *__restrict__ f64, Frame fr, double &x, double &y,
-                                            double &z) {
-    const double *r = f64 + fr.offset;
-    for (uint32_t i = 0; i < fr.nops; ++i, r += 10) {
-        int k = (int)r[0];
-        if (k == MC3D_XF_TRANSLATE) {  // Translate.Apply, transform.go:44-46
-            x = x + r[1];
-            y = y + r[2];
-            z = z + r[3];
-        } else if (k == MC3D_XF_SCALE) {  // Scale.Apply, transform.go:71-73
-            x = x * r[1];
-            y = y * r[1];
-            z = z * r[1];
-        } else if (k == MC3D_XF_VECSCALE) {  // VecScale.Apply, transform.go:103-105
-            x = x * r[1];
-            y = y * r[2];
-            z = z * r[3];
-        } else {  // Matrix3.MulColumn, matrix.go:131-137
-            double nx = r[1] * x + r[2] * y + r[3] * z;
-            double ny = r[4] * x + r[5] * y + r[6] * z;
-            double nz = r[7] * x + r[8] * y + r[9] * z;
-            x = nx, y = ny, z = nz;
-        }
-    }
+template <bool STATS>
+__device__ __forceinline__ void flush_stats(const GStats &gs, unsigned long long *__restrict__ out) {
+    if (!STATS) return;
+    __shared__ unsigned int red[4];
+    if (threadIdx.x < 4) red[threadIdx.x] = 0;
+    __syncthreads();
+    if (gs.evals) atomicAdd(&red[0], gs.evals);
+    if (gs.rechecks) atomicAdd(&red[1], gs.rechecks);
+    if (gs.mismatches) atomicAdd(&red[2], gs.mismatches);
+    if (gs.violations) atomicAdd(&red[3], gs.violations);
+    __syncthreads();
+    if (threadIdx.x < 4 && red[threadIdx.x]) atomicAdd(&out[threadIdx.x], (unsigned long long)red[threadIdx.x]);
 }
-
-// One primitive Contains in float64 (point already in the atom's frame).
-__device__ __forceinline__ bool atom64(uint32_t kind, const double *__restrict__ a, double x, double y, double z) {
-    switch (kind) {
-    case A_SPHERE: {  // Sphere.Contains: coord.Dist(center) <= r, shapes.go:29-31
-        double d1 = x - a[0], d2 = y - a[1], d3 = z - a[2];
-        return d1 * d1 + d2 * d2 + d3 * d3 <= a[4];
-    }
-    case A_RECT:  // Rect.Contains, shapes.go:171-173
-        return in_lo(x, a[0]) && in_lo(y, a[1]) && in_lo(z, a[2]) && in_hi(x, a[3]) && in_hi(y, a[4]) &&
-               in_hi(z, a[5]);
-    case A_CAPSULE: {  // Segment.Dist(c) <= r, primitives.go:568-593
-        double v2x = x - a[0], v2y = y - a[1], v2z = z - a[2];
-        double mag = a[6] * v2x + a[7] * v2y + a[8] * v2z;
-        double cx, cy, cz;
-        if (mag > a[9]) {
-            cx = a[3], cy = a[4], cz = a[5];
-        } else if (mag < 0) {
-            cx = a[0], cy = a[1], cz = a[2];
-        } else {
-            cx = a[6] * mag + a[0], cy = a[7] * mag + a[1], cz = a[8] * mag + a[2];
-        }
-        double d1 = x - cx, d2 = y - cy, d3 = z - cz;
-        return d1 * d1 + d2 * d2 + d3 * d3 <= a[11];
-    }
-    case A_CYLINDER: {  // Cylinder.Contains, shapes.go:588-597
-        double frac = (x - a[0]) * a[3] + (y - a[1]) * a[4] + (z - a[2]) * a[5];
-        if (frac < 0 || frac > a[6]) return false;
-        double px = a[0] + a[3] * frac, py = a[1] + a[4] * frac, pz = a[2] + a[5] * frac;
-        double d1 = px - x, d2 = py - y, d3 = pz - z;
-        return d1 * d1 + d2 * d2 + d3 * d3 <= a[8];
-    }
-    case A_CONE: {  // Cone.Contains, shapes.go:888-898
-        double frac = (x - a[0]) * a[3] + (y - a[1]) * a[4] + (z - a[2]) * a[5];
-        double rf = 1 - frac / a[6];
-        if (rf < 0 || rf > 1) return false;
-        double px = a[0] + a[3] * frac, py = a[1] + a[4] * frac, pz = a[2] + a[5] * frac;
-        double d1 = px - x, d2 = py - y, d3 = pz - z;
-        return sqrt(d1 * d1 + d2 * d2 + d3 * d3) <= a[7] * rf;
-    }
-    case A_CONESLICE: {  // ConeSlice.Contains, shapes.go:1073-1084
-        double frac = (x - a[0]) * a[3] + (y - a[1]) * a[4] + (z - a[2]) * a[5];
-        double rf = 1 - frac / a[6];
-        if (rf < 0 || rf > 1) return false;
-        double px = a[0] + a[3] * frac, py = a[1] + a[4] * frac, pz = a[2] + a[5] * frac;
-        double r = a[7] * rf + a[8] * (1 - rf);
-        double d1 = px - x, d2 = py - y, d3 = pz - z;
-        return sqrt(d1 * d1 + d2 * d2 + d3 * d3) <= r;
-    }
-    case A_TORUS: {  // Torus.Contains -> genericSDF, shapes.go:1363-1365, 1451-1484
-        double cx = x - a[0], cy = y - a[1], cz = z - a[2];
-        double u = a[3] * cx + a[4] * cy + a[5] * cz;
-        double v = a[6] * cx + a[7] * cy + a[8] * cz;
-        double on = sqrt(u * u + v * v);
-        if (on == 0) {
-            u = 1, v = 0, on = 1;
-        }
-        double s = a[9] / on;
-        u *= s;
-        v *= s;
-        double rx = a[3] * u + a[6] * v, ry = a[4] * u + a[7] * v, rz = a[5] * u + a[8] * v;
-        double d1 = rx - cx, d2 = ry - cy, d3 = rz - cz;
-        return a[10] - sqrt(d1 * d1 + d2 * d2 + d3 * d3) >= 0;
-    }
-    }
-    return false;
-}
-
-// Postfix boolean program over atoms; the stack is a bit stack in a register.
-__device__ __forceinline__ bool eval_point(const DevProg &P, double px, double py, double pz) {
-    uint32_t st = 0;
-    uint32_t cur = 0;
-    double qx = px, qy = py, qz = pz;
-    for (uint32_t pc = 0; pc < P.ncode; ++pc) {
-        Insn in = P.code[pc];
-        uint32_t op = insn_op(in.x);
-        if (op == OP_ATOM) {
-            uint32_t fr = insn_frame(in.x);
-            if (fr != cur) {
-                qx = px, qy = py, qz = pz;
-                apply_frame(P.f64, P.frames[fr], qx, qy, qz);
-                cur = fr;
-            }
-            st = (st << 1) | (uint32_t)atom64(insn_kind(in.x), P.f64 + in.y, qx, qy, qz);
-        } else if (op == OP_OR) {
-            uint32_t t = st & 1u;
-            st = (st >> 1) | t;
-        } else if (op == OP_AND) {
-            uint32_t t = st & 1u;
-            st = (st >> 1) & (~1u | t);
-        } else if (op == OP_ANDNOT) {  // second & !top
-            uint32_t t = st & 1u;
-            st = (st >> 1) & ~t;
-        } else if (op == OP_RANDNOT) {  // !second & top
-            uint32_t t = st & 1u;
-            st >>= 1;
-            st = (st & ~1u) | (t & ~st & 1u);
-        } else if (op == OP_NOT) {
-            st ^= 1u;
-        } else if (op == OP_TRUE) {
-            st = (st << 1) | 1u;
-        } else {
-            st <<= 1;
-        }
-    }
-    return st & 1u;
-}
-
-// ---------------------------------------------------------------------------
-// Kernels
-// ---------------------------------------------------------------------------
 struct Grid {
     const double *__restrict__ xs;
     const double *__restrict__ ys;
@@ -202,25 +78,6 @@ struct Grid {
 
 __device__ __forceinline__ size_t word_index(const Grid &g, int wx, int y, int z) {
     return ((size_t)z * g.ny + y) * g.W + wx;
-}
-
-// corners: one wave per occupancy word (64 x corners of one row).
-__global__ __launch_bounds__(256) void k_corners(DevProg P, Grid g, uint64_t *__restrict__ bits,
-                                                 uint32_t *__restrict__ flags, uint64_t nwords) {
-    const int lane = threadIdx.x & 63;
-    for (uint64_t w = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < nwords; w += (uint64_t)gridDim.x * 4) {
-        const int wx = (int)(w % g.W);
-        const uint64_t row = w / g.W;
-        const int y = (int)(row % g.ny), z = (int)(row / g.ny);
-        const int x = wx * 64 + lane;
-        bool inside = false;
-        if (x < g.nx) inside = eval_point(P, g.xs[x], g.ys[y], g.zs[z]);
-        uint64_t m = __ballot(inside);
-        if (lane == 0) bits[w] = m;
-        // panic("solid is true outside of bounds"), mc.go:687-689
-        if (inside && (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1))
-            atomicOr(flags, 1u);
-    }
 }
 
 struct Masks {
@@ -453,15 +310,177 @@ __global__ __launch_bounds__(256) void k_emit(Grid g, const uint64_t *__restrict
     }
 }
 
+
+// ---- plan: per-brick program simplification --------------------------------
+// Atom against a brick box: 0 certainly false, 1 certainly true, 2 evaluate.
+// "False" needs only the atom's conservative world box (solid.cpp widens it
+// far beyond float64 rounding); "true" is proven for untransformed bounds
+// checks / boxes (exact comparisons) and spheres (farthest corner inside the
+// threshold with a relative margin far above the rounding of the sum).
+__device__ __forceinline__ int classify_atom(const DevProg &P, uint32_t pc, Insn in, const double *lo,
+                                             const double *hi) {
+    const AtomBox bx = P.boxes[pc];
+    if (hi[0] < bx.lo[0] || lo[0] > bx.hi[0] || hi[1] < bx.lo[1] || lo[1] > bx.hi[1] || hi[2] < bx.lo[2] ||
+        lo[2] > bx.hi[2])
+        return 0;
+    if (insn_frame(in.x) == 0) {
+        const double *a = P.f64 + in.y;
+        const uint32_t kind = insn_kind(in.x);
+        if (kind == A_RECT) {
+            if (in_lo(lo[0], a[0]) && in_lo(lo[1], a[1]) && in_lo(lo[2], a[2]) && in_hi(hi[0], a[3]) &&
+                in_hi(hi[1], a[4]) && in_hi(hi[2], a[5]))
+                return 1;
+        } else if (kind == A_SPHERE) {
+            double s = 0;
+            for (int i = 0; i < 3; i++) {
+                double m = fmax(fabs(lo[i] - a[i]), fabs(hi[i] - a[i]));
+                s += m * m;
+            }
+            if (s * (1 + 1e-12) <= a[4]) return 1;
+        }
+    }
+    return 2;
+}
+
+constexpr uint32_t TAG_EXPR = 0u, TAG_FALSE = 1u << 30, TAG_TRUE = 2u << 30, TAG_MASK = 3u << 30;
+constexpr int PLAN_BLOCK = 128;
+constexpr int PLAN_STACK = 33;
+
+// PASS 0 writes counts[b] = {code slots to reserve, 1 if the brick needs
+// evaluation}; the reservation is the peak length during the walk, since
+// folding a constant can discard code already emitted.  PASS 1 rewrites the
+// code of active bricks at offs[b].x and sets info[b] for every brick.
+template <int PASS>
+__global__ __launch_bounds__(PLAN_BLOCK) void k_plan(DevProg P, Grid g, Bricks B, uint64_t nbricks,
+                                                     uint2 *__restrict__ counts, const uint2 *__restrict__ offs,
+                                                     uint2 *__restrict__ info, Insn *__restrict__ out) {
+    __shared__ uint32_t stk_all[PLAN_BLOCK * PLAN_STACK];
+    uint32_t *S = stk_all + threadIdx.x * PLAN_STACK;
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nbricks;
+         b += (uint64_t)gridDim.x * blockDim.x) {
+        const int bx = (int)(b % g.W);
+        const uint64_t r = b / g.W;
+        const int by = (int)(r % B.nby), bz = (int)(r / B.nby);
+        const int x0 = bx * 64, y0 = by * B.BY, z0 = bz * B.BZ;
+        const int x1 = min(x0 + 64, g.nx - 1), y1 = min(y0 + B.BY, g.ny - 1), z1 = min(z0 + B.BZ, g.nz - 1);
+        const double lo[3] = {g.xs[x0], g.ys[y0], g.zs[z0]};
+        const double hi[3] = {g.xs[x1], g.ys[y1], g.zs[z1]};
+        Insn *o = (PASS == 1 && counts[b].y) ? out + offs[b].x : nullptr;
+        uint32_t n = 0, peak = 0;
+        int sp = 0;
+        for (uint32_t pc = 0; pc < P.ncode; ++pc) {
+            const Insn in = P.code[pc];
+            const uint32_t op = insn_op(in.x);
+            if (op == OP_ATOM) {
+                int c = classify_atom(P, pc, in, lo, hi);
+                if (c == 2) {
+                    if (o) o[n] = in;
+                    S[sp++] = TAG_EXPR | n;
+                    n++;
+                } else {
+                    S[sp++] = c ? TAG_TRUE : TAG_FALSE;
+                }
+            } else if (op == OP_TRUE || op == OP_FALSE) {
+                S[sp++] = op == OP_TRUE ? TAG_TRUE : TAG_FALSE;
+            } else if (op == OP_NOT) {
+                uint32_t a = S[sp - 1];
+                if (a == TAG_TRUE) S[sp - 1] = TAG_FALSE;
+                else if (a == TAG_FALSE) S[sp - 1] = TAG_TRUE;
+                else {
+                    if (o) o[n] = in;
+                    n++;
+                }
+            } else {
+                const uint32_t bb = S[--sp], aa = S[--sp];
+                const bool aE = (aa & TAG_MASK) == TAG_EXPR, bE = (bb & TAG_MASK) == TAG_EXPR;
+                const bool aT = aa == TAG_TRUE, aF = aa == TAG_FALSE, bT = bb == TAG_TRUE, bF = bb == TAG_FALSE;
+                // start of the operands' code: constants emitted nothing
+                const uint32_t first = aE ? (aa & ~TAG_MASK) : (bE ? (bb & ~TAG_MASK) : n);
+                uint32_t res;
+                bool emit = false, emit_not = false;
+                if (op == OP_OR) {
+                    if (aT || bT) res = TAG_TRUE, n = first;
+                    else if (aF) res = bb;
+                    else if (bF) res = aa;
+                    else res = aa, emit = true;
+                } else if (op == OP_AND) {
+                    if (aF || bF) res = TAG_FALSE, n = first;
+                    else if (aT) res = bb;
+                    else if (bT) res = aa;
+                    else res = aa, emit = true;
+                } else if (op == OP_ANDNOT) {  // a & !b
+                    if (aF || bT) res = TAG_FALSE, n = first;
+                    else if (bF) res = aa;
+                    else if (aT) res = bb, emit_not = true;  // b's code is last
+                    else res = aa, emit = true;
+                } else {  // OP_RANDNOT: !a & b
+                    if (aT || bF) res = TAG_FALSE, n = first;
+                    else if (aF) res = bb;
+                    else if (bT) res = aa, emit_not = true;  // a's code is last
+                    else res = aa, emit = true;
+                }
+                if (emit || emit_not) {
+                    if (o) o[n] = emit ? in : Insn{OP_NOT, 0};
+                    n++;
+                }
+                S[sp++] = res;
+            }
+            peak = max(peak, n);
+        }
+        const uint32_t root = S[0];
+        const uint32_t len = root == TAG_FALSE ? 0u : (root == TAG_TRUE ? BRICK_TRUE : n);
+        const bool active = len != 0 && len != BRICK_TRUE;
+        if (PASS == 0) counts[b] = make_uint2(active ? peak : 0u, active ? 1u : 0u);
+        else info[b] = make_uint2(offs[b].x, len);
+    }
+}
+
+template <int MODE, bool STATS>
+__device__ __forceinline__ bool eval_brick(const DevProg &P, const Bricks &B, const Grid &g, int bx, int y, int z,
+                                           double px, double py, double pz, GStats &gs) {
+    if (!B.info) return eval_code<MODE, STATS>(P.code, P.ncode, P, px, py, pz, gs);
+    const uint2 bi = B.info[((size_t)(z / B.BZ) * B.nby + (y / B.BY)) * g.W + bx];
+    if (bi.y == 0) return false;
+    if (bi.y == BRICK_TRUE) return true;
+    return eval_code<MODE, STATS>(B.code + bi.x, bi.y, P, px, py, pz, gs);
+}
+
+// corners: one wave per occupancy word (64 x corners of one row).
+template <int MODE, bool STATS>
+__global__ __launch_bounds__(256) void k_corners(DevProg P, Bricks B, Grid g, uint64_t *__restrict__ bits,
+                                                 uint32_t *__restrict__ flags, unsigned long long *__restrict__ gst,
+                                                 uint64_t nwords) {
+    const int lane = threadIdx.x & 63;
+    GStats gs;
+    for (uint64_t w = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < nwords; w += (uint64_t)gridDim.x * 4) {
+        const int wx = (int)(w % g.W);
+        const uint64_t row = w / g.W;
+        const int y = (int)(row % g.ny), z = (int)(row / g.ny);
+        const int x = wx * 64 + lane;
+        bool inside = false;
+        if (x < g.nx) inside = eval_brick<MODE, STATS>(P, B, g, wx, y, z, g.xs[x], g.ys[y], g.zs[z], gs);
+        uint64_t m = __ballot(inside);
+        if (lane == 0) bits[w] = m;
+        // panic("solid is true outside of bounds"), mc.go:687-689
+        if (inside && (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1))
+            atomicOr(flags, 1u);
+    }
+    flush_stats<STATS>(gs, gst);
+}
+
 __device__ __forceinline__ bool corner_bit(const Grid &g, const uint64_t *__restrict__ bits, int x, int y, int z) {
     return (bits[word_index(g, x >> 6, y, z)] >> (x & 63)) & 1;
 }
 
 // bisect: mcSearchPoint (mc.go:223-258) per vertex; iters == 0 leaves the
-// edge midpoints of MarchingCubes (Coord3D.Mid, coords.go:196-198).
-__global__ __launch_bounds__(256) void k_bisect(DevProg P, Grid g, const uint64_t *__restrict__ bits,
+// edge midpoints of MarchingCubes (Coord3D.Mid, coords.go:196-198).  The
+// edge lies inside the brick of its low corner (bricks include their +1
+// corners), so that brick's program decides every midpoint.
+template <int MODE, bool STATS>
+__global__ __launch_bounds__(256) void k_bisect(DevProg P, Bricks B, Grid g, const uint64_t *__restrict__ bits,
                                                 const uint64_t *__restrict__ vkeys, double *__restrict__ verts,
-                                                uint64_t nverts, int iters) {
+                                                unsigned long long *__restrict__ gst, uint64_t nverts, int iters) {
+    GStats gs;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nverts;
          v += (uint64_t)gridDim.x * blockDim.x) {
         uint64_t key = vkeys[v];
@@ -482,7 +501,7 @@ __global__ __launch_bounds__(256) void k_bisect(DevProg P, Grid g, const uint64_
             for (int i = 0; i < iters; i++) {
                 double mid = (fp + tp) / 2;
                 p[axis] = mid;
-                if (eval_point(P, p[0], p[1], p[2])) tp = mid;
+                if (eval_brick<MODE, STATS>(P, B, g, x >> 6, y, z, p[0], p[1], p[2], gs)) tp = mid;
                 else fp = mid;
             }
             res = (fp + tp) / 2;
@@ -492,12 +511,15 @@ __global__ __launch_bounds__(256) void k_bisect(DevProg P, Grid g, const uint64_
         verts[3 * v + 1] = p[1];
         verts[3 * v + 2] = p[2];
     }
+    flush_stats<STATS>(gs, gst);
 }
 
+template <int MODE>
 __global__ __launch_bounds__(256) void k_contains(DevProg P, const double *__restrict__ pts, uint8_t *__restrict__ out,
                                                   uint64_t n) {
+    GStats gs;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        out[i] = eval_point(P, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]) ? 1 : 0;
+        out[i] = eval_code<MODE, false>(P.code, P.ncode, P, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], gs) ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -550,9 +572,10 @@ unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 20) {
 struct mc3d_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    DevBuf prog_code, prog_f64, prog_f32, prog_frames, table;
-    DevBuf axes, bits, counts, offs, tiles, totals, flags, vkeys, verts, tris, pts, out8;
-    hipEvent_t ev[6] = {};
+    DevBuf prog_code, prog_f64, prog_f32, prog_frames, prog_boxes, table;
+    DevBuf axes, bits, counts, offs, tiles, totals, flags, gstats, vkeys, pts, out8;
+    DevBuf bcounts, boffs, binfo, bcode;
+    hipEvent_t ev[8] = {};
     bool table_loaded = false;
 };
 
@@ -574,6 +597,9 @@ int upload_program(mc3d_ctx *ctx, const Program &P, DevProg &dp) {
     if ((rc = ctx->prog_f64.ensure(P.f64.size() * sizeof(double) + 16))) return rc;
     if ((rc = ctx->prog_f32.ensure(P.f32.size() * sizeof(float) + 16))) return rc;
     if ((rc = ctx->prog_frames.ensure(P.frames.size() * sizeof(Frame) + 16))) return rc;
+    if ((rc = ctx->prog_boxes.ensure(P.code.size() * sizeof(AtomBox) + 16))) return rc;
+    std::vector<AtomBox> boxes(P.code.size());
+    for (size_t i = 0; i < P.atom_pc.size(); i++) boxes[P.atom_pc[i]] = P.atom_box[i];
     HIP_TRY(hipMemcpyAsync(ctx->prog_code.p, P.code.data(), P.code.size() * sizeof(Insn), hipMemcpyHostToDevice,
                            ctx->stream));
     if (!P.f64.empty()) {
@@ -584,10 +610,15 @@ int upload_program(mc3d_ctx *ctx, const Program &P, DevProg &dp) {
     }
     HIP_TRY(hipMemcpyAsync(ctx->prog_frames.p, P.frames.data(), P.frames.size() * sizeof(Frame),
                            hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->prog_boxes.p, boxes.data(), boxes.size() * sizeof(AtomBox), hipMemcpyHostToDevice,
+                           ctx->stream));
+    // The host vectors above are pageable: the copies complete before return.
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     dp.code = ctx->prog_code.as<Insn>();
     dp.f64 = ctx->prog_f64.as<double>();
     dp.f32 = ctx->prog_f32.as<float>();
     dp.frames = ctx->prog_frames.as<Frame>();
+    dp.boxes = ctx->prog_boxes.as<AtomBox>();
     dp.ncode = (uint32_t)P.code.size();
     dp.pad = 0;
     return MC3D_OK;
@@ -618,6 +649,33 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
     return ms;
+}
+
+// Exclusive scan of n uint2 values from `in` into `out`; totals (64-bit) land
+// in ctx->totals.
+int scan2(mc3d_ctx *ctx, const uint2 *in, uint2 *out, uint64_t n) {
+    const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    int rc;
+    if ((rc = ctx->tiles.ensure(ntiles * sizeof(uint2)))) return rc;
+    k_scan_reduce<<<(unsigned)ntiles, SCAN_BLOCK, 0, ctx->stream>>>(in, n, ctx->tiles.as<uint2>());
+    k_scan_tiles<<<1, SCAN_BLOCK, 0, ctx->stream>>>(ctx->tiles.as<uint2>(), ntiles,
+                                                     ctx->totals.as<unsigned long long>());
+    k_scan_down<<<(unsigned)ntiles, SCAN_BLOCK, 0, ctx->stream>>>(in, n, ctx->tiles.as<uint2>(), out);
+    HIP_TRY(hipGetLastError());
+    return MC3D_OK;
+}
+
+template <int MODE, bool STATS>
+void launch_corners(mc3d_ctx *ctx, const DevProg &dp, const Bricks &B, const Grid &g, uint64_t nwords) {
+    k_corners<MODE, STATS><<<grid_for(nwords, 4), 256, 0, ctx->stream>>>(
+        dp, B, g, ctx->bits.as<uint64_t>(), ctx->flags.as<uint32_t>(), ctx->gstats.as<unsigned long long>(), nwords);
+}
+
+template <int MODE, bool STATS>
+void launch_bisect(mc3d_ctx *ctx, const DevProg &dp, const Bricks &B, const Grid &g, mc3d_mesh *m, int iters) {
+    k_bisect<MODE, STATS><<<grid_for(m->nv, 256), 256, 0, ctx->stream>>>(
+        dp, B, g, ctx->bits.as<uint64_t>(), ctx->vkeys.as<uint64_t>(), m->dverts.as<double>(),
+        ctx->gstats.as<unsigned long long>(), m->nv, iters);
 }
 
 }  // namespace
@@ -673,10 +731,12 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
     *out = nullptr;
     if (iters < 0) iters = 0;
     mc3d_options o{};
-    o.precision = MC3D_PREC_FP64;
+    o.precision = MC3D_PREC_FP32_GUARD;
+    o.cull = 1;
     if (opts) o = *opts;
+    const int mode = o.precision == MC3D_PREC_FP32_GUARD ? 1 : 0;
+    const bool stats = o.count_mismatch != 0;
     HIP_TRY(hipSetDevice(ctx->device));
-    auto t_start = std::chrono::steady_clock::now();
 
     // validateBounds, mc.go:15 / bounder.go:48-52
     double mn[3], mx[3];
@@ -729,32 +789,71 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
         HIP_TRY(hipMemcpy(ctx->table.p, case_table(), 256 * 16, hipMemcpyHostToDevice));
         ctx->table_loaded = true;
     }
-    const uint64_t ntiles = (nwords + SCAN_TILE - 1) / SCAN_TILE;
     if ((rc = ctx->bits.ensure(nwords * 8))) return rc;
     if ((rc = ctx->counts.ensure(nwords * sizeof(uint2)))) return rc;
     if ((rc = ctx->offs.ensure(nwords * sizeof(uint2)))) return rc;
-    if ((rc = ctx->tiles.ensure(ntiles * sizeof(uint2)))) return rc;
     if ((rc = ctx->totals.ensure(2 * sizeof(unsigned long long)))) return rc;
     if ((rc = ctx->flags.ensure(64))) return rc;
-    uint64_t *bits = ctx->bits.as<uint64_t>();
+    if ((rc = ctx->gstats.ensure(64))) return rc;
     HIP_TRY(hipMemsetAsync(ctx->flags.p, 0, 64, ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->gstats.p, 0, 64, ctx->stream));
+    // The spacer vectors are pageable host memory: finish their copies while
+    // they are alive.
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
 
+    std::unique_ptr<mc3d_mesh> mesh(new (std::nothrow) mc3d_mesh);
+    mc3d_mesh *m = mesh.get();
+    if (!m) return MC3D_E_OOM;
+    m->ctx = ctx;
+    mc3d_stats &st = m->stats;
+
+    // ---- plan ----
+    Bricks B{nullptr, nullptr, 1, 1, 1, 1};
     HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
-    k_corners<<<grid_for(nwords, 4), 256, 0, ctx->stream>>>(dp, g, bits, ctx->flags.as<uint32_t>(),
-                                                                       nwords);
-    HIP_TRY(hipGetLastError());
+    if (o.cull) {
+        int bsz = o.brick > 0 ? o.brick : 4;
+        B.BY = B.BZ = bsz;
+        B.nby = (g.ny + bsz - 1) / bsz;
+        B.nbz = (g.nz + bsz - 1) / bsz;
+        const uint64_t nb = (uint64_t)g.W * B.nby * B.nbz;
+        if ((rc = ctx->bcounts.ensure(nb * sizeof(uint2))) || (rc = ctx->boffs.ensure(nb * sizeof(uint2))) ||
+            (rc = ctx->binfo.ensure(nb * sizeof(uint2))))
+            return rc;
+        k_plan<0><<<grid_for(nb, PLAN_BLOCK), PLAN_BLOCK, 0, ctx->stream>>>(dp, g, B, nb, ctx->bcounts.as<uint2>(),
+                                                                          nullptr, nullptr, nullptr);
+        HIP_TRY(hipGetLastError());
+        if ((rc = scan2(ctx, ctx->bcounts.as<uint2>(), ctx->boffs.as<uint2>(), nb))) return rc;
+        unsigned long long bt[2];
+        HIP_TRY(hipMemcpyAsync(bt, ctx->totals.p, sizeof bt, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (bt[0] >= (1ull << 32)) {
+            set_error("brick programs exceed 2^32 instructions");
+            return MC3D_E_UNSUPPORTED;
+        }
+        if ((rc = ctx->bcode.ensure(bt[0] * sizeof(Insn) + 16))) return rc;
+        k_plan<1><<<grid_for(nb, PLAN_BLOCK), PLAN_BLOCK, 0, ctx->stream>>>(
+            dp, g, B, nb, ctx->bcounts.as<uint2>(), ctx->boffs.as<uint2>(), ctx->binfo.as<uint2>(),
+            ctx->bcode.as<Insn>());
+        HIP_TRY(hipGetLastError());
+        B.info = ctx->binfo.as<uint2>();
+        B.code = ctx->bcode.as<Insn>();
+        st.bricks_total = nb;
+        st.bricks_active = bt[1];
+    }
     HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
-    k_classify<<<grid_for(nwords, 256), 256, 0, ctx->stream>>>(g, bits, ctx->table.as<uint8_t>(),
-                                                                 ctx->counts.as<uint2>(), nwords);
-    HIP_TRY(hipGetLastError());
-    k_scan_reduce<<<(unsigned)ntiles, SCAN_BLOCK, 0, ctx->stream>>>(ctx->counts.as<uint2>(), nwords,
-                                                                      ctx->tiles.as<uint2>());
-    k_scan_tiles<<<1, SCAN_BLOCK, 0, ctx->stream>>>(ctx->tiles.as<uint2>(), ntiles,
-                                                     ctx->totals.as<unsigned long long>());
-    k_scan_down<<<(unsigned)ntiles, SCAN_BLOCK, 0, ctx->stream>>>(ctx->counts.as<uint2>(), nwords,
-                                                                    ctx->tiles.as<uint2>(), ctx->offs.as<uint2>());
+
+    // ---- corners ----
+    if (mode == 0) stats ? launch_corners<0, true>(ctx, dp, B, g, nwords) : launch_corners<0, false>(ctx, dp, B, g, nwords);
+    else stats ? launch_corners<1, true>(ctx, dp, B, g, nwords) : launch_corners<1, false>(ctx, dp, B, g, nwords);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
+
+    // ---- classify + scan ----
+    k_classify<<<grid_for(nwords, 256), 256, 0, ctx->stream>>>(g, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
+                                                                 ctx->counts.as<uint2>(), nwords);
+    HIP_TRY(hipGetLastError());
+    if ((rc = scan2(ctx, ctx->counts.as<uint2>(), ctx->offs.as<uint2>(), nwords))) return rc;
+    HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
     unsigned long long totals[2];
     uint32_t flag = 0;
     HIP_TRY(hipMemcpyAsync(totals, ctx->totals.p, sizeof totals, hipMemcpyDeviceToHost, ctx->stream));
@@ -769,40 +868,45 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
         return MC3D_E_UNSUPPORTED;
     }
     const uint64_t nv = totals[0], nt = totals[1];
-
-    std::unique_ptr<mc3d_mesh> mesh(new (std::nothrow) mc3d_mesh);
-    mc3d_mesh *m = mesh.get();
-    if (!m) return MC3D_E_OOM;
-    m->ctx = ctx;
     m->nv = nv;
     m->nt = nt;
     if ((rc = ctx->vkeys.ensure(nv * 8 + 8)) || (rc = m->dverts.ensure(nv * 24 + 8)) ||
         (rc = m->dtris.ensure(nt * 12 + 8)))
         return rc;
-    HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
-    k_emit<<<grid_for(nwords, 256), 256, 0, ctx->stream>>>(g, bits, ctx->table.as<uint8_t>(), ctx->offs.as<uint2>(),
-                                                             ctx->vkeys.as<uint64_t>(), m->dtris.as<uint32_t>(),
-                                                             nwords);
-    HIP_TRY(hipGetLastError());
+
+    // ---- emit ----
     HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
+    k_emit<<<grid_for(nwords, 256), 256, 0, ctx->stream>>>(g, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
+                                                             ctx->offs.as<uint2>(), ctx->vkeys.as<uint64_t>(),
+                                                             m->dtris.as<uint32_t>(), nwords);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
+
+    // ---- bisect ----
     if (nv) {
-        k_bisect<<<grid_for(nv, 256), 256, 0, ctx->stream>>>(dp, g, bits, ctx->vkeys.as<uint64_t>(),
-                                                              m->dverts.as<double>(), nv, iters);
+        if (mode == 0) stats ? launch_bisect<0, true>(ctx, dp, B, g, m, iters) : launch_bisect<0, false>(ctx, dp, B, g, m, iters);
+        else stats ? launch_bisect<1, true>(ctx, dp, B, g, m, iters) : launch_bisect<1, false>(ctx, dp, B, g, m, iters);
         HIP_TRY(hipGetLastError());
     }
-    HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream));
+    unsigned long long gs[4] = {0, 0, 0, 0};
+    if (stats) HIP_TRY(hipMemcpyAsync(gs, ctx->gstats.p, sizeof gs, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
 
-    mc3d_stats &st = m->stats;
     st.corners = (uint64_t)g.nx * g.ny * g.nz;
     st.vertices = nv;
     st.triangles = nt;
     st.bisection_evals = iters > 0 ? nv * (uint64_t)(iters + 1) : 0;
-    st.ms_corners = elapsed(ctx->ev[0], ctx->ev[1]);
-    st.ms_classify = elapsed(ctx->ev[1], ctx->ev[2]);
-    st.ms_emit = elapsed(ctx->ev[3], ctx->ev[4]);
-    st.ms_bisect = elapsed(ctx->ev[4], ctx->ev[5]);
-    st.ms_device = st.ms_corners + st.ms_classify + st.ms_emit + st.ms_bisect;
+    st.guard_evals = gs[0];
+    st.guard_rechecks = gs[1];
+    st.guard_mismatches = gs[2];
+    st.guard_violations = gs[3];
+    st.ms_plan = elapsed(ctx->ev[0], ctx->ev[1]);
+    st.ms_corners = elapsed(ctx->ev[1], ctx->ev[2]);
+    st.ms_classify = elapsed(ctx->ev[2], ctx->ev[3]);
+    st.ms_emit = elapsed(ctx->ev[4], ctx->ev[5]);
+    st.ms_bisect = elapsed(ctx->ev[5], ctx->ev[6]);
+    st.ms_device = st.ms_plan + st.ms_corners + st.ms_classify + st.ms_emit + st.ms_bisect;
     if (!o.device_only) {
         auto t0 = std::chrono::steady_clock::now();
         m->hverts.resize(nv * 3);
@@ -812,7 +916,6 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
         m->on_host = true;
         st.ms_d2h = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
-    (void)t_start;
     *out = mesh.release();
     return MC3D_OK;
 }
@@ -862,7 +965,6 @@ int mc3d_contains(mc3d_ctx *ctx, const mc3d_solid *s, const double *pts, size_t 
         set_error("mc3d_contains: null argument");
         return MC3D_E_INVALID_ARG;
     }
-    (void)precision;
     if (n == 0) return MC3D_OK;
     HIP_TRY(hipSetDevice(ctx->device));
     Program P;
@@ -877,7 +979,10 @@ int mc3d_contains(mc3d_ctx *ctx, const mc3d_solid *s, const double *pts, size_t 
     if ((rc = upload_program(ctx, P, dp))) return rc;
     if ((rc = ctx->pts.ensure(n * 24)) || (rc = ctx->out8.ensure(n))) return rc;
     HIP_TRY(hipMemcpyAsync(ctx->pts.p, pts, n * 24, hipMemcpyHostToDevice, ctx->stream));
-    k_contains<<<grid_for(n, 256), 256, 0, ctx->stream>>>(dp, ctx->pts.as<double>(), ctx->out8.as<uint8_t>(), n);
+    if (precision == MC3D_PREC_FP32_GUARD)
+        k_contains<1><<<grid_for(n, 256), 256, 0, ctx->stream>>>(dp, ctx->pts.as<double>(), ctx->out8.as<uint8_t>(), n);
+    else
+        k_contains<0><<<grid_for(n, 256), 256, 0, ctx->stream>>>(dp, ctx->pts.as<double>(), ctx->out8.as<uint8_t>(), n);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, ctx->out8.p, n, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));

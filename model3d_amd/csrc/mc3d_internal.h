@@ -82,6 +82,7 @@ struct Program {
     std::vector<Insn> code;
     std::vector<double> f64;
     std::vector<float> f32;        // float copies of f64 (guard-band path)
+    std::vector<std::pair<uint32_t, int>> f32_dir;  // f32 slots rounded down (-1) / up (+1)
     std::vector<Frame> frames;
     std::vector<uint32_t> atom_pc; // pc of each atom instruction
     std::vector<AtomBox> atom_box; // per atom, world space

@@ -179,6 +179,35 @@ double sqrt_le_threshold(double r) {
     return s;
 }
 
+// ---------------------------------------------------------------------------
+// Float32 guard bands (eval.h, atom32).
+//
+// Each guarded atom decides "s <= T" where s is a sum of three squares d_i^2
+// and T the float64 threshold.  In float32 every input is rounded (relative
+// error u = 2^-24) and every operation rounds once.  For a point within 2r of
+// the primitive ("near regime") the absolute error of each d_i is at most
+// e_i (derived per primitive below), so
+//   |q32 - s| <= E = sum_i [e_i (4r + e_i) + u (2r + e_i)^2] + 2.2 u * 12 r^2
+// (squares of the perturbed d_i, their rounding, and the two additions over a
+// sum of at most 12 r^2).  "Inside" is decided only when q32 <= T - E, which
+// forces the float32 d_i below ~r, hence the near regime, hence s <= T; an
+// inside point is in the near regime, so q32 <= T + E and it is never
+// decided "outside".  A relative 1e-12 T absorbs the float64 rounding of the
+// reference's own s.  Branch tests (capsule end caps, cylinder slab) get a
+// margin e_br on the projection; inside the margin the atom is re-decided in
+// float64.  Thresholds are rounded outward when stored as float.
+// ---------------------------------------------------------------------------
+const double kU = 1.0 / 16777216.0;  // 2^-24
+
+double band_E(const double e[3], double r, double T) {
+    double E = 0;
+    for (int i = 0; i < 3; i++) E += e[i] * (4 * r + e[i]) + kU * (2 * r + e[i]) * (2 * r + e[i]);
+    return E + 2.2 * kU * 12 * r * r + 1e-12 * fabs(T) + 1e-37;
+}
+
+// True when float32 evaluation is safe from overflow/underflow trouble.
+bool guard_ok(double mag, double r) { return isfinite(mag) && isfinite(r) && r > 1e-15 && mag < 1e15 && r < 1e15; }
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -476,6 +505,8 @@ struct Lowerer {
         P.atom_box.push_back(ab);
     }
     void op(uint32_t o) { P.code.push_back(Insn{o, 0}); }
+    // The float copy of param k of the last atom rounds down (-1) or up (+1).
+    void round_f32(uint32_t k, int dir) { P.f32_dir.push_back({P.code.back().y + k, dir}); }
 
     Box local_box(const Solid &s) {
         V3 mn, mx;
@@ -486,9 +517,19 @@ struct Lowerer {
     bool emit(const Solid &s) {
         const double *p = s.p;
         switch (s.kind) {
-        case K_SPHERE:
-            atom(A_SPHERE, {p[0], p[1], p[2], p[3], sqrt_le_threshold(p[3])}, local_box(s));
+        case K_SPHERE: {
+            const double r = p[3], T = sqrt_le_threshold(r);
+            double e[3], mag = 0;
+            for (int i = 0; i < 3; i++) {
+                e[i] = kU * (2 * fabs(p[i]) + 4 * r) * (1 + 4 * kU);
+                mag = std::max(mag, fabs(p[i]));
+            }
+            double E = band_E(e, r, T);
+            bool ok = guard_ok(mag, r);
+            atom(A_SPHERE, {p[0], p[1], p[2], r, T, ok ? T - E : -INFINITY, ok ? T + E : INFINITY}, local_box(s));
+            round_f32(5, -1), round_f32(6, +1);
             return true;
+        }
         case K_RECT:
             atom(A_RECT, {p[0], p[1], p[2], p[3], p[4], p[5]}, local_box(s));
             return true;
@@ -503,16 +544,56 @@ struct Lowerer {
             V3 v1 = sub(s1, s0);
             double nrm = norm(v1);
             V3 v = scale(v1, 1 / nrm);
-            atom(A_CAPSULE, {s0.x, s0.y, s0.z, s1.x, s1.y, s1.z, v.x, v.y, v.z, nrm, p[6], sqrt_le_threshold(p[6])},
+            const double r = p[6], T = sqrt_le_threshold(r);
+            // Near regime: |p - s0| <= R = L + 2r.
+            const double L = nrm, R = L + 2 * r;
+            const double a0[3] = {s0.x, s0.y, s0.z}, a1[3] = {s1.x, s1.y, s1.z};
+            double ev2 = 0, mag = R;
+            for (int i = 0; i < 3; i++) {
+                ev2 += kU * (2 * fabs(a0[i]) + 2 * R) * (1 + 4 * kU);
+                mag = std::max(mag, std::max(fabs(a0[i]), fabs(a1[i])) + R);
+            }
+            const double emag = (ev2 + 9 * kU * R) * 1.01 + 1e-12 * R;
+            const double ebr = (emag + 1.01 * kU * L + 2 * kU * R) * 1.001 + 1e-12 * R;
+            double e[3];
+            for (int i = 0; i < 3; i++) {
+                double ecl = emag + 4 * kU * R + 2 * kU * fabs(a0[i]) + kU * fabs(a1[i]);
+                e[i] = (kU * (fabs(a0[i]) + R + 2 * r) + ecl) * (1 + 4 * kU);
+            }
+            double E = band_E(e, r, T);
+            bool ok = guard_ok(mag, r) && isfinite(nrm) && nrm > 0;
+            atom(A_CAPSULE,
+                 {s0.x, s0.y, s0.z, s1.x, s1.y, s1.z, v.x, v.y, v.z, nrm, r, T, ok ? ebr : INFINITY,
+                  ok ? T - E : -INFINITY, ok ? T + E : INFINITY},
                  local_box(s));
+            round_f32(12, +1), round_f32(13, -1), round_f32(14, +1);
             return true;
         }
         case K_CYLINDER: {  // shapes.go:588-597, point-independent part
             V3 p1 = of(p), p2 = of(p + 3);
             V3 diff = sub(p1, p2);
             V3 dir = normalize(diff);
-            atom(A_CYLINDER, {p2.x, p2.y, p2.z, dir.x, dir.y, dir.z, norm(diff), p[6], sqrt_le_threshold(p[6])},
+            const double r = p[6], T = sqrt_le_threshold(r), L = norm(diff), R = L + 2 * r;
+            const double a0[3] = {p2.x, p2.y, p2.z};
+            double ev = 0, mag = R;
+            for (int i = 0; i < 3; i++) {
+                ev += kU * (2 * fabs(a0[i]) + 2 * R) * (1 + 4 * kU);
+                mag = std::max(mag, fabs(a0[i]) + R);
+            }
+            const double efrac = (ev + 9 * kU * R) * 1.01 + 1e-12 * R;
+            const double ebr = (efrac + 1.01 * kU * L + 2 * kU * R) * 1.001 + 1e-12 * R;
+            double e[3];
+            for (int i = 0; i < 3; i++) {
+                double eproj = efrac + 2 * kU * fabs(a0[i]) + 3 * kU * R;
+                e[i] = (eproj + kU * (fabs(a0[i]) + R + 2 * r)) * (1 + 4 * kU);
+            }
+            double E = band_E(e, r, T);
+            bool ok = guard_ok(mag, r) && isfinite(L) && L > 0;
+            atom(A_CYLINDER,
+                 {p2.x, p2.y, p2.z, dir.x, dir.y, dir.z, L, r, T, ok ? ebr : INFINITY, ok ? T - E : -INFINITY,
+                  ok ? T + E : INFINITY},
                  local_box(s));
+            round_f32(9, +1), round_f32(10, -1), round_f32(11, +1);
             return true;
         }
         case K_CONE: {  // shapes.go:888-898
@@ -642,6 +723,13 @@ bool lower_solid(const Solid &s, Program &P, std::string &msg) {
     }
     P.f32.resize(P.f64.size());
     for (size_t i = 0; i < P.f64.size(); i++) P.f32[i] = (float)P.f64[i];
+    for (const auto &d : P.f32_dir) {
+        const double v = P.f64[d.first];
+        float f = (float)v;
+        if (d.second < 0 && (double)f > v) f = nextafterf(f, -INFINITY);
+        if (d.second > 0 && (double)f < v) f = nextafterf(f, INFINITY);
+        P.f32[d.first] = f;
+    }
     return true;
 }
 

@@ -68,15 +68,21 @@ def MarchingCubesSearch(
     cull: bool = True,
     device: int = 0,
     count_mismatch: bool = False,
+    brick: int = 0,
 ) -> Mesh:
-    """model3d.MarchingCubesSearch (mc.go:45-49) on the GPU."""
+    """model3d.MarchingCubesSearch (mc.go:45-49) on the GPU.
+
+    precision: MC3D_PREC_FP32_GUARD (default) or MC3D_PREC_FP64; cull: per-brick
+    culling (brick = its y/z edge in corners, 0 = default); count_mismatch:
+    also gather float32-vs-float64 statistics into ``Mesh.stats``.  Every
+    setting yields the same triangles."""
     L = _lib.lib()
     ctx = _lib.context(device)
     b = LibBackend()
     h = s._build(b)
     out = ctypes.c_void_p()
     try:
-        opts = _options(precision, cull, count_mismatch=count_mismatch)
+        opts = _options(precision, cull, count_mismatch=count_mismatch, brick=brick)
         rc = L.mc3d_marching_cubes_search(ctx, h, float(delta), int(iters), ctypes.byref(opts), ctypes.byref(out))
         if rc != _lib.MC3D_OK:
             msg = _lib.last_error()

@@ -86,6 +86,30 @@ def test_csg64_full_size_bit_exact():
     assert np.array_equal(canonical(got64.triangles()), canonical(got.triangles()))
 
 
+def test_culling_and_brick_sizes_do_not_change_results():
+    s = workloads.csg64_solid(2)
+    print("csg64 d=0.01 cull/brick sweep", flush=True)
+    ref = canonical(m3.MarchingCubesSearch(s, 0.01, 8, precision=FP64, cull=False).triangles())
+    for precision in (FP64, GUARD):
+        for cull, brick in ((False, 0), (True, 1), (True, 2), (True, 4), (True, 8), (True, 16)):
+            got = m3.MarchingCubesSearch(s, 0.01, 8, precision=precision, cull=cull, brick=brick)
+            assert np.array_equal(canonical(got.triangles()), ref), (precision, cull, brick)
+
+
+@pytest.mark.parametrize("name", ["csg64", "sphere", "capsule", "cylinder", "csg", "transform_chain"])
+def test_guard_band_never_wrong(name):
+    """Every float32 decision the band calls sure agrees with float64."""
+    s = workloads.csg64_solid(2) if name == "csg64" else solid_catalogue()[name]
+    print(f"guard statistics {name}", flush=True)
+    got = m3.MarchingCubesSearch(s, 0.01, 8, precision=GUARD, count_mismatch=True)
+    st = got.stats
+    assert st["guard_evals"] > 0
+    assert st["guard_violations"] == 0
+    assert st["guard_rechecks"] < st["guard_evals"]
+    plain = m3.MarchingCubesSearch(s, 0.01, 8, precision=FP64)
+    assert np.array_equal(canonical(got.triangles()), canonical(plain.triangles()))
+
+
 def test_mesh_topology_and_volume():
     s = m3.Sphere((0.1, 0.3, -0.2), 1.0)
     got = m3.MarchingCubesSearch(s, 0.05, 8)
