@@ -166,9 +166,13 @@ __device__ __forceinline__ int atom32(uint32_t kind, const float *__restrict__ a
     }
     case A_CYLINDER: {  // [p2(3) dir(3) norm r T ebr Tlo Thi]
         float frac = (x - a[0]) * a[3] + (y - a[1]) * a[4] + (z - a[2]) * a[5];
+        // Slab test against the margin: frac - norm is compared as one rounded
+        // difference (a rounded norm + margin could swallow a point just past
+        // the end cap).
+        float dl = frac - a[6];
         *plain = false;
-        if (frac < -a[9] || frac > a[6] + a[9]) return 0;
-        bool amb = fabsf(frac) <= a[9] || fabsf(frac - a[6]) <= a[9];
+        if (frac < -a[9] || dl > a[9]) return 0;
+        bool amb = fabsf(frac) <= a[9] || fabsf(dl) <= a[9];
         float px = a[0] + a[3] * frac, py = a[1] + a[4] * frac, pz = a[2] + a[5] * frac;
         float d1 = px - x, d2 = py - y, d3 = pz - z;
         float q = d1 * d1 + d2 * d2 + d3 * d3;
