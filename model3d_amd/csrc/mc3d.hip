@@ -137,9 +137,25 @@ __device__ __forceinline__ int cube_case(const CubeCorners &cc, int i) {
     return c;
 }
 
-// classify: per word, number of crossed edges (vertices) and triangles.
+// Appends `item` for the lanes with `pred` set: one atomic per wave.  The
+// list order is arbitrary; consumers place their output by scanned offsets.
+__device__ __forceinline__ void wave_append(bool pred, uint32_t item, uint32_t *__restrict__ list,
+                                            uint32_t *__restrict__ count) {
+    const uint64_t bal = __ballot(pred);
+    if (!bal) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)bal) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(bal));
+    base = __shfl(base, leader);
+    if (pred) list[base + __popcll(bal & ((1ull << lane) - 1))] = item;
+}
+
+// classify: per word, number of crossed edges (vertices) and triangles; the
+// words with any are listed for emit.
 __global__ __launch_bounds__(256) void k_classify(Grid g, const uint64_t *__restrict__ bits,
                                                   const uint8_t *__restrict__ table, uint2 *__restrict__ counts,
+                                                  uint32_t *__restrict__ active, uint32_t *__restrict__ nactive,
                                                   uint64_t nwords) {
     __shared__ uint8_t ntri[256];
     ntri[threadIdx.x] = table[16 * threadIdx.x];
@@ -160,6 +176,7 @@ __global__ __launch_bounds__(256) void k_classify(Grid g, const uint64_t *__rest
             nt += ntri[cube_case(cc, i)];
         }
         counts[w] = make_uint2(nv, nt);
+        wave_append((nv | nt) != 0, (uint32_t)w, active, nactive);
     }
 }
 
@@ -246,66 +263,99 @@ __device__ __forceinline__ uint64_t vkey(int axis, int x, int y, int z) {
     return (uint64_t)axis | ((uint64_t)x << 2) | ((uint64_t)y << 22) | ((uint64_t)z << 42);
 }
 
-// Vertex index of the grid edge (axis, x, y, z): the word's base, then the
-// word's x, y, z crossing masks in that order.
-__device__ __forceinline__ uint32_t vertex_of_edge(const Grid &g, const uint64_t *__restrict__ bits,
-                                                   const uint2 *__restrict__ offs, int axis, int x, int y, int z) {
-    int wx = x >> 6, b = x & 63;
-    Masks m = word_masks(g, bits, wx, y, z);
-    uint32_t idx = offs[word_index(g, wx, y, z)].x;
-    uint64_t below = b ? (~0ull >> (64 - b)) : 0ull;
-    if (axis == 0) return idx + __popcll(m.cx & below);
-    idx += __popcll(m.cx);
-    if (axis == 1) return idx + __popcll(m.cy & below);
-    idx += __popcll(m.cy);
-    return idx + __popcll(m.cz & below);
+// Occupancy word, crossing masks and vertex base of one (row, word); vertex
+// numbering inside a word is: its x crossings, then y, then z, each in bit
+// order, starting at the word's scanned base.
+struct WordInfo {
+    uint64_t bits, cx, cy, cz;
+    uint32_t base, pad;
+};
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// emit: one wave per listed word, one lane per x position.  The 2 x 2 rows
+// by 2 words a cube of this word can touch are staged in LDS, so every lane
+// can number the vertex on any edge of its cube.
 __global__ __launch_bounds__(256) void k_emit(Grid g, const uint64_t *__restrict__ bits,
                                               const uint8_t *__restrict__ table, const uint2 *__restrict__ offs,
-                                              uint64_t *__restrict__ vkeys, uint32_t *__restrict__ tris,
-                                              uint64_t nwords) {
+                                              const uint32_t *__restrict__ active,
+                                              const uint32_t *__restrict__ nactive, uint64_t *__restrict__ vkeys,
+                                              uint32_t *__restrict__ tris) {
     __shared__ uint8_t tab[256 * 16];
+    __shared__ WordInfo wi_all[4][8];
     for (int i = threadIdx.x; i < 256 * 16; i += blockDim.x) tab[i] = table[i];
     __syncthreads();
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
-         w += (uint64_t)gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    WordInfo *wi = wi_all[wave];
+    const uint32_t nact = *nactive;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (uint32_t k = blockIdx.x * 4 + wave; k < nact; k += gridDim.x * 4) {
+        const uint64_t w = active[k];
         const int wx = (int)(w % g.W);
         const uint64_t row = w / g.W;
         const int y = (int)(row % g.ny), z = (int)(row / g.ny);
-        uint2 o = offs[w];
-        // vertices: x, then y, then z crossings, each in bit order
-        Masks m = word_masks(g, bits, wx, y, z);
-        uint32_t v = o.x;
-        uint64_t ms[3] = {m.cx, m.cy, m.cz};
-        for (int a = 0; a < 3; a++) {
-            uint64_t mm = ms[a];
-            while (mm) {
-                int i = __ffsll((unsigned long long)mm) - 1;
-                mm &= mm - 1;
-                vkeys[v++] = vkey(a, wx * 64 + i, y, z);
+        wave_sync();
+        if (lane < 8) {  // entry: bit 0 word +1, bit 1 row +1 in y, bit 2 row +1 in z
+            const int ex = wx + (lane & 1), ey = y + ((lane >> 1) & 1), ez = z + (lane >> 2);
+            WordInfo r{0, 0, 0, 0, 0, 0};
+            if (ex < g.W && ey < g.ny && ez < g.nz) {
+                const size_t wi_idx = word_index(g, ex, ey, ez);
+                Masks m = word_masks(g, bits, ex, ey, ez);
+                r.bits = bits[wi_idx];
+                r.cx = m.cx, r.cy = m.cy, r.cz = m.cz;
+                r.base = offs[wi_idx].x;
             }
+            wi[lane] = r;
         }
-        // triangles, cube by cube in x order, table order within a cube
-        CubeCorners cc = cube_corners(g, bits, wx, y, z);
-        uint64_t mixed = mixed_cubes(cc);
-        uint32_t t = o.y;
-        while (mixed) {
-            int i = __ffsll((unsigned long long)mixed) - 1;
-            mixed &= mixed - 1;
-            const uint8_t *row8 = tab + 16 * cube_case(cc, i);
-            int nt = row8[0];
-            int x = wx * 64 + i;
-            for (int k = 0; k < 3 * nt; k++) {
-                int e = row8[1 + k];
-                int axis = e >> 2, r = e & 3;
-                int ex = x, ey = y, ez = z;
-                if (axis == 0) ey += r & 1, ez += r >> 1;
-                else if (axis == 1) ex += r & 1, ez += r >> 1;
-                else ex += r & 1, ey += r >> 1;
-                tris[3 * (size_t)t + k] = vertex_of_edge(g, bits, offs, axis, ex, ey, ez);
+        wave_sync();
+        const int x = wx * 64 + lane;
+        // vertices of this word
+        {
+            const WordInfo me = wi[0];
+            uint32_t v = me.base;
+            if ((me.cx >> lane) & 1) vkeys[v + __popcll(me.cx & below)] = vkey(0, x, y, z);
+            v += __popcll(me.cx);
+            if ((me.cy >> lane) & 1) vkeys[v + __popcll(me.cy & below)] = vkey(1, x, y, z);
+            v += __popcll(me.cy);
+            if ((me.cz >> lane) & 1) vkeys[v + __popcll(me.cz & below)] = vkey(2, x, y, z);
+        }
+        // triangles of this lane's cube, placed by a ballot prefix sum
+        int cs = 0;
+        for (int dz = 0; dz < 2; dz++)
+            for (int dy = 0; dy < 2; dy++) {
+                const int e = 2 * dy + 4 * dz;
+                const uint64_t b = wi[(dy << 1) | (dz << 2)].bits, bn = wi[1 | (dy << 1) | (dz << 2)].bits;
+                cs |= (int)((b >> lane) & 1) << e;
+                cs |= (int)((((b >> 1) | (bn << 63)) >> lane) & 1) << (e + 1);
             }
-            t += nt;
+        const int nt = (cs == 0 || cs == 255) ? 0 : tab[16 * cs];
+        const uint32_t pre = __popcll(__ballot(nt & 1) & below) + 2 * __popcll(__ballot(nt & 2) & below) +
+                             4 * __popcll(__ballot(nt & 4) & below);
+        const uint32_t t = offs[w].y + pre;
+        for (int kk = 0; kk < 3 * nt; kk++) {
+            const int e = tab[16 * cs + 1 + kk];
+            const int axis = e >> 2, r = e & 3;
+            int ddx = 0, ddy = 0, ddz = 0;
+            if (axis == 0) ddy = r & 1, ddz = r >> 1;
+            else if (axis == 1) ddx = r & 1, ddz = r >> 1;
+            else ddx = r & 1, ddy = r >> 1;
+            const int ex = x + ddx;
+            const WordInfo &o = wi[((ex >> 6) - wx) | (ddy << 1) | (ddz << 2)];
+            const int bb = ex & 63;
+            const uint64_t bl = bb ? (~0ull >> (64 - bb)) : 0ull;
+            uint32_t idx = o.base;
+            if (axis == 0) {
+                idx += __popcll(o.cx & bl);
+            } else if (axis == 1) {
+                idx += __popcll(o.cx) + __popcll(o.cy & bl);
+            } else {
+                idx += __popcll(o.cx) + __popcll(o.cy) + __popcll(o.cz & bl);
+            }
+            tris[3 * (size_t)t + kk] = idx;
         }
     }
 }
@@ -353,7 +403,8 @@ constexpr int PLAN_STACK = 33;
 template <int PASS>
 __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(DevProg P, Grid g, Bricks B, uint64_t nbricks,
                                                      uint2 *__restrict__ counts, const uint2 *__restrict__ offs,
-                                                     uint2 *__restrict__ info, Insn *__restrict__ out) {
+                                                     uint2 *__restrict__ info, Insn *__restrict__ out,
+                                                     uint32_t *__restrict__ act_list) {
     __shared__ uint32_t stk_all[PLAN_BLOCK * PLAN_STACK];
     uint32_t *S = stk_all + threadIdx.x * PLAN_STACK;
     for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nbricks;
@@ -430,8 +481,12 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(DevProg P, Grid g, Bricks B
         const uint32_t root = S[0];
         const uint32_t len = root == TAG_FALSE ? 0u : (root == TAG_TRUE ? BRICK_TRUE : n);
         const bool active = len != 0 && len != BRICK_TRUE;
-        if (PASS == 0) counts[b] = make_uint2(active ? peak : 0u, active ? 1u : 0u);
-        else info[b] = make_uint2(offs[b].x, len);
+        if (PASS == 0) {
+            counts[b] = make_uint2(active ? peak : 0u, active ? 1u : 0u);
+        } else {
+            info[b] = make_uint2(offs[b].x, len);
+            if (active) act_list[offs[b].y] = (uint32_t)b;  // y: exclusive count of active bricks
+        }
     }
 }
 
@@ -464,6 +519,58 @@ __global__ __launch_bounds__(256) void k_corners(DevProg P, Bricks B, Grid g, ui
         // panic("solid is true outside of bounds"), mc.go:687-689
         if (inside && (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1))
             atomicOr(flags, 1u);
+    }
+    flush_stats<STATS>(gs, gst);
+}
+
+// Culled corners, part 1: words of constant bricks are written directly (all
+// false, or every valid corner true).
+__global__ __launch_bounds__(256) void k_fill_const(Bricks B, Grid g, uint64_t *__restrict__ bits,
+                                                    uint32_t *__restrict__ flags, uint64_t nwords) {
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        const int wx = (int)(w % g.W);
+        const uint64_t row = w / g.W;
+        const int y = (int)(row % g.ny), z = (int)(row / g.ny);
+        const uint32_t len = B.info[((size_t)(z / B.BZ) * B.nby + (y / B.BY)) * g.W + wx].y;
+        if (len == 0) {
+            bits[w] = 0;
+        } else if (len == BRICK_TRUE) {
+            const int nvalid = min(64, g.nx - wx * 64);
+            bits[w] = nvalid == 64 ? ~0ull : ((1ull << nvalid) - 1);
+            // panic("solid is true outside of bounds"), mc.go:687-689
+            if (y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1 || wx == 0 || wx * 64 + nvalid == g.nx)
+                atomicOr(flags, 1u);
+        }
+    }
+}
+
+// Culled corners, part 2: one workgroup per active brick; its 4 waves take
+// the brick's rows, all sharing the brick program (scalar loads).
+template <int MODE, bool STATS>
+__global__ __launch_bounds__(256) void k_corners_act(DevProg P, Bricks B, Grid g, const uint32_t *__restrict__ act,
+                                                     uint32_t nact, uint64_t *__restrict__ bits,
+                                                     uint32_t *__restrict__ flags,
+                                                     unsigned long long *__restrict__ gst) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    GStats gs;
+    for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x) {
+        const uint32_t b = act[k];
+        const int bx = (int)(b % g.W);
+        const uint32_t r = b / g.W;
+        const int by = (int)(r % B.nby), bz = (int)(r / B.nby);
+        const uint2 bi = B.info[b];
+        const int x = bx * 64 + lane;
+        for (int j = wave; j < B.BY * B.BZ; j += 4) {
+            const int y = by * B.BY + j % B.BY, z = bz * B.BZ + j / B.BY;
+            if (y >= g.ny || z >= g.nz) continue;  // wave-uniform
+            bool inside = false;
+            if (x < g.nx) inside = eval_code<MODE, STATS>(B.code + bi.x, bi.y, P, g.xs[x], g.ys[y], g.zs[z], gs);
+            const uint64_t m = __ballot(inside);
+            if (lane == 0) bits[word_index(g, bx, y, z)] = m;
+            if (inside && (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1))
+                atomicOr(flags, 1u);
+        }
     }
     flush_stats<STATS>(gs, gst);
 }
@@ -574,7 +681,7 @@ struct mc3d_ctx {
     hipStream_t stream = nullptr;
     DevBuf prog_code, prog_f64, prog_f32, prog_frames, prog_boxes, table;
     DevBuf axes, bits, counts, offs, tiles, totals, flags, gstats, vkeys, pts, out8;
-    DevBuf bcounts, boffs, binfo, bcode;
+    DevBuf bcounts, boffs, binfo, bcode, actb, actw;
     hipEvent_t ev[8] = {};
     bool table_loaded = false;
 };
@@ -666,9 +773,20 @@ int scan2(mc3d_ctx *ctx, const uint2 *in, uint2 *out, uint64_t n) {
 }
 
 template <int MODE, bool STATS>
-void launch_corners(mc3d_ctx *ctx, const DevProg &dp, const Bricks &B, const Grid &g, uint64_t nwords) {
-    k_corners<MODE, STATS><<<grid_for(nwords, 4), 256, 0, ctx->stream>>>(
-        dp, B, g, ctx->bits.as<uint64_t>(), ctx->flags.as<uint32_t>(), ctx->gstats.as<unsigned long long>(), nwords);
+void launch_corners(mc3d_ctx *ctx, const DevProg &dp, const Bricks &B, const Grid &g, uint64_t nwords,
+                    uint32_t nact) {
+    if (!B.info) {
+        k_corners<MODE, STATS><<<grid_for(nwords, 4), 256, 0, ctx->stream>>>(
+            dp, B, g, ctx->bits.as<uint64_t>(), ctx->flags.as<uint32_t>(), ctx->gstats.as<unsigned long long>(),
+            nwords);
+        return;
+    }
+    k_fill_const<<<grid_for(nwords, 256), 256, 0, ctx->stream>>>(B, g, ctx->bits.as<uint64_t>(),
+                                                                   ctx->flags.as<uint32_t>(), nwords);
+    if (nact)
+        k_corners_act<MODE, STATS><<<grid_for(nact, 1), 256, 0, ctx->stream>>>(
+            dp, B, g, ctx->actb.as<uint32_t>(), nact, ctx->bits.as<uint64_t>(), ctx->flags.as<uint32_t>(),
+            ctx->gstats.as<unsigned long long>());
 }
 
 template <int MODE, bool STATS>
@@ -789,7 +907,12 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
         HIP_TRY(hipMemcpy(ctx->table.p, case_table(), 256 * 16, hipMemcpyHostToDevice));
         ctx->table_loaded = true;
     }
+    if (nwords >= (1ull << 32)) {
+        set_error("grid too large: more than 2^32 occupancy words");
+        return MC3D_E_UNSUPPORTED;
+    }
     if ((rc = ctx->bits.ensure(nwords * 8))) return rc;
+    if ((rc = ctx->actw.ensure(nwords * sizeof(uint32_t)))) return rc;
     if ((rc = ctx->counts.ensure(nwords * sizeof(uint2)))) return rc;
     if ((rc = ctx->offs.ensure(nwords * sizeof(uint2)))) return rc;
     if ((rc = ctx->totals.ensure(2 * sizeof(unsigned long long)))) return rc;
@@ -816,11 +939,15 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
         B.nby = (g.ny + bsz - 1) / bsz;
         B.nbz = (g.nz + bsz - 1) / bsz;
         const uint64_t nb = (uint64_t)g.W * B.nby * B.nbz;
+        if (nb >= (1ull << 32)) {
+            set_error("too many culling bricks");
+            return MC3D_E_UNSUPPORTED;
+        }
         if ((rc = ctx->bcounts.ensure(nb * sizeof(uint2))) || (rc = ctx->boffs.ensure(nb * sizeof(uint2))) ||
-            (rc = ctx->binfo.ensure(nb * sizeof(uint2))))
+            (rc = ctx->binfo.ensure(nb * sizeof(uint2))) || (rc = ctx->actb.ensure(nb * sizeof(uint32_t))))
             return rc;
         k_plan<0><<<grid_for(nb, PLAN_BLOCK), PLAN_BLOCK, 0, ctx->stream>>>(dp, g, B, nb, ctx->bcounts.as<uint2>(),
-                                                                          nullptr, nullptr, nullptr);
+                                                                          nullptr, nullptr, nullptr, nullptr);
         HIP_TRY(hipGetLastError());
         if ((rc = scan2(ctx, ctx->bcounts.as<uint2>(), ctx->boffs.as<uint2>(), nb))) return rc;
         unsigned long long bt[2];
@@ -833,7 +960,7 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
         if ((rc = ctx->bcode.ensure(bt[0] * sizeof(Insn) + 16))) return rc;
         k_plan<1><<<grid_for(nb, PLAN_BLOCK), PLAN_BLOCK, 0, ctx->stream>>>(
             dp, g, B, nb, ctx->bcounts.as<uint2>(), ctx->boffs.as<uint2>(), ctx->binfo.as<uint2>(),
-            ctx->bcode.as<Insn>());
+            ctx->bcode.as<Insn>(), ctx->actb.as<uint32_t>());
         HIP_TRY(hipGetLastError());
         B.info = ctx->binfo.as<uint2>();
         B.code = ctx->bcode.as<Insn>();
@@ -843,23 +970,29 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
     HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
 
     // ---- corners ----
-    if (mode == 0) stats ? launch_corners<0, true>(ctx, dp, B, g, nwords) : launch_corners<0, false>(ctx, dp, B, g, nwords);
-    else stats ? launch_corners<1, true>(ctx, dp, B, g, nwords) : launch_corners<1, false>(ctx, dp, B, g, nwords);
+    const uint32_t nact = (uint32_t)st.bricks_active;
+    if (mode == 0)
+        stats ? launch_corners<0, true>(ctx, dp, B, g, nwords, nact) : launch_corners<0, false>(ctx, dp, B, g, nwords, nact);
+    else
+        stats ? launch_corners<1, true>(ctx, dp, B, g, nwords, nact) : launch_corners<1, false>(ctx, dp, B, g, nwords, nact);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
 
     // ---- classify + scan ----
+    uint32_t *nactw_dev = ctx->flags.as<uint32_t>() + 1;  // words with vertices or triangles
     k_classify<<<grid_for(nwords, 256), 256, 0, ctx->stream>>>(g, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
-                                                                 ctx->counts.as<uint2>(), nwords);
+                                                                 ctx->counts.as<uint2>(), ctx->actw.as<uint32_t>(),
+                                                                 nactw_dev, nwords);
     HIP_TRY(hipGetLastError());
     if ((rc = scan2(ctx, ctx->counts.as<uint2>(), ctx->offs.as<uint2>(), nwords))) return rc;
     HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
     unsigned long long totals[2];
-    uint32_t flag = 0;
+    uint32_t flag[2] = {0, 0};
     HIP_TRY(hipMemcpyAsync(totals, ctx->totals.p, sizeof totals, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(&flag, ctx->flags.p, sizeof flag, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(flag, ctx->flags.p, sizeof flag, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (flag) {
+    const uint32_t nactw = flag[1];
+    if (flag[0]) {
         set_error("solid is true outside of bounds");
         return MC3D_E_INSIDE_BOUNDARY;
     }
@@ -876,10 +1009,13 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
 
     // ---- emit ----
     HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
-    k_emit<<<grid_for(nwords, 256), 256, 0, ctx->stream>>>(g, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
-                                                             ctx->offs.as<uint2>(), ctx->vkeys.as<uint64_t>(),
-                                                             m->dtris.as<uint32_t>(), nwords);
-    HIP_TRY(hipGetLastError());
+    if (nactw) {
+        k_emit<<<grid_for(nactw, 4), 256, 0, ctx->stream>>>(g, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
+                                                              ctx->offs.as<uint2>(), ctx->actw.as<uint32_t>(),
+                                                              nactw_dev, ctx->vkeys.as<uint64_t>(),
+                                                              m->dtris.as<uint32_t>());
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
 
     // ---- bisect ----
