@@ -144,6 +144,7 @@ typedef struct {
     double ms_plan, ms_corners, ms_classify, ms_emit, ms_bisect; /* per stage (HIP events) */
     double ms_d2h;             /* copy of results to host */
     double ms_host_callbacks;  /* time spent in mc3d_contains_fn */
+    double ms_call;            /* host wall time of the whole call */
 } mc3d_stats;
 
 /* MarchingCubesSearch(s, delta, iters), model3d/mc.go:45-49 (iters == 0 is

@@ -71,6 +71,7 @@ class mc3d_stats(ctypes.Structure):
         ("ms_bisect", ctypes.c_double),
         ("ms_d2h", ctypes.c_double),
         ("ms_host_callbacks", ctypes.c_double),
+        ("ms_call", ctypes.c_double),
     ]
 
     def as_dict(self):

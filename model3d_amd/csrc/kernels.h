@@ -1,0 +1,660 @@
+// kernels.h -- the device stages of mc3d_marching_cubes_search
+// (model3d/mc.go:14-49), CDNA4 / gfx950, wave64.
+//
+//   plan      per brick (64 x BY x BZ corners), the solid's program folded
+//             against the brick's box: atoms whose conservative box misses
+//             the brick are false, bounds checks and spheres that contain it
+//             are true.  Constant bricks need no evaluation at all.
+//   corners   Solid.Contains at every grid corner into a bit-packed occupancy
+//             grid, one 64-bit word per 64 consecutive x corners of a row:
+//             constant bricks are filled directly, each active brick gets a
+//             workgroup whose waves share the brick program (scalar loads)
+//             and pack their row with a ballot (mc.go:614-692).
+//   classify  per word, crossing masks of the x/y/z grid edges (one vertex
+//             per crossed edge: the reference's vertex dedup, mesh.go:822-849)
+//             and the triangle count of its 64 cubes (mcLookupTable,
+//             mc.go:434-456); one exclusive scan gives vertex and triangle
+//             offsets and the list of words that emit anything.
+//   emit      one wave per listed word, a lane per cube: vertex keys (the
+//             owning edge) and triangles as uint32 vertex-index triples in
+//             table order (mc.go:20-35, 401-407).
+//   bisect    per vertex, mcSearchPoint (mc.go:223-258): the first probe is
+//             the edge's high corner, read from the occupancy grid (same
+//             point, same answer), then `iters` float64 midpoints.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "eval.h"
+
+namespace mc3d {
+
+constexpr uint32_t BRICK_TRUE = 0xffffffffu;  // info.y of a brick where the solid is true
+constexpr uint32_t BRICK_FULL = 0xffffffffu;  // info.x of a brick that uses the unfolded program
+
+// Per-call control block in device memory (zeroed by the upload).
+struct Ctl {
+    uint32_t inside;       // panic("solid is true outside of bounds") seen
+    uint32_t overflow;     // an output index exceeded the buffers' capacity
+    uint32_t nact_bricks;  // bricks whose folded program is not constant
+    uint32_t pad;
+    unsigned long long totals[4];  // scan totals: vertices, triangles, listed words
+    unsigned long long gstats[4];  // guard statistics (GStats order)
+};
+
+struct Grid {
+    const double *__restrict__ xs;
+    const double *__restrict__ ys;
+    const double *__restrict__ zs;
+    int nx, ny, nz, W;  // W = words per row = ceil(nx / 64)
+};
+
+// Per-brick programs: info[b] = {offset into code (or BRICK_FULL), length};
+// length 0: the solid is false everywhere in the brick; BRICK_TRUE: true.
+struct Bricks {
+    const uint2 *__restrict__ info;  // null: no culling, use the full program
+    const Insn *__restrict__ code;
+    int BY, BZ, nby, nbz;
+};
+
+__device__ __forceinline__ size_t word_index(const Grid &g, int wx, int y, int z) {
+    return ((size_t)z * g.ny + y) * g.W + wx;
+}
+
+__device__ __forceinline__ void word_coords(const Grid &g, uint64_t w, int &wx, int &y, int &z) {
+    wx = (int)(w % g.W);
+    const uint64_t row = w / g.W;
+    y = (int)(row % g.ny);
+    z = (int)(row / g.ny);
+}
+
+template <bool STATS>
+__device__ __forceinline__ void flush_stats(const GStats &gs, unsigned long long *__restrict__ out) {
+    if (!STATS) return;
+    __shared__ unsigned int red[4];
+    if (threadIdx.x < 4) red[threadIdx.x] = 0;
+    __syncthreads();
+    if (gs.evals) atomicAdd(&red[0], gs.evals);
+    if (gs.rechecks) atomicAdd(&red[1], gs.rechecks);
+    if (gs.mismatches) atomicAdd(&red[2], gs.mismatches);
+    if (gs.violations) atomicAdd(&red[3], gs.violations);
+    __syncthreads();
+    if (threadIdx.x < 4 && red[threadIdx.x]) atomicAdd(&out[threadIdx.x], (unsigned long long)red[threadIdx.x]);
+}
+
+// Appends `item` for the lanes with `pred` set, one atomic per wave.  The
+// list order is arbitrary: every consumer writes to places fixed by scans.
+__device__ __forceinline__ void wave_append(bool pred, uint32_t item, uint32_t *__restrict__ list,
+                                            uint32_t *__restrict__ count) {
+    const uint64_t bal = __ballot(pred);
+    if (!bal) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)bal) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(bal));
+    base = __shfl(base, leader);
+    if (pred) list[base + __popcll(bal & ((1ull << lane) - 1))] = item;
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+// Atom against a brick box: 0 certainly false, 1 certainly true, 2 evaluate.
+// "False" needs only the atom's conservative world box (solid.cpp widens it
+// far beyond float64 rounding); "true" is proven for untransformed boxes and
+// bounds checks (exact comparisons) and spheres (farthest brick corner inside
+// the threshold with a relative margin far above the rounding of the sum).
+__device__ __forceinline__ int classify_atom(const DevProg &P, uint32_t pc, Insn in, const double *lo,
+                                             const double *hi) {
+    const AtomBox bx = P.boxes[pc];
+    if (hi[0] < bx.lo[0] || lo[0] > bx.hi[0] || hi[1] < bx.lo[1] || lo[1] > bx.hi[1] || hi[2] < bx.lo[2] ||
+        lo[2] > bx.hi[2])
+        return 0;
+    if (insn_frame(in.x) == 0) {
+        const double *a = P.f64 + in.y;
+        const uint32_t kind = insn_kind(in.x);
+        if (kind == A_RECT) {
+            if (in_lo(lo[0], a[0]) && in_lo(lo[1], a[1]) && in_lo(lo[2], a[2]) && in_hi(hi[0], a[3]) &&
+                in_hi(hi[1], a[4]) && in_hi(hi[2], a[5]))
+                return 1;
+        } else if (kind == A_SPHERE) {
+            double s = 0;
+            for (int i = 0; i < 3; i++) {
+                double m = fmax(fabs(lo[i] - a[i]), fabs(hi[i] - a[i]));
+                s += m * m;
+            }
+            if (s * (1 + 1e-12) <= a[4]) return 1;
+        }
+    }
+    return 2;
+}
+
+constexpr uint32_t TAG_EXPR = 0u, TAG_FALSE = 1u << 30, TAG_TRUE = 2u << 30, TAG_MASK = 3u << 30;
+constexpr int PLAN_BLOCK = 128;
+constexpr int PLAN_STACK = 33;
+
+// One thread per brick walks the postfix program with a stack of
+// {false, true, expression starting at output position n} and writes the
+// folded program into the brick's slot of `cap` instructions.  A brick whose
+// folded code ever needs more than `cap` slots uses the unfolded program.
+__global__ __launch_bounds__(PLAN_BLOCK) void k_plan(DevProg P, Grid g, Bricks B, uint64_t nbricks, uint32_t cap,
+                                                     uint2 *__restrict__ info, Insn *__restrict__ out,
+                                                     uint32_t *__restrict__ act_list, Ctl *__restrict__ ctl) {
+    __shared__ uint32_t stk_all[PLAN_BLOCK * PLAN_STACK];
+    uint32_t *S = stk_all + threadIdx.x * PLAN_STACK;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t b_end = (nbricks + blockDim.x - 1) / blockDim.x * blockDim.x;  // whole waves take part
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < b_end; b += stride) {
+        bool active = false;
+        if (b < nbricks) {
+            const int bx = (int)(b % g.W);
+            const uint64_t r = b / g.W;
+            const int by = (int)(r % B.nby), bz = (int)(r / B.nby);
+            const int x0 = bx * 64, y0 = by * B.BY, z0 = bz * B.BZ;
+            const int x1 = min(x0 + 64, g.nx - 1), y1 = min(y0 + B.BY, g.ny - 1), z1 = min(z0 + B.BZ, g.nz - 1);
+            const double lo[3] = {g.xs[x0], g.ys[y0], g.zs[z0]};
+            const double hi[3] = {g.xs[x1], g.ys[y1], g.zs[z1]};
+            Insn *o = out + b * cap;
+            uint32_t n = 0;
+            bool over = false;
+            int sp = 0;
+            for (uint32_t pc = 0; pc < P.ncode; ++pc) {
+                const Insn in = P.code[pc];
+                const uint32_t op = insn_op(in.x);
+                if (op == OP_ATOM) {
+                    int c = classify_atom(P, pc, in, lo, hi);
+                    if (c == 2) {
+                        if (n < cap) o[n] = in;
+                        else over = true;
+                        S[sp++] = TAG_EXPR | n;
+                        n++;
+                    } else {
+                        S[sp++] = c ? TAG_TRUE : TAG_FALSE;
+                    }
+                } else if (op == OP_TRUE || op == OP_FALSE) {
+                    S[sp++] = op == OP_TRUE ? TAG_TRUE : TAG_FALSE;
+                } else if (op == OP_NOT) {
+                    uint32_t a = S[sp - 1];
+                    if (a == TAG_TRUE) S[sp - 1] = TAG_FALSE;
+                    else if (a == TAG_FALSE) S[sp - 1] = TAG_TRUE;
+                    else {
+                        if (n < cap) o[n] = in;
+                        else over = true;
+                        n++;
+                    }
+                } else {
+                    const uint32_t bb = S[--sp], aa = S[--sp];
+                    const bool aE = (aa & TAG_MASK) == TAG_EXPR, bE = (bb & TAG_MASK) == TAG_EXPR;
+                    const bool aT = aa == TAG_TRUE, aF = aa == TAG_FALSE, bT = bb == TAG_TRUE, bF = bb == TAG_FALSE;
+                    // start of the operands' code (constants emitted nothing)
+                    const uint32_t first = aE ? (aa & ~TAG_MASK) : (bE ? (bb & ~TAG_MASK) : n);
+                    uint32_t res;
+                    bool emit = false, emit_not = false;
+                    if (op == OP_OR) {
+                        if (aT || bT) res = TAG_TRUE, n = first;
+                        else if (aF) res = bb;
+                        else if (bF) res = aa;
+                        else res = aa, emit = true;
+                    } else if (op == OP_AND) {
+                        if (aF || bF) res = TAG_FALSE, n = first;
+                        else if (aT) res = bb;
+                        else if (bT) res = aa;
+                        else res = aa, emit = true;
+                    } else if (op == OP_ANDNOT) {  // a & !b
+                        if (aF || bT) res = TAG_FALSE, n = first;
+                        else if (bF) res = aa;
+                        else if (aT) res = bb, emit_not = true;  // b's code is last
+                        else res = aa, emit = true;
+                    } else {  // OP_RANDNOT: !a & b
+                        if (aT || bF) res = TAG_FALSE, n = first;
+                        else if (aF) res = bb;
+                        else if (bT) res = aa, emit_not = true;  // a's code is last
+                        else res = aa, emit = true;
+                    }
+                    if (emit || emit_not) {
+                        if (n < cap) o[n] = emit ? in : Insn{OP_NOT, 0};
+                        else over = true;
+                        n++;
+                    }
+                    S[sp++] = res;
+                }
+            }
+            const uint32_t root = S[0];
+            uint2 bi;
+            if (root == TAG_FALSE) bi = make_uint2(0, 0);
+            else if (root == TAG_TRUE) bi = make_uint2(0, BRICK_TRUE);
+            else if (over) bi = make_uint2(BRICK_FULL, P.ncode), active = true;
+            else bi = make_uint2((uint32_t)(b * cap), n), active = true;
+            info[b] = bi;
+        }
+        wave_append(active, (uint32_t)b, act_list, &ctl->nact_bricks);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// corners
+// ---------------------------------------------------------------------------
+// Without culling: one wave per occupancy word.
+template <int MODE, bool STATS>
+__global__ __launch_bounds__(256) void k_corners(DevProg P, Grid g, uint64_t *__restrict__ bits,
+                                                 Ctl *__restrict__ ctl, uint64_t nwords) {
+    const int lane = threadIdx.x & 63;
+    GStats gs;
+    for (uint64_t w = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < nwords; w += (uint64_t)gridDim.x * 4) {
+        int wx, y, z;
+        word_coords(g, w, wx, y, z);
+        const int x = wx * 64 + lane;
+        bool inside = false;
+        if (x < g.nx) inside = eval_code<MODE, STATS>(P.code, P.ncode, P, g.xs[x], g.ys[y], g.zs[z], gs);
+        const uint64_t m = __ballot(inside);
+        if (lane == 0) bits[w] = m;
+        // panic("solid is true outside of bounds"), mc.go:687-689
+        if (inside && (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1))
+            atomicOr(&ctl->inside, 1u);
+    }
+    flush_stats<STATS>(gs, ctl->gstats);
+}
+
+// Culled, part 1: words of constant bricks are written directly (all false,
+// or every valid corner true).
+__global__ __launch_bounds__(256) void k_fill_const(Bricks B, Grid g, uint64_t *__restrict__ bits,
+                                                    Ctl *__restrict__ ctl, uint64_t nwords) {
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        int wx, y, z;
+        word_coords(g, w, wx, y, z);
+        const uint32_t len = B.info[((size_t)(z / B.BZ) * B.nby + (y / B.BY)) * g.W + wx].y;
+        if (len == 0) {
+            bits[w] = 0;
+        } else if (len == BRICK_TRUE) {
+            const int nvalid = min(64, g.nx - wx * 64);
+            bits[w] = nvalid == 64 ? ~0ull : ((1ull << nvalid) - 1);
+            // panic("solid is true outside of bounds"), mc.go:687-689
+            if (y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1 || wx == 0 || wx * 64 + nvalid == g.nx)
+                atomicOr(&ctl->inside, 1u);
+        }
+    }
+}
+
+__device__ __forceinline__ const Insn *brick_code(const DevProg &P, const Bricks &B, uint2 bi) {
+    return bi.x == BRICK_FULL ? P.code : B.code + bi.x;
+}
+
+// Culled, part 2: one workgroup per active brick; its 4 waves take the
+// brick's rows and share the brick program.
+template <int MODE, bool STATS>
+__global__ __launch_bounds__(256) void k_corners_act(DevProg P, Bricks B, Grid g, const uint32_t *__restrict__ act,
+                                                     uint64_t *__restrict__ bits, Ctl *__restrict__ ctl) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t nact = ctl->nact_bricks;
+    GStats gs;
+    for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x) {
+        const uint32_t b = act[k];
+        const int bx = (int)(b % g.W);
+        const uint32_t r = b / g.W;
+        const int by = (int)(r % B.nby), bz = (int)(r / B.nby);
+        const uint2 bi = B.info[b];
+        const Insn *code = brick_code(P, B, bi);
+        const int x = bx * 64 + lane;
+        for (int j = wave; j < B.BY * B.BZ; j += 4) {
+            const int y = by * B.BY + j % B.BY, z = bz * B.BZ + j / B.BY;
+            if (y >= g.ny || z >= g.nz) continue;  // wave-uniform
+            bool inside = false;
+            if (x < g.nx) inside = eval_code<MODE, STATS>(code, bi.y, P, g.xs[x], g.ys[y], g.zs[z], gs);
+            const uint64_t m = __ballot(inside);
+            if (lane == 0) bits[word_index(g, bx, y, z)] = m;
+            if (inside && (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1))
+                atomicOr(&ctl->inside, 1u);
+        }
+    }
+    flush_stats<STATS>(gs, ctl->gstats);
+}
+
+// ---------------------------------------------------------------------------
+// classify
+// ---------------------------------------------------------------------------
+struct Masks {
+    uint64_t cx, cy, cz;
+};
+
+// Crossed-edge masks of one occupancy word: bit i of cx is the edge from
+// corner (64wx+i, y, z) to its +x neighbour, and likewise for y and z.
+// Padding bits (x >= nx) and boundary corners are zero, so no checks beyond
+// the neighbour words' existence are needed.
+__device__ __forceinline__ Masks word_masks(const Grid &g, const uint64_t *__restrict__ bits, int wx, int y, int z) {
+    const size_t w = word_index(g, wx, y, z);
+    const uint64_t b = bits[w];
+    const uint64_t bn = wx + 1 < g.W ? bits[w + 1] : 0;
+    const uint64_t by = y + 1 < g.ny ? bits[w + g.W] : 0;
+    const uint64_t bz = z + 1 < g.nz ? bits[w + (size_t)g.ny * g.W] : 0;
+    Masks m;
+    m.cx = b ^ ((b >> 1) | (bn << 63));
+    m.cy = b ^ by;
+    m.cz = b ^ bz;
+    return m;
+}
+
+__device__ __forceinline__ uint64_t row_word(const Grid &g, const uint64_t *__restrict__ bits, int wx, int y, int z) {
+    if (wx >= g.W || y >= g.ny || z >= g.nz) return 0;
+    return bits[word_index(g, wx, y, z)];
+}
+
+// c[k] bit i is corner k (k = dx + 2 dy + 4 dz, mc.go:260-301) of cube 64wx+i.
+struct CubeCorners {
+    uint64_t c[8];
+};
+__device__ __forceinline__ CubeCorners cube_corners(const Grid &g, const uint64_t *__restrict__ bits, int wx, int y,
+                                                    int z) {
+    CubeCorners cc;
+    for (int dz = 0; dz < 2; dz++)
+        for (int dy = 0; dy < 2; dy++) {
+            const uint64_t b = row_word(g, bits, wx, y + dy, z + dz);
+            const uint64_t bn = row_word(g, bits, wx + 1, y + dy, z + dz);
+            cc.c[2 * dy + 4 * dz] = b;
+            cc.c[1 + 2 * dy + 4 * dz] = (b >> 1) | (bn << 63);
+        }
+    return cc;
+}
+
+__device__ __forceinline__ uint64_t mixed_cubes(const CubeCorners &cc) {
+    uint64_t any = 0, all = ~0ull;
+    for (int k = 0; k < 8; k++) any |= cc.c[k], all &= cc.c[k];
+    return any & ~all;
+}
+
+__device__ __forceinline__ int cube_case(const CubeCorners &cc, int i) {
+    int c = 0;
+    for (int k = 0; k < 8; k++) c |= (int)((cc.c[k] >> i) & 1) << k;
+    return c;
+}
+
+// counts[w] = {crossed edges, triangles, 1 if the word emits anything, 0}.
+__global__ __launch_bounds__(256) void k_classify(Grid g, const uint64_t *__restrict__ bits,
+                                                  const uint8_t *__restrict__ table, uint4 *__restrict__ counts,
+                                                  uint64_t nwords) {
+    __shared__ uint8_t ntri[256];
+    ntri[threadIdx.x] = table[16 * threadIdx.x];
+    __syncthreads();
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        int wx, y, z;
+        word_coords(g, w, wx, y, z);
+        const Masks m = word_masks(g, bits, wx, y, z);
+        const uint32_t nv = __popcll(m.cx) + __popcll(m.cy) + __popcll(m.cz);
+        const CubeCorners cc = cube_corners(g, bits, wx, y, z);
+        uint64_t mixed = mixed_cubes(cc);
+        uint32_t nt = 0;
+        while (mixed) {
+            const int i = __ffsll((unsigned long long)mixed) - 1;
+            mixed &= mixed - 1;
+            nt += ntri[cube_case(cc, i)];
+        }
+        counts[w] = make_uint4(nv, nt, (nv | nt) != 0 ? 1u : 0u, 0u);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan of uint4 counts (reduce tiles, scan tile sums, downsweep)
+// ---------------------------------------------------------------------------
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+__device__ __forceinline__ uint4 add4(uint4 a, uint4 b) { return make_uint4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+
+__device__ __forceinline__ uint4 block_exclusive(uint4 v, uint4 *sh, uint4 &total) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
+        const uint4 o = t >= off ? sh[t - off] : make_uint4(0, 0, 0, 0);
+        __syncthreads();
+        sh[t] = add4(sh[t], o);
+        __syncthreads();
+    }
+    total = sh[SCAN_BLOCK - 1];
+    const uint4 r = t > 0 ? sh[t - 1] : make_uint4(0, 0, 0, 0);
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint4 *__restrict__ in, uint64_t n,
+                                                            uint4 *__restrict__ tile_sums) {
+    __shared__ uint4 sh[SCAN_BLOCK];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint4 s = make_uint4(0, 0, 0, 0);
+    for (int i = 0; i < SCAN_ITEMS; i++)
+        if (base + i < n) s = add4(s, in[base + i]);
+    uint4 total;
+    block_exclusive(s, sh, total);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of the tile sums in place; totals in 64 bits.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(uint4 *__restrict__ tile_sums, uint64_t ntiles,
+                                                           unsigned long long *__restrict__ totals) {
+    __shared__ uint4 sh[SCAN_BLOCK];
+    uint4 carry = make_uint4(0, 0, 0, 0);
+    unsigned long long t0 = 0, t1 = 0, t2 = 0;
+    for (uint64_t start = 0; start < ntiles; start += SCAN_BLOCK) {
+        const uint64_t i = start + threadIdx.x;
+        const uint4 v = i < ntiles ? tile_sums[i] : make_uint4(0, 0, 0, 0);
+        uint4 total;
+        const uint4 ex = block_exclusive(v, sh, total);
+        if (i < ntiles) tile_sums[i] = add4(ex, carry);
+        carry = add4(carry, total);
+        t0 += total.x, t1 += total.y, t2 += total.z;
+    }
+    if (threadIdx.x == 0) totals[0] = t0, totals[1] = t1, totals[2] = t2;
+}
+
+// Downsweep; words with .z set are also listed at their scanned .z.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint4 *__restrict__ in, uint64_t n,
+                                                          const uint4 *__restrict__ tile_sums,
+                                                          uint4 *__restrict__ out, uint32_t *__restrict__ list) {
+    __shared__ uint4 sh[SCAN_BLOCK];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint4 vals[SCAN_ITEMS];
+    uint4 s = make_uint4(0, 0, 0, 0);
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        vals[i] = base + i < n ? in[base + i] : make_uint4(0, 0, 0, 0);
+        s = add4(s, vals[i]);
+    }
+    uint4 total;
+    uint4 ex = add4(block_exclusive(s, sh, total), tile_sums[blockIdx.x]);
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        if (base + i < n) {
+            out[base + i] = ex;
+            if (list && vals[i].z) list[ex.z] = (uint32_t)(base + i);
+        }
+        ex = add4(ex, vals[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// emit
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t vkey(int axis, int x, int y, int z) {
+    return (uint64_t)axis | ((uint64_t)x << 2) | ((uint64_t)y << 22) | ((uint64_t)z << 42);
+}
+
+// Occupancy word, crossing masks and vertex base of one (row, word).  The
+// vertices of a word are numbered from its scanned base: its x crossings,
+// then y, then z, each in bit order.
+struct WordInfo {
+    uint64_t bits, cx, cy, cz;
+    uint32_t base, pad;
+};
+
+// One wave per listed word, one lane per x position.  The 2 x 2 rows by 2
+// words a cube of this word can touch are staged in LDS, so every lane can
+// number the vertex on any edge of its cube.  Writes at or beyond the
+// capacities are dropped and flagged (the host grows the buffers and runs
+// this stage again).
+__global__ __launch_bounds__(256) void k_emit(Grid g, const uint64_t *__restrict__ bits,
+                                              const uint8_t *__restrict__ table, const uint4 *__restrict__ offs,
+                                              const uint32_t *__restrict__ active, Ctl *__restrict__ ctl,
+                                              uint64_t *__restrict__ vkeys, uint32_t *__restrict__ tris,
+                                              uint64_t vcap, uint64_t tcap) {
+    __shared__ uint8_t tab[256 * 16];
+    __shared__ WordInfo wi_all[4][8];
+    for (int i = threadIdx.x; i < 256 * 16; i += blockDim.x) tab[i] = table[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    WordInfo *wi = wi_all[wave];
+    const uint32_t nact = (uint32_t)ctl->totals[2];
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    bool over = false;
+    for (uint32_t k = blockIdx.x * 4 + wave; k < nact; k += gridDim.x * 4) {
+        const uint64_t w = active[k];
+        int wx, y, z;
+        word_coords(g, w, wx, y, z);
+        wave_sync();
+        if (lane < 8) {  // entry bits: 0 next word, 1 row y+1, 2 row z+1
+            const int ex = wx + (lane & 1), ey = y + ((lane >> 1) & 1), ez = z + (lane >> 2);
+            WordInfo r{0, 0, 0, 0, 0, 0};
+            if (ex < g.W && ey < g.ny && ez < g.nz) {
+                const size_t idx = word_index(g, ex, ey, ez);
+                const Masks m = word_masks(g, bits, ex, ey, ez);
+                r.bits = bits[idx];
+                r.cx = m.cx, r.cy = m.cy, r.cz = m.cz;
+                r.base = offs[idx].x;
+            }
+            wi[lane] = r;
+        }
+        wave_sync();
+        const int x = wx * 64 + lane;
+        {  // vertices of this word
+            const WordInfo me = wi[0];
+            uint64_t v = me.base;
+            const uint64_t k0 = v + __popcll(me.cx & below);
+            if ((me.cx >> lane) & 1) {
+                if (k0 < vcap) vkeys[k0] = vkey(0, x, y, z);
+                else over = true;
+            }
+            v += __popcll(me.cx);
+            const uint64_t k1 = v + __popcll(me.cy & below);
+            if ((me.cy >> lane) & 1) {
+                if (k1 < vcap) vkeys[k1] = vkey(1, x, y, z);
+                else over = true;
+            }
+            v += __popcll(me.cy);
+            const uint64_t k2 = v + __popcll(me.cz & below);
+            if ((me.cz >> lane) & 1) {
+                if (k2 < vcap) vkeys[k2] = vkey(2, x, y, z);
+                else over = true;
+            }
+        }
+        // triangles of this lane's cube, placed by a ballot prefix sum
+        int cs = 0;
+        for (int dz = 0; dz < 2; dz++)
+            for (int dy = 0; dy < 2; dy++) {
+                const int e = 2 * dy + 4 * dz;
+                const uint64_t b = wi[(dy << 1) | (dz << 2)].bits, bn = wi[1 | (dy << 1) | (dz << 2)].bits;
+                cs |= (int)((b >> lane) & 1) << e;
+                cs |= (int)((((b >> 1) | (bn << 63)) >> lane) & 1) << (e + 1);
+            }
+        const int nt = tab[16 * cs];
+        const uint32_t pre = __popcll(__ballot(nt & 1) & below) + 2 * __popcll(__ballot(nt & 2) & below) +
+                             4 * __popcll(__ballot(nt & 4) & below);
+        const uint64_t t = offs[w].y + (uint64_t)pre;
+        if (nt && t + nt > tcap) over = true;
+        else
+            for (int kk = 0; kk < 3 * nt; kk++) {
+                const int e = tab[16 * cs + 1 + kk];
+                const int axis = e >> 2, r = e & 3;
+                int ddx = 0, ddy = 0, ddz = 0;
+                if (axis == 0) ddy = r & 1, ddz = r >> 1;
+                else if (axis == 1) ddx = r & 1, ddz = r >> 1;
+                else ddx = r & 1, ddy = r >> 1;
+                const int ex = x + ddx;
+                const WordInfo &o = wi[((ex >> 6) - wx) | (ddy << 1) | (ddz << 2)];
+                const int bb = ex & 63;
+                const uint64_t bl = bb ? (~0ull >> (64 - bb)) : 0ull;
+                uint32_t idx = o.base;
+                if (axis == 0) idx += __popcll(o.cx & bl);
+                else if (axis == 1) idx += __popcll(o.cx) + __popcll(o.cy & bl);
+                else idx += __popcll(o.cx) + __popcll(o.cy) + __popcll(o.cz & bl);
+                tris[3 * t + kk] = idx;
+            }
+    }
+    if (over) atomicOr(&ctl->overflow, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// bisect
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool corner_bit(const Grid &g, const uint64_t *__restrict__ bits, int x, int y, int z) {
+    return (bits[word_index(g, x >> 6, y, z)] >> (x & 63)) & 1;
+}
+
+template <int MODE, bool STATS>
+__device__ __forceinline__ bool eval_brick(const DevProg &P, const Bricks &B, const Grid &g, int bx, int y, int z,
+                                           double px, double py, double pz, GStats &gs) {
+    if (!B.info) return eval_code<MODE, STATS>(P.code, P.ncode, P, px, py, pz, gs);
+    const uint2 bi = B.info[((size_t)(z / B.BZ) * B.nby + (y / B.BY)) * g.W + bx];
+    if (bi.y == 0) return false;
+    if (bi.y == BRICK_TRUE) return true;
+    return eval_code<MODE, STATS>(brick_code(P, B, bi), bi.y, P, px, py, pz, gs);
+}
+
+// mcSearchPoint (mc.go:223-258) per vertex; iters == 0 leaves the edge
+// midpoints of MarchingCubes (Coord3D.Mid, coords.go:196-198).  An edge lies
+// inside the brick of its low corner (bricks include their +1 corners), so
+// that brick's program decides every midpoint.
+template <int MODE, bool STATS>
+__global__ __launch_bounds__(256) void k_bisect(DevProg P, Bricks B, Grid g, const uint64_t *__restrict__ bits,
+                                                const uint64_t *__restrict__ vkeys, double *__restrict__ verts,
+                                                Ctl *__restrict__ ctl, uint64_t vcap, int iters) {
+    GStats gs;
+    const uint64_t nv = min((uint64_t)ctl->totals[0], vcap);
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t key = vkeys[v];
+        const int axis = (int)(key & 3);
+        const int x = (int)((key >> 2) & 0xfffff), y = (int)((key >> 22) & 0xfffff), z = (int)((key >> 42) & 0xfffff);
+        double p[3] = {g.xs[x], g.ys[y], g.zs[z]};
+        double lo, hi;
+        int hx = x, hy = y, hz = z;
+        if (axis == 0) lo = g.xs[x], hi = g.xs[x + 1], hx++;
+        else if (axis == 1) lo = g.ys[y], hi = g.ys[y + 1], hy++;
+        else lo = g.zs[z], hi = g.zs[z + 1], hz++;
+        double res;
+        if (iters == 0) {
+            res = (lo + hi) * 0.5;
+        } else {
+            double fp = lo, tp = hi;
+            if (!corner_bit(g, bits, hx, hy, hz)) fp = hi, tp = lo;
+            for (int i = 0; i < iters; i++) {
+                const double mid = (fp + tp) / 2;
+                p[axis] = mid;
+                if (eval_brick<MODE, STATS>(P, B, g, x >> 6, y, z, p[0], p[1], p[2], gs)) tp = mid;
+                else fp = mid;
+            }
+            res = (fp + tp) / 2;
+        }
+        p[axis] = res;
+        verts[3 * v] = p[0];
+        verts[3 * v + 1] = p[1];
+        verts[3 * v + 2] = p[2];
+    }
+    flush_stats<STATS>(gs, ctl->gstats);
+}
+
+// Solid.Contains at arbitrary points (the full program, no bricks).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_contains(DevProg P, const double *__restrict__ pts, uint8_t *__restrict__ out,
+                                                  uint64_t n) {
+    GStats gs;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = eval_code<MODE, false>(P.code, P.ncode, P, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], gs) ? 1 : 0;
+}
+
+}  // namespace mc3d

@@ -110,6 +110,53 @@ def test_guard_band_never_wrong(name):
     assert np.array_equal(canonical(got.triangles()), canonical(plain.triangles()))
 
 
+def _search_on(ctx, solid, delta, iters):
+    """MarchingCubesSearch through the C-ABI on an explicit context."""
+    import ctypes
+
+    from model3d_amd import _lib
+    from model3d_amd.solids import LibBackend
+
+    L = _lib.lib()
+    b = LibBackend()
+    h = solid._build(b)
+    out = ctypes.c_void_p()
+    try:
+        _lib.check(L.mc3d_marching_cubes_search(ctx, h, delta, iters, None, ctypes.byref(out)))
+        nv, nt = ctypes.c_size_t(), ctypes.c_size_t()
+        L.mc3d_mesh_counts(out, ctypes.byref(nv), ctypes.byref(nt))
+        v = np.empty((nv.value, 3)), np.empty((nt.value, 3), dtype=np.uint32)
+        _lib.check(L.mc3d_mesh_copy(out, v[0].ctypes.data, v[1].ctypes.data))
+        return v[0][v[1].astype(np.int64)]
+    finally:
+        if out.value:
+            L.mc3d_mesh_free(out)
+        b.release(h)
+
+
+def test_output_capacity_growth_and_reuse():
+    """A fresh context starts with small output buffers: the first large mesh
+    overflows them and the output stages run again; later calls reuse them."""
+    import ctypes
+
+    from model3d_amd import _lib
+
+    L = _lib.lib()
+    ctx = ctypes.c_void_p()
+    _lib.check(L.mc3d_ctx_create(0, ctypes.byref(ctx)))
+    try:
+        big = workloads.csg64_solid(2)
+        print("fresh context: csg64 d=0.005, sphere, csg64 again", flush=True)
+        want_big = canonical(m3.MarchingCubesSearch(big, 0.005, 8).triangles())
+        assert np.array_equal(canonical(_search_on(ctx, big, 0.005, 8)), want_big)
+        small = m3.Sphere((0, 0, 0), 0.5)
+        want_small = canonical(m3.MarchingCubesSearch(small, 0.05, 8).triangles())
+        assert np.array_equal(canonical(_search_on(ctx, small, 0.05, 8)), want_small)
+        assert np.array_equal(canonical(_search_on(ctx, big, 0.005, 8)), want_big)
+    finally:
+        L.mc3d_ctx_destroy(ctx)
+
+
 def test_mesh_topology_and_volume():
     s = m3.Sphere((0.1, 0.3, -0.2), 1.0)
     got = m3.MarchingCubesSearch(s, 0.05, 8)
