@@ -188,69 +188,103 @@ __device__ __forceinline__ bool has_atom32(uint32_t kind) {
     return kind == A_SPHERE || kind == A_CAPSULE || kind == A_CYLINDER;
 }
 
-// Postfix boolean program over atoms; the stack is a bit stack in a register.
-// MODE 0: every atom in float64.  MODE 1: float32 with band, float64 in band.
+// One atom at one point, with the guard statistics.
+template <int MODE, bool STATS>
+__device__ __forceinline__ bool eval_atom(uint32_t kind, uint32_t off, const DevProg &P, double qx, double qy,
+                                          double qz, float fx, float fy, float fz, bool live, GStats &gs) {
+    if (MODE == 1 && has_atom32(kind)) {
+        bool plain;
+        const int r = atom32(kind, P.f32 + off, fx, fy, fz, &plain);
+        if (STATS && live) gs.evals++;
+        if (r < 0) {
+            const bool b = atom64(kind, P.f64 + off, qx, qy, qz);
+            if (STATS && live) gs.rechecks++, gs.mismatches += plain != b;
+            return b;
+        }
+        if (STATS && live) {
+            const bool t = atom64(kind, P.f64 + off, qx, qy, qz);
+            gs.mismatches += plain != t;
+            gs.violations += (r != 0) != t;
+        }
+        return r != 0;
+    }
+    return atom64(kind, P.f64 + off, qx, qy, qz);
+}
+
+// Postfix boolean program over atoms for NP points at once (one walk of the
+// program, whose instructions are wave-uniform scalar loads); each point's
+// stack is a bit stack in a register.  MODE 0: every atom in float64.
+// MODE 1: float32 with band, float64 in the band.  `live` only gates the
+// statistics.
+template <int MODE, bool STATS, int NP>
+__device__ __forceinline__ void eval_code_n(const Insn *__restrict__ code, uint32_t n, const DevProg &P,
+                                            const double *px, const double *py, const double *pz, const bool *live,
+                                            bool *res, GStats &gs) {
+    uint32_t st[NP];
+    double qx[NP], qy[NP], qz[NP];
+    float fx[NP], fy[NP], fz[NP];
+    for (int k = 0; k < NP; k++) {
+        st[k] = 0;
+        qx[k] = px[k], qy[k] = py[k], qz[k] = pz[k];
+        fx[k] = (float)px[k], fy[k] = (float)py[k], fz[k] = (float)pz[k];
+    }
+    uint32_t cur = 0;
+    for (uint32_t pc = 0; pc < n; ++pc) {
+        const Insn in = code[pc];
+        const uint32_t op = insn_op(in.x);
+        if (op == OP_ATOM) {
+            const uint32_t fr = insn_frame(in.x);
+            if (fr != cur) {
+                const Frame f = P.frames[fr];
+                for (int k = 0; k < NP; k++) {
+                    qx[k] = px[k], qy[k] = py[k], qz[k] = pz[k];
+                    apply_frame(P.f64, f, qx[k], qy[k], qz[k]);
+                    if (MODE == 1) fx[k] = (float)qx[k], fy[k] = (float)qy[k], fz[k] = (float)qz[k];
+                }
+                cur = fr;
+            }
+            const uint32_t kind = insn_kind(in.x);
+            for (int k = 0; k < NP; k++)
+                st[k] = (st[k] << 1) |
+                        (uint32_t)eval_atom<MODE, STATS>(kind, in.y, P, qx[k], qy[k], qz[k], fx[k], fy[k], fz[k],
+                                                         live[k], gs);
+        } else {
+            for (int k = 0; k < NP; k++) {
+                uint32_t s = st[k];
+                if (op == OP_OR) {
+                    const uint32_t t = s & 1u;
+                    s = (s >> 1) | t;
+                } else if (op == OP_AND) {
+                    const uint32_t t = s & 1u;
+                    s = (s >> 1) & (~1u | t);
+                } else if (op == OP_ANDNOT) {  // second & !top
+                    const uint32_t t = s & 1u;
+                    s = (s >> 1) & ~t;
+                } else if (op == OP_RANDNOT) {  // !second & top
+                    const uint32_t t = s & 1u;
+                    s >>= 1;
+                    s = (s & ~1u) | (t & ~s & 1u);
+                } else if (op == OP_NOT) {
+                    s ^= 1u;
+                } else if (op == OP_TRUE) {
+                    s = (s << 1) | 1u;
+                } else {
+                    s <<= 1;
+                }
+                st[k] = s;
+            }
+        }
+    }
+    for (int k = 0; k < NP; k++) res[k] = st[k] & 1u;
+}
+
 template <int MODE, bool STATS>
 __device__ __forceinline__ bool eval_code(const Insn *__restrict__ code, uint32_t n, const DevProg &P, double px,
                                           double py, double pz, GStats &gs) {
-    uint32_t st = 0;
-    uint32_t cur = 0;
-    double qx = px, qy = py, qz = pz;
-    float fx = (float)px, fy = (float)py, fz = (float)pz;
-    for (uint32_t pc = 0; pc < n; ++pc) {
-        Insn in = code[pc];
-        uint32_t op = insn_op(in.x);
-        if (op == OP_ATOM) {
-            uint32_t fr = insn_frame(in.x);
-            if (fr != cur) {
-                qx = px, qy = py, qz = pz;
-                apply_frame(P.f64, P.frames[fr], qx, qy, qz);
-                cur = fr;
-                if (MODE == 1) fx = (float)qx, fy = (float)qy, fz = (float)qz;
-            }
-            const uint32_t kind = insn_kind(in.x);
-            bool b;
-            if (MODE == 1 && has_atom32(kind)) {
-                bool plain;
-                int r = atom32(kind, P.f32 + in.y, fx, fy, fz, &plain);
-                if (STATS) gs.evals++;
-                if (r < 0) {
-                    b = atom64(kind, P.f64 + in.y, qx, qy, qz);
-                    if (STATS) gs.rechecks++, gs.mismatches += plain != b;
-                } else {
-                    b = r != 0;
-                    if (STATS) {
-                        bool t = atom64(kind, P.f64 + in.y, qx, qy, qz);
-                        gs.mismatches += plain != t;
-                        gs.violations += b != t;
-                    }
-                }
-            } else {
-                b = atom64(kind, P.f64 + in.y, qx, qy, qz);
-            }
-            st = (st << 1) | (uint32_t)b;
-        } else if (op == OP_OR) {
-            uint32_t t = st & 1u;
-            st = (st >> 1) | t;
-        } else if (op == OP_AND) {
-            uint32_t t = st & 1u;
-            st = (st >> 1) & (~1u | t);
-        } else if (op == OP_ANDNOT) {  // second & !top
-            uint32_t t = st & 1u;
-            st = (st >> 1) & ~t;
-        } else if (op == OP_RANDNOT) {  // !second & top
-            uint32_t t = st & 1u;
-            st >>= 1;
-            st = (st & ~1u) | (t & ~st & 1u);
-        } else if (op == OP_NOT) {
-            st ^= 1u;
-        } else if (op == OP_TRUE) {
-            st = (st << 1) | 1u;
-        } else {
-            st <<= 1;
-        }
-    }
-    return st & 1u;
+    const bool live = true;
+    bool r;
+    eval_code_n<MODE, STATS, 1>(code, n, P, &px, &py, &pz, &live, &r, gs);
+    return r;
 }
 
 }  // namespace mc3d

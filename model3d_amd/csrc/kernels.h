@@ -62,11 +62,15 @@ __device__ __forceinline__ size_t word_index(const Grid &g, int wx, int y, int z
     return ((size_t)z * g.ny + y) * g.W + wx;
 }
 
+// Word index -> (word column, row y, row z).  Word indices fit in 32 bits
+// (the host rejects larger grids), so the divisions stay 32-bit.
 __device__ __forceinline__ void word_coords(const Grid &g, uint64_t w, int &wx, int &y, int &z) {
-    wx = (int)(w % g.W);
-    const uint64_t row = w / g.W;
-    y = (int)(row % g.ny);
-    z = (int)(row / g.ny);
+    const uint32_t w32 = (uint32_t)w;
+    const uint32_t row = w32 / (uint32_t)g.W;
+    wx = (int)(w32 - row * (uint32_t)g.W);
+    const uint32_t zz = row / (uint32_t)g.ny;
+    y = (int)(row - zz * (uint32_t)g.ny);
+    z = (int)zz;
 }
 
 template <bool STATS>
@@ -111,9 +115,9 @@ __device__ __forceinline__ void wave_sync() {
 // far beyond float64 rounding); "true" is proven for untransformed boxes and
 // bounds checks (exact comparisons) and spheres (farthest brick corner inside
 // the threshold with a relative margin far above the rounding of the sum).
-__device__ __forceinline__ int classify_atom(const DevProg &P, uint32_t pc, Insn in, const double *lo,
-                                             const double *hi) {
-    const AtomBox bx = P.boxes[pc];
+__device__ __forceinline__ int classify_atom(const DevProg &P, const AtomBox *boxes, uint32_t pc, Insn in,
+                                             const double *lo, const double *hi) {
+    const AtomBox bx = boxes[pc];
     if (hi[0] < bx.lo[0] || lo[0] > bx.hi[0] || hi[1] < bx.lo[1] || lo[1] > bx.hi[1] || hi[2] < bx.lo[2] ||
         lo[2] > bx.hi[2])
         return 0;
@@ -140,23 +144,36 @@ constexpr uint32_t TAG_EXPR = 0u, TAG_FALSE = 1u << 30, TAG_TRUE = 2u << 30, TAG
 constexpr int PLAN_BLOCK = 128;
 constexpr int PLAN_STACK = 33;
 
+constexpr size_t PLAN_STACK_BYTES = PLAN_BLOCK * PLAN_STACK * sizeof(uint32_t);
+
 // One thread per brick walks the postfix program with a stack of
 // {false, true, expression starting at output position n} and writes the
 // folded program into the brick's slot of `cap` instructions.  A brick whose
 // folded code ever needs more than `cap` slots uses the unfolded program.
+// With `stage`, the program and its atom boxes are first copied to LDS.
 __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(DevProg P, Grid g, Bricks B, uint64_t nbricks, uint32_t cap,
                                                      uint2 *__restrict__ info, Insn *__restrict__ out,
-                                                     uint32_t *__restrict__ act_list, Ctl *__restrict__ ctl) {
-    __shared__ uint32_t stk_all[PLAN_BLOCK * PLAN_STACK];
-    uint32_t *S = stk_all + threadIdx.x * PLAN_STACK;
+                                                     uint32_t *__restrict__ act_list, Ctl *__restrict__ ctl,
+                                                     int stage) {
+    extern __shared__ __align__(16) char plan_smem[];
+    uint32_t *S = (uint32_t *)plan_smem + threadIdx.x * PLAN_STACK;
+    const Insn *code = P.code;
+    const AtomBox *boxes = P.boxes;
+    if (stage) {
+        Insn *sc = (Insn *)(plan_smem + PLAN_STACK_BYTES);
+        AtomBox *sb = (AtomBox *)(sc + P.ncode);
+        for (uint32_t i = threadIdx.x; i < P.ncode; i += blockDim.x) sc[i] = P.code[i], sb[i] = P.boxes[i];
+        __syncthreads();
+        code = sc, boxes = sb;
+    }
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t b_end = (nbricks + blockDim.x - 1) / blockDim.x * blockDim.x;  // whole waves take part
     for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < b_end; b += stride) {
         bool active = false;
         if (b < nbricks) {
-            const int bx = (int)(b % g.W);
-            const uint64_t r = b / g.W;
-            const int by = (int)(r % B.nby), bz = (int)(r / B.nby);
+            const uint32_t r = (uint32_t)b / (uint32_t)g.W;
+            const int bx = (int)((uint32_t)b - r * (uint32_t)g.W);
+            const int bz = (int)(r / (uint32_t)B.nby), by = (int)(r - (uint32_t)bz * (uint32_t)B.nby);
             const int x0 = bx * 64, y0 = by * B.BY, z0 = bz * B.BZ;
             const int x1 = min(x0 + 64, g.nx - 1), y1 = min(y0 + B.BY, g.ny - 1), z1 = min(z0 + B.BZ, g.nz - 1);
             const double lo[3] = {g.xs[x0], g.ys[y0], g.zs[z0]};
@@ -166,10 +183,10 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(DevProg P, Grid g, Bricks B
             bool over = false;
             int sp = 0;
             for (uint32_t pc = 0; pc < P.ncode; ++pc) {
-                const Insn in = P.code[pc];
+                const Insn in = code[pc];
                 const uint32_t op = insn_op(in.x);
                 if (op == OP_ATOM) {
-                    int c = classify_atom(P, pc, in, lo, hi);
+                    int c = classify_atom(P, boxes, pc, in, lo, hi);
                     if (c == 2) {
                         if (n < cap) o[n] = in;
                         else over = true;
@@ -288,30 +305,53 @@ __device__ __forceinline__ const Insn *brick_code(const DevProg &P, const Bricks
 }
 
 // Culled, part 2: one workgroup per active brick; its 4 waves take the
-// brick's rows and share the brick program.
+// brick's rows, four rows per walk of the brick program (every lane
+// evaluates four corners at once, so the program's scalar loads are paid once
+// per four rows).
+constexpr int ROWS_PER_WALK = 4;
+
 template <int MODE, bool STATS>
 __global__ __launch_bounds__(256) void k_corners_act(DevProg P, Bricks B, Grid g, const uint32_t *__restrict__ act,
                                                      uint64_t *__restrict__ bits, Ctl *__restrict__ ctl) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t nact = ctl->nact_bricks;
+    const int nrows = B.BY * B.BZ;
     GStats gs;
     for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x) {
         const uint32_t b = act[k];
-        const int bx = (int)(b % g.W);
-        const uint32_t r = b / g.W;
-        const int by = (int)(r % B.nby), bz = (int)(r / B.nby);
+        const uint32_t r = b / (uint32_t)g.W;
+        const int bx = (int)(b - r * (uint32_t)g.W);
+        const int bz = (int)(r / (uint32_t)B.nby), by = (int)(r - (uint32_t)bz * (uint32_t)B.nby);
         const uint2 bi = B.info[b];
         const Insn *code = brick_code(P, B, bi);
         const int x = bx * 64 + lane;
-        for (int j = wave; j < B.BY * B.BZ; j += 4) {
-            const int y = by * B.BY + j % B.BY, z = bz * B.BZ + j / B.BY;
-            if (y >= g.ny || z >= g.nz) continue;  // wave-uniform
-            bool inside = false;
-            if (x < g.nx) inside = eval_code<MODE, STATS>(code, bi.y, P, g.xs[x], g.ys[y], g.zs[z], gs);
-            const uint64_t m = __ballot(inside);
-            if (lane == 0) bits[word_index(g, bx, y, z)] = m;
-            if (inside && (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1))
-                atomicOr(&ctl->inside, 1u);
+        const bool xin = x < g.nx;
+        const double px0 = xin ? g.xs[x] : 0.0;
+        for (int j0 = wave; j0 < nrows; j0 += 4 * ROWS_PER_WALK) {
+            double px[ROWS_PER_WALK], py[ROWS_PER_WALK], pz[ROWS_PER_WALK];
+            bool ok[ROWS_PER_WALK], live[ROWS_PER_WALK], res[ROWS_PER_WALK];
+            int yy[ROWS_PER_WALK], zz[ROWS_PER_WALK];
+            for (int q = 0; q < ROWS_PER_WALK; q++) {
+                const int j = j0 + 4 * q;
+                yy[q] = by * B.BY + j % B.BY;
+                zz[q] = bz * B.BZ + j / B.BY;
+                ok[q] = j < nrows && yy[q] < g.ny && zz[q] < g.nz;  // wave-uniform
+                live[q] = ok[q] && xin;
+                px[q] = px0;
+                py[q] = ok[q] ? g.ys[yy[q]] : 0.0;
+                pz[q] = ok[q] ? g.zs[zz[q]] : 0.0;
+            }
+            eval_code_n<MODE, STATS, ROWS_PER_WALK>(code, bi.y, P, px, py, pz, live, res, gs);
+            for (int q = 0; q < ROWS_PER_WALK; q++) {
+                if (!ok[q]) continue;  // wave-uniform
+                const int y = yy[q], z = zz[q];
+                const bool inside = res[q] && xin;
+                const uint64_t m = __ballot(inside);
+                if (lane == 0) bits[word_index(g, bx, y, z)] = m;
+                // panic("solid is true outside of bounds"), mc.go:687-689
+                if (inside && (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1))
+                    atomicOr(&ctl->inside, 1u);
+            }
         }
     }
     flush_stats<STATS>(gs, ctl->gstats);

@@ -273,7 +273,7 @@ void launch_bisect(mc3d_ctx *ctx, const DevProg &dp, const Bricks &B, const Grid
 int run_output(mc3d_ctx *ctx, const Upload &U, const Bricks &B, const Grid &g, mc3d_mesh *m, int mode, bool stats,
                int iters, uint64_t vcap, uint64_t tcap, uint64_t nwords) {
     HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
-    k_emit<<<grid_for(nwords, 4, 1u << 16), 256, 0, ctx->stream>>>(
+    k_emit<<<grid_for(nwords, 4, 2048), 256, 0, ctx->stream>>>(
         g, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(), ctx->offs.as<uint4>(), ctx->actw.as<uint32_t>(),
         U.ctl, ctx->vkeys.as<uint64_t>(), m->dtris->as<uint32_t>(), vcap, tcap);
     HIP_TRY(hipGetLastError());
@@ -456,9 +456,12 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
         if ((rc = ctx->binfo.ensure(nb * sizeof(uint2))) || (rc = ctx->actb.ensure(nb * sizeof(uint32_t))) ||
             (rc = ctx->bcode.ensure(nb * cap * sizeof(Insn) + 16)))
             return rc;
-        k_plan<<<grid_for(nb, PLAN_BLOCK), PLAN_BLOCK, 0, ctx->stream>>>(U.dp, g, B, nb, cap, ctx->binfo.as<uint2>(),
-                                                                         ctx->bcode.as<Insn>(),
-                                                                         ctx->actb.as<uint32_t>(), U.ctl);
+        // Stage the program in LDS when it fits beside the per-thread stacks.
+        const size_t staged = PLAN_STACK_BYTES + P.code.size() * (sizeof(Insn) + sizeof(AtomBox));
+        const int stage = staged <= 64 * 1024 ? 1 : 0;
+        k_plan<<<grid_for(nb, PLAN_BLOCK), PLAN_BLOCK, stage ? staged : PLAN_STACK_BYTES, ctx->stream>>>(
+            U.dp, g, B, nb, cap, ctx->binfo.as<uint2>(), ctx->bcode.as<Insn>(), ctx->actb.as<uint32_t>(), U.ctl,
+            stage);
         HIP_TRY(hipGetLastError());
         B.info = ctx->binfo.as<uint2>();
         B.code = ctx->bcode.as<Insn>();
