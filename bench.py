@@ -36,7 +36,7 @@ def workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_gpu(solid, cfg, precision, cull, steps, warmup, count_mismatch=False):
+def run_gpu(solid, cfg, precision, cull, steps, warmup, count_mismatch=False, tile=None):
     from model3d_amd import _lib
     from model3d_amd.solids import LibBackend
 
@@ -47,6 +47,8 @@ def run_gpu(solid, cfg, precision, cull, steps, warmup, count_mismatch=False):
     opts = _lib.mc3d_options()
     opts.precision = precision
     opts.cull = 1 if cull else 0
+    if tile:
+        opts.brick, opts.rows_per_walk = tile["brick"], tile["rows_per_walk"]
     opts.device_only = 1
     opts.count_mismatch = 1 if count_mismatch else 0
     stats = []
@@ -120,7 +122,16 @@ def main():
     from model3d_amd import _lib
 
     solid, cfg = workload(args.workload)
-    wall, st = run_gpu(solid, cfg, _lib.MC3D_PREC_FP32_GUARD, True, args.steps, args.warmup)
+    # Tile shapes from the autotuner (untimed), then the timed steps.
+    from model3d_amd import autotune
+
+    tuned = {}
+    for label, prec in (("fp32_guard", _lib.MC3D_PREC_FP32_GUARD), ("fp64", _lib.MC3D_PREC_FP64)):
+        tuned[label] = autotune.tune(solid, cfg["delta"], cfg["iters"], precision=prec)
+    best = "fp32_guard" if tuned["fp32_guard"]["ms"] <= tuned["fp64"]["ms"] else "fp64"
+    precision = _lib.MC3D_PREC_FP32_GUARD if best == "fp32_guard" else _lib.MC3D_PREC_FP64
+    tile = tuned[best]
+    wall, st = run_gpu(solid, cfg, precision, True, args.steps, args.warmup, tile=tile)
     corners, tris, verts = st[0]["corners"], st[0]["triangles"], st[0]["vertices"]
     ms = 1e3 * wall / args.steps
     result = {
@@ -134,7 +145,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f32+f64" if best == "fp32_guard" else "f64",
         "data": "synthetic",
         "config": {
             "workload": f"{cfg['name']}_iters{cfg['iters']}",
@@ -142,10 +153,16 @@ def main():
             "vertices": verts,
             "triangles": tris,
             "bisection_evals": st[0]["bisection_evals"],
-            "precision": "fp32_guard",
+            "precision": best,
             "cull": True,
+            "brick": tile["brick"],
+            "rows_per_walk": tile["rows_per_walk"],
         },
         "tris_per_s": tris / (wall / args.steps),
+        "autotune": {
+            k: {"brick": v["brick"], "rows_per_walk": v["rows_per_walk"], "best_ms": v["ms"]}
+            for k, v in tuned.items()
+        },
     }
     stage_keys = ("ms_plan", "ms_corners", "ms_classify", "ms_emit", "ms_bisect", "ms_device")
     stages = {k: mean(st, k) for k in stage_keys}

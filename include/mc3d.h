@@ -125,8 +125,9 @@ typedef struct {
     int32_t cull;        /* 1: per-brick interval culling of primitives */
     int32_t device_only; /* 1: leave results on the device; mc3d_mesh_copy fetches them */
     int32_t count_mismatch; /* 1: also count float32 sign mismatches (slower; statistics only) */
-    int32_t brick;       /* brick edge in corners for culling (0: default) */
-    int32_t reserved[3];
+    int32_t brick;       /* culling brick: 64 x brick x brick corners (0: default 4) */
+    int32_t rows_per_walk; /* grid rows evaluated per walk of a brick program: 1, 2 or 4 (0: default 1) */
+    int32_t reserved[2];
 } mc3d_options;
 
 typedef struct {

@@ -47,7 +47,8 @@ class mc3d_options(ctypes.Structure):
         ("device_only", ctypes.c_int32),
         ("count_mismatch", ctypes.c_int32),
         ("brick", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("rows_per_walk", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 

@@ -188,38 +188,56 @@ __device__ __forceinline__ bool has_atom32(uint32_t kind) {
     return kind == A_SPHERE || kind == A_CAPSULE || kind == A_CYLINDER;
 }
 
-// One atom at one point, with the guard statistics.
+// Number of params of each atom kind (the layouts above / in solid.cpp).
+__device__ __forceinline__ uint32_t atom_nparams(uint32_t kind) {
+    switch (kind) {
+    case A_SPHERE: return 7;
+    case A_RECT: return 6;
+    case A_CAPSULE: return 15;
+    case A_CYLINDER: return 12;
+    case A_CONE: return 8;
+    case A_CONESLICE: return 9;
+    case A_TORUS: return 11;
+    case A_COLLIDER: return 7;
+    default: return 1;
+    }
+}
+constexpr uint32_t MAX_ATOM_PARAMS = 16;
+
+// One atom at one point, with the guard statistics.  f64/f32 point at the
+// atom's params.
 template <int MODE, bool STATS>
-__device__ __forceinline__ bool eval_atom(uint32_t kind, uint32_t off, const DevProg &P, double qx, double qy,
+__device__ __forceinline__ bool eval_atom(uint32_t kind, const double *f64, const float *f32, double qx, double qy,
                                           double qz, float fx, float fy, float fz, bool live, GStats &gs) {
     if (MODE == 1 && has_atom32(kind)) {
         bool plain;
-        const int r = atom32(kind, P.f32 + off, fx, fy, fz, &plain);
+        const int r = atom32(kind, f32, fx, fy, fz, &plain);
         if (STATS && live) gs.evals++;
         if (r < 0) {
-            const bool b = atom64(kind, P.f64 + off, qx, qy, qz);
+            const bool b = atom64(kind, f64, qx, qy, qz);
             if (STATS && live) gs.rechecks++, gs.mismatches += plain != b;
             return b;
         }
         if (STATS && live) {
-            const bool t = atom64(kind, P.f64 + off, qx, qy, qz);
+            const bool t = atom64(kind, f64, qx, qy, qz);
             gs.mismatches += plain != t;
             gs.violations += (r != 0) != t;
         }
         return r != 0;
     }
-    return atom64(kind, P.f64 + off, qx, qy, qz);
+    return atom64(kind, f64, qx, qy, qz);
 }
 
 // Postfix boolean program over atoms for NP points at once (one walk of the
-// program, whose instructions are wave-uniform scalar loads); each point's
-// stack is a bit stack in a register.  MODE 0: every atom in float64.
-// MODE 1: float32 with band, float64 in the band.  `live` only gates the
-// statistics.
-template <int MODE, bool STATS, int NP>
-__device__ __forceinline__ void eval_code_n(const Insn *__restrict__ code, uint32_t n, const DevProg &P,
-                                            const double *px, const double *py, const double *pz, const bool *live,
-                                            bool *res, GStats &gs) {
+// program); each point's stack is a bit stack in a register.  The program
+// and the params it refers to (insn.y offsets into f64 / f32) may live in
+// global memory or LDS; instructions are made wave-uniform with readfirstlane
+// so the walk branches on scalars.  MODE 0: every atom in float64.  MODE 1:
+// float32 with band, float64 in the band.  `live` only gates the statistics.
+template <int MODE, bool STATS, int NP, bool UNI>
+__device__ __forceinline__ void eval_code_n(const Insn *code, uint32_t n, const DevProg &P, const double *f64,
+                                            const float *f32, const double *px, const double *py, const double *pz,
+                                            const bool *live, bool *res, GStats &gs) {
     uint32_t st[NP];
     double qx[NP], qy[NP], qz[NP];
     float fx[NP], fy[NP], fz[NP];
@@ -230,7 +248,9 @@ __device__ __forceinline__ void eval_code_n(const Insn *__restrict__ code, uint3
     }
     uint32_t cur = 0;
     for (uint32_t pc = 0; pc < n; ++pc) {
-        const Insn in = code[pc];
+        Insn in = code[pc];
+        if (UNI) in.x = (uint32_t)__builtin_amdgcn_readfirstlane((int)in.x);
+        if (UNI) in.y = (uint32_t)__builtin_amdgcn_readfirstlane((int)in.y);
         const uint32_t op = insn_op(in.x);
         if (op == OP_ATOM) {
             const uint32_t fr = insn_frame(in.x);
@@ -245,9 +265,8 @@ __device__ __forceinline__ void eval_code_n(const Insn *__restrict__ code, uint3
             }
             const uint32_t kind = insn_kind(in.x);
             for (int k = 0; k < NP; k++)
-                st[k] = (st[k] << 1) |
-                        (uint32_t)eval_atom<MODE, STATS>(kind, in.y, P, qx[k], qy[k], qz[k], fx[k], fy[k], fz[k],
-                                                         live[k], gs);
+                st[k] = (st[k] << 1) | (uint32_t)eval_atom<MODE, STATS>(kind, f64 + in.y, f32 + in.y, qx[k], qy[k],
+                                                                         qz[k], fx[k], fy[k], fz[k], live[k], gs);
         } else {
             for (int k = 0; k < NP; k++) {
                 uint32_t s = st[k];
@@ -278,12 +297,12 @@ __device__ __forceinline__ void eval_code_n(const Insn *__restrict__ code, uint3
     for (int k = 0; k < NP; k++) res[k] = st[k] & 1u;
 }
 
-template <int MODE, bool STATS>
+template <int MODE, bool STATS, bool UNI>
 __device__ __forceinline__ bool eval_code(const Insn *__restrict__ code, uint32_t n, const DevProg &P, double px,
                                           double py, double pz, GStats &gs) {
     const bool live = true;
     bool r;
-    eval_code_n<MODE, STATS, 1>(code, n, P, &px, &py, &pz, &live, &r, gs);
+    eval_code_n<MODE, STATS, 1, UNI>(code, n, P, P.f64, P.f32, &px, &py, &pz, &live, &r, gs);
     return r;
 }
 

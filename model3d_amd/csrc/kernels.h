@@ -269,7 +269,7 @@ __global__ __launch_bounds__(256) void k_corners(DevProg P, Grid g, uint64_t *__
         word_coords(g, w, wx, y, z);
         const int x = wx * 64 + lane;
         bool inside = false;
-        if (x < g.nx) inside = eval_code<MODE, STATS>(P.code, P.ncode, P, g.xs[x], g.ys[y], g.zs[z], gs);
+        if (x < g.nx) inside = eval_code<MODE, STATS, true>(P.code, P.ncode, P, g.xs[x], g.ys[y], g.zs[z], gs);
         const uint64_t m = __ballot(inside);
         if (lane == 0) bits[w] = m;
         // panic("solid is true outside of bounds"), mc.go:687-689
@@ -304,15 +304,18 @@ __device__ __forceinline__ const Insn *brick_code(const DevProg &P, const Bricks
     return bi.x == BRICK_FULL ? P.code : B.code + bi.x;
 }
 
-// Culled, part 2: one workgroup per active brick; its 4 waves take the
-// brick's rows, four rows per walk of the brick program (every lane
-// evaluates four corners at once, so the program's scalar loads are paid once
-// per four rows).
-constexpr int ROWS_PER_WALK = 4;
+// Culled, part 2: one workgroup per active brick.  The brick's folded
+// program and the params of its atoms are first staged in LDS (one parallel
+// copy instead of a chain of dependent scalar loads during the walk), then
+// the 4 waves take the brick's rows, NP rows per walk of the program.
+constexpr uint32_t STAGE_INSNS = 64;
 
-template <int MODE, bool STATS>
+template <int MODE, bool STATS, int NP>
 __global__ __launch_bounds__(256) void k_corners_act(DevProg P, Bricks B, Grid g, const uint32_t *__restrict__ act,
                                                      uint64_t *__restrict__ bits, Ctl *__restrict__ ctl) {
+    __shared__ Insn s_code[STAGE_INSNS];
+    __shared__ double s_f64[STAGE_INSNS * MAX_ATOM_PARAMS];
+    __shared__ float s_f32[STAGE_INSNS * MAX_ATOM_PARAMS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t nact = ctl->nact_bricks;
     const int nrows = B.BY * B.BZ;
@@ -323,15 +326,35 @@ __global__ __launch_bounds__(256) void k_corners_act(DevProg P, Bricks B, Grid g
         const int bx = (int)(b - r * (uint32_t)g.W);
         const int bz = (int)(r / (uint32_t)B.nby), by = (int)(r - (uint32_t)bz * (uint32_t)B.nby);
         const uint2 bi = B.info[b];
-        const Insn *code = brick_code(P, B, bi);
+        const bool staged = bi.x != BRICK_FULL && bi.y <= STAGE_INSNS;  // block-uniform
+        __syncthreads();  // the previous brick's walks are done with the LDS copy
+        if (staged) {
+            // thread t copies param j of instruction i (t = i * 16 + j)
+            for (uint32_t t = threadIdx.x; t < bi.y * MAX_ATOM_PARAMS; t += blockDim.x) {
+                const uint32_t i = t / MAX_ATOM_PARAMS, j = t % MAX_ATOM_PARAMS;
+                Insn in = B.code[bi.x + i];
+                if (insn_op(in.x) == OP_ATOM && j < atom_nparams(insn_kind(in.x))) {
+                    s_f64[t] = P.f64[in.y + j];
+                    s_f32[t] = P.f32[in.y + j];
+                }
+                if (j == 0) {
+                    in.y = i * MAX_ATOM_PARAMS;
+                    s_code[i] = in;
+                }
+            }
+        }
+        __syncthreads();
+        const Insn *code = staged ? s_code : brick_code(P, B, bi);
+        const double *f64 = staged ? s_f64 : P.f64;
+        const float *f32 = staged ? s_f32 : P.f32;
         const int x = bx * 64 + lane;
         const bool xin = x < g.nx;
         const double px0 = xin ? g.xs[x] : 0.0;
-        for (int j0 = wave; j0 < nrows; j0 += 4 * ROWS_PER_WALK) {
-            double px[ROWS_PER_WALK], py[ROWS_PER_WALK], pz[ROWS_PER_WALK];
-            bool ok[ROWS_PER_WALK], live[ROWS_PER_WALK], res[ROWS_PER_WALK];
-            int yy[ROWS_PER_WALK], zz[ROWS_PER_WALK];
-            for (int q = 0; q < ROWS_PER_WALK; q++) {
+        for (int j0 = wave; j0 < nrows; j0 += 4 * NP) {
+            double px[NP], py[NP], pz[NP];
+            bool ok[NP], live[NP], res[NP];
+            int yy[NP], zz[NP];
+            for (int q = 0; q < NP; q++) {
                 const int j = j0 + 4 * q;
                 yy[q] = by * B.BY + j % B.BY;
                 zz[q] = bz * B.BZ + j / B.BY;
@@ -341,8 +364,8 @@ __global__ __launch_bounds__(256) void k_corners_act(DevProg P, Bricks B, Grid g
                 py[q] = ok[q] ? g.ys[yy[q]] : 0.0;
                 pz[q] = ok[q] ? g.zs[zz[q]] : 0.0;
             }
-            eval_code_n<MODE, STATS, ROWS_PER_WALK>(code, bi.y, P, px, py, pz, live, res, gs);
-            for (int q = 0; q < ROWS_PER_WALK; q++) {
+            eval_code_n<MODE, STATS, NP, true>(code, bi.y, P, f64, f32, px, py, pz, live, res, gs);
+            for (int q = 0; q < NP; q++) {
                 if (!ok[q]) continue;  // wave-uniform
                 const int y = yy[q], z = zz[q];
                 const bool inside = res[q] && xin;
@@ -545,6 +568,8 @@ __global__ __launch_bounds__(256) void k_emit(Grid g, const uint64_t *__restrict
                                               uint64_t vcap, uint64_t tcap) {
     __shared__ uint8_t tab[256 * 16];
     __shared__ WordInfo wi_all[4][8];
+    __shared__ uint32_t s_pre[4][64];
+    __shared__ uint8_t s_cs[4][64];
     for (int i = threadIdx.x; i < 256 * 16; i += blockDim.x) tab[i] = table[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -604,26 +629,40 @@ __global__ __launch_bounds__(256) void k_emit(Grid g, const uint64_t *__restrict
         const int nt = tab[16 * cs];
         const uint32_t pre = __popcll(__ballot(nt & 1) & below) + 2 * __popcll(__ballot(nt & 2) & below) +
                              4 * __popcll(__ballot(nt & 4) & below);
-        const uint64_t t = offs[w].y + (uint64_t)pre;
-        if (nt && t + nt > tcap) over = true;
-        else
-            for (int kk = 0; kk < 3 * nt; kk++) {
-                const int e = tab[16 * cs + 1 + kk];
-                const int axis = e >> 2, r = e & 3;
-                int ddx = 0, ddy = 0, ddz = 0;
-                if (axis == 0) ddy = r & 1, ddz = r >> 1;
-                else if (axis == 1) ddx = r & 1, ddz = r >> 1;
-                else ddx = r & 1, ddy = r >> 1;
-                const int ex = x + ddx;
-                const WordInfo &o = wi[((ex >> 6) - wx) | (ddy << 1) | (ddz << 2)];
-                const int bb = ex & 63;
-                const uint64_t bl = bb ? (~0ull >> (64 - bb)) : 0ull;
-                uint32_t idx = o.base;
-                if (axis == 0) idx += __popcll(o.cx & bl);
-                else if (axis == 1) idx += __popcll(o.cx) + __popcll(o.cy & bl);
-                else idx += __popcll(o.cx) + __popcll(o.cy) + __popcll(o.cz & bl);
-                tris[3 * t + kk] = idx;
-            }
+        const uint32_t tw = __shfl(pre + nt, 63);  // triangles of the word
+        const uint64_t t0 = offs[w].y;
+        if (t0 + tw > tcap) {
+            over = true;
+            continue;  // wave-uniform
+        }
+        // Spread the word's 3*tw triangle corners over the lanes: corner j
+        // belongs to triangle j/3 of the lane whose prefix covers it.
+        s_pre[wave][lane] = pre;
+        s_cs[wave][lane] = (uint8_t)cs;
+        wave_sync();
+        for (uint32_t j = lane; j < 3 * tw; j += 64) {
+            const uint32_t ti = j / 3, ec = j - 3 * ti;
+            int L = 0;
+            for (int step = 32; step; step >>= 1)
+                if (L + step < 64 && s_pre[wave][L + step] <= ti) L += step;
+            const int c = s_cs[wave][L];
+            const int kk = 3 * (int)(ti - s_pre[wave][L]) + (int)ec;
+            const int e = tab[16 * c + 1 + kk];
+            const int axis = e >> 2, r = e & 3;
+            int ddx = 0, ddy = 0, ddz = 0;
+            if (axis == 0) ddy = r & 1, ddz = r >> 1;
+            else if (axis == 1) ddx = r & 1, ddz = r >> 1;
+            else ddx = r & 1, ddy = r >> 1;
+            const int ex = wx * 64 + L + ddx;
+            const WordInfo &o = wi[((ex >> 6) - wx) | (ddy << 1) | (ddz << 2)];
+            const int bb = ex & 63;
+            const uint64_t bl = bb ? (~0ull >> (64 - bb)) : 0ull;
+            uint32_t idx = o.base;
+            if (axis == 0) idx += __popcll(o.cx & bl);
+            else if (axis == 1) idx += __popcll(o.cx) + __popcll(o.cy & bl);
+            else idx += __popcll(o.cx) + __popcll(o.cy) + __popcll(o.cz & bl);
+            tris[3 * t0 + j] = idx;
+        }
     }
     if (over) atomicOr(&ctl->overflow, 1u);
 }
@@ -638,11 +677,12 @@ __device__ __forceinline__ bool corner_bit(const Grid &g, const uint64_t *__rest
 template <int MODE, bool STATS>
 __device__ __forceinline__ bool eval_brick(const DevProg &P, const Bricks &B, const Grid &g, int bx, int y, int z,
                                            double px, double py, double pz, GStats &gs) {
-    if (!B.info) return eval_code<MODE, STATS>(P.code, P.ncode, P, px, py, pz, gs);
+    // Lanes of one wave may sit in different bricks: no readfirstlane here.
+    if (!B.info) return eval_code<MODE, STATS, false>(P.code, P.ncode, P, px, py, pz, gs);
     const uint2 bi = B.info[((size_t)(z / B.BZ) * B.nby + (y / B.BY)) * g.W + bx];
     if (bi.y == 0) return false;
     if (bi.y == BRICK_TRUE) return true;
-    return eval_code<MODE, STATS>(brick_code(P, B, bi), bi.y, P, px, py, pz, gs);
+    return eval_code<MODE, STATS, false>(brick_code(P, B, bi), bi.y, P, px, py, pz, gs);
 }
 
 // mcSearchPoint (mc.go:223-258) per vertex; iters == 0 leaves the edge
@@ -694,7 +734,9 @@ __global__ __launch_bounds__(256) void k_contains(DevProg P, const double *__res
                                                   uint64_t n) {
     GStats gs;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        out[i] = eval_code<MODE, false>(P.code, P.ncode, P, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], gs) ? 1 : 0;
+        out[i] = eval_code<MODE, false, true>(P.code, P.ncode, P, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], gs)
+                     ? 1
+                     : 0;
 }
 
 }  // namespace mc3d

@@ -251,15 +251,23 @@ int scan4(mc3d_ctx *ctx, const uint4 *in, uint4 *out, uint64_t n, unsigned long 
 
 template <int MODE, bool STATS>
 void launch_corners(mc3d_ctx *ctx, const DevProg &dp, const Bricks &B, const Grid &g, Ctl *ctl, uint64_t nwords,
-                    uint64_t nbricks) {
+                    uint64_t nbricks, int rows_per_walk) {
     if (!B.info) {
         k_corners<MODE, STATS><<<grid_for(nwords, 4), 256, 0, ctx->stream>>>(dp, g, ctx->bits.as<uint64_t>(), ctl,
                                                                              nwords);
         return;
     }
     k_fill_const<<<grid_for(nwords, 256), 256, 0, ctx->stream>>>(B, g, ctx->bits.as<uint64_t>(), ctl, nwords);
-    k_corners_act<MODE, STATS><<<grid_for(nbricks, 1, 1u << 15), 256, 0, ctx->stream>>>(
-        dp, B, g, ctx->actb.as<uint32_t>(), ctx->bits.as<uint64_t>(), ctl);
+    const unsigned grid = grid_for(nbricks, 1, 1u << 15);
+    if (rows_per_walk >= 4)
+        k_corners_act<MODE, STATS, 4><<<grid, 256, 0, ctx->stream>>>(dp, B, g, ctx->actb.as<uint32_t>(),
+                                                                     ctx->bits.as<uint64_t>(), ctl);
+    else if (rows_per_walk == 2)
+        k_corners_act<MODE, STATS, 2><<<grid, 256, 0, ctx->stream>>>(dp, B, g, ctx->actb.as<uint32_t>(),
+                                                                     ctx->bits.as<uint64_t>(), ctl);
+    else
+        k_corners_act<MODE, STATS, 1><<<grid, 256, 0, ctx->stream>>>(dp, B, g, ctx->actb.as<uint32_t>(),
+                                                                     ctx->bits.as<uint64_t>(), ctl);
 }
 
 template <int MODE, bool STATS>
@@ -470,12 +478,13 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
     HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
 
     // ---- corners ----
+    const int rpw = o.rows_per_walk > 0 ? o.rows_per_walk : 1;
     if (mode == 0)
-        stats ? launch_corners<0, true>(ctx, U.dp, B, g, U.ctl, nwords, nb)
-              : launch_corners<0, false>(ctx, U.dp, B, g, U.ctl, nwords, nb);
+        stats ? launch_corners<0, true>(ctx, U.dp, B, g, U.ctl, nwords, nb, rpw)
+              : launch_corners<0, false>(ctx, U.dp, B, g, U.ctl, nwords, nb, rpw);
     else
-        stats ? launch_corners<1, true>(ctx, U.dp, B, g, U.ctl, nwords, nb)
-              : launch_corners<1, false>(ctx, U.dp, B, g, U.ctl, nwords, nb);
+        stats ? launch_corners<1, true>(ctx, U.dp, B, g, U.ctl, nwords, nb, rpw)
+              : launch_corners<1, false>(ctx, U.dp, B, g, U.ctl, nwords, nb, rpw);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
 

@@ -49,13 +49,14 @@ class Mesh:
         return float(abs(np.sum(det / 6.0)))
 
 
-def _options(precision, cull, device_only=False, count_mismatch=False, brick=0):
+def _options(precision, cull, device_only=False, count_mismatch=False, brick=0, rows_per_walk=0):
     o = _lib.mc3d_options()
     o.precision = precision
     o.cull = 1 if cull else 0
     o.device_only = 1 if device_only else 0
     o.count_mismatch = 1 if count_mismatch else 0
     o.brick = brick
+    o.rows_per_walk = rows_per_walk
     return o
 
 
@@ -69,6 +70,7 @@ def MarchingCubesSearch(
     device: int = 0,
     count_mismatch: bool = False,
     brick: int = 0,
+    rows_per_walk: int = 0,
 ) -> Mesh:
     """model3d.MarchingCubesSearch (mc.go:45-49) on the GPU.
 
@@ -82,7 +84,7 @@ def MarchingCubesSearch(
     h = s._build(b)
     out = ctypes.c_void_p()
     try:
-        opts = _options(precision, cull, count_mismatch=count_mismatch, brick=brick)
+        opts = _options(precision, cull, count_mismatch=count_mismatch, brick=brick, rows_per_walk=rows_per_walk)
         rc = L.mc3d_marching_cubes_search(ctx, h, float(delta), int(iters), ctypes.byref(opts), ctypes.byref(out))
         if rc != _lib.MC3D_OK:
             msg = _lib.last_error()

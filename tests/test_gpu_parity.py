@@ -91,9 +91,10 @@ def test_culling_and_brick_sizes_do_not_change_results():
     print("csg64 d=0.01 cull/brick sweep", flush=True)
     ref = canonical(m3.MarchingCubesSearch(s, 0.01, 8, precision=FP64, cull=False).triangles())
     for precision in (FP64, GUARD):
-        for cull, brick in ((False, 0), (True, 1), (True, 2), (True, 4), (True, 8), (True, 16)):
-            got = m3.MarchingCubesSearch(s, 0.01, 8, precision=precision, cull=cull, brick=brick)
-            assert np.array_equal(canonical(got.triangles()), ref), (precision, cull, brick)
+        for cull, brick, rpw in ((False, 0, 0), (True, 1, 1), (True, 2, 2), (True, 4, 4), (True, 8, 1),
+                                 (True, 16, 4), (True, 3, 2)):
+            got = m3.MarchingCubesSearch(s, 0.01, 8, precision=precision, cull=cull, brick=brick, rows_per_walk=rpw)
+            assert np.array_equal(canonical(got.triangles()), ref), (precision, cull, brick, rpw)
 
 
 @pytest.mark.parametrize("name", ["csg64", "sphere", "capsule", "cylinder", "csg", "transform_chain"])
