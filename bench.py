@@ -33,6 +33,11 @@ def workload(name):
         return workloads.csg64_solid(c["seed"], c["n_prims"]), c
     if name == "sphere":
         return workloads.sphere_solid(), workloads.SPHERE_CONFIG
+    if name == "collider":
+        c = dict(workloads.COLLIDER_CONFIG)
+        solid, delta, ntri = workloads.collider_solid(c["seed"], c["n_prims"], c["mesh_delta"])
+        c["delta"], c["mesh_triangles"] = delta, ntri
+        return solid, c
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -88,6 +93,27 @@ def cpu_baseline(solid, cfg):
     except AttributeError:
         threads = os.cpu_count() or 1
     threads = max(1, min(threads, 16))
+    if cfg.get("mesh_triangles"):
+        # Brute-force ray parity over the whole mesh is far too slow for the
+        # full grid on the CPU: time Contains on a sample of grid corners.
+        import numpy as np
+
+        from oracle.binding import oracle_contains
+
+        mn, mx = solid.Min(), solid.Max()
+        rng = np.random.default_rng(0)
+        pts = rng.uniform(mn, mx, size=(2000, 3))
+        t0 = time.perf_counter()
+        oracle_contains(solid, pts)
+        wall = time.perf_counter() - t0
+        return {
+            "value": len(pts) / wall,
+            "unit": "corner evals/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"{len(pts)} Contains calls at random points in the bounds, one thread, brute-force ray "
+            f"parity over {cfg['mesh_triangles']} triangles, {wall:.2f} s",
+        }
     t0 = time.perf_counter()
     tris, info = oracle_search(solid, cfg["delta"], cfg["iters"], threads=threads)
     wall = time.perf_counter() - t0
@@ -107,7 +133,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="csg64", choices=["csg64", "sphere"])
+    ap.add_argument("--workload", default="csg64", choices=["csg64", "sphere", "collider"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
     args = ap.parse_args()

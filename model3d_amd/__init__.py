@@ -9,7 +9,7 @@ from ._lib import (  # noqa: F401
     MC3D_PREC_FP64,
     MC3DError,
 )
-from .mc import Contains, MarchingCubes, MarchingCubesSearch, Mesh, Panic, case_table  # noqa: F401
+from .mc import Contains, MarchingCubes, MarchingCubesSearch, Mesh, NewMeshRect, Panic, case_table  # noqa: F401
 from .solids import (  # noqa: F401
     CacheSolidBounds,
     Capsule,

@@ -17,9 +17,46 @@ struct DevProg {
     const float *__restrict__ f32;
     const Frame *__restrict__ frames;
     const AtomBox *__restrict__ boxes;  // world-space box per pc (atoms only)
+    const DevCollider *__restrict__ colliders;
     uint32_t ncode;
     uint32_t pad;
 };
+
+// Go math.Min(c, m) == m  <=>  m == -Inf || c >= m   (and the Max twin); this
+// is Rect.Contains / InBounds (shapes.go:171-173, bounder.go:23-27) including
+// the NaN and signed-zero cases.
+__device__ __forceinline__ bool in_lo(double c, double m) { return m == -INFINITY || c >= m; }
+__device__ __forceinline__ bool in_hi(double c, double m) { return m == INFINITY || c <= m; }
+
+// ColliderSolid.Contains with inset 0 (solid.go:292-300): InBounds, then the
+// parity of ray hits along the fixed direction (collisions.go:119-134).  Only
+// the triangles binned in the point's projected cell are tested; each test is
+// Triangle.rayCollision (primitives.go:207-249) from its point-independent
+// constants, counted by RayCollisions when `scale < 0` is false.
+__device__ __forceinline__ bool collider_contains(const DevCollider &C, const double *__restrict__ a, double x,
+                                                  double y, double z) {
+    if (!(in_lo(x, a[1]) && in_lo(y, a[2]) && in_lo(z, a[3]) && in_hi(x, a[4]) && in_hi(y, a[5]) && in_hi(z, a[6])))
+        return false;
+    const double fu = (x * C.u[0] + y * C.u[1] + z * C.u[2] - C.u0) * C.inv_cell;
+    const double fv = (x * C.v[0] + y * C.v[1] + z * C.v[2] - C.v0) * C.inv_cell;
+    if (!(fu >= 0 && fu < C.nu && fv >= 0 && fv < C.nv)) return false;  // no triangle can be hit
+    const uint32_t cell = (uint32_t)fv * (uint32_t)C.nu + (uint32_t)fu;
+    const uint32_t e = C.cell_start[cell + 1];
+    const double Dx = 0.5224892708603626, Dy = 0.10494477243214506, Dz = 0.43558938446126527;
+    uint32_t hits = 0;
+    for (uint32_t i = C.cell_start[cell]; i < e; i++) {
+        const double *r = C.rec + (size_t)RAY_REC * C.cell_tris[i];
+        const double ox = x - r[0], oy = y - r[1], oz = z - r[2];
+        const double bary1 = r[6] * (ox * r[3] + oy * r[4] + oz * r[5]);
+        if (bary1 < 0 || bary1 > 1) continue;
+        const double c2x = oy * r[9] - oz * r[8], c2y = oz * r[7] - ox * r[9], c2z = ox * r[8] - oy * r[7];
+        const double bary2 = r[6] * (Dx * c2x + Dy * c2y + Dz * c2z);
+        if (bary2 < 0 || bary1 + bary2 > 1) continue;
+        const double scale = r[6] * (r[10] * c2x + r[11] * c2y + r[12] * c2z);
+        hits += !(scale < 0);
+    }
+    return hits & 1u;
+}
 
 // Guard statistics, kept in registers and reduced once per block.
 struct GStats {
@@ -28,12 +65,6 @@ struct GStats {
     uint32_t mismatches = 0;  // plain float32 decision != float64 decision
     uint32_t violations = 0;  // band said sure but float64 disagrees (must stay 0)
 };
-
-// Go math.Min(c, m) == m  <=>  m == -Inf || c >= m   (and the Max twin); this
-// is Rect.Contains / InBounds (shapes.go:171-173, bounder.go:23-27) including
-// the NaN and signed-zero cases.
-__device__ __forceinline__ bool in_lo(double c, double m) { return m == -INFINITY || c >= m; }
-__device__ __forceinline__ bool in_hi(double c, double m) { return m == INFINITY || c <= m; }
 
 __device__ __forceinline__ void apply_frame(const double *__restrict__ f64, Frame fr, double &x, double &y,
                                             double &z) {
@@ -207,8 +238,10 @@ constexpr uint32_t MAX_ATOM_PARAMS = 16;
 // One atom at one point, with the guard statistics.  f64/f32 point at the
 // atom's params.
 template <int MODE, bool STATS>
-__device__ __forceinline__ bool eval_atom(uint32_t kind, const double *f64, const float *f32, double qx, double qy,
-                                          double qz, float fx, float fy, float fz, bool live, GStats &gs) {
+__device__ __forceinline__ bool eval_atom(const DevProg &P, uint32_t kind, const double *f64, const float *f32,
+                                          double qx, double qy, double qz, float fx, float fy, float fz, bool live,
+                                          GStats &gs) {
+    if (kind == A_COLLIDER) return collider_contains(P.colliders[(int)f64[0]], f64, qx, qy, qz);
     if (MODE == 1 && has_atom32(kind)) {
         bool plain;
         const int r = atom32(kind, f32, fx, fy, fz, &plain);
@@ -265,7 +298,7 @@ __device__ __forceinline__ void eval_code_n(const Insn *code, uint32_t n, const 
             }
             const uint32_t kind = insn_kind(in.x);
             for (int k = 0; k < NP; k++)
-                st[k] = (st[k] << 1) | (uint32_t)eval_atom<MODE, STATS>(kind, f64 + in.y, f32 + in.y, qx[k], qy[k],
+                st[k] = (st[k] << 1) | (uint32_t)eval_atom<MODE, STATS>(P, kind, f64 + in.y, f32 + in.y, qx[k], qy[k],
                                                                          qz[k], fx[k], fy[k], fz[k], live[k], gs);
         } else {
             for (int k = 0; k < NP; k++) {

@@ -125,9 +125,18 @@ size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 }  // namespace
 
+// A collider's ray grid on the device, kept per context (keyed by the
+// collider's id; the shared_ptr keeps the id from being reused).
+struct ColliderCache {
+    std::shared_ptr<const ColliderData> keep;
+    DevBuf rec, start, tris;
+    DevCollider dev{};
+};
+
 struct mc3d_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    std::vector<std::unique_ptr<ColliderCache>> colliders;  // most recent last
     HostBuf staging, ctl_host;
     DevBuf blob, table, bits, counts, offs, tiles, actw, actb, binfo, bcode, vkeys, pts, out8;
     std::shared_ptr<Pool> pool = std::make_shared<Pool>();
@@ -161,9 +170,53 @@ struct Upload {
     double *axes;
 };
 
+// The device copy of a collider's ray grid (uploaded once per context).
+int collider_on_device(mc3d_ctx *ctx, const std::shared_ptr<const ColliderData> &cd, DevCollider &out,
+                       size_t keep_at_least) {
+    for (size_t i = 0; i < ctx->colliders.size(); i++) {
+        if (ctx->colliders[i]->keep->id == cd->id) {
+            out = ctx->colliders[i]->dev;
+            std::unique_ptr<ColliderCache> hit = std::move(ctx->colliders[i]);  // now the most recent
+            ctx->colliders.erase(ctx->colliders.begin() + i);
+            ctx->colliders.push_back(std::move(hit));
+            return MC3D_OK;
+        }
+    }
+    const RayGrid &G = cd->ray_grid();
+    std::unique_ptr<ColliderCache> c(new ColliderCache);
+    c->keep = cd;
+    int rc;
+    if ((rc = c->rec.ensure(G.rec.size() * sizeof(double) + 8)) ||
+        (rc = c->start.ensure(G.cell_start.size() * sizeof(uint32_t) + 8)) ||
+        (rc = c->tris.ensure(G.cell_tris.size() * sizeof(uint32_t) + 8)))
+        return rc;
+    if (!G.rec.empty()) HIP_TRY(hipMemcpy(c->rec.p, G.rec.data(), G.rec.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->start.p, G.cell_start.data(), G.cell_start.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (!G.cell_tris.empty())
+        HIP_TRY(hipMemcpy(c->tris.p, G.cell_tris.data(), G.cell_tris.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    DevCollider &d = c->dev;
+    d.rec = c->rec.as<double>();
+    d.cell_start = c->start.as<uint32_t>();
+    d.cell_tris = c->tris.as<uint32_t>();
+    memcpy(d.u, G.u, sizeof d.u);
+    memcpy(d.v, G.v, sizeof d.v);
+    d.u0 = G.u0, d.v0 = G.v0, d.inv_cell = G.inv_cell, d.nu = G.nu, d.nv = G.nv;
+    out = d;
+    ctx->colliders.push_back(std::move(c));
+    // Evict the oldest entries, never one the current program uses (those
+    // are the newest `keep_at_least`, looked up or added in this upload).
+    while (ctx->colliders.size() > std::max<size_t>(4, keep_at_least)) ctx->colliders.erase(ctx->colliders.begin());
+    return MC3D_OK;
+}
+
 // Packs the lowered program, the grid axes and a zeroed control block into
 // the pinned staging buffer and copies it to the device in one transfer.
 int upload(mc3d_ctx *ctx, const Program &P, const std::vector<double> *ax, Upload &U) {
+    std::vector<DevCollider> cols(P.colliders.size());
+    for (size_t i = 0; i < cols.size(); i++) {
+        int rc = collider_on_device(ctx, P.colliders[i], cols[i], cols.size());
+        if (rc) return rc;
+    }
     std::vector<AtomBox> boxes(P.code.size());
     for (size_t i = 0; i < P.atom_pc.size(); i++) boxes[P.atom_pc[i]] = P.atom_box[i];
     size_t off = 0;
@@ -178,6 +231,7 @@ int upload(mc3d_ctx *ctx, const Program &P, const std::vector<double> *ax, Uploa
     const size_t o_f32 = place(P.f32.size() * sizeof(float));
     const size_t o_fr = place(P.frames.size() * sizeof(Frame));
     const size_t o_box = place(boxes.size() * sizeof(AtomBox));
+    const size_t o_col = place(cols.size() * sizeof(DevCollider));
     size_t n_ax = 0;
     if (ax) n_ax = ax[0].size() + ax[1].size() + ax[2].size();
     const size_t o_ax = place(n_ax * sizeof(double));
@@ -193,6 +247,7 @@ int upload(mc3d_ctx *ctx, const Program &P, const std::vector<double> *ax, Uploa
     }
     memcpy(h + o_fr, P.frames.data(), P.frames.size() * sizeof(Frame));
     memcpy(h + o_box, boxes.data(), boxes.size() * sizeof(AtomBox));
+    if (!cols.empty()) memcpy(h + o_col, cols.data(), cols.size() * sizeof(DevCollider));
     if (ax) {
         double *d = (double *)(h + o_ax);
         for (int a = 0; a < 3; a++) d = std::copy(ax[a].begin(), ax[a].end(), d);
@@ -204,6 +259,7 @@ int upload(mc3d_ctx *ctx, const Program &P, const std::vector<double> *ax, Uploa
     U.dp.f32 = (const float *)(dv + o_f32);
     U.dp.frames = (const Frame *)(dv + o_fr);
     U.dp.boxes = (const AtomBox *)(dv + o_box);
+    U.dp.colliders = (const DevCollider *)(dv + o_col);
     U.dp.ncode = (uint32_t)P.code.size();
     U.dp.pad = 0;
     U.ctl = (Ctl *)(dv + o_ctl);
@@ -212,10 +268,6 @@ int upload(mc3d_ctx *ctx, const Program &P, const std::vector<double> *ax, Uploa
 }
 
 int check_supported(const Program &P) {
-    if (!P.colliders.empty()) {
-        set_error("ColliderSolid lowering is not available in this build");
-        return MC3D_E_UNSUPPORTED;
-    }
     if (!P.host_atoms.empty()) {
         set_error("host-evaluated solids are not available in this build");
         return MC3D_E_UNSUPPORTED;

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -68,9 +69,38 @@ struct AtomBox {
     double lo[3], hi[3];
 };
 
+// ColliderContains casts every ray along one fixed direction
+// (model3d/collisions.go:120-125), so the triangles can be binned once in
+// the plane perpendicular to it: a query tests only the triangles whose
+// (widened) projected box holds its projection, with the reference's
+// float64 Moller-Trumbore leaf test (primitives.go:207-249).
+constexpr int RAY_REC = 13;  // t0(3) cross1(3) invDet v1(3) v2(3)
+struct RayGrid {
+    std::vector<double> rec;            // RAY_REC doubles per kept triangle
+    std::vector<uint32_t> cell_start;   // nu * nv + 1
+    std::vector<uint32_t> cell_tris;    // record indices
+    double u[3], v[3];                  // projection basis, perpendicular to the ray
+    double u0, v0, inv_cell;            // grid origin and 1 / cell size
+    int nu, nv;
+};
+
 struct ColliderData {
     std::vector<double> tris;   // 9 doubles per triangle
     double mn[3], mx[3];
+    uint64_t id = 0;            // unique per collider (device caches)
+    mutable std::once_flag grid_once;
+    mutable RayGrid grid;
+    const RayGrid &ray_grid() const;  // built on first use
+};
+
+// Device view of a RayGrid.
+struct DevCollider {
+    const double *rec;
+    const uint32_t *cell_start;
+    const uint32_t *cell_tris;
+    double u[3], v[3];
+    double u0, v0, inv_cell;
+    int nu, nv;
 };
 
 struct HostAtom {
@@ -86,7 +116,7 @@ struct Program {
     std::vector<Frame> frames;
     std::vector<uint32_t> atom_pc; // pc of each atom instruction
     std::vector<AtomBox> atom_box; // per atom, world space
-    std::vector<const ColliderData *> colliders;
+    std::vector<std::shared_ptr<const ColliderData>> colliders;
     std::vector<HostAtom> host_atoms;
     int max_stack = 0;
 };

@@ -419,6 +419,108 @@ std::vector<double> spacer_axis(double lo, double hi, double delta) {
 }
 
 // ---------------------------------------------------------------------------
+// Ray grid of a mesh collider
+// ---------------------------------------------------------------------------
+namespace {
+// Fixed direction of ColliderContains, collisions.go:120-125.
+const double kRayDir[3] = {0.5224892708603626, 0.10494477243214506, 0.43558938446126527};
+
+inline V3 cross(V3 a, V3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+}  // namespace
+
+const RayGrid &ColliderData::ray_grid() const {
+    std::call_once(grid_once, [this] {
+        RayGrid &G = grid;
+        const V3 D = of(kRayDir);
+        const V3 dn = normalize(D);
+        // Any orthonormal basis perpendicular to the ray serves the binning.
+        V3 u, v;
+        ortho_basis(dn, u, v);
+        G.u[0] = u.x, G.u[1] = u.y, G.u[2] = u.z;
+        G.v[0] = v.x, G.v[1] = v.y, G.v[2] = v.z;
+        const size_t nt = tris.size() / 9;
+        struct Box2 {
+            double lo[2], hi[2];
+        };
+        std::vector<Box2> boxes;
+        boxes.reserve(nt);
+        double ulo = INFINITY, uhi = -INFINITY, vlo = INFINITY, vhi = -INFINITY;
+        for (size_t i = 0; i < nt; i++) {
+            const double *t = &tris[9 * i];
+            const V3 t0 = of(t), t1 = of(t + 3), t2 = of(t + 6);
+            // Triangle.rayCollision's point-independent part (primitives.go:207-230):
+            // the parallel test, v1, v2, cross1, det; such triangles never hit.
+            const V3 nrm = normalize(cross(sub(t1, t0), sub(t2, t0)));
+            const double dd = dot(nrm, dn);
+            if (dd < 1e-8 && dd > -1e-8) continue;
+            const V3 v1 = sub(t1, t0), v2 = sub(t2, t0);
+            const V3 c1 = cross(D, v2);
+            const double det = dot(c1, v1);
+            if (det == 0) continue;
+            const double inv_det = 1 / det;
+            const double r[RAY_REC] = {t0.x, t0.y, t0.z, c1.x, c1.y, c1.z, inv_det, v1.x, v1.y, v1.z, v2.x, v2.y, v2.z};
+            G.rec.insert(G.rec.end(), r, r + RAY_REC);
+            // Projected box, widened far beyond the float64 error of a hit
+            // (barycentrics lose ~1/|n.d| digits for grazing triangles).
+            Box2 b{{INFINITY, INFINITY}, {-INFINITY, -INFINITY}};
+            double mag = 0;
+            for (const V3 &p : {t0, t1, t2}) {
+                const double pu = dot(p, u), pv = dot(p, v);
+                b.lo[0] = std::min(b.lo[0], pu), b.hi[0] = std::max(b.hi[0], pu);
+                b.lo[1] = std::min(b.lo[1], pv), b.hi[1] = std::max(b.hi[1], pv);
+                mag = std::max(mag, std::max(fabs(p.x), std::max(fabs(p.y), fabs(p.z))));
+            }
+            const double ext = std::max(b.hi[0] - b.lo[0], b.hi[1] - b.lo[1]);
+            const double w = ext * std::max(1e-6, 1e-13 / fabs(dd)) + 1e-9 * mag + 1e-300;
+            b.lo[0] -= w, b.lo[1] -= w, b.hi[0] += w, b.hi[1] += w;
+            boxes.push_back(b);
+            ulo = std::min(ulo, b.lo[0]), uhi = std::max(uhi, b.hi[0]);
+            vlo = std::min(vlo, b.lo[1]), vhi = std::max(vhi, b.hi[1]);
+        }
+        const size_t nk = boxes.size();
+        if (nk == 0) {
+            G.u0 = G.v0 = 0, G.inv_cell = 1, G.nu = G.nv = 1;
+            G.cell_start.assign(2, 0);
+            return;
+        }
+        // About two cells per kept triangle, square cells, at most 4096 a side.
+        const double su = std::max(uhi - ulo, 1e-300), sv = std::max(vhi - vlo, 1e-300);
+        double cell = sqrt(su * sv / (2.0 * (double)nk));
+        cell = std::max(cell, std::max(su, sv) / 4095.0);
+        G.u0 = ulo, G.v0 = vlo, G.inv_cell = 1 / cell;
+        // Every widened box maps below nu (same expression as the lookup).
+        G.nu = (int)((uhi - ulo) * G.inv_cell) + 1;
+        G.nv = (int)((vhi - vlo) * G.inv_cell) + 1;
+        auto cell_range = [&](double lo, double hi, double o, int n, int &a, int &b) {
+            // the device's lookup: floor((p - o) * inv_cell), monotone in p
+            double fa = (lo - o) * G.inv_cell, fb = (hi - o) * G.inv_cell;
+            a = (int)std::max(0.0, std::min((double)(n - 1), floor(fa)));
+            b = (int)std::max(0.0, std::min((double)(n - 1), floor(fb)));
+        };
+        std::vector<uint32_t> count((size_t)G.nu * G.nv + 1, 0);
+        for (size_t i = 0; i < nk; i++) {
+            int a0, a1, b0, b1;
+            cell_range(boxes[i].lo[0], boxes[i].hi[0], G.u0, G.nu, a0, a1);
+            cell_range(boxes[i].lo[1], boxes[i].hi[1], G.v0, G.nv, b0, b1);
+            for (int cv = b0; cv <= b1; cv++)
+                for (int cu = a0; cu <= a1; cu++) count[(size_t)cv * G.nu + cu]++;
+        }
+        G.cell_start.assign(count.size(), 0);
+        for (size_t c = 0; c + 1 < count.size(); c++) G.cell_start[c + 1] = G.cell_start[c] + count[c];
+        G.cell_tris.resize(G.cell_start.back());
+        std::vector<uint32_t> fill(G.cell_start.begin(), G.cell_start.end() - 1);
+        for (size_t i = 0; i < nk; i++) {
+            int a0, a1, b0, b1;
+            cell_range(boxes[i].lo[0], boxes[i].hi[0], G.u0, G.nu, a0, a1);
+            cell_range(boxes[i].lo[1], boxes[i].hi[1], G.v0, G.nv, b0, b1);
+            for (int cv = b0; cv <= b1; cv++)
+                for (int cu = a0; cu <= a1; cu++) G.cell_tris[fill[(size_t)cv * G.nu + cu]++] = (uint32_t)i;
+        }
+    });
+    return grid;
+}
+
+// ---------------------------------------------------------------------------
 // Lowering
 // ---------------------------------------------------------------------------
 namespace {
@@ -677,7 +779,7 @@ struct Lowerer {
             return true;
         }
         case K_COLLIDER: {
-            P.colliders.push_back(s.collider.get());
+            P.colliders.push_back(s.collider);
             const ColliderData &c = *s.collider;
             atom(A_COLLIDER, {(double)(P.colliders.size() - 1), c.mn[0], c.mn[1], c.mn[2], c.mx[0], c.mx[1], c.mx[2]},
                  local_box(s));
@@ -851,7 +953,11 @@ mc3d_solid *mc3d_force_bounds(mc3d_solid *child, const double min[3], const doub
 mc3d_solid *mc3d_collider_solid(const double *tris, size_t n) {
     if (!tris || n == 0) return set_error("mc3d_collider_solid: empty mesh"), nullptr;
     if (n >= (1u << 30)) return set_error("mc3d_collider_solid: mesh too large"), nullptr;
+    for (size_t i = 0; i < 9 * n; i++)
+        if (!isfinite(tris[i])) return set_error("mc3d_collider_solid: non-finite vertex"), nullptr;
+    static std::atomic<uint64_t> next_id{1};
     auto cd = std::make_shared<ColliderData>();
+    cd->id = next_id++;
     cd->tris.assign(tris, tris + 9 * n);
     // Union of triangle bounds (JoinedCollider min/max, collisions.go:225-234).
     for (size_t i = 0; i < n; i++) {

@@ -34,6 +34,43 @@ class Mesh:
         """(T, 3, 3) float64: triangle i = vertices[faces[i]]."""
         return self.vertices[self.faces.astype(np.int64)]
 
+    @classmethod
+    def from_triangles(cls, tris) -> "Mesh":
+        """Indexed mesh of a triangle list (vertices shared by exact bits)."""
+        t = np.ascontiguousarray(np.asarray(tris, dtype=np.float64).reshape(-1, 3))
+        keys = t.view(np.uint64)
+        _, first, inv = np.unique(keys, axis=0, return_index=True, return_inverse=True)
+        return cls(t[first], inv.reshape(-1, 3).astype(np.uint32))
+
+    def AddMesh(self, other: "Mesh") -> "Mesh":
+        """Mesh.AddMesh (mesh.go:399-402): the union of both triangle lists."""
+        m = Mesh.from_triangles(np.concatenate([self.triangles(), other.triangles()]))
+        self.vertices, self.faces = m.vertices, m.faces
+        return self
+
+
+def NewMeshRect(min, max) -> Mesh:
+    """model3d.NewMeshRect (mesh.go:132-162): 6 quads split by AddQuad
+    (mesh.go:389-397) into triangles {p1, p2, p4}, {p2, p3, p4}."""
+    mn, mx = [float(v) for v in min], [float(v) for v in max]
+
+    def point(x, y, z):
+        return (mx[0] if x else mn[0], mx[1] if y else mn[1], mx[2] if z else mn[2])
+
+    quads = [
+        (point(0, 0, 0), point(1, 0, 0), point(1, 0, 1), point(0, 0, 1)),
+        (point(1, 1, 1), point(1, 1, 0), point(0, 1, 0), point(0, 1, 1)),
+        (point(0, 0, 0), point(0, 0, 1), point(0, 1, 1), point(0, 1, 0)),
+        (point(1, 1, 1), point(1, 0, 1), point(1, 0, 0), point(1, 1, 0)),
+        (point(0, 0, 0), point(0, 1, 0), point(1, 1, 0), point(1, 0, 0)),
+        (point(1, 1, 1), point(0, 1, 1), point(0, 0, 1), point(1, 0, 1)),
+    ]
+    tris = []
+    for p1, p2, p3, p4 in quads:
+        tris.append((p1, p2, p4))
+        tris.append((p2, p3, p4))
+    return Mesh.from_triangles(np.array(tris))
+
     def NumTriangles(self) -> int:
         return int(len(self.faces))
 

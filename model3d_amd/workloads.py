@@ -109,8 +109,25 @@ SPHERE_CONFIG = dict(name="sphere_r1_d0.02", delta=0.02, iters=8)
 CSG64_CONFIG = dict(name="csg64_seed2_d0.005", seed=2, n_prims=64, delta=0.005, iters=8)
 
 
+COLLIDER_CONFIG = dict(name="collider_csg64_d0.02_256cubed", seed=2, n_prims=64, mesh_delta=0.02, iters=8)
+
+
 def sphere_solid():
     return Sphere((0.0, 0.0, 0.0), 1.0)
+
+
+def collider_solid(seed: int = 2, n_prims: int = 64, mesh_delta: float = 0.02, samples: int = 256):
+    """The CSG workload meshed at ``mesh_delta`` (on the GPU), wrapped as
+    NewColliderSolid(MeshToCollider(mesh)); returns (solid, delta) with delta
+    giving ``samples`` grid samples along the longest axis."""
+    from .mc import MarchingCubesSearch
+    from .solids import MeshCollider, NewColliderSolid
+
+    mesh = MarchingCubesSearch(csg64_solid(seed, n_prims), mesh_delta, 8)
+    solid = NewColliderSolid(MeshCollider(mesh.triangles()))
+    mn, mx = solid.Min(), solid.Max()
+    extent = max(mx[i] - mn[i] for i in range(3))
+    return solid, extent / (samples - 3), len(mesh.faces)
 
 
 def csg64_solid(seed: int = 2, n_prims: int = 64):

@@ -511,7 +511,8 @@ static int tri_ray_hit(const double *t, c3 origin, c3 dir) {
     double bary2 = inv_det * c3_dot(dir, cross2);
     if (bary2 < 0 || bary1 + bary2 > 1) return 0;
     double scale = inv_det * c3_dot(v2, cross2);
-    return scale >= 0;
+    /* RayCollisions: `if info == nil || scale < 0 { return 0 }` */
+    return !(scale < 0);
 }
 
 /* Fixed ray direction of ColliderContains, collisions.go:120-125. */

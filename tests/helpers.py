@@ -77,6 +77,15 @@ def solid_catalogue():
     mirrored = m3.TransformSolid(m3.Mirror((1.0, 1.0, 0.0)), cone)
     forced = m3.ForceSolidBounds(sph, (-0.7, -1.0, -0.7), (0.9, 0.6, 0.9))
     clipped = m3.ClipSolid(m3.JoinedSolid([sph, cyl]), (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+    # ColliderSolids: the two boxes of TestMarchingCubesFilter (mc_test.go:55-58),
+    # faces on grid planes (grazing rays), and a meshed sphere.
+    boxes = m3.NewMeshRect((-1, -1, -1), (0, 0, 0)).AddMesh(m3.NewMeshRect((0.1, 0, 0), (1, 1, 1)))
+    col_boxes = m3.NewColliderSolid(m3.MeshToCollider(boxes))
+    from oracle.binding import oracle_search
+
+    sph_tris, _ = oracle_search(m3.Sphere((0.05, -0.02, 0.1), 0.7), 0.1, 4, threads=8)
+    col_sph = m3.NewColliderSolid(m3.MeshCollider(sph_tris))
+    col_rot = m3.RotateSolid(col_sph, (0.0, 0.6, 0.8), 0.5)
     return {
         "sphere": sph,
         "rect": box,
@@ -93,4 +102,7 @@ def solid_catalogue():
         "mirrored": mirrored,
         "force_bounds": forced,
         "clipped": clipped,
+        "collider_boxes": col_boxes,
+        "collider_sphere": col_sph,
+        "collider_rotated": col_rot,
     }

@@ -158,6 +158,19 @@ def test_output_capacity_growth_and_reuse():
         L.mc3d_ctx_destroy(ctx)
 
 
+def test_collider_of_csg_mesh_bit_exact():
+    """A ColliderSolid over a 12k-triangle mesh of the CSG workload, meshed again."""
+    s = workloads.csg64_solid(2)
+    tris = m3.MarchingCubesSearch(s, 0.04, 8).triangles()
+    col = m3.NewColliderSolid(m3.MeshCollider(tris))
+    print(f"collider over {len(tris)} triangles at d=0.05", flush=True)
+    want, _ = oracle_search(col, 0.05, 8)
+    got = m3.MarchingCubesSearch(col, 0.05, 8)
+    assert same_triangles(got.triangles(), want)
+    # the meshed collider approximates the solid: volumes agree to the grid scale
+    assert abs(volume(got.triangles()) - volume(tris)) < 0.05 * abs(volume(tris))
+
+
 def test_mesh_topology_and_volume():
     s = m3.Sphere((0.1, 0.3, -0.2), 1.0)
     got = m3.MarchingCubesSearch(s, 0.05, 8)

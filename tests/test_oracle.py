@@ -329,6 +329,76 @@ class _Unbounded(m3.Solid):
         return np.ones(len(pts), dtype=bool)
 
 
+# ---------------------------------------------------------------------------
+# ColliderSolid (collisions.go:119-134, primitives.go:207-249)
+# ---------------------------------------------------------------------------
+RAY_DIR = np.array([0.5224892708603626, 0.10494477243214506, 0.43558938446126527])
+
+
+def two_boxes():
+    """The two-box mesh of TestMarchingCubesFilter (mc_test.go:55-58)."""
+    return m3.NewMeshRect((-1, -1, -1), (0, 0, 0)).AddMesh(m3.NewMeshRect((0.1, 0, 0), (1, 1, 1)))
+
+
+def numpy_ray_parity(tris: np.ndarray, pts: np.ndarray) -> np.ndarray:
+    """Independent numpy restatement of ColliderContains: per point, count
+    triangles hit by Triangle.rayCollision with scale >= 0 along RAY_DIR."""
+    t = tris.reshape(-1, 3, 3)
+    t0, t1, t2 = t[:, 0], t[:, 1], t[:, 2]
+    n = np.cross(t1 - t0, t2 - t0)
+    n = n / np.linalg.norm(n, axis=1, keepdims=True)
+    dn = RAY_DIR / np.linalg.norm(RAY_DIR)
+    keep = ~((n @ dn < 1e-8) & (n @ dn > -1e-8))
+    v1, v2 = t1 - t0, t2 - t0
+    c1 = np.cross(RAY_DIR, v2)
+    det = np.einsum("ij,ij->i", c1, v1)
+    keep &= det != 0
+    inv = 1 / np.where(det == 0, 1, det)
+    out = np.zeros(len(pts), dtype=bool)
+    for i, p in enumerate(pts):
+        o = p - t0
+        b1 = inv * np.einsum("ij,ij->i", o, c1)
+        c2 = np.cross(o, v1)
+        b2 = inv * (c2 @ RAY_DIR)
+        sc = inv * np.einsum("ij,ij->i", v2, c2)
+        hit = keep & ~((b1 < 0) | (b1 > 1)) & ~((b2 < 0) | (b1 + b2 > 1)) & ~(sc < 0)
+        out[i] = hit.sum() % 2 == 1
+    return out
+
+
+def test_collider_two_boxes_closed_form():
+    col = m3.NewColliderSolid(m3.MeshToCollider(two_boxes()))
+    rng = np.random.default_rng(2)
+    p = rng.uniform(-1.2, 1.2, size=(4000, 3))
+    in1 = np.all((p > -1) & (p < 0), axis=1)
+    in2 = np.all((p > [0.1, 0, 0]) & (p < 1), axis=1)
+    d = np.minimum(np.abs(p - np.round(p * 10) / 10).min(axis=1), 1)  # distance to the faces' planes
+    far = d > 1e-6
+    got = oracle_contains(col, p)
+    assert np.array_equal(got[far], (in1 | in2)[far])
+    mn, mx = oracle_bounds(col)
+    assert mn == (-1.0, -1.0, -1.0) and mx == (1.0, 1.0, 1.0)
+
+
+def test_collider_matches_numpy_restatement():
+    sph, _ = oracle_search(m3.Sphere((0.05, -0.02, 0.1), 0.8), 0.15, 4, threads=4)
+    col = m3.NewColliderSolid(m3.MeshCollider(sph))
+    rng = np.random.default_rng(4)
+    p = rng.uniform(-1, 1, size=(600, 3))
+    assert np.array_equal(oracle_contains(col, p), numpy_ray_parity(sph, p))
+    bx = two_boxes().triangles().reshape(-1, 9)
+    p = rng.uniform(-1.1, 1.1, size=(600, 3))
+    assert np.array_equal(oracle_contains(m3.NewColliderSolid(m3.MeshCollider(bx)), p), numpy_ray_parity(bx, p))
+
+
+def test_new_mesh_rect_is_closed_and_outward():
+    t = m3.NewMeshRect((0, 0, 0), (1, 2, 3)).triangles()
+    assert len(t) == 12
+    directed_once, paired = edge_topology(t)
+    assert directed_once and paired
+    assert abs(volume(t) - 6.0) < 1e-12
+
+
 def test_csg64_oracle_smoke():
     s = workloads.csg64_solid(2)
     tris, info = oracle_search(s, 0.04, 8, threads=8)
