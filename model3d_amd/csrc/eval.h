@@ -18,6 +18,9 @@ struct DevProg {
     const Frame *__restrict__ frames;
     const AtomBox *__restrict__ boxes;  // world-space box per pc (atoms only)
     const DevCollider *__restrict__ colliders;
+    // Answers of host-evaluated atoms: host_vals[atom * host_stride + point id].
+    const uint8_t *__restrict__ host_vals;
+    uint64_t host_stride;
     uint32_t ncode;
     uint32_t pad;
 };
@@ -237,11 +240,15 @@ constexpr uint32_t MAX_ATOM_PARAMS = 16;
 
 // One atom at one point, with the guard statistics.  f64/f32 point at the
 // atom's params.
-template <int MODE, bool STATS>
+// HOST: the program may hold host-evaluated atoms, whose answers for point
+// `pid` were filled in by the host beforehand (kernels without point ids are
+// never launched on such programs).
+template <int MODE, bool STATS, bool HOST>
 __device__ __forceinline__ bool eval_atom(const DevProg &P, uint32_t kind, const double *f64, const float *f32,
                                           double qx, double qy, double qz, float fx, float fy, float fz, bool live,
-                                          GStats &gs) {
+                                          uint64_t pid, GStats &gs) {
     if (kind == A_COLLIDER) return collider_contains(P.colliders[(int)f64[0]], f64, qx, qy, qz);
+    if (kind == A_HOST) return HOST && P.host_vals[(uint64_t)f64[0] * P.host_stride + pid] != 0;
     if (MODE == 1 && has_atom32(kind)) {
         bool plain;
         const int r = atom32(kind, f32, fx, fy, fz, &plain);
@@ -267,10 +274,10 @@ __device__ __forceinline__ bool eval_atom(const DevProg &P, uint32_t kind, const
 // global memory or LDS; instructions are made wave-uniform with readfirstlane
 // so the walk branches on scalars.  MODE 0: every atom in float64.  MODE 1:
 // float32 with band, float64 in the band.  `live` only gates the statistics.
-template <int MODE, bool STATS, int NP, bool UNI>
+template <int MODE, bool STATS, int NP, bool UNI, bool HOST = false>
 __device__ __forceinline__ void eval_code_n(const Insn *code, uint32_t n, const DevProg &P, const double *f64,
                                             const float *f32, const double *px, const double *py, const double *pz,
-                                            const bool *live, bool *res, GStats &gs) {
+                                            const bool *live, bool *res, GStats &gs, const uint64_t *pid = nullptr) {
     uint32_t st[NP];
     double qx[NP], qy[NP], qz[NP];
     float fx[NP], fy[NP], fz[NP];
@@ -298,8 +305,9 @@ __device__ __forceinline__ void eval_code_n(const Insn *code, uint32_t n, const 
             }
             const uint32_t kind = insn_kind(in.x);
             for (int k = 0; k < NP; k++)
-                st[k] = (st[k] << 1) | (uint32_t)eval_atom<MODE, STATS>(P, kind, f64 + in.y, f32 + in.y, qx[k], qy[k],
-                                                                         qz[k], fx[k], fy[k], fz[k], live[k], gs);
+                st[k] = (st[k] << 1) |
+                        (uint32_t)eval_atom<MODE, STATS, HOST>(P, kind, f64 + in.y, f32 + in.y, qx[k], qy[k], qz[k],
+                                                               fx[k], fy[k], fz[k], live[k], HOST ? pid[k] : 0, gs);
         } else {
             for (int k = 0; k < NP; k++) {
                 uint32_t s = st[k];
@@ -330,12 +338,12 @@ __device__ __forceinline__ void eval_code_n(const Insn *code, uint32_t n, const 
     for (int k = 0; k < NP; k++) res[k] = st[k] & 1u;
 }
 
-template <int MODE, bool STATS, bool UNI>
+template <int MODE, bool STATS, bool UNI, bool HOST = false>
 __device__ __forceinline__ bool eval_code(const Insn *__restrict__ code, uint32_t n, const DevProg &P, double px,
-                                          double py, double pz, GStats &gs) {
+                                          double py, double pz, GStats &gs, uint64_t pid = 0) {
     const bool live = true;
     bool r;
-    eval_code_n<MODE, STATS, 1, UNI>(code, n, P, P.f64, P.f32, &px, &py, &pz, &live, &r, gs);
+    eval_code_n<MODE, STATS, 1, UNI, HOST>(code, n, P, P.f64, P.f32, &px, &py, &pz, &live, &r, gs, &pid);
     return r;
 }
 

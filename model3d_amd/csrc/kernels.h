@@ -269,7 +269,9 @@ __global__ __launch_bounds__(256) void k_corners(DevProg P, Grid g, uint64_t *__
         word_coords(g, w, wx, y, z);
         const int x = wx * 64 + lane;
         bool inside = false;
-        if (x < g.nx) inside = eval_code<MODE, STATS, true>(P.code, P.ncode, P, g.xs[x], g.ys[y], g.zs[z], gs);
+        // Point id of a corner (host-evaluated atoms): x + nx * (y + ny * z).
+        const uint64_t pid = (uint64_t)x + (uint64_t)g.nx * ((uint64_t)y + (uint64_t)g.ny * (uint64_t)z);
+        if (x < g.nx) inside = eval_code<MODE, STATS, true, true>(P.code, P.ncode, P, g.xs[x], g.ys[y], g.zs[z], gs, pid);
         const uint64_t m = __ballot(inside);
         if (lane == 0) bits[w] = m;
         // panic("solid is true outside of bounds"), mc.go:687-689
@@ -734,9 +736,78 @@ __global__ __launch_bounds__(256) void k_contains(DevProg P, const double *__res
                                                   uint64_t n) {
     GStats gs;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        out[i] = eval_code<MODE, false, true>(P.code, P.ncode, P, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], gs)
+        out[i] = eval_code<MODE, false, true, true>(P.code, P.ncode, P, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2],
+                                                     gs, i)
                      ? 1
                      : 0;
+}
+
+// ---------------------------------------------------------------------------
+// host-evaluated atoms
+// ---------------------------------------------------------------------------
+// The points a host-evaluated atom is asked about, in the atom's own frame
+// (the inverse transforms above it, applied exactly as for native atoms):
+// points [start, start + n) of a grid's corners (pts == nullptr; corner id
+// x + nx * (y + ny * z)) or of a point list.
+__global__ __launch_bounds__(256) void k_host_points(DevProg P, Grid g, const double *__restrict__ pts,
+                                                     uint32_t frame, uint64_t start, uint64_t n,
+                                                     double *__restrict__ out) {
+    const Frame fr = P.frames[frame];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t id = start + i;
+        double x, y, z;
+        if (pts) {
+            x = pts[3 * id], y = pts[3 * id + 1], z = pts[3 * id + 2];
+        } else {
+            const uint64_t row = id / (uint64_t)g.nx;
+            x = g.xs[id - row * (uint64_t)g.nx];
+            y = g.ys[row % (uint64_t)g.ny];
+            z = g.zs[row / (uint64_t)g.ny];
+        }
+        apply_frame(P.f64, fr, x, y, z);
+        out[3 * i] = x, out[3 * i + 1] = y, out[3 * i + 2] = z;
+    }
+}
+
+// mcSearchPoint (mc.go:223-258) one probe at a time, for programs with
+// host-evaluated atoms: the host answers those atoms for every vertex's
+// probe between rounds.  state[2v..2v+1] = (false end, true end) on the
+// edge's axis; probe[3v..] = the next midpoint, which is also written to
+// verts (after the last round it is the result).  phase 0 sets up the state
+// from the corner bits; phase 1 decides the current probes and moves on.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bisect_host(DevProg P, Grid g, const uint64_t *__restrict__ bits,
+                                                     const uint64_t *__restrict__ vkeys, double *__restrict__ state,
+                                                     double *__restrict__ probe, double *__restrict__ verts,
+                                                     uint64_t nv, int phase) {
+    GStats gs;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t key = vkeys[v];
+        const int axis = (int)(key & 3);
+        const int x = (int)((key >> 2) & 0xfffff), y = (int)((key >> 22) & 0xfffff), z = (int)((key >> 42) & 0xfffff);
+        double p[3] = {g.xs[x], g.ys[y], g.zs[z]};
+        double fp, tp;
+        if (phase == 0) {
+            int hx = x, hy = y, hz = z;
+            double lo, hi;
+            if (axis == 0) lo = g.xs[x], hi = g.xs[x + 1], hx++;
+            else if (axis == 1) lo = g.ys[y], hi = g.ys[y + 1], hy++;
+            else lo = g.zs[z], hi = g.zs[z + 1], hz++;
+            fp = lo, tp = hi;
+            if (!corner_bit(g, bits, hx, hy, hz)) fp = hi, tp = lo;
+        } else {
+            fp = state[2 * v], tp = state[2 * v + 1];
+            const double mid = (fp + tp) / 2;
+            p[axis] = mid;
+            if (eval_code<MODE, false, false, true>(P.code, P.ncode, P, p[0], p[1], p[2], gs, v)) tp = mid;
+            else fp = mid;
+        }
+        state[2 * v] = fp, state[2 * v + 1] = tp;
+        p[axis] = (fp + tp) / 2;
+        probe[3 * v] = p[0], probe[3 * v + 1] = p[1], probe[3 * v + 2] = p[2];
+        verts[3 * v] = p[0], verts[3 * v + 1] = p[1], verts[3 * v + 2] = p[2];
+    }
 }
 
 }  // namespace mc3d

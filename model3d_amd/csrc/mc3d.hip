@@ -139,6 +139,8 @@ struct mc3d_ctx {
     std::vector<std::unique_ptr<ColliderCache>> colliders;  // most recent last
     HostBuf staging, ctl_host;
     DevBuf blob, table, bits, counts, offs, tiles, actw, actb, binfo, bcode, vkeys, pts, out8;
+    DevBuf hvals, hpts, hstate, hprobe;  // host-evaluated atoms: answers, points, bisection state
+    HostBuf hstage;                      // pinned points / answers of one callback batch
     std::shared_ptr<Pool> pool = std::make_shared<Pool>();
     uint64_t v_hint = 1u << 18, t_hint = 1u << 19;  // output capacities to start from
     hipEvent_t ev[8] = {};
@@ -260,6 +262,8 @@ int upload(mc3d_ctx *ctx, const Program &P, const std::vector<double> *ax, Uploa
     U.dp.frames = (const Frame *)(dv + o_fr);
     U.dp.boxes = (const AtomBox *)(dv + o_box);
     U.dp.colliders = (const DevCollider *)(dv + o_col);
+    U.dp.host_vals = nullptr;
+    U.dp.host_stride = 0;
     U.dp.ncode = (uint32_t)P.code.size();
     U.dp.pad = 0;
     U.ctl = (Ctl *)(dv + o_ctl);
@@ -267,11 +271,52 @@ int upload(mc3d_ctx *ctx, const Program &P, const std::vector<double> *ax, Uploa
     return MC3D_OK;
 }
 
-int check_supported(const Program &P) {
-    if (!P.host_atoms.empty()) {
-        set_error("host-evaluated solids are not available in this build");
-        return MC3D_E_UNSUPPORTED;
+// Frame of each host-evaluated atom, by host atom index.
+std::vector<uint32_t> host_frames(const Program &P) {
+    std::vector<uint32_t> fr(P.host_atoms.size(), 0);
+    for (uint32_t pc : P.atom_pc) {
+        const Insn in = P.code[pc];
+        if (insn_kind(in.x) == A_HOST) fr[(size_t)P.f64[in.y]] = insn_frame(in.x);
     }
+    return fr;
+}
+
+constexpr uint64_t HOST_BATCH = 1u << 20;  // points per callback
+
+// Answers every host-evaluated atom at n points -- grid corners (pts ==
+// nullptr) or a device point list -- through its callback, in batches, into
+// hvals[h * n + i], and points the program at them.  The GPU computes each
+// atom's local points; the host only runs the callbacks.
+int host_round(mc3d_ctx *ctx, const Program &P, const std::vector<uint32_t> &frames, Upload &U, const Grid &g,
+               const double *pts, uint64_t n, double &ms_cb) {
+    const size_t nh = P.host_atoms.size();
+    int rc;
+    if ((rc = ctx->hvals.ensure(nh * n + 1)) || (rc = ctx->hpts.ensure(HOST_BATCH * 24)) ||
+        (rc = ctx->hstage.ensure(HOST_BATCH * 25)))
+        return rc;
+    double *hp = (double *)ctx->hstage.p;
+    uint8_t *ho = (uint8_t *)(hp + 3 * HOST_BATCH);
+    for (size_t h = 0; h < nh; h++) {
+        for (uint64_t s = 0; s < n; s += HOST_BATCH) {
+            const uint64_t c = std::min(HOST_BATCH, n - s);
+            k_host_points<<<grid_for(c, 256), 256, 0, ctx->stream>>>(U.dp, g, pts, frames[h], s, c,
+                                                                    ctx->hpts.as<double>());
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(hp, ctx->hpts.p, c * 24, hipMemcpyDeviceToHost, ctx->stream));
+            // Also orders the previous batch's upload of `ho` before it is rewritten.
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            const auto t0 = std::chrono::steady_clock::now();
+            const int r = P.host_atoms[h].fn(P.host_atoms[h].user, hp, (size_t)c, ho);
+            ms_cb += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (r) {
+                set_error("host solid callback failed with status " + std::to_string(r));
+                return MC3D_E_CALLBACK;
+            }
+            HIP_TRY(hipMemcpyAsync(ctx->hvals.as<uint8_t>() + h * n + s, ho, c, hipMemcpyHostToDevice, ctx->stream));
+        }
+    }
+    U.dp.host_vals = ctx->hvals.as<uint8_t>();
+    U.dp.host_stride = n;
     return MC3D_OK;
 }
 
@@ -347,6 +392,57 @@ int run_output(mc3d_ctx *ctx, const Upload &U, const Bricks &B, const Grid &g, m
               : launch_bisect<1, false>(ctx, U.dp, B, g, U.ctl, verts, vcap, iters);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream));
+    return MC3D_OK;
+}
+
+int take_outputs(mc3d_ctx *ctx, mc3d_mesh *m, uint64_t vcap, uint64_t tcap);
+
+// The output stages of a program with host-evaluated atoms: the mesh size is
+// read first (one more synchronisation), then emit, then the bisection in
+// rounds with the host answering its atoms between them.  Leaves the final
+// control block in ctx->ctl_host.
+int run_output_host(mc3d_ctx *ctx, const Program &P, const std::vector<uint32_t> &frames, Upload &U, const Grid &g,
+                    mc3d_mesh *m, int mode, int iters, uint64_t nwords, double &ms_cb) {
+    Ctl *hc = (Ctl *)ctx->ctl_host.p;
+    HIP_TRY(hipMemcpyAsync(hc, U.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (hc->inside) {
+        set_error("solid is true outside of bounds");
+        return MC3D_E_INSIDE_BOUNDARY;
+    }
+    const uint64_t nv = hc->totals[0], nt = hc->totals[1];
+    if (nv >= (1ull << 32) || nt >= (1ull << 32)) {
+        set_error("mesh too large for 32-bit vertex indices");
+        return MC3D_E_UNSUPPORTED;
+    }
+    const uint64_t vcap = std::max<uint64_t>(nv, 1), tcap = std::max<uint64_t>(nt, 1);
+    int rc;
+    if ((rc = take_outputs(ctx, m, vcap, tcap)) || (rc = ctx->hstate.ensure(vcap * 16)) ||
+        (rc = ctx->hprobe.ensure(vcap * 24)))
+        return rc;
+    HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
+    k_emit<<<grid_for(nwords, 4, 2048), 256, 0, ctx->stream>>>(
+        g, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(), ctx->offs.as<uint4>(), ctx->actw.as<uint32_t>(),
+        U.ctl, ctx->vkeys.as<uint64_t>(), m->dtris->as<uint32_t>(), vcap, tcap);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
+    double *verts = m->dverts->as<double>(), *state = ctx->hstate.as<double>(), *probe = ctx->hprobe.as<double>();
+    const unsigned grid = grid_for(nv, 256, 1u << 14);
+    for (int round = 0; round <= iters; round++) {
+        if (round > 0 && (rc = host_round(ctx, P, frames, U, g, probe, nv, ms_cb))) return rc;
+        if (mode == 1)
+            k_bisect_host<1><<<grid, 256, 0, ctx->stream>>>(U.dp, g, ctx->bits.as<uint64_t>(),
+                                                            ctx->vkeys.as<uint64_t>(), state, probe, verts, nv,
+                                                            round > 0);
+        else
+            k_bisect_host<0><<<grid, 256, 0, ctx->stream>>>(U.dp, g, ctx->bits.as<uint64_t>(),
+                                                            ctx->vkeys.as<uint64_t>(), state, probe, verts, nv,
+                                                            round > 0);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream));
+    HIP_TRY(hipMemcpyAsync(hc, U.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return MC3D_OK;
 }
 
@@ -463,8 +559,12 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
         set_error(msg);
         return MC3D_E_UNSUPPORTED;
     }
-    int rc = check_supported(P);
-    if (rc) return rc;
+    int rc;
+    // A host-evaluated solid may be true outside its bounds, so nothing is
+    // folded by bounds: no culling.
+    const bool host = !P.host_atoms.empty();
+    const std::vector<uint32_t> frames = host_frames(P);
+    if (host) o.cull = 0;
 
     Grid g;
     g.nx = (int)ax[0].size(), g.ny = (int)ax[1].size(), g.nz = (int)ax[2].size();
@@ -509,6 +609,8 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
     m->device = ctx->device;
     m->pool = ctx->pool;
     mc3d_stats &st = m->stats;
+    if (host && (rc = host_round(ctx, P, frames, U, g, nullptr, (uint64_t)g.nx * g.ny * g.nz, st.ms_host_callbacks)))
+        return rc;
 
     // ---- plan ----
     HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
@@ -550,30 +652,35 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
     HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
 
     // ---- emit + bisect, sized from earlier calls ----
-    uint64_t vcap = ctx->v_hint, tcap = ctx->t_hint;
-    if ((rc = take_outputs(ctx, m, vcap, tcap))) return rc;
-    if ((rc = run_output(ctx, U, B, g, m, mode, stats, iters, vcap, tcap, nwords))) return rc;
     Ctl *hc = (Ctl *)ctx->ctl_host.p;
-    HIP_TRY(hipMemcpyAsync(hc, U.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (hc->inside) {
-        set_error("solid is true outside of bounds");
-        return MC3D_E_INSIDE_BOUNDARY;
-    }
-    const uint64_t nv = hc->totals[0], nt = hc->totals[1];
-    if (nv >= (1ull << 32) || nt >= (1ull << 32)) {
-        set_error("mesh too large for 32-bit vertex indices");
-        return MC3D_E_UNSUPPORTED;
-    }
-    if (nv > vcap || nt > tcap) {  // grow and run the output stages again
-        ctx->v_hint = std::max(ctx->v_hint, nv + nv / 4 + 1024);
-        ctx->t_hint = std::max(ctx->t_hint, nt + nt / 4 + 1024);
-        vcap = ctx->v_hint, tcap = ctx->t_hint;
+    if (host) {
+        if ((rc = run_output_host(ctx, P, frames, U, g, m, mode, iters, nwords, st.ms_host_callbacks))) return rc;
+    } else {
+        uint64_t vcap = ctx->v_hint, tcap = ctx->t_hint;
         if ((rc = take_outputs(ctx, m, vcap, tcap))) return rc;
         if ((rc = run_output(ctx, U, B, g, m, mode, stats, iters, vcap, tcap, nwords))) return rc;
         HIP_TRY(hipMemcpyAsync(hc, U.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (hc->inside) {
+            set_error("solid is true outside of bounds");
+            return MC3D_E_INSIDE_BOUNDARY;
+        }
+        const uint64_t nv = hc->totals[0], nt = hc->totals[1];
+        if (nv >= (1ull << 32) || nt >= (1ull << 32)) {
+            set_error("mesh too large for 32-bit vertex indices");
+            return MC3D_E_UNSUPPORTED;
+        }
+        if (nv > vcap || nt > tcap) {  // grow and run the output stages again
+            ctx->v_hint = std::max(ctx->v_hint, nv + nv / 4 + 1024);
+            ctx->t_hint = std::max(ctx->t_hint, nt + nt / 4 + 1024);
+            vcap = ctx->v_hint, tcap = ctx->t_hint;
+            if ((rc = take_outputs(ctx, m, vcap, tcap))) return rc;
+            if ((rc = run_output(ctx, U, B, g, m, mode, stats, iters, vcap, tcap, nwords))) return rc;
+            HIP_TRY(hipMemcpyAsync(hc, U.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+        }
     }
+    const uint64_t nv = hc->totals[0], nt = hc->totals[1];
     m->nv = nv;
     m->nt = nt;
 
@@ -659,12 +766,16 @@ int mc3d_contains(mc3d_ctx *ctx, const mc3d_solid *s, const double *pts, size_t 
         set_error(msg);
         return MC3D_E_UNSUPPORTED;
     }
-    int rc = check_supported(P);
-    if (rc) return rc;
+    int rc;
     Upload U;
     if ((rc = upload(ctx, P, nullptr, U))) return rc;
     if ((rc = ctx->pts.ensure(n * 24)) || (rc = ctx->out8.ensure(n))) return rc;
     HIP_TRY(hipMemcpyAsync(ctx->pts.p, pts, n * 24, hipMemcpyHostToDevice, ctx->stream));
+    if (!P.host_atoms.empty()) {
+        double ms_cb = 0;
+        const Grid none{nullptr, nullptr, nullptr, 1, 1, 1, 1};
+        if ((rc = host_round(ctx, P, host_frames(P), U, none, ctx->pts.as<double>(), n, ms_cb))) return rc;
+    }
     if (precision == MC3D_PREC_FP32_GUARD)
         k_contains<1><<<grid_for(n, 256), 256, 0, ctx->stream>>>(U.dp, ctx->pts.as<double>(), ctx->out8.as<uint8_t>(), n);
     else

@@ -48,6 +48,20 @@ class Mesh:
         self.vertices, self.faces = m.vertices, m.faces
         return self
 
+    def NumTriangles(self) -> int:
+        return int(len(self.faces))
+
+    def Volume(self) -> float:
+        """Mesh.Volume (measurements.go:18-29): |sum det(t)/6|."""
+        t = self.triangles()
+        a, b, c = t[:, 0], t[:, 1], t[:, 2]
+        det = (
+            a[:, 0] * (b[:, 1] * c[:, 2] - b[:, 2] * c[:, 1])
+            - a[:, 1] * (b[:, 0] * c[:, 2] - b[:, 2] * c[:, 0])
+            + a[:, 2] * (b[:, 0] * c[:, 1] - b[:, 1] * c[:, 0])
+        )
+        return float(abs(np.sum(det / 6.0)))
+
 
 def NewMeshRect(min, max) -> Mesh:
     """model3d.NewMeshRect (mesh.go:132-162): 6 quads split by AddQuad
@@ -70,20 +84,6 @@ def NewMeshRect(min, max) -> Mesh:
         tris.append((p1, p2, p4))
         tris.append((p2, p3, p4))
     return Mesh.from_triangles(np.array(tris))
-
-    def NumTriangles(self) -> int:
-        return int(len(self.faces))
-
-    def Volume(self) -> float:
-        """Mesh.Volume (measurements.go:18-29): |sum det(t)/6|."""
-        t = self.triangles()
-        a, b, c = t[:, 0], t[:, 1], t[:, 2]
-        det = (
-            a[:, 0] * (b[:, 1] * c[:, 2] - b[:, 2] * c[:, 1])
-            - a[:, 1] * (b[:, 0] * c[:, 2] - b[:, 2] * c[:, 0])
-            + a[:, 2] * (b[:, 0] * c[:, 1] - b[:, 1] * c[:, 0])
-        )
-        return float(abs(np.sum(det / 6.0)))
 
 
 def _options(precision, cull, device_only=False, count_mismatch=False, brick=0, rows_per_walk=0):

@@ -54,6 +54,32 @@ def volume(tris: np.ndarray) -> float:
     return float(np.sum(det / 6.0))
 
 
+class Superellipsoid(m3.Solid):
+    """A user-defined solid the library cannot lower (evaluated through the
+    host callback): x^4 + y^4 + z^4 <= r^4 about a centre.  Only products,
+    sums and a comparison, so numpy gives the same bits per point whatever
+    the batch size."""
+
+    def __init__(self, center, r):
+        self.c = np.asarray(center, dtype=np.float64)
+        self.r = float(r)
+        self.r4 = self.r * self.r * self.r * self.r
+
+    def Min(self):
+        return tuple(float(v) for v in self.c - self.r)
+
+    def Max(self):
+        return tuple(float(v) for v in self.c + self.r)
+
+    def ContainsBatch(self, pts):
+        q = np.asarray(pts, dtype=np.float64) - self.c
+        q = q * q
+        return q[:, 0] * q[:, 0] + q[:, 1] * q[:, 1] + q[:, 2] * q[:, 2] <= self.r4
+
+    def Contains(self, p):
+        return bool(self.ContainsBatch(np.asarray([p]))[0])
+
+
 def solid_catalogue():
     """Named solids covering every lowered type (small enough for the oracle
     at delta ~0.05)."""
@@ -86,6 +112,14 @@ def solid_catalogue():
     sph_tris, _ = oracle_search(m3.Sphere((0.05, -0.02, 0.1), 0.7), 0.1, 4, threads=8)
     col_sph = m3.NewColliderSolid(m3.MeshCollider(sph_tris))
     col_rot = m3.RotateSolid(col_sph, (0.0, 0.6, 0.8), 0.5)
+    # Host-evaluated solids: alone, and under transforms inside a CSG tree
+    # beside native atoms (the callback sees points in its own frame).
+    host = Superellipsoid((0.05, -0.1, 0.02), 0.75)
+    host_csg = m3.SubtractedSolid(
+        m3.JoinedSolid([m3.RotateSolid(m3.TranslateSolid(Superellipsoid((0, 0, 0), 0.5), (0.2, 0.0, 0.1)),
+                                       (0.3, 0.4, 0.5), 0.6), cap]),
+        m3.Sphere((0.3, 0.1, 0.2), 0.3),
+    )
     return {
         "sphere": sph,
         "rect": box,
@@ -105,4 +139,6 @@ def solid_catalogue():
         "collider_boxes": col_boxes,
         "collider_sphere": col_sph,
         "collider_rotated": col_rot,
+        "host_superellipsoid": host,
+        "host_in_csg": host_csg,
     }

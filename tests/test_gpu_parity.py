@@ -195,7 +195,6 @@ class _LyingBounds(m3.Solid):
         return np.linalg.norm(np.asarray(pts), axis=1) <= 1.0
 
 
-@pytest.mark.skip(reason="host-evaluated solids need the batch host fallback")
 def test_inside_boundary_panics():
     forced = m3.ForceSolidBounds(m3.Sphere((0, 0, 0), 0.5), (-0.4, -0.4, -0.4), (0.4, 0.4, 0.4))
     ok = m3.MarchingCubesSearch(forced, 0.1, 2)
@@ -203,6 +202,48 @@ def test_inside_boundary_panics():
     with pytest.raises(m3.Panic) as e:
         m3.MarchingCubesSearch(_LyingBounds(), 0.1, 0)
     assert "solid is true outside of bounds" in str(e.value)
+
+
+class _Failing(m3.Solid):
+    def Min(self):
+        return (-0.5, -0.5, -0.5)
+
+    def Max(self):
+        return (0.5, 0.5, 0.5)
+
+    def ContainsBatch(self, pts):
+        raise RuntimeError("user solid failed")
+
+
+def test_host_callback_error_is_reported():
+    with pytest.raises(m3.MC3DError) as e:
+        m3.MarchingCubesSearch(m3.JoinedSolid([m3.Sphere((0, 0, 0), 0.3), _Failing()]), 0.1, 2)
+    assert "callback" in str(e.value)
+    with pytest.raises(m3.MC3DError):
+        m3.Contains(_Failing(), np.zeros((4, 3)))
+    # the context is still usable afterwards
+    assert m3.MarchingCubesSearch(m3.Sphere((0, 0, 0), 0.3), 0.1, 2).NumTriangles() > 0
+
+
+def test_host_solid_batches_and_rounds():
+    """A host-evaluated solid on a grid of more than one callback batch
+    (2^20 points), with the bisection rounds answered by the host."""
+    from tests.helpers import Superellipsoid
+
+    s = m3.SubtractedSolid(Superellipsoid((0.0, 0.0, 0.0), 0.9), m3.Sphere((0.2, 0.1, 0.0), 0.5))
+    print("host solid d=0.015 (>1M corners)", flush=True)
+    meshes = []
+    for precision in (FP64, GUARD):
+        got = m3.MarchingCubesSearch(s, 0.015, 3, precision=precision)
+        assert got.stats["corners"] > (1 << 20)
+        assert got.stats["ms_host_callbacks"] > 0
+        meshes.append(canonical(got.triangles()))
+    assert np.array_equal(meshes[0], meshes[1])
+    directed_once, paired = edge_topology(got.triangles())
+    assert directed_once and paired
+    # The oracle asks the callback one point at a time: compare on a coarser grid.
+    want, _ = oracle_search(s, 0.05, 3, threads=8)
+    assert same_triangles(m3.MarchingCubesSearch(s, 0.05, 3).triangles(), want)
 
 
 def test_invalid_bounds_panics():

@@ -392,11 +392,12 @@ def test_collider_matches_numpy_restatement():
 
 
 def test_new_mesh_rect_is_closed_and_outward():
-    t = m3.NewMeshRect((0, 0, 0), (1, 2, 3)).triangles()
-    assert len(t) == 12
+    m = m3.NewMeshRect((0, 0, 0), (1, 2, 3))
+    t = m.triangles()
+    assert len(t) == 12 and m.NumTriangles() == 12
     directed_once, paired = edge_topology(t)
     assert directed_once and paired
-    assert abs(volume(t) - 6.0) < 1e-12
+    assert abs(volume(t) - 6.0) < 1e-12 and m.Volume() == 6.0
 
 
 def test_csg64_oracle_smoke():
