@@ -1,11 +1,25 @@
 """Benchmark: MarchingCubesSearch corner evals/s + tris/s on one MI355X.
 
-A step is one MarchingCubesSearch (model3d/mc.go:45-49) of the workload on
-the GPU, results left in HBM (device_only); the solid is lowered and uploaded
-inside the step.  Default workload: the seeded 64-primitive CSG tree clipped
-to [-1, 1]^3 at delta 0.005 (402^3 ~ 65M corners), iters 8.
+A step is one call of the library on the workload, results left in HBM
+(device_only); the solids are lowered and uploaded inside the step.
 
-Prints one JSON line (see DESIGN.md for every field).
+  csg64     (default) the seeded 64-primitive CSG tree clipped to [-1, 1]^3 at
+            delta 0.005 (402 x 395 x 402 ~ 64M corners), iters 8: one
+            MarchingCubesSearch (model3d/mc.go:45-49) per step
+  sphere    Sphere{0, 1} at delta 0.02, iters 8
+  collider  the CSG tree meshed at delta 0.02, as a ColliderSolid, 256 samples
+            along its longest axis, iters 8
+  batch     256 seeded CSG solids of 4..128 primitives on 32..256-cell grids,
+            iters 4..8, all in one mc3d_marching_cubes_search_batch call
+
+Prints one JSON line: the metric, the per-kernel times (HIP events on the
+library's stream), the roofline of the dominant kernel (algorithmic flops or
+bytes per launch over its average duration; HBM traffic per launch from the
+committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same
+configuration), the guard statistics, the plain one-solid-at-a-time float64
+GPU path, the end-to-end time with the copy to host, and the CPU baseline
+(the oracle, a C restatement of the reference algorithm, on this host's
+cores).
 """
 from __future__ import annotations
 
@@ -23,52 +37,127 @@ METRIC = "MarchingCubesSearch corner evals/s + tris/s, FP32+guard vs FP64, 1 GPU
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
+COPY_CALIBRATION_BYTES = 1 << 30
+CPU_SAMPLE_SECONDS = 10.0
+
+# The configuration each workload runs in, as chosen by model3d_amd.autotune
+# on an MI355X (`--autotune` searches again): precision, culling, brick edge,
+# grid rows per program walk, wavefronts per brick.
+DEFAULT_RUN = {
+    "csg64": dict(precision="fp64", cull=True, brick=4, rows_per_walk=2, waves_per_brick=4),
+    "sphere": dict(precision="fp64", cull=False, brick=0, rows_per_walk=1, waves_per_brick=4),
+    "collider": dict(precision="fp64", cull=True, brick=4, rows_per_walk=1, waves_per_brick=4),
+    "batch": dict(precision="fp64", cull=True, brick=4, rows_per_walk=1, waves_per_brick=2),
+}
+PLAIN_RUN = dict(precision="fp64", cull=False, brick=0, rows_per_walk=1, waves_per_brick=4)
+
+# Kernel of each timed stage (mc3d_stats.ms_*), for the roofline and the
+# rocprofv3 summaries.
+STAGE_KERNEL = {
+    "ms_plan": "k_plan",
+    "ms_fill": "k_fill_const",
+    "ms_corners": "k_corners_act",
+    "ms_classify": "k_classify",
+    "ms_scan": "k_scan_*",
+    "ms_emit": "k_emit",
+    "ms_bisect": "k_bisect",
+}
+SUMMED = ("corners", "vertices", "triangles", "bisection_evals", "bricks_total", "program_length")
 
 
 def workload(name):
+    """{"solids", "deltas", "iters", "cfg"} of a named workload."""
     from model3d_amd import workloads
 
     if name == "csg64":
         c = workloads.CSG64_CONFIG
-        return workloads.csg64_solid(c["seed"], c["n_prims"]), c
+        return dict(solids=[workloads.csg64_solid(c["seed"], c["n_prims"])], deltas=[c["delta"]],
+                    iters=[c["iters"]], cfg=c)
     if name == "sphere":
-        return workloads.sphere_solid(), workloads.SPHERE_CONFIG
+        c = workloads.SPHERE_CONFIG
+        return dict(solids=[workloads.sphere_solid()], deltas=[c["delta"]], iters=[c["iters"]], cfg=c)
     if name == "collider":
         c = dict(workloads.COLLIDER_CONFIG)
         solid, delta, ntri = workloads.collider_solid(c["seed"], c["n_prims"], c["mesh_delta"])
         c["delta"], c["mesh_triangles"] = delta, ntri
-        return solid, c
+        return dict(solids=[solid], deltas=[delta], iters=[c["iters"]], cfg=c)
+    if name == "batch":
+        c = workloads.BATCH_CONFIG
+        solids, deltas, iters = workloads.batch_solids(c["seed"], c["n_solids"], c["prims"], c["cells"], c["iters"])
+        return dict(solids=solids, deltas=deltas, iters=iters, cfg=c)
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_gpu(solid, cfg, precision, cull, steps, warmup, count_mismatch=False, tile=None):
+def precision_code(name):
+    from model3d_amd import _lib
+
+    return _lib.MC3D_PREC_FP32_GUARD if name == "fp32_guard" else _lib.MC3D_PREC_FP64
+
+
+def merge_stats(per_mesh):
+    """Call-level stats (timings, guard and operation counts are per call)
+    with the per-solid counts summed and the occupancy words of all grids."""
+    st = dict(per_mesh[0])
+    for k in SUMMED:
+        st[k] = sum(m[k] for m in per_mesh)
+    st["words"] = sum((m["grid_nx"] + 63) // 64 * m["grid_ny"] * m["grid_nz"] for m in per_mesh)
+    return st
+
+
+def run_gpu(work, run, steps, warmup, count_mismatch=False, device_only=True, each=False):
+    """`steps` timed steps; a step is one library call on all the solids (the
+    batch entry point for several), or with `each` one call per solid.
+    Returns (wall seconds, merged stats per step)."""
     from model3d_amd import _lib
     from model3d_amd.solids import LibBackend
 
     L = _lib.lib()
     ctx = _lib.context(0)
     b = LibBackend()
-    h = solid._build(b)
+    solids = work["solids"]
+    n = len(solids)
+    hs = [s._build(b) for s in solids]
     opts = _lib.mc3d_options()
-    opts.precision = precision
-    opts.cull = 1 if cull else 0
-    if tile:
-        opts.brick, opts.rows_per_walk = tile["brick"], tile["rows_per_walk"]
-    opts.device_only = 1
+    opts.precision = precision_code(run["precision"])
+    opts.cull = 1 if run["cull"] else 0
+    opts.brick, opts.rows_per_walk, opts.waves_per_brick = run["brick"], run["rows_per_walk"], run["waves_per_brick"]
+    opts.device_only = 1 if device_only else 0
     opts.count_mismatch = 1 if count_mismatch else 0
+    harr = (ctypes.c_void_p * n)(*hs)
+    darr = (ctypes.c_double * n)(*work["deltas"])
+    iarr = (ctypes.c_int32 * n)(*work["iters"])
     stats = []
 
+    def mesh_stats(m):
+        st = _lib.mc3d_stats()
+        L.mc3d_mesh_stats(m, ctypes.byref(st))
+        return st.as_dict()
+
     def step(record):
-        out = ctypes.c_void_p()
-        _lib.check(
-            L.mc3d_marching_cubes_search(ctx, h, cfg["delta"], cfg["iters"], ctypes.byref(opts), ctypes.byref(out)),
-            "mc3d_marching_cubes_search",
-        )
+        per = []
+        if n == 1 or each:
+            # one call per solid, each mesh consumed (freed) before the next call
+            for i in range(n):
+                one = ctypes.c_void_p()
+                _lib.check(L.mc3d_marching_cubes_search(ctx, hs[i], darr[i], iarr[i], ctypes.byref(opts),
+                                                        ctypes.byref(one)), "mc3d_marching_cubes_search")
+                if record:
+                    per.append(mesh_stats(one))
+                L.mc3d_mesh_free(one)
+        else:
+            outs = (ctypes.c_void_p * n)()
+            _lib.check(L.mc3d_marching_cubes_search_batch(ctx, harr, darr, iarr, n, ctypes.byref(opts), outs),
+                       "mc3d_marching_cubes_search_batch")
+            if record:
+                per = [mesh_stats(outs[i]) for i in range(n)]
+            for i in range(n):
+                L.mc3d_mesh_free(outs[i])
         if record:
-            st = _lib.mc3d_stats()
-            L.mc3d_mesh_stats(out, ctypes.byref(st))
-            stats.append(st.as_dict())
-        L.mc3d_mesh_free(out)
+            merged = merge_stats(per)
+            if n > 1 and each:  # one call per solid: its stage times add up
+                for k in list(STAGE_KERNEL) + ["ms_device"]:
+                    merged[k] = sum(m[k] for m in per)
+            stats.append(merged)
 
     for _ in range(warmup):
         step(False)
@@ -76,7 +165,8 @@ def run_gpu(solid, cfg, precision, cull, steps, warmup, count_mismatch=False, ti
     for _ in range(steps):
         step(True)
     wall = time.perf_counter() - t0
-    b.release(h)
+    for h in hs:
+        b.release(h)
     b.close()
     return wall, stats
 
@@ -85,21 +175,25 @@ def mean(stats, key):
     return sum(s[key] for s in stats) / max(1, len(stats))
 
 
-def cpu_baseline(solid, cfg):
-    from oracle.binding import oracle_search
-
+def host_threads():
     try:
         threads = len(os.sched_getaffinity(0))
     except AttributeError:
         threads = os.cpu_count() or 1
-    threads = max(1, min(threads, 16))
+    return max(1, min(threads, 16))
+
+
+def cpu_baseline(work):
+    from oracle.binding import oracle_contains, oracle_search
+
+    cfg = work["cfg"]
+    threads = host_threads()
     if cfg.get("mesh_triangles"):
         # Brute-force ray parity over the whole mesh is far too slow for the
         # full grid on the CPU: time Contains on a sample of grid corners.
         import numpy as np
 
-        from oracle.binding import oracle_contains
-
+        solid = work["solids"][0]
         mn, mx = solid.Min(), solid.Max()
         rng = np.random.default_rng(0)
         pts = rng.uniform(mn, mx, size=(2000, 3))
@@ -114,18 +208,113 @@ def cpu_baseline(solid, cfg):
             "sample": f"{len(pts)} Contains calls at random points in the bounds, one thread, brute-force ray "
             f"parity over {cfg['mesh_triangles']} triangles, {wall:.2f} s",
         }
-    t0 = time.perf_counter()
-    tris, info = oracle_search(solid, cfg["delta"], cfg["iters"], threads=threads)
-    wall = time.perf_counter() - t0
+    corners = ntris = 0
+    wall = 0.0
+    done = 0
+    for s, d, it in zip(work["solids"], work["deltas"], work["iters"]):
+        t0 = time.perf_counter()
+        tris, info = oracle_search(s, d, it, threads=threads)
+        wall += time.perf_counter() - t0
+        corners += info["corners"]
+        ntris += len(tris)
+        done += 1
+        if wall > CPU_SAMPLE_SECONDS:
+            break
+    n = len(work["solids"])
+    what = "the full workload once" if done == n else f"the first {done} of the {n} solids"
     return {
-        "value": info["corners"] / wall,
+        "value": corners / wall,
         "unit": "corner evals/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"the full workload once ({info['corners']} corners, {len(tris)} triangles), C restatement of "
-        f"mc.go with its plane-pipeline/parallel-search structure, {wall:.2f} s",
-        "tris_per_s": len(tris) / wall,
+        "sample": f"{what} ({corners} corners, {ntris} triangles), C restatement of mc.go with its "
+        f"plane-pipeline/parallel-search structure, {wall:.2f} s",
+        "tris_per_s": ntris / wall,
     }
+
+
+def algorithmic(stage, st):
+    """Algorithmic work of one launch of a stage's kernel(s): ("flop", f64, f32)
+    for the evaluation stages, ("bytes", B) for the grid / compaction stages.
+
+    fill    : writes the occupancy word of every word of a constant brick -> 8 B per word
+    classify: reads the occupancy grid once (N/8 B) and writes a 16 B count per word
+    scan    : reads the counts twice, writes the offsets (16 B each) and a 4 B
+              entry per listed word
+    emit    : reads the grid words it lists and their offsets, writes an 8 B key
+              per vertex and 12 B per triangle
+    plan    : writes an 8 B brick record per brick and a 4 B list entry per
+              active brick
+    corners / bisect: the counted operations of every evaluated primitive,
+              transform and midpoint (float64, float32)
+    """
+    n, v, t = st["corners"], st["vertices"], st["triangles"]
+    words = st["words"]
+    if stage == "ms_corners":
+        return ("flop", st["corner_flops64"], st["corner_flops32"])
+    if stage == "ms_bisect":
+        return ("flop", st["bisect_flops64"], st["bisect_flops32"])
+    b = {
+        "ms_plan": 8 * st["bricks_total"] + 4 * st["bricks_active"],
+        "ms_fill": 8 * words,
+        "ms_classify": n / 8 + 16 * words,
+        "ms_scan": 48 * words,
+        "ms_emit": n / 8 + 16 * words + 8 * v + 12 * t,
+    }[stage]
+    return ("bytes", b)
+
+
+def roofline_of(stage, ms, st):
+    kind = algorithmic(stage, st)
+    if kind[0] == "flop":
+        _, f64, f32 = kind
+        # float64-equivalent rate: float32 runs at twice the float64 vector rate
+        achieved = (f64 + f32 / 2) / (ms * 1e-3) / 1e12
+        r = {"bound": "mfma", "pipe": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+             "frac": achieved / FP64_PEAK_TFLOPS, "flops64_per_launch": f64, "flops32_per_launch": f32}
+    else:
+        achieved = kind[1] / (ms * 1e-3) / 1e9
+        r = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": achieved / HBM_PEAK_GBS, "bytes_per_launch": kind[1]}
+    r["traffic"] = None
+    r["kernel"] = STAGE_KERNEL[stage]
+    r["avg_ms"] = ms
+    return r
+
+
+def load_traffic(workload_name, run, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    of this workload and configuration, or (None, reason)."""
+    from model3d_amd.profiling import traffic_bytes
+
+    path = os.path.join(ROOT, "profiles", f"{workload_name}_pmc_traffic.json")
+    if not os.path.exists(path):
+        return None, "no rocprofv3 FETCH_SIZE/WRITE_SIZE summary for this workload"
+    s = json.load(open(path))
+    if s.get("config") != run:
+        return None, f"summary measured at {s.get('config')}, not at this configuration"
+    names = [k for k in s.get("kernels", {}) if k.split("<")[0] == kernel]
+    if len(names) != 1:
+        return None, f"kernel {kernel} not (uniquely) in the summary"
+    return traffic_bytes(s, names[0]), os.path.relpath(path, ROOT)
+
+
+def copy_calibration():
+    """One device-to-device hipMemcpy of COPY_CALIBRATION_BYTES (a plain copy
+    kernel, the largest of its dispatches in the run), for the FETCH_SIZE /
+    WRITE_SIZE calibration of a rocprofv3 --pmc run."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(COPY_CALIBRATION_BYTES)
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), n) != 0:
+            raise SystemExit("hipMalloc failed for the copy calibration")
+    ok = hip.hipMemset(src, 1, n) == 0 and hip.hipMemcpy(dst, src, n, 3) == 0  # hipMemcpyDeviceToDevice
+    ok = ok and hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    if not ok:
+        raise SystemExit("copy calibration failed")
 
 
 def main():
@@ -133,33 +322,53 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="csg64", choices=["csg64", "sphere", "collider"])
+    ap.add_argument("--workload", default="csg64", choices=sorted(DEFAULT_RUN))
+    ap.add_argument("--autotune", action="store_true", help="search tile shapes and precision first")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="only the warm-up and timed steps (for rocprofv3), plus the copy calibration kernel")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
             "bench.py measures one GPU only: this build runs on single-GPU boxes and has no multi-GPU path "
-            "(z-slab / per-solid sharding is a design, see DESIGN.md)",
+            "(z-slab / per-solid sharding is a design only)",
             file=sys.stderr,
         )
         raise SystemExit(2)
 
-    from model3d_amd import _lib
+    from model3d_amd import _lib  # noqa: F401  (fails loudly without the HIP library)
 
-    solid, cfg = workload(args.workload)
-    # Tile shapes from the autotuner (untimed), then the timed steps.
-    from model3d_amd import autotune
+    work = workload(args.workload)
+    cfg = work["cfg"]
+    run = dict(DEFAULT_RUN[args.workload])
+    tuned = None
+    if args.autotune:
+        from model3d_amd import autotune
 
-    tuned = {}
-    for label, prec in (("fp32_guard", _lib.MC3D_PREC_FP32_GUARD), ("fp64", _lib.MC3D_PREC_FP64)):
-        tuned[label] = autotune.tune(solid, cfg["delta"], cfg["iters"], precision=prec)
-    best = "fp32_guard" if tuned["fp32_guard"]["ms"] <= tuned["fp64"]["ms"] else "fp64"
-    precision = _lib.MC3D_PREC_FP32_GUARD if best == "fp32_guard" else _lib.MC3D_PREC_FP64
-    tile = tuned[best]
-    wall, st = run_gpu(solid, cfg, precision, True, args.steps, args.warmup, tile=tile)
-    corners, tris, verts = st[0]["corners"], st[0]["triangles"], st[0]["vertices"]
+        tuned = {}
+        single = len(work["solids"]) == 1
+        target = work["solids"][0] if single else work["solids"]
+        deltas = work["deltas"][0] if single else work["deltas"]
+        iters = work["iters"][0] if single else work["iters"]
+        for label in ("fp32_guard", "fp64"):
+            tuned[label] = autotune.tune(target, deltas, iters, precision=precision_code(label))
+        best = min(tuned, key=lambda k: tuned[k]["ms"])
+        run = dict(precision=best, cull=True, brick=tuned[best]["brick"], rows_per_walk=tuned[best]["rows_per_walk"],
+                   waves_per_brick=tuned[best]["waves_per_brick"])
+
+    if args.profile_run:
+        copy_calibration()
+        wall, st = run_gpu(work, run, args.steps, args.warmup)
+        print(json.dumps({"workload": args.workload, "config": run, "ms_per_step": 1e3 * wall / args.steps,
+                          "stages_ms": {k: mean(st, k) for k in STAGE_KERNEL}}), flush=True)
+        return
+
+    wall, st = run_gpu(work, run, args.steps, args.warmup)
+    s0 = st[0]
+    corners, tris, verts = s0["corners"], s0["triangles"], s0["vertices"]
     ms = 1e3 * wall / args.steps
+    nsolids = len(work["solids"])
     result = {
         "metric": METRIC,
         "value": corners / (wall / args.steps),
@@ -171,34 +380,40 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if best == "fp32_guard" else "f64",
+        "dtype": "f32+f64" if run["precision"] == "fp32_guard" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{cfg['name']}_iters{cfg['iters']}",
+            "workload": f"{cfg['name']}_iters{cfg['iters']}" if nsolids == 1 else cfg["name"],
+            "solids": nsolids,
             "corners": corners,
             "vertices": verts,
             "triangles": tris,
-            "bisection_evals": st[0]["bisection_evals"],
-            "precision": best,
-            "cull": True,
-            "brick": tile["brick"],
-            "rows_per_walk": tile["rows_per_walk"],
+            "bisection_evals": s0["bisection_evals"],
+            **run,
         },
         "tris_per_s": tris / (wall / args.steps),
-        "autotune": {
-            k: {"brick": v["brick"], "rows_per_walk": v["rows_per_walk"], "best_ms": v["ms"]}
-            for k, v in tuned.items()
-        },
     }
-    stage_keys = ("ms_plan", "ms_corners", "ms_classify", "ms_emit", "ms_bisect", "ms_device")
-    stages = {k: mean(st, k) for k in stage_keys}
+    if tuned:
+        result["autotune"] = {k: {"brick": v["brick"], "rows_per_walk": v["rows_per_walk"],
+                                  "waves_per_brick": v["waves_per_brick"], "best_ms": v["ms"]} for k, v in tuned.items()}
+    stages = {k: mean(st, k) for k in STAGE_KERNEL}
+    stages["ms_device"] = mean(st, "ms_device")
     result["stages_ms"] = stages
-    result["bricks"] = {"total": st[0]["bricks_total"], "active": st[0]["bricks_active"]}
+    result["bricks"] = {"total": s0["bricks_total"], "active": s0["bricks_active"], "full": s0["bricks_full"],
+                        "code_mean": s0["brick_code_total"] / max(1, s0["bricks_active"]),
+                        "code_max": s0["brick_code_max"], "program": s0["program_length"]}
 
-    # Guard statistics from one extra (untimed) run that also evaluates every
-    # float32 decision in float64.
-    _, sst = run_gpu(solid, cfg, _lib.MC3D_PREC_FP32_GUARD, True, 1, 0, count_mismatch=True)
-    g = sst[0]
+    # One untimed statistics run of the same configuration: the algorithmic
+    # operation counts of the evaluation stages.
+    _, sst = run_gpu(work, run, 1, 0, count_mismatch=True)
+    counts = dict(sst[0])
+
+    # Guard statistics (float32 decisions checked against float64).
+    if run["precision"] == "fp32_guard":
+        g = sst[0]
+    else:
+        _, gst = run_gpu(work, dict(run, precision="fp32_guard"), 1, 0, count_mismatch=True)
+        g = gst[0]
     result["guard"] = {
         "fp32_atom_evals": g["guard_evals"],
         "fp64_rechecks": g["guard_rechecks"],
@@ -207,59 +422,49 @@ def main():
         "band_violations": g["guard_violations"],
     }
 
-    if not args.no_fp64:
-        n64 = max(1, args.steps // 2)
-        for label, cull in (("fp64_plain", False), ("fp64_cull", True)):
-            wall64, st64 = run_gpu(solid, cfg, _lib.MC3D_PREC_FP64, cull, n64, 1)
-            ms64 = 1e3 * wall64 / n64
-            result[label] = {
-                "corner_evals_per_s": corners / (ms64 / 1e3),
-                "tris_per_s": tris / (ms64 / 1e3),
-                "ms_per_step": ms64,
-                "stages_ms": {k: mean(st64, k) for k in stage_keys},
-            }
-        result["speedup_vs_fp64_plain"] = result["fp64_plain"]["ms_per_step"] / ms
+    # Roofline of every kernel; the dominant one is the headline.
+    kernels = {}
+    for stage in STAGE_KERNEL:
+        if stages[stage] > 0:
+            kernels[STAGE_KERNEL[stage]] = roofline_of(stage, stages[stage], counts)
+    if not run["cull"]:
+        kernels.pop("k_corners_act", None)
+        kernels["k_corners"] = dict(roofline_of("ms_corners", stages["ms_corners"], counts), kernel="k_corners")
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
+    traffic, source = load_traffic(args.workload, run, dom.rstrip("*").rstrip("_"))
+    head = dict(kernels[dom])
+    head["traffic"] = traffic
+    head["traffic_source"] = source
+    result["roofline"] = head
+    result["kernels_roofline"] = {k: {x: r[x] for x in ("bound", "achieved", "unit", "frac", "avg_ms")}
+                                  for k, r in kernels.items()}
 
-    # Roofline of the dominant stage (HIP events on the library's stream).
-    dom = max(("ms_plan", "ms_corners", "ms_classify", "ms_emit", "ms_bisect"), key=lambda k: stages[k])
-    result["roofline"] = roofline(dom, stages[dom], st[0])
+    # End to end through the C-ABI with the copy of the meshes to the host.
+    n_e2e = max(1, args.steps // 2)
+    wall_e2e, est = run_gpu(work, run, n_e2e, 1, device_only=False)
+    result["e2e_with_d2h"] = {"ms_per_step": 1e3 * wall_e2e / n_e2e, "ms_d2h": mean(est, "ms_d2h")}
+
+    if not args.no_fp64:
+        # the plain float64 GPU path: every atom at every corner, one call per solid
+        n64 = max(1, min(args.steps // 2, 3))
+        wall64, st64 = run_gpu(work, PLAIN_RUN, n64, 1, each=True)
+        ms64 = 1e3 * wall64 / n64
+        result["fp64_plain"] = {
+            "corner_evals_per_s": corners / (ms64 / 1e3),
+            "tris_per_s": tris / (ms64 / 1e3),
+            "ms_per_step": ms64,
+            "stages_ms": {k: mean(st64, k) for k in STAGE_KERNEL},
+        }
+        result["speedup_vs_fp64_plain"] = ms64 / ms
+        if nsolids > 1:
+            # the same configuration, one call per solid
+            walle, _ = run_gpu(work, run, n64, 1, each=True)
+            result["one_call_per_solid"] = {"ms_per_step": 1e3 * walle / n64,
+                                            "speedup_of_batch": (1e3 * walle / n64) / ms}
 
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(solid, cfg)
+        result["cpu_baseline"] = cpu_baseline(work)
     print(json.dumps(result), flush=True)
-
-
-def roofline(stage, ms, st):
-    """Algorithmic bytes of the dominant stage per launch / its duration.
-
-    corners : writes the occupancy grid, 1 bit per corner  -> N/8 bytes
-    classify: reads the grid ~4x (row + 3 neighbour rows), writes 8 B counts
-              per word, scans them                          -> ~N/2 + 4*N/64*8
-    emit    : reads grid + offsets, writes 8 B per vertex key and 12 B per
-              triangle                                     -> N/8*4 + 8V + 12T
-    bisect  : reads 8 B key, writes 24 B per vertex         -> 32 V
-    """
-    n, v, t = st["corners"], st["vertices"], st["triangles"]
-    words = n / 64
-    algo = {
-        "ms_plan": st["bricks_total"] * 24,
-        "ms_corners": n / 8,
-        "ms_classify": 4 * n / 8 + words * 8 * 3,
-        "ms_emit": 4 * n / 8 + words * 8 + 8 * v + 12 * t,
-        "ms_bisect": 32 * v,
-    }[stage]
-    gbs = algo / (ms * 1e-3) / 1e9
-    return {
-        "bound": "hbm",
-        "kernel": {"ms_plan": "k_plan", "ms_corners": "k_corners", "ms_classify": "k_classify",
-                   "ms_emit": "k_emit", "ms_bisect": "k_bisect"}[stage],
-        "achieved": gbs,
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": gbs / HBM_PEAK_GBS,
-        "traffic": None,
-        "avg_ms": ms,
-    }
 
 
 if __name__ == "__main__":

@@ -127,7 +127,8 @@ typedef struct {
     int32_t count_mismatch; /* 1: also count float32 sign mismatches (slower; statistics only) */
     int32_t brick;       /* culling brick: 64 x brick x brick corners (0: default 4) */
     int32_t rows_per_walk; /* grid rows evaluated per walk of a brick program: 1, 2 or 4 (0: default 1) */
-    int32_t reserved[2];
+    int32_t waves_per_brick; /* wavefronts sharing one brick: 1, 2 or 4 (0: default 4) */
+    int32_t reserved;
 } mc3d_options;
 
 typedef struct {
@@ -141,8 +142,20 @@ typedef struct {
     uint64_t guard_violations; /* band-sure decision != float64 decision (count_mismatch only; must be 0) */
     uint64_t bricks_total;     /* culling bricks */
     uint64_t bricks_active;    /* bricks whose simplified program is not a constant */
+    /* Algorithmic operation counts of the corner and bisection stages, float64
+     * and float32 (count_mismatch only): per evaluated primitive a fixed count
+     * of add/sub/mul/div/sqrt/compare, plus transforms and midpoints. */
+    uint64_t corner_flops64, corner_flops32, bisect_flops64, bisect_flops32;
+    uint64_t grid_nx, grid_ny, grid_nz; /* |Xs|, |Ys|, |Zs| */
+    uint64_t brick_code_total;  /* instructions over all active bricks' folded programs */
+    uint64_t brick_code_max;    /* longest folded brick program */
+    uint64_t bricks_full;       /* active bricks whose folded program overflowed (full program used) */
+    uint64_t program_length;    /* instructions of the whole lowered program */
     double ms_device;          /* GPU time of the whole pipeline (HIP events) */
-    double ms_plan, ms_corners, ms_classify, ms_emit, ms_bisect; /* per stage (HIP events) */
+    /* Per kernel (HIP events on the context's stream): k_plan, k_fill_const,
+     * the corner kernel, k_classify, the three scan kernels, k_emit, the
+     * bisection (with host-evaluated atoms: its rounds, callbacks included). */
+    double ms_plan, ms_fill, ms_corners, ms_classify, ms_scan, ms_emit, ms_bisect;
     double ms_d2h;             /* copy of results to host */
     double ms_host_callbacks;  /* time spent in mc3d_contains_fn */
     double ms_call;            /* host wall time of the whole call */
@@ -152,6 +165,14 @@ typedef struct {
  * MarchingCubes, mc.go:14-37).  opts may be NULL (FP32 guard + culling). */
 int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
                                const mc3d_options *opts, mc3d_mesh **out);
+/* MarchingCubesSearch(solids[i], deltas[i], iters[i]) for i < n in one pass
+ * of the pipeline (one launch per stage for all solids; solids with
+ * host-evaluated atoms run alone).  out[i] receives solid i's mesh, an
+ * independent mc3d_mesh.  The reference meshes one solid per call
+ * (mc.go:45-49); this is that call over a list, for callers with many
+ * solids.  On error no mesh is returned and the message names the solid. */
+int mc3d_marching_cubes_search_batch(mc3d_ctx *ctx, const mc3d_solid *const *solids, const double *deltas,
+                                     const int32_t *iters, size_t n, const mc3d_options *opts, mc3d_mesh **out);
 int mc3d_mesh_counts(const mc3d_mesh *m, size_t *n_vertices, size_t *n_triangles);
 /* Copies the results to host arrays of 3*V doubles and 3*T uint32. */
 int mc3d_mesh_copy(mc3d_mesh *m, double *verts, uint32_t *tris);

@@ -9,7 +9,16 @@ from ._lib import (  # noqa: F401
     MC3D_PREC_FP64,
     MC3DError,
 )
-from .mc import Contains, MarchingCubes, MarchingCubesSearch, Mesh, NewMeshRect, Panic, case_table  # noqa: F401
+from .mc import (  # noqa: F401
+    Contains,
+    MarchingCubes,
+    MarchingCubesSearch,
+    MarchingCubesSearchBatch,
+    Mesh,
+    NewMeshRect,
+    Panic,
+    case_table,
+)
 from .solids import (  # noqa: F401
     CacheSolidBounds,
     Capsule,

@@ -48,7 +48,8 @@ class mc3d_options(ctypes.Structure):
         ("count_mismatch", ctypes.c_int32),
         ("brick", ctypes.c_int32),
         ("rows_per_walk", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("waves_per_brick", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -64,10 +65,23 @@ class mc3d_stats(ctypes.Structure):
         ("guard_violations", ctypes.c_uint64),
         ("bricks_total", ctypes.c_uint64),
         ("bricks_active", ctypes.c_uint64),
+        ("corner_flops64", ctypes.c_uint64),
+        ("corner_flops32", ctypes.c_uint64),
+        ("bisect_flops64", ctypes.c_uint64),
+        ("bisect_flops32", ctypes.c_uint64),
+        ("grid_nx", ctypes.c_uint64),
+        ("grid_ny", ctypes.c_uint64),
+        ("grid_nz", ctypes.c_uint64),
+        ("brick_code_total", ctypes.c_uint64),
+        ("brick_code_max", ctypes.c_uint64),
+        ("bricks_full", ctypes.c_uint64),
+        ("program_length", ctypes.c_uint64),
         ("ms_device", ctypes.c_double),
         ("ms_plan", ctypes.c_double),
+        ("ms_fill", ctypes.c_double),
         ("ms_corners", ctypes.c_double),
         ("ms_classify", ctypes.c_double),
+        ("ms_scan", ctypes.c_double),
         ("ms_emit", ctypes.c_double),
         ("ms_bisect", ctypes.c_double),
         ("ms_d2h", ctypes.c_double),
@@ -107,6 +121,11 @@ SIGNATURES = {
     "mc3d_marching_cubes_search": (
         ctypes.c_int,
         [_P, _P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(mc3d_options), ctypes.POINTER(_P)],
+    ),
+    "mc3d_marching_cubes_search_batch": (
+        ctypes.c_int,
+        [_P, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t,
+         ctypes.POINTER(mc3d_options), ctypes.POINTER(_P)],
     ),
     "mc3d_mesh_counts": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "mc3d_mesh_copy": (ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p]),

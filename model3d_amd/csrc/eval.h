@@ -37,7 +37,7 @@ __device__ __forceinline__ bool in_hi(double c, double m) { return m == INFINITY
 // Triangle.rayCollision (primitives.go:207-249) from its point-independent
 // constants, counted by RayCollisions when `scale < 0` is false.
 __device__ __forceinline__ bool collider_contains(const DevCollider &C, const double *__restrict__ a, double x,
-                                                  double y, double z) {
+                                                  double y, double z, uint32_t *candidates = nullptr) {
     if (!(in_lo(x, a[1]) && in_lo(y, a[2]) && in_lo(z, a[3]) && in_hi(x, a[4]) && in_hi(y, a[5]) && in_hi(z, a[6])))
         return false;
     const double fu = (x * C.u[0] + y * C.u[1] + z * C.u[2] - C.u0) * C.inv_cell;
@@ -45,6 +45,7 @@ __device__ __forceinline__ bool collider_contains(const DevCollider &C, const do
     if (!(fu >= 0 && fu < C.nu && fv >= 0 && fv < C.nv)) return false;  // no triangle can be hit
     const uint32_t cell = (uint32_t)fv * (uint32_t)C.nu + (uint32_t)fu;
     const uint32_t e = C.cell_start[cell + 1];
+    if (candidates) *candidates += e - C.cell_start[cell];
     const double Dx = 0.5224892708603626, Dy = 0.10494477243214506, Dz = 0.43558938446126527;
     uint32_t hits = 0;
     for (uint32_t i = C.cell_start[cell]; i < e; i++) {
@@ -61,13 +62,43 @@ __device__ __forceinline__ bool collider_contains(const DevCollider &C, const do
     return hits & 1u;
 }
 
-// Guard statistics, kept in registers and reduced once per block.
+// Guard statistics and algorithmic flop counts (statistics runs only), kept
+// in registers and reduced once per block.
 struct GStats {
     uint32_t evals = 0;       // float32 atom evaluations
     uint32_t rechecks = 0;    // of those, in the band -> float64
     uint32_t mismatches = 0;  // plain float32 decision != float64 decision
     uint32_t violations = 0;  // band said sure but float64 disagrees (must stay 0)
+    uint32_t flops64 = 0;     // float64 operations of the evaluated atoms and frames
+    uint32_t flops32 = 0;     // float32 operations of the guarded atoms
 };
+
+// Algorithmic operation counts (add, sub, mul, div, sqrt and compare each
+// count one) of one atom evaluation, from the sequences in atom64 / atom32
+// below.  A collider costs its bounds test and projection plus a fixed
+// Moller-Trumbore sequence per candidate triangle of the point's cell.
+__device__ __forceinline__ uint32_t atom_flops64(uint32_t kind) {
+    switch (kind) {
+    case A_SPHERE: return 9;
+    case A_RECT: return 6;
+    case A_CAPSULE: return 25;
+    case A_CYLINDER: return 25;
+    case A_CONE: return 29;
+    case A_CONESLICE: return 32;
+    case A_TORUS: return 41;
+    case A_COLLIDER: return 20;
+    default: return 0;
+    }
+}
+constexpr uint32_t COLLIDER_FLOPS_PER_CANDIDATE = 37;
+__device__ __forceinline__ uint32_t atom_flops32(uint32_t kind) {
+    switch (kind) {
+    case A_SPHERE: return 10;
+    case A_CAPSULE: return 29;
+    case A_CYLINDER: return 29;
+    default: return 0;
+    }
+}
 
 __device__ __forceinline__ void apply_frame(const double *__restrict__ f64, Frame fr, double &x, double &y,
                                             double &z) {
@@ -247,15 +278,20 @@ template <int MODE, bool STATS, bool HOST>
 __device__ __forceinline__ bool eval_atom(const DevProg &P, uint32_t kind, const double *f64, const float *f32,
                                           double qx, double qy, double qz, float fx, float fy, float fz, bool live,
                                           uint64_t pid, GStats &gs) {
-    if (kind == A_COLLIDER) return collider_contains(P.colliders[(int)f64[0]], f64, qx, qy, qz);
+    if (kind == A_COLLIDER) {
+        uint32_t cand = 0;
+        const bool b = collider_contains(P.colliders[(int)f64[0]], f64, qx, qy, qz, STATS ? &cand : nullptr);
+        if (STATS && live) gs.flops64 += atom_flops64(kind) + COLLIDER_FLOPS_PER_CANDIDATE * cand;
+        return b;
+    }
     if (kind == A_HOST) return HOST && P.host_vals[(uint64_t)f64[0] * P.host_stride + pid] != 0;
     if (MODE == 1 && has_atom32(kind)) {
         bool plain;
         const int r = atom32(kind, f32, fx, fy, fz, &plain);
-        if (STATS && live) gs.evals++;
+        if (STATS && live) gs.evals++, gs.flops32 += atom_flops32(kind);
         if (r < 0) {
             const bool b = atom64(kind, f64, qx, qy, qz);
-            if (STATS && live) gs.rechecks++, gs.mismatches += plain != b;
+            if (STATS && live) gs.rechecks++, gs.mismatches += plain != b, gs.flops64 += atom_flops64(kind);
             return b;
         }
         if (STATS && live) {
@@ -265,7 +301,18 @@ __device__ __forceinline__ bool eval_atom(const DevProg &P, uint32_t kind, const
         }
         return r != 0;
     }
+    if (STATS && live) gs.flops64 += atom_flops64(kind);
     return atom64(kind, f64, qx, qy, qz);
+}
+
+// Operation count of one frame (3 per translate / scale, 15 per matrix).
+__device__ __forceinline__ uint32_t frame_flops(const double *__restrict__ f64, Frame fr) {
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < fr.nops; ++i) {
+        const int k = (int)f64[fr.offset + 10 * i];
+        n += (k == MC3D_XF_TRANSLATE || k == MC3D_XF_SCALE || k == MC3D_XF_VECSCALE) ? 3 : 15;
+    }
+    return n;
 }
 
 // Postfix boolean program over atoms for NP points at once (one walk of the
@@ -300,6 +347,10 @@ __device__ __forceinline__ void eval_code_n(const Insn *code, uint32_t n, const 
                     qx[k] = px[k], qy[k] = py[k], qz[k] = pz[k];
                     apply_frame(P.f64, f, qx[k], qy[k], qz[k]);
                     if (MODE == 1) fx[k] = (float)qx[k], fy[k] = (float)qy[k], fz[k] = (float)qz[k];
+                }
+                if (STATS) {
+                    const uint32_t ff = frame_flops(P.f64, f);
+                    for (int k = 0; k < NP; k++) gs.flops64 += live[k] ? ff : 0;
                 }
                 cur = fr;
             }

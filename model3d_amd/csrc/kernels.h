@@ -21,6 +21,13 @@
 //   bisect    per vertex, mcSearchPoint (mc.go:223-258): the first probe is
 //             the edge's high corner, read from the occupancy grid (same
 //             point, same answer), then `iters` float64 midpoints.
+//
+// One launch of each stage serves every solid of a call: a call is a list of
+// segments (one per solid, a single MarchingCubesSearch is a list of one),
+// laid end to end in the occupancy words (ranges aligned to 64 words), the
+// bricks (aligned to 128) and the outputs (by the scan).  A wave finds its
+// segment from a table per 64 words / 128 bricks, so it is wave-uniform and
+// the segment's program, grid and brick shape are scalar loads.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -41,6 +48,10 @@ struct Ctl {
     uint32_t pad;
     unsigned long long totals[4];  // scan totals: vertices, triangles, listed words
     unsigned long long gstats[4];  // guard statistics (GStats order)
+    unsigned long long flops[4];   // statistics runs: float64, float32 ops of corners; the same of bisect
+    unsigned long long code_total; // instructions over all active bricks' folded programs
+    uint32_t code_max;             // longest folded program
+    uint32_t full_bricks;          // bricks whose folded program overflowed its slot
 };
 
 struct Grid {
@@ -58,6 +69,47 @@ struct Bricks {
     int BY, BZ, nby, nbz;
 };
 
+// One solid of a call.  B.info / B.code are the call's arrays, indexed by the
+// global brick number; word and brick numbers are local to the segment
+// unless named global.  Kernels also take the first segment by value (a
+// call of one solid reads it from the kernel arguments, no load).
+struct Seg {
+    DevProg P;
+    Grid g;
+    Bricks B;
+    uint64_t word0, nwords;    // global occupancy words [word0, word0 + nwords); word0 % 64 == 0
+    uint64_t brick0, nbricks;  // global bricks; brick0 % 128 == 0
+    int32_t iters;
+    int32_t stage;  // the plan may stage this segment's program in LDS
+};
+
+// Per-solid results, copied back with the control block.
+struct SegOut {
+    unsigned long long vbase, tbase;  // first vertex / triangle of the solid in the call's outputs
+    uint32_t inside;                  // panic("solid is true outside of bounds") for this solid
+    uint32_t pad[3];
+};
+
+constexpr int SEG_WORD_SHIFT = 6;   // segment table entry per 64 words
+constexpr int SEG_BRICK_SHIFT = 7;  // and per 128 bricks
+
+// The segment of a wave's items (all its items share one table entry).
+// MULTI: the call has several segments; otherwise segment 0, whose copy in
+// the kernel arguments is used (no load at all).
+template <bool MULTI>
+__device__ __forceinline__ uint32_t wave_seg(const uint32_t *__restrict__ table, uint64_t item, int shift) {
+    if (!MULTI) return 0;
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)table[item >> shift]);
+}
+
+// Segment si by value (scalar registers): loaded for a call of several,
+// the kernel-argument copy for a call of one.
+template <bool MULTI>
+__device__ __forceinline__ Seg pick_seg(const Seg *__restrict__ segs, const Seg &seg0, uint32_t si) {
+    if (MULTI) return segs[si];
+    return seg0;
+}
+
 __device__ __forceinline__ size_t word_index(const Grid &g, int wx, int y, int z) {
     return ((size_t)z * g.ny + y) * g.W + wx;
 }
@@ -74,17 +126,21 @@ __device__ __forceinline__ void word_coords(const Grid &g, uint64_t w, int &wx, 
 }
 
 template <bool STATS>
-__device__ __forceinline__ void flush_stats(const GStats &gs, unsigned long long *__restrict__ out) {
+__device__ __forceinline__ void flush_stats(const GStats &gs, unsigned long long *__restrict__ out,
+                                            unsigned long long *__restrict__ flops) {
     if (!STATS) return;
-    __shared__ unsigned int red[4];
-    if (threadIdx.x < 4) red[threadIdx.x] = 0;
+    __shared__ unsigned long long red[6];
+    if (threadIdx.x < 6) red[threadIdx.x] = 0;
     __syncthreads();
-    if (gs.evals) atomicAdd(&red[0], gs.evals);
-    if (gs.rechecks) atomicAdd(&red[1], gs.rechecks);
-    if (gs.mismatches) atomicAdd(&red[2], gs.mismatches);
-    if (gs.violations) atomicAdd(&red[3], gs.violations);
+    if (gs.evals) atomicAdd(&red[0], (unsigned long long)gs.evals);
+    if (gs.rechecks) atomicAdd(&red[1], (unsigned long long)gs.rechecks);
+    if (gs.mismatches) atomicAdd(&red[2], (unsigned long long)gs.mismatches);
+    if (gs.violations) atomicAdd(&red[3], (unsigned long long)gs.violations);
+    if (gs.flops64) atomicAdd(&red[4], (unsigned long long)gs.flops64);
+    if (gs.flops32) atomicAdd(&red[5], (unsigned long long)gs.flops32);
     __syncthreads();
-    if (threadIdx.x < 4 && red[threadIdx.x]) atomicAdd(&out[threadIdx.x], (unsigned long long)red[threadIdx.x]);
+    if (threadIdx.x < 4 && red[threadIdx.x]) atomicAdd(&out[threadIdx.x], red[threadIdx.x]);
+    if (threadIdx.x >= 4 && threadIdx.x < 6 && red[threadIdx.x]) atomicAdd(&flops[threadIdx.x - 4], red[threadIdx.x]);
 }
 
 // Appends `item` for the lanes with `pred` set, one atomic per wave.  The
@@ -151,26 +207,39 @@ constexpr size_t PLAN_STACK_BYTES = PLAN_BLOCK * PLAN_STACK * sizeof(uint32_t);
 // folded program into the brick's slot of `cap` instructions.  A brick whose
 // folded code ever needs more than `cap` slots uses the unfolded program.
 // With `stage`, the program and its atom boxes are first copied to LDS.
-__global__ __launch_bounds__(PLAN_BLOCK) void k_plan(DevProg P, Grid g, Bricks B, uint64_t nbricks, uint32_t cap,
-                                                     uint2 *__restrict__ info, Insn *__restrict__ out,
-                                                     uint32_t *__restrict__ act_list, Ctl *__restrict__ ctl,
-                                                     int stage) {
+template <bool MULTI>
+__global__ __launch_bounds__(PLAN_BLOCK) void k_plan(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
+                                                     const uint32_t *__restrict__ brick_seg, uint64_t nbricks_all,
+                                                     uint32_t cap, uint2 *__restrict__ info, Insn *__restrict__ out,
+                                                     uint32_t *__restrict__ act_list, Ctl *__restrict__ ctl) {
     extern __shared__ __align__(16) char plan_smem[];
     uint32_t *S = (uint32_t *)plan_smem + threadIdx.x * PLAN_STACK;
-    const Insn *code = P.code;
-    const AtomBox *boxes = P.boxes;
-    if (stage) {
-        Insn *sc = (Insn *)(plan_smem + PLAN_STACK_BYTES);
-        AtomBox *sb = (AtomBox *)(sc + P.ncode);
-        for (uint32_t i = threadIdx.x; i < P.ncode; i += blockDim.x) sc[i] = P.code[i], sb[i] = P.boxes[i];
-        __syncthreads();
-        code = sc, boxes = sb;
-    }
+    Insn *sc = (Insn *)(plan_smem + PLAN_STACK_BYTES);
+    uint32_t staged = ~0u;  // segment whose program is in LDS
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t b_end = (nbricks + blockDim.x - 1) / blockDim.x * blockDim.x;  // whole waves take part
-    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < b_end; b += stride) {
+    const uint64_t b_end = (nbricks_all + blockDim.x - 1) / blockDim.x * blockDim.x;  // whole waves take part
+    for (uint64_t gb = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gb < b_end; gb += stride) {
+        // A block's 128 bricks share one segment (ranges aligned to 128).
+        const uint32_t si = wave_seg<MULTI>(brick_seg, gb - threadIdx.x, SEG_BRICK_SHIFT);
+        const Seg sg = pick_seg<MULTI>(segs, seg0, si);
+        const DevProg &P = sg.P;
+        const Grid &g = sg.g;
+        const Bricks &B = sg.B;
+        const Insn *code = P.code;
+        const AtomBox *boxes = P.boxes;
+        if (sg.stage) {  // block-uniform
+            AtomBox *sb = (AtomBox *)(sc + P.ncode);
+            if (staged != si) {
+                __syncthreads();  // the previous segment's walks are done with the LDS copy
+                for (uint32_t i = threadIdx.x; i < P.ncode; i += blockDim.x) sc[i] = P.code[i], sb[i] = P.boxes[i];
+                __syncthreads();
+                staged = si;
+            }
+            code = sc, boxes = sb;
+        }
+        const uint64_t b = gb - sg.brick0;  // local brick
         bool active = false;
-        if (b < nbricks) {
+        if (gb < nbricks_all && b < sg.nbricks) {
             const uint32_t r = (uint32_t)b / (uint32_t)g.W;
             const int bx = (int)((uint32_t)b - r * (uint32_t)g.W);
             const int bz = (int)(r / (uint32_t)B.nby), by = (int)(r - (uint32_t)bz * (uint32_t)B.nby);
@@ -178,7 +247,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(DevProg P, Grid g, Bricks B
             const int x1 = min(x0 + 64, g.nx - 1), y1 = min(y0 + B.BY, g.ny - 1), z1 = min(z0 + B.BZ, g.nz - 1);
             const double lo[3] = {g.xs[x0], g.ys[y0], g.zs[z0]};
             const double hi[3] = {g.xs[x1], g.ys[y1], g.zs[z1]};
-            Insn *o = out + b * cap;
+            Insn *o = out + gb * cap;
             uint32_t n = 0;
             bool over = false;
             int sp = 0;
@@ -248,10 +317,15 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(DevProg P, Grid g, Bricks B
             if (root == TAG_FALSE) bi = make_uint2(0, 0);
             else if (root == TAG_TRUE) bi = make_uint2(0, BRICK_TRUE);
             else if (over) bi = make_uint2(BRICK_FULL, P.ncode), active = true;
-            else bi = make_uint2((uint32_t)(b * cap), n), active = true;
-            info[b] = bi;
+            else bi = make_uint2((uint32_t)(gb * cap), n), active = true;
+            info[gb] = bi;
+            if (active) {
+                atomicAdd(&ctl->code_total, (unsigned long long)bi.y);
+                atomicMax(&ctl->code_max, bi.y);
+                if (over) atomicAdd(&ctl->full_bricks, 1u);
+            }
         }
-        wave_append(active, (uint32_t)b, act_list, &ctl->nact_bricks);
+        wave_append(active, (uint32_t)gb, act_list, &ctl->nact_bricks);
     }
 }
 
@@ -259,12 +333,23 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(DevProg P, Grid g, Bricks B
 // corners
 // ---------------------------------------------------------------------------
 // Without culling: one wave per occupancy word.
-template <int MODE, bool STATS>
-__global__ __launch_bounds__(256) void k_corners(DevProg P, Grid g, uint64_t *__restrict__ bits,
-                                                 Ctl *__restrict__ ctl, uint64_t nwords) {
+template <int MODE, bool STATS, bool MULTI>
+__global__ __launch_bounds__(256) void k_corners(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
+                                                 const uint32_t *__restrict__ word_seg, uint64_t *__restrict__ bits,
+                                                 Ctl *__restrict__ ctl, SegOut *__restrict__ sout,
+                                                 uint64_t nwords_all) {
     const int lane = threadIdx.x & 63;
     GStats gs;
-    for (uint64_t w = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < nwords; w += (uint64_t)gridDim.x * 4) {
+    for (uint64_t gw = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); gw < nwords_all; gw += (uint64_t)gridDim.x * 4) {
+        const uint32_t si = wave_seg<MULTI>(word_seg, gw, SEG_WORD_SHIFT);
+        const Seg sg = pick_seg<MULTI>(segs, seg0, si);
+        const DevProg &P = sg.P;
+        const Grid &g = sg.g;
+        const uint64_t w = gw - sg.word0;
+        if (w >= sg.nwords) {  // alignment padding
+            if (lane == 0) bits[gw] = 0;
+            continue;
+        }
         int wx, y, z;
         word_coords(g, w, wx, y, z);
         const int x = wx * 64 + lane;
@@ -273,31 +358,43 @@ __global__ __launch_bounds__(256) void k_corners(DevProg P, Grid g, uint64_t *__
         const uint64_t pid = (uint64_t)x + (uint64_t)g.nx * ((uint64_t)y + (uint64_t)g.ny * (uint64_t)z);
         if (x < g.nx) inside = eval_code<MODE, STATS, true, true>(P.code, P.ncode, P, g.xs[x], g.ys[y], g.zs[z], gs, pid);
         const uint64_t m = __ballot(inside);
-        if (lane == 0) bits[w] = m;
+        if (lane == 0) bits[gw] = m;
         // panic("solid is true outside of bounds"), mc.go:687-689
         if (inside && (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1))
-            atomicOr(&ctl->inside, 1u);
+            atomicOr(&ctl->inside, 1u), atomicOr(&sout[si].inside, 1u);
     }
-    flush_stats<STATS>(gs, ctl->gstats);
+    flush_stats<STATS>(gs, ctl->gstats, ctl->flops);
 }
 
 // Culled, part 1: words of constant bricks are written directly (all false,
 // or every valid corner true).
-__global__ __launch_bounds__(256) void k_fill_const(Bricks B, Grid g, uint64_t *__restrict__ bits,
-                                                    Ctl *__restrict__ ctl, uint64_t nwords) {
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
-         w += (uint64_t)gridDim.x * blockDim.x) {
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_fill_const(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
+                                                    const uint32_t *__restrict__ word_seg,
+                                                    uint64_t *__restrict__ bits, Ctl *__restrict__ ctl,
+                                                    SegOut *__restrict__ sout, uint64_t nwords_all) {
+    for (uint64_t gw = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gw < nwords_all;
+         gw += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t si = wave_seg<MULTI>(word_seg, gw, SEG_WORD_SHIFT);
+        const Seg sg = pick_seg<MULTI>(segs, seg0, si);
+        const Grid &g = sg.g;
+        const Bricks &B = sg.B;
+        const uint64_t w = gw - sg.word0;
+        if (w >= sg.nwords) {  // alignment padding
+            bits[gw] = 0;
+            continue;
+        }
         int wx, y, z;
         word_coords(g, w, wx, y, z);
-        const uint32_t len = B.info[((size_t)(z / B.BZ) * B.nby + (y / B.BY)) * g.W + wx].y;
+        const uint32_t len = B.info[sg.brick0 + ((size_t)(z / B.BZ) * B.nby + (y / B.BY)) * g.W + wx].y;
         if (len == 0) {
-            bits[w] = 0;
+            bits[gw] = 0;
         } else if (len == BRICK_TRUE) {
             const int nvalid = min(64, g.nx - wx * 64);
-            bits[w] = nvalid == 64 ? ~0ull : ((1ull << nvalid) - 1);
+            bits[gw] = nvalid == 64 ? ~0ull : ((1ull << nvalid) - 1);
             // panic("solid is true outside of bounds"), mc.go:687-689
             if (y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1 || wx == 0 || wx * 64 + nvalid == g.nx)
-                atomicOr(&ctl->inside, 1u);
+                atomicOr(&ctl->inside, 1u), atomicOr(&sout[si].inside, 1u);
         }
     }
 }
@@ -306,28 +403,39 @@ __device__ __forceinline__ const Insn *brick_code(const DevProg &P, const Bricks
     return bi.x == BRICK_FULL ? P.code : B.code + bi.x;
 }
 
-// Culled, part 2: one workgroup per active brick.  The brick's folded
-// program and the params of its atoms are first staged in LDS (one parallel
-// copy instead of a chain of dependent scalar loads during the walk), then
-// the 4 waves take the brick's rows, NP rows per walk of the program.
-constexpr uint32_t STAGE_INSNS = 64;
+// Culled, part 2: WPB waves per active brick.  The brick's folded program
+// and the params of its atoms are first staged in LDS (one parallel copy, a
+// thread per param, instead of a chain of dependent scalar loads during the
+// walk), then the waves take the brick's rows, NP rows per walk of the
+// program.  Longer programs are walked from global memory.
+constexpr uint32_t STAGE_INSNS = 32;
 
-template <int MODE, bool STATS, int NP>
-__global__ __launch_bounds__(256) void k_corners_act(DevProg P, Bricks B, Grid g, const uint32_t *__restrict__ act,
-                                                     uint64_t *__restrict__ bits, Ctl *__restrict__ ctl) {
+template <int MODE, bool STATS, int NP, int WPB, bool MULTI>
+__global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
+                                                          const uint32_t *__restrict__ brick_seg,
+                                                          const uint32_t *__restrict__ act,
+                                                          uint64_t *__restrict__ bits_all, Ctl *__restrict__ ctl,
+                                                          SegOut *__restrict__ sout) {
     __shared__ Insn s_code[STAGE_INSNS];
     __shared__ double s_f64[STAGE_INSNS * MAX_ATOM_PARAMS];
     __shared__ float s_f32[STAGE_INSNS * MAX_ATOM_PARAMS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t nact = ctl->nact_bricks;
-    const int nrows = B.BY * B.BZ;
     GStats gs;
     for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x) {
-        const uint32_t b = act[k];
+        const uint32_t gb = act[k];
+        const uint32_t si = wave_seg<MULTI>(brick_seg, gb, SEG_BRICK_SHIFT);
+        const Seg sg = pick_seg<MULTI>(segs, seg0, si);
+        const DevProg &P = sg.P;
+        const Grid &g = sg.g;
+        const Bricks &B = sg.B;
+        const int nrows = B.BY * B.BZ;
+        uint64_t *bits = bits_all + sg.word0;
+        const uint32_t b = gb - (uint32_t)sg.brick0;
         const uint32_t r = b / (uint32_t)g.W;
         const int bx = (int)(b - r * (uint32_t)g.W);
         const int bz = (int)(r / (uint32_t)B.nby), by = (int)(r - (uint32_t)bz * (uint32_t)B.nby);
-        const uint2 bi = B.info[b];
+        const uint2 bi = B.info[gb];
         const bool staged = bi.x != BRICK_FULL && bi.y <= STAGE_INSNS;  // block-uniform
         __syncthreads();  // the previous brick's walks are done with the LDS copy
         if (staged) {
@@ -352,12 +460,12 @@ __global__ __launch_bounds__(256) void k_corners_act(DevProg P, Bricks B, Grid g
         const int x = bx * 64 + lane;
         const bool xin = x < g.nx;
         const double px0 = xin ? g.xs[x] : 0.0;
-        for (int j0 = wave; j0 < nrows; j0 += 4 * NP) {
+        for (int j0 = wave; j0 < nrows; j0 += WPB * NP) {
             double px[NP], py[NP], pz[NP];
             bool ok[NP], live[NP], res[NP];
             int yy[NP], zz[NP];
             for (int q = 0; q < NP; q++) {
-                const int j = j0 + 4 * q;
+                const int j = j0 + WPB * q;
                 yy[q] = by * B.BY + j % B.BY;
                 zz[q] = bz * B.BZ + j / B.BY;
                 ok[q] = j < nrows && yy[q] < g.ny && zz[q] < g.nz;  // wave-uniform
@@ -375,11 +483,11 @@ __global__ __launch_bounds__(256) void k_corners_act(DevProg P, Bricks B, Grid g
                 if (lane == 0) bits[word_index(g, bx, y, z)] = m;
                 // panic("solid is true outside of bounds"), mc.go:687-689
                 if (inside && (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1))
-                    atomicOr(&ctl->inside, 1u);
+                    atomicOr(&ctl->inside, 1u), atomicOr(&sout[si].inside, 1u);
             }
         }
     }
-    flush_stats<STATS>(gs, ctl->gstats);
+    flush_stats<STATS>(gs, ctl->gstats, ctl->flops);
 }
 
 // ---------------------------------------------------------------------------
@@ -441,14 +549,26 @@ __device__ __forceinline__ int cube_case(const CubeCorners &cc, int i) {
 }
 
 // counts[w] = {crossed edges, triangles, 1 if the word emits anything, 0}.
-__global__ __launch_bounds__(256) void k_classify(Grid g, const uint64_t *__restrict__ bits,
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_classify(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
+                                                  const uint32_t *__restrict__ word_seg,
+                                                  const uint64_t *__restrict__ bits_all,
                                                   const uint8_t *__restrict__ table, uint4 *__restrict__ counts,
-                                                  uint64_t nwords) {
+                                                  uint64_t nwords_all) {
     __shared__ uint8_t ntri[256];
     ntri[threadIdx.x] = table[16 * threadIdx.x];
     __syncthreads();
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
-         w += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint64_t gw = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gw < nwords_all;
+         gw += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t si = wave_seg<MULTI>(word_seg, gw, SEG_WORD_SHIFT);
+        const Seg sg = pick_seg<MULTI>(segs, seg0, si);
+        const Grid &g = sg.g;
+        const uint64_t *bits = bits_all + sg.word0;
+        const uint64_t w = gw - sg.word0;
+        if (w >= sg.nwords) {  // alignment padding
+            counts[gw] = make_uint4(0, 0, 0, 0);
+            continue;
+        }
         int wx, y, z;
         word_coords(g, w, wx, y, z);
         const Masks m = word_masks(g, bits, wx, y, z);
@@ -461,7 +581,7 @@ __global__ __launch_bounds__(256) void k_classify(Grid g, const uint64_t *__rest
             mixed &= mixed - 1;
             nt += ntri[cube_case(cc, i)];
         }
-        counts[w] = make_uint4(nv, nt, (nv | nt) != 0 ? 1u : 0u, 0u);
+        counts[gw] = make_uint4(nv, nt, (nv | nt) != 0 ? 1u : 0u, 0u);
     }
 }
 
@@ -563,11 +683,14 @@ struct WordInfo {
 // number the vertex on any edge of its cube.  Writes at or beyond the
 // capacities are dropped and flagged (the host grows the buffers and runs
 // this stage again).
-__global__ __launch_bounds__(256) void k_emit(Grid g, const uint64_t *__restrict__ bits,
-                                              const uint8_t *__restrict__ table, const uint4 *__restrict__ offs,
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
+                                              const uint32_t *__restrict__ word_seg,
+                                              const uint64_t *__restrict__ bits_all,
+                                              const uint8_t *__restrict__ table, const uint4 *__restrict__ offs_all,
                                               const uint32_t *__restrict__ active, Ctl *__restrict__ ctl,
-                                              uint64_t *__restrict__ vkeys, uint32_t *__restrict__ tris,
-                                              uint64_t vcap, uint64_t tcap) {
+                                              uint64_t *__restrict__ vkeys, uint32_t *__restrict__ vseg,
+                                              uint32_t *__restrict__ tris, uint64_t vcap, uint64_t tcap) {
     __shared__ uint8_t tab[256 * 16];
     __shared__ WordInfo wi_all[4][8];
     __shared__ uint32_t s_pre[4][64];
@@ -580,7 +703,14 @@ __global__ __launch_bounds__(256) void k_emit(Grid g, const uint64_t *__restrict
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
     bool over = false;
     for (uint32_t k = blockIdx.x * 4 + wave; k < nact; k += gridDim.x * 4) {
-        const uint64_t w = active[k];
+        const uint64_t gw = active[k];
+        const uint32_t si = wave_seg<MULTI>(word_seg, gw, SEG_WORD_SHIFT);
+        const Seg sg = pick_seg<MULTI>(segs, seg0, si);
+        const Grid &g = sg.g;
+        const uint64_t *bits = bits_all + sg.word0;
+        const uint4 *offs = offs_all + sg.word0;
+        const uint32_t vbase = offs[0].x;  // the solid's first vertex (triangles index from it)
+        const uint64_t w = gw - sg.word0;
         int wx, y, z;
         word_coords(g, w, wx, y, z);
         wave_sync();
@@ -603,20 +733,32 @@ __global__ __launch_bounds__(256) void k_emit(Grid g, const uint64_t *__restrict
             uint64_t v = me.base;
             const uint64_t k0 = v + __popcll(me.cx & below);
             if ((me.cx >> lane) & 1) {
-                if (k0 < vcap) vkeys[k0] = vkey(0, x, y, z);
-                else over = true;
+                if (k0 < vcap) {
+                    vkeys[k0] = vkey(0, x, y, z);
+                    if (vseg) vseg[k0] = si;
+                } else {
+                    over = true;
+                }
             }
             v += __popcll(me.cx);
             const uint64_t k1 = v + __popcll(me.cy & below);
             if ((me.cy >> lane) & 1) {
-                if (k1 < vcap) vkeys[k1] = vkey(1, x, y, z);
-                else over = true;
+                if (k1 < vcap) {
+                    vkeys[k1] = vkey(1, x, y, z);
+                    if (vseg) vseg[k1] = si;
+                } else {
+                    over = true;
+                }
             }
             v += __popcll(me.cy);
             const uint64_t k2 = v + __popcll(me.cz & below);
             if ((me.cz >> lane) & 1) {
-                if (k2 < vcap) vkeys[k2] = vkey(2, x, y, z);
-                else over = true;
+                if (k2 < vcap) {
+                    vkeys[k2] = vkey(2, x, y, z);
+                    if (vseg) vseg[k2] = si;
+                } else {
+                    over = true;
+                }
             }
         }
         // triangles of this lane's cube, placed by a ballot prefix sum
@@ -663,10 +805,20 @@ __global__ __launch_bounds__(256) void k_emit(Grid g, const uint64_t *__restrict
             if (axis == 0) idx += __popcll(o.cx & bl);
             else if (axis == 1) idx += __popcll(o.cx) + __popcll(o.cy & bl);
             else idx += __popcll(o.cx) + __popcll(o.cy) + __popcll(o.cz & bl);
-            tris[3 * t0 + j] = idx;
+            tris[3 * t0 + j] = idx - vbase;
         }
     }
     if (over) atomicOr(&ctl->overflow, 1u);
+}
+
+// Each solid's first vertex and triangle in the call's outputs.
+__global__ __launch_bounds__(256) void k_seg_ranges(const Seg *__restrict__ segs, uint32_t nsegs,
+                                                    const uint4 *__restrict__ offs, SegOut *__restrict__ sout) {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nsegs; s += gridDim.x * blockDim.x) {
+        const uint4 o = offs[segs[s].word0];
+        sout[s].vbase = o.x;
+        sout[s].tbase = o.y;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -677,11 +829,11 @@ __device__ __forceinline__ bool corner_bit(const Grid &g, const uint64_t *__rest
 }
 
 template <int MODE, bool STATS>
-__device__ __forceinline__ bool eval_brick(const DevProg &P, const Bricks &B, const Grid &g, int bx, int y, int z,
-                                           double px, double py, double pz, GStats &gs) {
+__device__ __forceinline__ bool eval_brick(const DevProg &P, const Bricks &B, const Grid &g, uint64_t brick0, int bx,
+                                           int y, int z, double px, double py, double pz, GStats &gs) {
     // Lanes of one wave may sit in different bricks: no readfirstlane here.
     if (!B.info) return eval_code<MODE, STATS, false>(P.code, P.ncode, P, px, py, pz, gs);
-    const uint2 bi = B.info[((size_t)(z / B.BZ) * B.nby + (y / B.BY)) * g.W + bx];
+    const uint2 bi = B.info[brick0 + ((size_t)(z / B.BZ) * B.nby + (y / B.BY)) * g.W + bx];
     if (bi.y == 0) return false;
     if (bi.y == BRICK_TRUE) return true;
     return eval_code<MODE, STATS, false>(brick_code(P, B, bi), bi.y, P, px, py, pz, gs);
@@ -692,13 +844,15 @@ __device__ __forceinline__ bool eval_brick(const DevProg &P, const Bricks &B, co
 // inside the brick of its low corner (bricks include their +1 corners), so
 // that brick's program decides every midpoint.
 template <int MODE, bool STATS>
-__global__ __launch_bounds__(256) void k_bisect(DevProg P, Bricks B, Grid g, const uint64_t *__restrict__ bits,
-                                                const uint64_t *__restrict__ vkeys, double *__restrict__ verts,
-                                                Ctl *__restrict__ ctl, uint64_t vcap, int iters) {
-    GStats gs;
-    const uint64_t nv = min((uint64_t)ctl->totals[0], vcap);
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
-         v += (uint64_t)gridDim.x * blockDim.x) {
+__device__ __forceinline__ void bisect_vertex(const Seg &sg, const uint64_t *__restrict__ bits_all,
+                                              const uint64_t *__restrict__ vkeys, double *__restrict__ verts,
+                                              uint64_t v, GStats &gs) {
+    const DevProg &P = sg.P;
+    const Grid &g = sg.g;
+    const Bricks &B = sg.B;
+    const uint64_t *bits = bits_all + sg.word0;
+    const int iters = sg.iters;
+    {
         const uint64_t key = vkeys[v];
         const int axis = (int)(key & 3);
         const int x = (int)((key >> 2) & 0xfffff), y = (int)((key >> 22) & 0xfffff), z = (int)((key >> 42) & 0xfffff);
@@ -717,17 +871,39 @@ __global__ __launch_bounds__(256) void k_bisect(DevProg P, Bricks B, Grid g, con
             for (int i = 0; i < iters; i++) {
                 const double mid = (fp + tp) / 2;
                 p[axis] = mid;
-                if (eval_brick<MODE, STATS>(P, B, g, x >> 6, y, z, p[0], p[1], p[2], gs)) tp = mid;
+                if (eval_brick<MODE, STATS>(P, B, g, sg.brick0, x >> 6, y, z, p[0], p[1], p[2], gs)) tp = mid;
                 else fp = mid;
             }
             res = (fp + tp) / 2;
+            if (STATS) gs.flops64 += 2 * (iters + 1);
         }
         p[axis] = res;
         verts[3 * v] = p[0];
         verts[3 * v + 1] = p[1];
         verts[3 * v + 2] = p[2];
     }
-    flush_stats<STATS>(gs, ctl->gstats);
+}
+
+template <int MODE, bool STATS, bool MULTI>
+__global__ __launch_bounds__(256) void k_bisect(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
+                                                const uint32_t *__restrict__ vseg,
+                                                const uint64_t *__restrict__ bits_all,
+                                                const uint64_t *__restrict__ vkeys, double *__restrict__ verts,
+                                                Ctl *__restrict__ ctl, uint64_t vcap) {
+    GStats gs;
+    const uint64_t nv = min((uint64_t)ctl->totals[0], vcap);
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        // Lanes may belong to different solids: take them one solid at a time.
+        const uint32_t mine = MULTI ? vseg[v] : 0u;
+        for (;;) {
+            const uint32_t si = (uint32_t)__builtin_amdgcn_readfirstlane((int)mine);
+            if (mine != si) continue;
+            bisect_vertex<MODE, STATS>(pick_seg<MULTI>(segs, seg0, si), bits_all, vkeys, verts, v, gs);
+            break;
+        }
+    }
+    flush_stats<STATS>(gs, ctl->gstats, ctl->flops + 2);
 }
 
 // Solid.Contains at arbitrary points (the full program, no bricks).

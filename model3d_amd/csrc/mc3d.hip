@@ -2,11 +2,13 @@
 // entry points for meshing (include/mc3d.h).  The device stages live in
 // kernels.h, Contains in eval.h, the solid tree and its lowering in solid.cpp.
 //
-// One call = one pinned upload of the lowered program and grid axes, the
-// stage kernels on the context's stream, and one synchronisation at the end;
-// every count a stage needs is read on the device.  Output buffers are
-// pooled per context and sized from earlier calls; a call whose mesh does
-// not fit grows them and re-runs the two output stages.
+// A call meshes a list of solids (one for mc3d_marching_cubes_search, many
+// for mc3d_marching_cubes_search_batch): one pinned upload of every lowered
+// program, the grid axes and a zeroed control block, then one launch per
+// stage for all solids on the context's stream and one synchronisation at
+// the end; every count a stage needs is read on the device.  Output buffers
+// are pooled per context and sized from earlier calls; a call whose meshes
+// do not fit grows them and re-runs the two output stages.
 //
 // Every float64 expression keeps the reference's operation order and is
 // compiled with -ffp-contract=off, so Contains decisions, topology and vertex
@@ -18,10 +20,12 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <memory>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kernels.h"
@@ -123,6 +127,19 @@ unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 20) {
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// The device outputs of one call (vertices, triangles), shared by the meshes
+// it returned and given back to the context's pool with the last of them.
+struct MeshBuffers {
+    std::shared_ptr<Pool> pool;
+    std::unique_ptr<DevBuf> verts, tris;
+    ~MeshBuffers() {
+        if (pool) {
+            pool->give(std::move(verts));
+            pool->give(std::move(tris));
+        }
+    }
+};
+
 }  // namespace
 
 // A collider's ray grid on the device, kept per context (keyed by the
@@ -137,40 +154,121 @@ struct mc3d_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<std::unique_ptr<ColliderCache>> colliders;  // most recent last
-    HostBuf staging, ctl_host;
-    DevBuf blob, table, bits, counts, offs, tiles, actw, actb, binfo, bcode, vkeys, pts, out8;
+    HostBuf staging, ctl_host, segstage, outstage;
+    DevBuf blob, table, bits, counts, offs, tiles, actw, actb, binfo, bcode, vkeys, vseg, pts, out8;
     DevBuf hvals, hpts, hstate, hprobe;  // host-evaluated atoms: answers, points, bisection state
     HostBuf hstage;                      // pinned points / answers of one callback batch
     std::shared_ptr<Pool> pool = std::make_shared<Pool>();
     uint64_t v_hint = 1u << 18, t_hint = 1u << 19;  // output capacities to start from
-    hipEvent_t ev[8] = {};
-    bool table_loaded = false;
+    hipEvent_t ev[10] = {};
 };
 
+// One solid's mesh: a slice [voff, voff + nv) / [toff, toff + nt) of its
+// call's outputs (triangle indices count from the slice's first vertex).
 struct mc3d_mesh {
     int device = 0;
-    uint64_t nv = 0, nt = 0;
+    uint64_t nv = 0, nt = 0, voff = 0, toff = 0;
     std::vector<double> hverts;
     std::vector<uint32_t> htris;
     bool on_host = false;
-    std::shared_ptr<Pool> pool;
-    std::unique_ptr<DevBuf> dverts, dtris;  // device results owned by the mesh
+    std::shared_ptr<MeshBuffers> buf;
     mc3d_stats stats{};
-    ~mc3d_mesh() {
-        if (pool) {
-            pool->give(std::move(dverts));
-            pool->give(std::move(dtris));
-        }
-    }
 };
 
 namespace {
 
-struct Upload {
-    DevProg dp;
-    Ctl *ctl;
-    double *axes;
+// One solid of a call, prepared on the host.
+struct Job {
+    const mc3d_solid *s = nullptr;
+    double delta = 0;
+    int iters = 0;
+    Program P;
+    std::vector<double> ax[3];
+    Grid g{};    // sizes; axis pointers set by the upload
+    Bricks B{};  // shape; array pointers set by the upload
+    uint64_t nwords = 0, nbricks = 0, word0 = 0, brick0 = 0;
+    uint32_t cap = 0;  // folded-program slot of its culling bricks (0: no culling)
+    bool host = false;
+    std::vector<uint32_t> frames;  // frame of each host-evaluated atom
 };
+
+// Frame of each host-evaluated atom, by host atom index.
+std::vector<uint32_t> host_frames(const Program &P) {
+    std::vector<uint32_t> fr(P.host_atoms.size(), 0);
+    for (uint32_t pc : P.atom_pc) {
+        const Insn in = P.code[pc];
+        if (insn_kind(in.x) == A_HOST) fr[(size_t)P.f64[in.y]] = insn_frame(in.x);
+    }
+    return fr;
+}
+
+std::string fmt_bounds(const double *mn, const double *mx) {
+    char buf[512];
+    snprintf(buf, sizeof buf,
+             "invalid bounds: min=model3d.Coord3D{X:%.17g, Y:%.17g, Z:%.17g} max=model3d.Coord3D{X:%.17g, Y:%.17g, "
+             "Z:%.17g}",
+             mn[0], mn[1], mn[2], mx[0], mx[1], mx[2]);
+    return buf;
+}
+
+// validateBounds (mc.go:15, bounder.go:48-52), the spacer (mc.go:594-608),
+// the lowering, and the culling bricks: 64 x B x B corners, B growing until
+// the fixed program slots fit.
+int prepare_job(Job &j, const mc3d_options &o) {
+    double mn[3], mx[3];
+    solid_bounds(*j.s->ptr, mn, mx);
+    if (!bounds_valid(*j.s->ptr)) {
+        set_error(fmt_bounds(mn, mx));
+        return MC3D_E_INVALID_BOUNDS;
+    }
+    if (!(j.delta > 0)) {
+        set_error("delta must be positive");
+        return MC3D_E_INVALID_ARG;
+    }
+    if (j.iters < 0) j.iters = 0;
+    for (int a = 0; a < 3; a++) {
+        j.ax[a] = spacer_axis(mn[a], mx[a], j.delta);
+        if (j.ax[a].size() >= (1u << 20)) {
+            set_error("grid too large: more than 2^20 samples on an axis");
+            return MC3D_E_UNSUPPORTED;
+        }
+        if (j.ax[a].size() < 2) {
+            set_error("grid has fewer than two samples on an axis");
+            return MC3D_E_INVALID_ARG;
+        }
+    }
+    std::string msg;
+    if (!lower_solid(*j.s->ptr, j.P, msg)) {
+        set_error(msg);
+        return MC3D_E_UNSUPPORTED;
+    }
+    j.host = !j.P.host_atoms.empty();
+    j.frames = host_frames(j.P);
+    Grid &g = j.g;
+    g.nx = (int)j.ax[0].size(), g.ny = (int)j.ax[1].size(), g.nz = (int)j.ax[2].size();
+    g.W = (g.nx + 63) / 64;
+    j.nwords = (uint64_t)g.W * g.ny * g.nz;
+    if (j.nwords >= (1ull << 32)) {
+        set_error("grid too large: more than 2^32 occupancy words");
+        return MC3D_E_UNSUPPORTED;
+    }
+    j.B = Bricks{nullptr, nullptr, 1, 1, 1, 1};
+    // A host-evaluated solid may be true outside its bounds, so nothing is
+    // folded by bounds: no culling.
+    if (o.cull && !j.host) {
+        int bsz = o.brick > 0 ? o.brick : 4;
+        j.cap = std::min<uint32_t>((uint32_t)j.P.code.size(), 64u);
+        for (;;) {
+            j.B.BY = j.B.BZ = bsz;
+            j.B.nby = (g.ny + bsz - 1) / bsz;
+            j.B.nbz = (g.nz + bsz - 1) / bsz;
+            j.nbricks = (uint64_t)g.W * j.B.nby * j.B.nbz;
+            if (j.nbricks * j.cap < (1ull << 27) || bsz >= 64) break;
+            bsz *= 2;
+        }
+    }
+    return MC3D_OK;
+}
 
 // The device copy of a collider's ray grid (uploaded once per context).
 int collider_on_device(mc3d_ctx *ctx, const std::shared_ptr<const ColliderData> &cd, DevCollider &out,
@@ -205,90 +303,157 @@ int collider_on_device(mc3d_ctx *ctx, const std::shared_ptr<const ColliderData> 
     d.u0 = G.u0, d.v0 = G.v0, d.inv_cell = G.inv_cell, d.nu = G.nu, d.nv = G.nv;
     out = d;
     ctx->colliders.push_back(std::move(c));
-    // Evict the oldest entries, never one the current program uses (those
-    // are the newest `keep_at_least`, looked up or added in this upload).
+    // Evict the oldest entries, never one the current call uses (those are
+    // the newest `keep_at_least`, looked up or added in this upload).
     while (ctx->colliders.size() > std::max<size_t>(4, keep_at_least)) ctx->colliders.erase(ctx->colliders.begin());
     return MC3D_OK;
 }
 
-// Packs the lowered program, the grid axes and a zeroed control block into
-// the pinned staging buffer and copies it to the device in one transfer.
-int upload(mc3d_ctx *ctx, const Program &P, const std::vector<double> *ax, Upload &U) {
-    std::vector<DevCollider> cols(P.colliders.size());
-    for (size_t i = 0; i < cols.size(); i++) {
-        int rc = collider_on_device(ctx, P.colliders[i], cols[i], cols.size());
-        if (rc) return rc;
+// Bytes of LDS the plan needs to stage a program of n instructions.
+size_t plan_stage_bytes(size_t n) { return PLAN_STACK_BYTES + n * (sizeof(Insn) + sizeof(AtomBox)); }
+constexpr size_t PLAN_STAGE_LIMIT = 64 * 1024;
+
+// Where a call's pieces sit in device memory.
+struct Upload {
+    Ctl *ctl = nullptr;
+    SegOut *sout = nullptr;
+    Seg *segs = nullptr;
+    uint32_t *word_seg = nullptr, *brick_seg = nullptr;
+    std::vector<Seg> hsegs;  // host copies of the segments
+    size_t back_bytes = 0;   // control block + per-solid results, copied back at the end
+};
+
+// Packs the control block, the per-solid results, the segments, the segment
+// tables and every job's program and grid axes into the pinned staging
+// buffer and copies it to the device in one transfer.
+int upload(mc3d_ctx *ctx, std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_all, Upload &U) {
+    const size_t n = jobs.size();
+    size_t ncol = 0;
+    for (const Job &j : jobs) ncol += j.P.colliders.size();
+    std::vector<std::vector<DevCollider>> cols(n);
+    for (size_t s = 0; s < n; s++) {
+        cols[s].resize(jobs[s].P.colliders.size());
+        for (size_t i = 0; i < cols[s].size(); i++) {
+            int rc = collider_on_device(ctx, jobs[s].P.colliders[i], cols[s][i], ncol);
+            if (rc) return rc;
+        }
     }
-    std::vector<AtomBox> boxes(P.code.size());
-    for (size_t i = 0; i < P.atom_pc.size(); i++) boxes[P.atom_pc[i]] = P.atom_box[i];
     size_t off = 0;
     auto place = [&](size_t bytes) {
         size_t o = off;
         off = align_up(off + bytes, 256);
         return o;
     };
-    const size_t o_ctl = place(sizeof(Ctl));
-    const size_t o_code = place(P.code.size() * sizeof(Insn));
-    const size_t o_f64 = place(P.f64.size() * sizeof(double));
-    const size_t o_f32 = place(P.f32.size() * sizeof(float));
-    const size_t o_fr = place(P.frames.size() * sizeof(Frame));
-    const size_t o_box = place(boxes.size() * sizeof(AtomBox));
-    const size_t o_col = place(cols.size() * sizeof(DevCollider));
-    size_t n_ax = 0;
-    if (ax) n_ax = ax[0].size() + ax[1].size() + ax[2].size();
-    const size_t o_ax = place(n_ax * sizeof(double));
+    // the per-solid results follow the control block directly (one copy back)
+    const size_t o_sout = align_up(sizeof(Ctl), 16);
+    U.back_bytes = o_sout + n * sizeof(SegOut);
+    off = align_up(U.back_bytes, 256);
+    const size_t o_segs = place(n * sizeof(Seg));
+    const bool tables = n > 1;
+    const size_t o_wseg = place(tables ? nwords_all / 64 * sizeof(uint32_t) : 0);
+    const size_t o_bseg = place(tables ? nbricks_all / 128 * sizeof(uint32_t) : 0);
+    struct Off {
+        size_t code, f64, f32, fr, box, col, ax;
+    };
+    std::vector<Off> po(n);
+    std::vector<std::vector<AtomBox>> boxes(n);
+    for (size_t s = 0; s < n; s++) {
+        const Program &P = jobs[s].P;
+        boxes[s].resize(P.code.size());
+        for (size_t i = 0; i < P.atom_pc.size(); i++) boxes[s][P.atom_pc[i]] = P.atom_box[i];
+        po[s].code = place(P.code.size() * sizeof(Insn));
+        // params padded so that staging a brick's atoms never reads past the array
+        po[s].f64 = place((P.f64.size() + MAX_ATOM_PARAMS) * sizeof(double));
+        po[s].f32 = place((P.f32.size() + MAX_ATOM_PARAMS) * sizeof(float));
+        po[s].fr = place(P.frames.size() * sizeof(Frame));
+        po[s].box = place(boxes[s].size() * sizeof(AtomBox));
+        po[s].col = place(cols[s].size() * sizeof(DevCollider));
+        po[s].ax = place((jobs[s].ax[0].size() + jobs[s].ax[1].size() + jobs[s].ax[2].size()) * sizeof(double));
+    }
     const size_t total = off;
     int rc;
     if ((rc = ctx->staging.ensure(total)) || (rc = ctx->blob.ensure(total))) return rc;
     char *h = (char *)ctx->staging.p;
-    memset(h + o_ctl, 0, sizeof(Ctl));
-    memcpy(h + o_code, P.code.data(), P.code.size() * sizeof(Insn));
-    if (!P.f64.empty()) {
-        memcpy(h + o_f64, P.f64.data(), P.f64.size() * sizeof(double));
-        memcpy(h + o_f32, P.f32.data(), P.f32.size() * sizeof(float));
-    }
-    memcpy(h + o_fr, P.frames.data(), P.frames.size() * sizeof(Frame));
-    memcpy(h + o_box, boxes.data(), boxes.size() * sizeof(AtomBox));
-    if (!cols.empty()) memcpy(h + o_col, cols.data(), cols.size() * sizeof(DevCollider));
-    if (ax) {
-        double *d = (double *)(h + o_ax);
-        for (int a = 0; a < 3; a++) d = std::copy(ax[a].begin(), ax[a].end(), d);
-    }
-    HIP_TRY(hipMemcpyAsync(ctx->blob.p, h, total, hipMemcpyHostToDevice, ctx->stream));
     char *dv = (char *)ctx->blob.p;
-    U.dp.code = (const Insn *)(dv + o_code);
-    U.dp.f64 = (const double *)(dv + o_f64);
-    U.dp.f32 = (const float *)(dv + o_f32);
-    U.dp.frames = (const Frame *)(dv + o_fr);
-    U.dp.boxes = (const AtomBox *)(dv + o_box);
-    U.dp.colliders = (const DevCollider *)(dv + o_col);
-    U.dp.host_vals = nullptr;
-    U.dp.host_stride = 0;
-    U.dp.ncode = (uint32_t)P.code.size();
-    U.dp.pad = 0;
-    U.ctl = (Ctl *)(dv + o_ctl);
-    U.axes = (double *)(dv + o_ax);
+    memset(h, 0, o_segs);  // control block and per-solid results start at zero
+    U.hsegs.assign(n, Seg{});
+    for (size_t s = 0; s < n; s++) {
+        Job &j = jobs[s];
+        const Program &P = j.P;
+        memcpy(h + po[s].code, P.code.data(), P.code.size() * sizeof(Insn));
+        memset(h + po[s].f64, 0, (P.f64.size() + MAX_ATOM_PARAMS) * sizeof(double));
+        memset(h + po[s].f32, 0, (P.f32.size() + MAX_ATOM_PARAMS) * sizeof(float));
+        if (!P.f64.empty()) {
+            memcpy(h + po[s].f64, P.f64.data(), P.f64.size() * sizeof(double));
+            memcpy(h + po[s].f32, P.f32.data(), P.f32.size() * sizeof(float));
+        }
+        memcpy(h + po[s].fr, P.frames.data(), P.frames.size() * sizeof(Frame));
+        memcpy(h + po[s].box, boxes[s].data(), boxes[s].size() * sizeof(AtomBox));
+        if (!cols[s].empty()) memcpy(h + po[s].col, cols[s].data(), cols[s].size() * sizeof(DevCollider));
+        double *d = (double *)(h + po[s].ax);
+        for (int a = 0; a < 3; a++) d = std::copy(j.ax[a].begin(), j.ax[a].end(), d);
+        Seg &sg = U.hsegs[s];
+        sg.P.code = (const Insn *)(dv + po[s].code);
+        sg.P.f64 = (const double *)(dv + po[s].f64);
+        sg.P.f32 = (const float *)(dv + po[s].f32);
+        sg.P.frames = (const Frame *)(dv + po[s].fr);
+        sg.P.boxes = (const AtomBox *)(dv + po[s].box);
+        sg.P.colliders = (const DevCollider *)(dv + po[s].col);
+        sg.P.host_vals = nullptr;
+        sg.P.host_stride = 0;
+        sg.P.ncode = (uint32_t)P.code.size();
+        sg.P.pad = 0;
+        if (!j.ax[0].empty()) {
+            const double *ax = (const double *)(dv + po[s].ax);
+            j.g.xs = ax, j.g.ys = ax + j.g.nx, j.g.zs = ax + j.g.nx + j.g.ny;
+        }
+        sg.g = j.g;
+        sg.B = j.B;
+        if (j.cap) sg.B.info = ctx->binfo.as<uint2>(), sg.B.code = ctx->bcode.as<Insn>();
+        sg.word0 = j.word0, sg.nwords = j.nwords;
+        sg.brick0 = j.brick0, sg.nbricks = j.nbricks;
+        sg.iters = j.iters;
+        sg.stage = j.cap && plan_stage_bytes(P.code.size()) <= PLAN_STAGE_LIMIT ? 1 : 0;
+    }
+    memcpy(h + o_segs, U.hsegs.data(), n * sizeof(Seg));
+    if (tables) {
+        uint32_t *ws = (uint32_t *)(h + o_wseg), *bs = (uint32_t *)(h + o_bseg);
+        for (size_t s = 0; s < n; s++) {
+            const uint64_t w1 = s + 1 < n ? jobs[s + 1].word0 : nwords_all;
+            for (uint64_t w = jobs[s].word0; w < w1; w += 64) ws[w / 64] = (uint32_t)s;
+            const uint64_t b1 = s + 1 < n ? jobs[s + 1].brick0 : nbricks_all;
+            for (uint64_t b = jobs[s].brick0; b < b1; b += 128) bs[b / 128] = (uint32_t)s;
+        }
+    }
+    U.ctl = (Ctl *)dv;
+    U.sout = (SegOut *)(dv + o_sout);
+    U.segs = (Seg *)(dv + o_segs);
+    U.word_seg = tables ? (uint32_t *)(dv + o_wseg) : nullptr;
+    U.brick_seg = tables ? (uint32_t *)(dv + o_bseg) : nullptr;
+    HIP_TRY(hipMemcpyAsync(ctx->blob.p, h, total, hipMemcpyHostToDevice, ctx->stream));
     return MC3D_OK;
 }
 
-// Frame of each host-evaluated atom, by host atom index.
-std::vector<uint32_t> host_frames(const Program &P) {
-    std::vector<uint32_t> fr(P.host_atoms.size(), 0);
-    for (uint32_t pc : P.atom_pc) {
-        const Insn in = P.code[pc];
-        if (insn_kind(in.x) == A_HOST) fr[(size_t)P.f64[in.y]] = insn_frame(in.x);
-    }
-    return fr;
+// Rewrites segment s on the device (after its host-evaluated answers moved).
+int update_seg(mc3d_ctx *ctx, const Upload &U, size_t s) {
+    int rc;
+    if ((rc = ctx->segstage.ensure(sizeof(Seg)))) return rc;
+    // Every caller synchronised the stream after the slot's previous copy.
+    memcpy(ctx->segstage.p, &U.hsegs[s], sizeof(Seg));
+    HIP_TRY(hipMemcpyAsync(U.segs + s, ctx->segstage.p, sizeof(Seg), hipMemcpyHostToDevice, ctx->stream));
+    return MC3D_OK;
 }
 
 constexpr uint64_t HOST_BATCH = 1u << 20;  // points per callback
 
-// Answers every host-evaluated atom at n points -- grid corners (pts ==
-// nullptr) or a device point list -- through its callback, in batches, into
-// hvals[h * n + i], and points the program at them.  The GPU computes each
-// atom's local points; the host only runs the callbacks.
-int host_round(mc3d_ctx *ctx, const Program &P, const std::vector<uint32_t> &frames, Upload &U, const Grid &g,
-               const double *pts, uint64_t n, double &ms_cb) {
+// Answers every host-evaluated atom of segment s at n points -- its grid
+// corners (pts == nullptr) or a device point list -- through its callback,
+// in batches, into hvals[h * n + i], and points the segment's program at
+// them.  The GPU computes each atom's local points; the host only runs the
+// callbacks.
+int host_round(mc3d_ctx *ctx, const Job &j, Upload &U, size_t s, const double *pts, uint64_t n, double &ms_cb) {
+    const Program &P = j.P;
+    DevProg &dp = U.hsegs[s].P;
     const size_t nh = P.host_atoms.size();
     int rc;
     if ((rc = ctx->hvals.ensure(nh * n + 1)) || (rc = ctx->hpts.ensure(HOST_BATCH * 24)) ||
@@ -297,9 +462,9 @@ int host_round(mc3d_ctx *ctx, const Program &P, const std::vector<uint32_t> &fra
     double *hp = (double *)ctx->hstage.p;
     uint8_t *ho = (uint8_t *)(hp + 3 * HOST_BATCH);
     for (size_t h = 0; h < nh; h++) {
-        for (uint64_t s = 0; s < n; s += HOST_BATCH) {
-            const uint64_t c = std::min(HOST_BATCH, n - s);
-            k_host_points<<<grid_for(c, 256), 256, 0, ctx->stream>>>(U.dp, g, pts, frames[h], s, c,
+        for (uint64_t s0 = 0; s0 < n; s0 += HOST_BATCH) {
+            const uint64_t c = std::min(HOST_BATCH, n - s0);
+            k_host_points<<<grid_for(c, 256), 256, 0, ctx->stream>>>(dp, U.hsegs[s].g, pts, j.frames[h], s0, c,
                                                                     ctx->hpts.as<double>());
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipMemcpyAsync(hp, ctx->hpts.p, c * 24, hipMemcpyDeviceToHost, ctx->stream));
@@ -312,21 +477,12 @@ int host_round(mc3d_ctx *ctx, const Program &P, const std::vector<uint32_t> &fra
                 set_error("host solid callback failed with status " + std::to_string(r));
                 return MC3D_E_CALLBACK;
             }
-            HIP_TRY(hipMemcpyAsync(ctx->hvals.as<uint8_t>() + h * n + s, ho, c, hipMemcpyHostToDevice, ctx->stream));
+            HIP_TRY(hipMemcpyAsync(ctx->hvals.as<uint8_t>() + h * n + s0, ho, c, hipMemcpyHostToDevice, ctx->stream));
         }
     }
-    U.dp.host_vals = ctx->hvals.as<uint8_t>();
-    U.dp.host_stride = n;
-    return MC3D_OK;
-}
-
-std::string fmt_bounds(const double *mn, const double *mx) {
-    char buf[512];
-    snprintf(buf, sizeof buf,
-             "invalid bounds: min=model3d.Coord3D{X:%.17g, Y:%.17g, Z:%.17g} max=model3d.Coord3D{X:%.17g, Y:%.17g, "
-             "Z:%.17g}",
-             mn[0], mn[1], mn[2], mx[0], mx[1], mx[2]);
-    return buf;
+    dp.host_vals = ctx->hvals.as<uint8_t>();
+    dp.host_stride = n;
+    return update_seg(ctx, U, s);
 }
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
@@ -346,65 +502,145 @@ int scan4(mc3d_ctx *ctx, const uint4 *in, uint4 *out, uint64_t n, unsigned long 
     return MC3D_OK;
 }
 
-template <int MODE, bool STATS>
-void launch_corners(mc3d_ctx *ctx, const DevProg &dp, const Bricks &B, const Grid &g, Ctl *ctl, uint64_t nwords,
-                    uint64_t nbricks, int rows_per_walk) {
-    if (!B.info) {
-        k_corners<MODE, STATS><<<grid_for(nwords, 4), 256, 0, ctx->stream>>>(dp, g, ctx->bits.as<uint64_t>(), ctl,
-                                                                             nwords);
+// Sizes and launch shapes of one call.
+struct Call {
+    uint32_t nsegs = 0;
+    uint64_t nwords = 0, nbricks = 0;  // over all segments, with alignment padding
+    uint32_t cap = 0;                  // folded-program slot (culling) of every brick
+    bool cull = false;
+    int mode = 0, rows_per_walk = 1, wpb = 4;
+    bool stats = false;
+};
+
+template <int MODE, bool STATS, int NP, bool MULTI>
+void launch_corners_act(mc3d_ctx *ctx, const Call &c, const Upload &U, unsigned grid) {
+    uint32_t *act = ctx->actb.as<uint32_t>();
+    uint64_t *bits = ctx->bits.as<uint64_t>();
+    const Seg &s0 = U.hsegs[0];
+    if (c.wpb == 1)
+        k_corners_act<MODE, STATS, NP, 1, MULTI>
+            <<<grid, 64, 0, ctx->stream>>>(U.segs, c.nsegs, s0, U.brick_seg, act, bits, U.ctl, U.sout);
+    else if (c.wpb == 2)
+        k_corners_act<MODE, STATS, NP, 2, MULTI>
+            <<<grid, 128, 0, ctx->stream>>>(U.segs, c.nsegs, s0, U.brick_seg, act, bits, U.ctl, U.sout);
+    else
+        k_corners_act<MODE, STATS, NP, 4, MULTI>
+            <<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, s0, U.brick_seg, act, bits, U.ctl, U.sout);
+}
+
+template <int MODE, bool STATS, bool MULTI>
+void launch_corners_m(mc3d_ctx *ctx, const Call &c, const Upload &U) {
+    const Seg &s0 = U.hsegs[0];
+    if (!c.cull) {
+        (void)hipEventRecord(ctx->ev[7], ctx->stream);
+        k_corners<MODE, STATS, MULTI><<<grid_for(c.nwords, 4), 256, 0, ctx->stream>>>(
+            U.segs, c.nsegs, s0, U.word_seg, ctx->bits.as<uint64_t>(), U.ctl, U.sout, c.nwords);
         return;
     }
-    k_fill_const<<<grid_for(nwords, 256), 256, 0, ctx->stream>>>(B, g, ctx->bits.as<uint64_t>(), ctl, nwords);
-    const unsigned grid = grid_for(nbricks, 1, 1u << 15);
-    if (rows_per_walk >= 4)
-        k_corners_act<MODE, STATS, 4><<<grid, 256, 0, ctx->stream>>>(dp, B, g, ctx->actb.as<uint32_t>(),
-                                                                     ctx->bits.as<uint64_t>(), ctl);
-    else if (rows_per_walk == 2)
-        k_corners_act<MODE, STATS, 2><<<grid, 256, 0, ctx->stream>>>(dp, B, g, ctx->actb.as<uint32_t>(),
-                                                                     ctx->bits.as<uint64_t>(), ctl);
+    k_fill_const<MULTI><<<grid_for(c.nwords, 256), 256, 0, ctx->stream>>>(
+        U.segs, c.nsegs, s0, U.word_seg, ctx->bits.as<uint64_t>(), U.ctl, U.sout, c.nwords);
+    (void)hipEventRecord(ctx->ev[7], ctx->stream);
+    const unsigned grid = grid_for(c.nbricks, 1, 1u << 15);
+    if (c.rows_per_walk >= 4)
+        launch_corners_act<MODE, STATS, 4, MULTI>(ctx, c, U, grid);
+    else if (c.rows_per_walk == 2)
+        launch_corners_act<MODE, STATS, 2, MULTI>(ctx, c, U, grid);
     else
-        k_corners_act<MODE, STATS, 1><<<grid, 256, 0, ctx->stream>>>(dp, B, g, ctx->actb.as<uint32_t>(),
-                                                                     ctx->bits.as<uint64_t>(), ctl);
+        launch_corners_act<MODE, STATS, 1, MULTI>(ctx, c, U, grid);
 }
 
 template <int MODE, bool STATS>
-void launch_bisect(mc3d_ctx *ctx, const DevProg &dp, const Bricks &B, const Grid &g, Ctl *ctl, double *verts,
-                   uint64_t vcap, int iters) {
-    k_bisect<MODE, STATS><<<grid_for(vcap, 256, 1u << 14), 256, 0, ctx->stream>>>(
-        dp, B, g, ctx->bits.as<uint64_t>(), ctx->vkeys.as<uint64_t>(), verts, ctl, vcap, iters);
+void launch_corners(mc3d_ctx *ctx, const Call &c, const Upload &U) {
+    c.nsegs > 1 ? launch_corners_m<MODE, STATS, true>(ctx, c, U) : launch_corners_m<MODE, STATS, false>(ctx, c, U);
+}
+
+template <int MODE, bool STATS>
+void launch_bisect(mc3d_ctx *ctx, const Call &c, const Upload &U, double *verts, uint64_t vcap) {
+    const unsigned grid = grid_for(vcap, 256, 1u << 14);
+    if (c.nsegs > 1)
+        k_bisect<MODE, STATS, true><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0],
+                                                                   ctx->vseg.as<uint32_t>(), ctx->bits.as<uint64_t>(),
+                                                                   ctx->vkeys.as<uint64_t>(), verts, U.ctl, vcap);
+    else
+        k_bisect<MODE, STATS, false><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], nullptr,
+                                                                    ctx->bits.as<uint64_t>(),
+                                                                    ctx->vkeys.as<uint64_t>(), verts, U.ctl, vcap);
+}
+
+int take_outputs(mc3d_ctx *ctx, MeshBuffers &mb, const Call &c, uint64_t vcap, uint64_t tcap) {
+    int rc;
+    if (!mb.verts || mb.verts->cap < vcap * 24) {
+        ctx->pool->give(std::move(mb.verts));
+        mb.verts = ctx->pool->take(vcap * 24);
+    }
+    if (!mb.tris || mb.tris->cap < tcap * 12) {
+        ctx->pool->give(std::move(mb.tris));
+        mb.tris = ctx->pool->take(tcap * 12);
+    }
+    if ((rc = mb.verts->ensure(vcap * 24)) || (rc = mb.tris->ensure(tcap * 12)) ||
+        (rc = ctx->vkeys.ensure(vcap * 8)))
+        return rc;
+    if (c.nsegs > 1 && (rc = ctx->vseg.ensure(vcap * 4))) return rc;
+    return MC3D_OK;
+}
+
+// Meshes keep their outputs until freed: a buffer sized for earlier, larger
+// calls is traded for one that fits (the copy is ordered on the stream), and
+// goes back to the pool for the next call.
+int fit_outputs(mc3d_ctx *ctx, MeshBuffers &mb, uint64_t nv, uint64_t nt) {
+    auto fit = [&](std::unique_ptr<DevBuf> &b, size_t need) -> int {
+        if (!b || b->cap <= 2 * need + (4u << 20)) return MC3D_OK;
+        std::unique_ptr<DevBuf> f = ctx->pool->take(need);
+        int rc = f->ensure(need);
+        if (rc) return rc;
+        if (need) HIP_TRY(hipMemcpyAsync(f->p, b->p, need, hipMemcpyDeviceToDevice, ctx->stream));
+        ctx->pool->give(std::move(b));
+        b = std::move(f);
+        return MC3D_OK;
+    };
+    int rc;
+    if ((rc = fit(mb.verts, nv * 24)) || (rc = fit(mb.tris, nt * 12))) return rc;
+    return MC3D_OK;
+}
+
+void launch_emit(mc3d_ctx *ctx, const Call &c, const Upload &U, MeshBuffers &mb, uint64_t vcap, uint64_t tcap) {
+    const unsigned grid = grid_for(c.nwords, 4, 2048);
+    if (c.nsegs > 1)
+        k_emit<true><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(),
+                                                    ctx->table.as<uint8_t>(), ctx->offs.as<uint4>(),
+                                                    ctx->actw.as<uint32_t>(), U.ctl, ctx->vkeys.as<uint64_t>(),
+                                                    ctx->vseg.as<uint32_t>(), mb.tris->as<uint32_t>(), vcap, tcap);
+    else
+        k_emit<false><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.word_seg,
+                                                     ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
+                                                     ctx->offs.as<uint4>(), ctx->actw.as<uint32_t>(), U.ctl,
+                                                     ctx->vkeys.as<uint64_t>(), nullptr, mb.tris->as<uint32_t>(), vcap,
+                                                     tcap);
 }
 
 // The output stages: emit (vertex keys, triangles) and bisect (vertices).
-int run_output(mc3d_ctx *ctx, const Upload &U, const Bricks &B, const Grid &g, mc3d_mesh *m, int mode, bool stats,
-               int iters, uint64_t vcap, uint64_t tcap, uint64_t nwords) {
+int run_output(mc3d_ctx *ctx, const Call &c, const Upload &U, MeshBuffers &mb, uint64_t vcap, uint64_t tcap) {
     HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
-    k_emit<<<grid_for(nwords, 4, 2048), 256, 0, ctx->stream>>>(
-        g, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(), ctx->offs.as<uint4>(), ctx->actw.as<uint32_t>(),
-        U.ctl, ctx->vkeys.as<uint64_t>(), m->dtris->as<uint32_t>(), vcap, tcap);
+    launch_emit(ctx, c, U, mb, vcap, tcap);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
-    double *verts = m->dverts->as<double>();
-    if (mode == 0)
-        stats ? launch_bisect<0, true>(ctx, U.dp, B, g, U.ctl, verts, vcap, iters)
-              : launch_bisect<0, false>(ctx, U.dp, B, g, U.ctl, verts, vcap, iters);
+    double *verts = mb.verts->as<double>();
+    if (c.mode == 0)
+        c.stats ? launch_bisect<0, true>(ctx, c, U, verts, vcap) : launch_bisect<0, false>(ctx, c, U, verts, vcap);
     else
-        stats ? launch_bisect<1, true>(ctx, U.dp, B, g, U.ctl, verts, vcap, iters)
-              : launch_bisect<1, false>(ctx, U.dp, B, g, U.ctl, verts, vcap, iters);
+        c.stats ? launch_bisect<1, true>(ctx, c, U, verts, vcap) : launch_bisect<1, false>(ctx, c, U, verts, vcap);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream));
     return MC3D_OK;
 }
 
-int take_outputs(mc3d_ctx *ctx, mc3d_mesh *m, uint64_t vcap, uint64_t tcap);
-
-// The output stages of a program with host-evaluated atoms: the mesh size is
-// read first (one more synchronisation), then emit, then the bisection in
-// rounds with the host answering its atoms between them.  Leaves the final
-// control block in ctx->ctl_host.
-int run_output_host(mc3d_ctx *ctx, const Program &P, const std::vector<uint32_t> &frames, Upload &U, const Grid &g,
-                    mc3d_mesh *m, int mode, int iters, uint64_t nwords, double &ms_cb) {
+// The output stages of a solid with host-evaluated atoms (a call of one):
+// the mesh size is read first (one more synchronisation), then emit, then
+// the bisection in rounds with the host answering its atoms between them.
+// Leaves the final control block in ctx->ctl_host.
+int run_output_host(mc3d_ctx *ctx, const Call &c, const Job &j, Upload &U, MeshBuffers &mb, double &ms_cb) {
     Ctl *hc = (Ctl *)ctx->ctl_host.p;
-    HIP_TRY(hipMemcpyAsync(hc, U.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(hc, U.ctl, U.back_bytes, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (hc->inside) {
         set_error("solid is true outside of bounds");
@@ -417,49 +653,243 @@ int run_output_host(mc3d_ctx *ctx, const Program &P, const std::vector<uint32_t>
     }
     const uint64_t vcap = std::max<uint64_t>(nv, 1), tcap = std::max<uint64_t>(nt, 1);
     int rc;
-    if ((rc = take_outputs(ctx, m, vcap, tcap)) || (rc = ctx->hstate.ensure(vcap * 16)) ||
+    if ((rc = take_outputs(ctx, mb, c, vcap, tcap)) || (rc = ctx->hstate.ensure(vcap * 16)) ||
         (rc = ctx->hprobe.ensure(vcap * 24)))
         return rc;
     HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
-    k_emit<<<grid_for(nwords, 4, 2048), 256, 0, ctx->stream>>>(
-        g, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(), ctx->offs.as<uint4>(), ctx->actw.as<uint32_t>(),
-        U.ctl, ctx->vkeys.as<uint64_t>(), m->dtris->as<uint32_t>(), vcap, tcap);
+    launch_emit(ctx, c, U, mb, vcap, tcap);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
-    double *verts = m->dverts->as<double>(), *state = ctx->hstate.as<double>(), *probe = ctx->hprobe.as<double>();
+    double *verts = mb.verts->as<double>(), *state = ctx->hstate.as<double>(), *probe = ctx->hprobe.as<double>();
     const unsigned grid = grid_for(nv, 256, 1u << 14);
-    for (int round = 0; round <= iters; round++) {
-        if (round > 0 && (rc = host_round(ctx, P, frames, U, g, probe, nv, ms_cb))) return rc;
-        if (mode == 1)
-            k_bisect_host<1><<<grid, 256, 0, ctx->stream>>>(U.dp, g, ctx->bits.as<uint64_t>(),
+    for (int round = 0; round <= j.iters; round++) {
+        if (round > 0 && (rc = host_round(ctx, j, U, 0, probe, nv, ms_cb))) return rc;
+        const DevProg &dp = U.hsegs[0].P;
+        if (c.mode == 1)
+            k_bisect_host<1><<<grid, 256, 0, ctx->stream>>>(dp, U.hsegs[0].g, ctx->bits.as<uint64_t>(),
                                                             ctx->vkeys.as<uint64_t>(), state, probe, verts, nv,
                                                             round > 0);
         else
-            k_bisect_host<0><<<grid, 256, 0, ctx->stream>>>(U.dp, g, ctx->bits.as<uint64_t>(),
+            k_bisect_host<0><<<grid, 256, 0, ctx->stream>>>(dp, U.hsegs[0].g, ctx->bits.as<uint64_t>(),
                                                             ctx->vkeys.as<uint64_t>(), state, probe, verts, nv,
                                                             round > 0);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream));
-    HIP_TRY(hipMemcpyAsync(hc, U.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(hc, U.ctl, U.back_bytes, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return MC3D_OK;
 }
 
-int take_outputs(mc3d_ctx *ctx, mc3d_mesh *m, uint64_t vcap, uint64_t tcap) {
+// Meshes `jobs` (prepared) in one pass of the pipeline; out[i] gets job i's
+// mesh.  A job with host-evaluated atoms is only ever run alone.
+int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::vector<mc3d_mesh *> &out) {
+    const auto t_call = std::chrono::steady_clock::now();
+    Call c;
+    c.nsegs = (uint32_t)jobs.size();
+    c.mode = o.precision == MC3D_PREC_FP32_GUARD ? 1 : 0;
+    c.stats = o.count_mismatch != 0;
+    c.rows_per_walk = o.rows_per_walk > 0 ? o.rows_per_walk : 1;
+    c.wpb = o.waves_per_brick > 0 ? o.waves_per_brick : 4;
+    c.cull = jobs[0].cap != 0;
+    const bool host = c.nsegs == 1 && jobs[0].host;
+    for (Job &j : jobs) {
+        j.word0 = c.nwords;
+        c.nwords += align_up(j.nwords, 64);
+        j.brick0 = c.nbricks;
+        c.nbricks += align_up(j.nbricks, 128);
+        c.cap = std::max(c.cap, j.cap);
+    }
+    if (c.nwords >= (1ull << 32)) {
+        set_error("call too large: more than 2^32 occupancy words");
+        return MC3D_E_UNSUPPORTED;
+    }
+    if (c.cull && c.nbricks * c.cap >= (1ull << 32) - 1) {
+        set_error("too many culling bricks");
+        return MC3D_E_UNSUPPORTED;
+    }
     int rc;
-    if (!m->dverts || m->dverts->cap < vcap * 24) {
-        ctx->pool->give(std::move(m->dverts));
-        m->dverts = ctx->pool->take(vcap * 24);
-    }
-    if (!m->dtris || m->dtris->cap < tcap * 12) {
-        ctx->pool->give(std::move(m->dtris));
-        m->dtris = ctx->pool->take(tcap * 12);
-    }
-    if ((rc = m->dverts->ensure(vcap * 24)) || (rc = m->dtris->ensure(tcap * 12)) ||
-        (rc = ctx->vkeys.ensure(vcap * 8)))
+    if ((rc = ctx->bits.ensure(c.nwords * 8)) || (rc = ctx->counts.ensure(c.nwords * sizeof(uint4))) ||
+        (rc = ctx->offs.ensure(c.nwords * sizeof(uint4))) || (rc = ctx->actw.ensure(c.nwords * sizeof(uint32_t))))
         return rc;
+    if (c.cull && ((rc = ctx->binfo.ensure(c.nbricks * sizeof(uint2))) ||
+                   (rc = ctx->actb.ensure(c.nbricks * sizeof(uint32_t))) ||
+                   (rc = ctx->bcode.ensure(c.nbricks * c.cap * sizeof(Insn) + 16))))
+        return rc;
+    Upload U;
+    if ((rc = upload(ctx, jobs, c.nwords, c.nbricks, U))) return rc;
+    if ((rc = ctx->ctl_host.ensure(U.back_bytes))) return rc;
+
+    std::shared_ptr<MeshBuffers> mb = std::make_shared<MeshBuffers>();
+    mb->pool = ctx->pool;
+    mc3d_stats st{};
+    if (host && (rc = host_round(ctx, jobs[0], U, 0, nullptr, (uint64_t)jobs[0].g.nx * jobs[0].g.ny * jobs[0].g.nz,
+                                 st.ms_host_callbacks)))
+        return rc;
+
+    // ---- plan ----
+    HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
+    if (c.cull) {
+        // Programs that fit are staged in LDS beside the per-thread stacks.
+        size_t lds = PLAN_STACK_BYTES;
+        for (size_t s = 0; s < jobs.size(); s++)
+            if (U.hsegs[s].stage) lds = std::max(lds, plan_stage_bytes(jobs[s].P.code.size()));
+        const unsigned pgrid = grid_for(c.nbricks, PLAN_BLOCK);
+        if (c.nsegs > 1)
+            k_plan<true><<<pgrid, PLAN_BLOCK, lds, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.brick_seg, c.nbricks,
+                                                                  c.cap, ctx->binfo.as<uint2>(),
+                                                                  ctx->bcode.as<Insn>(), ctx->actb.as<uint32_t>(), U.ctl);
+        else
+            k_plan<false><<<pgrid, PLAN_BLOCK, lds, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.brick_seg,
+                                                                   c.nbricks, c.cap, ctx->binfo.as<uint2>(),
+                                                                   ctx->bcode.as<Insn>(), ctx->actb.as<uint32_t>(),
+                                                                   U.ctl);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
+
+    // ---- corners ----
+    if (c.mode == 0)
+        c.stats ? launch_corners<0, true>(ctx, c, U) : launch_corners<0, false>(ctx, c, U);
+    else
+        c.stats ? launch_corners<1, true>(ctx, c, U) : launch_corners<1, false>(ctx, c, U);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
+
+    // ---- classify + scan (+ list of words that emit) ----
+    if (c.nsegs > 1)
+        k_classify<true><<<grid_for(c.nwords, 256), 256, 0, ctx->stream>>>(
+            U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
+            ctx->counts.as<uint4>(), c.nwords);
+    else
+        k_classify<false><<<grid_for(c.nwords, 256), 256, 0, ctx->stream>>>(
+            U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
+            ctx->counts.as<uint4>(), c.nwords);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx->ev[8], ctx->stream));
+    if ((rc = scan4(ctx, ctx->counts.as<uint4>(), ctx->offs.as<uint4>(), c.nwords, U.ctl->totals,
+                    ctx->actw.as<uint32_t>())))
+        return rc;
+    k_seg_ranges<<<grid_for(c.nsegs, 256), 256, 0, ctx->stream>>>(U.segs, c.nsegs, ctx->offs.as<uint4>(), U.sout);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
+
+    // ---- emit + bisect, sized from earlier calls ----
+    Ctl *hc = (Ctl *)ctx->ctl_host.p;
+    const SegOut *so = (const SegOut *)((const char *)hc + ((const char *)U.sout - (const char *)U.ctl));
+    if (host) {
+        if ((rc = run_output_host(ctx, c, jobs[0], U, *mb, st.ms_host_callbacks))) return rc;
+    } else {
+        uint64_t vcap = ctx->v_hint, tcap = ctx->t_hint;
+        if ((rc = take_outputs(ctx, *mb, c, vcap, tcap))) return rc;
+        if ((rc = run_output(ctx, c, U, *mb, vcap, tcap))) return rc;
+        HIP_TRY(hipMemcpyAsync(hc, U.ctl, U.back_bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (hc->inside) {
+            size_t bad = 0;
+            while (bad + 1 < jobs.size() && !so[bad].inside) bad++;
+            set_error(c.nsegs == 1 ? std::string("solid is true outside of bounds")
+                                   : "solid " + std::to_string(bad) + " of the batch is true outside of bounds");
+            return MC3D_E_INSIDE_BOUNDARY;
+        }
+        const uint64_t nv = hc->totals[0], nt = hc->totals[1];
+        if (nv >= (1ull << 32) || nt >= (1ull << 32)) {
+            set_error("mesh too large for 32-bit vertex indices");
+            return MC3D_E_UNSUPPORTED;
+        }
+        if (nv > vcap || nt > tcap) {  // grow and run the output stages again
+            ctx->v_hint = std::max(ctx->v_hint, nv + nv / 4 + 1024);
+            ctx->t_hint = std::max(ctx->t_hint, nt + nt / 4 + 1024);
+            vcap = ctx->v_hint, tcap = ctx->t_hint;
+            if ((rc = take_outputs(ctx, *mb, c, vcap, tcap))) return rc;
+            // the bisection's operation counts start again with the second run
+            HIP_TRY(hipMemsetAsync(U.ctl->flops + 2, 0, 2 * sizeof(unsigned long long), ctx->stream));
+            if ((rc = run_output(ctx, c, U, *mb, vcap, tcap))) return rc;
+            HIP_TRY(hipMemcpyAsync(hc, U.ctl, U.back_bytes, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+        }
+    }
+    const uint64_t nv = hc->totals[0], nt = hc->totals[1];
+    if ((rc = fit_outputs(ctx, *mb, nv, nt))) return rc;
+
+    st.bricks_active = hc->nact_bricks;
+    st.guard_evals = hc->gstats[0];
+    st.guard_rechecks = hc->gstats[1];
+    st.guard_mismatches = hc->gstats[2];
+    st.guard_violations = hc->gstats[3];
+    st.corner_flops64 = hc->flops[0];
+    st.corner_flops32 = hc->flops[1];
+    st.bisect_flops64 = hc->flops[2];
+    st.bisect_flops32 = hc->flops[3];
+    st.brick_code_total = hc->code_total;
+    st.brick_code_max = hc->code_max;
+    st.bricks_full = hc->full_bricks;
+    st.ms_plan = elapsed(ctx->ev[0], ctx->ev[1]);
+    st.ms_fill = elapsed(ctx->ev[1], ctx->ev[7]);
+    st.ms_corners = elapsed(ctx->ev[7], ctx->ev[2]);
+    st.ms_classify = elapsed(ctx->ev[2], ctx->ev[8]);
+    st.ms_scan = elapsed(ctx->ev[8], ctx->ev[3]);
+    st.ms_emit = elapsed(ctx->ev[4], ctx->ev[5]);
+    st.ms_bisect = elapsed(ctx->ev[5], ctx->ev[6]);
+    st.ms_device = elapsed(ctx->ev[0], ctx->ev[6]);
+
+    // One mesh per job: its slice of the outputs.
+    std::vector<std::unique_ptr<mc3d_mesh>> meshes(jobs.size());
+    for (size_t s = 0; s < jobs.size(); s++) {
+        meshes[s].reset(new (std::nothrow) mc3d_mesh);
+        if (!meshes[s]) return MC3D_E_OOM;
+        mc3d_mesh *m = meshes[s].get();
+        const Job &j = jobs[s];
+        m->device = ctx->device;
+        m->buf = mb;
+        m->voff = so[s].vbase;
+        m->toff = so[s].tbase;
+        m->nv = (s + 1 < jobs.size() ? so[s + 1].vbase : nv) - m->voff;
+        m->nt = (s + 1 < jobs.size() ? so[s + 1].tbase : nt) - m->toff;
+        m->stats = st;
+        mc3d_stats &ms = m->stats;
+        ms.corners = (uint64_t)j.g.nx * j.g.ny * j.g.nz;
+        ms.grid_nx = j.g.nx, ms.grid_ny = j.g.ny, ms.grid_nz = j.g.nz;
+        ms.vertices = m->nv;
+        ms.triangles = m->nt;
+        ms.bisection_evals = j.iters > 0 ? m->nv * (uint64_t)(j.iters + 1) : 0;
+        ms.bricks_total = j.nbricks;
+        ms.program_length = j.P.code.size();
+    }
+    if (!o.device_only) {
+        auto t0 = std::chrono::steady_clock::now();
+        // one copy of all outputs through pinned memory, then split per mesh
+        if ((rc = ctx->outstage.ensure(nv * 24 + nt * 12 + 16))) return rc;
+        char *hs = (char *)ctx->outstage.p;
+        if (nv) HIP_TRY(hipMemcpyAsync(hs, mb->verts->p, nv * 24, hipMemcpyDeviceToHost, ctx->stream));
+        if (nt) HIP_TRY(hipMemcpyAsync(hs + nv * 24, mb->tris->p, nt * 12, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        const double *hv = (const double *)hs;
+        const uint32_t *ht = (const uint32_t *)(hs + nv * 24);
+        for (auto &m : meshes) {
+            m->hverts.assign(hv + 3 * m->voff, hv + 3 * (m->voff + m->nv));
+            m->htris.assign(ht + 3 * m->toff, ht + 3 * (m->toff + m->nt));
+            m->on_host = true;
+        }
+        const double ms_d2h = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        for (auto &m : meshes) m->stats.ms_d2h = ms_d2h;
+    }
+    const double ms_call =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_call).count();
+    out.resize(jobs.size());
+    for (size_t s = 0; s < jobs.size(); s++) {
+        meshes[s]->stats.ms_call = ms_call;
+        out[s] = meshes[s].release();
+    }
     return MC3D_OK;
+}
+
+mc3d_options default_options(const mc3d_options *opts) {
+    mc3d_options o{};
+    o.precision = MC3D_PREC_FP32_GUARD;
+    o.cull = 1;
+    if (opts) o = *opts;
+    return o;
 }
 
 }  // namespace
@@ -492,7 +922,7 @@ int mc3d_ctx_create(int device, mc3d_ctx **out) {
             return MC3D_E_HIP;
         }
     }
-    if (c->ctl_host.ensure(sizeof(Ctl)) || c->table.ensure(256 * 16) ||
+    if (c->ctl_host.ensure(sizeof(Ctl) + 4096) || c->table.ensure(256 * 16) ||
         hipMemcpy(c->table.p, case_table(), 256 * 16, hipMemcpyHostToDevice) != hipSuccess) {
         set_error("mc3d_ctx_create: setup failed");
         mc3d_ctx_destroy(c);
@@ -519,197 +949,93 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
         return MC3D_E_INVALID_ARG;
     }
     *out = nullptr;
-    const auto t_call = std::chrono::steady_clock::now();
-    if (iters < 0) iters = 0;
-    mc3d_options o{};
-    o.precision = MC3D_PREC_FP32_GUARD;
-    o.cull = 1;
-    if (opts) o = *opts;
-    const int mode = o.precision == MC3D_PREC_FP32_GUARD ? 1 : 0;
-    const bool stats = o.count_mismatch != 0;
+    const mc3d_options o = default_options(opts);
     HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<Job> jobs(1);
+    jobs[0].s = s, jobs[0].delta = delta, jobs[0].iters = iters;
+    int rc = prepare_job(jobs[0], o);
+    if (rc) return rc;
+    std::vector<mc3d_mesh *> meshes;
+    if ((rc = run_call(ctx, jobs, o, meshes))) return rc;
+    *out = meshes[0];
+    return MC3D_OK;
+}
 
-    // validateBounds, mc.go:15 / bounder.go:48-52
-    double mn[3], mx[3];
-    solid_bounds(*s->ptr, mn, mx);
-    if (!bounds_valid(*s->ptr)) {
-        set_error(fmt_bounds(mn, mx));
-        return MC3D_E_INVALID_BOUNDS;
-    }
-    if (!(delta > 0)) {
-        set_error("delta must be positive");
+int mc3d_marching_cubes_search_batch(mc3d_ctx *ctx, const mc3d_solid *const *solids, const double *deltas,
+                                     const int32_t *iters, size_t n, const mc3d_options *opts, mc3d_mesh **out) {
+    if (!ctx || (n && (!solids || !deltas || !iters || !out))) {
+        set_error("mc3d_marching_cubes_search_batch: null argument");
         return MC3D_E_INVALID_ARG;
     }
-    // newSquareSpacer, mc.go:594-608
-    std::vector<double> ax[3];
-    for (int a = 0; a < 3; a++) {
-        ax[a] = spacer_axis(mn[a], mx[a], delta);
-        if (ax[a].size() >= (1u << 20)) {
-            set_error("grid too large: more than 2^20 samples on an axis");
-            return MC3D_E_UNSUPPORTED;
-        }
-        if (ax[a].size() < 2) {
-            set_error("grid has fewer than two samples on an axis");
+    for (size_t i = 0; i < n; i++) out[i] = nullptr;
+    for (size_t i = 0; i < n; i++) {
+        if (!solids[i]) {
+            set_error("mc3d_marching_cubes_search_batch: solid " + std::to_string(i) + " is null");
             return MC3D_E_INVALID_ARG;
         }
     }
-    Program P;
-    std::string msg;
-    if (!lower_solid(*s->ptr, P, msg)) {
-        set_error(msg);
-        return MC3D_E_UNSUPPORTED;
-    }
-    int rc;
-    // A host-evaluated solid may be true outside its bounds, so nothing is
-    // folded by bounds: no culling.
-    const bool host = !P.host_atoms.empty();
-    const std::vector<uint32_t> frames = host_frames(P);
-    if (host) o.cull = 0;
-
-    Grid g;
-    g.nx = (int)ax[0].size(), g.ny = (int)ax[1].size(), g.nz = (int)ax[2].size();
-    g.W = (g.nx + 63) / 64;
-    const uint64_t nwords = (uint64_t)g.W * g.ny * g.nz;
-    if (nwords >= (1ull << 32)) {
-        set_error("grid too large: more than 2^32 occupancy words");
-        return MC3D_E_UNSUPPORTED;
-    }
-
-    // Bricks: 64 x B x B corners; B grows until the fixed program slots fit.
-    Bricks B{nullptr, nullptr, 1, 1, 1, 1};
-    uint64_t nb = 0;
-    uint32_t cap = 0;
-    if (o.cull) {
-        int bsz = o.brick > 0 ? o.brick : 4;
-        cap = std::min<uint32_t>((uint32_t)P.code.size(), 64u);
-        for (;;) {
-            B.BY = B.BZ = bsz;
-            B.nby = (g.ny + bsz - 1) / bsz;
-            B.nbz = (g.nz + bsz - 1) / bsz;
-            nb = (uint64_t)g.W * B.nby * B.nbz;
-            if (nb * cap < (1ull << 27) || bsz >= 64) break;
-            bsz *= 2;
+    if (n == 0) return MC3D_OK;
+    const mc3d_options o = default_options(opts);
+    HIP_TRY(hipSetDevice(ctx->device));
+    // Bounds, spacers and lowering of every solid, on up to 16 host threads
+    // (solid trees are immutable; errors are kept per solid).
+    std::vector<Job> all(n);
+    std::vector<int> rcs(n, MC3D_OK);
+    std::vector<std::string> errs(n);
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        for (size_t i; (i = next.fetch_add(1)) < n;) {
+            all[i].s = solids[i], all[i].delta = deltas[i], all[i].iters = iters[i];
+            rcs[i] = prepare_job(all[i], o);
+            if (rcs[i]) errs[i] = mc3d_last_error();
         }
-        if (nb * cap >= (1ull << 32)) {
-            set_error("too many culling bricks");
-            return MC3D_E_UNSUPPORTED;
+    };
+    const size_t nthreads = std::min<size_t>({16, std::max(1u, std::thread::hardware_concurrency()), (n + 7) / 8});
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nthreads; t++) pool.emplace_back(work);
+    work();
+    for (std::thread &t : pool) t.join();
+    // Solids with host-evaluated atoms need their own callback rounds and run
+    // alone; all others share one pass of the pipeline.
+    std::vector<Job> batched, alone;
+    std::vector<size_t> batched_at, alone_at;
+    for (size_t i = 0; i < n; i++) {
+        if (rcs[i]) {
+            set_error("solid " + std::to_string(i) + " of the batch: " + errs[i]);
+            return rcs[i];
+        }
+        Job &j = all[i];
+        if (j.host) {
+            alone.push_back(std::move(j));
+            alone_at.push_back(i);
+        } else {
+            batched.push_back(std::move(j));
+            batched_at.push_back(i);
         }
     }
-
-    Upload U;
-    if ((rc = upload(ctx, P, ax, U))) return rc;
-    g.xs = U.axes, g.ys = U.axes + g.nx, g.zs = U.axes + g.nx + g.ny;
-    if ((rc = ctx->bits.ensure(nwords * 8)) || (rc = ctx->counts.ensure(nwords * sizeof(uint4))) ||
-        (rc = ctx->offs.ensure(nwords * sizeof(uint4))) || (rc = ctx->actw.ensure(nwords * sizeof(uint32_t))))
+    std::vector<mc3d_mesh *> got(n, nullptr);
+    auto fail = [&](int rc) {
+        for (mc3d_mesh *m : got) delete m;
         return rc;
-
-    std::unique_ptr<mc3d_mesh> mesh(new (std::nothrow) mc3d_mesh);
-    mc3d_mesh *m = mesh.get();
-    if (!m) return MC3D_E_OOM;
-    m->device = ctx->device;
-    m->pool = ctx->pool;
-    mc3d_stats &st = m->stats;
-    if (host && (rc = host_round(ctx, P, frames, U, g, nullptr, (uint64_t)g.nx * g.ny * g.nz, st.ms_host_callbacks)))
-        return rc;
-
-    // ---- plan ----
-    HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
-    if (o.cull) {
-        if ((rc = ctx->binfo.ensure(nb * sizeof(uint2))) || (rc = ctx->actb.ensure(nb * sizeof(uint32_t))) ||
-            (rc = ctx->bcode.ensure(nb * cap * sizeof(Insn) + 16)))
-            return rc;
-        // Stage the program in LDS when it fits beside the per-thread stacks.
-        const size_t staged = PLAN_STACK_BYTES + P.code.size() * (sizeof(Insn) + sizeof(AtomBox));
-        const int stage = staged <= 64 * 1024 ? 1 : 0;
-        k_plan<<<grid_for(nb, PLAN_BLOCK), PLAN_BLOCK, stage ? staged : PLAN_STACK_BYTES, ctx->stream>>>(
-            U.dp, g, B, nb, cap, ctx->binfo.as<uint2>(), ctx->bcode.as<Insn>(), ctx->actb.as<uint32_t>(), U.ctl,
-            stage);
-        HIP_TRY(hipGetLastError());
-        B.info = ctx->binfo.as<uint2>();
-        B.code = ctx->bcode.as<Insn>();
-        st.bricks_total = nb;
+    };
+    if (!batched.empty()) {
+        std::vector<mc3d_mesh *> ms;
+        int rc = run_call(ctx, batched, o, ms);
+        if (rc) return fail(rc);
+        for (size_t k = 0; k < ms.size(); k++) got[batched_at[k]] = ms[k];
     }
-    HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
-
-    // ---- corners ----
-    const int rpw = o.rows_per_walk > 0 ? o.rows_per_walk : 1;
-    if (mode == 0)
-        stats ? launch_corners<0, true>(ctx, U.dp, B, g, U.ctl, nwords, nb, rpw)
-              : launch_corners<0, false>(ctx, U.dp, B, g, U.ctl, nwords, nb, rpw);
-    else
-        stats ? launch_corners<1, true>(ctx, U.dp, B, g, U.ctl, nwords, nb, rpw)
-              : launch_corners<1, false>(ctx, U.dp, B, g, U.ctl, nwords, nb, rpw);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
-
-    // ---- classify + scan (+ list of words that emit) ----
-    k_classify<<<grid_for(nwords, 256), 256, 0, ctx->stream>>>(g, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
-                                                                 ctx->counts.as<uint4>(), nwords);
-    HIP_TRY(hipGetLastError());
-    if ((rc = scan4(ctx, ctx->counts.as<uint4>(), ctx->offs.as<uint4>(), nwords, U.ctl->totals,
-                    ctx->actw.as<uint32_t>())))
-        return rc;
-    HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
-
-    // ---- emit + bisect, sized from earlier calls ----
-    Ctl *hc = (Ctl *)ctx->ctl_host.p;
-    if (host) {
-        if ((rc = run_output_host(ctx, P, frames, U, g, m, mode, iters, nwords, st.ms_host_callbacks))) return rc;
-    } else {
-        uint64_t vcap = ctx->v_hint, tcap = ctx->t_hint;
-        if ((rc = take_outputs(ctx, m, vcap, tcap))) return rc;
-        if ((rc = run_output(ctx, U, B, g, m, mode, stats, iters, vcap, tcap, nwords))) return rc;
-        HIP_TRY(hipMemcpyAsync(hc, U.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        if (hc->inside) {
-            set_error("solid is true outside of bounds");
-            return MC3D_E_INSIDE_BOUNDARY;
+    for (size_t k = 0; k < alone.size(); k++) {
+        std::vector<Job> one;
+        one.push_back(std::move(alone[k]));
+        std::vector<mc3d_mesh *> ms;
+        int rc = run_call(ctx, one, o, ms);
+        if (rc) {
+            set_error("solid " + std::to_string(alone_at[k]) + " of the batch: " + mc3d_last_error());
+            return fail(rc);
         }
-        const uint64_t nv = hc->totals[0], nt = hc->totals[1];
-        if (nv >= (1ull << 32) || nt >= (1ull << 32)) {
-            set_error("mesh too large for 32-bit vertex indices");
-            return MC3D_E_UNSUPPORTED;
-        }
-        if (nv > vcap || nt > tcap) {  // grow and run the output stages again
-            ctx->v_hint = std::max(ctx->v_hint, nv + nv / 4 + 1024);
-            ctx->t_hint = std::max(ctx->t_hint, nt + nt / 4 + 1024);
-            vcap = ctx->v_hint, tcap = ctx->t_hint;
-            if ((rc = take_outputs(ctx, m, vcap, tcap))) return rc;
-            if ((rc = run_output(ctx, U, B, g, m, mode, stats, iters, vcap, tcap, nwords))) return rc;
-            HIP_TRY(hipMemcpyAsync(hc, U.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
-            HIP_TRY(hipStreamSynchronize(ctx->stream));
-        }
+        got[alone_at[k]] = ms[0];
     }
-    const uint64_t nv = hc->totals[0], nt = hc->totals[1];
-    m->nv = nv;
-    m->nt = nt;
-
-    st.corners = (uint64_t)g.nx * g.ny * g.nz;
-    st.vertices = nv;
-    st.triangles = nt;
-    st.bisection_evals = iters > 0 ? nv * (uint64_t)(iters + 1) : 0;
-    st.bricks_active = hc->nact_bricks;
-    st.guard_evals = hc->gstats[0];
-    st.guard_rechecks = hc->gstats[1];
-    st.guard_mismatches = hc->gstats[2];
-    st.guard_violations = hc->gstats[3];
-    st.ms_plan = elapsed(ctx->ev[0], ctx->ev[1]);
-    st.ms_corners = elapsed(ctx->ev[1], ctx->ev[2]);
-    st.ms_classify = elapsed(ctx->ev[2], ctx->ev[3]);
-    st.ms_emit = elapsed(ctx->ev[4], ctx->ev[5]);
-    st.ms_bisect = elapsed(ctx->ev[5], ctx->ev[6]);
-    st.ms_device = elapsed(ctx->ev[0], ctx->ev[6]);
-    if (!o.device_only) {
-        auto t0 = std::chrono::steady_clock::now();
-        m->hverts.resize(nv * 3);
-        m->htris.resize(nt * 3);
-        if (nv) HIP_TRY(hipMemcpy(m->hverts.data(), m->dverts->p, nv * 24, hipMemcpyDeviceToHost));
-        if (nt) HIP_TRY(hipMemcpy(m->htris.data(), m->dtris->p, nt * 12, hipMemcpyDeviceToHost));
-        m->on_host = true;
-        st.ms_d2h = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    }
-    st.ms_call = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_call).count();
-    *out = mesh.release();
+    for (size_t i = 0; i < n; i++) out[i] = got[i];
     return MC3D_OK;
 }
 
@@ -728,15 +1054,18 @@ int mc3d_mesh_copy(mc3d_mesh *m, double *verts, uint32_t *tris) {
         return MC3D_OK;
     }
     HIP_TRY(hipSetDevice(m->device));
-    if (verts && m->nv) HIP_TRY(hipMemcpy(verts, m->dverts->p, m->nv * 24, hipMemcpyDeviceToHost));
-    if (tris && m->nt) HIP_TRY(hipMemcpy(tris, m->dtris->p, m->nt * 12, hipMemcpyDeviceToHost));
+    if (verts && m->nv)
+        HIP_TRY(hipMemcpy(verts, m->buf->verts->as<double>() + 3 * m->voff, m->nv * 24, hipMemcpyDeviceToHost));
+    if (tris && m->nt)
+        HIP_TRY(hipMemcpy(tris, m->buf->tris->as<uint32_t>() + 3 * m->toff, m->nt * 12, hipMemcpyDeviceToHost));
     return MC3D_OK;
 }
 
 int mc3d_mesh_device_ptrs(const mc3d_mesh *m, const double **verts, const uint32_t **tris) {
     if (!m) return MC3D_E_INVALID_ARG;
-    if (verts) *verts = m->dverts ? m->dverts->as<double>() : nullptr;
-    if (tris) *tris = m->dtris ? m->dtris->as<uint32_t>() : nullptr;
+    const bool have = m->buf && m->buf->verts && m->buf->tris;
+    if (verts) *verts = have ? m->buf->verts->as<double>() + 3 * m->voff : nullptr;
+    if (tris) *tris = have ? m->buf->tris->as<uint32_t>() + 3 * m->toff : nullptr;
     return MC3D_OK;
 }
 
@@ -760,26 +1089,30 @@ int mc3d_contains(mc3d_ctx *ctx, const mc3d_solid *s, const double *pts, size_t 
     }
     if (n == 0) return MC3D_OK;
     HIP_TRY(hipSetDevice(ctx->device));
-    Program P;
+    std::vector<Job> jobs(1);
+    Job &j = jobs[0];
+    j.s = s;
     std::string msg;
-    if (!lower_solid(*s->ptr, P, msg)) {
+    if (!lower_solid(*s->ptr, j.P, msg)) {
         set_error(msg);
         return MC3D_E_UNSUPPORTED;
     }
+    j.frames = host_frames(j.P);
+    j.g = Grid{nullptr, nullptr, nullptr, 1, 1, 1, 1};
     int rc;
     Upload U;
-    if ((rc = upload(ctx, P, nullptr, U))) return rc;
+    if ((rc = upload(ctx, jobs, 0, 0, U))) return rc;
     if ((rc = ctx->pts.ensure(n * 24)) || (rc = ctx->out8.ensure(n))) return rc;
     HIP_TRY(hipMemcpyAsync(ctx->pts.p, pts, n * 24, hipMemcpyHostToDevice, ctx->stream));
-    if (!P.host_atoms.empty()) {
+    if (!j.P.host_atoms.empty()) {
         double ms_cb = 0;
-        const Grid none{nullptr, nullptr, nullptr, 1, 1, 1, 1};
-        if ((rc = host_round(ctx, P, host_frames(P), U, none, ctx->pts.as<double>(), n, ms_cb))) return rc;
+        if ((rc = host_round(ctx, j, U, 0, ctx->pts.as<double>(), n, ms_cb))) return rc;
     }
+    const DevProg &dp = U.hsegs[0].P;
     if (precision == MC3D_PREC_FP32_GUARD)
-        k_contains<1><<<grid_for(n, 256), 256, 0, ctx->stream>>>(U.dp, ctx->pts.as<double>(), ctx->out8.as<uint8_t>(), n);
+        k_contains<1><<<grid_for(n, 256), 256, 0, ctx->stream>>>(dp, ctx->pts.as<double>(), ctx->out8.as<uint8_t>(), n);
     else
-        k_contains<0><<<grid_for(n, 256), 256, 0, ctx->stream>>>(U.dp, ctx->pts.as<double>(), ctx->out8.as<uint8_t>(), n);
+        k_contains<0><<<grid_for(n, 256), 256, 0, ctx->stream>>>(dp, ctx->pts.as<double>(), ctx->out8.as<uint8_t>(), n);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, ctx->out8.p, n, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -86,7 +86,7 @@ def NewMeshRect(min, max) -> Mesh:
     return Mesh.from_triangles(np.array(tris))
 
 
-def _options(precision, cull, device_only=False, count_mismatch=False, brick=0, rows_per_walk=0):
+def _options(precision, cull, device_only=False, count_mismatch=False, brick=0, rows_per_walk=0, waves_per_brick=0):
     o = _lib.mc3d_options()
     o.precision = precision
     o.cull = 1 if cull else 0
@@ -94,6 +94,7 @@ def _options(precision, cull, device_only=False, count_mismatch=False, brick=0, 
     o.count_mismatch = 1 if count_mismatch else 0
     o.brick = brick
     o.rows_per_walk = rows_per_walk
+    o.waves_per_brick = waves_per_brick
     return o
 
 
@@ -108,11 +109,13 @@ def MarchingCubesSearch(
     count_mismatch: bool = False,
     brick: int = 0,
     rows_per_walk: int = 0,
+    waves_per_brick: int = 0,
 ) -> Mesh:
     """model3d.MarchingCubesSearch (mc.go:45-49) on the GPU.
 
     precision: MC3D_PREC_FP32_GUARD (default) or MC3D_PREC_FP64; cull: per-brick
-    culling (brick = its y/z edge in corners, 0 = default); count_mismatch:
+    culling (brick = its y/z edge in corners, 0 = default; rows_per_walk and
+    waves_per_brick shape the corner kernel); count_mismatch:
     also gather float32-vs-float64 statistics into ``Mesh.stats``.  Every
     setting yields the same triangles."""
     L = _lib.lib()
@@ -121,25 +124,78 @@ def MarchingCubesSearch(
     h = s._build(b)
     out = ctypes.c_void_p()
     try:
-        opts = _options(precision, cull, count_mismatch=count_mismatch, brick=brick, rows_per_walk=rows_per_walk)
-        rc = L.mc3d_marching_cubes_search(ctx, h, float(delta), int(iters), ctypes.byref(opts), ctypes.byref(out))
-        if rc != _lib.MC3D_OK:
-            msg = _lib.last_error()
-            if rc in (_lib.MC3D_E_INVALID_BOUNDS, _lib.MC3D_E_INSIDE_BOUNDARY, _lib.MC3D_E_NOT_ON_EDGE):
-                raise Panic(rc, msg)
-            raise _lib.MC3DError(rc, msg)
-        nv, nt = ctypes.c_size_t(), ctypes.c_size_t()
-        _lib.check(L.mc3d_mesh_counts(out, ctypes.byref(nv), ctypes.byref(nt)), "mc3d_mesh_counts")
-        verts = np.empty((nv.value, 3), dtype=np.float64)
-        faces = np.empty((nt.value, 3), dtype=np.uint32)
-        _lib.check(L.mc3d_mesh_copy(out, verts.ctypes.data, faces.ctypes.data), "mc3d_mesh_copy")
-        st = _lib.mc3d_stats()
-        _lib.check(L.mc3d_mesh_stats(out, ctypes.byref(st)), "mc3d_mesh_stats")
-        return Mesh(verts, faces, st.as_dict())
+        opts = _options(precision, cull, count_mismatch=count_mismatch, brick=brick, rows_per_walk=rows_per_walk,
+                        waves_per_brick=waves_per_brick)
+        _raise_for(L.mc3d_marching_cubes_search(ctx, h, float(delta), int(iters), ctypes.byref(opts),
+                                                ctypes.byref(out)))
+        return _fetch(L, out)
     finally:
         if out.value:
             L.mc3d_mesh_free(out)
         b.release(h)
+        b.close()
+
+
+def _raise_for(rc: int):
+    """A reference panic as Panic (same message), anything else as MC3DError."""
+    if rc != _lib.MC3D_OK:
+        msg = _lib.last_error()
+        if rc in (_lib.MC3D_E_INVALID_BOUNDS, _lib.MC3D_E_INSIDE_BOUNDARY, _lib.MC3D_E_NOT_ON_EDGE):
+            raise Panic(rc, msg)
+        raise _lib.MC3DError(rc, msg)
+
+
+def _fetch(L, out) -> Mesh:
+    nv, nt = ctypes.c_size_t(), ctypes.c_size_t()
+    _lib.check(L.mc3d_mesh_counts(out, ctypes.byref(nv), ctypes.byref(nt)), "mc3d_mesh_counts")
+    verts = np.empty((nv.value, 3), dtype=np.float64)
+    faces = np.empty((nt.value, 3), dtype=np.uint32)
+    _lib.check(L.mc3d_mesh_copy(out, verts.ctypes.data, faces.ctypes.data), "mc3d_mesh_copy")
+    st = _lib.mc3d_stats()
+    _lib.check(L.mc3d_mesh_stats(out, ctypes.byref(st)), "mc3d_mesh_stats")
+    return Mesh(verts, faces, st.as_dict())
+
+
+def MarchingCubesSearchBatch(
+    solids,
+    deltas,
+    iters,
+    *,
+    precision: int = _lib.MC3D_PREC_FP32_GUARD,
+    cull: bool = True,
+    device: int = 0,
+    count_mismatch: bool = False,
+    brick: int = 0,
+    rows_per_walk: int = 0,
+    waves_per_brick: int = 0,
+) -> list:
+    """MarchingCubesSearch (mc.go:45-49) of every solid in one pass of the GPU
+    pipeline; ``deltas`` and ``iters`` are per solid (or one value for all).
+    Returns one Mesh per solid, each identical to MarchingCubesSearch's."""
+    solids = list(solids)
+    n = len(solids)
+    deltas = [float(d) for d in (deltas if hasattr(deltas, "__len__") else [deltas] * n)]
+    iters = [int(i) for i in (iters if hasattr(iters, "__len__") else [iters] * n)]
+    if len(deltas) != n or len(iters) != n:
+        raise ValueError("deltas and iters need one entry per solid")
+    L = _lib.lib()
+    ctx = _lib.context(device)
+    b = LibBackend()
+    hs = [s._build(b) for s in solids]
+    outs = (ctypes.c_void_p * max(1, n))()
+    try:
+        opts = _options(precision, cull, count_mismatch=count_mismatch, brick=brick, rows_per_walk=rows_per_walk,
+                        waves_per_brick=waves_per_brick)
+        _raise_for(L.mc3d_marching_cubes_search_batch(
+            ctx, (ctypes.c_void_p * max(1, n))(*hs), (ctypes.c_double * max(1, n))(*deltas),
+            (ctypes.c_int32 * max(1, n))(*iters), n, ctypes.byref(opts), outs))
+        return [_fetch(L, outs[i]) for i in range(n)]
+    finally:
+        for i in range(n):
+            if outs[i]:
+                L.mc3d_mesh_free(outs[i])
+        for h in hs:
+            b.release(h)
         b.close()
 
 

@@ -9,6 +9,8 @@ from __future__ import annotations
 
 import math
 
+import numpy as np
+
 from .solids import (
     Capsule,
     ClipSolid,
@@ -88,7 +90,10 @@ def random_csg(seed: int, n_prims: int = 64, p_union: float = 0.7, p_sub: float 
     while len(nodes) > 1:
         i = rng.randint(0, len(nodes) - 1)
         a, ca = nodes.pop(i)
-        j = min(range(len(nodes)), key=lambda k: sum((nodes[k][1][t] - ca[t]) ** 2 for t in range(3)))
+        # nearest remaining centre (first of equals): ((dx^2 + dy^2) + dz^2)
+        d = np.array([c for _, c in nodes], dtype=np.float64) - np.array(ca, dtype=np.float64)
+        d = d * d
+        j = int(np.argmin((d[:, 0] + d[:, 1]) + d[:, 2]))
         b, cb = nodes.pop(j)
         u = rng.uniform()
         if u < p_union:
@@ -110,6 +115,30 @@ CSG64_CONFIG = dict(name="csg64_seed2_d0.005", seed=2, n_prims=64, delta=0.005, 
 
 
 COLLIDER_CONFIG = dict(name="collider_csg64_d0.02_256cubed", seed=2, n_prims=64, mesh_delta=0.02, iters=8)
+
+
+# Many solids of uneven size in one call: primitive counts ~U{4..128}, grid
+# edges ~U{32..256} cells, iters ~U{4..8}, each a clipped random CSG tree.
+BATCH_CONFIG = dict(name="batch256_seed4", seed=4, n_solids=256, prims=(4, 128), cells=(32, 256), iters=(4, 8))
+
+
+def batch_solids(seed: int = 4, n_solids: int = 256, prims=(4, 128), cells=(32, 256), iters=(4, 8)):
+    """(solids, deltas, iters) of the batch workload: solid i is
+    random_csg(its own seed, U{prims}) and its delta gives U{cells} cells
+    along its longest bounds axis."""
+    rng = SplitMix64(seed)
+    solids, deltas, its = [], [], []
+    for _ in range(n_solids):
+        n_prims = rng.randint(*prims)
+        n_cells = rng.randint(*cells)
+        it = rng.randint(*iters)
+        s = random_csg(rng.next_u64(), n_prims)
+        mn, mx = s.Min(), s.Max()
+        extent = max(mx[k] - mn[k] for k in range(3))
+        solids.append(s)
+        deltas.append(extent / n_cells)
+        its.append(it)
+    return solids, deltas, its
 
 
 def sphere_solid():

@@ -51,6 +51,9 @@ def test_mesh_parity_catalogue(name, precision):
     got = m3.MarchingCubesSearch(s, 0.04, 8, precision=precision)
     assert len(want) > 0
     assert same_triangles(got.triangles(), want)
+    # one wave per brick, four rows per walk, small bricks
+    got = m3.MarchingCubesSearch(s, 0.04, 8, precision=precision, brick=2, rows_per_walk=4, waves_per_brick=1)
+    assert same_triangles(got.triangles(), want)
 
 
 @pytest.mark.parametrize("iters", [0, 1, 8])
@@ -91,10 +94,12 @@ def test_culling_and_brick_sizes_do_not_change_results():
     print("csg64 d=0.01 cull/brick sweep", flush=True)
     ref = canonical(m3.MarchingCubesSearch(s, 0.01, 8, precision=FP64, cull=False).triangles())
     for precision in (FP64, GUARD):
-        for cull, brick, rpw in ((False, 0, 0), (True, 1, 1), (True, 2, 2), (True, 4, 4), (True, 8, 1),
-                                 (True, 16, 4), (True, 3, 2)):
-            got = m3.MarchingCubesSearch(s, 0.01, 8, precision=precision, cull=cull, brick=brick, rows_per_walk=rpw)
-            assert np.array_equal(canonical(got.triangles()), ref), (precision, cull, brick, rpw)
+        for cull, brick, rpw, wpb in ((False, 0, 0, 0), (True, 1, 1, 4), (True, 2, 2, 1), (True, 4, 4, 2),
+                                      (True, 8, 1, 1), (True, 16, 4, 4), (True, 3, 2, 2), (True, 4, 2, 1),
+                                      (True, 8, 4, 1)):
+            got = m3.MarchingCubesSearch(s, 0.01, 8, precision=precision, cull=cull, brick=brick, rows_per_walk=rpw,
+                                         waves_per_brick=wpb)
+            assert np.array_equal(canonical(got.triangles()), ref), (precision, cull, brick, rpw, wpb)
 
 
 @pytest.mark.parametrize("name", ["csg64", "sphere", "capsule", "cylinder", "csg", "transform_chain"])
