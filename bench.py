@@ -11,6 +11,8 @@ A step is one call of the library on the workload, results left in HBM
             along its longest axis, iters 8
   batch     256 seeded CSG solids of 4..128 primitives on 32..256-cell grids,
             iters 4..8, all in one mc3d_marching_cubes_search_batch call
+  large     a seeded 512-primitive CSG tree in [-1, 1]^3 at delta 0.002
+            (1002^3 ~ 1.0G corners), iters 8
 
 Prints one JSON line: the metric, the per-kernel times (HIP events on the
 library's stream), the roofline of the dominant kernel (algorithmic flops or
@@ -48,6 +50,7 @@ DEFAULT_RUN = {
     "sphere": dict(precision="fp64", cull=False, brick=0, rows_per_walk=1, waves_per_brick=4),
     "collider": dict(precision="fp64", cull=True, brick=4, rows_per_walk=1, waves_per_brick=4),
     "batch": dict(precision="fp64", cull=True, brick=4, rows_per_walk=1, waves_per_brick=2),
+    "large": dict(precision="fp64", cull=True, brick=4, rows_per_walk=2, waves_per_brick=4),
 }
 PLAIN_RUN = dict(precision="fp64", cull=False, brick=0, rows_per_walk=1, waves_per_brick=4)
 
@@ -81,6 +84,10 @@ def workload(name):
         solid, delta, ntri = workloads.collider_solid(c["seed"], c["n_prims"], c["mesh_delta"])
         c["delta"], c["mesh_triangles"] = delta, ntri
         return dict(solids=[solid], deltas=[delta], iters=[c["iters"]], cfg=c)
+    if name == "large":
+        c = dict(workloads.LARGE_CONFIG, cpu_delta=0.008)
+        return dict(solids=[workloads.large_solid(c["seed"], c["n_prims"])], deltas=[c["delta"]],
+                    iters=[c["iters"]], cfg=c)
     if name == "batch":
         c = workloads.BATCH_CONFIG
         solids, deltas, iters = workloads.batch_solids(c["seed"], c["n_solids"], c["prims"], c["cells"], c["iters"])
@@ -211,7 +218,9 @@ def cpu_baseline(work):
     corners = ntris = 0
     wall = 0.0
     done = 0
+    coarse = cfg.get("cpu_delta")  # the full grid takes the C restatement minutes: a coarser grid of the same solid
     for s, d, it in zip(work["solids"], work["deltas"], work["iters"]):
+        d = coarse or d
         t0 = time.perf_counter()
         tris, info = oracle_search(s, d, it, threads=threads)
         wall += time.perf_counter() - t0
@@ -222,6 +231,8 @@ def cpu_baseline(work):
             break
     n = len(work["solids"])
     what = "the full workload once" if done == n else f"the first {done} of the {n} solids"
+    if coarse:
+        what = f"the same solid at delta {coarse} instead of {work['deltas'][0]}"
     return {
         "value": corners / wall,
         "unit": "corner evals/s",

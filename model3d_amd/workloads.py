@@ -141,6 +141,15 @@ def batch_solids(seed: int = 4, n_solids: int = 256, prims=(4, 128), cells=(32, 
     return solids, deltas, its
 
 
+# One large solid: a 512-primitive tree in [-1, 1]^3 at delta 0.002
+# (1002^3 ~ 1.0G corners, ~50M triangles).
+LARGE_CONFIG = dict(name="csg512_seed9_d0.002", seed=9, n_prims=512, delta=0.002, iters=8)
+
+
+def large_solid(seed: int = 9, n_prims: int = 512):
+    return random_csg(seed, n_prims)
+
+
 def sphere_solid():
     return Sphere((0.0, 0.0, 0.0), 1.0)
 

@@ -255,3 +255,27 @@ def test_invalid_bounds_panics():
     with pytest.raises(m3.Panic) as e:
         m3.MarchingCubesSearch(m3.Sphere((0, 0, 0), float("nan")), 0.1, 0)
     assert "invalid bounds" in str(e.value)
+
+
+def test_large_workload_properties():
+    """The 1G-corner workload (512 primitives, delta 0.002): float32 guard and
+    float64 agree bit for bit, every triangle index is in range, and the mesh
+    is a closed, consistently oriented surface (each directed edge once, with
+    its reverse) -- checked on the device's exact vertex ids."""
+    c = workloads.LARGE_CONFIG
+    s = workloads.large_solid(c["seed"], c["n_prims"])
+    print("large: 512 primitives at delta 0.002", flush=True)
+    a = m3.MarchingCubesSearch(s, c["delta"], c["iters"], precision=GUARD)
+    assert a.stats["corners"] > 10**9 and a.stats["triangles"] > 10**7
+    b = m3.MarchingCubesSearch(s, c["delta"], c["iters"], precision=FP64)
+    assert np.array_equal(a.vertices.view(np.uint64), b.vertices.view(np.uint64))
+    assert np.array_equal(a.faces, b.faces)
+    f = a.faces.astype(np.int64)
+    assert f.max() < len(a.vertices)
+    nv = len(a.vertices)
+    e = np.stack([f, np.roll(f, -1, axis=1)], axis=2).reshape(-1, 2)
+    key = e[:, 0] * nv + e[:, 1]
+    rev = e[:, 1] * nv + e[:, 0]
+    key.sort()
+    assert np.all(key[1:] != key[:-1])  # every directed edge once
+    assert np.all(np.isin(rev, key))  # and its reverse is there
