@@ -48,7 +48,7 @@ CPU_SAMPLE_SECONDS = 10.0
 DEFAULT_RUN = {
     "csg64": dict(precision="fp64", cull=True, brick=4, rows_per_walk=2, waves_per_brick=4),
     "sphere": dict(precision="fp64", cull=False, brick=0, rows_per_walk=1, waves_per_brick=4),
-    "collider": dict(precision="fp64", cull=True, brick=4, rows_per_walk=1, waves_per_brick=4),
+    "collider": dict(precision="fp64", cull=False, brick=0, rows_per_walk=1, waves_per_brick=4),
     "batch": dict(precision="fp64", cull=True, brick=4, rows_per_walk=1, waves_per_brick=2),
     "large": dict(precision="fp64", cull=True, brick=4, rows_per_walk=2, waves_per_brick=4),
 }
@@ -305,6 +305,9 @@ def load_traffic(workload_name, run, kernel):
     if s.get("config") != run:
         return None, f"summary measured at {s.get('config')}, not at this configuration"
     names = [k for k in s.get("kernels", {}) if k.split("<")[0] == kernel]
+    if len(names) > 1:  # instantiations of several precisions: the run's own (template argument MODE)
+        mode = "1" if run["precision"] == "fp32_guard" else "0"
+        names = [k for k in names if k.split("<")[1].split(",")[0].strip() == mode]
     if len(names) != 1:
         return None, f"kernel {kernel} not (uniquely) in the summary"
     return traffic_bytes(s, names[0]), os.path.relpath(path, ROOT)
