@@ -22,8 +22,28 @@ struct DevProg {
     const uint8_t *__restrict__ host_vals;
     uint64_t host_stride;
     uint32_t ncode;
-    uint32_t pad;
+    uint32_t nf64;  // float64 params (without the padding after them)
 };
+
+// The same pointer, known to address global memory: pointers loaded from
+// memory are generic, and generic (flat) accesses go through the vector
+// memory pipeline even when they would hit LDS; a cast from the global
+// address space lets the compiler emit global loads instead.
+template <class T>
+__device__ __forceinline__ T *global_ptr(T *p) {
+    return (T *)(__attribute__((address_space(1))) T *)p;
+}
+
+__device__ __forceinline__ DevProg as_global(DevProg P) {
+    P.code = global_ptr(P.code);
+    P.f64 = global_ptr(P.f64);
+    P.f32 = global_ptr(P.f32);
+    P.frames = global_ptr(P.frames);
+    P.boxes = global_ptr(P.boxes);
+    P.colliders = global_ptr(P.colliders);
+    P.host_vals = global_ptr(P.host_vals);
+    return P;
+}
 
 // Go math.Min(c, m) == m  <=>  m == -Inf || c >= m   (and the Max twin); this
 // is Rect.Contains / InBounds (shapes.go:171-173, bounder.go:23-27) including
@@ -44,12 +64,15 @@ __device__ __forceinline__ bool collider_contains(const DevCollider &C, const do
     const double fv = (x * C.v[0] + y * C.v[1] + z * C.v[2] - C.v0) * C.inv_cell;
     if (!(fu >= 0 && fu < C.nu && fv >= 0 && fv < C.nv)) return false;  // no triangle can be hit
     const uint32_t cell = (uint32_t)fv * (uint32_t)C.nu + (uint32_t)fu;
-    const uint32_t e = C.cell_start[cell + 1];
-    if (candidates) *candidates += e - C.cell_start[cell];
+    const uint32_t *__restrict__ cs = global_ptr(C.cell_start);
+    const uint32_t *__restrict__ ct = global_ptr(C.cell_tris);
+    const double *__restrict__ rec = global_ptr(C.rec);
+    const uint32_t e = cs[cell + 1];
+    if (candidates) *candidates += e - cs[cell];
     const double Dx = 0.5224892708603626, Dy = 0.10494477243214506, Dz = 0.43558938446126527;
     uint32_t hits = 0;
-    for (uint32_t i = C.cell_start[cell]; i < e; i++) {
-        const double *r = C.rec + (size_t)RAY_REC * C.cell_tris[i];
+    for (uint32_t i = cs[cell]; i < e; i++) {
+        const double *r = rec + (size_t)RAY_REC * ct[i];
         const double ox = x - r[0], oy = y - r[1], oz = z - r[2];
         const double bary1 = r[6] * (ox * r[3] + oy * r[4] + oz * r[5]);
         if (bary1 < 0 || bary1 > 1) continue;

@@ -103,11 +103,14 @@ __device__ __forceinline__ uint32_t wave_seg(const uint32_t *__restrict__ table,
 }
 
 // Segment si by value (scalar registers): loaded for a call of several,
-// the kernel-argument copy for a call of one.
+// the kernel-argument copy for a call of one; its pointers marked global.
 template <bool MULTI>
 __device__ __forceinline__ Seg pick_seg(const Seg *__restrict__ segs, const Seg &seg0, uint32_t si) {
-    if (MULTI) return segs[si];
-    return seg0;
+    Seg s = MULTI ? segs[si] : seg0;
+    s.P = as_global(s.P);
+    s.g.xs = global_ptr(s.g.xs), s.g.ys = global_ptr(s.g.ys), s.g.zs = global_ptr(s.g.zs);
+    s.B.info = global_ptr(s.B.info), s.B.code = global_ptr(s.B.code);
+    return s;
 }
 
 __device__ __forceinline__ size_t word_index(const Grid &g, int wx, int y, int z) {
@@ -171,14 +174,14 @@ __device__ __forceinline__ void wave_sync() {
 // far beyond float64 rounding); "true" is proven for untransformed boxes and
 // bounds checks (exact comparisons) and spheres (farthest brick corner inside
 // the threshold with a relative margin far above the rounding of the sum).
-__device__ __forceinline__ int classify_atom(const DevProg &P, const AtomBox *boxes, uint32_t pc, Insn in,
+__device__ __forceinline__ int classify_atom(const double *f64, const AtomBox *boxes, uint32_t pc, Insn in,
                                              const double *lo, const double *hi) {
     const AtomBox bx = boxes[pc];
     if (hi[0] < bx.lo[0] || lo[0] > bx.hi[0] || hi[1] < bx.lo[1] || lo[1] > bx.hi[1] || hi[2] < bx.lo[2] ||
         lo[2] > bx.hi[2])
         return 0;
     if (insn_frame(in.x) == 0) {
-        const double *a = P.f64 + in.y;
+        const double *a = f64 + in.y;
         const uint32_t kind = insn_kind(in.x);
         if (kind == A_RECT) {
             if (in_lo(lo[0], a[0]) && in_lo(lo[1], a[1]) && in_lo(lo[2], a[2]) && in_hi(hi[0], a[3]) &&
@@ -202,11 +205,84 @@ constexpr int PLAN_STACK = 33;
 
 constexpr size_t PLAN_STACK_BYTES = PLAN_BLOCK * PLAN_STACK * sizeof(uint32_t);
 
+// Folds the postfix program against one brick's box (lo, hi): a stack of
+// {false, true, expression starting at output position n}; the folded code
+// goes to o[0, cap) (`over` if it needs more).  S is the thread's stack.
+__device__ __forceinline__ void plan_fold(const Insn *code, const AtomBox *boxes, const double *f64, uint32_t ncode,
+                                          uint32_t *S, const double *lo, const double *hi, Insn *o, uint32_t cap,
+                                          uint32_t &n, bool &over) {
+    int sp = 0;
+    for (uint32_t pc = 0; pc < ncode; ++pc) {
+        const Insn in = code[pc];
+        const uint32_t op = insn_op(in.x);
+        if (op == OP_ATOM) {
+            int c = classify_atom(f64, boxes, pc, in, lo, hi);
+            if (c == 2) {
+                if (n < cap) o[n] = in;
+                else over = true;
+                S[sp++] = TAG_EXPR | n;
+                n++;
+            } else {
+                S[sp++] = c ? TAG_TRUE : TAG_FALSE;
+            }
+        } else if (op == OP_TRUE || op == OP_FALSE) {
+            S[sp++] = op == OP_TRUE ? TAG_TRUE : TAG_FALSE;
+        } else if (op == OP_NOT) {
+            uint32_t a = S[sp - 1];
+            if (a == TAG_TRUE) S[sp - 1] = TAG_FALSE;
+            else if (a == TAG_FALSE) S[sp - 1] = TAG_TRUE;
+            else {
+                if (n < cap) o[n] = in;
+                else over = true;
+                n++;
+            }
+        } else {
+            const uint32_t bb = S[--sp], aa = S[--sp];
+            const bool aE = (aa & TAG_MASK) == TAG_EXPR, bE = (bb & TAG_MASK) == TAG_EXPR;
+            const bool aT = aa == TAG_TRUE, aF = aa == TAG_FALSE, bT = bb == TAG_TRUE, bF = bb == TAG_FALSE;
+            // start of the operands' code (constants emitted nothing)
+            const uint32_t first = aE ? (aa & ~TAG_MASK) : (bE ? (bb & ~TAG_MASK) : n);
+            uint32_t res;
+            bool emit = false, emit_not = false;
+            if (op == OP_OR) {
+                if (aT || bT) res = TAG_TRUE, n = first;
+                else if (aF) res = bb;
+                else if (bF) res = aa;
+                else res = aa, emit = true;
+            } else if (op == OP_AND) {
+                if (aF || bF) res = TAG_FALSE, n = first;
+                else if (aT) res = bb;
+                else if (bT) res = aa;
+                else res = aa, emit = true;
+            } else if (op == OP_ANDNOT) {  // a & !b
+                if (aF || bT) res = TAG_FALSE, n = first;
+                else if (bF) res = aa;
+                else if (aT) res = bb, emit_not = true;  // b's code is last
+                else res = aa, emit = true;
+            } else {  // OP_RANDNOT: !a & b
+                if (aT || bF) res = TAG_FALSE, n = first;
+                else if (aF) res = bb;
+                else if (bT) res = aa, emit_not = true;  // a's code is last
+                else res = aa, emit = true;
+            }
+            if (emit || emit_not) {
+                if (n < cap) o[n] = emit ? in : Insn{OP_NOT, 0};
+                else over = true;
+                n++;
+            }
+            S[sp++] = res;
+        }
+    }
+}
+
+
+
 // One thread per brick walks the postfix program with a stack of
 // {false, true, expression starting at output position n} and writes the
 // folded program into the brick's slot of `cap` instructions.  A brick whose
 // folded code ever needs more than `cap` slots uses the unfolded program.
-// With `stage`, the program and its atom boxes are first copied to LDS.
+// With `stage`, the program, its atom boxes and its params are first copied
+// to LDS.  The per-brick statistics are reduced per wave.
 template <bool MULTI>
 __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
                                                      const uint32_t *__restrict__ brick_seg, uint64_t nbricks_all,
@@ -216,6 +292,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(const Seg *__restrict__ seg
     uint32_t *S = (uint32_t *)plan_smem + threadIdx.x * PLAN_STACK;
     Insn *sc = (Insn *)(plan_smem + PLAN_STACK_BYTES);
     uint32_t staged = ~0u;  // segment whose program is in LDS
+    const int lane = threadIdx.x & 63;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t b_end = (nbricks_all + blockDim.x - 1) / blockDim.x * blockDim.x;  // whole waves take part
     for (uint64_t gb = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gb < b_end; gb += stride) {
@@ -227,18 +304,22 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(const Seg *__restrict__ seg
         const Bricks &B = sg.B;
         const Insn *code = P.code;
         const AtomBox *boxes = P.boxes;
+        const double *f64 = P.f64;
         if (sg.stage) {  // block-uniform
             AtomBox *sb = (AtomBox *)(sc + P.ncode);
+            double *sf = (double *)(sb + P.ncode);
             if (staged != si) {
                 __syncthreads();  // the previous segment's walks are done with the LDS copy
                 for (uint32_t i = threadIdx.x; i < P.ncode; i += blockDim.x) sc[i] = P.code[i], sb[i] = P.boxes[i];
+                for (uint32_t i = threadIdx.x; i < P.nf64; i += blockDim.x) sf[i] = P.f64[i];
                 __syncthreads();
                 staged = si;
             }
-            code = sc, boxes = sb;
+            code = sc, boxes = sb, f64 = sf;
         }
         const uint64_t b = gb - sg.brick0;  // local brick
-        bool active = false;
+        bool active = false, full = false;
+        uint32_t len = 0;
         if (gb < nbricks_all && b < sg.nbricks) {
             const uint32_t r = (uint32_t)b / (uint32_t)g.W;
             const int bx = (int)((uint32_t)b - r * (uint32_t)g.W);
@@ -250,68 +331,13 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(const Seg *__restrict__ seg
             Insn *o = out + gb * cap;
             uint32_t n = 0;
             bool over = false;
-            int sp = 0;
-            for (uint32_t pc = 0; pc < P.ncode; ++pc) {
-                const Insn in = code[pc];
-                const uint32_t op = insn_op(in.x);
-                if (op == OP_ATOM) {
-                    int c = classify_atom(P, boxes, pc, in, lo, hi);
-                    if (c == 2) {
-                        if (n < cap) o[n] = in;
-                        else over = true;
-                        S[sp++] = TAG_EXPR | n;
-                        n++;
-                    } else {
-                        S[sp++] = c ? TAG_TRUE : TAG_FALSE;
-                    }
-                } else if (op == OP_TRUE || op == OP_FALSE) {
-                    S[sp++] = op == OP_TRUE ? TAG_TRUE : TAG_FALSE;
-                } else if (op == OP_NOT) {
-                    uint32_t a = S[sp - 1];
-                    if (a == TAG_TRUE) S[sp - 1] = TAG_FALSE;
-                    else if (a == TAG_FALSE) S[sp - 1] = TAG_TRUE;
-                    else {
-                        if (n < cap) o[n] = in;
-                        else over = true;
-                        n++;
-                    }
-                } else {
-                    const uint32_t bb = S[--sp], aa = S[--sp];
-                    const bool aE = (aa & TAG_MASK) == TAG_EXPR, bE = (bb & TAG_MASK) == TAG_EXPR;
-                    const bool aT = aa == TAG_TRUE, aF = aa == TAG_FALSE, bT = bb == TAG_TRUE, bF = bb == TAG_FALSE;
-                    // start of the operands' code (constants emitted nothing)
-                    const uint32_t first = aE ? (aa & ~TAG_MASK) : (bE ? (bb & ~TAG_MASK) : n);
-                    uint32_t res;
-                    bool emit = false, emit_not = false;
-                    if (op == OP_OR) {
-                        if (aT || bT) res = TAG_TRUE, n = first;
-                        else if (aF) res = bb;
-                        else if (bF) res = aa;
-                        else res = aa, emit = true;
-                    } else if (op == OP_AND) {
-                        if (aF || bF) res = TAG_FALSE, n = first;
-                        else if (aT) res = bb;
-                        else if (bT) res = aa;
-                        else res = aa, emit = true;
-                    } else if (op == OP_ANDNOT) {  // a & !b
-                        if (aF || bT) res = TAG_FALSE, n = first;
-                        else if (bF) res = aa;
-                        else if (aT) res = bb, emit_not = true;  // b's code is last
-                        else res = aa, emit = true;
-                    } else {  // OP_RANDNOT: !a & b
-                        if (aT || bF) res = TAG_FALSE, n = first;
-                        else if (aF) res = bb;
-                        else if (bT) res = aa, emit_not = true;  // a's code is last
-                        else res = aa, emit = true;
-                    }
-                    if (emit || emit_not) {
-                        if (n < cap) o[n] = emit ? in : Insn{OP_NOT, 0};
-                        else over = true;
-                        n++;
-                    }
-                    S[sp++] = res;
-                }
-            }
+            // two copies of the fold, so that each reads with typed (LDS or
+            // global) instructions rather than generic ones
+            if (sg.stage)
+                plan_fold(sc, (const AtomBox *)(sc + P.ncode), (const double *)((const AtomBox *)(sc + P.ncode) + P.ncode),
+                          P.ncode, S, lo, hi, o, cap, n, over);
+            else
+                plan_fold(P.code, P.boxes, P.f64, P.ncode, S, lo, hi, o, cap, n, over);
             const uint32_t root = S[0];
             uint2 bi;
             if (root == TAG_FALSE) bi = make_uint2(0, 0);
@@ -319,11 +345,17 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(const Seg *__restrict__ seg
             else if (over) bi = make_uint2(BRICK_FULL, P.ncode), active = true;
             else bi = make_uint2((uint32_t)(gb * cap), n), active = true;
             info[gb] = bi;
-            if (active) {
-                atomicAdd(&ctl->code_total, (unsigned long long)bi.y);
-                atomicMax(&ctl->code_max, bi.y);
-                if (over) atomicAdd(&ctl->full_bricks, 1u);
-            }
+            len = active ? bi.y : 0;
+            full = over;
+        }
+        // per-wave statistics: one atomic each
+        uint32_t sum = len, mx = len;
+        for (int d = 32; d; d >>= 1) sum += __shfl_xor(sum, d), mx = max(mx, (uint32_t)__shfl_xor(mx, d));
+        const uint32_t nfull = (uint32_t)__popcll(__ballot(full));
+        if (lane == 0 && sum) {
+            atomicAdd(&ctl->code_total, (unsigned long long)sum);
+            atomicMax(&ctl->code_max, mx);
+            if (nfull) atomicAdd(&ctl->full_bricks, nfull);
         }
         wave_append(active, (uint32_t)gb, act_list, &ctl->nact_bricks);
     }
@@ -454,9 +486,6 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
             }
         }
         __syncthreads();
-        const Insn *code = staged ? s_code : brick_code(P, B, bi);
-        const double *f64 = staged ? s_f64 : P.f64;
-        const float *f32 = staged ? s_f32 : P.f32;
         const int x = bx * 64 + lane;
         const bool xin = x < g.nx;
         const double px0 = xin ? g.xs[x] : 0.0;
@@ -474,7 +503,13 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
                 py[q] = ok[q] ? g.ys[yy[q]] : 0.0;
                 pz[q] = ok[q] ? g.zs[zz[q]] : 0.0;
             }
-            eval_code_n<MODE, STATS, NP, true>(code, bi.y, P, f64, f32, px, py, pz, live, res, gs);
+            // two copies of the walk, so that each reads with typed (LDS or
+            // global) instructions rather than generic ones
+            if (staged)
+                eval_code_n<MODE, STATS, NP, true>(s_code, bi.y, P, s_f64, s_f32, px, py, pz, live, res, gs);
+            else
+                eval_code_n<MODE, STATS, NP, true>(brick_code(P, B, bi), bi.y, P, P.f64, P.f32, px, py, pz, live, res,
+                                                   gs);
             for (int q = 0; q < NP; q++) {
                 if (!ok[q]) continue;  // wave-uniform
                 const int y = yy[q], z = zz[q];

@@ -309,8 +309,11 @@ int collider_on_device(mc3d_ctx *ctx, const std::shared_ptr<const ColliderData> 
     return MC3D_OK;
 }
 
-// Bytes of LDS the plan needs to stage a program of n instructions.
-size_t plan_stage_bytes(size_t n) { return PLAN_STACK_BYTES + n * (sizeof(Insn) + sizeof(AtomBox)); }
+// Bytes of LDS the plan needs to stage a program: instructions, their atom
+// boxes and the float64 params.
+size_t plan_stage_bytes(const Program &P) {
+    return PLAN_STACK_BYTES + P.code.size() * (sizeof(Insn) + sizeof(AtomBox)) + P.f64.size() * sizeof(double);
+}
 constexpr size_t PLAN_STAGE_LIMIT = 64 * 1024;
 
 // Where a call's pieces sit in device memory.
@@ -402,7 +405,7 @@ int upload(mc3d_ctx *ctx, std::vector<Job> &jobs, uint64_t nwords_all, uint64_t 
         sg.P.host_vals = nullptr;
         sg.P.host_stride = 0;
         sg.P.ncode = (uint32_t)P.code.size();
-        sg.P.pad = 0;
+        sg.P.nf64 = (uint32_t)P.f64.size();
         if (!j.ax[0].empty()) {
             const double *ax = (const double *)(dv + po[s].ax);
             j.g.xs = ax, j.g.ys = ax + j.g.nx, j.g.zs = ax + j.g.nx + j.g.ny;
@@ -413,7 +416,7 @@ int upload(mc3d_ctx *ctx, std::vector<Job> &jobs, uint64_t nwords_all, uint64_t 
         sg.word0 = j.word0, sg.nwords = j.nwords;
         sg.brick0 = j.brick0, sg.nbricks = j.nbricks;
         sg.iters = j.iters;
-        sg.stage = j.cap && plan_stage_bytes(P.code.size()) <= PLAN_STAGE_LIMIT ? 1 : 0;
+        sg.stage = j.cap && plan_stage_bytes(P) <= PLAN_STAGE_LIMIT ? 1 : 0;
     }
     memcpy(h + o_segs, U.hsegs.data(), n * sizeof(Seg));
     if (tables) {
@@ -733,7 +736,7 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
         // Programs that fit are staged in LDS beside the per-thread stacks.
         size_t lds = PLAN_STACK_BYTES;
         for (size_t s = 0; s < jobs.size(); s++)
-            if (U.hsegs[s].stage) lds = std::max(lds, plan_stage_bytes(jobs[s].P.code.size()));
+            if (U.hsegs[s].stage) lds = std::max(lds, plan_stage_bytes(jobs[s].P));
         const unsigned pgrid = grid_for(c.nbricks, PLAN_BLOCK);
         if (c.nsegs > 1)
             k_plan<true><<<pgrid, PLAN_BLOCK, lds, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.brick_seg, c.nbricks,
@@ -770,8 +773,10 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
     if ((rc = scan4(ctx, ctx->counts.as<uint4>(), ctx->offs.as<uint4>(), c.nwords, U.ctl->totals,
                     ctx->actw.as<uint32_t>())))
         return rc;
-    k_seg_ranges<<<grid_for(c.nsegs, 256), 256, 0, ctx->stream>>>(U.segs, c.nsegs, ctx->offs.as<uint4>(), U.sout);
-    HIP_TRY(hipGetLastError());
+    if (c.nsegs > 1) {  // a single solid starts at vertex 0 and triangle 0 (zeroed by the upload)
+        k_seg_ranges<<<grid_for(c.nsegs, 256), 256, 0, ctx->stream>>>(U.segs, c.nsegs, ctx->offs.as<uint4>(), U.sout);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
 
     // ---- emit + bisect, sized from earlier calls ----
