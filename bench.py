@@ -46,11 +46,11 @@ CPU_SAMPLE_SECONDS = 10.0
 # on an MI355X (`--autotune` searches again): precision, culling, brick edge,
 # grid rows per program walk, wavefronts per brick.
 DEFAULT_RUN = {
-    "csg64": dict(precision="fp64", cull=True, brick=4, rows_per_walk=2, waves_per_brick=4),
+    "csg64": dict(precision="fp64", cull=True, brick=8, rows_per_walk=1, waves_per_brick=4),
     "sphere": dict(precision="fp64", cull=False, brick=0, rows_per_walk=1, waves_per_brick=4),
     "collider": dict(precision="fp64", cull=False, brick=0, rows_per_walk=1, waves_per_brick=4),
-    "batch": dict(precision="fp64", cull=True, brick=4, rows_per_walk=1, waves_per_brick=2),
-    "large": dict(precision="fp64", cull=True, brick=4, rows_per_walk=2, waves_per_brick=4),
+    "batch": dict(precision="fp64", cull=True, brick=8, rows_per_walk=1, waves_per_brick=4),
+    "large": dict(precision="fp64", cull=True, brick=8, rows_per_walk=1, waves_per_brick=4),
 }
 PLAIN_RUN = dict(precision="fp64", cull=False, brick=0, rows_per_walk=1, waves_per_brick=4)
 

@@ -710,7 +710,7 @@ __device__ __forceinline__ uint64_t vkey(int axis, int x, int y, int z) {
 // then y, then z, each in bit order.
 struct WordInfo {
     uint64_t bits, cx, cy, cz;
-    uint32_t base, pad;
+    uint32_t base, tbase;  // scanned first vertex and first triangle of the word
 };
 
 // One wave per listed word, one lane per x position.  The 2 x 2 rows by 2
@@ -737,14 +737,18 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
     const uint32_t nact = (uint32_t)ctl->totals[2];
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
     bool over = false;
-    for (uint32_t k = blockIdx.x * 4 + wave; k < nact; k += gridDim.x * 4) {
-        const uint64_t gw = active[k];
+    const uint32_t kstep = gridDim.x * 4;
+    uint32_t k = blockIdx.x * 4 + wave;
+    uint64_t gw_next = k < nact ? active[k] : 0;
+    for (; k < nact; k += kstep) {
+        const uint64_t gw = gw_next;
+        if (k + kstep < nact) gw_next = active[k + kstep];  // the next word, while this one is processed
         const uint32_t si = wave_seg<MULTI>(word_seg, gw, SEG_WORD_SHIFT);
         const Seg sg = pick_seg<MULTI>(segs, seg0, si);
         const Grid &g = sg.g;
         const uint64_t *bits = bits_all + sg.word0;
         const uint4 *offs = offs_all + sg.word0;
-        const uint32_t vbase = offs[0].x;  // the solid's first vertex (triangles index from it)
+        const uint32_t vbase = MULTI ? offs[0].x : 0u;  // the solid's first vertex (triangles index from it)
         const uint64_t w = gw - sg.word0;
         int wx, y, z;
         word_coords(g, w, wx, y, z);
@@ -757,7 +761,9 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
                 const Masks m = word_masks(g, bits, ex, ey, ez);
                 r.bits = bits[idx];
                 r.cx = m.cx, r.cy = m.cy, r.cz = m.cz;
-                r.base = offs[idx].x;
+                const uint4 o = offs[idx];
+                r.base = o.x;
+                r.tbase = o.y;
             }
             wi[lane] = r;
         }
@@ -809,7 +815,7 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
         const uint32_t pre = __popcll(__ballot(nt & 1) & below) + 2 * __popcll(__ballot(nt & 2) & below) +
                              4 * __popcll(__ballot(nt & 4) & below);
         const uint32_t tw = __shfl(pre + nt, 63);  // triangles of the word
-        const uint64_t t0 = offs[w].y;
+        const uint64_t t0 = wi[0].tbase;
         if (t0 + tw > tcap) {
             over = true;
             continue;  // wave-uniform
@@ -863,17 +869,6 @@ __device__ __forceinline__ bool corner_bit(const Grid &g, const uint64_t *__rest
     return (bits[word_index(g, x >> 6, y, z)] >> (x & 63)) & 1;
 }
 
-template <int MODE, bool STATS>
-__device__ __forceinline__ bool eval_brick(const DevProg &P, const Bricks &B, const Grid &g, uint64_t brick0, int bx,
-                                           int y, int z, double px, double py, double pz, GStats &gs) {
-    // Lanes of one wave may sit in different bricks: no readfirstlane here.
-    if (!B.info) return eval_code<MODE, STATS, false>(P.code, P.ncode, P, px, py, pz, gs);
-    const uint2 bi = B.info[brick0 + ((size_t)(z / B.BZ) * B.nby + (y / B.BY)) * g.W + bx];
-    if (bi.y == 0) return false;
-    if (bi.y == BRICK_TRUE) return true;
-    return eval_code<MODE, STATS, false>(brick_code(P, B, bi), bi.y, P, px, py, pz, gs);
-}
-
 // mcSearchPoint (mc.go:223-258) per vertex; iters == 0 leaves the edge
 // midpoints of MarchingCubes (Coord3D.Mid, coords.go:196-198).  An edge lies
 // inside the brick of its low corner (bricks include their +1 corners), so
@@ -903,10 +898,23 @@ __device__ __forceinline__ void bisect_vertex(const Seg &sg, const uint64_t *__r
         } else {
             double fp = lo, tp = hi;
             if (!corner_bit(g, bits, hx, hy, hz)) fp = hi, tp = lo;
+            // The edge's brick program, looked up once for all midpoints.
+            const Insn *code = P.code;
+            uint32_t ncode = P.ncode;
+            int konst = -1;
+            if (B.info) {
+                const uint2 bi = B.info[sg.brick0 + ((size_t)(z / B.BZ) * B.nby + (y / B.BY)) * g.W + (x >> 6)];
+                if (bi.y == 0) konst = 0;
+                else if (bi.y == BRICK_TRUE) konst = 1;
+                else code = brick_code(P, B, bi), ncode = bi.y;
+            }
             for (int i = 0; i < iters; i++) {
                 const double mid = (fp + tp) / 2;
                 p[axis] = mid;
-                if (eval_brick<MODE, STATS>(P, B, g, sg.brick0, x >> 6, y, z, p[0], p[1], p[2], gs)) tp = mid;
+                // Lanes of one wave may sit in different bricks: no readfirstlane here.
+                const bool in = konst >= 0 ? konst != 0
+                                           : eval_code<MODE, STATS, false>(code, ncode, P, p[0], p[1], p[2], gs);
+                if (in) tp = mid;
                 else fp = mid;
             }
             res = (fp + tp) / 2;
