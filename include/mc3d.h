@@ -159,6 +159,9 @@ typedef struct {
     double ms_d2h;             /* copy of results to host */
     double ms_host_callbacks;  /* time spent in mc3d_contains_fn */
     double ms_call;            /* host wall time of the whole call */
+    double ms_prepare;         /* host: bounds, grid axes and lowering of the call's solids */
+    double ms_enqueue;         /* host: upload and kernel launches, up to the wait for the results */
+    double ms_wait;            /* host: waiting for the device at the end of the call */
 } mc3d_stats;
 
 /* MarchingCubesSearch(s, delta, iters), model3d/mc.go:45-49 (iters == 0 is

@@ -87,6 +87,9 @@ class mc3d_stats(ctypes.Structure):
         ("ms_d2h", ctypes.c_double),
         ("ms_host_callbacks", ctypes.c_double),
         ("ms_call", ctypes.c_double),
+        ("ms_prepare", ctypes.c_double),
+        ("ms_enqueue", ctypes.c_double),
+        ("ms_wait", ctypes.c_double),
     ]
 
     def as_dict(self):

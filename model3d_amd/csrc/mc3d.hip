@@ -686,8 +686,14 @@ int run_output_host(mc3d_ctx *ctx, const Call &c, const Job &j, Upload &U, MeshB
 
 // Meshes `jobs` (prepared) in one pass of the pipeline; out[i] gets job i's
 // mesh.  A job with host-evaluated atoms is only ever run alone.
-int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::vector<mc3d_mesh *> &out) {
-    const auto t_call = std::chrono::steady_clock::now();
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::vector<mc3d_mesh *> &out,
+             std::chrono::steady_clock::time_point t_call, double ms_prepare) {
+    const auto t_enqueue = std::chrono::steady_clock::now();
+    double ms_enqueue = 0, ms_wait = 0;
     Call c;
     c.nsegs = (uint32_t)jobs.size();
     c.mode = o.precision == MC3D_PREC_FP32_GUARD ? 1 : 0;
@@ -789,7 +795,10 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
         if ((rc = take_outputs(ctx, *mb, c, vcap, tcap))) return rc;
         if ((rc = run_output(ctx, c, U, *mb, vcap, tcap))) return rc;
         HIP_TRY(hipMemcpyAsync(hc, U.ctl, U.back_bytes, hipMemcpyDeviceToHost, ctx->stream));
+        ms_enqueue = ms_since(t_enqueue);
+        const auto t_wait = std::chrono::steady_clock::now();
         HIP_TRY(hipStreamSynchronize(ctx->stream));
+        ms_wait = ms_since(t_wait);
         if (hc->inside) {
             size_t bad = 0;
             while (bad + 1 < jobs.size() && !so[bad].inside) bad++;
@@ -879,11 +888,13 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
         const double ms_d2h = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         for (auto &m : meshes) m->stats.ms_d2h = ms_d2h;
     }
-    const double ms_call =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_call).count();
+    const double ms_call = ms_since(t_call);
     out.resize(jobs.size());
     for (size_t s = 0; s < jobs.size(); s++) {
         meshes[s]->stats.ms_call = ms_call;
+        meshes[s]->stats.ms_prepare = ms_prepare;
+        meshes[s]->stats.ms_enqueue = ms_enqueue;
+        meshes[s]->stats.ms_wait = ms_wait;
         out[s] = meshes[s].release();
     }
     return MC3D_OK;
@@ -954,14 +965,16 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
         return MC3D_E_INVALID_ARG;
     }
     *out = nullptr;
+    const auto t_call = std::chrono::steady_clock::now();
     const mc3d_options o = default_options(opts);
     HIP_TRY(hipSetDevice(ctx->device));
     std::vector<Job> jobs(1);
     jobs[0].s = s, jobs[0].delta = delta, jobs[0].iters = iters;
     int rc = prepare_job(jobs[0], o);
     if (rc) return rc;
+    const double ms_prepare = ms_since(t_call);
     std::vector<mc3d_mesh *> meshes;
-    if ((rc = run_call(ctx, jobs, o, meshes))) return rc;
+    if ((rc = run_call(ctx, jobs, o, meshes, t_call, ms_prepare))) return rc;
     *out = meshes[0];
     return MC3D_OK;
 }
@@ -980,6 +993,7 @@ int mc3d_marching_cubes_search_batch(mc3d_ctx *ctx, const mc3d_solid *const *sol
         }
     }
     if (n == 0) return MC3D_OK;
+    const auto t_call = std::chrono::steady_clock::now();
     const mc3d_options o = default_options(opts);
     HIP_TRY(hipSetDevice(ctx->device));
     // Bounds, spacers and lowering of every solid, on up to 16 host threads
@@ -1018,6 +1032,7 @@ int mc3d_marching_cubes_search_batch(mc3d_ctx *ctx, const mc3d_solid *const *sol
             batched_at.push_back(i);
         }
     }
+    const double ms_prepare = ms_since(t_call);
     std::vector<mc3d_mesh *> got(n, nullptr);
     auto fail = [&](int rc) {
         for (mc3d_mesh *m : got) delete m;
@@ -1025,7 +1040,7 @@ int mc3d_marching_cubes_search_batch(mc3d_ctx *ctx, const mc3d_solid *const *sol
     };
     if (!batched.empty()) {
         std::vector<mc3d_mesh *> ms;
-        int rc = run_call(ctx, batched, o, ms);
+        int rc = run_call(ctx, batched, o, ms, t_call, ms_prepare);
         if (rc) return fail(rc);
         for (size_t k = 0; k < ms.size(); k++) got[batched_at[k]] = ms[k];
     }
@@ -1033,7 +1048,7 @@ int mc3d_marching_cubes_search_batch(mc3d_ctx *ctx, const mc3d_solid *const *sol
         std::vector<Job> one;
         one.push_back(std::move(alone[k]));
         std::vector<mc3d_mesh *> ms;
-        int rc = run_call(ctx, one, o, ms);
+        int rc = run_call(ctx, one, o, ms, std::chrono::steady_clock::now(), 0.0);
         if (rc) {
             set_error("solid " + std::to_string(alone_at[k]) + " of the batch: " + mc3d_last_error());
             return fail(rc);

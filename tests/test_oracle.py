@@ -197,6 +197,19 @@ def test_contains_textbook_predicates():
     got = oracle_contains(co, p)
     assert np.array_equal(got[_far(margin)], want[_far(margin)])
 
+    cs = cat["cone_slice"]
+    p1, p2 = np.array(cs.P1), np.array(cs.P2)
+    ax = p2 - p1
+    L = np.linalg.norm(ax)
+    t = (p - p1) @ ax / L  # distance along the axis from P1
+    radial = np.linalg.norm((p - p1) - np.outer(t / L, ax), axis=1)
+    r = cs.R1 * (1 - t / L) + cs.R2 * (t / L)  # radius interpolates from R1 at P1 to R2 at P2
+    margin = np.minimum(np.minimum(np.abs(t), np.abs(t - L)), np.abs(radial - r))
+    want = (t >= 0) & (t <= L) & (radial <= r)
+    got = oracle_contains(cs, p)
+    assert np.array_equal(got[_far(margin)], want[_far(margin)])
+    assert want[_far(margin)].sum() > 100  # the sample reaches inside
+
     tor = cat["torus"]
     axis = np.array(tor.Axis) / np.linalg.norm(tor.Axis)
     q = p - tor.Center
