@@ -34,6 +34,18 @@ __device__ __forceinline__ T *global_ptr(T *p) {
     return (T *)(__attribute__((address_space(1))) T *)p;
 }
 
+// The same pointer in the constant address space (memory that nothing
+// writes while the kernel runs): a load through it at a wave-uniform
+// address becomes a scalar load, served by the scalar data cache into
+// scalar registers that vector instructions read as operands.  The walks
+// take such pointers as their own types (a cast back to a generic pointer
+// would be folded away together with the address space).
+#define MC3D_CONST __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const MC3D_CONST T *scalar_ptr(const T *p) {
+    return (const MC3D_CONST T *)p;
+}
+
 __device__ __forceinline__ DevProg as_global(DevProg P) {
     P.code = global_ptr(P.code);
     P.f64 = global_ptr(P.f64);
@@ -123,9 +135,9 @@ __device__ __forceinline__ uint32_t atom_flops32(uint32_t kind) {
     }
 }
 
-__device__ __forceinline__ void apply_frame(const double *__restrict__ f64, Frame fr, double &x, double &y,
-                                            double &z) {
-    const double *r = f64 + fr.offset;
+template <class F64P>
+__device__ __forceinline__ void apply_frame(F64P f64, Frame fr, double &x, double &y, double &z) {
+    F64P r = f64 + fr.offset;
     for (uint32_t i = 0; i < fr.nops; ++i, r += 10) {
         int k = (int)r[0];
         if (k == MC3D_XF_TRANSLATE) {  // Translate.Apply, transform.go:44-46
@@ -152,14 +164,16 @@ __device__ __forceinline__ void apply_frame(const double *__restrict__ f64, Fram
 // One primitive Contains in float64 (point already in the atom's frame).
 // Constants that do not depend on the point were computed on the host with
 // the reference's operation order (solid.cpp).
-__device__ __forceinline__ bool atom64(uint32_t kind, const double *__restrict__ a, double x, double y, double z) {
+template <class F64P>
+__device__ __forceinline__ bool atom64(uint32_t kind, F64P a, double x, double y, double z) {
     switch (kind) {
     case A_SPHERE: {  // Sphere.Contains: coord.Dist(center) <= r, shapes.go:29-31
         double d1 = x - a[0], d2 = y - a[1], d3 = z - a[2];
         return d1 * d1 + d2 * d2 + d3 * d3 <= a[4];  // sqrt(s) <= r  <=>  s <= T
     }
     case A_RECT:  // Rect.Contains, shapes.go:171-173
-        return in_lo(x, a[0]) && in_lo(y, a[1]) && in_lo(z, a[2]) && in_hi(x, a[3]) && in_hi(y, a[4]) &&
+        // all six tests (no short circuit): the params load together
+        return in_lo(x, a[0]) & in_lo(y, a[1]) & in_lo(z, a[2]) & in_hi(x, a[3]) & in_hi(y, a[4]) &
                in_hi(z, a[5]);
     case A_CAPSULE: {  // Segment.Dist(c) <= r, primitives.go:568-593
         double v2x = x - a[0], v2y = y - a[1], v2z = z - a[2];
@@ -225,8 +239,8 @@ __device__ __forceinline__ bool atom64(uint32_t kind, const double *__restrict__
 // primitive; points farther away can only be decided "outside", which is
 // then correct (solid.cpp, guard_*).  *plain gets the unguarded float32
 // decision for the mismatch statistic.
-__device__ __forceinline__ int atom32(uint32_t kind, const float *__restrict__ a, float x, float y, float z,
-                                      bool *plain) {
+template <class F32P>
+__device__ __forceinline__ int atom32(uint32_t kind, F32P a, float x, float y, float z, bool *plain) {
     switch (kind) {
     case A_SPHERE: {  // [c0 c1 c2 r T Tlo Thi]
         float d1 = x - a[0], d2 = y - a[1], d3 = z - a[2];
@@ -297,13 +311,14 @@ constexpr uint32_t MAX_ATOM_PARAMS = 16;
 // HOST: the program may hold host-evaluated atoms, whose answers for point
 // `pid` were filled in by the host beforehand (kernels without point ids are
 // never launched on such programs).
-template <int MODE, bool STATS, bool HOST>
-__device__ __forceinline__ bool eval_atom(const DevProg &P, uint32_t kind, const double *f64, const float *f32,
-                                          double qx, double qy, double qz, float fx, float fy, float fz, bool live,
-                                          uint64_t pid, GStats &gs) {
+template <int MODE, bool STATS, bool HOST, class F64P, class F32P>
+__device__ __forceinline__ bool eval_atom(const DevProg &P, uint32_t kind, F64P f64, F32P f32, double qx, double qy,
+                                          double qz, float fx, float fy, float fz, bool live, uint64_t pid,
+                                          GStats &gs) {
     if (kind == A_COLLIDER) {
         uint32_t cand = 0;
-        const bool b = collider_contains(P.colliders[(int)f64[0]], f64, qx, qy, qz, STATS ? &cand : nullptr);
+        const bool b = collider_contains(P.colliders[(int)f64[0]], (const double *)f64, qx, qy, qz,
+                                         STATS ? &cand : nullptr);
         if (STATS && live) gs.flops64 += atom_flops64(kind) + COLLIDER_FLOPS_PER_CANDIDATE * cand;
         return b;
     }
@@ -329,7 +344,8 @@ __device__ __forceinline__ bool eval_atom(const DevProg &P, uint32_t kind, const
 }
 
 // Operation count of one frame (3 per translate / scale, 15 per matrix).
-__device__ __forceinline__ uint32_t frame_flops(const double *__restrict__ f64, Frame fr) {
+template <class F64P>
+__device__ __forceinline__ uint32_t frame_flops(F64P f64, Frame fr) {
     uint32_t n = 0;
     for (uint32_t i = 0; i < fr.nops; ++i) {
         const int k = (int)f64[fr.offset + 10 * i];
@@ -340,13 +356,16 @@ __device__ __forceinline__ uint32_t frame_flops(const double *__restrict__ f64, 
 
 // Postfix boolean program over atoms for NP points at once (one walk of the
 // program); each point's stack is a bit stack in a register.  The program
-// and the params it refers to (insn.y offsets into f64 / f32) may live in
-// global memory or LDS; instructions are made wave-uniform with readfirstlane
-// so the walk branches on scalars.  MODE 0: every atom in float64.  MODE 1:
+// and the params it refers to (insn.y offsets into f64 / f32; frames carry
+// their params there too) are read through the given pointer types: with
+// UNI the whole wave walks one program, read with scalar loads through
+// constant-space pointers (scalar_ptr), so the walk branches on scalars;
+// otherwise each lane reads its own.  MODE 0: every atom in float64.  MODE 1:
 // float32 with band, float64 in the band.  `live` only gates the statistics.
-template <int MODE, bool STATS, int NP, bool UNI, bool HOST = false>
-__device__ __forceinline__ void eval_code_n(const Insn *code, uint32_t n, const DevProg &P, const double *f64,
-                                            const float *f32, const double *px, const double *py, const double *pz,
+template <int MODE, bool STATS, int NP, bool UNI, bool HOST = false, class CodeP, class F64P, class F32P,
+          class FrameP>
+__device__ __forceinline__ void eval_code_n(CodeP code, uint32_t n, const DevProg &P, F64P f64, F32P f32,
+                                            FrameP frames, const double *px, const double *py, const double *pz,
                                             const bool *live, bool *res, GStats &gs, const uint64_t *pid = nullptr) {
     uint32_t st[NP];
     double qx[NP], qy[NP], qz[NP];
@@ -358,21 +377,21 @@ __device__ __forceinline__ void eval_code_n(const Insn *code, uint32_t n, const 
     }
     uint32_t cur = 0;
     for (uint32_t pc = 0; pc < n; ++pc) {
-        Insn in = code[pc];
+        Insn in{code[pc].x, code[pc].y};  // field-wise: the pointer may be in any address space
         if (UNI) in.x = (uint32_t)__builtin_amdgcn_readfirstlane((int)in.x);
         if (UNI) in.y = (uint32_t)__builtin_amdgcn_readfirstlane((int)in.y);
         const uint32_t op = insn_op(in.x);
         if (op == OP_ATOM) {
             const uint32_t fr = insn_frame(in.x);
             if (fr != cur) {
-                const Frame f = P.frames[fr];
+                const Frame f{frames[fr].offset, frames[fr].nops};
                 for (int k = 0; k < NP; k++) {
                     qx[k] = px[k], qy[k] = py[k], qz[k] = pz[k];
-                    apply_frame(P.f64, f, qx[k], qy[k], qz[k]);
+                    apply_frame(f64, f, qx[k], qy[k], qz[k]);
                     if (MODE == 1) fx[k] = (float)qx[k], fy[k] = (float)qy[k], fz[k] = (float)qz[k];
                 }
                 if (STATS) {
-                    const uint32_t ff = frame_flops(P.f64, f);
+                    const uint32_t ff = frame_flops(f64, f);
                     for (int k = 0; k < NP; k++) gs.flops64 += live[k] ? ff : 0;
                 }
                 cur = fr;
@@ -417,7 +436,12 @@ __device__ __forceinline__ bool eval_code(const Insn *__restrict__ code, uint32_
                                           double py, double pz, GStats &gs, uint64_t pid = 0) {
     const bool live = true;
     bool r;
-    eval_code_n<MODE, STATS, 1, UNI, HOST>(code, n, P, P.f64, P.f32, &px, &py, &pz, &live, &r, gs, &pid);
+    if constexpr (UNI)  // one program for the whole wave: scalar loads
+        eval_code_n<MODE, STATS, 1, UNI, HOST>(scalar_ptr(code), n, P, scalar_ptr(P.f64), scalar_ptr(P.f32),
+                                               scalar_ptr(P.frames), &px, &py, &pz, &live, &r, gs, &pid);
+    else
+        eval_code_n<MODE, STATS, 1, UNI, HOST>(code, n, P, P.f64, P.f32, P.frames, &px, &py, &pz, &live, &r, gs,
+                                               &pid);
     return r;
 }
 

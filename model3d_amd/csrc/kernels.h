@@ -33,6 +33,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "eval.h"
 
 namespace mc3d {
@@ -159,6 +161,13 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t item, uint32_t *
     base = __shfl(base, leader);
     if (pred) list[base + __popcll(bal & ((1ull << lane) - 1))] = item;
 }
+
+// s_waitcnt operand for vmcnt(0) alone (expcnt and lgkmcnt at their maxima).
+constexpr int VMCNT0 = 0x0f70;
+
+// Index of this wave in its block as a scalar: loops and row arithmetic
+// derived from it stay in scalar registers.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -302,9 +311,6 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(const Seg *__restrict__ seg
         const DevProg &P = sg.P;
         const Grid &g = sg.g;
         const Bricks &B = sg.B;
-        const Insn *code = P.code;
-        const AtomBox *boxes = P.boxes;
-        const double *f64 = P.f64;
         if (sg.stage) {  // block-uniform
             AtomBox *sb = (AtomBox *)(sc + P.ncode);
             double *sf = (double *)(sb + P.ncode);
@@ -315,7 +321,6 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(const Seg *__restrict__ seg
                 __syncthreads();
                 staged = si;
             }
-            code = sc, boxes = sb, f64 = sf;
         }
         const uint64_t b = gb - sg.brick0;  // local brick
         bool active = false, full = false;
@@ -372,7 +377,7 @@ __global__ __launch_bounds__(256) void k_corners(const Seg *__restrict__ segs, u
                                                  uint64_t nwords_all) {
     const int lane = threadIdx.x & 63;
     GStats gs;
-    for (uint64_t gw = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); gw < nwords_all; gw += (uint64_t)gridDim.x * 4) {
+    for (uint64_t gw = (uint64_t)blockIdx.x * 4 + wave_id(); gw < nwords_all; gw += (uint64_t)gridDim.x * 4) {
         const uint32_t si = wave_seg<MULTI>(word_seg, gw, SEG_WORD_SHIFT);
         const Seg sg = pick_seg<MULTI>(segs, seg0, si);
         const DevProg &P = sg.P;
@@ -435,60 +440,100 @@ __device__ __forceinline__ const Insn *brick_code(const DevProg &P, const Bricks
     return bi.x == BRICK_FULL ? P.code : B.code + bi.x;
 }
 
-// Culled, part 2: WPB waves per active brick.  The brick's folded program
-// and the params of its atoms are first staged in LDS (one parallel copy, a
-// thread per param, instead of a chain of dependent scalar loads during the
-// walk), then the waves take the brick's rows, NP rows per walk of the
-// program.  Longer programs are walked from global memory.
-constexpr uint32_t STAGE_INSNS = 32;
+// One row of 64 corners decided: the occupancy word and, for bricks on the
+// grid's boundary, the check that no boundary corner is inside
+// (panic("solid is true outside of bounds"), mc.go:687-689).
+__device__ __forceinline__ void row_out(const Grid &g, uint64_t *__restrict__ bits, int bx, int y, int z, int x,
+                                        bool inside, bool edge, Ctl *__restrict__ ctl,
+                                        SegOut *__restrict__ so) {
+    const uint64_t m = __ballot(inside);
+    if ((threadIdx.x & 63) == 0) bits[word_index(g, bx, y, z)] = m;
+    if (edge && inside && (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1))
+        atomicOr(&ctl->inside, 1u), atomicOr(&so->inside, 1u);
+}
 
+// The rows of a brick that wave `wave` of WPB takes (row j = jz * BY + jy,
+// j = wave, wave + WPB, ...), each decided by eval(py, pz) for the lane's x.
+template <int WPB, class F>
+__device__ __forceinline__ void brick_rows(const Grid &g, const Bricks &B, uint64_t *__restrict__ bits, int bx,
+                                           int by, int bz, int wave, int x, bool xin, bool edge,
+                                           Ctl *__restrict__ ctl, SegOut *__restrict__ so, F &&eval) {
+    int jy = wave, jz = 0;
+    while (jy >= B.BY) jy -= B.BY, jz++;
+    const int y0 = by * B.BY, z0 = bz * B.BZ;
+    const int jz_end = min(B.BZ, g.nz - z0), jy_end = min(B.BY, g.ny - y0);
+    const auto ys = scalar_ptr(g.ys) + y0, zs = scalar_ptr(g.zs) + z0;
+    while (jz < jz_end) {
+        if (jy < jy_end) {
+            const bool inside = eval(ys[jy], zs[jz]) && xin;
+            row_out(g, bits, bx, y0 + jy, z0 + jz, x, inside, edge, ctl, so);
+        }
+        jy += WPB;
+        while (jy >= B.BY) jy -= B.BY, jz++;
+    }
+}
+
+// Culled, part 2: WPB waves per active brick.  A brick whose folded program
+// is one untransformed atom has the atom kind dispatched once for all its
+// rows; other programs are walked per NP rows.  Either way the walk is
+// wave-uniform: instructions, params and row coordinates come in through
+// scalar loads.
 template <int MODE, bool STATS, int NP, int WPB, bool MULTI>
 __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
                                                           const uint32_t *__restrict__ brick_seg,
                                                           const uint32_t *__restrict__ act,
                                                           uint64_t *__restrict__ bits_all, Ctl *__restrict__ ctl,
                                                           SegOut *__restrict__ sout) {
-    __shared__ Insn s_code[STAGE_INSNS];
-    __shared__ double s_f64[STAGE_INSNS * MAX_ATOM_PARAMS];
-    __shared__ float s_f32[STAGE_INSNS * MAX_ATOM_PARAMS];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const uint32_t nact = ctl->nact_bricks;
     GStats gs;
     for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x) {
-        const uint32_t gb = act[k];
+        const uint32_t gb = scalar_ptr(act)[k];
         const uint32_t si = wave_seg<MULTI>(brick_seg, gb, SEG_BRICK_SHIFT);
         const Seg sg = pick_seg<MULTI>(segs, seg0, si);
         const DevProg &P = sg.P;
         const Grid &g = sg.g;
         const Bricks &B = sg.B;
+        SegOut *so = &sout[si];
         const int nrows = B.BY * B.BZ;
         uint64_t *bits = bits_all + sg.word0;
         const uint32_t b = gb - (uint32_t)sg.brick0;
         const uint32_t r = b / (uint32_t)g.W;
         const int bx = (int)(b - r * (uint32_t)g.W);
         const int bz = (int)(r / (uint32_t)B.nby), by = (int)(r - (uint32_t)bz * (uint32_t)B.nby);
-        const uint2 bi = B.info[gb];
-        const bool staged = bi.x != BRICK_FULL && bi.y <= STAGE_INSNS;  // block-uniform
-        __syncthreads();  // the previous brick's walks are done with the LDS copy
-        if (staged) {
-            // thread t copies param j of instruction i (t = i * 16 + j)
-            for (uint32_t t = threadIdx.x; t < bi.y * MAX_ATOM_PARAMS; t += blockDim.x) {
-                const uint32_t i = t / MAX_ATOM_PARAMS, j = t % MAX_ATOM_PARAMS;
-                Insn in = B.code[bi.x + i];
-                if (insn_op(in.x) == OP_ATOM && j < atom_nparams(insn_kind(in.x))) {
-                    s_f64[t] = P.f64[in.y + j];
-                    s_f32[t] = P.f32[in.y + j];
-                }
-                if (j == 0) {
-                    in.y = i * MAX_ATOM_PARAMS;
-                    s_code[i] = in;
-                }
-            }
-        }
-        __syncthreads();
+        // the brick touches the grid's boundary faces
+        const bool edge = bx == 0 || bx * 64 + 64 >= g.nx || by == 0 || (by + 1) * B.BY >= g.ny || bz == 0 ||
+                          (bz + 1) * B.BZ >= g.nz;
+        const uint64_t biw = scalar_ptr((const uint64_t *)B.info)[gb];
+        const uint2 bi = make_uint2((uint32_t)biw, (uint32_t)(biw >> 32));
+        const auto code = scalar_ptr(brick_code(P, B, bi));
+        const auto f64 = scalar_ptr(P.f64);
+        const auto f32 = scalar_ptr(P.f32);
         const int x = bx * 64 + lane;
         const bool xin = x < g.nx;
         const double px0 = xin ? g.xs[x] : 0.0;
+        const float fx0 = (float)px0;
+        // Wait for that load here: one counter covers vector loads and
+        // stores, so a wait for it inside the row loop would also wait for
+        // the previous rows' stores to complete.
+        __builtin_amdgcn_s_waitcnt(VMCNT0);
+        const Insn in0{code[0].x, code[0].y};
+        const uint32_t kind0 = insn_kind(in0.x);
+        if (bi.y == 1 && insn_op(in0.x) == OP_ATOM && insn_frame(in0.x) == 0 &&
+            (kind0 == A_SPHERE || kind0 == A_RECT || kind0 == A_CAPSULE || kind0 == A_CYLINDER)) {
+            // one atom: dispatched here, once for the brick's rows
+            auto rows = [&](auto KIND) {
+                brick_rows<WPB>(g, B, bits, bx, by, bz, wave, x, xin, edge, ctl, so, [&](double py, double pz) {
+                    return eval_atom<MODE, STATS, false>(P, decltype(KIND)::value, f64 + in0.y, f32 + in0.y, px0,
+                                                         py, pz, fx0, (float)py, (float)pz, xin, 0, gs);
+                });
+            };
+            if (kind0 == A_SPHERE) rows(std::integral_constant<uint32_t, A_SPHERE>());
+            else if (kind0 == A_RECT) rows(std::integral_constant<uint32_t, A_RECT>());
+            else if (kind0 == A_CAPSULE) rows(std::integral_constant<uint32_t, A_CAPSULE>());
+            else rows(std::integral_constant<uint32_t, A_CYLINDER>());
+            continue;
+        }
         for (int j0 = wave; j0 < nrows; j0 += WPB * NP) {
             double px[NP], py[NP], pz[NP];
             bool ok[NP], live[NP], res[NP];
@@ -500,26 +545,13 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
                 ok[q] = j < nrows && yy[q] < g.ny && zz[q] < g.nz;  // wave-uniform
                 live[q] = ok[q] && xin;
                 px[q] = px0;
-                py[q] = ok[q] ? g.ys[yy[q]] : 0.0;
-                pz[q] = ok[q] ? g.zs[zz[q]] : 0.0;
+                py[q] = ok[q] ? scalar_ptr(g.ys)[yy[q]] : 0.0;
+                pz[q] = ok[q] ? scalar_ptr(g.zs)[zz[q]] : 0.0;
             }
-            // two copies of the walk, so that each reads with typed (LDS or
-            // global) instructions rather than generic ones
-            if (staged)
-                eval_code_n<MODE, STATS, NP, true>(s_code, bi.y, P, s_f64, s_f32, px, py, pz, live, res, gs);
-            else
-                eval_code_n<MODE, STATS, NP, true>(brick_code(P, B, bi), bi.y, P, P.f64, P.f32, px, py, pz, live, res,
-                                                   gs);
-            for (int q = 0; q < NP; q++) {
-                if (!ok[q]) continue;  // wave-uniform
-                const int y = yy[q], z = zz[q];
-                const bool inside = res[q] && xin;
-                const uint64_t m = __ballot(inside);
-                if (lane == 0) bits[word_index(g, bx, y, z)] = m;
-                // panic("solid is true outside of bounds"), mc.go:687-689
-                if (inside && (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1))
-                    atomicOr(&ctl->inside, 1u), atomicOr(&sout[si].inside, 1u);
-            }
+            eval_code_n<MODE, STATS, NP, true>(code, bi.y, P, f64, f32, scalar_ptr(P.frames), px, py, pz, live, res,
+                                               gs);
+            for (int q = 0; q < NP; q++)
+                if (ok[q]) row_out(g, bits, bx, yy[q], zz[q], x, res[q] && xin, edge, ctl, so);  // ok: wave-uniform
         }
     }
     flush_stats<STATS>(gs, ctl->gstats, ctl->flops);
@@ -732,7 +764,7 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
     __shared__ uint8_t s_cs[4][64];
     for (int i = threadIdx.x; i < 256 * 16; i += blockDim.x) tab[i] = table[i];
     __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     WordInfo *wi = wi_all[wave];
     const uint32_t nact = (uint32_t)ctl->totals[2];
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
