@@ -119,6 +119,24 @@ struct Pool {
     }
 };
 
+// memcpy of large host buffers split over up to 16 threads (the first touch
+// of freshly allocated pages dominates a single-threaded copy).
+void parallel_copy(void *dst, const void *src, size_t bytes) {
+    constexpr size_t CHUNK = 8u << 20;
+    const size_t nt = std::min<size_t>(16, (bytes + CHUNK - 1) / CHUNK);
+    if (nt <= 1) {
+        if (bytes) memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t per = (bytes + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (size_t i = 0; i < nt; i++) {
+        const size_t a = i * per, b = std::min(bytes, a + per);
+        if (a < b) th.emplace_back([=] { memcpy((char *)dst + a, (const char *)src + a, b - a); });
+    }
+    for (std::thread &t : th) t.join();
+}
+
 unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 20) {
     uint64_t b = (n + block - 1) / block;
     if (b == 0) b = 1;
@@ -168,8 +186,8 @@ struct mc3d_ctx {
 struct mc3d_mesh {
     int device = 0;
     uint64_t nv = 0, nt = 0, voff = 0, toff = 0;
-    std::vector<double> hverts;
-    std::vector<uint32_t> htris;
+    std::unique_ptr<double[]> hverts;  // on_host: 3 nv (allocated without initialisation)
+    std::unique_ptr<uint32_t[]> htris;  // on_host: 3 nt
     bool on_host = false;
     std::shared_ptr<MeshBuffers> buf;
     mc3d_stats stats{};
@@ -881,8 +899,10 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
         const double *hv = (const double *)hs;
         const uint32_t *ht = (const uint32_t *)(hs + nv * 24);
         for (auto &m : meshes) {
-            m->hverts.assign(hv + 3 * m->voff, hv + 3 * (m->voff + m->nv));
-            m->htris.assign(ht + 3 * m->toff, ht + 3 * (m->toff + m->nt));
+            m->hverts.reset(new double[3 * m->nv]);
+            m->htris.reset(new uint32_t[3 * m->nt]);
+            parallel_copy(m->hverts.get(), hv + 3 * m->voff, m->nv * 24);
+            parallel_copy(m->htris.get(), ht + 3 * m->toff, m->nt * 12);
             m->on_host = true;
         }
         const double ms_d2h = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1069,8 +1089,8 @@ int mc3d_mesh_counts(const mc3d_mesh *m, size_t *nv, size_t *nt) {
 int mc3d_mesh_copy(mc3d_mesh *m, double *verts, uint32_t *tris) {
     if (!m) return MC3D_E_INVALID_ARG;
     if (m->on_host) {
-        if (verts && m->nv) memcpy(verts, m->hverts.data(), m->nv * 24);
-        if (tris && m->nt) memcpy(tris, m->htris.data(), m->nt * 12);
+        if (verts && m->nv) parallel_copy(verts, m->hverts.get(), m->nv * 24);
+        if (tris && m->nt) parallel_copy(tris, m->htris.get(), m->nt * 12);
         return MC3D_OK;
     }
     HIP_TRY(hipSetDevice(m->device));
