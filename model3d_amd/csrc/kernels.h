@@ -568,38 +568,46 @@ struct Masks {
 // corner (64wx+i, y, z) to its +x neighbour, and likewise for y and z.
 // Padding bits (x >= nx) and boundary corners are zero, so no checks beyond
 // the neighbour words' existence are needed.
-__device__ __forceinline__ Masks word_masks(const Grid &g, const uint64_t *__restrict__ bits, int wx, int y, int z) {
+// The 2 x 2 x 2 words from (wx, y, z): n[k] is word (wx + (k & 1),
+// y + ((k >> 1) & 1), z + (k >> 2)), zero beyond the grid.  All eight are
+// loaded from clamped addresses (no branches), so the loads issue together.
+struct Nbhd {
+    uint64_t n[8];
+};
+__device__ __forceinline__ Nbhd load_nbhd(const Grid &g, const uint64_t *__restrict__ bits, int wx, int y, int z) {
+    const bool hx = wx + 1 < g.W, hy = y + 1 < g.ny, hz = z + 1 < g.nz;
     const size_t w = word_index(g, wx, y, z);
-    const uint64_t b = bits[w];
-    const uint64_t bn = wx + 1 < g.W ? bits[w + 1] : 0;
-    const uint64_t by = y + 1 < g.ny ? bits[w + g.W] : 0;
-    const uint64_t bz = z + 1 < g.nz ? bits[w + (size_t)g.ny * g.W] : 0;
-    Masks m;
-    m.cx = b ^ ((b >> 1) | (bn << 63));
-    m.cy = b ^ by;
-    m.cz = b ^ bz;
-    return m;
+    const size_t dx = hx ? 1 : 0, dy = hy ? (size_t)g.W : 0, dz = hz ? (size_t)g.ny * g.W : 0;
+    Nbhd r;
+    for (int k = 0; k < 8; k++) r.n[k] = bits[w + ((k & 1) ? dx : 0) + ((k & 2) ? dy : 0) + ((k & 4) ? dz : 0)];
+    for (int k = 0; k < 8; k++)
+        if (((k & 1) && !hx) || ((k & 2) && !hy) || ((k & 4) && !hz)) r.n[k] = 0;
+    return r;
 }
 
-__device__ __forceinline__ uint64_t row_word(const Grid &g, const uint64_t *__restrict__ bits, int wx, int y, int z) {
-    if (wx >= g.W || y >= g.ny || z >= g.nz) return 0;
-    return bits[word_index(g, wx, y, z)];
+// Crossed-edge masks of word n[0]: bit i of cx is the edge from corner
+// (64wx+i, y, z) to its +x neighbour, and likewise for y and z.  Padding
+// bits (x >= nx) and boundary corners are zero, so no checks beyond the
+// neighbour words' existence are needed.
+__device__ __forceinline__ Masks nbhd_masks(const Nbhd &h) {
+    Masks m;
+    m.cx = h.n[0] ^ ((h.n[0] >> 1) | (h.n[1] << 63));
+    m.cy = h.n[0] ^ h.n[2];
+    m.cz = h.n[0] ^ h.n[4];
+    return m;
 }
 
 // c[k] bit i is corner k (k = dx + 2 dy + 4 dz, mc.go:260-301) of cube 64wx+i.
 struct CubeCorners {
     uint64_t c[8];
 };
-__device__ __forceinline__ CubeCorners cube_corners(const Grid &g, const uint64_t *__restrict__ bits, int wx, int y,
-                                                    int z) {
+__device__ __forceinline__ CubeCorners nbhd_corners(const Nbhd &h) {
     CubeCorners cc;
-    for (int dz = 0; dz < 2; dz++)
-        for (int dy = 0; dy < 2; dy++) {
-            const uint64_t b = row_word(g, bits, wx, y + dy, z + dz);
-            const uint64_t bn = row_word(g, bits, wx + 1, y + dy, z + dz);
-            cc.c[2 * dy + 4 * dz] = b;
-            cc.c[1 + 2 * dy + 4 * dz] = (b >> 1) | (bn << 63);
-        }
+    for (int r = 0; r < 4; r++) {  // r = dy + 2 dz
+        const uint64_t b = h.n[2 * r], bn = h.n[2 * r + 1];
+        cc.c[2 * r] = b;
+        cc.c[2 * r + 1] = (b >> 1) | (bn << 63);
+    }
     return cc;
 }
 
@@ -638,9 +646,10 @@ __global__ __launch_bounds__(256) void k_classify(const Seg *__restrict__ segs, 
         }
         int wx, y, z;
         word_coords(g, w, wx, y, z);
-        const Masks m = word_masks(g, bits, wx, y, z);
+        const Nbhd h = load_nbhd(g, bits, wx, y, z);
+        const Masks m = nbhd_masks(h);
         const uint32_t nv = __popcll(m.cx) + __popcll(m.cy) + __popcll(m.cz);
-        const CubeCorners cc = cube_corners(g, bits, wx, y, z);
+        const CubeCorners cc = nbhd_corners(h);
         uint64_t mixed = mixed_cubes(cc);
         uint32_t nt = 0;
         while (mixed) {
@@ -787,13 +796,22 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
         wave_sync();
         if (lane < 8) {  // entry bits: 0 next word, 1 row y+1, 2 row z+1
             const int ex = wx + (lane & 1), ey = y + ((lane >> 1) & 1), ez = z + (lane >> 2);
+            // loads from clamped addresses (no branches), zeroed where the
+            // word or its neighbour lies beyond the grid
+            const bool in = ex < g.W && ey < g.ny && ez < g.nz;
+            const int cx = min(ex, g.W - 1), cy = min(ey, g.ny - 1), cz = min(ez, g.nz - 1);
+            const size_t idx = word_index(g, cx, cy, cz);
+            const bool hx = cx + 1 < g.W, hy = cy + 1 < g.ny, hz = cz + 1 < g.nz;
+            const uint64_t b = bits[idx];
+            const uint64_t bn = bits[idx + (hx ? 1 : 0)], by = bits[idx + (hy ? (size_t)g.W : 0)];
+            const uint64_t bz = bits[idx + (hz ? (size_t)g.ny * g.W : 0)];
+            const uint4 o = offs[idx];
             WordInfo r{0, 0, 0, 0, 0, 0};
-            if (ex < g.W && ey < g.ny && ez < g.nz) {
-                const size_t idx = word_index(g, ex, ey, ez);
-                const Masks m = word_masks(g, bits, ex, ey, ez);
-                r.bits = bits[idx];
-                r.cx = m.cx, r.cy = m.cy, r.cz = m.cz;
-                const uint4 o = offs[idx];
+            if (in) {
+                r.bits = b;
+                r.cx = b ^ ((b >> 1) | ((hx ? bn : 0) << 63));
+                r.cy = b ^ (hy ? by : 0);
+                r.cz = b ^ (hz ? bz : 0);
                 r.base = o.x;
                 r.tbase = o.y;
             }
