@@ -168,6 +168,20 @@ typedef struct {
  * MarchingCubes, mc.go:14-37).  opts may be NULL (FP32 guard + culling). */
 int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
                                const mc3d_options *opts, mc3d_mesh **out);
+/* MarchingCubesInterior(s, delta, iters), model3d/mc.go:55-60: the mesh of
+ * MarchingCubesSearch plus, for every vertex, a nearby point known to be
+ * inside the solid -- the bisection bracket's inside end (mc.go:252-255; with
+ * iters == 0 the inside corner of the vertex's edge).  The reference returns
+ * the points as a CoordMap keyed by vertex; here they come aligned with the
+ * vertices, from mc3d_mesh_copy_interior. */
+int mc3d_marching_cubes_interior(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
+                                 const mc3d_options *opts, mc3d_mesh **out);
+/* MarchingCubesConj(s, delta, iters, xforms...), model3d/mc.go:66-71: the
+ * search of TransformSolid(JoinedTransform(xforms), s), its vertices then
+ * mapped by the joined transform's Inverse() (Mesh.Transform, mesh.go:613-642)
+ * on the device. */
+int mc3d_marching_cubes_conj(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
+                             const mc3d_xform *xforms, size_t n, const mc3d_options *opts, mc3d_mesh **out);
 /* MarchingCubesSearch(solids[i], deltas[i], iters[i]) for i < n in one pass
  * of the pipeline (one launch per stage for all solids; solids with
  * host-evaluated atoms run alone).  out[i] receives solid i's mesh, an
@@ -182,6 +196,10 @@ int mc3d_mesh_copy(mc3d_mesh *m, double *verts, uint32_t *tris);
 /* Device pointers of the results (owned by the mesh; valid until mc3d_mesh_free). */
 int mc3d_mesh_device_ptrs(const mc3d_mesh *m, const double **verts, const uint32_t **tris);
 int mc3d_mesh_stats(const mc3d_mesh *m, mc3d_stats *out);
+/* The interior point of every vertex (3 doubles each, in mc3d_mesh_copy's
+ * vertex order) of a mesh from mc3d_marching_cubes_interior;
+ * MC3D_E_INVALID_ARG for any other mesh. */
+int mc3d_mesh_copy_interior(mc3d_mesh *m, double *interior);
 void mc3d_mesh_free(mc3d_mesh *m);
 
 /* Solid.Contains for n host points on the GPU (out[i] = 0/1). */

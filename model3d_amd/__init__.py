@@ -12,12 +12,15 @@ from ._lib import (  # noqa: F401
 from .mc import (  # noqa: F401
     Contains,
     MarchingCubes,
+    MarchingCubesConj,
+    MarchingCubesInterior,
     MarchingCubesSearch,
     MarchingCubesSearchBatch,
     Mesh,
     NewMeshRect,
     Panic,
     case_table,
+    interior_map,
 )
 from .solids import (  # noqa: F401
     CacheSolidBounds,

@@ -125,6 +125,15 @@ SIGNATURES = {
         ctypes.c_int,
         [_P, _P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(mc3d_options), ctypes.POINTER(_P)],
     ),
+    "mc3d_marching_cubes_interior": (
+        ctypes.c_int,
+        [_P, _P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(mc3d_options), ctypes.POINTER(_P)],
+    ),
+    "mc3d_marching_cubes_conj": (
+        ctypes.c_int,
+        [_P, _P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(mc3d_xform), ctypes.c_size_t,
+         ctypes.POINTER(mc3d_options), ctypes.POINTER(_P)],
+    ),
     "mc3d_marching_cubes_search_batch": (
         ctypes.c_int,
         [_P, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t,
@@ -132,6 +141,7 @@ SIGNATURES = {
     ),
     "mc3d_mesh_counts": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "mc3d_mesh_copy": (ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p]),
+    "mc3d_mesh_copy_interior": (ctypes.c_int, [_P, ctypes.c_void_p]),
     "mc3d_mesh_device_ptrs": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]),
     "mc3d_mesh_stats": (ctypes.c_int, [_P, ctypes.POINTER(mc3d_stats)]),
     "mc3d_mesh_free": (None, [_P]),

@@ -926,7 +926,7 @@ __device__ __forceinline__ bool corner_bit(const Grid &g, const uint64_t *__rest
 template <int MODE, bool STATS>
 __device__ __forceinline__ void bisect_vertex(const Seg &sg, const uint64_t *__restrict__ bits_all,
                                               const uint64_t *__restrict__ vkeys, double *__restrict__ verts,
-                                              uint64_t v, GStats &gs) {
+                                              double *__restrict__ interior, uint64_t v, GStats &gs) {
     const DevProg &P = sg.P;
     const Grid &g = sg.g;
     const Bricks &B = sg.B;
@@ -942,9 +942,10 @@ __device__ __forceinline__ void bisect_vertex(const Seg &sg, const uint64_t *__r
         if (axis == 0) lo = g.xs[x], hi = g.xs[x + 1], hx++;
         else if (axis == 1) lo = g.ys[y], hi = g.ys[y + 1], hy++;
         else lo = g.zs[z], hi = g.zs[z + 1], hz++;
-        double res;
+        double res, inside;  // the vertex; the bracket's inside end (mc.go:252-255)
         if (iters == 0) {
             res = (lo + hi) * 0.5;
+            inside = interior && !corner_bit(g, bits, hx, hy, hz) ? lo : hi;
         } else {
             double fp = lo, tp = hi;
             if (!corner_bit(g, bits, hx, hy, hz)) fp = hi, tp = lo;
@@ -968,12 +969,19 @@ __device__ __forceinline__ void bisect_vertex(const Seg &sg, const uint64_t *__r
                 else fp = mid;
             }
             res = (fp + tp) / 2;
+            inside = tp;
             if (STATS) gs.flops64 += 2 * (iters + 1);
         }
         p[axis] = res;
         verts[3 * v] = p[0];
         verts[3 * v + 1] = p[1];
         verts[3 * v + 2] = p[2];
+        if (interior) {
+            p[axis] = inside;
+            interior[3 * v] = p[0];
+            interior[3 * v + 1] = p[1];
+            interior[3 * v + 2] = p[2];
+        }
     }
 }
 
@@ -982,7 +990,7 @@ __global__ __launch_bounds__(256) void k_bisect(const Seg *__restrict__ segs, ui
                                                 const uint32_t *__restrict__ vseg,
                                                 const uint64_t *__restrict__ bits_all,
                                                 const uint64_t *__restrict__ vkeys, double *__restrict__ verts,
-                                                Ctl *__restrict__ ctl, uint64_t vcap) {
+                                                double *__restrict__ interior, Ctl *__restrict__ ctl, uint64_t vcap) {
     GStats gs;
     const uint64_t nv = min((uint64_t)ctl->totals[0], vcap);
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
@@ -992,7 +1000,7 @@ __global__ __launch_bounds__(256) void k_bisect(const Seg *__restrict__ segs, ui
         for (;;) {
             const uint32_t si = (uint32_t)__builtin_amdgcn_readfirstlane((int)mine);
             if (mine != si) continue;
-            bisect_vertex<MODE, STATS>(pick_seg<MULTI>(segs, seg0, si), bits_all, vkeys, verts, v, gs);
+            bisect_vertex<MODE, STATS>(pick_seg<MULTI>(segs, seg0, si), bits_all, vkeys, verts, interior, v, gs);
             break;
         }
     }
@@ -1048,7 +1056,7 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_bisect_host(DevProg P, Grid g, const uint64_t *__restrict__ bits,
                                                      const uint64_t *__restrict__ vkeys, double *__restrict__ state,
                                                      double *__restrict__ probe, double *__restrict__ verts,
-                                                     uint64_t nv, int phase) {
+                                                     double *__restrict__ interior, uint64_t nv, int phase) {
     GStats gs;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
          v += (uint64_t)gridDim.x * blockDim.x) {
@@ -1076,6 +1084,26 @@ __global__ __launch_bounds__(256) void k_bisect_host(DevProg P, Grid g, const ui
         p[axis] = (fp + tp) / 2;
         probe[3 * v] = p[0], probe[3 * v + 1] = p[1], probe[3 * v + 2] = p[2];
         verts[3 * v] = p[0], verts[3 * v + 1] = p[1], verts[3 * v + 2] = p[2];
+        if (interior) {  // the bracket's inside end after this round (mc.go:252-255)
+            p[axis] = tp;
+            interior[3 * v] = p[0], interior[3 * v + 1] = p[1], interior[3 * v + 2] = p[2];
+        }
+    }
+}
+
+// Mesh.Transform(t.Inverse()) of MarchingCubesConj (mc.go:66-71,
+// mesh.go:613-642): every vertex mapped by the inverse chain, given as frame
+// records {kind, v[9]} (the Apply of each op, transform.go:44-46, 71-73,
+// 103-105, 129-131).
+__global__ __launch_bounds__(256) void k_inverse_apply(const double *__restrict__ recs, uint32_t nops,
+                                                       double *__restrict__ verts, const Ctl *__restrict__ ctl,
+                                                       uint64_t vcap) {
+    const uint64_t nv = min((uint64_t)ctl->totals[0], vcap);
+    const Frame f{0, nops};
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (uint64_t)gridDim.x * blockDim.x) {
+        double x = verts[3 * v], y = verts[3 * v + 1], z = verts[3 * v + 2];
+        apply_frame(recs, f, x, y, z);
+        verts[3 * v] = x, verts[3 * v + 1] = y, verts[3 * v + 2] = z;
     }
 }
 

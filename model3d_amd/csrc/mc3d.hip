@@ -149,11 +149,12 @@ size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 // it returned and given back to the context's pool with the last of them.
 struct MeshBuffers {
     std::shared_ptr<Pool> pool;
-    std::unique_ptr<DevBuf> verts, tris;
+    std::unique_ptr<DevBuf> verts, tris, interior;  // interior: MarchingCubesInterior only
     ~MeshBuffers() {
         if (pool) {
             pool->give(std::move(verts));
             pool->give(std::move(tris));
+            pool->give(std::move(interior));
         }
     }
 };
@@ -188,7 +189,8 @@ struct mc3d_mesh {
     uint64_t nv = 0, nt = 0, voff = 0, toff = 0;
     std::unique_ptr<double[]> hverts;  // on_host: 3 nv (allocated without initialisation)
     std::unique_ptr<uint32_t[]> htris;  // on_host: 3 nt
-    bool on_host = false;
+    std::unique_ptr<double[]> hinterior;  // on_host with interior points: 3 nv
+    bool on_host = false, has_interior = false;
     std::shared_ptr<MeshBuffers> buf;
     mc3d_stats stats{};
 };
@@ -208,6 +210,8 @@ struct Job {
     uint32_t cap = 0;  // folded-program slot of its culling bricks (0: no culling)
     bool host = false;
     std::vector<uint32_t> frames;  // frame of each host-evaluated atom
+    bool interior = false;         // MarchingCubesInterior: keep each vertex's inside point
+    std::vector<double> conj;      // MarchingCubesConj: inverse chain applied to the vertices
 };
 
 // Frame of each host-evaluated atom, by host atom index.
@@ -342,6 +346,8 @@ struct Upload {
     uint32_t *word_seg = nullptr, *brick_seg = nullptr;
     std::vector<Seg> hsegs;  // host copies of the segments
     size_t back_bytes = 0;   // control block + per-solid results, copied back at the end
+    const double *conj = nullptr;  // job 0's MarchingCubesConj inverse chain (records of 10 doubles)
+    uint32_t conj_nops = 0;
 };
 
 // Packs the control block, the per-solid results, the segments, the segment
@@ -373,6 +379,7 @@ int upload(mc3d_ctx *ctx, std::vector<Job> &jobs, uint64_t nwords_all, uint64_t 
     const bool tables = n > 1;
     const size_t o_wseg = place(tables ? nwords_all / 64 * sizeof(uint32_t) : 0);
     const size_t o_bseg = place(tables ? nbricks_all / 128 * sizeof(uint32_t) : 0);
+    const size_t o_conj = place(jobs[0].conj.size() * sizeof(double));
     struct Off {
         size_t code, f64, f32, fr, box, col, ax;
     };
@@ -437,6 +444,11 @@ int upload(mc3d_ctx *ctx, std::vector<Job> &jobs, uint64_t nwords_all, uint64_t 
         sg.stage = j.cap && plan_stage_bytes(P) <= PLAN_STAGE_LIMIT ? 1 : 0;
     }
     memcpy(h + o_segs, U.hsegs.data(), n * sizeof(Seg));
+    if (!jobs[0].conj.empty()) {
+        memcpy(h + o_conj, jobs[0].conj.data(), jobs[0].conj.size() * sizeof(double));
+        U.conj = (const double *)(dv + o_conj);
+        U.conj_nops = (uint32_t)(jobs[0].conj.size() / 10);
+    }
     if (tables) {
         uint32_t *ws = (uint32_t *)(h + o_wseg), *bs = (uint32_t *)(h + o_bseg);
         for (size_t s = 0; s < n; s++) {
@@ -531,6 +543,7 @@ struct Call {
     bool cull = false;
     int mode = 0, rows_per_walk = 1, wpb = 4;
     bool stats = false;
+    bool interior = false;  // a call of one MarchingCubesInterior solid
 };
 
 template <int MODE, bool STATS, int NP, bool MULTI>
@@ -576,16 +589,18 @@ void launch_corners(mc3d_ctx *ctx, const Call &c, const Upload &U) {
 }
 
 template <int MODE, bool STATS>
-void launch_bisect(mc3d_ctx *ctx, const Call &c, const Upload &U, double *verts, uint64_t vcap) {
+void launch_bisect(mc3d_ctx *ctx, const Call &c, const Upload &U, double *verts, double *interior,
+                   uint64_t vcap) {
     const unsigned grid = grid_for(vcap, 256, 1u << 14);
     if (c.nsegs > 1)
-        k_bisect<MODE, STATS, true><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0],
-                                                                   ctx->vseg.as<uint32_t>(), ctx->bits.as<uint64_t>(),
-                                                                   ctx->vkeys.as<uint64_t>(), verts, U.ctl, vcap);
+        k_bisect<MODE, STATS, true><<<grid, 256, 0, ctx->stream>>>(
+            U.segs, c.nsegs, U.hsegs[0], ctx->vseg.as<uint32_t>(), ctx->bits.as<uint64_t>(),
+            ctx->vkeys.as<uint64_t>(), verts, interior, U.ctl, vcap);
     else
         k_bisect<MODE, STATS, false><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], nullptr,
                                                                     ctx->bits.as<uint64_t>(),
-                                                                    ctx->vkeys.as<uint64_t>(), verts, U.ctl, vcap);
+                                                                    ctx->vkeys.as<uint64_t>(), verts, interior,
+                                                                    U.ctl, vcap);
 }
 
 int take_outputs(mc3d_ctx *ctx, MeshBuffers &mb, const Call &c, uint64_t vcap, uint64_t tcap) {
@@ -602,6 +617,13 @@ int take_outputs(mc3d_ctx *ctx, MeshBuffers &mb, const Call &c, uint64_t vcap, u
         (rc = ctx->vkeys.ensure(vcap * 8)))
         return rc;
     if (c.nsegs > 1 && (rc = ctx->vseg.ensure(vcap * 4))) return rc;
+    if (c.interior) {
+        if (!mb.interior || mb.interior->cap < vcap * 24) {
+            ctx->pool->give(std::move(mb.interior));
+            mb.interior = ctx->pool->take(vcap * 24);
+        }
+        if ((rc = mb.interior->ensure(vcap * 24))) return rc;
+    }
     return MC3D_OK;
 }
 
@@ -620,7 +642,8 @@ int fit_outputs(mc3d_ctx *ctx, MeshBuffers &mb, uint64_t nv, uint64_t nt) {
         return MC3D_OK;
     };
     int rc;
-    if ((rc = fit(mb.verts, nv * 24)) || (rc = fit(mb.tris, nt * 12))) return rc;
+    if ((rc = fit(mb.verts, nv * 24)) || (rc = fit(mb.tris, nt * 12)) || (rc = fit(mb.interior, nv * 24)))
+        return rc;
     return MC3D_OK;
 }
 
@@ -646,11 +669,19 @@ int run_output(mc3d_ctx *ctx, const Call &c, const Upload &U, MeshBuffers &mb, u
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
     double *verts = mb.verts->as<double>();
+    double *inter = c.interior ? mb.interior->as<double>() : nullptr;
     if (c.mode == 0)
-        c.stats ? launch_bisect<0, true>(ctx, c, U, verts, vcap) : launch_bisect<0, false>(ctx, c, U, verts, vcap);
+        c.stats ? launch_bisect<0, true>(ctx, c, U, verts, inter, vcap)
+                : launch_bisect<0, false>(ctx, c, U, verts, inter, vcap);
     else
-        c.stats ? launch_bisect<1, true>(ctx, c, U, verts, vcap) : launch_bisect<1, false>(ctx, c, U, verts, vcap);
+        c.stats ? launch_bisect<1, true>(ctx, c, U, verts, inter, vcap)
+                : launch_bisect<1, false>(ctx, c, U, verts, inter, vcap);
     HIP_TRY(hipGetLastError());
+    if (U.conj) {
+        k_inverse_apply<<<grid_for(vcap, 256, 1u << 14), 256, 0, ctx->stream>>>(U.conj, U.conj_nops, verts, U.ctl,
+                                                                                 vcap);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream));
     return MC3D_OK;
 }
@@ -682,18 +713,23 @@ int run_output_host(mc3d_ctx *ctx, const Call &c, const Job &j, Upload &U, MeshB
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
     double *verts = mb.verts->as<double>(), *state = ctx->hstate.as<double>(), *probe = ctx->hprobe.as<double>();
+    double *inter = c.interior ? mb.interior->as<double>() : nullptr;
     const unsigned grid = grid_for(nv, 256, 1u << 14);
     for (int round = 0; round <= j.iters; round++) {
         if (round > 0 && (rc = host_round(ctx, j, U, 0, probe, nv, ms_cb))) return rc;
         const DevProg &dp = U.hsegs[0].P;
         if (c.mode == 1)
             k_bisect_host<1><<<grid, 256, 0, ctx->stream>>>(dp, U.hsegs[0].g, ctx->bits.as<uint64_t>(),
-                                                            ctx->vkeys.as<uint64_t>(), state, probe, verts, nv,
-                                                            round > 0);
+                                                            ctx->vkeys.as<uint64_t>(), state, probe, verts, inter,
+                                                            nv, round > 0);
         else
             k_bisect_host<0><<<grid, 256, 0, ctx->stream>>>(dp, U.hsegs[0].g, ctx->bits.as<uint64_t>(),
-                                                            ctx->vkeys.as<uint64_t>(), state, probe, verts, nv,
-                                                            round > 0);
+                                                            ctx->vkeys.as<uint64_t>(), state, probe, verts, inter,
+                                                            nv, round > 0);
+        HIP_TRY(hipGetLastError());
+    }
+    if (U.conj) {
+        k_inverse_apply<<<grid_for(nv, 256, 1u << 14), 256, 0, ctx->stream>>>(U.conj, U.conj_nops, verts, U.ctl, nv);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream));
@@ -719,6 +755,7 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
     c.rows_per_walk = o.rows_per_walk > 0 ? o.rows_per_walk : 1;
     c.wpb = o.waves_per_brick > 0 ? o.waves_per_brick : 4;
     c.cull = jobs[0].cap != 0;
+    c.interior = c.nsegs == 1 && jobs[0].interior;
     const bool host = c.nsegs == 1 && jobs[0].host;
     for (Job &j : jobs) {
         j.word0 = c.nwords;
@@ -874,6 +911,7 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
         const Job &j = jobs[s];
         m->device = ctx->device;
         m->buf = mb;
+        m->has_interior = c.interior;
         m->voff = so[s].vbase;
         m->toff = so[s].tbase;
         m->nv = (s + 1 < jobs.size() ? so[s + 1].vbase : nv) - m->voff;
@@ -891,10 +929,14 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
     if (!o.device_only) {
         auto t0 = std::chrono::steady_clock::now();
         // one copy of all outputs through pinned memory, then split per mesh
-        if ((rc = ctx->outstage.ensure(nv * 24 + nt * 12 + 16))) return rc;
+        const size_t ibytes = c.interior ? nv * 24 : 0;
+        if ((rc = ctx->outstage.ensure(nv * 24 + nt * 12 + ibytes + 16))) return rc;
         char *hs = (char *)ctx->outstage.p;
         if (nv) HIP_TRY(hipMemcpyAsync(hs, mb->verts->p, nv * 24, hipMemcpyDeviceToHost, ctx->stream));
         if (nt) HIP_TRY(hipMemcpyAsync(hs + nv * 24, mb->tris->p, nt * 12, hipMemcpyDeviceToHost, ctx->stream));
+        if (ibytes)
+            HIP_TRY(hipMemcpyAsync(hs + nv * 24 + nt * 12, mb->interior->p, ibytes, hipMemcpyDeviceToHost,
+                                   ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         const double *hv = (const double *)hs;
         const uint32_t *ht = (const uint32_t *)(hs + nv * 24);
@@ -903,6 +945,11 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
             m->htris.reset(new uint32_t[3 * m->nt]);
             parallel_copy(m->hverts.get(), hv + 3 * m->voff, m->nv * 24);
             parallel_copy(m->htris.get(), ht + 3 * m->toff, m->nt * 12);
+            if (ibytes) {
+                const double *hi = (const double *)(hs + nv * 24 + nt * 12);
+                m->hinterior.reset(new double[3 * m->nv]);
+                parallel_copy(m->hinterior.get(), hi + 3 * m->voff, m->nv * 24);
+            }
             m->on_host = true;
         }
         const double ms_d2h = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -978,18 +1025,20 @@ void mc3d_ctx_destroy(mc3d_ctx *ctx) {
     delete ctx;
 }
 
-int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
-                               const mc3d_options *opts, mc3d_mesh **out) {
-    if (!ctx || !s || !out) {
-        set_error("mc3d_marching_cubes_search: null argument");
-        return MC3D_E_INVALID_ARG;
-    }
+namespace {
+// One solid, one call: MarchingCubesSearch, with interior points
+// (MarchingCubesInterior) or with an inverse chain for the vertices
+// (MarchingCubesConj).
+int search_one(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters, const mc3d_options *opts,
+               mc3d_mesh **out, bool interior, std::vector<double> conj) {
     *out = nullptr;
     const auto t_call = std::chrono::steady_clock::now();
     const mc3d_options o = default_options(opts);
     HIP_TRY(hipSetDevice(ctx->device));
     std::vector<Job> jobs(1);
     jobs[0].s = s, jobs[0].delta = delta, jobs[0].iters = iters;
+    jobs[0].interior = interior;
+    jobs[0].conj = std::move(conj);
     int rc = prepare_job(jobs[0], o);
     if (rc) return rc;
     const double ms_prepare = ms_since(t_call);
@@ -997,6 +1046,40 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
     if ((rc = run_call(ctx, jobs, o, meshes, t_call, ms_prepare))) return rc;
     *out = meshes[0];
     return MC3D_OK;
+}
+}  // namespace
+
+int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
+                               const mc3d_options *opts, mc3d_mesh **out) {
+    if (!ctx || !s || !out) {
+        set_error("mc3d_marching_cubes_search: null argument");
+        return MC3D_E_INVALID_ARG;
+    }
+    return search_one(ctx, s, delta, iters, opts, out, false, {});
+}
+
+int mc3d_marching_cubes_interior(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
+                                 const mc3d_options *opts, mc3d_mesh **out) {
+    if (!ctx || !s || !out) {
+        set_error("mc3d_marching_cubes_interior: null argument");
+        return MC3D_E_INVALID_ARG;
+    }
+    return search_one(ctx, s, delta, iters, opts, out, true, {});
+}
+
+int mc3d_marching_cubes_conj(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
+                             const mc3d_xform *xforms, size_t n, const mc3d_options *opts, mc3d_mesh **out) {
+    if (!ctx || !s || !out || (n && !xforms)) {
+        set_error("mc3d_marching_cubes_conj: null argument");
+        return MC3D_E_INVALID_ARG;
+    }
+    *out = nullptr;
+    // TransformSolid(JoinedTransform(xforms), s), mc.go:67-68
+    mc3d_solid *t = mc3d_transform(const_cast<mc3d_solid *>(s), xforms, n);
+    if (!t) return MC3D_E_INVALID_ARG;  // the error is set
+    const int rc = search_one(ctx, t, delta, iters, opts, out, false, inverse_records(xforms, n));
+    mc3d_solid_release(t);
+    return rc;
 }
 
 int mc3d_marching_cubes_search_batch(mc3d_ctx *ctx, const mc3d_solid *const *solids, const double *deltas,
@@ -1098,6 +1181,21 @@ int mc3d_mesh_copy(mc3d_mesh *m, double *verts, uint32_t *tris) {
         HIP_TRY(hipMemcpy(verts, m->buf->verts->as<double>() + 3 * m->voff, m->nv * 24, hipMemcpyDeviceToHost));
     if (tris && m->nt)
         HIP_TRY(hipMemcpy(tris, m->buf->tris->as<uint32_t>() + 3 * m->toff, m->nt * 12, hipMemcpyDeviceToHost));
+    return MC3D_OK;
+}
+
+int mc3d_mesh_copy_interior(mc3d_mesh *m, double *interior) {
+    if (!m || !m->has_interior) {
+        set_error("mc3d_mesh_copy_interior: not a mesh of mc3d_marching_cubes_interior");
+        return MC3D_E_INVALID_ARG;
+    }
+    if (!interior || !m->nv) return MC3D_OK;
+    if (m->on_host) {
+        parallel_copy(interior, m->hinterior.get(), m->nv * 24);
+        return MC3D_OK;
+    }
+    HIP_TRY(hipSetDevice(m->device));
+    HIP_TRY(hipMemcpy(interior, m->buf->interior->as<double>() + 3 * m->voff, m->nv * 24, hipMemcpyDeviceToHost));
     return MC3D_OK;
 }
 

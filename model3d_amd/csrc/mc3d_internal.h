@@ -127,6 +127,8 @@ using SolidPtr = std::shared_ptr<const Solid>;
 // Lowers a solid tree to a device program.  Returns false (with msg) when a
 // node cannot be represented.
 bool lower_solid(const Solid &s, Program &out, std::string &msg);
+// JoinedTransform(xforms).Inverse() as frame records {kind, v[9]} per op.
+std::vector<double> inverse_records(const mc3d_xform *xforms, size_t n);
 void solid_bounds(const Solid &s, double mn[3], double mx[3]);
 bool bounds_valid(const Solid &s);
 

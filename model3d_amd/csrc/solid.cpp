@@ -862,6 +862,23 @@ mc3d_solid *wrap(Solid *s) {
 std::mutex g_ref_mu;
 }  // namespace
 
+namespace mc3d {
+// JoinedTransform(xforms).Inverse() (transform.go:213-219) as frame records
+// {kind, v[9]}: the inverses of the ops, last op first.
+std::vector<double> inverse_records(const mc3d_xform *xforms, size_t n) {
+    std::vector<double> r;
+    for (size_t i = n; i-- > 0;) {
+        Xf t;
+        t.kind = xforms[i].kind;
+        memcpy(t.v, xforms[i].v, sizeof t.v);
+        const Xf inv = xf_inverse(t);
+        r.push_back((double)inv.kind);
+        r.insert(r.end(), inv.v, inv.v + 9);
+    }
+    return r;
+}
+}  // namespace mc3d
+
 extern "C" {
 
 mc3d_solid *mc3d_sphere(const double center[3], double radius) {

@@ -136,6 +136,80 @@ def MarchingCubesSearch(
         b.close()
 
 
+def MarchingCubesInterior(
+    s: Solid,
+    delta: float,
+    iters: int,
+    *,
+    precision: int = _lib.MC3D_PREC_FP32_GUARD,
+    cull: bool = True,
+    device: int = 0,
+):
+    """model3d.MarchingCubesInterior (mc.go:55-60) on the GPU: the mesh of
+    MarchingCubesSearch and, per vertex, a nearby point known to be inside
+    the solid (the bracket's inside end, mc.go:252-255).  The reference
+    returns a CoordMap from vertex to interior point; here ``interior[i]`` is
+    the point of ``mesh.vertices[i]`` (``interior_map`` builds the mapping)."""
+    L = _lib.lib()
+    ctx = _lib.context(device)
+    b = LibBackend()
+    h = s._build(b)
+    out = ctypes.c_void_p()
+    try:
+        opts = _options(precision, cull)
+        _raise_for(L.mc3d_marching_cubes_interior(ctx, h, float(delta), int(iters), ctypes.byref(opts),
+                                                  ctypes.byref(out)))
+        mesh = _fetch(L, out)
+        interior = np.empty((len(mesh.vertices), 3), dtype=np.float64)
+        _lib.check(L.mc3d_mesh_copy_interior(out, interior.ctypes.data), "mc3d_mesh_copy_interior")
+        return mesh, interior
+    finally:
+        if out.value:
+            L.mc3d_mesh_free(out)
+        b.release(h)
+        b.close()
+
+
+def interior_map(mesh: Mesh, interior: np.ndarray) -> dict:
+    """The CoordMap[Coord3D] of MarchingCubesInterior: vertex -> interior point."""
+    return {tuple(v): tuple(p) for v, p in zip(mesh.vertices.tolist(), interior.tolist())}
+
+
+def MarchingCubesConj(
+    s: Solid,
+    delta: float,
+    iters: int,
+    *xforms,
+    precision: int = _lib.MC3D_PREC_FP32_GUARD,
+    cull: bool = True,
+    device: int = 0,
+) -> Mesh:
+    """model3d.MarchingCubesConj (mc.go:66-71) on the GPU: MarchingCubesSearch
+    of TransformSolid(JoinedTransform(xforms), s), then every vertex mapped by
+    the joined transform's Inverse() (on the device)."""
+    L = _lib.lib()
+    ctx = _lib.context(device)
+    b = LibBackend()
+    h = s._build(b)
+    out = ctypes.c_void_p()
+    ops = [op for t in xforms for op in t._ops()]
+    arr = (_lib.mc3d_xform * max(1, len(ops)))()
+    for i, (k, v) in enumerate(ops):
+        arr[i].kind = k
+        for j in range(9):
+            arr[i].v[j] = v[j]
+    try:
+        opts = _options(precision, cull)
+        _raise_for(L.mc3d_marching_cubes_conj(ctx, h, float(delta), int(iters), arr, len(ops), ctypes.byref(opts),
+                                              ctypes.byref(out)))
+        return _fetch(L, out)
+    finally:
+        if out.value:
+            L.mc3d_mesh_free(out)
+        b.release(h)
+        b.close()
+
+
 def _raise_for(rc: int):
     """A reference panic as Panic (same message), anything else as MC3DError."""
     if rc != _lib.MC3D_OK:

@@ -47,6 +47,12 @@ _SIG = {
         ctypes.c_int,
         [_P, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)],
     ),
+    "or_marching_cubes_interior": (
+        ctypes.c_int,
+        [_P, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)],
+    ),
+    "or_mesh_interior": (_dp, [_P, ctypes.POINTER(ctypes.c_long)]),
+    "or_inverse_apply": (None, [ctypes.POINTER(ctypes.c_int), _dp, ctypes.c_int, _dp, ctypes.c_long]),
     "or_mesh_ntris": (ctypes.c_long, [_P]),
     "or_mesh_tris": (_dp, [_P]),
     "or_mesh_info": (
@@ -189,15 +195,18 @@ class OracleError(RuntimeError):
         self.code = code
 
 
-def oracle_search(solid, delta, iters, threads=None):
-    """MarchingCubesSearch on the CPU: returns (triangles (T, 9), info dict)."""
+def oracle_search(solid, delta, iters, threads=None, interior=False):
+    """MarchingCubesSearch on the CPU: returns (triangles (T, 9), info dict).
+    With ``interior`` (MarchingCubesInterior) info["interior"] holds, per
+    unique vertex, its searched position and interior point (V, 6)."""
     if threads is None:
         threads = os.cpu_count() or 1
     o = OracleSolid(solid)
     L = o.backend.L
     m = _P()
     t0 = time.perf_counter()
-    rc = L.or_marching_cubes_search(o.h, float(delta), int(iters), int(threads), ctypes.byref(m))
+    fn = L.or_marching_cubes_interior if interior else L.or_marching_cubes_search
+    rc = fn(o.h, float(delta), int(iters), int(threads), ctypes.byref(m))
     wall = time.perf_counter() - t0
     if rc:
         raise OracleError(rc)
@@ -213,9 +222,36 @@ def oracle_search(solid, delta, iters, threads=None):
             corners=c.value, vertices=v.value, search_evals=e.value, ms_scan=s1.value, ms_search=s2.value, wall_s=wall,
             threads=threads,
         )
+        if interior:
+            ni = ctypes.c_long()
+            p = L.or_mesh_interior(m, ctypes.byref(ni))
+            info["interior"] = (
+                np.ctypeslib.as_array(p, shape=(ni.value * 6,)).reshape(-1, 6).copy() if ni.value else np.zeros((0, 6))
+            )
         return tris, info
     finally:
         L.or_mesh_free(m)
+
+
+def oracle_inverse_apply(transform, pts) -> np.ndarray:
+    """transform.Inverse().Apply at every point (rows of 3), on the CPU."""
+    ops = transform._ops()
+    kinds = (ctypes.c_int * max(1, len(ops)))(*[k for k, _ in ops])
+    params = (ctypes.c_double * max(9, 9 * len(ops)))(*[x for _, v in ops for x in v])
+    out = np.ascontiguousarray(np.array(pts, dtype=np.float64).reshape(-1, 3))
+    lib().or_inverse_apply(kinds, params, len(ops), out.ctypes.data_as(_dp), len(out))
+    return out
+
+
+def oracle_conj(solid, delta, iters, *xforms, threads=None):
+    """MarchingCubesConj (mc.go:66-71) on the CPU: the search of
+    TransformSolid(JoinedTransform(xforms), solid), each vertex then mapped by
+    the joined transform's inverse.  Returns triangles (T, 9)."""
+    from model3d_amd.solids import JoinedTransform, TransformSolid
+
+    joined = JoinedTransform(xforms)
+    tris, info = oracle_search(TransformSolid(joined, solid), delta, iters, threads)
+    return oracle_inverse_apply(joined, tris).reshape(-1, 9), info
 
 
 def oracle_case_table() -> np.ndarray:

@@ -902,6 +902,8 @@ struct or_mesh {
     double *tris;          /* 9 doubles per triangle */
     long n_corners, n_vertices, n_search_evals;
     double ms_scan, ms_search;
+    double *interior;      /* MarchingCubesInterior: per unique vertex {searched vertex, interior point} */
+    long n_interior;
 };
 
 static int mesh_push(or_mesh *m, const c3 *v) {
@@ -958,7 +960,8 @@ static int lookup_edge_point(const spacer *sp, const double *arr, int *axis, dou
     return OR_E_NOT_ON_EDGE;
 }
 
-static int search_point(const or_node *s, int iters, const spacer *sp, const double *c, double *out) {
+static int search_point(const or_node *s, int iters, const spacer *sp, const double *c, double *out,
+                        double *interior) {
     int axis;
     double false_pt, true_pt;
     int err = lookup_edge_point(sp, c, &axis, &false_pt, &true_pt);
@@ -979,6 +982,10 @@ static int search_point(const or_node *s, int iters, const spacer *sp, const dou
     }
     arr[axis] = (false_pt + true_pt) / 2;
     memcpy(out, arr, 3 * sizeof(double));
+    if (interior) { /* mc.go:252-255 */
+        arr[axis] = true_pt;
+        memcpy(interior, arr, 3 * sizeof(double));
+    }
     return 0;
 }
 
@@ -988,6 +995,7 @@ typedef struct {
     int iters;
     const double *in;
     double *out;
+    double *interior; /* or NULL */
     long n;
     long next;
     int error;
@@ -1004,7 +1012,8 @@ static void *search_worker(void *arg) {
         if (start >= st->n) return NULL;
         long end = start + 256 < st->n ? start + 256 : st->n;
         for (long i = start; i < end; i++) {
-            int err = search_point(st->s, st->iters, st->sp, st->in + 3 * i, st->out + 3 * i);
+            int err = search_point(st->s, st->iters, st->sp, st->in + 3 * i, st->out + 3 * i,
+                                   st->interior ? st->interior + 3 * i : NULL);
             if (err) {
                 pthread_mutex_lock(&st->mu);
                 st->error = err;
@@ -1021,9 +1030,11 @@ static double now_ms(void) {
 }
 
 /* mcSearch, mc.go:182-221: unique vertices, parallel search, in-place
- * rewrite of every triangle corner that equals the original vertex. */
-static int mc_search(const or_node *s, double delta, int iters, int threads, or_mesh *m) {
-    if (iters == 0) return 0;
+ * rewrite of every triangle corner that equals the original vertex.  With
+ * want_interior the interior point of each unique vertex is kept beside its
+ * searched position (the CoordMap of MarchingCubesInterior, mc.go:191-198). */
+static int mc_search(const or_node *s, double delta, int iters, int threads, or_mesh *m, int want_interior) {
+    if (iters == 0 && !want_interior) return 0;
     spacer sp;
     int err = spacer_make(s, delta, &sp);
     if (err) {
@@ -1063,6 +1074,7 @@ static int mc_search(const or_node *s, double delta, int iters, int threads, or_
     }
     free(table);
     double *outv = (double *)malloc((nv > 0 ? nv : 1) * 3 * sizeof(double));
+    double *inter = want_interior ? (double *)malloc((nv > 0 ? nv : 1) * 3 * sizeof(double)) : NULL;
     search_state st;
     memset(&st, 0, sizeof st);
     st.s = s;
@@ -1070,6 +1082,7 @@ static int mc_search(const or_node *s, double delta, int iters, int threads, or_
     st.iters = iters;
     st.in = uniq;
     st.out = outv;
+    st.interior = inter;
     st.n = nv;
     pthread_mutex_init(&st.mu, NULL);
     if (threads < 1) threads = 1;
@@ -1081,7 +1094,16 @@ static int mc_search(const or_node *s, double delta, int iters, int threads, or_
     err = st.error;
     if (!err) {
         for (long k = 0; k < nslot; k++) memcpy(m->tris + 3 * k, outv + 3 * slot_vertex[k], 3 * sizeof(double));
+        if (inter) {
+            m->interior = (double *)malloc((nv > 0 ? nv : 1) * 6 * sizeof(double));
+            for (long v = 0; v < nv; v++) {
+                memcpy(m->interior + 6 * v, outv + 3 * v, 3 * sizeof(double));
+                memcpy(m->interior + 6 * v + 3, inter + 3 * v, 3 * sizeof(double));
+            }
+            m->n_interior = nv;
+        }
     }
+    free(inter);
     m->n_vertices = nv;
     m->n_search_evals = nv * (long)(1 + iters);
     free(outv);
@@ -1091,8 +1113,10 @@ static int mc_search(const or_node *s, double delta, int iters, int threads, or_
     return err;
 }
 
-/* MarchingCubes (mc.go:14-37) followed by mcSearch (mc.go:182-221). */
-int or_marching_cubes_search(const or_node *s, double delta, int iters, int threads, or_mesh **out) {
+/* MarchingCubes (mc.go:14-37) followed by mcSearch (mc.go:182-221); with
+ * want_interior, MarchingCubesInterior (mc.go:55-60). */
+static int marching_cubes(const or_node *s, double delta, int iters, int threads, int want_interior,
+                          or_mesh **out) {
     *out = NULL;
     if (!bounds_valid(s)) return OR_E_INVALID_BOUNDS;
     pthread_once(&g_table_once, init_table);
@@ -1190,7 +1214,7 @@ int or_marching_cubes_search(const or_node *s, double delta, int iters, int thre
     spacer_free(&sp);
     double t1 = now_ms();
     m->ms_scan = t1 - t0;
-    if (!err) err = mc_search(s, delta, iters, threads, m);
+    if (!err) err = mc_search(s, delta, iters, threads, m, want_interior);
     m->ms_search = now_ms() - t1;
     if (err) {
         or_mesh_free(m);
@@ -1198,6 +1222,40 @@ int or_marching_cubes_search(const or_node *s, double delta, int iters, int thre
     }
     *out = m;
     return 0;
+}
+
+int or_marching_cubes_search(const or_node *s, double delta, int iters, int threads, or_mesh **out) {
+    return marching_cubes(s, delta, iters, threads, 0, out);
+}
+
+int or_marching_cubes_interior(const or_node *s, double delta, int iters, int threads, or_mesh **out) {
+    return marching_cubes(s, delta, iters, threads, 1, out);
+}
+
+const double *or_mesh_interior(const or_mesh *m, long *n) {
+    *n = m->n_interior;
+    return m->interior;
+}
+
+/* JoinedTransform(xforms).Inverse().Apply at every point (transform.go:
+ * 199-204, 213-219, with each op's Inverse and Apply): the vertex map that
+ * MarchingCubesConj (mc.go:66-71) applies through Mesh.Transform
+ * (mesh.go:613-642). */
+void or_inverse_apply(const int *kinds, const double *params, int nops, double *pts, long n) {
+    xf_op *inv = (xf_op *)malloc((nops > 0 ? nops : 1) * sizeof(xf_op));
+    for (int i = 0; i < nops; i++) {
+        xf_op t;
+        memset(&t, 0, sizeof t);
+        t.kind = kinds[nops - 1 - i];
+        memcpy(t.v, params + 9 * (nops - 1 - i), 9 * sizeof(double));
+        inv[i] = xf_inverse(t);
+    }
+    for (long k = 0; k < n; k++) {
+        c3 c = c3_of(pts + 3 * k);
+        for (int i = 0; i < nops; i++) c = xf_apply(&inv[i], c);
+        pts[3 * k] = c.x, pts[3 * k + 1] = c.y, pts[3 * k + 2] = c.z;
+    }
+    free(inv);
 }
 
 long or_mesh_ntris(const or_mesh *m) { return m->ntri; }
@@ -1212,6 +1270,7 @@ void or_mesh_info(const or_mesh *m, long *corners, long *vertices, long *search_
 }
 void or_mesh_free(or_mesh *m) {
     if (!m) return;
+    free(m->interior);
     free(m->tris);
     free(m);
 }

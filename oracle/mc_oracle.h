@@ -67,6 +67,13 @@ int or_rotations(int32_t *out);  /* up to 24 rows x 8 */
 long or_spacer(const or_node *s, double delta, int axis, double *out);
 
 int or_marching_cubes_search(const or_node *s, double delta, int iters, int threads, or_mesh **out);
+/* MarchingCubesInterior: the mesh plus, per unique vertex, 6 doubles
+ * {searched vertex, interior point} (or_mesh_interior). */
+int or_marching_cubes_interior(const or_node *s, double delta, int iters, int threads, or_mesh **out);
+const double *or_mesh_interior(const or_mesh *m, long *n);
+/* JoinedTransform(ops).Inverse().Apply at n points in place (3 doubles each);
+ * ops as or_transform takes them. */
+void or_inverse_apply(const int *kinds, const double *params, int nops, double *pts, long n);
 long or_mesh_ntris(const or_mesh *m);
 const double *or_mesh_tris(const or_mesh *m);
 void or_mesh_info(const or_mesh *m, long *corners, long *vertices, long *search_evals, double *ms_scan,

@@ -19,7 +19,9 @@ from oracle.binding import (
     OracleError,
     oracle_bounds,
     oracle_case_table,
+    oracle_conj,
     oracle_contains,
+    oracle_inverse_apply,
     oracle_rotations,
     oracle_search,
     oracle_spacer,
@@ -419,3 +421,82 @@ def test_csg64_oracle_smoke():
     assert info["corners"] > 0 and len(tris) > 1000
     directed_once, paired = edge_topology(tris)
     assert directed_once and paired
+
+
+# ---------------------------------------------------------------------------
+# MarchingCubesInterior (mc.go:55-60, 191-198, 252-255)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("iters", [0, 1, 5])
+def test_interior_points_on_sphere(iters):
+    """Per vertex: the interior point is inside the unit sphere (textbook
+    |p| <= 1, Go's Dist operation order), lies on the vertex's grid edge and
+    is the bracket end half a final bracket away; the searched vertices and
+    the triangles equal those of MarchingCubesSearch."""
+    delta = 0.1
+    s = m3.Sphere((0, 0, 0), 1.0)
+    tris, info = oracle_search(s, delta, iters, threads=8, interior=True)
+    plain, _ = oracle_search(s, delta, iters, threads=8)
+    assert np.array_equal(canonical(tris), canonical(plain))
+    pairs = info["interior"]
+    assert len(pairs) == info["vertices"] == len(np.unique(tris.reshape(-1, 3), axis=0))
+    assert {tuple(r) for r in pairs[:, :3].tolist()} == {tuple(r) for r in tris.reshape(-1, 3).tolist()}
+    ax = oracle_spacer(s, delta)
+    step = ax[0][1] - ax[0][0]
+    for v, ip in zip(pairs[:, :3], pairs[:, 3:]):
+        x, y, z = ip
+        assert math.sqrt(x * x + y * y + z * z) <= 1.0
+        diff = [i for i in range(3) if v[i] != ip[i]]
+        assert len(diff) == 1  # the same grid edge, moved along its axis only
+        a = diff[0]
+        assert 0 < abs(v[a] - ip[a]) <= step / 2 ** (iters + 1) * (1 + 1e-9)
+
+
+def test_interior_points_on_csg_are_inside():
+    s = solid_catalogue()["csg"]
+    _, info = oracle_search(s, 0.05, 4, threads=8, interior=True)
+    pts = info["interior"][:, 3:]
+    assert oracle_contains(s, pts).all()
+    # the searched vertex itself is not guaranteed inside; the interior point is
+    assert len(pts) > 100
+
+
+# ---------------------------------------------------------------------------
+# MarchingCubesConj (mc.go:66-71): the inverse map of the vertices
+# ---------------------------------------------------------------------------
+def test_inverse_apply_elementary_ops():
+    """Inverse().Apply per op (transform.go:52-54, 83-85, 114-116, 153-155):
+    translate subtracts, scale divides, a rotation's inverse is its
+    transpose, and the joined inverse runs the ops backwards."""
+    rng = np.random.default_rng(9)
+    p = rng.uniform(-2, 2, size=(50, 3))
+    t = np.array([0.25, -0.5, 0.125])
+    assert np.array_equal(oracle_inverse_apply(m3.Translate(t), p), p + (-t))
+    assert np.array_equal(oracle_inverse_apply(m3.Scale(4.0), p), p * 0.25)
+    assert np.array_equal(oracle_inverse_apply(m3.VecScale((2.0, 4.0, 0.5)), p), p * np.array([0.5, 0.25, 2.0]))
+    rot = m3.Rotation((0.0, 0.6, 0.8), 0.7)
+    R = np.array(rot.Matrix).reshape(3, 3)
+    assert np.allclose(oracle_inverse_apply(rot, p), p @ R, atol=1e-12)  # R^T p per row
+    # joined: Translate then Scale; the inverse scales back first, then subtracts
+    j = m3.JoinedTransform([m3.Translate(t), m3.Scale(2.0)])
+    assert np.array_equal(oracle_inverse_apply(j, p), p * 0.5 + (-t))
+
+
+def test_conj_power_of_two_scale_is_exact():
+    """Conj by Scale(2) meshes the doubled sphere and halves its vertices
+    exactly: x2 gives back the plain search of the doubled sphere, bit for
+    bit (a forward instead of inverse map would be 4x off)."""
+    s = m3.Sphere((0.1, -0.2, 0.05), 0.6)
+    got, _ = oracle_conj(s, 0.05, 3, m3.Scale(2.0), threads=8)
+    big, _ = oracle_search(m3.TransformSolid(m3.Scale(2.0), s), 0.05, 3, threads=8)
+    assert np.array_equal(canonical(got * 2.0), canonical(big))
+
+
+def test_conj_rotation_stays_on_surface():
+    """Conj by a rotation and translation: every vertex within one grid step
+    of the original unit sphere (closed form)."""
+    s = m3.Sphere((0, 0, 0), 1.0)
+    delta = 0.08
+    got, _ = oracle_conj(s, delta, 4, m3.Rotation((1.0, 0.0, 0.0), 0.4), m3.Translate((0.3, 0.0, -0.2)), threads=8)
+    r = np.linalg.norm(got.reshape(-1, 3), axis=1)
+    assert np.all(np.abs(r - 1.0) <= delta)
+    assert abs(volume(got) - 4 * math.pi / 3) < 0.05
