@@ -202,6 +202,22 @@ int mc3d_mesh_stats(const mc3d_mesh *m, mc3d_stats *out);
 int mc3d_mesh_copy_interior(mc3d_mesh *m, double *interior);
 void mc3d_mesh_free(mc3d_mesh *m);
 
+/* Binary STL, as EncodeSTL / WriteSTL write it (model3d/export.go:20-66,
+ * fileformats/stl.go:17-58): 80 zero bytes, the triangle count (uint32),
+ * then per triangle the float32 cast of Triangle.Normal() computed in
+ * float64 (primitives.go:27-33), the float32 casts of its three vertices and
+ * two zero bytes; little-endian.  Triangles in the given order (the
+ * reference writes Mesh.TriangleSlice's map order; SaveGroupedSTL's
+ * GroupTriangles reordering is not reproduced).  The encoding runs on the
+ * device.  out must hold mc3d_stl_bytes(T) bytes; more than 2^32 - 1
+ * triangles is MC3D_E_UNSUPPORTED ("too many triangles for STL format"). */
+size_t mc3d_stl_bytes(size_t n_triangles);
+/* a mesh of this context's device, from its device buffers */
+int mc3d_mesh_encode_stl(mc3d_ctx *ctx, const mc3d_mesh *m, uint8_t *out, size_t out_bytes);
+/* host arrays: verts 3 doubles per vertex, tris 3 vertex indices per triangle */
+int mc3d_encode_stl(mc3d_ctx *ctx, const double *verts, size_t nv, const uint32_t *tris, size_t nt, uint8_t *out,
+                    size_t out_bytes);
+
 /* Solid.Contains for n host points on the GPU (out[i] = 0/1). */
 int mc3d_contains(mc3d_ctx *ctx, const mc3d_solid *s, const double *pts, size_t n, int32_t precision,
                   uint8_t *out);

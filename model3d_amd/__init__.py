@@ -11,6 +11,7 @@ from ._lib import (  # noqa: F401
 )
 from .mc import (  # noqa: F401
     Contains,
+    EncodeSTL,
     MarchingCubes,
     MarchingCubesConj,
     MarchingCubesInterior,
@@ -19,6 +20,7 @@ from .mc import (  # noqa: F401
     Mesh,
     NewMeshRect,
     Panic,
+    WriteSTL,
     case_table,
     interior_map,
 )

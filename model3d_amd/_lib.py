@@ -142,6 +142,12 @@ SIGNATURES = {
     "mc3d_mesh_counts": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "mc3d_mesh_copy": (ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p]),
     "mc3d_mesh_copy_interior": (ctypes.c_int, [_P, ctypes.c_void_p]),
+    "mc3d_stl_bytes": (ctypes.c_size_t, [ctypes.c_size_t]),
+    "mc3d_mesh_encode_stl": (ctypes.c_int, [_P, _P, ctypes.c_void_p, ctypes.c_size_t]),
+    "mc3d_encode_stl": (
+        ctypes.c_int,
+        [_P, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t],
+    ),
     "mc3d_mesh_device_ptrs": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]),
     "mc3d_mesh_stats": (ctypes.c_int, [_P, ctypes.POINTER(mc3d_stats)]),
     "mc3d_mesh_free": (None, [_P]),

@@ -1107,4 +1107,35 @@ __global__ __launch_bounds__(256) void k_inverse_apply(const double *__restrict_
     }
 }
 
+// Binary STL body (export.go:37-66, fileformats/stl.go:41-58): per triangle
+// the float32 cast of Triangle.Normal() -- the cross product of (t1 - t0) and
+// (t2 - t0), scaled by 1 / Norm() in float64 (primitives.go:27-33,
+// coords.go:201-222, 379-381) -- then the float32 casts of the three
+// vertices and two zero bytes: 50 bytes, written as 25 little-endian
+// halfwords (a record starts at an even offset).
+__global__ __launch_bounds__(256) void k_encode_stl(const double *__restrict__ verts, const uint32_t *__restrict__ tris,
+                                                    uint64_t nt, uint16_t *__restrict__ body) {
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += (uint64_t)gridDim.x * blockDim.x) {
+        double p[3][3];
+        for (int k = 0; k < 3; k++) {
+            const uint64_t v = tris[3 * t + k];
+            p[k][0] = verts[3 * v], p[k][1] = verts[3 * v + 1], p[k][2] = verts[3 * v + 2];
+        }
+        const double ax = p[1][0] - p[0][0], ay = p[1][1] - p[0][1], az = p[1][2] - p[0][2];
+        const double bx = p[2][0] - p[0][0], by = p[2][1] - p[0][1], bz = p[2][2] - p[0][2];
+        const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+        const double s = 1 / sqrt(cx * cx + cy * cy + cz * cz);
+        float rec[12] = {(float)(cx * s), (float)(cy * s), (float)(cz * s)};
+        for (int k = 0; k < 3; k++)
+            for (int d = 0; d < 3; d++) rec[3 + 3 * k + d] = (float)p[k][d];
+        uint16_t *o = body + 25 * t;
+        for (int w = 0; w < 12; w++) {
+            const uint32_t u = __float_as_uint(rec[w]);
+            o[2 * w] = (uint16_t)(u & 0xffffu);
+            o[2 * w + 1] = (uint16_t)(u >> 16);
+        }
+        o[24] = 0;
+    }
+}
+
 }  // namespace mc3d

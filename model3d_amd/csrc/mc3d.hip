@@ -177,6 +177,7 @@ struct mc3d_ctx {
     DevBuf blob, table, bits, counts, offs, tiles, actw, actb, binfo, bcode, vkeys, vseg, pts, out8;
     DevBuf hvals, hpts, hstate, hprobe;  // host-evaluated atoms: answers, points, bisection state
     HostBuf hstage;                      // pinned points / answers of one callback batch
+    DevBuf stl, stl_in;                  // STL encoding: body bytes; uploaded host triangles
     std::shared_ptr<Pool> pool = std::make_shared<Pool>();
     uint64_t v_hint = 1u << 18, t_hint = 1u << 19;  // output capacities to start from
     hipEvent_t ev[10] = {};
@@ -1182,6 +1183,75 @@ int mc3d_mesh_copy(mc3d_mesh *m, double *verts, uint32_t *tris) {
     if (tris && m->nt)
         HIP_TRY(hipMemcpy(tris, m->buf->tris->as<uint32_t>() + 3 * m->toff, m->nt * 12, hipMemcpyDeviceToHost));
     return MC3D_OK;
+}
+
+size_t mc3d_stl_bytes(size_t n_triangles) { return 84 + 50 * n_triangles; }
+
+namespace {
+// The binary STL of nt device triangles into out (host): the body by
+// k_encode_stl into device scratch, one copy through pinned memory, the
+// 84-byte header written here.
+int encode_stl_device(mc3d_ctx *ctx, const double *verts, const uint32_t *tris, uint64_t nt, uint8_t *out,
+                      size_t out_bytes) {
+    if (nt >= (1ull << 32)) {
+        set_error("too many triangles for STL format");  // export.go:38-40
+        return MC3D_E_UNSUPPORTED;
+    }
+    if (out_bytes < mc3d_stl_bytes(nt)) {
+        set_error("STL buffer too small: need " + std::to_string(mc3d_stl_bytes(nt)) + " bytes");
+        return MC3D_E_INVALID_ARG;
+    }
+    memset(out, 0, 80);
+    const uint32_t n32 = (uint32_t)nt;
+    memcpy(out + 80, &n32, 4);  // little-endian host
+    if (!nt) return MC3D_OK;
+    const size_t body = 50 * nt;
+    int rc;
+    if ((rc = ctx->stl.ensure(body)) || (rc = ctx->outstage.ensure(body))) return rc;
+    k_encode_stl<<<grid_for(nt, 256, 1u << 16), 256, 0, ctx->stream>>>(verts, tris, nt, ctx->stl.as<uint16_t>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(ctx->outstage.p, ctx->stl.p, body, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    parallel_copy(out + 84, ctx->outstage.p, body);
+    return MC3D_OK;
+}
+}  // namespace
+
+int mc3d_mesh_encode_stl(mc3d_ctx *ctx, const mc3d_mesh *m, uint8_t *out, size_t out_bytes) {
+    if (!ctx || !m || !out) {
+        set_error("mc3d_mesh_encode_stl: null argument");
+        return MC3D_E_INVALID_ARG;
+    }
+    if (m->device != ctx->device || !m->buf || !m->buf->verts || !m->buf->tris) {
+        set_error("mc3d_mesh_encode_stl: the mesh is not on this context's device");
+        return MC3D_E_INVALID_ARG;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    // triangle indices count from the mesh's first vertex
+    return encode_stl_device(ctx, m->buf->verts->as<double>() + 3 * m->voff,
+                             m->buf->tris->as<uint32_t>() + 3 * m->toff, m->nt, out, out_bytes);
+}
+
+int mc3d_encode_stl(mc3d_ctx *ctx, const double *verts, size_t nv, const uint32_t *tris, size_t nt, uint8_t *out,
+                    size_t out_bytes) {
+    if (!ctx || !out || (nv && !verts) || (nt && !tris)) {
+        set_error("mc3d_encode_stl: null argument");
+        return MC3D_E_INVALID_ARG;
+    }
+    for (size_t i = 0; i < 3 * nt; i++)
+        if (tris[i] >= nv) {
+            set_error("mc3d_encode_stl: triangle " + std::to_string(i / 3) + " refers to vertex " +
+                      std::to_string(tris[i]) + " of " + std::to_string(nv));
+            return MC3D_E_INVALID_ARG;
+        }
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = ctx->stl_in.ensure(nv * 24 + nt * 12 + 16))) return rc;
+    double *dv = ctx->stl_in.as<double>();
+    uint32_t *dt = (uint32_t *)(ctx->stl_in.as<char>() + align_up(nv * 24, 16));
+    if (nv) HIP_TRY(hipMemcpyAsync(dv, verts, nv * 24, hipMemcpyHostToDevice, ctx->stream));
+    if (nt) HIP_TRY(hipMemcpyAsync(dt, tris, nt * 12, hipMemcpyHostToDevice, ctx->stream));
+    return encode_stl_device(ctx, dv, dt, nt, out, out_bytes);
 }
 
 int mc3d_mesh_copy_interior(mc3d_mesh *m, double *interior) {

@@ -34,6 +34,11 @@ class Mesh:
         """(T, 3, 3) float64: triangle i = vertices[faces[i]]."""
         return self.vertices[self.faces.astype(np.int64)]
 
+    def EncodeSTL(self, device: int = 0) -> bytes:
+        """Mesh.EncodeSTL (mesh.go:652-655): binary STL of the triangles in
+        this mesh's order, encoded on the GPU (see EncodeSTL)."""
+        return EncodeSTL(self, device=device)
+
     @classmethod
     def from_triangles(cls, tris) -> "Mesh":
         """Indexed mesh of a triangle list (vertices shared by exact bits)."""
@@ -208,6 +213,37 @@ def MarchingCubesConj(
             L.mc3d_mesh_free(out)
         b.release(h)
         b.close()
+
+
+def EncodeSTL(mesh_or_triangles, device: int = 0) -> bytes:
+    """model3d.EncodeSTL (export.go:20-26) on the GPU: 80 zero bytes, the
+    triangle count, then per triangle the float32 normal (Triangle.Normal in
+    float64), the float32 vertices and two zero bytes.  Accepts a Mesh or a
+    (T, 3, 3) / (T, 9) triangle array."""
+    if isinstance(mesh_or_triangles, Mesh):
+        v = np.ascontiguousarray(mesh_or_triangles.vertices, dtype=np.float64)
+        f = np.ascontiguousarray(mesh_or_triangles.faces, dtype=np.uint32)
+    else:
+        t = np.asarray(mesh_or_triangles, dtype=np.float64).reshape(-1, 9)
+        v = np.ascontiguousarray(t.reshape(-1, 3))
+        f = np.arange(len(v), dtype=np.uint32).reshape(-1, 3)
+    L = _lib.lib()
+    ctx = _lib.context(device)
+    n = L.mc3d_stl_bytes(len(f))
+    out = np.empty(n, dtype=np.uint8)
+    _raise_for(L.mc3d_encode_stl(ctx, v.ctypes.data, len(v), f.ctypes.data, len(f), out.ctypes.data, n))
+    return out.tobytes()
+
+
+def WriteSTL(w, mesh_or_triangles, device: int = 0) -> None:
+    """model3d.WriteSTL (export.go:28-35): the binary STL to a path or a
+    writable binary file object."""
+    data = EncodeSTL(mesh_or_triangles, device=device)
+    if hasattr(w, "write"):
+        w.write(data)
+    else:
+        with open(w, "wb") as fh:
+            fh.write(data)
 
 
 def _raise_for(rc: int):

@@ -53,6 +53,7 @@ _SIG = {
     ),
     "or_mesh_interior": (_dp, [_P, ctypes.POINTER(ctypes.c_long)]),
     "or_inverse_apply": (None, [ctypes.POINTER(ctypes.c_int), _dp, ctypes.c_int, _dp, ctypes.c_long]),
+    "or_encode_stl": (None, [_dp, ctypes.c_long, ctypes.c_void_p]),
     "or_mesh_ntris": (ctypes.c_long, [_P]),
     "or_mesh_tris": (_dp, [_P]),
     "or_mesh_info": (
@@ -252,6 +253,14 @@ def oracle_conj(solid, delta, iters, *xforms, threads=None):
     joined = JoinedTransform(xforms)
     tris, info = oracle_search(TransformSolid(joined, solid), delta, iters, threads)
     return oracle_inverse_apply(joined, tris).reshape(-1, 9), info
+
+
+def oracle_encode_stl(tris) -> bytes:
+    """EncodeSTL (export.go:20-66) of a triangle list (T, 9) on the CPU."""
+    t = np.ascontiguousarray(np.asarray(tris, dtype=np.float64).reshape(-1, 9))
+    out = np.zeros(84 + 50 * len(t), dtype=np.uint8)
+    lib().or_encode_stl(t.ctypes.data_as(_dp), len(t), out.ctypes.data)
+    return out.tobytes()
 
 
 def oracle_case_table() -> np.ndarray:

@@ -1258,6 +1258,37 @@ void or_inverse_apply(const int *kinds, const double *params, int nops, double *
     free(inv);
 }
 
+/* EncodeSTL (export.go:20-66) with STLWriter (fileformats/stl.go:17-58):
+ * 80 zero bytes, uint32 count, then per triangle float32(Normal()) with
+ * Normal = Cross(t1 - t0, t2 - t0).Normalize() in float64
+ * (primitives.go:27-33, coords.go:201-222, 267-269, 379-381), the float32
+ * vertices and two zero bytes, little-endian.  tris: 9 doubles each. */
+static void put_f32(uint8_t *o, float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    o[0] = (uint8_t)u, o[1] = (uint8_t)(u >> 8), o[2] = (uint8_t)(u >> 16), o[3] = (uint8_t)(u >> 24);
+}
+
+void or_encode_stl(const double *tris, long nt, uint8_t *out) {
+    memset(out, 0, 80);
+    const uint32_t n = (uint32_t)nt;
+    out[80] = (uint8_t)n, out[81] = (uint8_t)(n >> 8), out[82] = (uint8_t)(n >> 16), out[83] = (uint8_t)(n >> 24);
+    for (long t = 0; t < nt; t++) {
+        const double *p = tris + 9 * t;
+        c3 a = v3(p[3] + -p[0], p[4] + -p[1], p[5] + -p[2]); /* Sub = Add(Scale(-1)) */
+        c3 b = v3(p[6] + -p[0], p[7] + -p[1], p[8] + -p[2]);
+        c3 c = v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+        double norm = sqrt(c.x * c.x + c.y * c.y + c.z * c.z);
+        double s = 1 / norm;
+        uint8_t *o = out + 84 + 50 * t;
+        put_f32(o, (float)(c.x * s));
+        put_f32(o + 4, (float)(c.y * s));
+        put_f32(o + 8, (float)(c.z * s));
+        for (int k = 0; k < 9; k++) put_f32(o + 12 + 4 * k, (float)p[k]);
+        o[48] = o[49] = 0;
+    }
+}
+
 long or_mesh_ntris(const or_mesh *m) { return m->ntri; }
 const double *or_mesh_tris(const or_mesh *m) { return m->tris; }
 void or_mesh_info(const or_mesh *m, long *corners, long *vertices, long *search_evals, double *ms_scan,

@@ -74,6 +74,8 @@ const double *or_mesh_interior(const or_mesh *m, long *n);
 /* JoinedTransform(ops).Inverse().Apply at n points in place (3 doubles each);
  * ops as or_transform takes them. */
 void or_inverse_apply(const int *kinds, const double *params, int nops, double *pts, long n);
+/* EncodeSTL of nt triangles (9 doubles each) into out (84 + 50 nt bytes). */
+void or_encode_stl(const double *tris, long nt, uint8_t *out);
 long or_mesh_ntris(const or_mesh *m);
 const double *or_mesh_tris(const or_mesh *m);
 void or_mesh_info(const or_mesh *m, long *corners, long *vertices, long *search_evals, double *ms_scan,

@@ -1,6 +1,7 @@
 """GPU parity of the entry points beside MarchingCubesSearch that reuse its
-kernels: MarchingCubesInterior (mc.go:55-60) and MarchingCubesConj
-(mc.go:66-71), through the C-ABI, bit-exact against the oracle."""
+kernels -- MarchingCubesInterior (mc.go:55-60), MarchingCubesConj
+(mc.go:66-71) -- and of the binary STL encoder (export.go:20-66), through
+the C-ABI, bit-exact against the oracle."""
 import ctypes
 
 import numpy as np
@@ -8,7 +9,7 @@ import pytest
 
 import model3d_amd as m3
 from model3d_amd import _lib
-from oracle.binding import oracle_conj, oracle_contains, oracle_search
+from oracle.binding import oracle_conj, oracle_contains, oracle_encode_stl, oracle_search
 from tests.helpers import canonical, same_triangles, solid_catalogue
 
 pytestmark = pytest.mark.gpu
@@ -108,3 +109,66 @@ def test_conj_without_culling_and_rejects_bad_kind():
 
     with pytest.raises(_lib.MC3DError):
         m3.MarchingCubesConj(s, 0.05, 3, Bad((0, 0, 0)))
+
+
+def test_stl_random_triangles_match_oracle():
+    rng = np.random.default_rng(11)
+    tris = rng.normal(size=(5000, 9)) * 3
+    tris[17] = np.tile(tris[17][:3], 3)  # degenerate: NaN normal, same bits as the oracle's
+    got = m3.EncodeSTL(tris)
+    assert got == oracle_encode_stl(tris)
+    assert m3.EncodeSTL(np.zeros((0, 9))) == oracle_encode_stl(np.zeros((0, 9)))
+
+
+@pytest.mark.parametrize("name", ["sphere", "csg", "torus"])
+def test_stl_of_searched_mesh(name, tmp_path):
+    s = solid_catalogue()[name]
+    mesh = m3.MarchingCubesSearch(s, 0.04, 6)
+    data = mesh.EncodeSTL()
+    assert data == oracle_encode_stl(mesh.triangles())
+    p = tmp_path / "m.stl"
+    m3.WriteSTL(str(p), mesh)
+    assert p.read_bytes() == data
+
+
+def test_stl_from_device_mesh():
+    """mc3d_mesh_encode_stl (from the mesh's device buffers) equals the
+    encoding of the same mesh copied to the host, for one solid of a batch."""
+    from model3d_amd.solids import LibBackend
+
+    L = _lib.lib()
+    ctx = _lib.context(0)
+    b = LibBackend()
+    solids = [solid_catalogue()["csg"], m3.Sphere((0, 0, 0), 0.8)]
+    hs = [x._build(b) for x in solids]
+    outs = (ctypes.c_void_p * 2)()
+    opts = _lib.mc3d_options()
+    opts.precision, opts.cull, opts.device_only = m3.MC3D_PREC_FP64, 1, 1
+    try:
+        _lib.check(L.mc3d_marching_cubes_search_batch(ctx, (ctypes.c_void_p * 2)(*hs), (ctypes.c_double * 2)(0.05, 0.03),
+                                                      (ctypes.c_int32 * 2)(4, 5), 2, ctypes.byref(opts), outs), "batch")
+        for i in range(2):
+            nv, nt = ctypes.c_size_t(), ctypes.c_size_t()
+            L.mc3d_mesh_counts(outs[i], ctypes.byref(nv), ctypes.byref(nt))
+            v = np.empty((nv.value, 3))
+            f = np.empty((nt.value, 3), dtype=np.uint32)
+            _lib.check(L.mc3d_mesh_copy(outs[i], v.ctypes.data, f.ctypes.data), "copy")
+            n = L.mc3d_stl_bytes(nt.value)
+            buf = np.empty(n, dtype=np.uint8)
+            _lib.check(L.mc3d_mesh_encode_stl(ctx, outs[i], buf.ctypes.data, n), "mesh stl")
+            assert buf.tobytes() == oracle_encode_stl(v[f.astype(np.int64)])
+            small = np.empty(n - 1, dtype=np.uint8)
+            assert L.mc3d_mesh_encode_stl(ctx, outs[i], small.ctypes.data, n - 1) == _lib.MC3D_E_INVALID_ARG
+    finally:
+        for i in range(2):
+            if outs[i]:
+                L.mc3d_mesh_free(outs[i])
+        for h in hs:
+            b.release(h)
+        b.close()
+
+
+def test_stl_rejects_bad_index():
+    v = np.zeros((3, 3))
+    with pytest.raises(_lib.MC3DError):
+        m3.EncodeSTL(m3.Mesh(v, np.array([[0, 1, 3]], dtype=np.uint32)))

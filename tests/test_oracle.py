@@ -21,6 +21,7 @@ from oracle.binding import (
     oracle_case_table,
     oracle_conj,
     oracle_contains,
+    oracle_encode_stl,
     oracle_inverse_apply,
     oracle_rotations,
     oracle_search,
@@ -500,3 +501,46 @@ def test_conj_rotation_stays_on_surface():
     r = np.linalg.norm(got.reshape(-1, 3), axis=1)
     assert np.all(np.abs(r - 1.0) <= delta)
     assert abs(volume(got) - 4 * math.pi / 3) < 0.05
+
+
+# ---------------------------------------------------------------------------
+# EncodeSTL (export.go:20-66, fileformats/stl.go:17-58)
+# ---------------------------------------------------------------------------
+def _parse_stl(data):
+    """Binary STL reader (stdlib struct): header, count, (normals, vertices, trailers)."""
+    n = struct.unpack_from("<I", data, 80)[0]
+    assert len(data) == 84 + 50 * n
+    recs = [struct.unpack_from("<12fH", data, 84 + 50 * i) for i in range(n)]
+    arr = np.array([r[:12] for r in recs], dtype=np.float32).reshape(n, 4, 3)
+    return data[:80], n, arr[:, 0], arr[:, 1:], [r[12] for r in recs]
+
+
+def test_stl_known_answer():
+    tri = np.array([[0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0]])
+    want = bytes(80) + struct.pack("<I", 1) + struct.pack("<12f", 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1, 0) + bytes(2)
+    assert oracle_encode_stl(tri) == want
+    assert oracle_encode_stl(np.zeros((0, 9))) == bytes(80) + struct.pack("<I", 0)
+
+
+def test_stl_round_trip_and_normals():
+    """The reference's TestImportSTL / TestSTLBinary: the vertices read back
+    are the float32 casts; normals are unit, orthogonal to the edges and
+    right-handed (t1 - t0) x (t2 - t0); a degenerate triangle's normal is NaN
+    (1 / 0 scaled by 0)."""
+    rng = np.random.default_rng(3)
+    tris = rng.normal(size=(200, 9))
+    tris[7] = np.tile(tris[7][:3], 3)  # degenerate
+    header, n, normals, verts, trailers = _parse_stl(oracle_encode_stl(tris))
+    assert header == bytes(80) and n == 200 and all(t == 0 for t in trailers)
+    assert np.array_equal(verts.reshape(-1, 9), tris.astype(np.float32))
+    assert np.isnan(normals[7]).all()
+    ok = np.arange(200) != 7
+    t = tris[ok].reshape(-1, 3, 3)
+    e1, e2 = t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]
+    cr = np.cross(e1, e2)
+    nn = normals[ok].astype(np.float64)
+    assert np.allclose(np.linalg.norm(nn, axis=1), 1, atol=1e-6)
+    scale = np.linalg.norm(e1, axis=1) + np.linalg.norm(e2, axis=1)
+    assert (np.abs(np.sum(nn * e1, axis=1)) <= 1e-5 * scale).all()
+    assert (np.abs(np.sum(nn * e2, axis=1)) <= 1e-5 * scale).all()
+    assert (np.sum(nn * cr, axis=1) > 0).all()
