@@ -182,6 +182,35 @@ int mc3d_marching_cubes_interior(mc3d_ctx *ctx, const mc3d_solid *s, double delt
  * on the device. */
 int mc3d_marching_cubes_conj(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
                              const mc3d_xform *xforms, size_t n, const mc3d_options *opts, mc3d_mesh **out);
+/* A block filter for MarchingCubesFilter: nonzero when the rect [min, max]
+ * may touch the surface (it may be conservative, never wrong the other way). */
+typedef int (*mc3d_rect_filter_fn)(void *user, const double min[3], const double max[3]);
+/* MeshToCollider(mesh).RectCollision (collisions.go:138-179, 344-356;
+ * Triangle.RectCollision, primitives.go:448-495) over n triangles (9 doubles
+ * each): true when a triangle whose bounding box overlaps the rect collides
+ * with it.  mc3d_rect_collider_collides has the mc3d_rect_filter_fn shape
+ * (user = the collider) and may be called from several threads. */
+typedef struct mc3d_rect_collider mc3d_rect_collider;
+mc3d_rect_collider *mc3d_rect_collider_create(const double *triangles, size_t n);
+int mc3d_rect_collider_collides(void *collider, const double min[3], const double max[3]);
+void mc3d_rect_collider_free(mc3d_rect_collider *c);
+/* MarchingCubesFilter(s, f, delta), model3d/mc.go:73-135: marching cubes
+ * only in the blocks f accepts -- the reference's block recursion (mcBlock
+ * Pieces/Split, mc.go:724-797, asked on the calling thread), its triangles
+ * those of the cubes of accepted blocks.  Every corner is still evaluated, so
+ * "solid is true outside of bounds" covers the whole grid. */
+int mc3d_marching_cubes_filter(mc3d_ctx *ctx, const mc3d_solid *s, mc3d_rect_filter_fn f, void *user, double delta,
+                               const mc3d_options *opts, mc3d_mesh **out);
+/* MarchingCubesSearchFilter(s, f, delta, iters), mc.go:137-143 */
+int mc3d_marching_cubes_search_filter(mc3d_ctx *ctx, const mc3d_solid *s, mc3d_rect_filter_fn f, void *user,
+                                      double delta, int iters, const mc3d_options *opts, mc3d_mesh **out);
+/* MarchingCubesC2F(s, bigDelta, smallDelta, extraSpace, iters), mc.go:145-180:
+ * the coarse search on the GPU, its triangles as a rect collider on the host
+ * (MeshToCollider(...).RectCollision, 16 threads), then the filtered fine
+ * search.  bigDelta < smallDelta is MC3D_E_INVALID_ARG with the reference's
+ * panic text. */
+int mc3d_marching_cubes_c2f(mc3d_ctx *ctx, const mc3d_solid *s, double big_delta, double small_delta,
+                            double extra_space, int iters, const mc3d_options *opts, mc3d_mesh **out);
 /* MarchingCubesSearch(solids[i], deltas[i], iters[i]) for i < n in one pass
  * of the pipeline (one launch per stage for all solids; solids with
  * host-evaluated atoms run alone).  out[i] receives solid i's mesh, an

@@ -99,6 +99,7 @@ class mc3d_stats(ctypes.Structure):
 CONTAINS_FN = ctypes.CFUNCTYPE(
     ctypes.c_int, ctypes.c_void_p, _dp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint8)
 )
+RECT_FILTER_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, _dp, _dp)
 
 # Every symbol include/mc3d.h declares, with (restype, argtypes).
 SIGNATURES = {
@@ -143,6 +144,23 @@ SIGNATURES = {
     "mc3d_mesh_copy": (ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p]),
     "mc3d_mesh_copy_interior": (ctypes.c_int, [_P, ctypes.c_void_p]),
     "mc3d_stl_bytes": (ctypes.c_size_t, [ctypes.c_size_t]),
+    "mc3d_marching_cubes_filter": (
+        ctypes.c_int,
+        [_P, _P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(mc3d_options), ctypes.POINTER(_P)],
+    ),
+    "mc3d_marching_cubes_search_filter": (
+        ctypes.c_int,
+        [_P, _P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.POINTER(mc3d_options),
+         ctypes.POINTER(_P)],
+    ),
+    "mc3d_marching_cubes_c2f": (
+        ctypes.c_int,
+        [_P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(mc3d_options),
+         ctypes.POINTER(_P)],
+    ),
+    "mc3d_rect_collider_create": (_P, [_dp, ctypes.c_size_t]),
+    "mc3d_rect_collider_collides": (ctypes.c_int, [_P, _D3, _D3]),
+    "mc3d_rect_collider_free": (None, [_P]),
     "mc3d_mesh_encode_stl": (ctypes.c_int, [_P, _P, ctypes.c_void_p, ctypes.c_size_t]),
     "mc3d_encode_stl": (
         ctypes.c_int,

@@ -1,0 +1,301 @@
+// MarchingCubesFilter's block recursion and MarchingCubesC2F's coarse-mesh
+// filter, on the host.  The result of the recursion is a bit per cube (the
+// cubes of accepted blocks); the device pipeline then emits triangles only
+// for those cubes and vertices only for the edges they use.
+//
+//   MarchingCubesFilter / SearchFilter / C2F   model3d/mc.go:83-180
+//   mcBlock (Bounds, Volume, Pieces, Split)    model3d/mc.go:724-797
+//   MeshToCollider(...).RectCollision          model3d/collisions.go:138-142, 169-179, 344-356
+//   Triangle.RectCollision / SegmentCollision  model3d/primitives.go:181-249, 440-495
+//   Segment.RectCollision, NewSegment          model3d/primitives.go:547-554, 632-638
+//   rayCollisionWithBounds                     model3d/bvh.go:322-351
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <thread>
+
+#include "mc3d_internal.h"
+
+namespace mc3d {
+namespace {
+
+struct V {
+    double x, y, z;
+};
+V mk(double x, double y, double z) { return V{x, y, z}; }
+V sub(V a, V b) { return mk(a.x + -b.x, a.y + -b.y, a.z + -b.z); }  // Sub = Add(Scale(-1)), coords.go:267-269
+double dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+V cross(V a, V b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+double norm(V a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+V scale(V a, double s) { return mk(a.x * s, a.y * s, a.z * s); }
+V normalize(V a) { return scale(a, 1 / norm(a)); }
+double comp(V a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+// math.Min / math.Max with Go's NaN and signed-zero rules
+double go_min(double a, double b) {
+    if (isinf(a) && a < 0) return a;
+    if (isinf(b) && b < 0) return b;
+    if (isnan(a) || isnan(b)) return NAN;
+    if (a == 0 && a == b) return signbit(a) ? a : b;
+    return a < b ? a : b;
+}
+double go_max(double a, double b) {
+    if (isinf(a) && a > 0) return a;
+    if (isinf(b) && b > 0) return b;
+    if (isnan(a) || isnan(b)) return NAN;
+    if (a == 0 && a == b) return signbit(a) ? b : a;
+    return a > b ? a : b;
+}
+
+struct Seg {
+    V a, b;
+};
+// NewSegment, primitives.go:547-554
+Seg new_segment(V p1, V p2) {
+    if (p1.x < p2.x || (p1.x == p2.x && p1.y < p2.y) || (p1.x == p2.x && p1.y == p2.y && p1.z < p2.z)) return {p1, p2};
+    return {p2, p1};
+}
+
+// rayCollisionWithBounds, bvh.go:322-351
+void ray_bounds(V origin, V dir, const double *mn, const double *mx, double &min_frac, double &max_frac) {
+    min_frac = -INFINITY, max_frac = INFINITY;
+    for (int axis = 0; axis < 3; axis++) {
+        const double o = comp(origin, axis), rate = comp(dir, axis);
+        if (rate == 0) {
+            if (o < mn[axis] || o > mx[axis]) {
+                min_frac = 0, max_frac = -1;
+                return;
+            }
+            continue;
+        }
+        double t1 = (mn[axis] - o) / rate, t2 = (mx[axis] - o) / rate;
+        if (t1 > t2) std::swap(t1, t2);
+        if (t2 < 0) {
+            min_frac = 0, max_frac = -1;
+            return;
+        }
+        if (t1 > min_frac) min_frac = t1;
+        if (t2 < max_frac) max_frac = t2;
+    }
+}
+
+// Segment.RectCollision, primitives.go:632-638
+bool segment_rect(const Seg &s, const double *mn, const double *mx) {
+    double lo, hi;
+    ray_bounds(s.a, sub(s.b, s.a), mn, mx, lo, hi);
+    return hi >= lo && hi >= 0 && lo <= 1;
+}
+
+// Triangle.rayCollision's scale (primitives.go:207-249) for FirstRayCollision
+// (181-188): false when there is no collision at scale >= 0.
+bool tri_ray(const V *t, V origin, V dir, double &scale_out) {
+    const V n = normalize(cross(sub(t[1], t[0]), sub(t[2], t[0])));  // Triangle.Normal
+    const double d = dot(n, normalize(dir));
+    if (d < 1e-8 && d > -1e-8) return false;
+    const V v1 = sub(t[1], t[0]), v2 = sub(t[2], t[0]);
+    const V cross1 = cross(dir, v2);
+    const double det = dot(cross1, v1);
+    if (det == 0) return false;
+    const double inv = 1 / det;
+    const V o = sub(origin, t[0]);
+    const double bary1 = inv * dot(o, cross1);
+    if (bary1 < 0 || bary1 > 1) return false;
+    const V cross2 = cross(o, v1);
+    const double bary2 = inv * dot(dir, cross2);
+    if (bary2 < 0 || bary1 + bary2 > 1) return false;
+    scale_out = inv * dot(v2, cross2);
+    return scale_out >= 0;
+}
+
+// Triangle.SegmentCollision, primitives.go:440-445
+bool tri_segment(const V *t, const Seg &s) {
+    double sc;
+    return tri_ray(t, s.a, sub(s.b, s.a), sc) && sc <= 1;
+}
+
+// Triangle.RectCollision, primitives.go:448-495
+bool tri_rect(const V *t, const double *mn, const double *mx) {
+    for (int i = 0; i < 3; i++)
+        if (segment_rect(new_segment(t[i], t[(i + 1) % 3]), mn, mx)) return true;
+    for (double x : {mn[0], mx[0]}) {
+        const Seg segs[4] = {
+            new_segment(mk(x, mn[1], mn[2]), mk(x, mn[1], mx[2])),
+            new_segment(mk(x, mn[1], mn[2]), mk(x, mx[1], mn[2])),
+            new_segment(mk(x, mx[1], mn[2]), mk(x, mx[1], mx[2])),
+            new_segment(mk(x, mn[1], mx[2]), mk(x, mx[1], mx[2])),
+        };
+        for (const Seg &s : segs)
+            if (tri_segment(t, s)) return true;
+    }
+    const Seg segs[4] = {
+        new_segment(mk(mn[0], mn[1], mn[2]), mk(mx[0], mn[1], mn[2])),
+        new_segment(mk(mn[0], mx[1], mn[2]), mk(mx[0], mx[1], mn[2])),
+        new_segment(mk(mn[0], mn[1], mx[2]), mk(mx[0], mn[1], mx[2])),
+        new_segment(mk(mn[0], mx[1], mx[2]), mk(mx[0], mx[1], mx[2])),
+    };
+    for (const Seg &s : segs)
+        if (tri_segment(t, s)) return true;
+    return false;
+}
+
+// The bounds test of a joined collider, collisions.go:344-350: the rect and
+// the box overlap unless max(rmin, bmin) > min(rmax, bmax) on some axis.
+bool box_overlap(const double *rmn, const double *rmx, const double *bmn, const double *bmx) {
+    for (int a = 0; a < 3; a++) {
+        const double lo = go_max(rmn[a], bmn[a]), hi = go_min(rmx[a], bmx[a]);
+        if (go_min(lo, hi) != lo) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+// A bounding-volume tree over the coarse mesh's triangles: a rect collides
+// with the mesh if it collides with a triangle whose box it overlaps.
+void RectMesh::build(const double *tris9, size_t n) {
+    tris.assign(tris9, tris9 + 9 * n);
+    order.resize(n);
+    for (size_t i = 0; i < n; i++) order[i] = (uint32_t)i;
+    nodes.clear();
+    if (n) split(0, (uint32_t)n);
+}
+
+uint32_t RectMesh::split(uint32_t lo, uint32_t hi) {
+    Node nd;
+    for (int a = 0; a < 3; a++) nd.mn[a] = INFINITY, nd.mx[a] = -INFINITY;
+    for (uint32_t i = lo; i < hi; i++) {
+        const double *t = &tris[9 * (size_t)order[i]];
+        for (int k = 0; k < 3; k++)
+            for (int a = 0; a < 3; a++) nd.mn[a] = go_min(nd.mn[a], t[3 * k + a]), nd.mx[a] = go_max(nd.mx[a], t[3 * k + a]);
+    }
+    nd.lo = lo, nd.hi = hi;
+    const uint32_t id = (uint32_t)nodes.size();
+    nodes.push_back(nd);
+    if (hi - lo > 4) {
+        int axis = 0;
+        for (int a = 1; a < 3; a++)
+            if (nd.mx[a] - nd.mn[a] > nd.mx[axis] - nd.mn[axis]) axis = a;
+        const uint32_t mid = (lo + hi) / 2;
+        std::nth_element(order.begin() + lo, order.begin() + mid, order.begin() + hi, [&](uint32_t p, uint32_t q) {
+            const double *a = &tris[9 * (size_t)p], *b = &tris[9 * (size_t)q];
+            return a[axis] + a[3 + axis] + a[6 + axis] < b[axis] + b[3 + axis] + b[6 + axis];
+        });
+        const uint32_t l = split(lo, mid);
+        const uint32_t r = split(mid, hi);
+        nodes[id].left = l, nodes[id].right = r;
+    }
+    return id;
+}
+
+bool RectMesh::collides(const double *mn, const double *mx) const {
+    if (nodes.empty()) return false;
+    uint32_t stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const Node &nd = nodes[stack[--sp]];
+        if (!box_overlap(mn, mx, nd.mn, nd.mx)) continue;
+        if (nd.left == NONE) {
+            for (uint32_t i = nd.lo; i < nd.hi; i++) {
+                const double *p = &tris[9 * (size_t)order[i]];
+                const V t[3] = {mk(p[0], p[1], p[2]), mk(p[3], p[4], p[5]), mk(p[6], p[7], p[8])};
+                double tmn[3], tmx[3];  // Triangle.Min/Max, primitives.go:35-43
+                for (int a = 0; a < 3; a++)
+                    tmn[a] = go_min(go_min(p[a], p[3 + a]), p[6 + a]), tmx[a] = go_max(go_max(p[a], p[3 + a]), p[6 + a]);
+                if (box_overlap(mn, mx, tmn, tmx) && tri_rect(t, mn, mx)) return true;
+            }
+        } else {
+            stack[sp++] = nd.left;
+            stack[sp++] = nd.right;
+        }
+    }
+    return false;
+}
+
+namespace {
+
+struct Block {
+    int mn[3], mx[3];  // corner indices; cubes [mn, mx)
+    long volume() const { return (long)(mx[0] - mn[0]) * (mx[1] - mn[1]) * (mx[2] - mn[2]); }
+    // mcBlock.Split, mc.go:777-797
+    void split(Block &b1, Block &b2) const {
+        const int lx = mx[0] - mn[0], ly = mx[1] - mn[1], lz = mx[2] - mn[2];
+        int axis = 0;
+        if (ly >= lx && ly >= lz) axis = 1;
+        else if (lz >= lx && lz >= ly) axis = 2;
+        b1 = *this, b2 = *this;
+        b1.mx[axis] = (mx[axis] + mn[axis]) / 2;
+        b2.mn[axis] = b1.mx[axis];
+    }
+};
+
+struct Recursion {
+    const std::vector<double> *ax;
+    double eps;
+    const std::function<bool(const double *, const double *)> *f;
+    // mcBlock.Bounds(epsilon), mc.go:736-740
+    bool accept(const Block &b) const {
+        const double mn[3] = {ax[0][b.mn[0]] + -eps, ax[1][b.mn[1]] + -eps, ax[2][b.mn[2]] + -eps};
+        const double mx[3] = {ax[0][b.mx[0]] + eps, ax[1][b.mx[1]] + eps, ax[2][b.mx[2]] + eps};
+        return (*f)(mn, mx);
+    }
+    // mcBlock.Pieces, mc.go:760-775
+    template <class Leaf>
+    void pieces(const Block &b, long min_volume, Leaf &&leaf) const {
+        if (!accept(b)) return;
+        if (b.volume() / 2 < min_volume) {
+            leaf(b);
+        } else {
+            Block b1, b2;
+            b.split(b1, b2);
+            pieces(b1, min_volume, leaf);
+            pieces(b2, min_volume, leaf);
+        }
+    }
+};
+
+}  // namespace
+
+int filter_keep_mask(const std::vector<double> ax[3], double delta,
+                     const std::function<bool(const double *, const double *)> &f, int threads,
+                     std::vector<uint64_t> &keep) {
+    const int nx = (int)ax[0].size(), ny = (int)ax[1].size(), nz = (int)ax[2].size();
+    const int W = (nx + 63) / 64;
+    keep.assign((size_t)W * ny * nz, 0);
+    Recursion R{ax, delta * 1e-3, &f};
+    const Block root{{0, 0, 0}, {nx - 1, ny - 1, nz - 1}};
+    const long sub_volume = 64;
+    const long divide = std::max(root.volume() / 4096, sub_volume);  // mc.go:128
+    std::vector<Block> queue;
+    R.pieces(root, divide, [&](const Block &b) { queue.push_back(b); });
+    // each queued block split further (mc.go:105-107), on `threads` threads
+    std::vector<std::vector<Block>> leaves(queue.size());
+    auto work = [&](size_t i0, size_t step) {
+        for (size_t i = i0; i < queue.size(); i += step)
+            R.pieces(queue[i], sub_volume, [&](const Block &b) { leaves[i].push_back(b); });
+    };
+    const size_t nt = std::max<size_t>(1, std::min<size_t>((size_t)threads, queue.size()));
+    if (nt == 1) {
+        work(0, 1);
+    } else {
+        std::vector<std::thread> th;
+        for (size_t t = 0; t < nt; t++) th.emplace_back(work, t, nt);
+        for (auto &t : th) t.join();
+    }
+    // the cubes of every accepted leaf, as bits of the occupancy word layout
+    for (const auto &ls : leaves)
+        for (const Block &b : ls)
+            for (int z = b.mn[2]; z < b.mx[2]; z++)
+                for (int y = b.mn[1]; y < b.mx[1]; y++) {
+                    uint64_t *row = &keep[((size_t)z * ny + y) * W];
+                    for (int x = b.mn[0]; x < b.mx[0];) {
+                        const int w = x >> 6, lo = x & 63, hi = std::min(64, b.mx[0] - (w << 6));
+                        const uint64_t m = (hi >= 64 ? ~0ull : ((1ull << hi) - 1)) & (~0ull << lo);
+                        row[w] |= m;
+                        x = (w << 6) + hi;
+                    }
+                }
+    return MC3D_OK;
+}
+
+}  // namespace mc3d

@@ -597,6 +597,27 @@ __device__ __forceinline__ Masks nbhd_masks(const Nbhd &h) {
     return m;
 }
 
+// With a block filter (MarchingCubesFilter): keep has a bit per cube of an
+// accepted block, in the occupancy layout.  An edge gives a vertex only when
+// a kept cube uses it: an x edge at corner (x, y, z) belongs to cubes
+// (x, y-1..y, z-1..z), a y edge to (x-1..x, y, z-1..z), a z edge to
+// (x-1..x, y-1..y, z).
+__device__ __forceinline__ uint64_t keep_word(const Grid &g, const uint64_t *__restrict__ keep, int wx, int y, int z) {
+    if (wx < 0 || y < 0 || z < 0) return 0;
+    return keep[word_index(g, wx, y, z)];
+}
+__device__ __forceinline__ void keep_masks(const Grid &g, const uint64_t *__restrict__ keep, int wx, int y, int z,
+                                           Masks &m) {
+    const uint64_t k00 = keep_word(g, keep, wx, y, z), k10 = keep_word(g, keep, wx, y - 1, z);
+    const uint64_t k01 = keep_word(g, keep, wx, y, z - 1), k11 = keep_word(g, keep, wx, y - 1, z - 1);
+    const uint64_t p00 = keep_word(g, keep, wx - 1, y, z), p01 = keep_word(g, keep, wx - 1, y, z - 1);
+    const uint64_t p10 = keep_word(g, keep, wx - 1, y - 1, z);
+    m.cx &= k00 | k10 | k01 | k11;
+    const uint64_t ky = k00 | k01, kz = k00 | k10;
+    m.cy &= ky | (ky << 1) | ((p00 | p01) >> 63);
+    m.cz &= kz | (kz << 1) | ((p00 | p10) >> 63);
+}
+
 // c[k] bit i is corner k (k = dx + 2 dy + 4 dz, mc.go:260-301) of cube 64wx+i.
 struct CubeCorners {
     uint64_t c[8];
@@ -629,7 +650,7 @@ __global__ __launch_bounds__(256) void k_classify(const Seg *__restrict__ segs, 
                                                   const uint32_t *__restrict__ word_seg,
                                                   const uint64_t *__restrict__ bits_all,
                                                   const uint8_t *__restrict__ table, uint4 *__restrict__ counts,
-                                                  uint64_t nwords_all) {
+                                                  const uint64_t *__restrict__ keep_all, uint64_t nwords_all) {
     __shared__ uint8_t ntri[256];
     ntri[threadIdx.x] = table[16 * threadIdx.x];
     __syncthreads();
@@ -647,10 +668,15 @@ __global__ __launch_bounds__(256) void k_classify(const Seg *__restrict__ segs, 
         int wx, y, z;
         word_coords(g, w, wx, y, z);
         const Nbhd h = load_nbhd(g, bits, wx, y, z);
-        const Masks m = nbhd_masks(h);
-        const uint32_t nv = __popcll(m.cx) + __popcll(m.cy) + __popcll(m.cz);
+        Masks m = nbhd_masks(h);
         const CubeCorners cc = nbhd_corners(h);
         uint64_t mixed = mixed_cubes(cc);
+        if (keep_all) {  // a block filter: only the cubes of accepted blocks
+            const uint64_t *keep = keep_all + sg.word0;
+            keep_masks(g, keep, wx, y, z, m);
+            mixed &= keep[w];
+        }
+        const uint32_t nv = __popcll(m.cx) + __popcll(m.cy) + __popcll(m.cz);
         uint32_t nt = 0;
         while (mixed) {
             const int i = __ffsll((unsigned long long)mixed) - 1;
@@ -766,7 +792,8 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
                                               const uint8_t *__restrict__ table, const uint4 *__restrict__ offs_all,
                                               const uint32_t *__restrict__ active, Ctl *__restrict__ ctl,
                                               uint64_t *__restrict__ vkeys, uint32_t *__restrict__ vseg,
-                                              uint32_t *__restrict__ tris, uint64_t vcap, uint64_t tcap) {
+                                              uint32_t *__restrict__ tris, const uint64_t *__restrict__ keep_all,
+                                              uint64_t vcap, uint64_t tcap) {
     __shared__ uint8_t tab[256 * 16];
     __shared__ WordInfo wi_all[4][8];
     __shared__ uint32_t s_pre[4][64];
@@ -809,9 +836,9 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
             WordInfo r{0, 0, 0, 0, 0, 0};
             if (in) {
                 r.bits = b;
-                r.cx = b ^ ((b >> 1) | ((hx ? bn : 0) << 63));
-                r.cy = b ^ (hy ? by : 0);
-                r.cz = b ^ (hz ? bz : 0);
+                Masks m{b ^ ((b >> 1) | ((hx ? bn : 0) << 63)), b ^ (hy ? by : 0), b ^ (hz ? bz : 0)};
+                if (keep_all) keep_masks(g, keep_all + sg.word0, ex, ey, ez, m);
+                r.cx = m.cx, r.cy = m.cy, r.cz = m.cz;
                 r.base = o.x;
                 r.tbase = o.y;
             }
@@ -861,6 +888,7 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
                 cs |= (int)((b >> lane) & 1) << e;
                 cs |= (int)((((b >> 1) | (bn << 63)) >> lane) & 1) << (e + 1);
             }
+        if (keep_all && !((keep_all[sg.word0 + w] >> lane) & 1)) cs = 0;  // not in an accepted block
         const int nt = tab[16 * cs];
         const uint32_t pre = __popcll(__ballot(nt & 1) & below) + 2 * __popcll(__ballot(nt & 2) & below) +
                              4 * __popcll(__ballot(nt & 4) & below);

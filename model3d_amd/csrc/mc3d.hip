@@ -178,6 +178,7 @@ struct mc3d_ctx {
     DevBuf hvals, hpts, hstate, hprobe;  // host-evaluated atoms: answers, points, bisection state
     HostBuf hstage;                      // pinned points / answers of one callback batch
     DevBuf stl, stl_in;                  // STL encoding: body bytes; uploaded host triangles
+    DevBuf keep;                         // block filter: cube mask
     std::shared_ptr<Pool> pool = std::make_shared<Pool>();
     uint64_t v_hint = 1u << 18, t_hint = 1u << 19;  // output capacities to start from
     hipEvent_t ev[10] = {};
@@ -213,6 +214,7 @@ struct Job {
     std::vector<uint32_t> frames;  // frame of each host-evaluated atom
     bool interior = false;         // MarchingCubesInterior: keep each vertex's inside point
     std::vector<double> conj;      // MarchingCubesConj: inverse chain applied to the vertices
+    std::vector<uint64_t> keep;    // MarchingCubesFilter: cubes of accepted blocks (occupancy layout)
 };
 
 // Frame of each host-evaluated atom, by host atom index.
@@ -545,6 +547,7 @@ struct Call {
     int mode = 0, rows_per_walk = 1, wpb = 4;
     bool stats = false;
     bool interior = false;  // a call of one MarchingCubesInterior solid
+    const uint64_t *keep = nullptr;  // a call of one filtered solid: its cube mask on the device
 };
 
 template <int MODE, bool STATS, int NP, bool MULTI>
@@ -654,13 +657,14 @@ void launch_emit(mc3d_ctx *ctx, const Call &c, const Upload &U, MeshBuffers &mb,
         k_emit<true><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(),
                                                     ctx->table.as<uint8_t>(), ctx->offs.as<uint4>(),
                                                     ctx->actw.as<uint32_t>(), U.ctl, ctx->vkeys.as<uint64_t>(),
-                                                    ctx->vseg.as<uint32_t>(), mb.tris->as<uint32_t>(), vcap, tcap);
+                                                    ctx->vseg.as<uint32_t>(), mb.tris->as<uint32_t>(), nullptr, vcap,
+                                                    tcap);
     else
         k_emit<false><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.word_seg,
                                                      ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
                                                      ctx->offs.as<uint4>(), ctx->actw.as<uint32_t>(), U.ctl,
-                                                     ctx->vkeys.as<uint64_t>(), nullptr, mb.tris->as<uint32_t>(), vcap,
-                                                     tcap);
+                                                     ctx->vkeys.as<uint64_t>(), nullptr, mb.tris->as<uint32_t>(), c.keep,
+                                                     vcap, tcap);
 }
 
 // The output stages: emit (vertex keys, triangles) and bisect (vertices).
@@ -781,6 +785,12 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
                    (rc = ctx->actb.ensure(c.nbricks * sizeof(uint32_t))) ||
                    (rc = ctx->bcode.ensure(c.nbricks * c.cap * sizeof(Insn) + 16))))
         return rc;
+    if (c.nsegs == 1 && !jobs[0].keep.empty()) {
+        if ((rc = ctx->keep.ensure(jobs[0].keep.size() * 8))) return rc;
+        HIP_TRY(hipMemcpyAsync(ctx->keep.p, jobs[0].keep.data(), jobs[0].keep.size() * 8, hipMemcpyHostToDevice,
+                               ctx->stream));
+        c.keep = ctx->keep.as<uint64_t>();
+    }
     Upload U;
     if ((rc = upload(ctx, jobs, c.nwords, c.nbricks, U))) return rc;
     if ((rc = ctx->ctl_host.ensure(U.back_bytes))) return rc;
@@ -825,11 +835,11 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
     if (c.nsegs > 1)
         k_classify<true><<<grid_for(c.nwords, 256), 256, 0, ctx->stream>>>(
             U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
-            ctx->counts.as<uint4>(), c.nwords);
+            ctx->counts.as<uint4>(), nullptr, c.nwords);
     else
         k_classify<false><<<grid_for(c.nwords, 256), 256, 0, ctx->stream>>>(
             U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
-            ctx->counts.as<uint4>(), c.nwords);
+            ctx->counts.as<uint4>(), c.keep, c.nwords);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[8], ctx->stream));
     if ((rc = scan4(ctx, ctx->counts.as<uint4>(), ctx->offs.as<uint4>(), c.nwords, U.ctl->totals,
@@ -1030,8 +1040,11 @@ namespace {
 // One solid, one call: MarchingCubesSearch, with interior points
 // (MarchingCubesInterior) or with an inverse chain for the vertices
 // (MarchingCubesConj).
+using RectFilter = std::function<bool(const double *, const double *)>;
+
 int search_one(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters, const mc3d_options *opts,
-               mc3d_mesh **out, bool interior, std::vector<double> conj) {
+               mc3d_mesh **out, bool interior, std::vector<double> conj, const RectFilter *filter = nullptr,
+               int filter_threads = 1) {
     *out = nullptr;
     const auto t_call = std::chrono::steady_clock::now();
     const mc3d_options o = default_options(opts);
@@ -1042,6 +1055,7 @@ int search_one(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters, cons
     jobs[0].conj = std::move(conj);
     int rc = prepare_job(jobs[0], o);
     if (rc) return rc;
+    if (filter && (rc = filter_keep_mask(jobs[0].ax, delta, *filter, filter_threads, jobs[0].keep))) return rc;
     const double ms_prepare = ms_since(t_call);
     std::vector<mc3d_mesh *> meshes;
     if ((rc = run_call(ctx, jobs, o, meshes, t_call, ms_prepare))) return rc;
@@ -1049,6 +1063,83 @@ int search_one(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters, cons
     return MC3D_OK;
 }
 }  // namespace
+
+int mc3d_marching_cubes_search_filter(mc3d_ctx *ctx, const mc3d_solid *s, mc3d_rect_filter_fn f, void *user,
+                                      double delta, int iters, const mc3d_options *opts, mc3d_mesh **out) {
+    if (!ctx || !s || !f || !out) {
+        set_error("mc3d_marching_cubes_search_filter: null argument");
+        return MC3D_E_INVALID_ARG;
+    }
+    const RectFilter filter = [f, user](const double *mn, const double *mx) { return f(user, mn, mx) != 0; };
+    return search_one(ctx, s, delta, iters, opts, out, false, {}, &filter, 1);
+}
+
+struct mc3d_rect_collider {
+    RectMesh mesh;
+};
+
+mc3d_rect_collider *mc3d_rect_collider_create(const double *triangles, size_t n) {
+    if (n && !triangles) {
+        set_error("mc3d_rect_collider_create: null triangles");
+        return nullptr;
+    }
+    mc3d_rect_collider *c = new (std::nothrow) mc3d_rect_collider;
+    if (!c) {
+        set_error("out of memory");
+        return nullptr;
+    }
+    c->mesh.build(triangles, n);
+    return c;
+}
+
+int mc3d_rect_collider_collides(void *collider, const double min[3], const double max[3]) {
+    return collider && ((const mc3d_rect_collider *)collider)->mesh.collides(min, max) ? 1 : 0;
+}
+
+void mc3d_rect_collider_free(mc3d_rect_collider *c) { delete c; }
+
+int mc3d_marching_cubes_filter(mc3d_ctx *ctx, const mc3d_solid *s, mc3d_rect_filter_fn f, void *user, double delta,
+                               const mc3d_options *opts, mc3d_mesh **out) {
+    return mc3d_marching_cubes_search_filter(ctx, s, f, user, delta, 0, opts, out);
+}
+
+int mc3d_marching_cubes_c2f(mc3d_ctx *ctx, const mc3d_solid *s, double big_delta, double small_delta,
+                            double extra_space, int iters, const mc3d_options *opts, mc3d_mesh **out) {
+    if (!ctx || !s || !out) {
+        set_error("mc3d_marching_cubes_c2f: null argument");
+        return MC3D_E_INVALID_ARG;
+    }
+    *out = nullptr;
+    if (big_delta < small_delta) {
+        set_error("bigDelta should be >= smallDelta for efficiency");  // mc.go:162-164
+        return MC3D_E_INVALID_ARG;
+    }
+    extra_space += 2 * big_delta * sqrt(3.0);  // mc.go:167
+    // the coarse mesh (mc.go:169), its triangles on the host
+    mc3d_options co = default_options(opts);
+    co.device_only = 1;
+    mc3d_mesh *coarse = nullptr;
+    int rc = search_one(ctx, s, big_delta, iters, &co, &coarse, false, {});
+    if (rc) return rc;
+    std::vector<double> v(coarse->nv * 3);
+    std::vector<uint32_t> t(coarse->nt * 3);
+    rc = mc3d_mesh_copy(coarse, v.data(), t.data());
+    const size_t nt = coarse->nt;
+    mc3d_mesh_free(coarse);
+    if (rc) return rc;
+    std::vector<double> tris(nt * 9);
+    for (size_t i = 0; i < 3 * nt; i++)
+        for (int d = 0; d < 3; d++) tris[3 * i + d] = v[3 * (size_t)t[i] + d];
+    RectMesh collider;  // MeshToCollider(coarseMesh), mc.go:170
+    collider.build(tris.data(), nt);
+    // filter = collider.RectCollision(r.Expand(extraSpace)), mc.go:171-174
+    const RectFilter filter = [&collider, extra_space](const double *mn, const double *mx) {
+        const double emn[3] = {mn[0] + -extra_space, mn[1] + -extra_space, mn[2] + -extra_space};
+        const double emx[3] = {mx[0] + extra_space, mx[1] + extra_space, mx[2] + extra_space};
+        return collider.collides(emn, emx);
+    };
+    return search_one(ctx, s, small_delta, iters, opts, out, false, {}, &filter, 16);
+}
 
 int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
                                const mc3d_options *opts, mc3d_mesh **out) {

@@ -4,6 +4,7 @@
 
 #include <stdint.h>
 
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -129,6 +130,33 @@ using SolidPtr = std::shared_ptr<const Solid>;
 bool lower_solid(const Solid &s, Program &out, std::string &msg);
 // JoinedTransform(xforms).Inverse() as frame records {kind, v[9]} per op.
 std::vector<double> inverse_records(const mc3d_xform *xforms, size_t n);
+
+// MarchingCubesFilter's block recursion (mc.go:83-135, 724-797) over the grid
+// axes: keep gets one bit per cube of an accepted block, in the occupancy
+// word layout (cube (x, y, z) is bit x % 64 of word (x / 64, y, z)).  f is
+// asked for blocks' bounds, from `threads` threads below the first level.
+int filter_keep_mask(const std::vector<double> ax[3], double delta,
+                     const std::function<bool(const double *, const double *)> &f, int threads,
+                     std::vector<uint64_t> &keep);
+
+// MeshToCollider(mesh).RectCollision (collisions.go:138-179, 344-356;
+// primitives.go:448-495) over a triangle list: true when a triangle whose
+// bounding box overlaps the rect collides with it.
+struct RectMesh {
+    static constexpr uint32_t NONE = 0xffffffffu;
+    struct Node {
+        double mn[3], mx[3];
+        uint32_t lo = 0, hi = 0, left = NONE, right = NONE;
+    };
+    std::vector<double> tris;  // 9 doubles per triangle
+    std::vector<uint32_t> order;
+    std::vector<Node> nodes;
+    void build(const double *tris9, size_t n);
+    bool collides(const double *mn, const double *mx) const;
+
+  private:
+    uint32_t split(uint32_t lo, uint32_t hi);
+};
 void solid_bounds(const Solid &s, double mn[3], double mx[3]);
 bool bounds_valid(const Solid &s);
 

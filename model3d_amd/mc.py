@@ -246,6 +246,74 @@ def WriteSTL(w, mesh_or_triangles, device: int = 0) -> None:
             fh.write(data)
 
 
+def _rect_filter(f):
+    """(C function, user pointer, keep-alive) for a block filter: a
+    MeshCollider's RectCollision runs natively; any other callable f(rect)
+    is called back with a Rect."""
+    from .solids import MeshCollider, Rect
+
+    if getattr(f, "__func__", None) is MeshCollider.RectCollision and isinstance(f.__self__, MeshCollider):
+        L = _lib.lib()
+        return ctypes.cast(L.mc3d_rect_collider_collides, ctypes.c_void_p), f.__self__._rect_handle(), f.__self__
+
+    def cb(user, mn, mx):
+        return 1 if f(Rect((mn[0], mn[1], mn[2]), (mx[0], mx[1], mx[2]))) else 0
+
+    fn = _lib.RECT_FILTER_FN(cb)
+    return ctypes.cast(fn, ctypes.c_void_p), None, fn
+
+
+def MarchingCubesSearchFilter(s: Solid, f, delta: float, iters: int, *, precision: int = _lib.MC3D_PREC_FP32_GUARD,
+                              cull: bool = True, device: int = 0) -> Mesh:
+    """model3d.MarchingCubesSearchFilter (mc.go:137-143) on the GPU: marching
+    cubes only in the blocks for which f(rect) is true (the reference's block
+    recursion, mc.go:73-135, 724-797), then the vertex search."""
+    L = _lib.lib()
+    ctx = _lib.context(device)
+    b = LibBackend()
+    h = s._build(b)
+    out = ctypes.c_void_p()
+    fn, user, keep = _rect_filter(f)
+    try:
+        opts = _options(precision, cull)
+        _raise_for(L.mc3d_marching_cubes_search_filter(ctx, h, fn, user, float(delta), int(iters), ctypes.byref(opts),
+                                                       ctypes.byref(out)))
+        return _fetch(L, out)
+    finally:
+        del keep
+        if out.value:
+            L.mc3d_mesh_free(out)
+        b.release(h)
+        b.close()
+
+
+def MarchingCubesFilter(s: Solid, f, delta: float, **opts) -> Mesh:
+    """model3d.MarchingCubesFilter (mc.go:73-135): MarchingCubes in the
+    blocks f accepts (no vertex search)."""
+    return MarchingCubesSearchFilter(s, f, delta, 0, **opts)
+
+
+def MarchingCubesC2F(s: Solid, bigDelta: float, smallDelta: float, extraSpace: float, iters: int, *,
+                     precision: int = _lib.MC3D_PREC_FP32_GUARD, cull: bool = True, device: int = 0) -> Mesh:
+    """model3d.MarchingCubesC2F (mc.go:145-180) on the GPU: a coarse search,
+    its mesh as a rect collider (host, 16 threads), the filtered fine search."""
+    L = _lib.lib()
+    ctx = _lib.context(device)
+    b = LibBackend()
+    h = s._build(b)
+    out = ctypes.c_void_p()
+    try:
+        opts = _options(precision, cull)
+        _raise_for(L.mc3d_marching_cubes_c2f(ctx, h, float(bigDelta), float(smallDelta), float(extraSpace), int(iters),
+                                             ctypes.byref(opts), ctypes.byref(out)))
+        return _fetch(L, out)
+    finally:
+        if out.value:
+            L.mc3d_mesh_free(out)
+        b.release(h)
+        b.close()
+
+
 def _raise_for(rc: int):
     """A reference panic as Panic (same message), anything else as MC3DError."""
     if rc != _lib.MC3D_OK:

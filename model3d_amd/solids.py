@@ -403,6 +403,31 @@ class MeshCollider:
         if len(t) == 0:
             raise ValueError("empty mesh")
         self.triangles = t
+        self._rect = None
+
+    def _rect_handle(self):
+        """The library's rect collider over these triangles (built once)."""
+        if self._rect is None:
+            L = _lib.lib()
+            h = L.mc3d_rect_collider_create(self.triangles.ctypes.data_as(_lib._dp), len(self.triangles))
+            if not h:
+                raise _lib.MC3DError(_lib.MC3D_E_OOM, _lib.last_error())
+            self._rect = h
+        return self._rect
+
+    def RectCollision(self, rect) -> bool:
+        """MultiCollider.RectCollision (collisions.go:344-356,
+        primitives.go:448-495): does any triangle touch the rect."""
+        L = _lib.lib()
+        return bool(L.mc3d_rect_collider_collides(self._rect_handle(), _lib._D3(*rect.MinVal), _lib._D3(*rect.MaxVal)))
+
+    def __del__(self):
+        if getattr(self, "_rect", None):
+            try:
+                _lib.lib().mc3d_rect_collider_free(self._rect)
+            except Exception:
+                pass
+            self._rect = None
 
 
 def MeshToCollider(mesh) -> MeshCollider:

@@ -20,6 +20,7 @@ _D3 = ctypes.c_double * 3
 _P = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
 HOST_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, _dp)
+RECT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, _dp, _dp)
 
 _SIG = {
     "or_sphere": (_P, [_D3, ctypes.c_double]),
@@ -54,6 +55,11 @@ _SIG = {
     "or_mesh_interior": (_dp, [_P, ctypes.POINTER(ctypes.c_long)]),
     "or_inverse_apply": (None, [ctypes.POINTER(ctypes.c_int), _dp, ctypes.c_int, _dp, ctypes.c_long]),
     "or_encode_stl": (None, [_dp, ctypes.c_long, ctypes.c_void_p]),
+    "or_marching_cubes_filter": (
+        ctypes.c_int,
+        [_P, RECT_FN, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)],
+    ),
+    "or_tris_rect_collision": (ctypes.c_int, [_dp, ctypes.c_long, _D3, _D3]),
     "or_mesh_ntris": (ctypes.c_long, [_P]),
     "or_mesh_tris": (_dp, [_P]),
     "or_mesh_info": (
@@ -261,6 +267,55 @@ def oracle_encode_stl(tris) -> bytes:
     out = np.zeros(84 + 50 * len(t), dtype=np.uint8)
     lib().or_encode_stl(t.ctypes.data_as(_dp), len(t), out.ctypes.data)
     return out.tobytes()
+
+
+def oracle_search_filter(solid, rect_filter, delta, iters, threads=None):
+    """MarchingCubesSearchFilter (MarchingCubesFilter for iters 0) on the CPU:
+    rect_filter(min, max) -> bool for each block.  Returns triangles (T, 9)."""
+    if threads is None:
+        threads = os.cpu_count() or 1
+    o = OracleSolid(solid)
+    L = o.backend.L
+
+    def cb(user, mn, mx):
+        return 1 if rect_filter((mn[0], mn[1], mn[2]), (mx[0], mx[1], mx[2])) else 0
+
+    fn = RECT_FN(cb)
+    m = _P()
+    rc = L.or_marching_cubes_filter(o.h, fn, None, float(delta), int(iters), int(threads), ctypes.byref(m))
+    if rc:
+        raise OracleError(rc)
+    try:
+        n = L.or_mesh_ntris(m)
+        return np.ctypeslib.as_array(L.or_mesh_tris(m), shape=(n * 9,)).reshape(n, 9).copy() if n else np.zeros((0, 9))
+    finally:
+        L.or_mesh_free(m)
+
+
+def oracle_rect_collider(tris):
+    """MeshToCollider(mesh).RectCollision for a triangle list (T, 9): a
+    function of (min, max), brute force over the triangles."""
+    t = np.ascontiguousarray(np.asarray(tris, dtype=np.float64).reshape(-1, 9))
+    L = lib()
+
+    def collides(mn, mx):
+        return bool(L.or_tris_rect_collision(t.ctypes.data_as(_dp), len(t), _D3(*mn), _D3(*mx)))
+
+    return collides
+
+
+def oracle_c2f(solid, big_delta, small_delta, extra_space, iters, threads=None):
+    """MarchingCubesC2F (mc.go:145-180) on the CPU."""
+    import math
+
+    extra = extra_space + 2 * big_delta * math.sqrt(3)
+    coarse, _ = oracle_search(solid, big_delta, iters, threads)
+    collides = oracle_rect_collider(coarse)
+
+    def f(mn, mx):  # Rect.Expand, shapes.go:339-344
+        return collides(tuple(v + -extra for v in mn), tuple(v + extra for v in mx))
+
+    return oracle_search_filter(solid, f, small_delta, iters, threads)
 
 
 def oracle_case_table() -> np.ndarray:

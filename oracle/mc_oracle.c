@@ -1113,6 +1113,207 @@ static int mc_search(const or_node *s, double delta, int iters, int threads, or_
     return err;
 }
 
+/* ------------------------------------------------------------------------ */
+/* MarchingCubesFilter, mc.go:73-135, with mcBlock / mcBlockCache            */
+/* (mc.go:724-858): single-threaded, block by block.                          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { long mn[3], mx[3]; } mc_block;
+
+static long block_volume(const mc_block *b) {
+    return (b->mx[0] - b->mn[0]) * (b->mx[1] - b->mn[1]) * (b->mx[2] - b->mn[2]);
+}
+
+typedef struct {
+    const or_node *s;
+    const spacer *sp;
+    or_rect_fn f;
+    void *user;
+    double eps;
+    or_mesh *m;
+    uint8_t *vals;
+    int err;
+} filter_state;
+
+/* blockFilter: f(m.Bounds(epsilon)), mc.go:95-101, 736-740 */
+static int block_accept(filter_state *st, const mc_block *b) {
+    double mn[3], mx[3];
+    for (int a = 0; a < 3; a++) {
+        mn[a] = st->sp->v[a][b->mn[a]] + -st->eps;
+        mx[a] = st->sp->v[a][b->mx[a]] + st->eps;
+    }
+    return st->f(st->user, mn, mx) != 0;
+}
+
+/* mcBlockCache.Populate + the cube loop of MarchingCubesFilter (mc.go:104-121,
+ * 808-858) */
+static void process_block(filter_state *st, const mc_block *b) {
+    const spacer *sp = st->sp;
+    long sx = b->mx[0] - b->mn[0] + 1, sy = b->mx[1] - b->mn[1] + 1, sz = b->mx[2] - b->mn[2] + 1;
+    uint8_t *v = (uint8_t *)realloc(st->vals, (size_t)(sx * sy * sz));
+    if (!v) { st->err = OR_E_OOM; return; }
+    st->vals = v;
+    long k = 0;
+    for (long z = b->mn[2]; z <= b->mx[2]; z++) {
+        int ez = z == 0 || z == sp->n[2] - 1;
+        for (long y = b->mn[1]; y <= b->mx[1]; y++) {
+            int ey = y == 0 || y == sp->n[1] - 1;
+            for (long x = b->mn[0]; x <= b->mx[0]; x++) {
+                int ex = x == 0 || x == sp->n[0] - 1;
+                int value = node_contains(st->s, v3(sp->v[0][x], sp->v[1][y], sp->v[2][z]));
+                if (value && (ex || ey || ez)) { st->err = OR_E_INSIDE_BOUNDARY; return; }
+                v[k++] = (uint8_t)value;
+            }
+        }
+    }
+    for (long z = b->mn[2]; z < b->mx[2]; z++)
+        for (long y = b->mn[1]; y < b->mx[1]; y++)
+            for (long x = b->mn[0]; x < b->mx[0]; x++) {
+                int bits = 0, mask = 1;
+                for (int z1 = 0; z1 < 2; z1++) /* getSquare(z) | getSquare(z+1) << 4 */
+                    for (int y1 = 0; y1 < 2; y1++)
+                        for (int x1 = 0; x1 < 2; x1++, mask <<= 1) {
+                            long i = ((z + z1 - b->mn[2]) * sy + (y + y1 - b->mn[1])) * sx + (x + x1 - b->mn[0]);
+                            if (v[i]) bits |= mask;
+                        }
+                const mc_case *cs = &g_table[bits];
+                if (cs->ntri <= 0) continue;
+                c3 mn = v3(sp->v[0][x], sp->v[1][y], sp->v[2][z]);
+                c3 mx = v3(sp->v[0][x + 1], sp->v[1][y + 1], sp->v[2][z + 1]);
+                c3 corners[8];
+                for (int c = 0; c < 8; c++)
+                    corners[c] = v3(c & 1 ? mx.x : mn.x, c & 2 ? mx.y : mn.y, c & 4 ? mx.z : mn.z);
+                for (int t = 0; t < cs->ntri; t++) {
+                    c3 tri[3];
+                    for (int e = 0; e < 3; e++) tri[e] = c3_mid(corners[cs->t[t][2 * e]], corners[cs->t[t][2 * e + 1]]);
+                    if (mesh_push(st->m, tri)) { st->err = OR_E_OOM; return; }
+                }
+            }
+}
+
+/* mcBlock.Pieces with Split, mc.go:760-797; leaf != 0: process, else recurse
+ * again with the sub-division volume (the second level, mc.go:105-107). */
+static void block_pieces(filter_state *st, const mc_block *b, long min_volume, int second) {
+    if (st->err || !block_accept(st, b)) return;
+    if (block_volume(b) / 2 < min_volume) {
+        if (second) process_block(st, b);
+        else block_pieces(st, b, 64, 1);
+        return;
+    }
+    long lx = b->mx[0] - b->mn[0], ly = b->mx[1] - b->mn[1], lz = b->mx[2] - b->mn[2];
+    int axis = 0;
+    if (ly >= lx && ly >= lz) axis = 1;
+    else if (lz >= lx && lz >= ly) axis = 2;
+    mc_block b1 = *b, b2 = *b;
+    b1.mx[axis] = (b->mx[axis] + b->mn[axis]) / 2;
+    b2.mn[axis] = b1.mx[axis];
+    block_pieces(st, &b1, min_volume, second);
+    block_pieces(st, &b2, min_volume, second);
+}
+
+int or_marching_cubes_filter(const or_node *s, or_rect_fn f, void *user, double delta, int iters, int threads,
+                             or_mesh **out) {
+    *out = NULL;
+    if (!bounds_valid(s)) return OR_E_INVALID_BOUNDS;
+    pthread_once(&g_table_once, init_table);
+    spacer sp;
+    int err = spacer_make(s, delta, &sp);
+    if (err) { spacer_free(&sp); return err; }
+    or_mesh *m = (or_mesh *)calloc(1, sizeof(or_mesh));
+    m->n_corners = sp.n[0] * sp.n[1] * sp.n[2];
+    filter_state st = {s, &sp, f, user, delta * 1e-3, m, NULL, 0};
+    mc_block root = {{0, 0, 0}, {sp.n[0] - 1, sp.n[1] - 1, sp.n[2] - 1}};
+    long divide = block_volume(&root) / 4096;
+    if (divide < 64) divide = 64; /* essentials.MaxInt(rootBlock.Volume()/4096, subDivideVolume) */
+    block_pieces(&st, &root, divide, 0);
+    free(st.vals);
+    spacer_free(&sp);
+    err = st.err;
+    if (!err) err = mc_search(s, delta, iters, threads, m, 0);
+    if (err) { or_mesh_free(m); return err; }
+    *out = m;
+    return 0;
+}
+
+/* RectCollision of a triangle list (MeshToCollider(...).RectCollision, the
+ * OR over its triangles of Triangle.RectCollision, primitives.go:448-495),
+ * brute force. */
+static int ray_scale(const double *t, c3 origin, c3 dir, double *scale) {
+    double d = c3_dot(tri_normal(t), c3_normalize(dir));
+    if (d < 1e-8 && d > -1e-8) return 0;
+    c3 t0 = c3_of(t), t1 = c3_of(t + 3), t2 = c3_of(t + 6);
+    c3 v1 = c3_sub(t1, t0), v2 = c3_sub(t2, t0);
+    c3 cross1 = c3_cross(dir, v2);
+    double det = c3_dot(cross1, v1);
+    if (det == 0) return 0;
+    double inv_det = 1 / det;
+    c3 o = c3_sub(origin, t0);
+    double bary1 = inv_det * c3_dot(o, cross1);
+    if (bary1 < 0 || bary1 > 1) return 0;
+    c3 cross2 = c3_cross(o, v1);
+    double bary2 = inv_det * c3_dot(dir, cross2);
+    if (bary2 < 0 || bary1 + bary2 > 1) return 0;
+    *scale = inv_det * c3_dot(v2, cross2);
+    return *scale >= 0; /* FirstRayCollision, primitives.go:181-188 */
+}
+
+static void new_seg(c3 p1, c3 p2, c3 *a, c3 *b) { /* NewSegment, primitives.go:547-554 */
+    if (p1.x < p2.x || (p1.x == p2.x && p1.y < p2.y) || (p1.x == p2.x && p1.y == p2.y && p1.z < p2.z)) {
+        *a = p1, *b = p2;
+    } else {
+        *a = p2, *b = p1;
+    }
+}
+
+static int seg_rect(c3 a, c3 b, const double *mn, const double *mx) { /* primitives.go:632-638, bvh.go:322-351 */
+    c3 dir = c3_sub(b, a);
+    double o[3] = {a.x, a.y, a.z}, r[3] = {dir.x, dir.y, dir.z};
+    double min_frac = -INFINITY, max_frac = INFINITY;
+    for (int axis = 0; axis < 3; axis++) {
+        if (r[axis] == 0) {
+            if (o[axis] < mn[axis] || o[axis] > mx[axis]) { min_frac = 0, max_frac = -1; break; }
+            continue;
+        }
+        double t1 = (mn[axis] - o[axis]) / r[axis], t2 = (mx[axis] - o[axis]) / r[axis];
+        if (t1 > t2) { double tt = t1; t1 = t2; t2 = tt; }
+        if (t2 < 0) { min_frac = 0, max_frac = -1; break; }
+        if (t1 > min_frac) min_frac = t1;
+        if (t2 < max_frac) max_frac = t2;
+    }
+    return max_frac >= min_frac && max_frac >= 0 && min_frac <= 1;
+}
+
+static int tri_seg(const double *t, c3 p1, c3 p2) { /* Triangle.SegmentCollision, primitives.go:440-445 */
+    c3 a, b;
+    new_seg(p1, p2, &a, &b);
+    double sc;
+    return ray_scale(t, a, c3_sub(b, a), &sc) && sc <= 1;
+}
+
+static int tri_rect(const double *t, const double *mn, const double *mx) {
+    for (int i = 0; i < 3; i++) {
+        c3 a, b;
+        new_seg(c3_of(t + 3 * i), c3_of(t + 3 * ((i + 1) % 3)), &a, &b);
+        if (seg_rect(a, b, mn, mx)) return 1;
+    }
+    for (int k = 0; k < 2; k++) {
+        double x = k ? mx[0] : mn[0];
+        if (tri_seg(t, v3(x, mn[1], mn[2]), v3(x, mn[1], mx[2])) || tri_seg(t, v3(x, mn[1], mn[2]), v3(x, mx[1], mn[2])) ||
+            tri_seg(t, v3(x, mx[1], mn[2]), v3(x, mx[1], mx[2])) || tri_seg(t, v3(x, mn[1], mx[2]), v3(x, mx[1], mx[2])))
+            return 1;
+    }
+    return tri_seg(t, v3(mn[0], mn[1], mn[2]), v3(mx[0], mn[1], mn[2])) ||
+           tri_seg(t, v3(mn[0], mx[1], mn[2]), v3(mx[0], mx[1], mn[2])) ||
+           tri_seg(t, v3(mn[0], mn[1], mx[2]), v3(mx[0], mn[1], mx[2])) ||
+           tri_seg(t, v3(mn[0], mx[1], mx[2]), v3(mx[0], mx[1], mx[2]));
+}
+
+int or_tris_rect_collision(const double *tris, long nt, const double *mn, const double *mx) {
+    for (long i = 0; i < nt; i++)
+        if (tri_rect(tris + 9 * i, mn, mx)) return 1;
+    return 0;
+}
+
 /* MarchingCubes (mc.go:14-37) followed by mcSearch (mc.go:182-221); with
  * want_interior, MarchingCubesInterior (mc.go:55-60). */
 static int marching_cubes(const or_node *s, double delta, int iters, int threads, int want_interior,

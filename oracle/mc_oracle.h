@@ -74,6 +74,13 @@ const double *or_mesh_interior(const or_mesh *m, long *n);
 /* JoinedTransform(ops).Inverse().Apply at n points in place (3 doubles each);
  * ops as or_transform takes them. */
 void or_inverse_apply(const int *kinds, const double *params, int nops, double *pts, long n);
+/* MarchingCubesSearchFilter (MarchingCubesFilter when iters == 0): the
+ * reference's block recursion, corners evaluated block by block. */
+typedef int (*or_rect_fn)(void *user, const double *mn, const double *mx);
+int or_marching_cubes_filter(const or_node *s, or_rect_fn f, void *user, double delta, int iters, int threads,
+                             or_mesh **out);
+/* 1 if any of nt triangles (9 doubles each) collides with the rect. */
+int or_tris_rect_collision(const double *tris, long nt, const double *mn, const double *mx);
 /* EncodeSTL of nt triangles (9 doubles each) into out (84 + 50 nt bytes). */
 void or_encode_stl(const double *tris, long nt, uint8_t *out);
 long or_mesh_ntris(const or_mesh *m);

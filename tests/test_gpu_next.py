@@ -1,7 +1,8 @@
 """GPU parity of the entry points beside MarchingCubesSearch that reuse its
 kernels -- MarchingCubesInterior (mc.go:55-60), MarchingCubesConj
-(mc.go:66-71) -- and of the binary STL encoder (export.go:20-66), through
-the C-ABI, bit-exact against the oracle."""
+(mc.go:66-71), MarchingCubesFilter / SearchFilter / C2F (mc.go:73-180) --
+and of the binary STL encoder (export.go:20-66), through the C-ABI,
+bit-exact against the oracle."""
 import ctypes
 
 import numpy as np
@@ -9,7 +10,15 @@ import pytest
 
 import model3d_amd as m3
 from model3d_amd import _lib
-from oracle.binding import oracle_conj, oracle_contains, oracle_encode_stl, oracle_search
+from oracle.binding import (
+    oracle_c2f,
+    oracle_conj,
+    oracle_contains,
+    oracle_encode_stl,
+    oracle_rect_collider,
+    oracle_search,
+    oracle_search_filter,
+)
 from tests.helpers import canonical, same_triangles, solid_catalogue
 
 pytestmark = pytest.mark.gpu
@@ -172,3 +181,63 @@ def test_stl_rejects_bad_index():
     v = np.zeros((3, 3))
     with pytest.raises(_lib.MC3DError):
         m3.EncodeSTL(m3.Mesh(v, np.array([[0, 1, 3]], dtype=np.uint32)))
+
+
+def _hash_filter(mn, mx):
+    """A deterministic, spatially irregular block filter."""
+    h = int(abs(mn[0] * 7919 + mn[1] * 104729 + mn[2] * 1299709 + mx[0] * 31 + mx[1] * 17 + mx[2] * 13) * 1e6)
+    return h % 5 != 0
+
+
+FILTERS = {
+    "half_space": lambda mn, mx: mn[0] < 0.1,
+    "slab": lambda mn, mx: mx[2] > -0.3 and mn[2] < 0.4,
+    "irregular": _hash_filter,
+}
+
+
+@pytest.mark.parametrize("fname", list(FILTERS))
+@pytest.mark.parametrize("name", ["sphere", "csg", "torus", "rotated"])
+@pytest.mark.parametrize("iters", [0, 5])
+def test_search_filter_parity(fname, name, iters):
+    s = solid_catalogue()[name]
+    f = FILTERS[fname]
+    print(f"filter {name} {fname} iters={iters}", flush=True)
+    got = m3.MarchingCubesSearchFilter(s, lambda r: f(r.MinVal, r.MaxVal), 0.04, iters, precision=FP64)
+    want = oracle_search_filter(s, f, 0.04, iters, threads=8)
+    assert same_triangles(got.triangles(), want)
+    # every vertex is used by a triangle (no vertex of a dropped cube)
+    assert len(np.unique(got.faces)) == len(got.vertices)
+
+
+def test_filter_with_native_collider_and_culling_shapes():
+    """MarchingCubesFilter(solid, collider.RectCollision, delta) with the
+    collider's test run natively; the same with culling off and brick 2."""
+    mesh_tris, _ = oracle_search(m3.Sphere((0.1, 0.3, -0.2), 1.0), 0.15, 3, threads=8)
+    collider = m3.MeshToCollider(mesh_tris)
+    solid = m3.NewColliderSolid(collider)
+    want = oracle_search_filter(solid, oracle_rect_collider(mesh_tris), 0.1, 0, threads=8)
+    base, _ = oracle_search(solid, 0.1, 0, threads=8)
+    assert same_triangles(want, base)  # TestMarchingCubesFilter, mc_test.go:44-67
+    for kw in (dict(), dict(cull=False)):
+        got = m3.MarchingCubesFilter(solid, collider.RectCollision, 0.1, **kw)
+        assert same_triangles(got.triangles(), want)
+
+
+@pytest.mark.parametrize("name", ["sphere", "csg", "collider_boxes"])
+@pytest.mark.parametrize("precision", [FP64, GUARD])
+def test_c2f_parity(name, precision):
+    s = solid_catalogue()[name]
+    print(f"c2f {name} precision={precision}", flush=True)
+    got = m3.MarchingCubesC2F(s, 0.1, 0.05, 0.0, 6, precision=precision)
+    want = oracle_c2f(s, 0.1, 0.05, 0.0, 6, threads=8)
+    assert same_triangles(got.triangles(), want)
+
+
+def test_c2f_equals_search_and_rejects_small_big_delta():
+    s = m3.Sphere((0.1, 0.3, -0.2), 1.0)
+    a = m3.MarchingCubesSearch(s, 0.05, 8)
+    b = m3.MarchingCubesC2F(s, 0.1, 0.05, 0, 8)  # TestMarchingCubesC2F, mc_test.go:69-94
+    assert np.array_equal(canonical(a.triangles()), canonical(b.triangles()))
+    with pytest.raises(_lib.MC3DError, match="bigDelta should be >= smallDelta"):
+        m3.MarchingCubesC2F(s, 0.05, 0.1, 0, 8)

@@ -21,10 +21,13 @@ from oracle.binding import (
     oracle_case_table,
     oracle_conj,
     oracle_contains,
+    oracle_c2f,
     oracle_encode_stl,
     oracle_inverse_apply,
+    oracle_rect_collider,
     oracle_rotations,
     oracle_search,
+    oracle_search_filter,
     oracle_spacer,
 )
 from tests.helpers import canonical, edge_topology, solid_catalogue, volume
@@ -544,3 +547,87 @@ def test_stl_round_trip_and_normals():
     assert (np.abs(np.sum(nn * e1, axis=1)) <= 1e-5 * scale).all()
     assert (np.abs(np.sum(nn * e2, axis=1)) <= 1e-5 * scale).all()
     assert (np.sum(nn * cr, axis=1) > 0).all()
+
+
+# ---------------------------------------------------------------------------
+# MarchingCubesFilter / SearchFilter / C2F (mc.go:73-180)
+# ---------------------------------------------------------------------------
+def test_rect_collision_closed_form():
+    """Triangle.RectCollision over a cube's surface (NewMeshRect): a rect
+    crossing a face collides, one strictly inside or outside does not, one
+    enclosing the cube does (the triangles' edges lie in it)."""
+    f = oracle_rect_collider(m3.NewMeshRect((0, 0, 0), (1, 1, 1)).triangles())
+    assert f((0.5, 0.5, -0.1), (0.6, 0.6, 0.1))
+    assert f((0.9, 0.2, 0.2), (1.1, 0.3, 0.3))
+    assert not f((0.4, 0.4, 0.4), (0.6, 0.6, 0.6))
+    assert not f((1.1, 1.1, 1.1), (2, 2, 2))
+    assert f((-1, -1, -1), (2, 2, 2))
+    # a rect edge piercing a face, no triangle edge inside the rect
+    assert f((0.45, 0.45, -0.5), (0.55, 0.55, 0.5))
+
+
+def test_library_rect_collider_matches_oracle():
+    """The library's rect collider (a bounding-volume tree, host code) and the
+    oracle's brute force agree on random rects around a sphere mesh."""
+    tris, _ = oracle_search(m3.Sphere((0.1, 0, 0), 0.8), 0.1, 4, threads=8)
+    lib_c = m3.MeshToCollider(tris)
+    brute = oracle_rect_collider(tris)
+    rng = np.random.default_rng(4)
+    hits = 0
+    for _ in range(3000):
+        c = rng.uniform(-1.2, 1.2, size=3)
+        h = rng.uniform(0.0, 0.3, size=3)
+        mn, mx = tuple(c - h), tuple(c + h)
+        want = brute(mn, mx)
+        hits += want
+        assert lib_c.RectCollision(m3.Rect(mn, mx)) == want
+    assert 300 < hits < 2700
+
+
+def test_filter_always_true_is_marching_cubes():
+    s = solid_catalogue()["csg"]
+    for it in (0, 3):
+        base, _ = oracle_search(s, 0.05, it, threads=8)
+        got = oracle_search_filter(s, lambda mn, mx: True, 0.05, it)
+        assert np.array_equal(canonical(got), canonical(base))
+    assert len(oracle_search_filter(s, lambda mn, mx: False, 0.05, 3)) == 0
+
+
+def test_filter_collider_solid_equals_mc():
+    """TestMarchingCubesFilter (mc_test.go:44-67): a collider solid filtered
+    by its own collider's RectCollision equals plain MarchingCubes."""
+    sphere_mesh, _ = oracle_search(m3.Sphere((0.1, 0.3, -0.2), 1.0), 0.15, 3, threads=8)
+    for mesh in (sphere_mesh, np.concatenate([m3.NewMeshRect((-1, -1, -1), (0, 0, 0)).triangles().reshape(-1, 9),
+                                               m3.NewMeshRect((0.1, 0, 0), (1, 1, 1)).triangles().reshape(-1, 9)])):
+        solid = m3.NewColliderSolid(m3.MeshToCollider(mesh))
+        base, _ = oracle_search(solid, 0.1, 0, threads=8)
+        got = oracle_search_filter(solid, oracle_rect_collider(mesh), 0.1, 0)
+        assert np.array_equal(canonical(got), canonical(base))
+
+
+def test_filter_half_space_keeps_only_its_cubes():
+    """A filter accepting blocks with min x < 0: every triangle lies in a cube
+    whose low x corner is below the largest accepted block edge, and the
+    triangles are a subset of the unfiltered ones."""
+    s = m3.Sphere((0, 0, 0), 1.0)
+    base, _ = oracle_search(s, 0.1, 0, threads=8)
+    got = oracle_search_filter(s, lambda mn, mx: mn[0] < 0, 0.1, 0)
+    assert 0 < len(got) < len(base)
+    bset = {tuple(r) for r in canonical(base).tolist()}
+    assert all(tuple(r) in bset for r in canonical(got).tolist())
+    assert got.reshape(-1, 3)[:, 0].min() < -0.9
+
+
+def test_c2f_equals_search():
+    """TestMarchingCubesC2F (mc_test.go:69-94)."""
+    s = m3.Sphere((0.1, 0.3, -0.2), 1.0)
+    a, _ = oracle_search(s, 0.05, 8, threads=8)
+    assert np.array_equal(canonical(oracle_c2f(s, 0.1, 0.05, 0, 8)), canonical(a))
+    mesh = np.concatenate([m3.NewMeshRect((-1, -1, -1), (0, 0, 0)).triangles().reshape(-1, 9),
+                           m3.NewMeshRect((0.3, 0, 0), (1, 1, 1)).triangles().reshape(-1, 9)])
+    boxes = m3.NewColliderSolid(m3.MeshToCollider(mesh))
+    a, _ = oracle_search(boxes, 0.1, 8, threads=8)
+    b = oracle_c2f(boxes, 0.2, 0.1, 0, 8)
+    assert np.array_equal(canonical(b), canonical(a))
+    ok, _ = edge_topology(b)
+    assert ok
