@@ -331,6 +331,97 @@ def copy_calibration():
         raise SystemExit("copy calibration failed")
 
 
+def next_rows(work, run, steps=5):
+    """The entry points next to MarchingCubesSearch on one solid of the
+    workload, results left on the device, ms per call (after one warm-up):
+    MarchingCubesInterior, MarchingCubesConj (a rotation and a translation),
+    MarchingCubesSearchFilter with the native rect collider of the solid's
+    mesh at 4x delta, MarchingCubesC2F (4x delta to delta) and the STL
+    encoding of the mesh (from its device buffers)."""
+    import numpy as np
+
+    from model3d_amd import _lib
+    from model3d_amd.solids import LibBackend, Rotation, Translate
+
+    L = _lib.lib()
+    ctx = _lib.context(0)
+    b = LibBackend()
+    solid, delta, iters = work["solids"][0], work["deltas"][0], work["iters"][0]
+    h = solid._build(b)
+    o = _lib.mc3d_options()
+    o.precision, o.cull, o.device_only = precision_code(run["precision"]), 1 if run["cull"] else 0, 1
+    o.brick, o.rows_per_walk, o.waves_per_brick = run["brick"], run["rows_per_walk"], run["waves_per_brick"]
+    out = {}
+
+    def timed(name, call, extra=None):
+        ms, st = [], None
+        for i in range(steps + 1):
+            m = ctypes.c_void_p()
+            t0 = time.perf_counter()
+            _lib.check(call(m), name)
+            dt = time.perf_counter() - t0
+            if i:
+                ms.append(1e3 * dt)
+            if st is None:
+                s_ = _lib.mc3d_stats()
+                L.mc3d_mesh_stats(m, ctypes.byref(s_))
+                st = s_.as_dict()
+            if extra and i == steps:
+                extra(m, st)
+            L.mc3d_mesh_free(m)
+        r = {"ms_per_call": sum(ms) / len(ms), "corners": st["corners"], "triangles": st["triangles"],
+             "corner_evals_per_s": st["corners"] / (sum(ms) / len(ms) / 1e3), "ms_device": st["ms_device"]}
+        out[name] = r
+        return r
+
+    try:
+        timed("interior", lambda m: L.mc3d_marching_cubes_interior(ctx, h, delta, iters, ctypes.byref(o), ctypes.byref(m)))
+        xf = [op for t in (Rotation((0.0, 0.6, 0.8), 0.5), Translate((0.1, -0.2, 0.05))) for op in t._ops()]
+        arr = (_lib.mc3d_xform * len(xf))()
+        for i, (k, v) in enumerate(xf):
+            arr[i].kind = k
+            for j in range(9):
+                arr[i].v[j] = v[j]
+        timed("conj", lambda m: L.mc3d_marching_cubes_conj(ctx, h, delta, iters, arr, len(xf), ctypes.byref(o),
+                                                            ctypes.byref(m)))
+        # the solid's own mesh at 4x delta as the filter's collider
+        coarse = ctypes.c_void_p()
+        _lib.check(L.mc3d_marching_cubes_search(ctx, h, 4 * delta, iters, ctypes.byref(o), ctypes.byref(coarse)), "coarse")
+        nv, nt = ctypes.c_size_t(), ctypes.c_size_t()
+        L.mc3d_mesh_counts(coarse, ctypes.byref(nv), ctypes.byref(nt))
+        v = np.empty((nv.value, 3))
+        f = np.empty((nt.value, 3), dtype=np.uint32)
+        L.mc3d_mesh_copy(coarse, v.ctypes.data, f.ctypes.data)
+        L.mc3d_mesh_free(coarse)
+        tris = np.ascontiguousarray(v[f.astype(np.int64)].reshape(-1, 9))
+        col = L.mc3d_rect_collider_create(tris.ctypes.data_as(_lib._dp), len(tris))
+        fn = ctypes.cast(L.mc3d_rect_collider_collides, ctypes.c_void_p)
+        r = timed("search_filter", lambda m: L.mc3d_marching_cubes_search_filter(ctx, h, fn, col, delta, iters,
+                                                                                 ctypes.byref(o), ctypes.byref(m)))
+        r["collider_triangles"] = len(tris)
+        L.mc3d_rect_collider_free(col)
+        timed("c2f", lambda m: L.mc3d_marching_cubes_c2f(ctx, h, 4 * delta, delta, 0.0, iters, ctypes.byref(o),
+                                                         ctypes.byref(m)))
+
+        def stl(m, st):
+            n = L.mc3d_stl_bytes(st["triangles"])
+            buf = np.empty(n, dtype=np.uint8)
+            t = []
+            for _ in range(steps):
+                t0 = time.perf_counter()
+                _lib.check(L.mc3d_mesh_encode_stl(ctx, m, buf.ctypes.data, n), "stl")
+                t.append(1e3 * (time.perf_counter() - t0))
+            out["stl"] = {"ms_per_encode": sum(t) / len(t), "bytes": n,
+                          "GB_per_s_to_host": n / (sum(t) / len(t) / 1e3) / 1e9}
+
+        timed("search", lambda m: L.mc3d_marching_cubes_search(ctx, h, delta, iters, ctypes.byref(o), ctypes.byref(m)),
+              extra=stl)
+    finally:
+        b.release(h)
+        b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -475,6 +566,9 @@ def main():
             walle, _ = run_gpu(work, run, n64, 1, each=True)
             result["one_call_per_solid"] = {"ms_per_step": 1e3 * walle / n64,
                                             "speedup_of_batch": (1e3 * walle / n64) / ms}
+
+    if nsolids == 1 and args.workload in ("csg64", "sphere"):
+        result["next_rows"] = next_rows(work, run)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(work)

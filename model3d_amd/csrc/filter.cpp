@@ -268,11 +268,23 @@ int filter_keep_mask(const std::vector<double> ax[3], double delta,
     const long divide = std::max(root.volume() / 4096, sub_volume);  // mc.go:128
     std::vector<Block> queue;
     R.pieces(root, divide, [&](const Block &b) { queue.push_back(b); });
-    // each queued block split further (mc.go:105-107), on `threads` threads
-    std::vector<std::vector<Block>> leaves(queue.size());
+    // each queued block split further (mc.go:105-107), on `threads` threads;
+    // the cubes of every accepted leaf become bits of the occupancy word
+    // layout (atomic ORs: leaves of different blocks can share a word)
+    auto set_cubes = [&](const Block &b) {
+        for (int z = b.mn[2]; z < b.mx[2]; z++)
+            for (int y = b.mn[1]; y < b.mx[1]; y++) {
+                uint64_t *row = &keep[((size_t)z * ny + y) * W];
+                for (int x = b.mn[0]; x < b.mx[0];) {
+                    const int w = x >> 6, lo = x & 63, hi = std::min(64, b.mx[0] - (w << 6));
+                    const uint64_t m = (hi >= 64 ? ~0ull : ((1ull << hi) - 1)) & (~0ull << lo);
+                    __atomic_fetch_or(&row[w], m, __ATOMIC_RELAXED);
+                    x = (w << 6) + hi;
+                }
+            }
+    };
     auto work = [&](size_t i0, size_t step) {
-        for (size_t i = i0; i < queue.size(); i += step)
-            R.pieces(queue[i], sub_volume, [&](const Block &b) { leaves[i].push_back(b); });
+        for (size_t i = i0; i < queue.size(); i += step) R.pieces(queue[i], sub_volume, set_cubes);
     };
     const size_t nt = std::max<size_t>(1, std::min<size_t>((size_t)threads, queue.size()));
     if (nt == 1) {
@@ -282,19 +294,6 @@ int filter_keep_mask(const std::vector<double> ax[3], double delta,
         for (size_t t = 0; t < nt; t++) th.emplace_back(work, t, nt);
         for (auto &t : th) t.join();
     }
-    // the cubes of every accepted leaf, as bits of the occupancy word layout
-    for (const auto &ls : leaves)
-        for (const Block &b : ls)
-            for (int z = b.mn[2]; z < b.mx[2]; z++)
-                for (int y = b.mn[1]; y < b.mx[1]; y++) {
-                    uint64_t *row = &keep[((size_t)z * ny + y) * W];
-                    for (int x = b.mn[0]; x < b.mx[0];) {
-                        const int w = x >> 6, lo = x & 63, hi = std::min(64, b.mx[0] - (w << 6));
-                        const uint64_t m = (hi >= 64 ? ~0ull : ((1ull << hi) - 1)) & (~0ull << lo);
-                        row[w] |= m;
-                        x = (w << 6) + hi;
-                    }
-                }
     return MC3D_OK;
 }
 

@@ -1071,7 +1071,10 @@ int mc3d_marching_cubes_search_filter(mc3d_ctx *ctx, const mc3d_solid *s, mc3d_r
         return MC3D_E_INVALID_ARG;
     }
     const RectFilter filter = [f, user](const double *mn, const double *mx) { return f(user, mn, mx) != 0; };
-    return search_one(ctx, s, delta, iters, opts, out, false, {}, &filter, 1);
+    // a caller's filter is asked on this thread; the library's own rect
+    // collider is thread-safe and runs on 16
+    const int threads = f == mc3d_rect_collider_collides ? 16 : 1;
+    return search_one(ctx, s, delta, iters, opts, out, false, {}, &filter, threads);
 }
 
 struct mc3d_rect_collider {
