@@ -173,8 +173,8 @@ __device__ __forceinline__ bool atom64(uint32_t kind, F64P a, double x, double y
     }
     case A_RECT:  // Rect.Contains, shapes.go:171-173
         // all six tests (no short circuit): the params load together
-        return in_lo(x, a[0]) & in_lo(y, a[1]) & in_lo(z, a[2]) & in_hi(x, a[3]) & in_hi(y, a[4]) &
-               in_hi(z, a[5]);
+        return ((int)in_lo(x, a[0]) & (int)in_lo(y, a[1]) & (int)in_lo(z, a[2]) & (int)in_hi(x, a[3]) &
+                (int)in_hi(y, a[4]) & (int)in_hi(z, a[5])) != 0;
     case A_CAPSULE: {  // Segment.Dist(c) <= r, primitives.go:568-593
         double v2x = x - a[0], v2y = y - a[1], v2z = z - a[2];
         double mag = a[6] * v2x + a[7] * v2y + a[8] * v2z;

@@ -558,6 +558,56 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
 }
 
 // ---------------------------------------------------------------------------
+// exclusive scan of uint4 counts (block sums written by classify, a scan of
+// those, downsweep)
+// ---------------------------------------------------------------------------
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+constexpr int CLASSIFY_BLOCK = 1024;  // words per classify block and per block sum
+constexpr int SUMS_BLOCK = 1024;      // threads of the one block scanning the block sums
+static_assert(SCAN_TILE % CLASSIFY_BLOCK == 0, "a scan tile starts at a classify block");
+
+__device__ __forceinline__ uint4 add4(uint4 a, uint4 b) { return make_uint4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+
+__device__ __forceinline__ uint4 sub4(uint4 a, uint4 b) { return make_uint4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w); }
+
+// Exclusive scan over the block: an inclusive scan in each wave (shuffles),
+// then the wave totals through LDS (sh: NT / 64 entries).
+template <int NT = SCAN_BLOCK>
+__device__ __forceinline__ uint4 block_exclusive(uint4 v, uint4 *sh, uint4 &total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint4 inc = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint4 o = make_uint4(__shfl_up(inc.x, d), __shfl_up(inc.y, d), __shfl_up(inc.z, d), __shfl_up(inc.w, d));
+        if (lane >= d) inc = add4(inc, o);
+    }
+    if (lane == 63) sh[wave] = inc;
+    __syncthreads();
+    uint4 pre = make_uint4(0, 0, 0, 0);
+    total = make_uint4(0, 0, 0, 0);
+    for (int w = 0; w < NT / 64; w++) {
+        const uint4 t = sh[w];
+        if (w < wave) pre = add4(pre, t);
+        total = add4(total, t);
+    }
+    __syncthreads();  // sh is reused by the caller's next scan
+    return add4(pre, sub4(inc, v));
+}
+
+// Sum over the block (sh: NT / 64 entries); every thread gets it.
+template <int NT>
+__device__ __forceinline__ uint4 block_sum(uint4 v, uint4 *sh) {
+    for (int d = 32; d; d >>= 1)
+        v = add4(v, make_uint4(__shfl_xor(v.x, d), __shfl_xor(v.y, d), __shfl_xor(v.z, d), __shfl_xor(v.w, d)));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint4 t = make_uint4(0, 0, 0, 0);
+    for (int w = 0; w < NT / 64; w++) t = add4(t, sh[w]);
+    return t;
+}
+
+// ---------------------------------------------------------------------------
 // classify
 // ---------------------------------------------------------------------------
 struct Masks {
@@ -644,109 +694,90 @@ __device__ __forceinline__ int cube_case(const CubeCorners &cc, int i) {
     return c;
 }
 
-// counts[w] = {crossed edges, triangles, 1 if the word emits anything, 0}.
+// counts[w] = {crossed edges, triangles, 1 if the word emits anything, 0};
+// a word per thread, and per block (CLASSIFY_BLOCK words) the block's sum for
+// the scan: part_sums[blockIdx.x].
 template <bool MULTI>
-__global__ __launch_bounds__(256) void k_classify(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
-                                                  const uint32_t *__restrict__ word_seg,
-                                                  const uint64_t *__restrict__ bits_all,
-                                                  const uint8_t *__restrict__ table, uint4 *__restrict__ counts,
-                                                  const uint64_t *__restrict__ keep_all, uint64_t nwords_all) {
+__global__ __launch_bounds__(CLASSIFY_BLOCK) void k_classify(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
+                                                         const uint32_t *__restrict__ word_seg,
+                                                         const uint64_t *__restrict__ bits_all,
+                                                         const uint8_t *__restrict__ table, uint4 *__restrict__ counts,
+                                                         uint4 *__restrict__ part_sums,
+                                                         const uint64_t *__restrict__ keep_all, uint64_t nwords_all) {
     __shared__ uint8_t ntri[256];
-    ntri[threadIdx.x] = table[16 * threadIdx.x];
+    __shared__ uint4 sh[CLASSIFY_BLOCK / 64];
+    if (threadIdx.x < 256) ntri[threadIdx.x] = table[16 * threadIdx.x];
     __syncthreads();
-    for (uint64_t gw = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gw < nwords_all;
-         gw += (uint64_t)gridDim.x * blockDim.x) {
+    uint4 c = make_uint4(0, 0, 0, 0);
+    // consecutive words per wave (segment ranges are aligned to 64)
+    const uint64_t gw = (uint64_t)blockIdx.x * CLASSIFY_BLOCK + threadIdx.x;
+    if (gw < nwords_all) {  // wave-uniform: nwords_all is a multiple of 64
         const uint32_t si = wave_seg<MULTI>(word_seg, gw, SEG_WORD_SHIFT);
         const Seg sg = pick_seg<MULTI>(segs, seg0, si);
         const Grid &g = sg.g;
         const uint64_t *bits = bits_all + sg.word0;
         const uint64_t w = gw - sg.word0;
-        if (w >= sg.nwords) {  // alignment padding
-            counts[gw] = make_uint4(0, 0, 0, 0);
-            continue;
+        if (w < sg.nwords) {  // else alignment padding
+            int wx, y, z;
+            word_coords(g, w, wx, y, z);
+            const Nbhd h = load_nbhd(g, bits, wx, y, z);
+            Masks m = nbhd_masks(h);
+            const CubeCorners cc = nbhd_corners(h);
+            uint64_t mixed = mixed_cubes(cc);
+            if (keep_all) {  // a block filter: only the cubes of accepted blocks
+                const uint64_t *keep = keep_all + sg.word0;
+                keep_masks(g, keep, wx, y, z, m);
+                mixed &= keep[w];
+            }
+            const uint32_t nv = __popcll(m.cx) + __popcll(m.cy) + __popcll(m.cz);
+            uint32_t nt = 0;
+            while (mixed) {
+                const int i = __ffsll((unsigned long long)mixed) - 1;
+                mixed &= mixed - 1;
+                nt += ntri[cube_case(cc, i)];
+            }
+            c = make_uint4(nv, nt, (nv | nt) != 0 ? 1u : 0u, 0u);
         }
-        int wx, y, z;
-        word_coords(g, w, wx, y, z);
-        const Nbhd h = load_nbhd(g, bits, wx, y, z);
-        Masks m = nbhd_masks(h);
-        const CubeCorners cc = nbhd_corners(h);
-        uint64_t mixed = mixed_cubes(cc);
-        if (keep_all) {  // a block filter: only the cubes of accepted blocks
-            const uint64_t *keep = keep_all + sg.word0;
-            keep_masks(g, keep, wx, y, z, m);
-            mixed &= keep[w];
-        }
-        const uint32_t nv = __popcll(m.cx) + __popcll(m.cy) + __popcll(m.cz);
-        uint32_t nt = 0;
-        while (mixed) {
-            const int i = __ffsll((unsigned long long)mixed) - 1;
-            mixed &= mixed - 1;
-            nt += ntri[cube_case(cc, i)];
-        }
-        counts[gw] = make_uint4(nv, nt, (nv | nt) != 0 ? 1u : 0u, 0u);
+        counts[gw] = c;
     }
+    const uint4 t = block_sum<CLASSIFY_BLOCK>(c, sh);
+    if (threadIdx.x == 0) part_sums[blockIdx.x] = t;
 }
 
-// ---------------------------------------------------------------------------
-// exclusive scan of uint4 counts (reduce tiles, scan tile sums, downsweep)
-// ---------------------------------------------------------------------------
-constexpr int SCAN_BLOCK = 256;
-constexpr int SCAN_ITEMS = 8;
-constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
-
-__device__ __forceinline__ uint4 add4(uint4 a, uint4 b) { return make_uint4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
-
-__device__ __forceinline__ uint4 block_exclusive(uint4 v, uint4 *sh, uint4 &total) {
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
-        const uint4 o = t >= off ? sh[t - off] : make_uint4(0, 0, 0, 0);
-        __syncthreads();
-        sh[t] = add4(sh[t], o);
-        __syncthreads();
-    }
-    total = sh[SCAN_BLOCK - 1];
-    const uint4 r = t > 0 ? sh[t - 1] : make_uint4(0, 0, 0, 0);
-    __syncthreads();
-    return r;
-}
-
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint4 *__restrict__ in, uint64_t n,
-                                                            uint4 *__restrict__ tile_sums) {
-    __shared__ uint4 sh[SCAN_BLOCK];
-    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
-    uint4 s = make_uint4(0, 0, 0, 0);
-    for (int i = 0; i < SCAN_ITEMS; i++)
-        if (base + i < n) s = add4(s, in[base + i]);
-    uint4 total;
-    block_exclusive(s, sh, total);
-    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
-}
-
-// One block: exclusive scan of the tile sums in place; totals in 64 bits.
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(uint4 *__restrict__ tile_sums, uint64_t ntiles,
+// One block: exclusive scan of the classify blocks' sums in place, SCAN_ITEMS
+// per thread; totals in 64 bits.
+__global__ __launch_bounds__(SUMS_BLOCK) void k_scan_tiles(uint4 *__restrict__ part_sums, uint64_t nparts,
                                                            unsigned long long *__restrict__ totals) {
-    __shared__ uint4 sh[SCAN_BLOCK];
+    __shared__ uint4 sh[SUMS_BLOCK / 64];
     uint4 carry = make_uint4(0, 0, 0, 0);
     unsigned long long t0 = 0, t1 = 0, t2 = 0;
-    for (uint64_t start = 0; start < ntiles; start += SCAN_BLOCK) {
-        const uint64_t i = start + threadIdx.x;
-        const uint4 v = i < ntiles ? tile_sums[i] : make_uint4(0, 0, 0, 0);
+    for (uint64_t start = 0; start < nparts; start += (uint64_t)SUMS_BLOCK * SCAN_ITEMS) {
+        const uint64_t base = start + (uint64_t)threadIdx.x * SCAN_ITEMS;
+        uint4 vals[SCAN_ITEMS];
+        uint4 s = make_uint4(0, 0, 0, 0);
+        for (int i = 0; i < SCAN_ITEMS; i++) {
+            vals[i] = base + i < nparts ? part_sums[base + i] : make_uint4(0, 0, 0, 0);
+            s = add4(s, vals[i]);
+        }
         uint4 total;
-        const uint4 ex = block_exclusive(v, sh, total);
-        if (i < ntiles) tile_sums[i] = add4(ex, carry);
+        uint4 ex = add4(block_exclusive<SUMS_BLOCK>(s, sh, total), carry);
+        for (int i = 0; i < SCAN_ITEMS; i++) {
+            if (base + i < nparts) part_sums[base + i] = ex;
+            ex = add4(ex, vals[i]);
+        }
         carry = add4(carry, total);
         t0 += total.x, t1 += total.y, t2 += total.z;
     }
     if (threadIdx.x == 0) totals[0] = t0, totals[1] = t1, totals[2] = t2;
 }
 
-// Downsweep; words with .z set are also listed at their scanned .z.
+// Downsweep over tiles of SCAN_TILE words (the exclusive scan of the
+// classify blocks' sums gives each tile's offset); words with .z set are also
+// listed at their scanned .z.
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint4 *__restrict__ in, uint64_t n,
-                                                          const uint4 *__restrict__ tile_sums,
+                                                          const uint4 *__restrict__ part_sums,
                                                           uint4 *__restrict__ out, uint32_t *__restrict__ list) {
-    __shared__ uint4 sh[SCAN_BLOCK];
+    __shared__ uint4 sh[SCAN_BLOCK / 64];
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     uint4 vals[SCAN_ITEMS];
     uint4 s = make_uint4(0, 0, 0, 0);
@@ -755,7 +786,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint4 *__restric
         s = add4(s, vals[i]);
     }
     uint4 total;
-    uint4 ex = add4(block_exclusive(s, sh, total), tile_sums[blockIdx.x]);
+    uint4 ex = add4(block_exclusive(s, sh, total), part_sums[(uint64_t)blockIdx.x * (SCAN_TILE / CLASSIFY_BLOCK)]);
     for (int i = 0; i < SCAN_ITEMS; i++) {
         if (base + i < n) {
             out[base + i] = ex;

@@ -527,12 +527,11 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+// The scan of the counts whose per-block sums k_classify wrote.
 int scan4(mc3d_ctx *ctx, const uint4 *in, uint4 *out, uint64_t n, unsigned long long *totals, uint32_t *list) {
     const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-    int rc;
-    if ((rc = ctx->tiles.ensure(ntiles * sizeof(uint4)))) return rc;
-    k_scan_reduce<<<(unsigned)ntiles, SCAN_BLOCK, 0, ctx->stream>>>(in, n, ctx->tiles.as<uint4>());
-    k_scan_tiles<<<1, SCAN_BLOCK, 0, ctx->stream>>>(ctx->tiles.as<uint4>(), ntiles, totals);
+    k_scan_tiles<<<1, SUMS_BLOCK, 0, ctx->stream>>>(ctx->tiles.as<uint4>(), (n + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK,
+                                                    totals);
     k_scan_down<<<(unsigned)ntiles, SCAN_BLOCK, 0, ctx->stream>>>(in, n, ctx->tiles.as<uint4>(), out, list);
     HIP_TRY(hipGetLastError());
     return MC3D_OK;
@@ -777,9 +776,11 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
         set_error("too many culling bricks");
         return MC3D_E_UNSUPPORTED;
     }
+    const unsigned ctiles = (unsigned)((c.nwords + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK);  // classify blocks
     int rc;
     if ((rc = ctx->bits.ensure(c.nwords * 8)) || (rc = ctx->counts.ensure(c.nwords * sizeof(uint4))) ||
-        (rc = ctx->offs.ensure(c.nwords * sizeof(uint4))) || (rc = ctx->actw.ensure(c.nwords * sizeof(uint32_t))))
+        (rc = ctx->offs.ensure(c.nwords * sizeof(uint4))) || (rc = ctx->actw.ensure(c.nwords * sizeof(uint32_t))) ||
+        (rc = ctx->tiles.ensure((uint64_t)ctiles * sizeof(uint4))))
         return rc;
     if (c.cull && ((rc = ctx->binfo.ensure(c.nbricks * sizeof(uint2))) ||
                    (rc = ctx->actb.ensure(c.nbricks * sizeof(uint32_t))) ||
@@ -833,13 +834,13 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
 
     // ---- classify + scan (+ list of words that emit) ----
     if (c.nsegs > 1)
-        k_classify<true><<<grid_for(c.nwords, 256), 256, 0, ctx->stream>>>(
+        k_classify<true><<<ctiles, CLASSIFY_BLOCK, 0, ctx->stream>>>(
             U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
-            ctx->counts.as<uint4>(), nullptr, c.nwords);
+            ctx->counts.as<uint4>(), ctx->tiles.as<uint4>(), nullptr, c.nwords);
     else
-        k_classify<false><<<grid_for(c.nwords, 256), 256, 0, ctx->stream>>>(
+        k_classify<false><<<ctiles, CLASSIFY_BLOCK, 0, ctx->stream>>>(
             U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
-            ctx->counts.as<uint4>(), c.keep, c.nwords);
+            ctx->counts.as<uint4>(), ctx->tiles.as<uint4>(), c.keep, c.nwords);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[8], ctx->stream));
     if ((rc = scan4(ctx, ctx->counts.as<uint4>(), ctx->offs.as<uint4>(), c.nwords, U.ctl->totals,
