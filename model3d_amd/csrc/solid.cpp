@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <initializer_list>
 #include <mutex>
 
 #include "mc3d_internal.h"
@@ -226,7 +227,34 @@ struct Solid {
     double mn[3] = {0}, mx[3] = {0};  // stored bounds (funcSolid / ColliderSolid)
     std::shared_ptr<ColliderData> collider;
     HostAtom host{nullptr, nullptr};
+    // Stack entries the node's code needs, emitting the needier operand
+    // first (set when the node is made; its children are made before it).
+    int need = 1;
 };
+
+// Solid::need from the children's.
+static int node_need(const Solid &s) {
+    switch (s.kind) {
+    case K_JOINED:
+    case K_INTERSECTED: {
+        // operands in decreasing need; each after the first sits on one
+        // accumulated entry: max(n0, n1 + 1) over the two largest
+        int n0 = 0, n1 = -1;
+        for (auto &c : s.children) {
+            if (c->need > n0) n1 = n0, n0 = c->need;
+            else if (c->need > n1) n1 = c->need;
+        }
+        return s.children.size() > 1 ? std::max(n0, n1 + 1) : n0;
+    }
+    case K_SUBTRACTED: {
+        const int a = s.children[0]->need, b = s.children[1]->need;
+        return a == b ? a + 1 : std::max(a, b);
+    }
+    case K_TRANSFORM:
+    case K_FORCEBOUNDS: return std::max(s.children[0]->need, 2);
+    default: return 1;
+    }
+}
 
 static void min_max(const Solid &s, V3 &mn, V3 &mx) {
     const double *p = s.p;
@@ -571,30 +599,11 @@ struct Lowerer {
     std::vector<std::vector<Xf>> levels;  // forward transforms, outermost first
     std::vector<int> frame_of_level;      // frame id at each depth
 
-    int need(const Solid &s) {
-        switch (s.kind) {
-        case K_JOINED:
-        case K_INTERSECTED: {
-            std::vector<int> n;
-            for (auto &c : s.children) n.push_back(need(*c));
-            std::sort(n.rbegin(), n.rend());
-            int r = n[0];
-            for (size_t i = 1; i < n.size(); i++) r = std::max(r, n[i] + 1);
-            return r;
-        }
-        case K_SUBTRACTED: {
-            int a = need(*s.children[0]), b = need(*s.children[1]);
-            return a == b ? a + 1 : std::max(a, b);
-        }
-        case K_TRANSFORM:
-        case K_FORCEBOUNDS: return std::max(need(*s.children[0]), 2);
-        default: return 1;
-        }
-    }
+    static int need(const Solid &s) { return s.need; }
 
     uint32_t cur_frame() const { return frame_of_level.back(); }
 
-    void atom(uint32_t kind, const std::vector<double> &params, Box local) {
+    void atom(uint32_t kind, std::initializer_list<double> params, Box local) {
         Insn in;
         in.x = OP_ATOM | (kind << 8) | (cur_frame() << 16);
         in.y = (uint32_t)P.f64.size();
@@ -720,10 +729,15 @@ struct Lowerer {
         }
         case K_JOINED:
         case K_INTERSECTED: {
-            std::vector<std::pair<int, size_t>> order;
-            for (size_t i = 0; i < s.children.size(); i++) order.push_back({-need(*s.children[i]), i});
-            std::stable_sort(order.begin(), order.end());
-            for (size_t k = 0; k < order.size(); k++) {
+            // operands in decreasing need, ties in their order
+            const size_t nc = s.children.size();
+            std::pair<int, size_t> small[8];
+            std::vector<std::pair<int, size_t>> big;
+            std::pair<int, size_t> *order = small;
+            if (nc > 8) big.resize(nc), order = big.data();
+            for (size_t i = 0; i < nc; i++) order[i] = {-need(*s.children[i]), i};
+            std::stable_sort(order, order + nc);
+            for (size_t k = 0; k < nc; k++) {
                 if (!emit(*s.children[order[k].second])) return false;
                 if (k > 0) op(s.kind == K_JOINED ? OP_OR : OP_AND);
             }
@@ -813,6 +827,7 @@ bool lower_solid(const Solid &s, Program &P, std::string &msg) {
     P = Program();
     P.frames.push_back(Frame{0, 0});  // identity frame
     Lowerer L{P, msg, {}, {0}};
+    P.code.reserve(256), P.f64.reserve(2048), P.atom_pc.reserve(128), P.atom_box.reserve(128), P.f32_dir.reserve(256);
     if (!L.emit(s)) return false;
     P.max_stack = max_stack_depth(P);
     if (P.max_stack > 32) {
@@ -850,6 +865,7 @@ using namespace mc3d;
 
 namespace {
 mc3d_solid *wrap(Solid *s) {
+    s->need = node_need(*s);
     mc3d_solid *h = new (std::nothrow) mc3d_solid;
     if (!h) {
         delete s;
