@@ -7,6 +7,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "mc3d_internal.h"
 
 namespace mc3d {
@@ -306,6 +308,24 @@ __device__ __forceinline__ uint32_t atom_nparams(uint32_t kind) {
 }
 constexpr uint32_t MAX_ATOM_PARAMS = 16;
 
+// The N params of one atom copied into vector registers (uniform values): a
+// loop over a brick's rows reads them from there rather than reloading them
+// (scalar registers are too few to hold them across the loop).
+template <class T, int N>
+struct RegParams {
+    T v[N];
+    __device__ __forceinline__ T operator[](int i) const { return v[i]; }
+};
+template <int N, class T, class SrcP>
+__device__ __forceinline__ RegParams<T, N> reg_params(SrcP src) {
+    RegParams<T, N> r;
+    for (int i = 0; i < N; i++) {
+        r.v[i] = src[i];
+        asm volatile("" : "+v"(r.v[i]));  // kept in a vector register
+    }
+    return r;
+}
+
 // One atom at one point, with the guard statistics.  f64/f32 point at the
 // atom's params.
 // HOST: the program may hold host-evaluated atoms, whose answers for point
@@ -315,29 +335,33 @@ template <int MODE, bool STATS, bool HOST, class F64P, class F32P>
 __device__ __forceinline__ bool eval_atom(const DevProg &P, uint32_t kind, F64P f64, F32P f32, double qx, double qy,
                                           double qz, float fx, float fy, float fz, bool live, uint64_t pid,
                                           GStats &gs) {
-    if (kind == A_COLLIDER) {
-        uint32_t cand = 0;
-        const bool b = collider_contains(P.colliders[(int)f64[0]], (const double *)f64, qx, qy, qz,
-                                         STATS ? &cand : nullptr);
-        if (STATS && live) gs.flops64 += atom_flops64(kind) + COLLIDER_FLOPS_PER_CANDIDATE * cand;
-        return b;
-    }
-    if (kind == A_HOST) return HOST && P.host_vals[(uint64_t)f64[0] * P.host_stride + pid] != 0;
-    if (MODE == 1 && has_atom32(kind)) {
-        bool plain;
-        const int r = atom32(kind, f32, fx, fy, fz, &plain);
-        if (STATS && live) gs.evals++, gs.flops32 += atom_flops32(kind);
-        if (r < 0) {
-            const bool b = atom64(kind, f64, qx, qy, qz);
-            if (STATS && live) gs.rechecks++, gs.mismatches += plain != b, gs.flops64 += atom_flops64(kind);
+    if constexpr (std::is_pointer<F64P>::value) {  // params in memory (not RegParams)
+        if (kind == A_COLLIDER) {
+            uint32_t cand = 0;
+            const bool b = collider_contains(P.colliders[(int)f64[0]], (const double *)f64, qx, qy, qz,
+                                             STATS ? &cand : nullptr);
+            if (STATS && live) gs.flops64 += atom_flops64(kind) + COLLIDER_FLOPS_PER_CANDIDATE * cand;
             return b;
         }
-        if (STATS && live) {
-            const bool t = atom64(kind, f64, qx, qy, qz);
-            gs.mismatches += plain != t;
-            gs.violations += (r != 0) != t;
+        if (kind == A_HOST) return HOST && P.host_vals[(uint64_t)f64[0] * P.host_stride + pid] != 0;
+    }
+    if constexpr (MODE == 1) {
+        if (has_atom32(kind)) {
+            bool plain;
+            const int r = atom32(kind, f32, fx, fy, fz, &plain);
+            if (STATS && live) gs.evals++, gs.flops32 += atom_flops32(kind);
+            if (r < 0) {
+                const bool b = atom64(kind, f64, qx, qy, qz);
+                if (STATS && live) gs.rechecks++, gs.mismatches += plain != b, gs.flops64 += atom_flops64(kind);
+                return b;
+            }
+            if (STATS && live) {
+                const bool t = atom64(kind, f64, qx, qy, qz);
+                gs.mismatches += plain != t;
+                gs.violations += (r != 0) != t;
+            }
+            return r != 0;
         }
-        return r != 0;
     }
     if (STATS && live) gs.flops64 += atom_flops64(kind);
     return atom64(kind, f64, qx, qy, qz);
@@ -362,6 +386,21 @@ __device__ __forceinline__ uint32_t frame_flops(F64P f64, Frame fr) {
 // constant-space pointers (scalar_ptr), so the walk branches on scalars;
 // otherwise each lane reads its own.  MODE 0: every atom in float64.  MODE 1:
 // float32 with band, float64 in the band.  `live` only gates the statistics.
+// One non-atom instruction on a bit stack (bit 0 the top; see eval_code_n).
+__device__ __forceinline__ uint32_t stack_op(uint32_t op, uint32_t s) {
+    const uint32_t t = s & 1u;
+    if (op == OP_OR) return (s >> 1) | t;
+    if (op == OP_AND) return (s >> 1) & (~1u | t);
+    if (op == OP_ANDNOT) return (s >> 1) & ~t;  // second & !top
+    if (op == OP_RANDNOT) {                     // !second & top
+        s >>= 1;
+        return (s & ~1u) | (t & ~s & 1u);
+    }
+    if (op == OP_NOT) return s ^ 1u;
+    if (op == OP_TRUE) return (s << 1) | 1u;
+    return s << 1;  // OP_FALSE
+}
+
 template <int MODE, bool STATS, int NP, bool UNI, bool HOST = false, class CodeP, class F64P, class F32P,
           class FrameP>
 __device__ __forceinline__ void eval_code_n(CodeP code, uint32_t n, const DevProg &P, F64P f64, F32P f32,

@@ -440,38 +440,59 @@ __device__ __forceinline__ const Insn *brick_code(const DevProg &P, const Bricks
     return bi.x == BRICK_FULL ? P.code : B.code + bi.x;
 }
 
-// One row of 64 corners decided: the occupancy word and, for bricks on the
-// grid's boundary, the check that no boundary corner is inside
-// (panic("solid is true outside of bounds"), mc.go:687-689).
-__device__ __forceinline__ void row_out(const Grid &g, uint64_t *__restrict__ bits, int bx, int y, int z, int x,
-                                        bool inside, bool edge, Ctl *__restrict__ ctl,
-                                        SegOut *__restrict__ so) {
+// One row of 64 corners decided: the occupancy word (at `word`) and, for
+// bricks on the grid's boundary, the check that no boundary corner is
+// inside (panic("solid is true outside of bounds"), mc.go:687-689); xedge:
+// the lanes on the x faces, redge: the row lies on a y or z face.
+__device__ __forceinline__ void row_out(uint64_t *__restrict__ word, bool inside, bool edge, uint64_t xedge,
+                                        bool redge, Ctl *__restrict__ ctl, SegOut *__restrict__ so) {
     const uint64_t m = __ballot(inside);
-    if ((threadIdx.x & 63) == 0) bits[word_index(g, bx, y, z)] = m;
-    if (edge && inside && (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1))
-        atomicOr(&ctl->inside, 1u), atomicOr(&so->inside, 1u);
+    if ((threadIdx.x & 63) == 0) {
+        *word = m;
+        if (edge && (m & (redge ? ~0ull : xedge))) atomicOr(&ctl->inside, 1u), atomicOr(&so->inside, 1u);
+    }
 }
 
-// The rows of a brick that wave `wave` of WPB takes (row j = jz * BY + jy,
-// j = wave, wave + WPB, ...), each decided by eval(py, pz) for the lane's x.
+// The rows of a brick that wave `wave` of WPB takes (row j = jz * BY + jy
+// with j % WPB == wave), each decided by eval(py, pz) for the lane's x.
 template <int WPB, class F>
 __device__ __forceinline__ void brick_rows(const Grid &g, const Bricks &B, uint64_t *__restrict__ bits, int bx,
                                            int by, int bz, int wave, int x, bool xin, bool edge,
                                            Ctl *__restrict__ ctl, SegOut *__restrict__ so, F &&eval) {
-    int jy = wave, jz = 0;
-    while (jy >= B.BY) jy -= B.BY, jz++;
     const int y0 = by * B.BY, z0 = bz * B.BZ;
     const int jz_end = min(B.BZ, g.nz - z0), jy_end = min(B.BY, g.ny - y0);
     const auto ys = scalar_ptr(g.ys) + y0, zs = scalar_ptr(g.zs) + z0;
-    while (jz < jz_end) {
-        if (jy < jy_end) {
-            const bool inside = eval(ys[jy], zs[jz]) && xin;
-            row_out(g, bits, bx, y0 + jy, z0 + jz, x, inside, edge, ctl, so);
+    const uint64_t xedge = __ballot(x == 0 || x == g.nx - 1);
+    const size_t W = (size_t)g.W, plane = (size_t)g.ny * W;
+    uint64_t *zrow = bits + word_index(g, bx, y0, z0);  // word of row (0, jz)
+    for (int jz = 0; jz < jz_end; jz++, zrow += plane) {
+        const int z = z0 + jz;
+        const double pz = zs[jz];
+        const bool zedge = z == 0 || z == g.nz - 1;
+        for (int jy = ((wave - jz * B.BY) % WPB + WPB) % WPB; jy < jy_end; jy += WPB) {
+            const int y = y0 + jy;
+            const bool inside = eval(ys[jy], pz) && xin;
+            row_out(zrow + (size_t)jy * W, inside, edge, xedge, zedge || y == 0 || y == g.ny - 1, ctl, so);
         }
-        jy += WPB;
-        while (jy >= B.BY) jy -= B.BY, jz++;
     }
 }
+
+// A folded program whose atoms are all untransformed spheres, boxes,
+// capsules or cylinders (scalar loads: the walk is wave-uniform).
+template <class CodeP>
+__device__ __forceinline__ bool fast_atoms(CodeP code, uint32_t n) {
+    for (uint32_t pc = 0; pc < n; ++pc) {
+        const uint32_t x = (uint32_t)__builtin_amdgcn_readfirstlane((int)code[pc].x);
+        if (insn_op(x) != OP_ATOM) continue;
+        const uint32_t kind = insn_kind(x);
+        if (insn_frame(x) != 0 || !(kind == A_SPHERE || kind == A_RECT || kind == A_CAPSULE || kind == A_CYLINDER))
+            return false;
+    }
+    return true;
+}
+
+// Rows one wave of a brick can take in the instruction-major walk.
+constexpr int WAVE_ROWS = 16;
 
 // Culled, part 2: WPB waves per active brick.  A brick whose folded program
 // is one untransformed atom has the atom kind dispatched once for all its
@@ -523,15 +544,76 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
             (kind0 == A_SPHERE || kind0 == A_RECT || kind0 == A_CAPSULE || kind0 == A_CYLINDER)) {
             // one atom: dispatched here, once for the brick's rows
             auto rows = [&](auto KIND) {
+                constexpr uint32_t K = decltype(KIND)::value;
+                constexpr int N = K == A_SPHERE ? 7 : K == A_RECT ? 6 : K == A_CAPSULE ? 15 : 12;
+                const auto p64 = reg_params<N, double>(f64 + in0.y);
+                const auto p32 = reg_params<(MODE == 1 ? N : 1), float>(f32 + in0.y);
                 brick_rows<WPB>(g, B, bits, bx, by, bz, wave, x, xin, edge, ctl, so, [&](double py, double pz) {
-                    return eval_atom<MODE, STATS, false>(P, decltype(KIND)::value, f64 + in0.y, f32 + in0.y, px0,
-                                                         py, pz, fx0, (float)py, (float)pz, xin, 0, gs);
+                    return eval_atom<MODE, STATS, false>(P, K, p64, p32, px0, py, pz, fx0, (float)py, (float)pz,
+                                                         xin, 0, gs);
                 });
             };
             if (kind0 == A_SPHERE) rows(std::integral_constant<uint32_t, A_SPHERE>());
             else if (kind0 == A_RECT) rows(std::integral_constant<uint32_t, A_RECT>());
             else if (kind0 == A_CAPSULE) rows(std::integral_constant<uint32_t, A_CAPSULE>());
             else rows(std::integral_constant<uint32_t, A_CYLINDER>());
+            continue;
+        }
+        const uint64_t xedge = __ballot(x == 0 || x == g.nx - 1);
+        const int wrows = (nrows - wave + WPB - 1) / WPB;  // rows of this wave: j = wave + WPB * i
+        if (wrows <= WAVE_ROWS && fast_atoms(code, bi.y)) {
+            // Several untransformed atoms of the four common kinds: each
+            // instruction is decoded once for all the wave's rows (a bit
+            // stack per row), each atom with its params in vector registers.
+            // row i of the wave: (jy, jz) of j = wave + WPB * i, stepped
+            // without divisions; rows past the grid are skipped
+            const int y0 = by * B.BY, z0 = bz * B.BZ;
+            const int jy_end = min(B.BY, g.ny - y0), jz_end = min(B.BZ, g.nz - z0);
+            int jy0 = wave, jz0 = 0;
+            while (jy0 >= B.BY) jy0 -= B.BY, jz0++;
+            auto next_row = [&](int &jy, int &jz) {
+                jy += WPB;
+                while (jy >= B.BY) jy -= B.BY, jz++;
+            };
+            uint32_t st[WAVE_ROWS];
+            for (int i = 0; i < WAVE_ROWS; i++) st[i] = 0;
+            const auto ys = scalar_ptr(g.ys) + y0, zs = scalar_ptr(g.zs) + z0;
+            for (uint32_t pc = 0; pc < bi.y; ++pc) {
+                Insn in{code[pc].x, code[pc].y};
+                in.x = (uint32_t)__builtin_amdgcn_readfirstlane((int)in.x);
+                in.y = (uint32_t)__builtin_amdgcn_readfirstlane((int)in.y);
+                const uint32_t op = insn_op(in.x);
+                if (op == OP_ATOM) {
+                    auto atom_rows = [&](auto KIND) {
+                        constexpr uint32_t K = decltype(KIND)::value;
+                        constexpr int N = K == A_SPHERE ? 7 : K == A_RECT ? 6 : K == A_CAPSULE ? 15 : 12;
+                        const auto p64 = reg_params<N, double>(f64 + in.y);
+                        const auto p32 = reg_params<(MODE == 1 ? N : 1), float>(f32 + in.y);
+                        int jy = jy0, jz = jz0;
+                        for (int i = 0; i < WAVE_ROWS; i++, next_row(jy, jz))
+                            if (i < wrows && jy < jy_end && jz < jz_end) {  // wave-uniform
+                                const double py = ys[jy], pz = zs[jz];
+                                const bool v = eval_atom<MODE, STATS, false>(P, K, p64, p32, px0, py, pz, fx0,
+                                                                             (float)py, (float)pz, xin, 0, gs);
+                                st[i] = (st[i] << 1) | (uint32_t)v;
+                            }
+                    };
+                    const uint32_t kind = insn_kind(in.x);
+                    if (kind == A_SPHERE) atom_rows(std::integral_constant<uint32_t, A_SPHERE>());
+                    else if (kind == A_RECT) atom_rows(std::integral_constant<uint32_t, A_RECT>());
+                    else if (kind == A_CAPSULE) atom_rows(std::integral_constant<uint32_t, A_CAPSULE>());
+                    else atom_rows(std::integral_constant<uint32_t, A_CYLINDER>());
+                } else {
+                    for (int i = 0; i < WAVE_ROWS; i++) st[i] = stack_op(op, st[i]);
+                }
+            }
+            int jy = jy0, jz = jz0;
+            for (int i = 0; i < WAVE_ROWS; i++, next_row(jy, jz))
+                if (i < wrows && jy < jy_end && jz < jz_end) {
+                    const int y = y0 + jy, z = z0 + jz;
+                    row_out(bits + word_index(g, bx, y, z), (st[i] & 1u) && xin, edge, xedge,
+                            y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1, ctl, so);
+                }
             continue;
         }
         for (int j0 = wave; j0 < nrows; j0 += WPB * NP) {
@@ -551,7 +633,9 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
             eval_code_n<MODE, STATS, NP, true>(code, bi.y, P, f64, f32, scalar_ptr(P.frames), px, py, pz, live, res,
                                                gs);
             for (int q = 0; q < NP; q++)
-                if (ok[q]) row_out(g, bits, bx, yy[q], zz[q], x, res[q] && xin, edge, ctl, so);  // ok: wave-uniform
+                if (ok[q])  // wave-uniform
+                    row_out(bits + word_index(g, bx, yy[q], zz[q]), res[q] && xin, edge, xedge,
+                            yy[q] == 0 || yy[q] == g.ny - 1 || zz[q] == 0 || zz[q] == g.nz - 1, ctl, so);
         }
     }
     flush_stats<STATS>(gs, ctl->gstats, ctl->flops);
