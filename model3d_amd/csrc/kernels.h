@@ -778,14 +778,15 @@ __device__ __forceinline__ int cube_case(const CubeCorners &cc, int i) {
     return c;
 }
 
-// counts[w] = {crossed edges, triangles, 1 if the word emits anything, 0};
-// a word per thread, and per block (CLASSIFY_BLOCK words) the block's sum for
-// the scan: part_sums[blockIdx.x].
+// counts[w] = crossed edges | triangles << 8 | (1 if the word emits anything)
+// << 17 (at most 192 edges and 320 triangles per word); a word per thread,
+// and per block (CLASSIFY_BLOCK words) the block's sums {edges, triangles,
+// emitting words, 0} for the scan: part_sums[blockIdx.x].
 template <bool MULTI>
 __global__ __launch_bounds__(CLASSIFY_BLOCK) void k_classify(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
                                                          const uint32_t *__restrict__ word_seg,
                                                          const uint64_t *__restrict__ bits_all,
-                                                         const uint8_t *__restrict__ table, uint4 *__restrict__ counts,
+                                                         const uint8_t *__restrict__ table, uint32_t *__restrict__ counts,
                                                          uint4 *__restrict__ part_sums,
                                                          const uint64_t *__restrict__ keep_all, uint64_t nwords_all) {
     __shared__ uint8_t ntri[256];
@@ -822,7 +823,7 @@ __global__ __launch_bounds__(CLASSIFY_BLOCK) void k_classify(const Seg *__restri
             }
             c = make_uint4(nv, nt, (nv | nt) != 0 ? 1u : 0u, 0u);
         }
-        counts[gw] = c;
+        counts[gw] = c.x | (c.y << 8) | (c.z << 17);
     }
     const uint4 t = block_sum<CLASSIFY_BLOCK>(c, sh);
     if (threadIdx.x == 0) part_sums[blockIdx.x] = t;
@@ -858,22 +859,23 @@ __global__ __launch_bounds__(SUMS_BLOCK) void k_scan_tiles(uint4 *__restrict__ p
 // Downsweep over tiles of SCAN_TILE words (the exclusive scan of the
 // classify blocks' sums gives each tile's offset); words with .z set are also
 // listed at their scanned .z.
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint4 *__restrict__ in, uint64_t n,
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *__restrict__ in, uint64_t n,
                                                           const uint4 *__restrict__ part_sums,
-                                                          uint4 *__restrict__ out, uint32_t *__restrict__ list) {
+                                                          uint2 *__restrict__ out, uint32_t *__restrict__ list) {
     __shared__ uint4 sh[SCAN_BLOCK / 64];
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     uint4 vals[SCAN_ITEMS];
     uint4 s = make_uint4(0, 0, 0, 0);
     for (int i = 0; i < SCAN_ITEMS; i++) {
-        vals[i] = base + i < n ? in[base + i] : make_uint4(0, 0, 0, 0);
+        const uint32_t c = base + i < n ? in[base + i] : 0u;
+        vals[i] = make_uint4(c & 0xffu, (c >> 8) & 0x1ffu, c >> 17, 0u);
         s = add4(s, vals[i]);
     }
     uint4 total;
     uint4 ex = add4(block_exclusive(s, sh, total), part_sums[(uint64_t)blockIdx.x * (SCAN_TILE / CLASSIFY_BLOCK)]);
     for (int i = 0; i < SCAN_ITEMS; i++) {
         if (base + i < n) {
-            out[base + i] = ex;
+            out[base + i] = make_uint2(ex.x, ex.y);
             if (list && vals[i].z) list[ex.z] = (uint32_t)(base + i);
         }
         ex = add4(ex, vals[i]);
@@ -904,7 +906,7 @@ template <bool MULTI>
 __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
                                               const uint32_t *__restrict__ word_seg,
                                               const uint64_t *__restrict__ bits_all,
-                                              const uint8_t *__restrict__ table, const uint4 *__restrict__ offs_all,
+                                              const uint8_t *__restrict__ table, const uint2 *__restrict__ offs_all,
                                               const uint32_t *__restrict__ active, Ctl *__restrict__ ctl,
                                               uint64_t *__restrict__ vkeys, uint32_t *__restrict__ vseg,
                                               uint32_t *__restrict__ tris, const uint64_t *__restrict__ keep_all,
@@ -930,7 +932,7 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
         const Seg sg = pick_seg<MULTI>(segs, seg0, si);
         const Grid &g = sg.g;
         const uint64_t *bits = bits_all + sg.word0;
-        const uint4 *offs = offs_all + sg.word0;
+        const uint2 *offs = offs_all + sg.word0;
         const uint32_t vbase = MULTI ? offs[0].x : 0u;  // the solid's first vertex (triangles index from it)
         const uint64_t w = gw - sg.word0;
         int wx, y, z;
@@ -947,7 +949,7 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
             const uint64_t b = bits[idx];
             const uint64_t bn = bits[idx + (hx ? 1 : 0)], by = bits[idx + (hy ? (size_t)g.W : 0)];
             const uint64_t bz = bits[idx + (hz ? (size_t)g.ny * g.W : 0)];
-            const uint4 o = offs[idx];
+            const uint2 o = offs[idx];
             WordInfo r{0, 0, 0, 0, 0, 0};
             if (in) {
                 r.bits = b;
@@ -1047,9 +1049,9 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
 
 // Each solid's first vertex and triangle in the call's outputs.
 __global__ __launch_bounds__(256) void k_seg_ranges(const Seg *__restrict__ segs, uint32_t nsegs,
-                                                    const uint4 *__restrict__ offs, SegOut *__restrict__ sout) {
+                                                    const uint2 *__restrict__ offs, SegOut *__restrict__ sout) {
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nsegs; s += gridDim.x * blockDim.x) {
-        const uint4 o = offs[segs[s].word0];
+        const uint2 o = offs[segs[s].word0];
         sout[s].vbase = o.x;
         sout[s].tbase = o.y;
     }

@@ -528,7 +528,7 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 // The scan of the counts whose per-block sums k_classify wrote.
-int scan4(mc3d_ctx *ctx, const uint4 *in, uint4 *out, uint64_t n, unsigned long long *totals, uint32_t *list) {
+int scan4(mc3d_ctx *ctx, const uint32_t *in, uint2 *out, uint64_t n, unsigned long long *totals, uint32_t *list) {
     const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     k_scan_tiles<<<1, SUMS_BLOCK, 0, ctx->stream>>>(ctx->tiles.as<uint4>(), (n + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK,
                                                     totals);
@@ -654,14 +654,14 @@ void launch_emit(mc3d_ctx *ctx, const Call &c, const Upload &U, MeshBuffers &mb,
     const unsigned grid = grid_for(c.nwords, 4, 2048);
     if (c.nsegs > 1)
         k_emit<true><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(),
-                                                    ctx->table.as<uint8_t>(), ctx->offs.as<uint4>(),
+                                                    ctx->table.as<uint8_t>(), ctx->offs.as<uint2>(),
                                                     ctx->actw.as<uint32_t>(), U.ctl, ctx->vkeys.as<uint64_t>(),
                                                     ctx->vseg.as<uint32_t>(), mb.tris->as<uint32_t>(), nullptr, vcap,
                                                     tcap);
     else
         k_emit<false><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.word_seg,
                                                      ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
-                                                     ctx->offs.as<uint4>(), ctx->actw.as<uint32_t>(), U.ctl,
+                                                     ctx->offs.as<uint2>(), ctx->actw.as<uint32_t>(), U.ctl,
                                                      ctx->vkeys.as<uint64_t>(), nullptr, mb.tris->as<uint32_t>(), c.keep,
                                                      vcap, tcap);
 }
@@ -778,8 +778,8 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
     }
     const unsigned ctiles = (unsigned)((c.nwords + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK);  // classify blocks
     int rc;
-    if ((rc = ctx->bits.ensure(c.nwords * 8)) || (rc = ctx->counts.ensure(c.nwords * sizeof(uint4))) ||
-        (rc = ctx->offs.ensure(c.nwords * sizeof(uint4))) || (rc = ctx->actw.ensure(c.nwords * sizeof(uint32_t))) ||
+    if ((rc = ctx->bits.ensure(c.nwords * 8)) || (rc = ctx->counts.ensure(c.nwords * sizeof(uint32_t))) ||
+        (rc = ctx->offs.ensure(c.nwords * sizeof(uint2))) || (rc = ctx->actw.ensure(c.nwords * sizeof(uint32_t))) ||
         (rc = ctx->tiles.ensure((uint64_t)ctiles * sizeof(uint4))))
         return rc;
     if (c.cull && ((rc = ctx->binfo.ensure(c.nbricks * sizeof(uint2))) ||
@@ -836,18 +836,18 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
     if (c.nsegs > 1)
         k_classify<true><<<ctiles, CLASSIFY_BLOCK, 0, ctx->stream>>>(
             U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
-            ctx->counts.as<uint4>(), ctx->tiles.as<uint4>(), nullptr, c.nwords);
+            ctx->counts.as<uint32_t>(), ctx->tiles.as<uint4>(), nullptr, c.nwords);
     else
         k_classify<false><<<ctiles, CLASSIFY_BLOCK, 0, ctx->stream>>>(
             U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
-            ctx->counts.as<uint4>(), ctx->tiles.as<uint4>(), c.keep, c.nwords);
+            ctx->counts.as<uint32_t>(), ctx->tiles.as<uint4>(), c.keep, c.nwords);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[8], ctx->stream));
-    if ((rc = scan4(ctx, ctx->counts.as<uint4>(), ctx->offs.as<uint4>(), c.nwords, U.ctl->totals,
+    if ((rc = scan4(ctx, ctx->counts.as<uint32_t>(), ctx->offs.as<uint2>(), c.nwords, U.ctl->totals,
                     ctx->actw.as<uint32_t>())))
         return rc;
     if (c.nsegs > 1) {  // a single solid starts at vertex 0 and triangle 0 (zeroed by the upload)
-        k_seg_ranges<<<grid_for(c.nsegs, 256), 256, 0, ctx->stream>>>(U.segs, c.nsegs, ctx->offs.as<uint4>(), U.sout);
+        k_seg_ranges<<<grid_for(c.nsegs, 256), 256, 0, ctx->stream>>>(U.segs, c.nsegs, ctx->offs.as<uint2>(), U.sout);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
