@@ -249,11 +249,12 @@ def algorithmic(stage, st):
     for the evaluation stages, ("bytes", B) for the grid / compaction stages.
 
     fill    : writes the occupancy word of every word of a constant brick -> 8 B per word
-    classify: reads the occupancy grid once (N/8 B) and writes a 16 B count per word
-    scan    : reads the counts twice, writes the offsets (16 B each) and a 4 B
-              entry per listed word
-    emit    : reads the grid words it lists and their offsets, writes an 8 B key
-              per vertex and 12 B per triangle
+    classify: reads the occupancy grid once (N/8 B) and writes a 4 B packed count
+              per word
+    scan    : reads the counts (4 B), writes the offsets (8 B) per word (the
+              block sums and the 4 B entries of listed words are small)
+    emit    : reads the grid words it lists and their 8 B offsets, writes an 8 B
+              key per vertex and 12 B per triangle
     plan    : writes an 8 B brick record per brick and a 4 B list entry per
               active brick
     corners / bisect: the counted operations of every evaluated primitive,
@@ -268,9 +269,9 @@ def algorithmic(stage, st):
     b = {
         "ms_plan": 8 * st["bricks_total"] + 4 * st["bricks_active"],
         "ms_fill": 8 * words,
-        "ms_classify": n / 8 + 16 * words,
-        "ms_scan": 48 * words,
-        "ms_emit": n / 8 + 16 * words + 8 * v + 12 * t,
+        "ms_classify": n / 8 + 4 * words,
+        "ms_scan": 12 * words,
+        "ms_emit": n / 8 + 8 * words + 8 * v + 12 * t,
     }[stage]
     return ("bytes", b)
 
