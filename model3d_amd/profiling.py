@@ -60,30 +60,32 @@ def trace_table(path: str) -> list:
     agg = collections.OrderedDict()
     for name, dur, vgpr, sgpr, lds, grid, wg in kernel_rows(path):
         k = short_name(name)
-        a = agg.setdefault(k, {"calls": 0, "total": 0, "min": None, "max": 0, "vgpr": 0, "sgpr": 0, "lds": 0})
+        a = agg.setdefault(k, {"calls": 0, "total": 0, "min": None, "max": 0, "vgpr": 0, "sgpr": 0, "lds": 0,
+                               "durs": []})
         a["calls"] += 1
+        a["durs"].append(dur)
         a["total"] += dur
         a["min"] = dur if a["min"] is None else min(a["min"], dur)
         a["max"] = max(a["max"], dur)
         a["vgpr"], a["sgpr"], a["lds"] = max(a["vgpr"], vgpr), max(a["sgpr"], sgpr), max(a["lds"], lds)
     rows = sorted(agg.items(), key=lambda kv: -kv[1]["total"])
-    return [(k, a["calls"], a["total"] / a["calls"] / 1e3, a["min"] / 1e3, a["max"] / 1e3, a["total"] / 1e3,
-             a["vgpr"], a["sgpr"], a["lds"]) for k, a in rows]
+    return [(k, a["calls"], a["total"] / a["calls"] / 1e3, sorted(a["durs"])[(len(a["durs"]) - 1) // 2] / 1e3,
+             a["min"] / 1e3, a["max"] / 1e3, a["total"] / 1e3, a["vgpr"], a["sgpr"], a["lds"]) for k, a in rows]
 
 
 def trace_markdown(path: str, title: str = "", command: str = "") -> str:
     rows = trace_table(path)
-    total = sum(r[5] for r in rows) or 1.0
+    total = sum(r[6] for r in rows) or 1.0
     lines = []
     if title:
         lines += [f"# {title}", ""]
     if command:
         lines += ["Command:", "", "```", command, "```", ""]
-    lines += ["| kernel | calls | avg us | min us | max us | total us | % | VGPR | SGPR | LDS B |",
-              "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
-    for k, calls, avg, mn, mx, tot, vgpr, sgpr, lds in rows:
-        lines.append(f"| `{k}` | {calls} | {avg:.2f} | {mn:.2f} | {mx:.2f} | {tot:.1f} | {100 * tot / total:.1f} | "
-                     f"{vgpr} | {sgpr} | {lds} |")
+    lines += ["| kernel | calls | avg us | median us | min us | max us | total us | % | VGPR | SGPR | LDS B |",
+              "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for k, calls, avg, med, mn, mx, tot, vgpr, sgpr, lds in rows:
+        lines.append(f"| `{k}` | {calls} | {avg:.2f} | {med:.2f} | {mn:.2f} | {mx:.2f} | {tot:.1f} | "
+                     f"{100 * tot / total:.1f} | {vgpr} | {sgpr} | {lds} |")
     return "\n".join(lines) + "\n"
 
 
