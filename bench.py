@@ -58,7 +58,6 @@ PLAIN_RUN = dict(precision="fp64", cull=False, brick=0, rows_per_walk=1, waves_p
 # rocprofv3 summaries.
 STAGE_KERNEL = {
     "ms_plan": "k_plan",
-    "ms_fill": "k_fill_const",
     "ms_corners": "k_corners_act",
     "ms_classify": "k_classify",
     "ms_scan": "k_scan_*",
@@ -248,15 +247,15 @@ def algorithmic(stage, st):
     """Algorithmic work of one launch of a stage's kernel(s): ("flop", f64, f32)
     for the evaluation stages, ("bytes", B) for the grid / compaction stages.
 
-    fill    : writes the occupancy word of every word of a constant brick -> 8 B per word
     classify: reads the occupancy grid once (N/8 B) and writes a 4 B packed count
               per word
     scan    : reads the counts (4 B), writes the offsets (8 B) per word (the
               block sums and the 4 B entries of listed words are small)
     emit    : reads the grid words it lists and their 8 B offsets, writes an 8 B
               key per vertex and 12 B per triangle
-    plan    : writes an 8 B brick record per brick and a 4 B list entry per
-              active brick
+    plan    : writes an 8 B brick record per brick, a 4 B list entry per
+              active brick and the 8 B occupancy words of the constant bricks
+              (their share of the words taken as their share of the bricks)
     corners / bisect: the counted operations of every evaluated primitive,
               transform and midpoint (float64, float32)
     """
@@ -267,8 +266,8 @@ def algorithmic(stage, st):
     if stage == "ms_bisect":
         return ("flop", st["bisect_flops64"], st["bisect_flops32"])
     b = {
-        "ms_plan": 8 * st["bricks_total"] + 4 * st["bricks_active"],
-        "ms_fill": 8 * words,
+        "ms_plan": 8 * st["bricks_total"] + 4 * st["bricks_active"]
+        + 8 * words * (1 - st["bricks_active"] / max(1, st["bricks_total"])),
         "ms_classify": n / 8 + 4 * words,
         "ms_scan": 12 * words,
         "ms_emit": n / 8 + 8 * words + 8 * v + 12 * t,

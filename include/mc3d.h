@@ -152,9 +152,11 @@ typedef struct {
     uint64_t bricks_full;       /* active bricks whose folded program overflowed (full program used) */
     uint64_t program_length;    /* instructions of the whole lowered program */
     double ms_device;          /* GPU time of the whole pipeline (HIP events) */
-    /* Per kernel (HIP events on the context's stream): k_plan, k_fill_const,
-     * the corner kernel, k_classify, the three scan kernels, k_emit, the
-     * bisection (with host-evaluated atoms: its rounds, callbacks included). */
+    /* Per kernel (HIP events on the context's stream): k_plan (which also
+     * writes the occupancy words of constant bricks), ms_fill (kept for the
+     * layout; 0), the corner kernel, k_classify, the two scan kernels,
+     * k_emit, the bisection (with host-evaluated atoms: its rounds,
+     * callbacks included). */
     double ms_plan, ms_fill, ms_corners, ms_classify, ms_scan, ms_emit, ms_bisect;
     double ms_d2h;             /* copy of results to host */
     double ms_host_callbacks;  /* time spent in mc3d_contains_fn */

@@ -290,13 +290,18 @@ __device__ __forceinline__ void plan_fold(const Insn *code, const AtomBox *boxes
 // {false, true, expression starting at output position n} and writes the
 // folded program into the brick's slot of `cap` instructions.  A brick whose
 // folded code ever needs more than `cap` slots uses the unfolded program.
+// A brick where the solid is constant gets its occupancy words here (all
+// false, or every valid corner true, with the check that no boundary corner
+// is inside: panic("solid is true outside of bounds"), mc.go:687-689); the
+// stores cost little in a kernel whose waves mostly wait on the fold.
 // With `stage`, the program, its atom boxes and its params are first copied
 // to LDS.  The per-brick statistics are reduced per wave.
 template <bool MULTI>
 __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
                                                      const uint32_t *__restrict__ brick_seg, uint64_t nbricks_all,
                                                      uint32_t cap, uint2 *__restrict__ info, Insn *__restrict__ out,
-                                                     uint32_t *__restrict__ act_list, Ctl *__restrict__ ctl) {
+                                                     uint32_t *__restrict__ act_list, uint64_t *__restrict__ bits_all,
+                                                     Ctl *__restrict__ ctl, SegOut *__restrict__ sout) {
     extern __shared__ __align__(16) char plan_smem[];
     uint32_t *S = (uint32_t *)plan_smem + threadIdx.x * PLAN_STACK;
     Insn *sc = (Insn *)(plan_smem + PLAN_STACK_BYTES);
@@ -350,6 +355,19 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(const Seg *__restrict__ seg
             else if (over) bi = make_uint2(BRICK_FULL, P.ncode), active = true;
             else bi = make_uint2((uint32_t)(gb * cap), n), active = true;
             info[gb] = bi;
+            if (!active) {  // the constant brick's words
+                const bool t = root == TAG_TRUE;
+                const int nvalid = min(64, g.nx - x0);
+                const uint64_t word = t ? (nvalid == 64 ? ~0ull : ((1ull << nvalid) - 1)) : 0ull;
+                const int jz_end = min(B.BZ, g.nz - z0), jy_end = min(B.BY, g.ny - y0);
+                const size_t W = (size_t)g.W, plane = (size_t)g.ny * W;
+                uint64_t *zrow = bits_all + sg.word0 + word_index(g, bx, y0, z0);
+                for (int jz = 0; jz < jz_end; jz++, zrow += plane)
+                    for (int jy = 0; jy < jy_end; jy++) zrow[(size_t)jy * W] = word;
+                if (t && (bx == 0 || x0 + nvalid == g.nx || y0 == 0 || y0 + jy_end == g.ny || z0 == 0 ||
+                          z0 + jz_end == g.nz))
+                    atomicOr(&ctl->inside, 1u), atomicOr(&sout[si].inside, 1u);
+            }
             len = active ? bi.y : 0;
             full = over;
         }
@@ -401,39 +419,6 @@ __global__ __launch_bounds__(256) void k_corners(const Seg *__restrict__ segs, u
             atomicOr(&ctl->inside, 1u), atomicOr(&sout[si].inside, 1u);
     }
     flush_stats<STATS>(gs, ctl->gstats, ctl->flops);
-}
-
-// Culled, part 1: words of constant bricks are written directly (all false,
-// or every valid corner true).
-template <bool MULTI>
-__global__ __launch_bounds__(256) void k_fill_const(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
-                                                    const uint32_t *__restrict__ word_seg,
-                                                    uint64_t *__restrict__ bits, Ctl *__restrict__ ctl,
-                                                    SegOut *__restrict__ sout, uint64_t nwords_all) {
-    for (uint64_t gw = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gw < nwords_all;
-         gw += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t si = wave_seg<MULTI>(word_seg, gw, SEG_WORD_SHIFT);
-        const Seg sg = pick_seg<MULTI>(segs, seg0, si);
-        const Grid &g = sg.g;
-        const Bricks &B = sg.B;
-        const uint64_t w = gw - sg.word0;
-        if (w >= sg.nwords) {  // alignment padding
-            bits[gw] = 0;
-            continue;
-        }
-        int wx, y, z;
-        word_coords(g, w, wx, y, z);
-        const uint32_t len = B.info[sg.brick0 + ((size_t)(z / B.BZ) * B.nby + (y / B.BY)) * g.W + wx].y;
-        if (len == 0) {
-            bits[gw] = 0;
-        } else if (len == BRICK_TRUE) {
-            const int nvalid = min(64, g.nx - wx * 64);
-            bits[gw] = nvalid == 64 ? ~0ull : ((1ull << nvalid) - 1);
-            // panic("solid is true outside of bounds"), mc.go:687-689
-            if (y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1 || wx == 0 || wx * 64 + nvalid == g.nx)
-                atomicOr(&ctl->inside, 1u), atomicOr(&sout[si].inside, 1u);
-        }
-    }
 }
 
 __device__ __forceinline__ const Insn *brick_code(const DevProg &P, const Bricks &B, uint2 bi) {

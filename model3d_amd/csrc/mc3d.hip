@@ -569,14 +569,11 @@ template <int MODE, bool STATS, bool MULTI>
 void launch_corners_m(mc3d_ctx *ctx, const Call &c, const Upload &U) {
     const Seg &s0 = U.hsegs[0];
     if (!c.cull) {
-        (void)hipEventRecord(ctx->ev[7], ctx->stream);
         k_corners<MODE, STATS, MULTI><<<grid_for(c.nwords, 4), 256, 0, ctx->stream>>>(
             U.segs, c.nsegs, s0, U.word_seg, ctx->bits.as<uint64_t>(), U.ctl, U.sout, c.nwords);
         return;
     }
-    k_fill_const<MULTI><<<grid_for(c.nwords, 256), 256, 0, ctx->stream>>>(
-        U.segs, c.nsegs, s0, U.word_seg, ctx->bits.as<uint64_t>(), U.ctl, U.sout, c.nwords);
-    (void)hipEventRecord(ctx->ev[7], ctx->stream);
+    // (the words of constant bricks were written by the plan)
     const unsigned grid = grid_for(c.nbricks, 1, 1u << 15);
     if (c.rows_per_walk >= 4)
         launch_corners_act<MODE, STATS, 4, MULTI>(ctx, c, U, grid);
@@ -812,14 +809,13 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
             if (U.hsegs[s].stage) lds = std::max(lds, plan_stage_bytes(jobs[s].P));
         const unsigned pgrid = grid_for(c.nbricks, PLAN_BLOCK);
         if (c.nsegs > 1)
-            k_plan<true><<<pgrid, PLAN_BLOCK, lds, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.brick_seg, c.nbricks,
-                                                                  c.cap, ctx->binfo.as<uint2>(),
-                                                                  ctx->bcode.as<Insn>(), ctx->actb.as<uint32_t>(), U.ctl);
+            k_plan<true><<<pgrid, PLAN_BLOCK, lds, ctx->stream>>>(
+                U.segs, c.nsegs, U.hsegs[0], U.brick_seg, c.nbricks, c.cap, ctx->binfo.as<uint2>(),
+                ctx->bcode.as<Insn>(), ctx->actb.as<uint32_t>(), ctx->bits.as<uint64_t>(), U.ctl, U.sout);
         else
-            k_plan<false><<<pgrid, PLAN_BLOCK, lds, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.brick_seg,
-                                                                   c.nbricks, c.cap, ctx->binfo.as<uint2>(),
-                                                                   ctx->bcode.as<Insn>(), ctx->actb.as<uint32_t>(),
-                                                                   U.ctl);
+            k_plan<false><<<pgrid, PLAN_BLOCK, lds, ctx->stream>>>(
+                U.segs, c.nsegs, U.hsegs[0], U.brick_seg, c.nbricks, c.cap, ctx->binfo.as<uint2>(),
+                ctx->bcode.as<Insn>(), ctx->actb.as<uint32_t>(), ctx->bits.as<uint64_t>(), U.ctl, U.sout);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
@@ -906,8 +902,8 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
     st.brick_code_max = hc->code_max;
     st.bricks_full = hc->full_bricks;
     st.ms_plan = elapsed(ctx->ev[0], ctx->ev[1]);
-    st.ms_fill = elapsed(ctx->ev[1], ctx->ev[7]);
-    st.ms_corners = elapsed(ctx->ev[7], ctx->ev[2]);
+    st.ms_fill = 0;  // the plan writes the constant bricks' words
+    st.ms_corners = elapsed(ctx->ev[1], ctx->ev[2]);
     st.ms_classify = elapsed(ctx->ev[2], ctx->ev[8]);
     st.ms_scan = elapsed(ctx->ev[8], ctx->ev[3]);
     st.ms_emit = elapsed(ctx->ev[4], ctx->ev[5]);
