@@ -900,6 +900,8 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
     __shared__ WordInfo wi_all[4][8];
     __shared__ uint32_t s_pre[4][64];
     __shared__ uint8_t s_cs[4][64];
+    // (table: the case table with each edge decoded to axis | dx << 2 |
+    // dy << 3 | dz << 4, its offset from the cube's low corner)
     for (int i = threadIdx.x; i < 256 * 16; i += blockDim.x) tab[i] = table[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = wave_id();
@@ -929,11 +931,13 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
             // word or its neighbour lies beyond the grid
             const bool in = ex < g.W && ey < g.ny && ez < g.nz;
             const int cx = min(ex, g.W - 1), cy = min(ey, g.ny - 1), cz = min(ez, g.nz - 1);
-            const size_t idx = word_index(g, cx, cy, cz);
+            // a segment's word indices fit in 32 bits (checked on the host)
+            const uint32_t W = (uint32_t)g.W, plane = (uint32_t)g.ny * W;
+            const uint32_t idx = (uint32_t)cz * plane + (uint32_t)cy * W + (uint32_t)cx;
             const bool hx = cx + 1 < g.W, hy = cy + 1 < g.ny, hz = cz + 1 < g.nz;
             const uint64_t b = bits[idx];
-            const uint64_t bn = bits[idx + (hx ? 1 : 0)], by = bits[idx + (hy ? (size_t)g.W : 0)];
-            const uint64_t bz = bits[idx + (hz ? (size_t)g.ny * g.W : 0)];
+            const uint64_t bn = bits[idx + (hx ? 1u : 0u)], by = bits[idx + (hy ? W : 0u)];
+            const uint64_t bz = bits[idx + (hz ? plane : 0u)];
             const uint2 o = offs[idx];
             WordInfo r{0, 0, 0, 0, 0, 0};
             if (in) {
@@ -948,33 +952,33 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
         }
         wave_sync();
         const int x = wx * 64 + lane;
-        {  // vertices of this word
+        {  // vertices of this word (vertex numbers are 32-bit, as the scan)
             const WordInfo me = wi[0];
-            uint64_t v = me.base;
-            const uint64_t k0 = v + __popcll(me.cx & below);
+            const uint64_t yz = vkey(0, 0, y, z);  // the row's part of the keys
+            const uint32_t c0 = __popcll(me.cx), c1 = __popcll(me.cy);
+            const uint32_t k0 = me.base + __popcll(me.cx & below);
+            const uint32_t k1 = me.base + c0 + __popcll(me.cy & below);
+            const uint32_t k2 = me.base + c0 + c1 + __popcll(me.cz & below);
+            const uint32_t xk = (uint32_t)x << 2;
             if ((me.cx >> lane) & 1) {
                 if (k0 < vcap) {
-                    vkeys[k0] = vkey(0, x, y, z);
+                    vkeys[k0] = yz | xk;
                     if (vseg) vseg[k0] = si;
                 } else {
                     over = true;
                 }
             }
-            v += __popcll(me.cx);
-            const uint64_t k1 = v + __popcll(me.cy & below);
             if ((me.cy >> lane) & 1) {
                 if (k1 < vcap) {
-                    vkeys[k1] = vkey(1, x, y, z);
+                    vkeys[k1] = yz | (xk | 1u);
                     if (vseg) vseg[k1] = si;
                 } else {
                     over = true;
                 }
             }
-            v += __popcll(me.cy);
-            const uint64_t k2 = v + __popcll(me.cz & below);
             if ((me.cz >> lane) & 1) {
                 if (k2 < vcap) {
-                    vkeys[k2] = vkey(2, x, y, z);
+                    vkeys[k2] = yz | (xk | 2u);
                     if (vseg) vseg[k2] = si;
                 } else {
                     over = true;
@@ -1013,19 +1017,15 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
             const int c = s_cs[wave][L];
             const int kk = 3 * (int)(ti - s_pre[wave][L]) + (int)ec;
             const int e = tab[16 * c + 1 + kk];
-            const int axis = e >> 2, r = e & 3;
-            int ddx = 0, ddy = 0, ddz = 0;
-            if (axis == 0) ddy = r & 1, ddz = r >> 1;
-            else if (axis == 1) ddx = r & 1, ddz = r >> 1;
-            else ddx = r & 1, ddy = r >> 1;
-            const int ex = wx * 64 + L + ddx;
-            const WordInfo &o = wi[((ex >> 6) - wx) | (ddy << 1) | (ddz << 2)];
+            const int axis = e & 3, ddx = (e >> 2) & 1, ddy = (e >> 3) & 1, ddz = (e >> 4) & 1;
+            const int ex = L + ddx;  // x within this word (64: the next word's first)
+            const WordInfo &o = wi[(ex >> 6) | (ddy << 1) | (ddz << 2)];
             const int bb = ex & 63;
             const uint64_t bl = bb ? (~0ull >> (64 - bb)) : 0ull;
-            uint32_t idx = o.base;
-            if (axis == 0) idx += __popcll(o.cx & bl);
-            else if (axis == 1) idx += __popcll(o.cx) + __popcll(o.cy & bl);
-            else idx += __popcll(o.cx) + __popcll(o.cy) + __popcll(o.cz & bl);
+            // vertices of the word before this edge: its x crossings, then y, then z
+            const uint64_t m = axis == 0 ? o.cx : (axis == 1 ? o.cy : o.cz);
+            const uint32_t idx = o.base + (axis > 0 ? (uint32_t)__popcll(o.cx) : 0u) +
+                                 (axis > 1 ? (uint32_t)__popcll(o.cy) : 0u) + (uint32_t)__popcll(m & bl);
             tris[3 * t0 + j] = idx - vbase;
         }
     }

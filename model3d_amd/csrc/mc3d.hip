@@ -174,7 +174,7 @@ struct mc3d_ctx {
     hipStream_t stream = nullptr;
     std::vector<std::unique_ptr<ColliderCache>> colliders;  // most recent last
     HostBuf staging, ctl_host, segstage, outstage;
-    DevBuf blob, table, bits, counts, offs, tiles, actw, actb, binfo, bcode, vkeys, vseg, pts, out8;
+    DevBuf blob, table, etable, bits, counts, offs, tiles, actw, actb, binfo, bcode, vkeys, vseg, pts, out8;
     DevBuf hvals, hpts, hstate, hprobe;  // host-evaluated atoms: answers, points, bisection state
     HostBuf hstage;                      // pinned points / answers of one callback batch
     DevBuf stl, stl_in;                  // STL encoding: body bytes; uploaded host triangles
@@ -651,13 +651,13 @@ void launch_emit(mc3d_ctx *ctx, const Call &c, const Upload &U, MeshBuffers &mb,
     const unsigned grid = grid_for(c.nwords, 4, 2048);
     if (c.nsegs > 1)
         k_emit<true><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(),
-                                                    ctx->table.as<uint8_t>(), ctx->offs.as<uint2>(),
+                                                    ctx->etable.as<uint8_t>(), ctx->offs.as<uint2>(),
                                                     ctx->actw.as<uint32_t>(), U.ctl, ctx->vkeys.as<uint64_t>(),
                                                     ctx->vseg.as<uint32_t>(), mb.tris->as<uint32_t>(), nullptr, vcap,
                                                     tcap);
     else
         k_emit<false><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.word_seg,
-                                                     ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
+                                                     ctx->bits.as<uint64_t>(), ctx->etable.as<uint8_t>(),
                                                      ctx->offs.as<uint2>(), ctx->actw.as<uint32_t>(), U.ctl,
                                                      ctx->vkeys.as<uint64_t>(), nullptr, mb.tris->as<uint32_t>(), c.keep,
                                                      vcap, tcap);
@@ -1013,8 +1013,24 @@ int mc3d_ctx_create(int device, mc3d_ctx **out) {
             return MC3D_E_HIP;
         }
     }
-    if (c->ctl_host.ensure(sizeof(Ctl) + 4096) || c->table.ensure(256 * 16) ||
-        hipMemcpy(c->table.p, case_table(), 256 * 16, hipMemcpyHostToDevice) != hipSuccess) {
+    // The emitter's copy of the case table: each edge e = 4 * axis + r
+    // decoded to axis | dx << 2 | dy << 3 | dz << 4 (its offset from the
+    // cube's low corner); entry 0 of a case stays its triangle count.
+    uint8_t etab[256 * 16];
+    const uint8_t *tab = case_table();
+    for (int i = 0; i < 256 * 16; i++) {
+        const int t = tab[i];
+        if ((i & 15) == 0) {
+            etab[i] = (uint8_t)t;
+            continue;
+        }
+        const int axis = (t >> 2) & 3, d0 = t & 1, d1 = (t >> 1) & 1;  // d0, d1: the other two axes in x, y, z order
+        const int dx = axis == 0 ? 0 : d0, dy = axis == 0 ? d0 : (axis == 1 ? 0 : d1), dz = axis == 2 ? 0 : d1;
+        etab[i] = (uint8_t)(axis | (dx << 2) | (dy << 3) | (dz << 4));
+    }
+    if (c->ctl_host.ensure(sizeof(Ctl) + 4096) || c->table.ensure(256 * 16) || c->etable.ensure(256 * 16) ||
+        hipMemcpy(c->table.p, tab, 256 * 16, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->etable.p, etab, 256 * 16, hipMemcpyHostToDevice) != hipSuccess) {
         set_error("mc3d_ctx_create: setup failed");
         mc3d_ctx_destroy(c);
         return MC3D_E_HIP;
