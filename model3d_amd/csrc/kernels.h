@@ -492,6 +492,7 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
                                                           SegOut *__restrict__ sout) {
     const int lane = threadIdx.x & 63, wave = wave_id();
     const uint32_t nact = ctl->nact_bricks;
+    __shared__ double2 row_yz[WPB][WAVE_ROWS];  // (y, z) of each wave's rows (instruction-major walk)
     GStats gs;
     for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x) {
         const uint32_t gb = scalar_ptr(act)[k];
@@ -550,10 +551,19 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
             // Several untransformed atoms of the four common kinds: each
             // instruction is decoded once for all the wave's rows (a bit
             // stack per row), each atom with its params in vector registers.
-            // row i of the wave: (jy, jz) of j = wave + WPB * i, stepped
-            // without divisions; rows past the grid are skipped
+            // Row i of the wave is j = wave + WPB * i: lane i puts its
+            // coordinates in LDS once (rows past the grid: bit i of rmask
+            // clear), so the atoms read them with a broadcast.
             const int y0 = by * B.BY, z0 = bz * B.BZ;
             const int jy_end = min(B.BY, g.ny - y0), jz_end = min(B.BZ, g.nz - z0);
+            bool rok = false;
+            if (lane < WAVE_ROWS) {
+                const int j = wave + WPB * lane, jy = j % B.BY, jz = j / B.BY;
+                rok = lane < wrows && jy < jy_end && jz < jz_end;
+                if (rok) row_yz[wave][lane] = make_double2(g.ys[y0 + jy], g.zs[z0 + jz]);
+            }
+            const uint32_t rmask = (uint32_t)__ballot(rok);
+            wave_sync();
             int jy0 = wave, jz0 = 0;
             while (jy0 >= B.BY) jy0 -= B.BY, jz0++;
             auto next_row = [&](int &jy, int &jz) {
@@ -562,7 +572,6 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
             };
             uint32_t st[WAVE_ROWS];
             for (int i = 0; i < WAVE_ROWS; i++) st[i] = 0;
-            const auto ys = scalar_ptr(g.ys) + y0, zs = scalar_ptr(g.zs) + z0;
             for (uint32_t pc = 0; pc < bi.y; ++pc) {
                 Insn in{code[pc].x, code[pc].y};
                 in.x = (uint32_t)__builtin_amdgcn_readfirstlane((int)in.x);
@@ -574,10 +583,10 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
                         constexpr int N = K == A_SPHERE ? 7 : K == A_RECT ? 6 : K == A_CAPSULE ? 15 : 12;
                         const auto p64 = reg_params<N, double>(f64 + in.y);
                         const auto p32 = reg_params<(MODE == 1 ? N : 1), float>(f32 + in.y);
-                        int jy = jy0, jz = jz0;
-                        for (int i = 0; i < WAVE_ROWS; i++, next_row(jy, jz))
-                            if (i < wrows && jy < jy_end && jz < jz_end) {  // wave-uniform
-                                const double py = ys[jy], pz = zs[jz];
+                        for (int i = 0; i < WAVE_ROWS; i++)
+                            if ((rmask >> i) & 1u) {  // wave-uniform
+                                const double2 q = row_yz[wave][i];
+                                const double py = q.x, pz = q.y;
                                 const bool v = eval_atom<MODE, STATS, false>(P, K, p64, p32, px0, py, pz, fx0,
                                                                              (float)py, (float)pz, xin, 0, gs);
                                 st[i] = (st[i] << 1) | (uint32_t)v;
@@ -594,7 +603,7 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
             }
             int jy = jy0, jz = jz0;
             for (int i = 0; i < WAVE_ROWS; i++, next_row(jy, jz))
-                if (i < wrows && jy < jy_end && jz < jz_end) {
+                if ((rmask >> i) & 1u) {
                     const int y = y0 + jy, z = z0 + jz;
                     row_out(bits + word_index(g, bx, y, z), (st[i] & 1u) && xin, edge, xedge,
                             y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1, ctl, so);
