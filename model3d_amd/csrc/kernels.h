@@ -438,12 +438,46 @@ __device__ __forceinline__ void row_out(uint64_t *__restrict__ word, bool inside
     }
 }
 
+// A row of a brick as a wave walks it: its coordinates, its word (index in
+// the segment's occupancy grid) and whether it lies on a y or z face.
+struct RowInfo {
+    double y, z;
+    uint32_t word, redge;
+};
+
+// Lane i describes the wave's row i (row j = wave + WPB * i of the brick,
+// j = jz * BY + jy) in rows[i], once per brick, so the walks read a row with
+// one broadcast instead of stepping and loading; returns the rows inside the
+// grid as a mask.  Needs BY * BZ <= 64 * WPB.
+template <int WPB>
+__device__ __forceinline__ uint64_t wave_rows(const Grid &g, const Bricks &B, int bx, int by, int bz, int wave,
+                                              RowInfo *rows) {
+    const int lane = threadIdx.x & 63;
+    const int j = wave + WPB * lane, jy = j % B.BY, jz = j / B.BY;
+    const int y = by * B.BY + jy, z = bz * B.BZ + jz;
+    const bool ok = j < B.BY * B.BZ && y < g.ny && z < g.nz;
+    if (ok)
+        rows[lane] = RowInfo{g.ys[y], g.zs[z], (uint32_t)word_index(g, bx, y, z),
+                             (y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1) ? 1u : 0u};
+    wave_sync();
+    return __ballot(ok);
+}
+
 // The rows of a brick that wave `wave` of WPB takes (row j = jz * BY + jy
 // with j % WPB == wave), each decided by eval(py, pz) for the lane's x.
 template <int WPB, class F>
 __device__ __forceinline__ void brick_rows(const Grid &g, const Bricks &B, uint64_t *__restrict__ bits, int bx,
                                            int by, int bz, int wave, int x, bool xin, bool edge,
-                                           Ctl *__restrict__ ctl, SegOut *__restrict__ so, F &&eval) {
+                                           Ctl *__restrict__ ctl, SegOut *__restrict__ so, RowInfo *rows, F &&eval) {
+    if (B.BY * B.BZ <= 64 * WPB) {  // the rows from LDS
+        const uint64_t xedge = __ballot(x == 0 || x == g.nx - 1);
+        for (uint64_t m = wave_rows<WPB>(g, B, bx, by, bz, wave, rows); m; m &= m - 1) {
+            const RowInfo r = rows[__ffsll((unsigned long long)m) - 1];
+            const bool inside = eval(r.y, r.z) && xin;
+            row_out(bits + r.word, inside, edge, xedge, r.redge != 0, ctl, so);
+        }
+        return;
+    }
     const int y0 = by * B.BY, z0 = bz * B.BZ;
     const int jz_end = min(B.BZ, g.nz - z0), jy_end = min(B.BY, g.ny - y0);
     const auto ys = scalar_ptr(g.ys) + y0, zs = scalar_ptr(g.zs) + z0;
@@ -492,7 +526,7 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
                                                           SegOut *__restrict__ sout) {
     const int lane = threadIdx.x & 63, wave = wave_id();
     const uint32_t nact = ctl->nact_bricks;
-    __shared__ double2 row_yz[WPB][WAVE_ROWS];  // (y, z) of each wave's rows (instruction-major walk)
+    __shared__ RowInfo row_info[WPB][64];  // each wave's rows of the current brick
     GStats gs;
     for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x) {
         const uint32_t gb = scalar_ptr(act)[k];
@@ -534,7 +568,8 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
                 constexpr int N = K == A_SPHERE ? 7 : K == A_RECT ? 6 : K == A_CAPSULE ? 15 : 12;
                 const auto p64 = reg_params<N, double>(f64 + in0.y);
                 const auto p32 = reg_params<(MODE == 1 ? N : 1), float>(f32 + in0.y);
-                brick_rows<WPB>(g, B, bits, bx, by, bz, wave, x, xin, edge, ctl, so, [&](double py, double pz) {
+                brick_rows<WPB>(g, B, bits, bx, by, bz, wave, x, xin, edge, ctl, so, row_info[wave],
+                                [&](double py, double pz) {
                     return eval_atom<MODE, STATS, false>(P, K, p64, p32, px0, py, pz, fx0, (float)py, (float)pz,
                                                          xin, 0, gs);
                 });
@@ -551,25 +586,10 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
             // Several untransformed atoms of the four common kinds: each
             // instruction is decoded once for all the wave's rows (a bit
             // stack per row), each atom with its params in vector registers.
-            // Row i of the wave is j = wave + WPB * i: lane i puts its
-            // coordinates in LDS once (rows past the grid: bit i of rmask
-            // clear), so the atoms read them with a broadcast.
-            const int y0 = by * B.BY, z0 = bz * B.BZ;
-            const int jy_end = min(B.BY, g.ny - y0), jz_end = min(B.BZ, g.nz - z0);
-            bool rok = false;
-            if (lane < WAVE_ROWS) {
-                const int j = wave + WPB * lane, jy = j % B.BY, jz = j / B.BY;
-                rok = lane < wrows && jy < jy_end && jz < jz_end;
-                if (rok) row_yz[wave][lane] = make_double2(g.ys[y0 + jy], g.zs[z0 + jz]);
-            }
-            const uint32_t rmask = (uint32_t)__ballot(rok);
-            wave_sync();
-            int jy0 = wave, jz0 = 0;
-            while (jy0 >= B.BY) jy0 -= B.BY, jz0++;
-            auto next_row = [&](int &jy, int &jz) {
-                jy += WPB;
-                while (jy >= B.BY) jy -= B.BY, jz++;
-            };
+            // Row i of the wave from LDS (wave_rows; rows past the grid:
+            // bit i of rmask clear).
+            RowInfo *rows = row_info[wave];
+            const uint32_t rmask = (uint32_t)wave_rows<WPB>(g, B, bx, by, bz, wave, rows);
             uint32_t st[WAVE_ROWS];
             for (int i = 0; i < WAVE_ROWS; i++) st[i] = 0;
             for (uint32_t pc = 0; pc < bi.y; ++pc) {
@@ -585,8 +605,7 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
                         const auto p32 = reg_params<(MODE == 1 ? N : 1), float>(f32 + in.y);
                         for (int i = 0; i < WAVE_ROWS; i++)
                             if ((rmask >> i) & 1u) {  // wave-uniform
-                                const double2 q = row_yz[wave][i];
-                                const double py = q.x, pz = q.y;
+                                const double py = rows[i].y, pz = rows[i].z;
                                 const bool v = eval_atom<MODE, STATS, false>(P, K, p64, p32, px0, py, pz, fx0,
                                                                              (float)py, (float)pz, xin, 0, gs);
                                 st[i] = (st[i] << 1) | (uint32_t)v;
@@ -601,13 +620,8 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
                     for (int i = 0; i < WAVE_ROWS; i++) st[i] = stack_op(op, st[i]);
                 }
             }
-            int jy = jy0, jz = jz0;
-            for (int i = 0; i < WAVE_ROWS; i++, next_row(jy, jz))
-                if ((rmask >> i) & 1u) {
-                    const int y = y0 + jy, z = z0 + jz;
-                    row_out(bits + word_index(g, bx, y, z), (st[i] & 1u) && xin, edge, xedge,
-                            y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1, ctl, so);
-                }
+            for (int i = 0; i < WAVE_ROWS; i++)
+                if ((rmask >> i) & 1u) row_out(bits + rows[i].word, (st[i] & 1u) && xin, edge, xedge, rows[i].redge != 0, ctl, so);
             continue;
         }
         for (int j0 = wave; j0 < nrows; j0 += WPB * NP) {
