@@ -786,17 +786,44 @@ __device__ __forceinline__ int cube_case(const CubeCorners &cc, int i) {
     return c;
 }
 
+// Brick-major order of a one-solid culled call's words (all words of brick
+// 0 row by row, then brick 1, ...): the scan numbers vertices in this order,
+// so the vertices of a brick come out together and the bisection's waves
+// mostly walk one brick program.  Position i holds word (wx, y, z), or none
+// for the rows of bricks past the grid.  BY and BZ are powers of two (the
+// host uses this order only then).
+__device__ __forceinline__ uint32_t bm_index(const Grid &g, const Bricks &B, int wx, int y, int z) {
+    const int sy = __ffs(B.BY) - 1, sz = __ffs(B.BZ) - 1;
+    const int by = y >> sy, jy = y & (B.BY - 1), bz = z >> sz, jz = z & (B.BZ - 1);
+    return ((((uint32_t)(bz * B.nby + by) * (uint32_t)g.W + (uint32_t)wx) << (sy + sz)) |
+            (uint32_t)((jz << sy) | jy));
+}
+__device__ __forceinline__ bool bm_word(const Grid &g, const Bricks &B, uint32_t i, int &wx, int &y, int &z) {
+    const int sy = __ffs(B.BY) - 1, sz = __ffs(B.BZ) - 1;
+    const uint32_t b = i >> (sy + sz), r = i & ((1u << (sy + sz)) - 1);
+    const int jz = (int)(r >> sy), jy = (int)(r & (uint32_t)(B.BY - 1));
+    const uint32_t t = b / (uint32_t)g.W;
+    wx = (int)(b - t * (uint32_t)g.W);
+    const int bz = (int)(t / (uint32_t)B.nby), by = (int)t - bz * B.nby;
+    y = by * B.BY + jy;
+    z = bz * B.BZ + jz;
+    return y < g.ny && z < g.nz;
+}
+
 // counts[w] = crossed edges | triangles << 8 | (1 if the word emits anything)
 // << 17 (at most 192 edges and 320 triangles per word); a word per thread,
 // and per block (CLASSIFY_BLOCK words) the block's sums {edges, triangles,
-// emitting words, 0} for the scan: part_sums[blockIdx.x].
+// emitting words, 0} for the scan: part_sums[blockIdx.x].  With `bm` (one
+// solid, culled) thread i takes the word at brick-major position i and
+// counts[i] is that word's (nwords_all: the positions).
 template <bool MULTI>
 __global__ __launch_bounds__(CLASSIFY_BLOCK) void k_classify(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
                                                          const uint32_t *__restrict__ word_seg,
                                                          const uint64_t *__restrict__ bits_all,
                                                          const uint8_t *__restrict__ table, uint32_t *__restrict__ counts,
                                                          uint4 *__restrict__ part_sums,
-                                                         const uint64_t *__restrict__ keep_all, uint64_t nwords_all) {
+                                                         const uint64_t *__restrict__ keep_all, uint64_t nwords_all,
+                                                         bool bm) {
     __shared__ uint8_t ntri[256];
     __shared__ uint4 sh[CLASSIFY_BLOCK / 64];
     if (threadIdx.x < 256) ntri[threadIdx.x] = table[16 * threadIdx.x];
@@ -809,10 +836,17 @@ __global__ __launch_bounds__(CLASSIFY_BLOCK) void k_classify(const Seg *__restri
         const Seg sg = pick_seg<MULTI>(segs, seg0, si);
         const Grid &g = sg.g;
         const uint64_t *bits = bits_all + sg.word0;
-        const uint64_t w = gw - sg.word0;
-        if (w < sg.nwords) {  // else alignment padding
-            int wx, y, z;
-            word_coords(g, w, wx, y, z);
+        uint64_t w = gw - sg.word0;
+        int wx = 0, y = 0, z = 0;
+        bool has;
+        if (MULTI || !bm) {
+            has = w < sg.nwords;  // else alignment padding
+            if (has) word_coords(g, w, wx, y, z);
+        } else {
+            has = bm_word(g, sg.B, (uint32_t)gw, wx, y, z);
+            w = word_index(g, wx, y, z);
+        }
+        if (has) {
             const Nbhd h = load_nbhd(g, bits, wx, y, z);
             Masks m = nbhd_masks(h);
             const CubeCorners cc = nbhd_corners(h);
@@ -869,7 +903,8 @@ __global__ __launch_bounds__(SUMS_BLOCK) void k_scan_tiles(uint4 *__restrict__ p
 // listed at their scanned .z.
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *__restrict__ in, uint64_t n,
                                                           const uint4 *__restrict__ part_sums,
-                                                          uint2 *__restrict__ out, uint32_t *__restrict__ list) {
+                                                          uint2 *__restrict__ out, uint32_t *__restrict__ list,
+                                                          const Seg seg0, bool bm) {
     __shared__ uint4 sh[SCAN_BLOCK / 64];
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     uint4 vals[SCAN_ITEMS];
@@ -884,7 +919,15 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *__rest
     for (int i = 0; i < SCAN_ITEMS; i++) {
         if (base + i < n) {
             out[base + i] = make_uint2(ex.x, ex.y);
-            if (list && vals[i].z) list[ex.z] = (uint32_t)(base + i);
+            if (list && vals[i].z) {
+                uint32_t w = (uint32_t)(base + i);
+                if (bm) {  // brick-major position -> word
+                    int wx, y, z;
+                    bm_word(seg0.g, seg0.B, w, wx, y, z);
+                    w = (uint32_t)word_index(seg0.g, wx, y, z);
+                }
+                list[ex.z] = w;
+            }
         }
         ex = add4(ex, vals[i]);
     }
@@ -918,7 +961,7 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
                                               const uint32_t *__restrict__ active, Ctl *__restrict__ ctl,
                                               uint64_t *__restrict__ vkeys, uint32_t *__restrict__ vseg,
                                               uint32_t *__restrict__ tris, const uint64_t *__restrict__ keep_all,
-                                              uint64_t vcap, uint64_t tcap) {
+                                              uint64_t vcap, uint64_t tcap, bool bm) {
     __shared__ uint8_t tab[256 * 16];
     __shared__ WordInfo wi_all[4][8];
     __shared__ uint32_t s_pre[4][64];
@@ -961,7 +1004,7 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
             const uint64_t b = bits[idx];
             const uint64_t bn = bits[idx + (hx ? 1u : 0u)], by = bits[idx + (hy ? W : 0u)];
             const uint64_t bz = bits[idx + (hz ? plane : 0u)];
-            const uint2 o = offs[idx];
+            const uint2 o = offs[bm && !MULTI ? bm_index(g, sg.B, cx, cy, cz) : idx];
             WordInfo r{0, 0, 0, 0, 0, 0};
             if (in) {
                 r.bits = b;
@@ -1112,14 +1155,32 @@ __device__ __forceinline__ void bisect_vertex(const Seg &sg, const uint64_t *__r
                 else if (bi.y == BRICK_TRUE) konst = 1;
                 else code = brick_code(P, B, bi), ncode = bi.y;
             }
-            for (int i = 0; i < iters; i++) {
-                const double mid = (fp + tp) / 2;
-                p[axis] = mid;
-                // Lanes of one wave may sit in different bricks: no readfirstlane here.
-                const bool in = konst >= 0 ? konst != 0
-                                           : eval_code<MODE, STATS, false>(code, ncode, P, p[0], p[1], p[2], gs);
-                if (in) tp = mid;
-                else fp = mid;
+            // One program for every lane of the wave (no culling, or lanes
+            // of one brick: vertices are numbered brick by brick): walked
+            // with scalar loads.
+            const uint64_t cp = (uint64_t)code;
+            const uint64_t cp0 = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)cp) |
+                                 ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(cp >> 32)) << 32);
+            const uint32_t n0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)ncode);
+            const int k0 = __builtin_amdgcn_readfirstlane(konst);
+            if (__ballot(cp != cp0 || ncode != n0 || konst != k0) == 0 && k0 < 0) {
+                const Insn *code0 = (const Insn *)cp0;
+                for (int i = 0; i < iters; i++) {
+                    const double mid = (fp + tp) / 2;
+                    p[axis] = mid;
+                    if (eval_code<MODE, STATS, true>(code0, n0, P, p[0], p[1], p[2], gs)) tp = mid;
+                    else fp = mid;
+                }
+            } else {
+                for (int i = 0; i < iters; i++) {
+                    const double mid = (fp + tp) / 2;
+                    p[axis] = mid;
+                    // Lanes of one wave may sit in different bricks: no readfirstlane here.
+                    const bool in = konst >= 0 ? konst != 0
+                                               : eval_code<MODE, STATS, false>(code, ncode, P, p[0], p[1], p[2], gs);
+                    if (in) tp = mid;
+                    else fp = mid;
+                }
             }
             res = (fp + tp) / 2;
             inside = tp;

@@ -528,11 +528,12 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 // The scan of the counts whose per-block sums k_classify wrote.
-int scan4(mc3d_ctx *ctx, const uint32_t *in, uint2 *out, uint64_t n, unsigned long long *totals, uint32_t *list) {
+int scan4(mc3d_ctx *ctx, const uint32_t *in, uint2 *out, uint64_t n, unsigned long long *totals, uint32_t *list,
+          const Seg &seg0, bool bm) {
     const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     k_scan_tiles<<<1, SUMS_BLOCK, 0, ctx->stream>>>(ctx->tiles.as<uint4>(), (n + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK,
                                                     totals);
-    k_scan_down<<<(unsigned)ntiles, SCAN_BLOCK, 0, ctx->stream>>>(in, n, ctx->tiles.as<uint4>(), out, list);
+    k_scan_down<<<(unsigned)ntiles, SCAN_BLOCK, 0, ctx->stream>>>(in, n, ctx->tiles.as<uint4>(), out, list, seg0, bm);
     HIP_TRY(hipGetLastError());
     return MC3D_OK;
 }
@@ -547,6 +548,8 @@ struct Call {
     bool stats = false;
     bool interior = false;  // a call of one MarchingCubesInterior solid
     const uint64_t *keep = nullptr;  // a call of one filtered solid: its cube mask on the device
+    bool bm = false;     // the scan takes the words in brick-major order (one culled solid)
+    uint64_t nscan = 0;  // scan positions: words, or brick-major positions
 };
 
 template <int MODE, bool STATS, int NP, bool MULTI>
@@ -654,13 +657,13 @@ void launch_emit(mc3d_ctx *ctx, const Call &c, const Upload &U, MeshBuffers &mb,
                                                     ctx->etable.as<uint8_t>(), ctx->offs.as<uint2>(),
                                                     ctx->actw.as<uint32_t>(), U.ctl, ctx->vkeys.as<uint64_t>(),
                                                     ctx->vseg.as<uint32_t>(), mb.tris->as<uint32_t>(), nullptr, vcap,
-                                                    tcap);
+                                                    tcap, false);
     else
         k_emit<false><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.word_seg,
                                                      ctx->bits.as<uint64_t>(), ctx->etable.as<uint8_t>(),
                                                      ctx->offs.as<uint2>(), ctx->actw.as<uint32_t>(), U.ctl,
                                                      ctx->vkeys.as<uint64_t>(), nullptr, mb.tris->as<uint32_t>(), c.keep,
-                                                     vcap, tcap);
+                                                     vcap, tcap, c.bm);
 }
 
 // The output stages: emit (vertex keys, triangles) and bisect (vertices).
@@ -773,10 +776,18 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
         set_error("too many culling bricks");
         return MC3D_E_UNSUPPORTED;
     }
-    const unsigned ctiles = (unsigned)((c.nwords + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK);  // classify blocks
+    // One culled solid: the scan takes its words brick by brick (the
+    // positions include the rows of bricks past the grid).
+    c.nscan = c.nwords;
+    const int BY = jobs[0].B.BY, BZ = jobs[0].B.BZ;
+    if (c.cull && c.nsegs == 1 && !host && (BY & (BY - 1)) == 0 && (BZ & (BZ - 1)) == 0) {
+        const uint64_t n = c.nbricks * (uint64_t)BY * (uint64_t)BZ;
+        if (n < (1ull << 32)) c.bm = true, c.nscan = n;
+    }
+    const unsigned ctiles = (unsigned)((c.nscan + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK);  // classify blocks
     int rc;
-    if ((rc = ctx->bits.ensure(c.nwords * 8)) || (rc = ctx->counts.ensure(c.nwords * sizeof(uint32_t))) ||
-        (rc = ctx->offs.ensure(c.nwords * sizeof(uint2))) || (rc = ctx->actw.ensure(c.nwords * sizeof(uint32_t))) ||
+    if ((rc = ctx->bits.ensure(c.nwords * 8)) || (rc = ctx->counts.ensure(c.nscan * sizeof(uint32_t))) ||
+        (rc = ctx->offs.ensure(c.nscan * sizeof(uint2))) || (rc = ctx->actw.ensure(c.nwords * sizeof(uint32_t))) ||
         (rc = ctx->tiles.ensure((uint64_t)ctiles * sizeof(uint4))))
         return rc;
     if (c.cull && ((rc = ctx->binfo.ensure(c.nbricks * sizeof(uint2))) ||
@@ -832,15 +843,15 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
     if (c.nsegs > 1)
         k_classify<true><<<ctiles, CLASSIFY_BLOCK, 0, ctx->stream>>>(
             U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
-            ctx->counts.as<uint32_t>(), ctx->tiles.as<uint4>(), nullptr, c.nwords);
+            ctx->counts.as<uint32_t>(), ctx->tiles.as<uint4>(), nullptr, c.nwords, false);
     else
         k_classify<false><<<ctiles, CLASSIFY_BLOCK, 0, ctx->stream>>>(
             U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
-            ctx->counts.as<uint32_t>(), ctx->tiles.as<uint4>(), c.keep, c.nwords);
+            ctx->counts.as<uint32_t>(), ctx->tiles.as<uint4>(), c.keep, c.nscan, c.bm);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[8], ctx->stream));
-    if ((rc = scan4(ctx, ctx->counts.as<uint32_t>(), ctx->offs.as<uint2>(), c.nwords, U.ctl->totals,
-                    ctx->actw.as<uint32_t>())))
+    if ((rc = scan4(ctx, ctx->counts.as<uint32_t>(), ctx->offs.as<uint2>(), c.nscan, U.ctl->totals,
+                    ctx->actw.as<uint32_t>(), U.hsegs[0], c.bm)))
         return rc;
     if (c.nsegs > 1) {  // a single solid starts at vertex 0 and triangle 0 (zeroed by the upload)
         k_seg_ranges<<<grid_for(c.nsegs, 256), 256, 0, ctx->stream>>>(U.segs, c.nsegs, ctx->offs.as<uint2>(), U.sout);
