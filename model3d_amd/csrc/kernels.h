@@ -183,14 +183,18 @@ __device__ __forceinline__ void wave_sync() {
 // far beyond float64 rounding); "true" is proven for untransformed boxes and
 // bounds checks (exact comparisons) and spheres (farthest brick corner inside
 // the threshold with a relative margin far above the rounding of the sum).
-__device__ __forceinline__ int classify_atom(const double *f64, const AtomBox *boxes, uint32_t pc, Insn in,
-                                             const double *lo, const double *hi) {
-    const AtomBox bx = boxes[pc];
+// The program is read through the given pointer types (LDS, or constant
+// memory with scalar loads).
+template <class F64P, class BoxP>
+__device__ __forceinline__ int classify_atom(F64P f64, BoxP boxes, uint32_t pc, Insn in, const double *lo,
+                                             const double *hi) {
+    AtomBox bx;
+    for (int i = 0; i < 3; i++) bx.lo[i] = boxes[pc].lo[i], bx.hi[i] = boxes[pc].hi[i];
     if (hi[0] < bx.lo[0] || lo[0] > bx.hi[0] || hi[1] < bx.lo[1] || lo[1] > bx.hi[1] || hi[2] < bx.lo[2] ||
         lo[2] > bx.hi[2])
         return 0;
     if (insn_frame(in.x) == 0) {
-        const double *a = f64 + in.y;
+        const auto a = f64 + in.y;
         const uint32_t kind = insn_kind(in.x);
         if (kind == A_RECT) {
             if (in_lo(lo[0], a[0]) && in_lo(lo[1], a[1]) && in_lo(lo[2], a[2]) && in_hi(hi[0], a[3]) &&
@@ -217,12 +221,13 @@ constexpr size_t PLAN_STACK_BYTES = PLAN_BLOCK * PLAN_STACK * sizeof(uint32_t);
 // Folds the postfix program against one brick's box (lo, hi): a stack of
 // {false, true, expression starting at output position n}; the folded code
 // goes to o[0, cap) (`over` if it needs more).  S is the thread's stack.
-__device__ __forceinline__ void plan_fold(const Insn *code, const AtomBox *boxes, const double *f64, uint32_t ncode,
-                                          uint32_t *S, const double *lo, const double *hi, Insn *o, uint32_t cap,
-                                          uint32_t &n, bool &over) {
+template <class CodeP, class BoxP, class F64P>
+__device__ __forceinline__ void plan_fold(CodeP code, BoxP boxes, F64P f64, uint32_t ncode, uint32_t *S,
+                                          const double *lo, const double *hi, Insn *o, uint32_t cap, uint32_t &n,
+                                          bool &over) {
     int sp = 0;
     for (uint32_t pc = 0; pc < ncode; ++pc) {
-        const Insn in = code[pc];
+        const Insn in{code[pc].x, code[pc].y};  // field-wise: the pointer may be in any address space
         const uint32_t op = insn_op(in.x);
         if (op == OP_ATOM) {
             int c = classify_atom(f64, boxes, pc, in, lo, hi);
@@ -341,13 +346,15 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(const Seg *__restrict__ seg
             Insn *o = out + gb * cap;
             uint32_t n = 0;
             bool over = false;
-            // two copies of the fold, so that each reads with typed (LDS or
-            // global) instructions rather than generic ones
+            // two copies of the fold, so that each reads with typed
+            // instructions: LDS, or scalar loads for programs too large to
+            // stage (vector loads would also wait for the fold's stores)
             if (sg.stage)
                 plan_fold(sc, (const AtomBox *)(sc + P.ncode), (const double *)((const AtomBox *)(sc + P.ncode) + P.ncode),
                           P.ncode, S, lo, hi, o, cap, n, over);
             else
-                plan_fold(P.code, P.boxes, P.f64, P.ncode, S, lo, hi, o, cap, n, over);
+                plan_fold(scalar_ptr(P.code), scalar_ptr(P.boxes), scalar_ptr(P.f64), P.ncode, S, lo, hi, o, cap, n,
+                          over);
             const uint32_t root = S[0];
             uint2 bi;
             if (root == TAG_FALSE) bi = make_uint2(0, 0);
