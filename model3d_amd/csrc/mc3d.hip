@@ -651,7 +651,9 @@ int fit_outputs(mc3d_ctx *ctx, MeshBuffers &mb, uint64_t nv, uint64_t nt) {
 }
 
 void launch_emit(mc3d_ctx *ctx, const Call &c, const Upload &U, MeshBuffers &mb, uint64_t vcap, uint64_t tcap) {
-    const unsigned grid = grid_for(c.nwords, 4, 8192);  // ~4 listed words per wave: even tails
+    // 2048 blocks, up to 8192 for large grids (a few listed words per wave:
+    // even tails; each block also stages the case table)
+    const unsigned grid = std::max(2048u, grid_for(c.nwords, 128, 8192));
     if (c.nsegs > 1)
         k_emit<true><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(),
                                                     ctx->etable.as<uint8_t>(), ctx->offs.as<uint2>(),
