@@ -969,13 +969,14 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
                                               uint64_t *__restrict__ vkeys, uint32_t *__restrict__ vseg,
                                               uint32_t *__restrict__ tris, const uint64_t *__restrict__ keep_all,
                                               uint64_t vcap, uint64_t tcap, bool bm) {
-    __shared__ uint8_t tab[256 * 16];
+    __shared__ __align__(16) uint8_t tab[256 * 16];
     __shared__ WordInfo wi_all[4][8];
     __shared__ uint32_t s_pre[4][64];
     __shared__ uint8_t s_cs[4][64];
     // (table: the case table with each edge decoded to axis | dx << 2 |
     // dy << 3 | dz << 4, its offset from the cube's low corner)
-    for (int i = threadIdx.x; i < 256 * 16; i += blockDim.x) tab[i] = table[i];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x)  // 16 bytes a thread
+        reinterpret_cast<uint4 *>(tab)[i] = reinterpret_cast<const uint4 *>(table)[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = wave_id();
     WordInfo *wi = wi_all[wave];
