@@ -314,21 +314,20 @@ def load_traffic(workload_name, run, kernel):
 
 
 def copy_calibration():
-    """One device-to-device hipMemcpy of COPY_CALIBRATION_BYTES (a plain copy
-    kernel, the largest of its dispatches in the run), for the FETCH_SIZE /
-    WRITE_SIZE calibration of a rocprofv3 --pmc run."""
-    hip = ctypes.CDLL("libamdhip64.so")
-    n = ctypes.c_size_t(COPY_CALIBRATION_BYTES)
-    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
-    for p in (src, dst):
-        if hip.hipMalloc(ctypes.byref(p), n) != 0:
-            raise SystemExit("hipMalloc failed for the copy calibration")
-    ok = hip.hipMemset(src, 1, n) == 0 and hip.hipMemcpy(dst, src, n, 3) == 0  # hipMemcpyDeviceToDevice
-    ok = ok and hip.hipDeviceSynchronize() == 0
-    hip.hipFree(src)
-    hip.hipFree(dst)
-    if not ok:
-        raise SystemExit("copy calibration failed")
+    """Two plain streams of COPY_CALIBRATION_BYTES for the FETCH_SIZE /
+    WRITE_SIZE calibration of a rocprofv3 --pmc run: the library's
+    k_stream_read reads them once, k_stream_write writes them once (each
+    after an untimed filling pass).  Returns their bandwidths in GB/s."""
+    from model3d_amd import _lib
+
+    L = _lib.lib()
+    ctx = _lib.context(0)
+    out = {}
+    for name, write in (("read", 0), ("write", 1)):
+        ms = ctypes.c_double()
+        _lib.check(L.mc3d_diag_stream(ctx, COPY_CALIBRATION_BYTES, write, ctypes.byref(ms)), "mc3d_diag_stream")
+        out[name] = COPY_CALIBRATION_BYTES / (ms.value * 1e-3) / 1e9
+    return out
 
 
 def next_rows(work, run, steps=5):
@@ -463,10 +462,10 @@ def main():
                    waves_per_brick=tuned[best]["waves_per_brick"])
 
     if args.profile_run:
-        copy_calibration()
+        streams = copy_calibration()
         wall, st = run_gpu(work, run, args.steps, args.warmup)
         print(json.dumps({"workload": args.workload, "config": run, "ms_per_step": 1e3 * wall / args.steps,
-                          "stages_ms": {k: mean(st, k) for k in STAGE_KERNEL}}), flush=True)
+                          "stages_ms": {k: mean(st, k) for k in STAGE_KERNEL}, "stream_GBps": streams}), flush=True)
         return
 
     wall, st = run_gpu(work, run, args.steps, args.warmup)

@@ -42,6 +42,8 @@ enum {
     MC3D_E_UNSUPPORTED = 7,
     /* a host Contains callback reported failure */
     MC3D_E_CALLBACK = 8,
+    /* an exception inside the library (reported, never thrown across the ABI) */
+    MC3D_E_INTERNAL = 9,
 };
 
 typedef struct mc3d_ctx mc3d_ctx;
@@ -123,12 +125,15 @@ enum {
 typedef struct {
     int32_t precision;   /* MC3D_PREC_* */
     int32_t cull;        /* 1: per-brick interval culling of primitives */
-    int32_t device_only; /* 1: leave results on the device; mc3d_mesh_copy fetches them */
+    int32_t device_only; /* 1: leave results on the device; 0: copy them to pinned host memory
+                          * (mc3d_mesh_host_ptrs), the triangles while the bisection runs */
     int32_t count_mismatch; /* 1: also count float32 sign mismatches (slower; statistics only) */
     int32_t brick;       /* culling brick: 64 x brick x brick corners (0: default 4) */
     int32_t rows_per_walk; /* grid rows evaluated per walk of a brick program: 1, 2 or 4 (0: default 1) */
     int32_t waves_per_brick; /* wavefronts sharing one brick: 1, 2 or 4 (0: default 4) */
-    int32_t reserved;
+    int32_t no_graph;    /* 1: enqueue the stages one by one (per-stage timings in mc3d_stats); 0: a call
+                          * left on the device runs its chain as one HIP graph launch, captured once per
+                          * call shape and replayed (only ms_device is timed then) */
 } mc3d_options;
 
 typedef struct {
@@ -224,6 +229,12 @@ int mc3d_marching_cubes_search_batch(mc3d_ctx *ctx, const mc3d_solid *const *sol
 int mc3d_mesh_counts(const mc3d_mesh *m, size_t *n_vertices, size_t *n_triangles);
 /* Copies the results to host arrays of 3*V doubles and 3*T uint32. */
 int mc3d_mesh_copy(mc3d_mesh *m, double *verts, uint32_t *tris);
+/* The results of a mesh searched with device_only == 0, in the library's
+ * pinned host memory (3*V doubles, 3*T uint32; valid until mc3d_mesh_free):
+ * a caller that builds its own mesh objects (the reference's *Mesh of
+ * *Triangle, mc.go:20-35) reads them in place, with no further copy.
+ * MC3D_E_INVALID_ARG for a mesh left on the device. */
+int mc3d_mesh_host_ptrs(const mc3d_mesh *m, const double **verts, const uint32_t **tris);
 /* Device pointers of the results (owned by the mesh; valid until mc3d_mesh_free). */
 int mc3d_mesh_device_ptrs(const mc3d_mesh *m, const double **verts, const uint32_t **tris);
 int mc3d_mesh_stats(const mc3d_mesh *m, mc3d_stats *out);
@@ -258,6 +269,22 @@ int mc3d_contains(mc3d_ctx *ctx, const mc3d_solid *s, const double *pts, size_t 
  * 15 cube-edge ids (3 per triangle, CCW from outside).  Edge id e: axis
  * e/4 (0=x, 1=y, 2=z), and the two remaining corner bits in e%4. */
 int mc3d_case_table(uint8_t out[256 * 16]);
+
+/* Diagnostics: one pass of a plain streaming kernel over `bytes` of device
+ * memory -- k_stream_read reads every byte once, k_stream_write (write != 0)
+ * writes every byte once -- after an untimed pass that fills the buffer;
+ * *ms gets the timed pass's duration (HIP events).  Calibrates the HBM
+ * traffic counters (FETCH_SIZE, WRITE_SIZE) against known bytes and
+ * measures the bandwidth a plain stream reaches.  No counterpart in the
+ * reference. */
+int mc3d_diag_stream(mc3d_ctx *ctx, size_t bytes, int32_t write, double *ms);
+
+/* Diagnostics (host only, no GPU): folds random programs against random
+ * brick boxes with the brick plan's two folds -- every instruction in turn,
+ * and with subtrees whose atoms all miss the brick skipped -- and counts the
+ * cases whose folded code, root or overflow differ (must be 0).  No
+ * counterpart in the reference; it checks the library's own culling. */
+int mc3d_selftest_plan_fold(uint64_t seed, uint32_t programs, uint64_t *checked, uint64_t *mismatches);
 
 const char *mc3d_last_error(void);
 const char *mc3d_version(void);

@@ -22,6 +22,7 @@ MC3D_E_HIP = 5
 MC3D_E_OOM = 6
 MC3D_E_UNSUPPORTED = 7
 MC3D_E_CALLBACK = 8
+MC3D_E_INTERNAL = 9
 
 MC3D_XF_TRANSLATE = 1
 MC3D_XF_SCALE = 2
@@ -49,7 +50,7 @@ class mc3d_options(ctypes.Structure):
         ("brick", ctypes.c_int32),
         ("rows_per_walk", ctypes.c_int32),
         ("waves_per_brick", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("no_graph", ctypes.c_int32),
     ]
 
 
@@ -143,6 +144,7 @@ SIGNATURES = {
     "mc3d_mesh_counts": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "mc3d_mesh_copy": (ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p]),
     "mc3d_mesh_copy_interior": (ctypes.c_int, [_P, ctypes.c_void_p]),
+    "mc3d_mesh_host_ptrs": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]),
     "mc3d_stl_bytes": (ctypes.c_size_t, [ctypes.c_size_t]),
     "mc3d_marching_cubes_filter": (
         ctypes.c_int,
@@ -174,6 +176,10 @@ SIGNATURES = {
         [_P, _P, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32, ctypes.c_void_p],
     ),
     "mc3d_case_table": (ctypes.c_int, [ctypes.c_void_p]),
+    "mc3d_diag_stream": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
+    "mc3d_selftest_plan_fold": (
+        ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+    ),
     "mc3d_last_error": (ctypes.c_char_p, []),
     "mc3d_version": (ctypes.c_char_p, []),
 }

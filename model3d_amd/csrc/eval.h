@@ -18,7 +18,7 @@ struct DevProg {
     const double *__restrict__ f64;
     const float *__restrict__ f32;
     const Frame *__restrict__ frames;
-    const AtomBox *__restrict__ boxes;  // world-space box per pc (atoms only)
+    const PlanNode *__restrict__ plan;  // per pc, the node record of the brick plan
     const DevCollider *__restrict__ colliders;
     // Answers of host-evaluated atoms: host_vals[atom * host_stride + point id].
     const uint8_t *__restrict__ host_vals;
@@ -53,7 +53,7 @@ __device__ __forceinline__ DevProg as_global(DevProg P) {
     P.f64 = global_ptr(P.f64);
     P.f32 = global_ptr(P.f32);
     P.frames = global_ptr(P.frames);
-    P.boxes = global_ptr(P.boxes);
+    P.plan = global_ptr(P.plan);
     P.colliders = global_ptr(P.colliders);
     P.host_vals = global_ptr(P.host_vals);
     return P;
@@ -62,8 +62,8 @@ __device__ __forceinline__ DevProg as_global(DevProg P) {
 // Go math.Min(c, m) == m  <=>  m == -Inf || c >= m   (and the Max twin); this
 // is Rect.Contains / InBounds (shapes.go:171-173, bounder.go:23-27) including
 // the NaN and signed-zero cases.
-__device__ __forceinline__ bool in_lo(double c, double m) { return m == -INFINITY || c >= m; }
-__device__ __forceinline__ bool in_hi(double c, double m) { return m == INFINITY || c <= m; }
+MC3D_HD inline bool in_lo(double c, double m) { return m == -INFINITY || c >= m; }
+MC3D_HD inline bool in_hi(double c, double m) { return m == INFINITY || c <= m; }
 
 // ColliderSolid.Contains with inset 0 (solid.go:292-300): InBounds, then the
 // parity of ray hits along the fixed direction (collisions.go:119-134).  Only

@@ -1,0 +1,220 @@
+// k_stages.hip -- launchers of the grid and compaction stages: the brick
+// plan, classify and its scan, emit, the per-solid output ranges, the
+// MarchingCubesConj vertex map and the STL encoder.
+#include "kernels.h"
+#include "launch.h"
+
+namespace mc3d {
+
+// One block: exclusive scan of the classify blocks' sums in place, SCAN_ITEMS
+// per thread; totals in 64 bits.
+__global__ __launch_bounds__(SUMS_BLOCK) void k_scan_tiles(uint4 *__restrict__ part_sums, uint64_t nparts,
+                                                           unsigned long long *__restrict__ totals) {
+    __shared__ uint4 sh[SUMS_BLOCK / 64];
+    uint4 carry = make_uint4(0, 0, 0, 0);
+    unsigned long long t0 = 0, t1 = 0, t2 = 0;
+    for (uint64_t start = 0; start < nparts; start += (uint64_t)SUMS_BLOCK * SCAN_ITEMS) {
+        const uint64_t base = start + (uint64_t)threadIdx.x * SCAN_ITEMS;
+        uint4 vals[SCAN_ITEMS];
+        uint4 s = make_uint4(0, 0, 0, 0);
+        for (int i = 0; i < SCAN_ITEMS; i++) {
+            vals[i] = base + i < nparts ? part_sums[base + i] : make_uint4(0, 0, 0, 0);
+            s = add4(s, vals[i]);
+        }
+        uint4 total;
+        uint4 ex = add4(block_exclusive<SUMS_BLOCK>(s, sh, total), carry);
+        for (int i = 0; i < SCAN_ITEMS; i++) {
+            if (base + i < nparts) part_sums[base + i] = ex;
+            ex = add4(ex, vals[i]);
+        }
+        carry = add4(carry, total);
+        t0 += total.x, t1 += total.y, t2 += total.z;
+    }
+    if (threadIdx.x == 0) totals[0] = t0, totals[1] = t1, totals[2] = t2;
+}
+
+// Downsweep over tiles of SCAN_TILE words (the exclusive scan of the
+// classify blocks' sums gives each tile's offset); words with .z set are also
+// listed at their scanned .z.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *__restrict__ in, uint64_t n,
+                                                          const uint4 *__restrict__ part_sums,
+                                                          uint2 *__restrict__ out, uint32_t *__restrict__ list,
+                                                          const Seg seg0, bool bm) {
+    __shared__ uint4 sh[SCAN_BLOCK / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint4 vals[SCAN_ITEMS];
+    uint4 s = make_uint4(0, 0, 0, 0);
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const uint32_t c = base + i < n ? in[base + i] : 0u;
+        vals[i] = make_uint4(c & 0xffu, (c >> 8) & 0x1ffu, c >> 17, 0u);
+        s = add4(s, vals[i]);
+    }
+    uint4 total;
+    uint4 ex = add4(block_exclusive(s, sh, total), part_sums[(uint64_t)blockIdx.x * (SCAN_TILE / CLASSIFY_BLOCK)]);
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        if (base + i < n) {
+            out[base + i] = make_uint2(ex.x, ex.y);
+            if (list && vals[i].z) {
+                uint32_t w = (uint32_t)(base + i);
+                if (bm) {  // brick-major position -> word
+                    int wx, y, z;
+                    bm_word(seg0.g, seg0.B, w, wx, y, z);
+                    w = (uint32_t)word_index(seg0.g, wx, y, z);
+                }
+                list[ex.z] = w;
+            }
+        }
+        ex = add4(ex, vals[i]);
+    }
+}
+
+// Each solid's first vertex and triangle in the call's outputs.
+__global__ __launch_bounds__(256) void k_seg_ranges(const Seg *__restrict__ segs, uint32_t nsegs,
+                                                    const uint2 *__restrict__ offs, SegOut *__restrict__ sout) {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nsegs; s += gridDim.x * blockDim.x) {
+        const uint2 o = offs[segs[s].word0];
+        sout[s].vbase = o.x;
+        sout[s].tbase = o.y;
+    }
+}
+
+// Mesh.Transform(t.Inverse()) of MarchingCubesConj (mc.go:66-71,
+// mesh.go:613-642): every vertex mapped by the inverse chain, given as frame
+// records {kind, v[9]} (the Apply of each op, transform.go:44-46, 71-73,
+// 103-105, 129-131).
+__global__ __launch_bounds__(256) void k_inverse_apply(const double *__restrict__ recs, uint32_t nops,
+                                                       double *__restrict__ verts, const Ctl *__restrict__ ctl,
+                                                       uint64_t vcap) {
+    const uint64_t nv = min((uint64_t)ctl->totals[0], vcap);
+    const Frame f{0, nops};
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (uint64_t)gridDim.x * blockDim.x) {
+        double x = verts[3 * v], y = verts[3 * v + 1], z = verts[3 * v + 2];
+        apply_frame(recs, f, x, y, z);
+        verts[3 * v] = x, verts[3 * v + 1] = y, verts[3 * v + 2] = z;
+    }
+}
+
+// Binary STL body (export.go:37-66, fileformats/stl.go:41-58): per triangle
+// the float32 cast of Triangle.Normal() -- the cross product of (t1 - t0) and
+// (t2 - t0), scaled by 1 / Norm() in float64 (primitives.go:27-33,
+// coords.go:201-222, 379-381) -- then the float32 casts of the three
+// vertices and two zero bytes: 50 bytes, written as 25 little-endian
+// halfwords (a record starts at an even offset).
+__global__ __launch_bounds__(256) void k_encode_stl(const double *__restrict__ verts, const uint32_t *__restrict__ tris,
+                                                    uint64_t nt, uint16_t *__restrict__ body) {
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += (uint64_t)gridDim.x * blockDim.x) {
+        double p[3][3];
+        for (int k = 0; k < 3; k++) {
+            const uint64_t v = tris[3 * t + k];
+            p[k][0] = verts[3 * v], p[k][1] = verts[3 * v + 1], p[k][2] = verts[3 * v + 2];
+        }
+        const double ax = p[1][0] - p[0][0], ay = p[1][1] - p[0][1], az = p[1][2] - p[0][2];
+        const double bx = p[2][0] - p[0][0], by = p[2][1] - p[0][1], bz = p[2][2] - p[0][2];
+        const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+        const double s = 1 / sqrt(cx * cx + cy * cy + cz * cz);
+        float rec[12] = {(float)(cx * s), (float)(cy * s), (float)(cz * s)};
+        for (int k = 0; k < 3; k++)
+            for (int d = 0; d < 3; d++) rec[3 + 3 * k + d] = (float)p[k][d];
+        uint16_t *o = body + 25 * t;
+        for (int w = 0; w < 12; w++) {
+            const uint32_t u = __float_as_uint(rec[w]);
+            o[2 * w] = (uint16_t)(u & 0xffffu);
+            o[2 * w + 1] = (uint16_t)(u >> 16);
+        }
+        o[24] = 0;
+    }
+}
+
+
+// Streams of known size for the calibration of the HBM traffic counters
+// (and a measure of the achievable bandwidth): k_stream_read reads n 16-byte
+// items once and writes one word per block; k_stream_write writes n items.
+__global__ __launch_bounds__(256) void k_stream_read(const uint4 *__restrict__ in, uint64_t n,
+                                                     uint32_t *__restrict__ out) {
+    uint32_t acc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint4 v = in[i];
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    for (int d = 32; d; d >>= 1) acc ^= __shfl_xor(acc, d);
+    __shared__ uint32_t red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0] ^ red[1] ^ red[2] ^ red[3];
+}
+
+__global__ __launch_bounds__(256) void k_stream_write(uint4 *__restrict__ out, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = make_uint4(v, v + 1, v + 2, (uint32_t)i);
+}
+
+hipError_t launch_stream(bool write, void *buf, uint64_t items, uint32_t *out, hipStream_t st) {
+    const unsigned grid = 8192;
+    if (write)
+        k_stream_write<<<grid, 256, 0, st>>>((uint4 *)buf, items, 7u);
+    else
+        k_stream_read<<<grid, 256, 0, st>>>((const uint4 *)buf, items, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_plan(const Pipe &p, const Bufs &b, hipStream_t st) {
+    // about one wave per unit of bricks (a grid-stride loop takes the rest)
+    const unsigned pgrid = grid_for((p.nbricks + p.plan_gsz - 1) / p.plan_gsz, PLAN_BLOCK / 64, 1u << 16);
+    if (p.nsegs > 1)
+        k_plan<true><<<pgrid, PLAN_BLOCK, p.plan_lds, st>>>(p.segs, p.nsegs, p.seg0, p.brick_seg, p.nbricks, p.cap,
+                                                            b.binfo, b.bcode, b.actb, b.bits, p.ctl, p.sout,
+                                                            p.plan_gsz, false);
+    else
+        k_plan<false><<<pgrid, PLAN_BLOCK, p.plan_lds, st>>>(p.segs, p.nsegs, p.seg0, p.brick_seg, p.nbricks, p.cap,
+                                                             b.binfo, b.bcode, b.actb, b.bits, p.ctl, p.sout,
+                                                             p.plan_gsz, p.plan_tile);
+    return hipGetLastError();
+}
+
+hipError_t launch_classify(const Pipe &p, const Bufs &b, hipStream_t st) {
+    const unsigned ctiles = (unsigned)((p.nscan + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK);
+    if (p.nsegs > 1)
+        k_classify<true><<<ctiles, CLASSIFY_BLOCK, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits, b.table,
+                                                            b.counts, b.tiles, nullptr, p.nwords, false);
+    else
+        k_classify<false><<<ctiles, CLASSIFY_BLOCK, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits, b.table,
+                                                             b.counts, b.tiles, b.keep, p.nscan, p.bm);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(const Pipe &p, const Bufs &b, hipStream_t st) {
+    // the scan of the counts whose per-block sums classify wrote
+    const uint64_t ntiles = (p.nscan + SCAN_TILE - 1) / SCAN_TILE;
+    k_scan_tiles<<<1, SUMS_BLOCK, 0, st>>>(b.tiles, (p.nscan + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK, p.ctl->totals);
+    k_scan_down<<<(unsigned)ntiles, SCAN_BLOCK, 0, st>>>(b.counts, p.nscan, b.tiles, b.offs, b.actw, p.seg0, p.bm);
+    if (p.nsegs > 1)  // a single solid starts at vertex 0 and triangle 0 (zeroed by the upload)
+        k_seg_ranges<<<grid_for(p.nsegs, 256), 256, 0, st>>>(p.segs, p.nsegs, b.offs, p.sout);
+    return hipGetLastError();
+}
+
+hipError_t launch_emit(const Pipe &p, const Bufs &b, uint32_t *tris, uint64_t vcap, uint64_t tcap, hipStream_t st) {
+    // a wave takes EMIT_G listed words at a time: 512 to 4096 blocks of four
+    // waves, from the grid's words (each block also stages the case table)
+    const unsigned grid = std::max(512u, grid_for(p.nwords, p.emit_div, 4096));
+    if (p.nsegs > 1)
+        k_emit<true><<<grid, 256, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits, b.etable, b.offs, b.actw,
+                                           p.ctl, b.vkeys, b.vseg, tris, nullptr, vcap, tcap, false);
+    else
+        k_emit<false><<<grid, 256, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits, b.etable, b.offs, b.actw,
+                                            p.ctl, b.vkeys, nullptr, tris, b.keep, vcap, tcap, p.bm);
+    return hipGetLastError();
+}
+
+hipError_t launch_inverse_apply(const double *recs, uint32_t nops, double *verts, const Ctl *ctl, uint64_t vcap,
+                                hipStream_t st) {
+    k_inverse_apply<<<grid_for(vcap, 256, 1u << 14), 256, 0, st>>>(recs, nops, verts, ctl, vcap);
+    return hipGetLastError();
+}
+
+hipError_t launch_encode_stl(const double *verts, const uint32_t *tris, uint64_t nt, uint16_t *body,
+                             hipStream_t st) {
+    k_encode_stl<<<grid_for(nt, 256, 1u << 16), 256, 0, st>>>(verts, tris, nt, body);
+    return hipGetLastError();
+}
+
+}  // namespace mc3d

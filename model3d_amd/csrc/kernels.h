@@ -35,65 +35,11 @@
 
 #include <type_traits>
 
-#include "eval.h"
+#include "pipeline.h"
+#include "plan.h"
 
 namespace mc3d {
 
-constexpr uint32_t BRICK_TRUE = 0xffffffffu;  // info.y of a brick where the solid is true
-constexpr uint32_t BRICK_FULL = 0xffffffffu;  // info.x of a brick that uses the unfolded program
-
-// Per-call control block in device memory (zeroed by the upload).
-struct Ctl {
-    uint32_t inside;       // panic("solid is true outside of bounds") seen
-    uint32_t overflow;     // an output index exceeded the buffers' capacity
-    uint32_t nact_bricks;  // bricks whose folded program is not constant
-    uint32_t pad;
-    unsigned long long totals[4];  // scan totals: vertices, triangles, listed words
-    unsigned long long gstats[4];  // guard statistics (GStats order)
-    unsigned long long flops[4];   // statistics runs: float64, float32 ops of corners; the same of bisect
-    unsigned long long code_total; // instructions over all active bricks' folded programs
-    uint32_t code_max;             // longest folded program
-    uint32_t full_bricks;          // bricks whose folded program overflowed its slot
-};
-
-struct Grid {
-    const double *__restrict__ xs;
-    const double *__restrict__ ys;
-    const double *__restrict__ zs;
-    int nx, ny, nz, W;  // W = words per row = ceil(nx / 64)
-};
-
-// Per-brick programs: info[b] = {offset into code (or BRICK_FULL), length};
-// length 0: the solid is false everywhere in the brick; BRICK_TRUE: true.
-struct Bricks {
-    const uint2 *__restrict__ info;  // null: no culling, use the full program
-    const Insn *__restrict__ code;
-    int BY, BZ, nby, nbz;
-};
-
-// One solid of a call.  B.info / B.code are the call's arrays, indexed by the
-// global brick number; word and brick numbers are local to the segment
-// unless named global.  Kernels also take the first segment by value (a
-// call of one solid reads it from the kernel arguments, no load).
-struct Seg {
-    DevProg P;
-    Grid g;
-    Bricks B;
-    uint64_t word0, nwords;    // global occupancy words [word0, word0 + nwords); word0 % 64 == 0
-    uint64_t brick0, nbricks;  // global bricks; brick0 % 128 == 0
-    int32_t iters;
-    int32_t stage;  // the plan may stage this segment's program in LDS
-};
-
-// Per-solid results, copied back with the control block.
-struct SegOut {
-    unsigned long long vbase, tbase;  // first vertex / triangle of the solid in the call's outputs
-    uint32_t inside;                  // panic("solid is true outside of bounds") for this solid
-    uint32_t pad[3];
-};
-
-constexpr int SEG_WORD_SHIFT = 6;   // segment table entry per 64 words
-constexpr int SEG_BRICK_SHIFT = 7;  // and per 128 bricks
 
 // The segment of a wave's items (all its items share one table entry).
 // MULTI: the call has several segments; otherwise segment 0, whose copy in
@@ -178,183 +124,109 @@ __device__ __forceinline__ void wave_sync() {
 // ---------------------------------------------------------------------------
 // plan
 // ---------------------------------------------------------------------------
-// Atom against a brick box: 0 certainly false, 1 certainly true, 2 evaluate.
-// "False" needs only the atom's conservative world box (solid.cpp widens it
-// far beyond float64 rounding); "true" is proven for untransformed boxes and
-// bounds checks (exact comparisons) and spheres (farthest brick corner inside
-// the threshold with a relative margin far above the rounding of the sum).
-// The program is read through the given pointer types (LDS, or constant
-// memory with scalar loads).
-template <class F64P, class BoxP>
-__device__ __forceinline__ int classify_atom(F64P f64, BoxP boxes, uint32_t pc, Insn in, const double *lo,
-                                             const double *hi) {
-    AtomBox bx;
-    for (int i = 0; i < 3; i++) bx.lo[i] = boxes[pc].lo[i], bx.hi[i] = boxes[pc].hi[i];
-    if (hi[0] < bx.lo[0] || lo[0] > bx.hi[0] || hi[1] < bx.lo[1] || lo[1] > bx.hi[1] || hi[2] < bx.lo[2] ||
-        lo[2] > bx.hi[2])
-        return 0;
-    if (insn_frame(in.x) == 0) {
-        const auto a = f64 + in.y;
-        const uint32_t kind = insn_kind(in.x);
-        if (kind == A_RECT) {
-            if (in_lo(lo[0], a[0]) && in_lo(lo[1], a[1]) && in_lo(lo[2], a[2]) && in_hi(hi[0], a[3]) &&
-                in_hi(hi[1], a[4]) && in_hi(hi[2], a[5]))
-                return 1;
-        } else if (kind == A_SPHERE) {
-            double s = 0;
-            for (int i = 0; i < 3; i++) {
-                double m = fmax(fabs(lo[i] - a[i]), fabs(hi[i] - a[i]));
-                s += m * m;
-            }
-            if (s * (1 + 1e-12) <= a[4]) return 1;
-        }
-    }
-    return 2;
+// One thread per brick folds the postfix program (plan_fold_skip: a subtree
+// whose atoms miss every brick of the wave is replaced by its constant
+// without a walk, so the walk stays the same for the whole wave; a wave's
+// bricks are consecutive, so its constant bricks' words are written row by
+// row) and writes
+// the folded program into the brick's slot of `cap` instructions.  A brick
+// whose folded code ever needs more than `cap` slots uses the unfolded
+// program.  A brick where the solid is constant gets its occupancy words
+// here (all false, or every valid corner true, with the check that no
+// boundary corner is inside: panic("solid is true outside of bounds"),
+// mc.go:687-689).  With `stage`, the program's node records are first copied
+// to LDS; otherwise the uniform walk reads them with scalar loads.  The per-brick statistics are reduced per wave.
+// Bricks a plan wave takes at once: its first `gsz` lanes (a power of two
+// <= 64) hold `gsz` consecutive bricks, or with `tile` (a call of one
+// solid) a tile of one word column by ty brick rows by gsz / ty brick
+// planes, ty the power of two nearest sqrt(gsz).  The fewer and closer a
+// wave's bricks, the more subtrees miss all of them; with fewer bricks per
+// wave more waves are in flight, which is what a latency-bound walk needs.
+template <bool MULTI>
+__device__ __forceinline__ uint64_t plan_units(const Seg &s0, uint64_t nbricks_all, uint32_t gsz, bool tile) {
+    if (MULTI || !tile) return (nbricks_all + gsz - 1) / gsz;
+    const uint32_t ty = 1u << ((31 - __clz(gsz)) / 2), tz = gsz / ty;
+    return (uint64_t)s0.g.W * ((s0.B.nby + ty - 1) / ty) * ((s0.B.nbz + tz - 1) / tz);
 }
 
-constexpr uint32_t TAG_EXPR = 0u, TAG_FALSE = 1u << 30, TAG_TRUE = 2u << 30, TAG_MASK = 3u << 30;
-constexpr int PLAN_BLOCK = 128;
-constexpr int PLAN_STACK = 33;
-
-constexpr size_t PLAN_STACK_BYTES = PLAN_BLOCK * PLAN_STACK * sizeof(uint32_t);
-
-// Folds the postfix program against one brick's box (lo, hi): a stack of
-// {false, true, expression starting at output position n}; the folded code
-// goes to o[0, cap) (`over` if it needs more).  S is the thread's stack.
-template <class CodeP, class BoxP, class F64P>
-__device__ __forceinline__ void plan_fold(CodeP code, BoxP boxes, F64P f64, uint32_t ncode, uint32_t *S,
-                                          const double *lo, const double *hi, Insn *o, uint32_t cap, uint32_t &n,
-                                          bool &over) {
-    int sp = 0;
-    for (uint32_t pc = 0; pc < ncode; ++pc) {
-        const Insn in{code[pc].x, code[pc].y};  // field-wise: the pointer may be in any address space
-        const uint32_t op = insn_op(in.x);
-        if (op == OP_ATOM) {
-            int c = classify_atom(f64, boxes, pc, in, lo, hi);
-            if (c == 2) {
-                if (n < cap) o[n] = in;
-                else over = true;
-                S[sp++] = TAG_EXPR | n;
-                n++;
-            } else {
-                S[sp++] = c ? TAG_TRUE : TAG_FALSE;
-            }
-        } else if (op == OP_TRUE || op == OP_FALSE) {
-            S[sp++] = op == OP_TRUE ? TAG_TRUE : TAG_FALSE;
-        } else if (op == OP_NOT) {
-            uint32_t a = S[sp - 1];
-            if (a == TAG_TRUE) S[sp - 1] = TAG_FALSE;
-            else if (a == TAG_FALSE) S[sp - 1] = TAG_TRUE;
-            else {
-                if (n < cap) o[n] = in;
-                else over = true;
-                n++;
-            }
-        } else {
-            const uint32_t bb = S[--sp], aa = S[--sp];
-            const bool aE = (aa & TAG_MASK) == TAG_EXPR, bE = (bb & TAG_MASK) == TAG_EXPR;
-            const bool aT = aa == TAG_TRUE, aF = aa == TAG_FALSE, bT = bb == TAG_TRUE, bF = bb == TAG_FALSE;
-            // start of the operands' code (constants emitted nothing)
-            const uint32_t first = aE ? (aa & ~TAG_MASK) : (bE ? (bb & ~TAG_MASK) : n);
-            uint32_t res;
-            bool emit = false, emit_not = false;
-            if (op == OP_OR) {
-                if (aT || bT) res = TAG_TRUE, n = first;
-                else if (aF) res = bb;
-                else if (bF) res = aa;
-                else res = aa, emit = true;
-            } else if (op == OP_AND) {
-                if (aF || bF) res = TAG_FALSE, n = first;
-                else if (aT) res = bb;
-                else if (bT) res = aa;
-                else res = aa, emit = true;
-            } else if (op == OP_ANDNOT) {  // a & !b
-                if (aF || bT) res = TAG_FALSE, n = first;
-                else if (bF) res = aa;
-                else if (aT) res = bb, emit_not = true;  // b's code is last
-                else res = aa, emit = true;
-            } else {  // OP_RANDNOT: !a & b
-                if (aT || bF) res = TAG_FALSE, n = first;
-                else if (aF) res = bb;
-                else if (bT) res = aa, emit_not = true;  // a's code is last
-                else res = aa, emit = true;
-            }
-            if (emit || emit_not) {
-                if (n < cap) o[n] = emit ? in : Insn{OP_NOT, 0};
-                else over = true;
-                n++;
-            }
-            S[sp++] = res;
-        }
-    }
-}
-
-
-
-// One thread per brick walks the postfix program with a stack of
-// {false, true, expression starting at output position n} and writes the
-// folded program into the brick's slot of `cap` instructions.  A brick whose
-// folded code ever needs more than `cap` slots uses the unfolded program.
-// A brick where the solid is constant gets its occupancy words here (all
-// false, or every valid corner true, with the check that no boundary corner
-// is inside: panic("solid is true outside of bounds"), mc.go:687-689); the
-// stores cost little in a kernel whose waves mostly wait on the fold.
-// With `stage`, the program, its atom boxes and its params are first copied
-// to LDS.  The per-brick statistics are reduced per wave.
 template <bool MULTI>
 __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
                                                      const uint32_t *__restrict__ brick_seg, uint64_t nbricks_all,
                                                      uint32_t cap, uint2 *__restrict__ info, Insn *__restrict__ out,
                                                      uint32_t *__restrict__ act_list, uint64_t *__restrict__ bits_all,
-                                                     Ctl *__restrict__ ctl, SegOut *__restrict__ sout) {
+                                                     Ctl *__restrict__ ctl, SegOut *__restrict__ sout, uint32_t gsz,
+                                                     bool tile) {
     extern __shared__ __align__(16) char plan_smem[];
     uint32_t *S = (uint32_t *)plan_smem + threadIdx.x * PLAN_STACK;
-    Insn *sc = (Insn *)(plan_smem + PLAN_STACK_BYTES);
     uint32_t staged = ~0u;  // segment whose program is in LDS
-    const int lane = threadIdx.x & 63;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t b_end = (nbricks_all + blockDim.x - 1) / blockDim.x * blockDim.x;  // whole waves take part
-    for (uint64_t gb = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gb < b_end; gb += stride) {
-        // A block's 128 bricks share one segment (ranges aligned to 128).
-        const uint32_t si = wave_seg<MULTI>(brick_seg, gb - threadIdx.x, SEG_BRICK_SHIFT);
+    const int lane = threadIdx.x & 63, wave = wave_id();
+    constexpr int WPB = PLAN_BLOCK / 64;
+    tile = tile && !MULTI;
+    const uint64_t nunits = plan_units<MULTI>(seg0, nbricks_all, gsz, tile);
+    const uint64_t uend = (nunits + WPB - 1) / WPB * WPB;  // every wave of a block runs the same iterations
+    for (uint64_t u0 = (uint64_t)blockIdx.x * WPB; u0 < uend; u0 += (uint64_t)gridDim.x * WPB) {
+        const uint64_t u = u0 + wave;
+        // this lane's brick
+        uint64_t gb;
+        bool valid;
+        if (tile) {
+            const uint32_t ty = 1u << ((31 - __clz(gsz)) / 2), tz = gsz / ty;
+            const uint32_t W = (uint32_t)seg0.g.W, nty = (uint32_t)(seg0.B.nby + ty - 1) / ty;
+            const uint32_t bx = (uint32_t)(u % W), t = (uint32_t)(u / W);
+            const uint32_t by = (t % nty) * ty + (uint32_t)lane % ty, bz = (t / nty) * tz + (uint32_t)lane / ty;
+            valid = (uint32_t)lane < gsz && u < nunits && by < (uint32_t)seg0.B.nby && bz < (uint32_t)seg0.B.nbz;
+            gb = valid ? ((uint64_t)bz * seg0.B.nby + by) * W + bx : 0;
+        } else {
+            gb = u * gsz + lane;
+            valid = (uint32_t)lane < gsz && gb < nbricks_all;
+        }
+        // A block's bricks share one segment (ranges aligned to 128; a block
+        // takes 2 gsz <= 128 consecutive bricks).
+        const uint32_t si = wave_seg<MULTI>(brick_seg, u0 * gsz, SEG_BRICK_SHIFT);
         const Seg sg = pick_seg<MULTI>(segs, seg0, si);
         const DevProg &P = sg.P;
         const Grid &g = sg.g;
         const Bricks &B = sg.B;
+        // a staged program's node records in LDS
+        PlanNode *sn = (PlanNode *)(plan_smem + PLAN_STACK_BYTES);
         if (sg.stage) {  // block-uniform
-            AtomBox *sb = (AtomBox *)(sc + P.ncode);
-            double *sf = (double *)(sb + P.ncode);
             if (staged != si) {
                 __syncthreads();  // the previous segment's walks are done with the LDS copy
-                for (uint32_t i = threadIdx.x; i < P.ncode; i += blockDim.x) sc[i] = P.code[i], sb[i] = P.boxes[i];
-                for (uint32_t i = threadIdx.x; i < P.nf64; i += blockDim.x) sf[i] = P.f64[i];
+                const uint4 *src = (const uint4 *)P.plan;
+                uint4 *dst = (uint4 *)sn;
+                for (uint32_t i = threadIdx.x; i < P.ncode * (sizeof(PlanNode) / 16); i += blockDim.x) dst[i] = src[i];
                 __syncthreads();
                 staged = si;
             }
         }
         const uint64_t b = gb - sg.brick0;  // local brick
+        valid = valid && b < sg.nbricks;
         bool active = false, full = false;
         uint32_t len = 0;
-        if (gb < nbricks_all && b < sg.nbricks) {
-            const uint32_t r = (uint32_t)b / (uint32_t)g.W;
-            const int bx = (int)((uint32_t)b - r * (uint32_t)g.W);
-            const int bz = (int)(r / (uint32_t)B.nby), by = (int)(r - (uint32_t)bz * (uint32_t)B.nby);
-            const int x0 = bx * 64, y0 = by * B.BY, z0 = bz * B.BZ;
-            const int x1 = min(x0 + 64, g.nx - 1), y1 = min(y0 + B.BY, g.ny - 1), z1 = min(z0 + B.BZ, g.nz - 1);
-            const double lo[3] = {g.xs[x0], g.ys[y0], g.zs[z0]};
-            const double hi[3] = {g.xs[x1], g.ys[y1], g.zs[z1]};
-            Insn *o = out + gb * cap;
-            uint32_t n = 0;
-            bool over = false;
-            // two copies of the fold, so that each reads with typed
-            // instructions: LDS, or scalar loads for programs too large to
-            // stage (vector loads would also wait for the fold's stores)
-            if (sg.stage)
-                plan_fold(sc, (const AtomBox *)(sc + P.ncode), (const double *)((const AtomBox *)(sc + P.ncode) + P.ncode),
-                          P.ncode, S, lo, hi, o, cap, n, over);
-            else
-                plan_fold(scalar_ptr(P.code), scalar_ptr(P.boxes), scalar_ptr(P.f64), P.ncode, S, lo, hi, o, cap, n,
-                          over);
+        // the brick's box (a lane without a brick takes the first brick's
+        // and keeps its output slot empty)
+        const uint64_t bl = valid ? b : 0;
+        const uint32_t r = (uint32_t)bl / (uint32_t)g.W;
+        const int bx = (int)((uint32_t)bl - r * (uint32_t)g.W);
+        const int bz = (int)(r / (uint32_t)B.nby), by = (int)(r - (uint32_t)bz * (uint32_t)B.nby);
+        const int x0 = bx * 64, y0 = by * B.BY, z0 = bz * B.BZ;
+        const int x1 = min(x0 + 64, g.nx - 1), y1 = min(y0 + B.BY, g.ny - 1), z1 = min(z0 + B.BZ, g.nz - 1);
+        const double lo[3] = {g.xs[x0], g.ys[y0], g.zs[z0]};
+        const double hi[3] = {g.xs[x1], g.ys[y1], g.zs[z1]};
+        Insn *o = out + (valid ? gb * cap : 0);
+        const uint32_t ocap = valid ? cap : 0u;
+        uint32_t n = 0;
+        bool over = false;
+        // two copies of the fold, so that each reads with typed
+        // instructions: LDS, or scalar loads for programs too large to stage
+        if (sg.stage) {
+            auto miss = [&](const PlanNode &t) { return __ballot(valid && !node_misses(t, lo, hi)) == 0; };
+            plan_fold_nodes((const PlanNode *)sn, P.ncode, S, lo, hi, o, ocap, n, over, miss);
+        } else {
+            auto miss = [&](const PlanNode &t) { return __ballot(valid && !node_misses(t, lo, hi)) == 0; };
+            plan_fold_nodes(scalar_ptr(P.plan), P.ncode, S, lo, hi, o, ocap, n, over, miss);
+        }
+        if (valid) {
             const uint32_t root = S[0];
             uint2 bi;
             if (root == TAG_FALSE) bi = make_uint2(0, 0);
@@ -660,12 +532,6 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
 // exclusive scan of uint4 counts (block sums written by classify, a scan of
 // those, downsweep)
 // ---------------------------------------------------------------------------
-constexpr int SCAN_BLOCK = 256;
-constexpr int SCAN_ITEMS = 8;
-constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
-constexpr int CLASSIFY_BLOCK = 1024;  // words per classify block and per block sum
-constexpr int SUMS_BLOCK = 1024;      // threads of the one block scanning the block sums
-static_assert(SCAN_TILE % CLASSIFY_BLOCK == 0, "a scan tile starts at a classify block");
 
 __device__ __forceinline__ uint4 add4(uint4 a, uint4 b) { return make_uint4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
 
@@ -878,67 +744,7 @@ __global__ __launch_bounds__(CLASSIFY_BLOCK) void k_classify(const Seg *__restri
     if (threadIdx.x == 0) part_sums[blockIdx.x] = t;
 }
 
-// One block: exclusive scan of the classify blocks' sums in place, SCAN_ITEMS
-// per thread; totals in 64 bits.
-__global__ __launch_bounds__(SUMS_BLOCK) void k_scan_tiles(uint4 *__restrict__ part_sums, uint64_t nparts,
-                                                           unsigned long long *__restrict__ totals) {
-    __shared__ uint4 sh[SUMS_BLOCK / 64];
-    uint4 carry = make_uint4(0, 0, 0, 0);
-    unsigned long long t0 = 0, t1 = 0, t2 = 0;
-    for (uint64_t start = 0; start < nparts; start += (uint64_t)SUMS_BLOCK * SCAN_ITEMS) {
-        const uint64_t base = start + (uint64_t)threadIdx.x * SCAN_ITEMS;
-        uint4 vals[SCAN_ITEMS];
-        uint4 s = make_uint4(0, 0, 0, 0);
-        for (int i = 0; i < SCAN_ITEMS; i++) {
-            vals[i] = base + i < nparts ? part_sums[base + i] : make_uint4(0, 0, 0, 0);
-            s = add4(s, vals[i]);
-        }
-        uint4 total;
-        uint4 ex = add4(block_exclusive<SUMS_BLOCK>(s, sh, total), carry);
-        for (int i = 0; i < SCAN_ITEMS; i++) {
-            if (base + i < nparts) part_sums[base + i] = ex;
-            ex = add4(ex, vals[i]);
-        }
-        carry = add4(carry, total);
-        t0 += total.x, t1 += total.y, t2 += total.z;
-    }
-    if (threadIdx.x == 0) totals[0] = t0, totals[1] = t1, totals[2] = t2;
-}
 
-// Downsweep over tiles of SCAN_TILE words (the exclusive scan of the
-// classify blocks' sums gives each tile's offset); words with .z set are also
-// listed at their scanned .z.
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *__restrict__ in, uint64_t n,
-                                                          const uint4 *__restrict__ part_sums,
-                                                          uint2 *__restrict__ out, uint32_t *__restrict__ list,
-                                                          const Seg seg0, bool bm) {
-    __shared__ uint4 sh[SCAN_BLOCK / 64];
-    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
-    uint4 vals[SCAN_ITEMS];
-    uint4 s = make_uint4(0, 0, 0, 0);
-    for (int i = 0; i < SCAN_ITEMS; i++) {
-        const uint32_t c = base + i < n ? in[base + i] : 0u;
-        vals[i] = make_uint4(c & 0xffu, (c >> 8) & 0x1ffu, c >> 17, 0u);
-        s = add4(s, vals[i]);
-    }
-    uint4 total;
-    uint4 ex = add4(block_exclusive(s, sh, total), part_sums[(uint64_t)blockIdx.x * (SCAN_TILE / CLASSIFY_BLOCK)]);
-    for (int i = 0; i < SCAN_ITEMS; i++) {
-        if (base + i < n) {
-            out[base + i] = make_uint2(ex.x, ex.y);
-            if (list && vals[i].z) {
-                uint32_t w = (uint32_t)(base + i);
-                if (bm) {  // brick-major position -> word
-                    int wx, y, z;
-                    bm_word(seg0.g, seg0.B, w, wx, y, z);
-                    w = (uint32_t)word_index(seg0.g, wx, y, z);
-                }
-                list[ex.z] = w;
-            }
-        }
-        ex = add4(ex, vals[i]);
-    }
-}
 
 // ---------------------------------------------------------------------------
 // emit
@@ -955,11 +761,51 @@ struct WordInfo {
     uint32_t base, tbase;  // scanned first vertex and first triangle of the word
 };
 
-// One wave per listed word, one lane per x position.  The 2 x 2 rows by 2
-// words a cube of this word can touch are staged in LDS, so every lane can
-// number the vertex on any edge of its cube.  Writes at or beyond the
-// capacities are dropped and flagged (the host grows the buffers and runs
-// this stage again).
+// Listed words a wave takes per iteration of emit.
+constexpr int EMIT_G = 8;
+static_assert(3 * EMIT_G <= 64 && EMIT_G <= 32, "emit's per-group prefixes fit one wave");
+
+// A word of a wave's group, as emit's later steps need it (LDS).
+struct GroupWord {
+    uint32_t y, z, x0;  // row, and the x of the word's first corner
+    uint32_t si;        // segment
+    uint32_t vsolid;    // the solid's first vertex (its triangles index from there)
+    uint32_t pad;
+    uint64_t keep;      // block filter: the word's cubes of accepted blocks
+};
+
+// Position of the r-th set bit of m (r < popcount(m)).
+__device__ __forceinline__ int select_bit(uint64_t m, int r) {
+    int pos = 0;
+    uint32_t h = (uint32_t)m;
+    const int c = __popc(h);
+    if (r >= c) r -= c, h = (uint32_t)(m >> 32), pos = 32;
+    for (int w = 16; w; w >>= 1) {
+        const int cw = __popc(h & ((1u << w) - 1u));
+        if (r >= cw) r -= cw, h >>= w, pos += w;
+    }
+    return pos;
+}
+
+// Largest i < n with pre[i] <= j (pre ascending from pre[0] == 0; n <= 64).
+__device__ __forceinline__ int owner(const uint32_t *pre, int n, uint32_t j) {
+    int L = 0;
+    for (int step = 32; step; step >>= 1)
+        if (L + step < n && pre[L + step] <= j) L += step;
+    return L;
+}
+
+// Emit: a wave takes EMIT_G consecutive listed words at a time.  Their
+// vertices and triangles are consecutive in the outputs (the scan numbers
+// them in list order), so the group's work is flattened: the 2 x 2 rows by
+// 2 words around each word are staged in LDS (a lane per word and
+// neighbour), then the group's vertices are spread over the lanes (vertex j
+// of the group: its word, axis and crossing found from per-word prefix
+// counts), then its mixed cubes (a lane per cube, its case from the staged
+// bits), then the corners of its triangles (corner j belongs to triangle
+// j / 3 of the cube whose prefix covers it).  No lane works on a cube
+// without triangles.  Writes at or beyond the capacities are dropped and
+// flagged (the host grows the buffers and runs this stage again).
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
                                               const uint32_t *__restrict__ word_seg,
@@ -970,8 +816,13 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
                                               uint32_t *__restrict__ tris, const uint64_t *__restrict__ keep_all,
                                               uint64_t vcap, uint64_t tcap, bool bm) {
     __shared__ __align__(16) uint8_t tab[256 * 16];
-    __shared__ WordInfo wi_all[4][8];
+    __shared__ WordInfo wi_all[4][EMIT_G][8];
+    __shared__ GroupWord gw_all[4][EMIT_G];
+    __shared__ uint64_t mixed_all[4][EMIT_G];
+    __shared__ uint32_t vpre_all[4][3 * EMIT_G];
+    __shared__ uint32_t mpre_all[4][EMIT_G];
     __shared__ uint32_t s_pre[4][64];
+    __shared__ uint16_t s_cube[4][64];
     __shared__ uint8_t s_cs[4][64];
     // (table: the case table with each edge decoded to axis | dx << 2 |
     // dy << 3 | dz << 4, its offset from the cube's low corner)
@@ -979,142 +830,178 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
         reinterpret_cast<uint4 *>(tab)[i] = reinterpret_cast<const uint4 *>(table)[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = wave_id();
-    WordInfo *wi = wi_all[wave];
+    WordInfo(*wi)[8] = wi_all[wave];
+    GroupWord *gwd = gw_all[wave];
+    uint64_t *mixed = mixed_all[wave];
+    uint32_t *vpre = vpre_all[wave], *mpre = mpre_all[wave];
     const uint32_t nact = (uint32_t)ctl->totals[2];
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
     bool over = false;
-    const uint32_t kstep = gridDim.x * 4;
-    uint32_t k = blockIdx.x * 4 + wave;
-    uint64_t gw_next = k < nact ? active[k] : 0;
-    for (; k < nact; k += kstep) {
-        const uint64_t gw = gw_next;
-        if (k + kstep < nact) gw_next = active[k + kstep];  // the next word, while this one is processed
-        const uint32_t si = wave_seg<MULTI>(word_seg, gw, SEG_WORD_SHIFT);
-        const Seg sg = pick_seg<MULTI>(segs, seg0, si);
-        const Grid &g = sg.g;
-        const uint64_t *bits = bits_all + sg.word0;
-        const uint2 *offs = offs_all + sg.word0;
-        const uint32_t vbase = MULTI ? offs[0].x : 0u;  // the solid's first vertex (triangles index from it)
-        const uint64_t w = gw - sg.word0;
-        int wx, y, z;
-        word_coords(g, w, wx, y, z);
-        wave_sync();
-        if (lane < 8) {  // entry bits: 0 next word, 1 row y+1, 2 row z+1
-            const int ex = wx + (lane & 1), ey = y + ((lane >> 1) & 1), ez = z + (lane >> 2);
-            // loads from clamped addresses (no branches), zeroed where the
-            // word or its neighbour lies beyond the grid
-            const bool in = ex < g.W && ey < g.ny && ez < g.nz;
-            const int cx = min(ex, g.W - 1), cy = min(ey, g.ny - 1), cz = min(ez, g.nz - 1);
-            // a segment's word indices fit in 32 bits (checked on the host)
-            const uint32_t W = (uint32_t)g.W, plane = (uint32_t)g.ny * W;
-            const uint32_t idx = (uint32_t)cz * plane + (uint32_t)cy * W + (uint32_t)cx;
-            const bool hx = cx + 1 < g.W, hy = cy + 1 < g.ny, hz = cz + 1 < g.nz;
-            const uint64_t b = bits[idx];
-            const uint64_t bn = bits[idx + (hx ? 1u : 0u)], by = bits[idx + (hy ? W : 0u)];
-            const uint64_t bz = bits[idx + (hz ? plane : 0u)];
-            const uint2 o = offs[bm && !MULTI ? bm_index(g, sg.B, cx, cy, cz) : idx];
+    const uint32_t kstep = gridDim.x * 4 * EMIT_G;
+    for (uint32_t k0 = (blockIdx.x * 4 + wave) * EMIT_G; k0 < nact; k0 += kstep) {
+        const int ng = (int)min((uint32_t)EMIT_G, nact - k0);  // words in the group
+        wave_sync();  // the previous group's LDS reads are done
+        // 1. Item 8 g + e stages neighbour e of word g: entry bits 0 next
+        //    word, 1 row y+1, 2 row z+1.
+        for (int q = lane; q < 8 * EMIT_G; q += 64) {
+            const int g = q >> 3, e = q & 7;
             WordInfo r{0, 0, 0, 0, 0, 0};
-            if (in) {
-                r.bits = b;
-                Masks m{b ^ ((b >> 1) | ((hx ? bn : 0) << 63)), b ^ (hy ? by : 0), b ^ (hz ? bz : 0)};
-                if (keep_all) keep_masks(g, keep_all + sg.word0, ex, ey, ez, m);
-                r.cx = m.cx, r.cy = m.cy, r.cz = m.cz;
-                r.base = o.x;
-                r.tbase = o.y;
+            if (g < ng) {
+                const uint32_t gw = active[k0 + g];
+                const uint32_t si = MULTI ? word_seg[gw >> SEG_WORD_SHIFT] : 0u;
+                Grid g_;
+                uint64_t word0;
+                if (MULTI) {
+                    const Seg *sp = &segs[si];
+                    g_.nx = sp->g.nx, g_.ny = sp->g.ny, g_.nz = sp->g.nz, g_.W = sp->g.W;
+                    word0 = sp->word0;
+                } else {
+                    g_ = seg0.g;
+                    word0 = seg0.word0;
+                }
+                const uint64_t *bits = bits_all + word0;
+                const uint2 *offs = offs_all + word0;
+                int wx, y, z;
+                word_coords(g_, gw - word0, wx, y, z);
+                const int ex = wx + (e & 1), ey = y + ((e >> 1) & 1), ez = z + (e >> 2);
+                // loads from clamped addresses (no branches), zeroed where
+                // the word or its neighbour lies beyond the grid
+                const bool in = ex < g_.W && ey < g_.ny && ez < g_.nz;
+                const int cx = min(ex, g_.W - 1), cy = min(ey, g_.ny - 1), cz = min(ez, g_.nz - 1);
+                // a segment's word indices fit in 32 bits (checked on the host)
+                const uint32_t W = (uint32_t)g_.W, plane = (uint32_t)g_.ny * W;
+                const uint32_t idx = (uint32_t)cz * plane + (uint32_t)cy * W + (uint32_t)cx;
+                const bool hx = cx + 1 < g_.W, hy = cy + 1 < g_.ny, hz = cz + 1 < g_.nz;
+                const uint64_t b = bits[idx];
+                const uint64_t bn = bits[idx + (hx ? 1u : 0u)], by = bits[idx + (hy ? W : 0u)];
+                const uint64_t bz = bits[idx + (hz ? plane : 0u)];
+                const uint2 o = offs[(!MULTI && bm) ? bm_index(g_, seg0.B, cx, cy, cz) : idx];
+                if (in) {
+                    r.bits = b;
+                    Masks m{b ^ ((b >> 1) | ((hx ? bn : 0) << 63)), b ^ (hy ? by : 0), b ^ (hz ? bz : 0)};
+                    if (!MULTI && keep_all) keep_masks(g_, keep_all + word0, ex, ey, ez, m);
+                    r.cx = m.cx, r.cy = m.cy, r.cz = m.cz;
+                    r.base = o.x;
+                    r.tbase = o.y;
+                }
+                if (e == 0) {
+                    const uint32_t vsolid = MULTI ? offs[0].x : 0u;
+                    const uint64_t kw = (!MULTI && keep_all) ? keep_all[word0 + idx] : ~0ull;
+                    gwd[g] = GroupWord{(uint32_t)y, (uint32_t)z, (uint32_t)wx * 64u, si, vsolid, 0u, kw};
+                }
             }
-            wi[lane] = r;
+            wi[g][e] = r;
         }
         wave_sync();
-        const int x = wx * 64 + lane;
-        {  // vertices of this word (vertex numbers are 32-bit, as the scan)
-            const WordInfo me = wi[0];
-            const uint64_t yz = vkey(0, 0, y, z);  // the row's part of the keys
-            const uint32_t c0 = __popcll(me.cx), c1 = __popcll(me.cy);
-            const uint32_t k0 = me.base + __popcll(me.cx & below);
-            const uint32_t k1 = me.base + c0 + __popcll(me.cy & below);
-            const uint32_t k2 = me.base + c0 + c1 + __popcll(me.cz & below);
-            const uint32_t xk = (uint32_t)x << 2;
-            if ((me.cx >> lane) & 1) {
-                if (k0 < vcap) {
-                    vkeys[k0] = yz | xk;
-                    if (vseg) vseg[k0] = si;
-                } else {
-                    over = true;
-                }
+        const uint32_t vbase = wi[0][0].base, tbase = wi[0][0].tbase;  // the group's first vertex / triangle
+        // 2. Vertex keys.  Lane 3 g + a counts word g's crossings on axis a;
+        //    their exclusive prefix numbers the group's vertices.
+        uint32_t nvg;
+        {
+            uint32_t c = 0;
+            if (lane < 3 * EMIT_G && lane / 3 < ng) {
+                const WordInfo &me = wi[lane / 3][0];
+                const int a = lane % 3;
+                c = (uint32_t)__popcll(a == 0 ? me.cx : (a == 1 ? me.cy : me.cz));
             }
-            if ((me.cy >> lane) & 1) {
-                if (k1 < vcap) {
-                    vkeys[k1] = yz | (xk | 1u);
-                    if (vseg) vseg[k1] = si;
-                } else {
-                    over = true;
-                }
+            uint32_t inc = c;
+            for (int d = 1; d < 3 * EMIT_G; d <<= 1) {
+                const uint32_t t = __shfl_up(inc, d);
+                if (lane >= d) inc += t;
             }
-            if ((me.cz >> lane) & 1) {
-                if (k2 < vcap) {
-                    vkeys[k2] = yz | (xk | 2u);
-                    if (vseg) vseg[k2] = si;
-                } else {
-                    over = true;
-                }
-            }
+            if (lane < 3 * EMIT_G) vpre[lane] = inc - c;
+            nvg = __shfl(inc, 3 * EMIT_G - 1);
         }
-        // triangles of this lane's cube, placed by a ballot prefix sum
-        int cs = 0;
-        for (int dz = 0; dz < 2; dz++)
-            for (int dy = 0; dy < 2; dy++) {
-                const int e = 2 * dy + 4 * dz;
-                const uint64_t b = wi[(dy << 1) | (dz << 2)].bits, bn = wi[1 | (dy << 1) | (dz << 2)].bits;
-                cs |= (int)((b >> lane) & 1) << e;
-                cs |= (int)((((b >> 1) | (bn << 63)) >> lane) & 1) << (e + 1);
-            }
-        if (keep_all && !((keep_all[sg.word0 + w] >> lane) & 1)) cs = 0;  // not in an accepted block
-        const int nt = tab[16 * cs];
-        const uint32_t pre = __popcll(__ballot(nt & 1) & below) + 2 * __popcll(__ballot(nt & 2) & below) +
-                             4 * __popcll(__ballot(nt & 4) & below);
-        const uint32_t tw = __shfl(pre + nt, 63);  // triangles of the word
-        const uint64_t t0 = wi[0].tbase;
-        if (t0 + tw > tcap) {
-            over = true;
-            continue;  // wave-uniform
-        }
-        // Spread the word's 3*tw triangle corners over the lanes: corner j
-        // belongs to triangle j/3 of the lane whose prefix covers it.
-        s_pre[wave][lane] = pre;
-        s_cs[wave][lane] = (uint8_t)cs;
         wave_sync();
-        for (uint32_t j = lane; j < 3 * tw; j += 64) {
-            const uint32_t ti = j / 3, ec = j - 3 * ti;
-            int L = 0;
-            for (int step = 32; step; step >>= 1)
-                if (L + step < 64 && s_pre[wave][L + step] <= ti) L += step;
-            const int c = s_cs[wave][L];
-            const int kk = 3 * (int)(ti - s_pre[wave][L]) + (int)ec;
-            const int e = tab[16 * c + 1 + kk];
-            const int axis = e & 3, ddx = (e >> 2) & 1, ddy = (e >> 3) & 1, ddz = (e >> 4) & 1;
-            const int ex = L + ddx;  // x within this word (64: the next word's first)
-            const WordInfo &o = wi[(ex >> 6) | (ddy << 1) | (ddz << 2)];
-            const int bb = ex & 63;
-            const uint64_t bl = bb ? (~0ull >> (64 - bb)) : 0ull;
-            // vertices of the word before this edge: its x crossings, then y, then z
-            const uint64_t m = axis == 0 ? o.cx : (axis == 1 ? o.cy : o.cz);
-            const uint32_t idx = o.base + (axis > 0 ? (uint32_t)__popcll(o.cx) : 0u) +
-                                 (axis > 1 ? (uint32_t)__popcll(o.cy) : 0u) + (uint32_t)__popcll(m & bl);
-            tris[3 * t0 + j] = idx - vbase;
+        for (uint32_t j = lane; j < nvg; j += 64) {
+            const int q = owner(vpre, 3 * EMIT_G, j), g = q / 3, a = q - 3 * g;
+            const WordInfo &me = wi[g][0];
+            const uint64_t m = a == 0 ? me.cx : (a == 1 ? me.cy : me.cz);
+            const int bit = select_bit(m, (int)(j - vpre[q]));
+            const GroupWord gw = gwd[g];
+            const uint64_t v = (uint64_t)vbase + j;
+            if (v < vcap) {
+                vkeys[v] = vkey(a, (int)gw.x0 + bit, (int)gw.y, (int)gw.z);
+                if (MULTI) vseg[v] = gw.si;
+            } else {
+                over = true;
+            }
+        }
+        // 3. Each word's mixed cubes (lane g); their prefix numbers the
+        //    group's mixed cubes.
+        uint32_t nmg;
+        {
+            uint32_t c = 0;
+            if (lane < ng) {
+                Nbhd h;
+                for (int k = 0; k < 8; k++) h.n[k] = wi[lane][k].bits;
+                const uint64_t m = mixed_cubes(nbhd_corners(h)) & gwd[lane].keep;
+                mixed[lane] = m;
+                c = (uint32_t)__popcll(m);
+            }
+            uint32_t inc = c;
+            for (int d = 1; d < EMIT_G; d <<= 1) {
+                const uint32_t t = __shfl_up(inc, d);
+                if (lane >= d) inc += t;
+            }
+            if (lane < EMIT_G) mpre[lane] = inc - c;
+            nmg = __shfl(inc, EMIT_G - 1);
+        }
+        wave_sync();
+        // 4. Triangles, up to 64 mixed cubes at a time.
+        uint32_t tdone = 0;
+        for (uint32_t c0 = 0; c0 < nmg; c0 += 64) {
+            const uint32_t j = c0 + lane;
+            int cs = 0, g = 0, i = 0;
+            if (j < nmg) {
+                g = owner(mpre, ng, j);
+                i = select_bit(mixed[g], (int)(j - mpre[g]));
+                for (int dz = 0; dz < 2; dz++)
+                    for (int dy = 0; dy < 2; dy++) {
+                        const int e = 2 * dy + 4 * dz;
+                        const uint64_t b = wi[g][(dy << 1) | (dz << 2)].bits, bn = wi[g][1 | (dy << 1) | (dz << 2)].bits;
+                        cs |= (int)((b >> i) & 1) << e;
+                        cs |= (int)((((b >> 1) | (bn << 63)) >> i) & 1) << (e + 1);
+                    }
+            }
+            const int nt = tab[16 * cs];
+            const uint32_t pre = __popcll(__ballot(nt & 1) & below) + 2 * __popcll(__ballot(nt & 2) & below) +
+                                 4 * __popcll(__ballot(nt & 4) & below);
+            const uint32_t tw = __shfl(pre + nt, 63);  // triangles of these cubes
+            s_pre[wave][lane] = pre;
+            s_cs[wave][lane] = (uint8_t)cs;
+            s_cube[wave][lane] = (uint16_t)((g << 6) | i);
+            wave_sync();
+            const uint64_t t0 = (uint64_t)tbase + tdone;
+            if (t0 + tw > tcap) {
+                over = true;
+            } else {
+                // corner jj belongs to triangle jj / 3 of the cube whose prefix covers it
+                for (uint32_t jj = lane; jj < 3 * tw; jj += 64) {
+                    const uint32_t ti = jj / 3, ec = jj - 3 * ti;
+                    const int L = owner(s_pre[wave], 64, ti);
+                    const int c = s_cs[wave][L], gi = s_cube[wave][L], gg = gi >> 6, ii = gi & 63;
+                    const int kk = 3 * (int)(ti - s_pre[wave][L]) + (int)ec;
+                    const int e = tab[16 * c + 1 + kk];
+                    const int axis = e & 3, ddx = (e >> 2) & 1, ddy = (e >> 3) & 1, ddz = (e >> 4) & 1;
+                    const int ex = ii + ddx;  // x within the word (64: the next word's first)
+                    const WordInfo &o = wi[gg][(ex >> 6) | (ddy << 1) | (ddz << 2)];
+                    const int bb = ex & 63;
+                    const uint64_t bl = bb ? (~0ull >> (64 - bb)) : 0ull;
+                    // vertices of the word before this edge: its x crossings, then y, then z
+                    const uint64_t m = axis == 0 ? o.cx : (axis == 1 ? o.cy : o.cz);
+                    const uint32_t idx = o.base + (axis > 0 ? (uint32_t)__popcll(o.cx) : 0u) +
+                                         (axis > 1 ? (uint32_t)__popcll(o.cy) : 0u) + (uint32_t)__popcll(m & bl);
+                    tris[3 * t0 + jj] = idx - gwd[gg].vsolid;
+                }
+            }
+            tdone += tw;
+            wave_sync();  // s_pre / s_cs / s_cube are rewritten by the next chunk
         }
     }
     if (over) atomicOr(&ctl->overflow, 1u);
 }
 
-// Each solid's first vertex and triangle in the call's outputs.
-__global__ __launch_bounds__(256) void k_seg_ranges(const Seg *__restrict__ segs, uint32_t nsegs,
-                                                    const uint2 *__restrict__ offs, SegOut *__restrict__ sout) {
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nsegs; s += gridDim.x * blockDim.x) {
-        const uint2 o = offs[segs[s].word0];
-        sout[s].vbase = o.x;
-        sout[s].tbase = o.y;
-    }
-}
 
 // ---------------------------------------------------------------------------
 // bisect
@@ -1241,32 +1128,6 @@ __global__ __launch_bounds__(256) void k_contains(DevProg P, const double *__res
                      : 0;
 }
 
-// ---------------------------------------------------------------------------
-// host-evaluated atoms
-// ---------------------------------------------------------------------------
-// The points a host-evaluated atom is asked about, in the atom's own frame
-// (the inverse transforms above it, applied exactly as for native atoms):
-// points [start, start + n) of a grid's corners (pts == nullptr; corner id
-// x + nx * (y + ny * z)) or of a point list.
-__global__ __launch_bounds__(256) void k_host_points(DevProg P, Grid g, const double *__restrict__ pts,
-                                                     uint32_t frame, uint64_t start, uint64_t n,
-                                                     double *__restrict__ out) {
-    const Frame fr = P.frames[frame];
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t id = start + i;
-        double x, y, z;
-        if (pts) {
-            x = pts[3 * id], y = pts[3 * id + 1], z = pts[3 * id + 2];
-        } else {
-            const uint64_t row = id / (uint64_t)g.nx;
-            x = g.xs[id - row * (uint64_t)g.nx];
-            y = g.ys[row % (uint64_t)g.ny];
-            z = g.zs[row / (uint64_t)g.ny];
-        }
-        apply_frame(P.f64, fr, x, y, z);
-        out[3 * i] = x, out[3 * i + 1] = y, out[3 * i + 2] = z;
-    }
-}
 
 // mcSearchPoint (mc.go:223-258) one probe at a time, for programs with
 // host-evaluated atoms: the host answers those atoms for every vertex's
@@ -1313,51 +1174,6 @@ __global__ __launch_bounds__(256) void k_bisect_host(DevProg P, Grid g, const ui
     }
 }
 
-// Mesh.Transform(t.Inverse()) of MarchingCubesConj (mc.go:66-71,
-// mesh.go:613-642): every vertex mapped by the inverse chain, given as frame
-// records {kind, v[9]} (the Apply of each op, transform.go:44-46, 71-73,
-// 103-105, 129-131).
-__global__ __launch_bounds__(256) void k_inverse_apply(const double *__restrict__ recs, uint32_t nops,
-                                                       double *__restrict__ verts, const Ctl *__restrict__ ctl,
-                                                       uint64_t vcap) {
-    const uint64_t nv = min((uint64_t)ctl->totals[0], vcap);
-    const Frame f{0, nops};
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (uint64_t)gridDim.x * blockDim.x) {
-        double x = verts[3 * v], y = verts[3 * v + 1], z = verts[3 * v + 2];
-        apply_frame(recs, f, x, y, z);
-        verts[3 * v] = x, verts[3 * v + 1] = y, verts[3 * v + 2] = z;
-    }
-}
 
-// Binary STL body (export.go:37-66, fileformats/stl.go:41-58): per triangle
-// the float32 cast of Triangle.Normal() -- the cross product of (t1 - t0) and
-// (t2 - t0), scaled by 1 / Norm() in float64 (primitives.go:27-33,
-// coords.go:201-222, 379-381) -- then the float32 casts of the three
-// vertices and two zero bytes: 50 bytes, written as 25 little-endian
-// halfwords (a record starts at an even offset).
-__global__ __launch_bounds__(256) void k_encode_stl(const double *__restrict__ verts, const uint32_t *__restrict__ tris,
-                                                    uint64_t nt, uint16_t *__restrict__ body) {
-    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += (uint64_t)gridDim.x * blockDim.x) {
-        double p[3][3];
-        for (int k = 0; k < 3; k++) {
-            const uint64_t v = tris[3 * t + k];
-            p[k][0] = verts[3 * v], p[k][1] = verts[3 * v + 1], p[k][2] = verts[3 * v + 2];
-        }
-        const double ax = p[1][0] - p[0][0], ay = p[1][1] - p[0][1], az = p[1][2] - p[0][2];
-        const double bx = p[2][0] - p[0][0], by = p[2][1] - p[0][1], bz = p[2][2] - p[0][2];
-        const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
-        const double s = 1 / sqrt(cx * cx + cy * cy + cz * cz);
-        float rec[12] = {(float)(cx * s), (float)(cy * s), (float)(cz * s)};
-        for (int k = 0; k < 3; k++)
-            for (int d = 0; d < 3; d++) rec[3 + 3 * k + d] = (float)p[k][d];
-        uint16_t *o = body + 25 * t;
-        for (int w = 0; w < 12; w++) {
-            const uint32_t u = __float_as_uint(rec[w]);
-            o[2 * w] = (uint16_t)(u & 0xffffu);
-            o[2 * w + 1] = (uint16_t)(u >> 16);
-        }
-        o[24] = 0;
-    }
-}
 
 }  // namespace mc3d

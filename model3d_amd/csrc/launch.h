@@ -1,0 +1,94 @@
+// launch.h -- host-side launchers of the device stages (kernels.h).  Each
+// group of kernels is compiled in a translation unit of its own (the corner
+// kernel once per precision and statistics mode), so the library builds in
+// parallel; mc3d.hip only sees these declarations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "pipeline.h"
+
+namespace mc3d {
+
+// Device buffers of one call: the context's grow-only buffers, or pieces of
+// the fixed scratch array of the allocation-free entry point.
+struct Bufs {
+    uint64_t *bits = nullptr;    // occupancy, one word per 64 x corners of a row
+    uint32_t *counts = nullptr;  // packed per scan position
+    uint2 *offs = nullptr;       // scanned {first vertex, first triangle}
+    uint32_t *actw = nullptr;    // listed (emitting) words
+    uint4 *tiles = nullptr;      // classify block sums
+    uint2 *binfo = nullptr;      // per brick {code offset, length}
+    uint32_t *actb = nullptr;    // active bricks
+    Insn *bcode = nullptr;       // folded brick programs
+    uint64_t *vkeys = nullptr;   // owning edge of each vertex
+    uint32_t *vseg = nullptr;    // segment of each vertex (calls of several solids)
+    const uint8_t *table = nullptr;   // 256-case table
+    const uint8_t *etable = nullptr;  // the same with edges decoded for emit
+    const uint64_t *keep = nullptr;   // block filter cube mask (one filtered solid)
+};
+
+// One call as the stage kernels see it.
+struct Pipe {
+    const Seg *segs = nullptr;
+    uint32_t nsegs = 0;
+    Seg seg0{};
+    const uint32_t *word_seg = nullptr, *brick_seg = nullptr;  // segment tables (calls of several)
+    Ctl *ctl = nullptr;
+    SegOut *sout = nullptr;
+    uint64_t nwords = 0, nbricks = 0, nscan = 0;
+    uint32_t cap = 0;
+    bool cull = false, bm = false, stats = false;
+    int mode = 0, rows_per_walk = 1, wpb = 4;
+    size_t plan_lds = 0;  // dynamic LDS of the plan (its stacks, the staged program)
+    uint32_t plan_gsz = 64;  // bricks per plan wave
+    bool plan_tile = false;  // plan waves take compact tiles of bricks (one solid)
+    uint32_t emit_div = 1024;  // emit blocks: occupancy words / emit_div, in [512, 4096]
+};
+
+hipError_t launch_plan(const Pipe &p, const Bufs &b, hipStream_t st);
+// Instantiated once per (MODE, STATS) in k_corners.hip.
+template <int MODE, bool STATS>
+hipError_t launch_corners_ms(const Pipe &p, const Bufs &b, hipStream_t st);
+template <>
+hipError_t launch_corners_ms<0, false>(const Pipe &p, const Bufs &b, hipStream_t st);
+template <>
+hipError_t launch_corners_ms<0, true>(const Pipe &p, const Bufs &b, hipStream_t st);
+template <>
+hipError_t launch_corners_ms<1, false>(const Pipe &p, const Bufs &b, hipStream_t st);
+template <>
+hipError_t launch_corners_ms<1, true>(const Pipe &p, const Bufs &b, hipStream_t st);
+inline hipError_t launch_corners(const Pipe &p, const Bufs &b, hipStream_t st) {
+    if (p.mode == 0) return p.stats ? launch_corners_ms<0, true>(p, b, st) : launch_corners_ms<0, false>(p, b, st);
+    return p.stats ? launch_corners_ms<1, true>(p, b, st) : launch_corners_ms<1, false>(p, b, st);
+}
+hipError_t launch_classify(const Pipe &p, const Bufs &b, hipStream_t st);
+// the scan of classify's counts, the list of emitting words and (calls of
+// several solids) each solid's output ranges
+hipError_t launch_scan(const Pipe &p, const Bufs &b, hipStream_t st);
+hipError_t launch_emit(const Pipe &p, const Bufs &b, uint32_t *tris, uint64_t vcap, uint64_t tcap, hipStream_t st);
+hipError_t launch_bisect(const Pipe &p, const Bufs &b, double *verts, double *interior, uint64_t vcap,
+                         hipStream_t st);
+hipError_t launch_bisect_host(int mode, const DevProg &P, const Grid &g, const uint64_t *bits,
+                              const uint64_t *vkeys, double *state, double *probe, double *verts, double *interior,
+                              uint64_t nv, int phase, hipStream_t st);
+hipError_t launch_host_points(const DevProg &P, const Grid &g, const double *pts, uint32_t frame, uint64_t start,
+                              uint64_t n, double *out, hipStream_t st);
+hipError_t launch_contains(int mode, const DevProg &P, const double *pts, uint8_t *out, uint64_t n, hipStream_t st);
+hipError_t launch_inverse_apply(const double *recs, uint32_t nops, double *verts, const Ctl *ctl, uint64_t vcap,
+                                hipStream_t st);
+// reads (write false) or writes `items` 16-byte items of buf; the read form
+// writes one word per block (8192 blocks) to out
+hipError_t launch_stream(bool write, void *buf, uint64_t items, uint32_t *out, hipStream_t st);
+hipError_t launch_encode_stl(const double *verts, const uint32_t *tris, uint64_t nt, uint16_t *body,
+                             hipStream_t st);
+
+inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 20) {
+    uint64_t b = (n + block - 1) / block;
+    if (b == 0) b = 1;
+    return (unsigned)(b < cap ? b : cap);
+}
+
+}  // namespace mc3d

@@ -1,14 +1,16 @@
 // mc3d.hip -- host side of the MI355X marching-cubes pipeline and the C-ABI
 // entry points for meshing (include/mc3d.h).  The device stages live in
-// kernels.h, Contains in eval.h, the solid tree and its lowering in solid.cpp.
+// kernels.h (launched from k_*.hip through launch.h), Contains in eval.h, the
+// solid tree and its lowering in solid.cpp.
 //
 // A call meshes a list of solids (one for mc3d_marching_cubes_search, many
-// for mc3d_marching_cubes_search_batch): one pinned upload of every lowered
-// program, the grid axes and a zeroed control block, then one launch per
-// stage for all solids on the context's stream and one synchronisation at
-// the end; every count a stage needs is read on the device.  Output buffers
-// are pooled per context and sized from earlier calls; a call whose meshes
-// do not fit grows them and re-runs the two output stages.
+// for mc3d_marching_cubes_search_batch): one upload of every lowered program,
+// the grid axes and a zeroed control block, then one launch per stage for all
+// solids on the context's stream and one synchronisation at the end; every
+// count a stage needs is read on the device.  Output buffers are pooled per
+// context and sized from earlier calls; a call whose meshes do not fit grows
+// them and re-runs the two output stages.  Meshes wanted on the host land in
+// pooled pinned memory: the triangles are copied while the bisection runs.
 //
 // Every float64 expression keeps the reference's operation order and is
 // compiled with -ffp-contract=off, so Contains decisions, topology and vertex
@@ -22,14 +24,17 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <exception>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
 #include <vector>
 
-#include "kernels.h"
+#include "launch.h"
 #include "mc3d_internal.h"
+#include "plan.h"
 
 using namespace mc3d;
 
@@ -43,6 +48,25 @@ namespace {
             return MC3D_E_HIP;                                                               \
         }                                                                                    \
     } while (0)
+
+// Runs an entry point's body; a C++ exception (an allocation that failed
+// inside a container, a thread that could not start) becomes a status
+// instead of unwinding through the C ABI.
+template <class F>
+int guarded(F &&f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        set_error("out of host memory");
+        return MC3D_E_OOM;
+    } catch (const std::exception &e) {
+        set_error(std::string("internal error: ") + e.what());
+        return MC3D_E_INTERNAL;
+    } catch (...) {
+        set_error("internal error");
+        return MC3D_E_INTERNAL;
+    }
+}
 
 // Grow-only device buffer.
 struct DevBuf {
@@ -75,10 +99,13 @@ struct DevBuf {
     }
 };
 
-// Pinned host staging buffer (grow-only).
+// Pinned host buffer (grow-only).
 struct HostBuf {
     void *p = nullptr;
     size_t cap = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf &) = delete;
+    HostBuf &operator=(const HostBuf &) = delete;
     ~HostBuf() {
         if (p) (void)hipHostFree(p);
     }
@@ -97,24 +124,34 @@ struct HostBuf {
         cap = want;
         return MC3D_OK;
     }
+    template <class T>
+    T *as() const {
+        return (T *)p;
+    }
 };
 
-// Device buffers released by freed meshes, reused by later calls.
+// Buffers released by freed meshes, reused by later calls (device buffers,
+// or pinned host buffers).
+template <class B>
 struct Pool {
-    std::vector<std::unique_ptr<DevBuf>> free;
-    std::unique_ptr<DevBuf> take(size_t bytes) {
+    std::mutex mu;  // meshes may be freed on any thread
+    std::vector<std::unique_ptr<B>> free;
+    std::unique_ptr<B> take(size_t bytes) {
+        std::lock_guard<std::mutex> lk(mu);
         size_t best = free.size();
         for (size_t i = 0; i < free.size(); i++)
             if (free[i]->cap >= bytes && (best == free.size() || free[i]->cap < free[best]->cap)) best = i;
         if (best < free.size()) {
-            std::unique_ptr<DevBuf> b = std::move(free[best]);
+            std::unique_ptr<B> b = std::move(free[best]);
             free.erase(free.begin() + best);
             return b;
         }
-        return std::unique_ptr<DevBuf>(new DevBuf);
+        return std::unique_ptr<B>(new B);
     }
-    void give(std::unique_ptr<DevBuf> b) {
-        if (b && b->p) free.push_back(std::move(b));
+    void give(std::unique_ptr<B> b) {
+        if (!b || !b->p) return;
+        std::lock_guard<std::mutex> lk(mu);
+        free.push_back(std::move(b));
         while (free.size() > 8) free.erase(free.begin());
     }
 };
@@ -122,7 +159,7 @@ struct Pool {
 // memcpy of large host buffers split over up to 16 threads (the first touch
 // of freshly allocated pages dominates a single-threaded copy).
 void parallel_copy(void *dst, const void *src, size_t bytes) {
-    constexpr size_t CHUNK = 8u << 20;
+    constexpr size_t CHUNK = 4u << 20;
     const size_t nt = std::min<size_t>(16, (bytes + CHUNK - 1) / CHUNK);
     if (nt <= 1) {
         if (bytes) memcpy(dst, src, bytes);
@@ -130,32 +167,33 @@ void parallel_copy(void *dst, const void *src, size_t bytes) {
     }
     const size_t per = (bytes + nt - 1) / nt;
     std::vector<std::thread> th;
-    for (size_t i = 0; i < nt; i++) {
+    for (size_t i = 1; i < nt; i++) {
         const size_t a = i * per, b = std::min(bytes, a + per);
         if (a < b) th.emplace_back([=] { memcpy((char *)dst + a, (const char *)src + a, b - a); });
     }
+    memcpy(dst, src, std::min(bytes, per));
     for (std::thread &t : th) t.join();
-}
-
-unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 20) {
-    uint64_t b = (n + block - 1) / block;
-    if (b == 0) b = 1;
-    return (unsigned)(b < cap ? b : cap);
 }
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-// The device outputs of one call (vertices, triangles), shared by the meshes
-// it returned and given back to the context's pool with the last of them.
+// The outputs of one call (vertices, triangles), shared by the meshes it
+// returned and given back to the context's pools with the last of them.
+// `host` holds the meshes' copies in pinned memory: all vertices, then all
+// triangles, then all interior points.
 struct MeshBuffers {
-    std::shared_ptr<Pool> pool;
+    std::shared_ptr<Pool<DevBuf>> pool;
+    std::shared_ptr<Pool<HostBuf>> hpool;
     std::unique_ptr<DevBuf> verts, tris, interior;  // interior: MarchingCubesInterior only
+    std::unique_ptr<HostBuf> host;
+    uint64_t nv = 0, nt = 0;  // of the whole call (the layout of `host`)
     ~MeshBuffers() {
         if (pool) {
             pool->give(std::move(verts));
             pool->give(std::move(tris));
             pool->give(std::move(interior));
         }
+        if (hpool) hpool->give(std::move(host));
     }
 };
 
@@ -169,19 +207,30 @@ struct ColliderCache {
     DevCollider dev{};
 };
 
+enum { EV_PLAN0, EV_PLAN1, EV_CORNERS, EV_CLASSIFY, EV_SCAN, EV_EMIT0, EV_EMIT1, EV_OUT, EV_CTL, EV_N };
+
 struct mc3d_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy = nullptr;  // device-to-host copies of results (overlapping the bisection)
     std::vector<std::unique_ptr<ColliderCache>> colliders;  // most recent last
-    HostBuf staging, ctl_host, segstage, outstage;
+    HostBuf staging, ctl_host, ctl_early, segstage, outstage;
     DevBuf blob, table, etable, bits, counts, offs, tiles, actw, actb, binfo, bcode, vkeys, vseg, pts, out8;
     DevBuf hvals, hpts, hstate, hprobe;  // host-evaluated atoms: answers, points, bisection state
     HostBuf hstage;                      // pinned points / answers of one callback batch
     DevBuf stl, stl_in;                  // STL encoding: body bytes; uploaded host triangles
     DevBuf keep;                         // block filter: cube mask
-    std::shared_ptr<Pool> pool = std::make_shared<Pool>();
+    std::shared_ptr<Pool<DevBuf>> pool = std::make_shared<Pool<DevBuf>>();
+    std::shared_ptr<Pool<HostBuf>> hpool = std::make_shared<Pool<HostBuf>>();
     uint64_t v_hint = 1u << 18, t_hint = 1u << 19;  // output capacities to start from
-    hipEvent_t ev[10] = {};
+    hipEvent_t ev[EV_N] = {};
+    // Captured launch chains of recent call shapes (most recent last).
+    struct Graph {
+        std::vector<uint8_t> key;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Graph> graphs;
+    bool graphs_off = true;  // a capture failed on this device: enqueue stage by stage
 };
 
 // One solid's mesh: a slice [voff, voff + nv) / [toff, toff + nt) of its
@@ -189,12 +238,14 @@ struct mc3d_ctx {
 struct mc3d_mesh {
     int device = 0;
     uint64_t nv = 0, nt = 0, voff = 0, toff = 0;
-    std::unique_ptr<double[]> hverts;  // on_host: 3 nv (allocated without initialisation)
-    std::unique_ptr<uint32_t[]> htris;  // on_host: 3 nt
-    std::unique_ptr<double[]> hinterior;  // on_host with interior points: 3 nv
     bool on_host = false, has_interior = false;
     std::shared_ptr<MeshBuffers> buf;
     mc3d_stats stats{};
+    const double *hverts() const { return buf->host->as<double>() + 3 * voff; }
+    const uint32_t *htris() const { return (const uint32_t *)(buf->host->as<double>() + 3 * buf->nv) + 3 * toff; }
+    const double *hinterior() const {
+        return (const double *)((const char *)buf->host->p + align_up(buf->nv * 24 + buf->nt * 12, 256)) + 3 * voff;
+    }
 };
 
 namespace {
@@ -336,13 +387,81 @@ int collider_on_device(mc3d_ctx *ctx, const std::shared_ptr<const ColliderData> 
 
 // Bytes of LDS the plan needs to stage a program: instructions, their atom
 // boxes and the float64 params.
-size_t plan_stage_bytes(const Program &P) {
-    return PLAN_STACK_BYTES + P.code.size() * (sizeof(Insn) + sizeof(AtomBox)) + P.f64.size() * sizeof(double);
-}
+size_t plan_stage_bytes(const Program &P) { return PLAN_STACK_BYTES + P.code.size() * sizeof(PlanNode); }
 constexpr size_t PLAN_STAGE_LIMIT = 64 * 1024;
 
-// Where a call's pieces sit in device memory.
+// The node records of a postfix program for the brick plan (PlanNode): per
+// node its instruction, link, all-false value, box (atoms: their own;
+// operators: the union of their atoms'; constants: empty; a NaN bound makes
+// the union unbounded on that axis) and the params of the "certainly true"
+// tests.  boxes: each atom's box, by pc.
+void plan_nodes(const Program &P, const std::vector<AtomBox> &boxes, std::vector<PlanNode> &nodes) {
+    const size_t n = P.code.size();
+    nodes.assign(n, PlanNode{});
+    std::vector<uint32_t> st, start(n);
+    st.reserve(64);
+    auto unite = [](PlanNode &u, const PlanNode &a, const PlanNode &b) {
+        for (int k = 0; k < 3; k++) {
+            u.lo[k] = (isnan(a.lo[k]) || isnan(b.lo[k])) ? -INFINITY : std::min(a.lo[k], b.lo[k]);
+            u.hi[k] = (isnan(a.hi[k]) || isnan(b.hi[k])) ? INFINITY : std::max(a.hi[k], b.hi[k]);
+        }
+    };
+    for (size_t i = 0; i < n; i++) {
+        PlanNode &nd = nodes[i];
+        nd.x = P.code[i].x, nd.y = P.code[i].y;
+        const uint32_t op = insn_op(nd.x);
+        if (op == OP_ATOM || op == OP_TRUE || op == OP_FALSE) {
+            start[i] = (uint32_t)i;
+            nd.link = (uint32_t)i, nd.cfalse = op == OP_TRUE ? 1u : 0u;
+            if (op == OP_ATOM) {
+                memcpy(nd.lo, boxes[i].lo, sizeof nd.lo);
+                memcpy(nd.hi, boxes[i].hi, sizeof nd.hi);
+                const uint32_t kind = insn_kind(nd.x);
+                const double *a = P.f64.data() + nd.y;
+                if (kind == A_RECT) memcpy(nd.prm, a, 6 * sizeof(double));
+                if (kind == A_SPHERE) nd.prm[0] = a[0], nd.prm[1] = a[1], nd.prm[2] = a[2], nd.prm[3] = a[4];
+            } else {
+                for (int k = 0; k < 3; k++) nd.lo[k] = INFINITY, nd.hi[k] = -INFINITY;
+            }
+        } else if (op == OP_NOT) {
+            const uint32_t c = st.back();
+            st.pop_back();
+            start[i] = start[c];
+            memcpy(nd.lo, nodes[c].lo, sizeof nd.lo);
+            memcpy(nd.hi, nodes[c].hi, sizeof nd.hi);
+            nd.link = c, nd.cfalse = nodes[c].cfalse ^ 1u;
+        } else {
+            const uint32_t b = st.back();
+            st.pop_back();
+            const uint32_t a = st.back();
+            st.pop_back();
+            const uint32_t fa = nodes[a].cfalse, fb = nodes[b].cfalse;
+            start[i] = start[a];
+            unite(nd, nodes[a], nodes[b]);
+            nd.link = a;
+            nd.cfalse = op == OP_OR ? (fa | fb) : op == OP_AND ? (fa & fb) : op == OP_ANDNOT ? (fa & ~fb & 1u)
+                                                                                             : (~fa & fb & 1u);
+        }
+        st.push_back((uint32_t)i);
+    }
+    // each leaf links to the outermost node starting at it (the last in postfix order)
+    for (size_t i = 0; i < n; i++)
+        if (start[i] != i) nodes[start[i]].link = (uint32_t)i;
+}
+
+// Where a call's pieces sit in the program blob (offsets), and the blob's
+// device address once placed.
 struct Upload {
+    // layout (bytes from the blob's start)
+    size_t total = 0, o_sout = 0, o_segs = 0, o_wseg = 0, o_bseg = 0, o_conj = 0;
+    struct Off {
+        size_t code, f64, f32, fr, nodes, col, ax;
+    };
+    std::vector<Off> po;
+    std::vector<std::vector<PlanNode>> nodes;
+    std::vector<std::vector<DevCollider>> cols;
+    bool tables = false;
+    // placed
     Ctl *ctl = nullptr;
     SegOut *sout = nullptr;
     Seg *segs = nullptr;
@@ -353,21 +472,11 @@ struct Upload {
     uint32_t conj_nops = 0;
 };
 
-// Packs the control block, the per-solid results, the segments, the segment
-// tables and every job's program and grid axes into the pinned staging
-// buffer and copies it to the device in one transfer.
-int upload(mc3d_ctx *ctx, std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_all, Upload &U) {
+// The blob's layout: the control block, the per-solid results, the
+// segments, the segment tables and every job's program and grid axes.
+// `cols` holds each job's colliders as the device sees them.
+void layout_upload(const std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_all, Upload &U) {
     const size_t n = jobs.size();
-    size_t ncol = 0;
-    for (const Job &j : jobs) ncol += j.P.colliders.size();
-    std::vector<std::vector<DevCollider>> cols(n);
-    for (size_t s = 0; s < n; s++) {
-        cols[s].resize(jobs[s].P.colliders.size());
-        for (size_t i = 0; i < cols[s].size(); i++) {
-            int rc = collider_on_device(ctx, jobs[s].P.colliders[i], cols[s][i], ncol);
-            if (rc) return rc;
-        }
-    }
     size_t off = 0;
     auto place = [&](size_t bytes) {
         size_t o = off;
@@ -375,99 +484,141 @@ int upload(mc3d_ctx *ctx, std::vector<Job> &jobs, uint64_t nwords_all, uint64_t 
         return o;
     };
     // the per-solid results follow the control block directly (one copy back)
-    const size_t o_sout = align_up(sizeof(Ctl), 16);
-    U.back_bytes = o_sout + n * sizeof(SegOut);
+    U.o_sout = align_up(sizeof(Ctl), 16);
+    U.back_bytes = U.o_sout + n * sizeof(SegOut);
     off = align_up(U.back_bytes, 256);
-    const size_t o_segs = place(n * sizeof(Seg));
-    const bool tables = n > 1;
-    const size_t o_wseg = place(tables ? nwords_all / 64 * sizeof(uint32_t) : 0);
-    const size_t o_bseg = place(tables ? nbricks_all / 128 * sizeof(uint32_t) : 0);
-    const size_t o_conj = place(jobs[0].conj.size() * sizeof(double));
-    struct Off {
-        size_t code, f64, f32, fr, box, col, ax;
-    };
-    std::vector<Off> po(n);
-    std::vector<std::vector<AtomBox>> boxes(n);
+    U.o_segs = place(n * sizeof(Seg));
+    U.tables = n > 1;
+    U.o_wseg = place(U.tables ? nwords_all / 64 * sizeof(uint32_t) : 0);
+    U.o_bseg = place(U.tables ? nbricks_all / 128 * sizeof(uint32_t) : 0);
+    U.o_conj = place(jobs[0].conj.size() * sizeof(double));
+    U.po.resize(n);
+    U.nodes.resize(n);
     for (size_t s = 0; s < n; s++) {
         const Program &P = jobs[s].P;
-        boxes[s].resize(P.code.size());
-        for (size_t i = 0; i < P.atom_pc.size(); i++) boxes[s][P.atom_pc[i]] = P.atom_box[i];
-        po[s].code = place(P.code.size() * sizeof(Insn));
+        std::vector<AtomBox> boxes(P.code.size());
+        for (size_t i = 0; i < P.atom_pc.size(); i++) boxes[P.atom_pc[i]] = P.atom_box[i];
+        plan_nodes(P, boxes, U.nodes[s]);
+        Upload::Off &po = U.po[s];
+        po.code = place(P.code.size() * sizeof(Insn));
         // params padded so that staging a brick's atoms never reads past the array
-        po[s].f64 = place((P.f64.size() + MAX_ATOM_PARAMS) * sizeof(double));
-        po[s].f32 = place((P.f32.size() + MAX_ATOM_PARAMS) * sizeof(float));
-        po[s].fr = place(P.frames.size() * sizeof(Frame));
-        po[s].box = place(boxes[s].size() * sizeof(AtomBox));
-        po[s].col = place(cols[s].size() * sizeof(DevCollider));
-        po[s].ax = place((jobs[s].ax[0].size() + jobs[s].ax[1].size() + jobs[s].ax[2].size()) * sizeof(double));
+        po.f64 = place((P.f64.size() + MAX_ATOM_PARAMS) * sizeof(double));
+        po.f32 = place((P.f32.size() + MAX_ATOM_PARAMS) * sizeof(float));
+        po.fr = place(P.frames.size() * sizeof(Frame));
+        po.nodes = place(U.nodes[s].size() * sizeof(PlanNode));
+        po.col = place(P.colliders.size() * sizeof(DevCollider));
+        po.ax = place((jobs[s].ax[0].size() + jobs[s].ax[1].size() + jobs[s].ax[2].size()) * sizeof(double));
     }
-    const size_t total = off;
-    int rc;
-    if ((rc = ctx->staging.ensure(total)) || (rc = ctx->blob.ensure(total))) return rc;
-    char *h = (char *)ctx->staging.p;
-    char *dv = (char *)ctx->blob.p;
-    memset(h, 0, o_segs);  // control block and per-solid results start at zero
+    U.total = off;
+}
+
+// Writes the blob laid out by layout_upload into host memory h, for the
+// device address dv (the caller copies it there); fills the segments'
+// device pointers (culling bricks from b).
+void fill_upload(std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_all, const Bufs &b, char *h, char *dv,
+                 Upload &U) {
+    const size_t n = jobs.size();
+    memset(h, 0, U.o_segs);  // control block and per-solid results start at zero
     U.hsegs.assign(n, Seg{});
     for (size_t s = 0; s < n; s++) {
         Job &j = jobs[s];
         const Program &P = j.P;
-        memcpy(h + po[s].code, P.code.data(), P.code.size() * sizeof(Insn));
-        memset(h + po[s].f64, 0, (P.f64.size() + MAX_ATOM_PARAMS) * sizeof(double));
-        memset(h + po[s].f32, 0, (P.f32.size() + MAX_ATOM_PARAMS) * sizeof(float));
+        const Upload::Off &po = U.po[s];
+        memcpy(h + po.code, P.code.data(), P.code.size() * sizeof(Insn));
+        memset(h + po.f64, 0, (P.f64.size() + MAX_ATOM_PARAMS) * sizeof(double));
+        memset(h + po.f32, 0, (P.f32.size() + MAX_ATOM_PARAMS) * sizeof(float));
         if (!P.f64.empty()) {
-            memcpy(h + po[s].f64, P.f64.data(), P.f64.size() * sizeof(double));
-            memcpy(h + po[s].f32, P.f32.data(), P.f32.size() * sizeof(float));
+            memcpy(h + po.f64, P.f64.data(), P.f64.size() * sizeof(double));
+            memcpy(h + po.f32, P.f32.data(), P.f32.size() * sizeof(float));
         }
-        memcpy(h + po[s].fr, P.frames.data(), P.frames.size() * sizeof(Frame));
-        memcpy(h + po[s].box, boxes[s].data(), boxes[s].size() * sizeof(AtomBox));
-        if (!cols[s].empty()) memcpy(h + po[s].col, cols[s].data(), cols[s].size() * sizeof(DevCollider));
-        double *d = (double *)(h + po[s].ax);
+        memcpy(h + po.fr, P.frames.data(), P.frames.size() * sizeof(Frame));
+        memcpy(h + po.nodes, U.nodes[s].data(), U.nodes[s].size() * sizeof(PlanNode));
+        if (!U.cols[s].empty()) memcpy(h + po.col, U.cols[s].data(), U.cols[s].size() * sizeof(DevCollider));
+        double *d = (double *)(h + po.ax);
         for (int a = 0; a < 3; a++) d = std::copy(j.ax[a].begin(), j.ax[a].end(), d);
         Seg &sg = U.hsegs[s];
-        sg.P.code = (const Insn *)(dv + po[s].code);
-        sg.P.f64 = (const double *)(dv + po[s].f64);
-        sg.P.f32 = (const float *)(dv + po[s].f32);
-        sg.P.frames = (const Frame *)(dv + po[s].fr);
-        sg.P.boxes = (const AtomBox *)(dv + po[s].box);
-        sg.P.colliders = (const DevCollider *)(dv + po[s].col);
+        sg.P.code = (const Insn *)(dv + po.code);
+        sg.P.f64 = (const double *)(dv + po.f64);
+        sg.P.f32 = (const float *)(dv + po.f32);
+        sg.P.frames = (const Frame *)(dv + po.fr);
+        sg.P.plan = (const PlanNode *)(dv + po.nodes);
+        sg.P.colliders = (const DevCollider *)(dv + po.col);
         sg.P.host_vals = nullptr;
         sg.P.host_stride = 0;
         sg.P.ncode = (uint32_t)P.code.size();
         sg.P.nf64 = (uint32_t)P.f64.size();
         if (!j.ax[0].empty()) {
-            const double *ax = (const double *)(dv + po[s].ax);
+            const double *ax = (const double *)(dv + po.ax);
             j.g.xs = ax, j.g.ys = ax + j.g.nx, j.g.zs = ax + j.g.nx + j.g.ny;
         }
         sg.g = j.g;
         sg.B = j.B;
-        if (j.cap) sg.B.info = ctx->binfo.as<uint2>(), sg.B.code = ctx->bcode.as<Insn>();
+        if (j.cap) sg.B.info = b.binfo, sg.B.code = b.bcode;
         sg.word0 = j.word0, sg.nwords = j.nwords;
         sg.brick0 = j.brick0, sg.nbricks = j.nbricks;
         sg.iters = j.iters;
         sg.stage = j.cap && plan_stage_bytes(P) <= PLAN_STAGE_LIMIT ? 1 : 0;
     }
-    memcpy(h + o_segs, U.hsegs.data(), n * sizeof(Seg));
+    memcpy(h + U.o_segs, U.hsegs.data(), n * sizeof(Seg));
     if (!jobs[0].conj.empty()) {
-        memcpy(h + o_conj, jobs[0].conj.data(), jobs[0].conj.size() * sizeof(double));
-        U.conj = (const double *)(dv + o_conj);
+        memcpy(h + U.o_conj, jobs[0].conj.data(), jobs[0].conj.size() * sizeof(double));
+        U.conj = (const double *)(dv + U.o_conj);
         U.conj_nops = (uint32_t)(jobs[0].conj.size() / 10);
     }
-    if (tables) {
-        uint32_t *ws = (uint32_t *)(h + o_wseg), *bs = (uint32_t *)(h + o_bseg);
+    if (U.tables) {
+        uint32_t *ws = (uint32_t *)(h + U.o_wseg), *bs = (uint32_t *)(h + U.o_bseg);
         for (size_t s = 0; s < n; s++) {
             const uint64_t w1 = s + 1 < n ? jobs[s + 1].word0 : nwords_all;
             for (uint64_t w = jobs[s].word0; w < w1; w += 64) ws[w / 64] = (uint32_t)s;
             const uint64_t b1 = s + 1 < n ? jobs[s + 1].brick0 : nbricks_all;
-            for (uint64_t b = jobs[s].brick0; b < b1; b += 128) bs[b / 128] = (uint32_t)s;
+            for (uint64_t bb = jobs[s].brick0; bb < b1; bb += 128) bs[bb / 128] = (uint32_t)s;
         }
     }
     U.ctl = (Ctl *)dv;
-    U.sout = (SegOut *)(dv + o_sout);
-    U.segs = (Seg *)(dv + o_segs);
-    U.word_seg = tables ? (uint32_t *)(dv + o_wseg) : nullptr;
-    U.brick_seg = tables ? (uint32_t *)(dv + o_bseg) : nullptr;
-    HIP_TRY(hipMemcpyAsync(ctx->blob.p, h, total, hipMemcpyHostToDevice, ctx->stream));
+    U.sout = (SegOut *)(dv + U.o_sout);
+    U.segs = (Seg *)(dv + U.o_segs);
+    U.word_seg = U.tables ? (uint32_t *)(dv + U.o_wseg) : nullptr;
+    U.brick_seg = U.tables ? (uint32_t *)(dv + U.o_bseg) : nullptr;
+}
+
+// The colliders of every job on this context's device.
+int upload_colliders(mc3d_ctx *ctx, const std::vector<Job> &jobs, Upload &U) {
+    const size_t n = jobs.size();
+    size_t ncol = 0;
+    for (const Job &j : jobs) ncol += j.P.colliders.size();
+    U.cols.assign(n, {});
+    for (size_t s = 0; s < n; s++) {
+        U.cols[s].resize(jobs[s].P.colliders.size());
+        for (size_t i = 0; i < U.cols[s].size(); i++) {
+            int rc = collider_on_device(ctx, jobs[s].P.colliders[i], U.cols[s][i], ncol);
+            if (rc) return rc;
+        }
+    }
     return MC3D_OK;
+}
+
+// The whole blob in the context's pinned staging buffer, for one transfer
+// (enqueue_upload).
+int prepare_upload(mc3d_ctx *ctx, std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_all, const Bufs &b,
+                   Upload &U) {
+    int rc;
+    if ((rc = upload_colliders(ctx, jobs, U))) return rc;
+    layout_upload(jobs, nwords_all, nbricks_all, U);
+    if ((rc = ctx->staging.ensure(U.total)) || (rc = ctx->blob.ensure(U.total))) return rc;
+    fill_upload(jobs, nwords_all, nbricks_all, b, (char *)ctx->staging.p, (char *)ctx->blob.p, U);
+    return MC3D_OK;
+}
+
+int enqueue_upload(mc3d_ctx *ctx, const Upload &U) {
+    HIP_TRY(hipMemcpyAsync(ctx->blob.p, ctx->staging.p, U.total, hipMemcpyHostToDevice, ctx->stream));
+    return MC3D_OK;
+}
+
+int upload(mc3d_ctx *ctx, std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_all, const Bufs &b,
+           Upload &U) {
+    int rc;
+    if ((rc = prepare_upload(ctx, jobs, nwords_all, nbricks_all, b, U))) return rc;
+    return enqueue_upload(ctx, U);
 }
 
 // Rewrites segment s on the device (after its host-evaluated answers moved).
@@ -500,9 +651,7 @@ int host_round(mc3d_ctx *ctx, const Job &j, Upload &U, size_t s, const double *p
     for (size_t h = 0; h < nh; h++) {
         for (uint64_t s0 = 0; s0 < n; s0 += HOST_BATCH) {
             const uint64_t c = std::min(HOST_BATCH, n - s0);
-            k_host_points<<<grid_for(c, 256), 256, 0, ctx->stream>>>(dp, U.hsegs[s].g, pts, j.frames[h], s0, c,
-                                                                    ctx->hpts.as<double>());
-            HIP_TRY(hipGetLastError());
+            HIP_TRY(launch_host_points(dp, U.hsegs[s].g, pts, j.frames[h], s0, c, ctx->hpts.as<double>(), ctx->stream));
             HIP_TRY(hipMemcpyAsync(hp, ctx->hpts.p, c * 24, hipMemcpyDeviceToHost, ctx->stream));
             // Also orders the previous batch's upload of `ho` before it is rewritten.
             HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -527,15 +676,8 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-// The scan of the counts whose per-block sums k_classify wrote.
-int scan4(mc3d_ctx *ctx, const uint32_t *in, uint2 *out, uint64_t n, unsigned long long *totals, uint32_t *list,
-          const Seg &seg0, bool bm) {
-    const uint64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-    k_scan_tiles<<<1, SUMS_BLOCK, 0, ctx->stream>>>(ctx->tiles.as<uint4>(), (n + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK,
-                                                    totals);
-    k_scan_down<<<(unsigned)ntiles, SCAN_BLOCK, 0, ctx->stream>>>(in, n, ctx->tiles.as<uint4>(), out, list, seg0, bm);
-    HIP_TRY(hipGetLastError());
-    return MC3D_OK;
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 // Sizes and launch shapes of one call.
@@ -547,63 +689,130 @@ struct Call {
     int mode = 0, rows_per_walk = 1, wpb = 4;
     bool stats = false;
     bool interior = false;  // a call of one MarchingCubesInterior solid
-    const uint64_t *keep = nullptr;  // a call of one filtered solid: its cube mask on the device
-    bool bm = false;     // the scan takes the words in brick-major order (one culled solid)
-    uint64_t nscan = 0;  // scan positions: words, or brick-major positions
+    bool host = false;      // a call of one solid with host-evaluated atoms
+    bool bm = false;        // the scan takes the words in brick-major order (one culled solid)
+    uint64_t nscan = 0;     // scan positions: words, or brick-major positions
+    size_t plan_lds = 0;
 };
 
-template <int MODE, bool STATS, int NP, bool MULTI>
-void launch_corners_act(mc3d_ctx *ctx, const Call &c, const Upload &U, unsigned grid) {
-    uint32_t *act = ctx->actb.as<uint32_t>();
-    uint64_t *bits = ctx->bits.as<uint64_t>();
-    const Seg &s0 = U.hsegs[0];
-    if (c.wpb == 1)
-        k_corners_act<MODE, STATS, NP, 1, MULTI>
-            <<<grid, 64, 0, ctx->stream>>>(U.segs, c.nsegs, s0, U.brick_seg, act, bits, U.ctl, U.sout);
-    else if (c.wpb == 2)
-        k_corners_act<MODE, STATS, NP, 2, MULTI>
-            <<<grid, 128, 0, ctx->stream>>>(U.segs, c.nsegs, s0, U.brick_seg, act, bits, U.ctl, U.sout);
-    else
-        k_corners_act<MODE, STATS, NP, 4, MULTI>
-            <<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, s0, U.brick_seg, act, bits, U.ctl, U.sout);
-}
-
-template <int MODE, bool STATS, bool MULTI>
-void launch_corners_m(mc3d_ctx *ctx, const Call &c, const Upload &U) {
-    const Seg &s0 = U.hsegs[0];
-    if (!c.cull) {
-        k_corners<MODE, STATS, MULTI><<<grid_for(c.nwords, 4), 256, 0, ctx->stream>>>(
-            U.segs, c.nsegs, s0, U.word_seg, ctx->bits.as<uint64_t>(), U.ctl, U.sout, c.nwords);
-        return;
+// The call's sizes; places every job's words and bricks.
+int plan_call(std::vector<Job> &jobs, const mc3d_options &o, Call &c) {
+    c.nsegs = (uint32_t)jobs.size();
+    c.mode = o.precision == MC3D_PREC_FP32_GUARD ? 1 : 0;
+    c.stats = o.count_mismatch != 0;
+    c.rows_per_walk = o.rows_per_walk > 0 ? o.rows_per_walk : 1;
+    c.wpb = o.waves_per_brick > 0 ? o.waves_per_brick : 4;
+    c.cull = jobs[0].cap != 0;
+    c.interior = c.nsegs == 1 && jobs[0].interior;
+    c.host = c.nsegs == 1 && jobs[0].host;
+    for (Job &j : jobs) {
+        j.word0 = c.nwords;
+        c.nwords += align_up(j.nwords, 64);
+        j.brick0 = c.nbricks;
+        c.nbricks += align_up(j.nbricks, 128);
+        c.cap = std::max(c.cap, j.cap);
     }
-    // (the words of constant bricks were written by the plan)
-    const unsigned grid = grid_for(c.nbricks, 1, 1u << 15);
-    if (c.rows_per_walk >= 4)
-        launch_corners_act<MODE, STATS, 4, MULTI>(ctx, c, U, grid);
-    else if (c.rows_per_walk == 2)
-        launch_corners_act<MODE, STATS, 2, MULTI>(ctx, c, U, grid);
-    else
-        launch_corners_act<MODE, STATS, 1, MULTI>(ctx, c, U, grid);
+    if (c.nwords >= (1ull << 32)) {
+        set_error("call too large: more than 2^32 occupancy words");
+        return MC3D_E_UNSUPPORTED;
+    }
+    if (c.cull && c.nbricks * c.cap >= (1ull << 32) - 1) {
+        set_error("too many culling bricks");
+        return MC3D_E_UNSUPPORTED;
+    }
+    // One culled solid: the scan takes its words brick by brick (the
+    // positions include the rows of bricks past the grid).
+    c.nscan = c.nwords;
+    const int BY = jobs[0].B.BY, BZ = jobs[0].B.BZ;
+    if (c.cull && c.nsegs == 1 && !c.host && (BY & (BY - 1)) == 0 && (BZ & (BZ - 1)) == 0) {
+        const uint64_t n = c.nbricks * (uint64_t)BY * (uint64_t)BZ;
+        if (n < (1ull << 32)) c.bm = true, c.nscan = n;
+    }
+    c.plan_lds = PLAN_STACK_BYTES;
+    if (c.cull)
+        for (const Job &j : jobs)
+            if (plan_stage_bytes(j.P) <= PLAN_STAGE_LIMIT) c.plan_lds = std::max(c.plan_lds, plan_stage_bytes(j.P));
+    return MC3D_OK;
 }
 
-template <int MODE, bool STATS>
-void launch_corners(mc3d_ctx *ctx, const Call &c, const Upload &U) {
-    c.nsegs > 1 ? launch_corners_m<MODE, STATS, true>(ctx, c, U) : launch_corners_m<MODE, STATS, false>(ctx, c, U);
+uint64_t classify_blocks(const Call &c) { return (c.nscan + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK; }
+
+// The context's buffers for a call (grown as needed).
+int ctx_buffers(mc3d_ctx *ctx, const Call &c, Bufs &b) {
+    int rc;
+    if ((rc = ctx->bits.ensure(c.nwords * 8)) || (rc = ctx->counts.ensure(c.nscan * sizeof(uint32_t))) ||
+        (rc = ctx->offs.ensure(c.nscan * sizeof(uint2))) || (rc = ctx->actw.ensure(c.nwords * sizeof(uint32_t))) ||
+        (rc = ctx->tiles.ensure(classify_blocks(c) * sizeof(uint4))))
+        return rc;
+    if (c.cull && ((rc = ctx->binfo.ensure(c.nbricks * sizeof(uint2))) ||
+                   (rc = ctx->actb.ensure(c.nbricks * sizeof(uint32_t))) ||
+                   (rc = ctx->bcode.ensure(c.nbricks * c.cap * sizeof(Insn) + 16))))
+        return rc;
+    b.bits = ctx->bits.as<uint64_t>();
+    b.counts = ctx->counts.as<uint32_t>();
+    b.offs = ctx->offs.as<uint2>();
+    b.actw = ctx->actw.as<uint32_t>();
+    b.tiles = ctx->tiles.as<uint4>();
+    b.binfo = c.cull ? ctx->binfo.as<uint2>() : nullptr;
+    b.actb = c.cull ? ctx->actb.as<uint32_t>() : nullptr;
+    b.bcode = c.cull ? ctx->bcode.as<Insn>() : nullptr;
+    b.table = ctx->table.as<uint8_t>();
+    b.etable = ctx->etable.as<uint8_t>();
+    return MC3D_OK;
 }
 
-template <int MODE, bool STATS>
-void launch_bisect(mc3d_ctx *ctx, const Call &c, const Upload &U, double *verts, double *interior,
-                   uint64_t vcap) {
-    const unsigned grid = grid_for(vcap, 256, 1u << 14);
-    if (c.nsegs > 1)
-        k_bisect<MODE, STATS, true><<<grid, 256, 0, ctx->stream>>>(
-            U.segs, c.nsegs, U.hsegs[0], ctx->vseg.as<uint32_t>(), ctx->bits.as<uint64_t>(),
-            ctx->vkeys.as<uint64_t>(), verts, interior, U.ctl, vcap);
-    else
-        k_bisect<MODE, STATS, false><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], nullptr,
-                                                                    ctx->bits.as<uint64_t>(),
-                                                                    ctx->vkeys.as<uint64_t>(), verts, interior,
-                                                                    U.ctl, vcap);
+Pipe make_pipe(const Call &c, const Upload &U) {
+    Pipe p;
+    memset((void *)&p, 0, sizeof p);  // padding included: graph keys compare the bytes
+    p.segs = U.segs;
+    p.nsegs = c.nsegs;
+    p.seg0 = U.hsegs[0];
+    p.word_seg = U.word_seg;
+    p.brick_seg = U.brick_seg;
+    p.ctl = U.ctl;
+    p.sout = U.sout;
+    p.nwords = c.nwords;
+    p.nbricks = c.nbricks;
+    p.nscan = c.nscan;
+    p.cap = c.cap;
+    p.cull = c.cull;
+    p.bm = c.bm;
+    p.stats = c.stats;
+    p.mode = c.mode;
+    p.rows_per_walk = c.rows_per_walk;
+    p.wpb = c.wpb;
+    p.plan_lds = c.plan_lds;
+    p.plan_gsz = 64;
+    p.plan_tile = false;
+    p.emit_div = 1024;
+    return p;
+}
+
+// plan, corners, classify and scan, with events between them (ev: the
+// context's events, or null).
+int enqueue_front(const Pipe &p, const Bufs &b, hipStream_t st, const hipEvent_t *ev) {
+    if (ev) HIP_TRY(hipEventRecord(ev[EV_PLAN0], st));
+    if (p.cull) HIP_TRY(launch_plan(p, b, st));
+    if (ev) HIP_TRY(hipEventRecord(ev[EV_PLAN1], st));
+    HIP_TRY(launch_corners(p, b, st));
+    if (ev) HIP_TRY(hipEventRecord(ev[EV_CORNERS], st));
+    HIP_TRY(launch_classify(p, b, st));
+    if (ev) HIP_TRY(hipEventRecord(ev[EV_CLASSIFY], st));
+    HIP_TRY(launch_scan(p, b, st));
+    if (ev) HIP_TRY(hipEventRecord(ev[EV_SCAN], st));
+    return MC3D_OK;
+}
+
+// emit (vertex keys, triangles) and bisect (vertices), then the conj map.
+int enqueue_output(const Pipe &p, const Bufs &b, const Upload &U, double *verts, uint32_t *tris, double *interior,
+                   uint64_t vcap, uint64_t tcap, hipStream_t st, const hipEvent_t *ev) {
+    if (ev) HIP_TRY(hipEventRecord(ev[EV_EMIT0], st));
+    HIP_TRY(launch_emit(p, b, tris, vcap, tcap, st));
+    if (ev) HIP_TRY(hipEventRecord(ev[EV_EMIT1], st));
+    HIP_TRY(launch_bisect(p, b, verts, interior, vcap, st));
+    if (U.conj) HIP_TRY(launch_inverse_apply(U.conj, U.conj_nops, verts, p.ctl, vcap, st));
+    if (ev) HIP_TRY(hipEventRecord(ev[EV_OUT], st));
+    return MC3D_OK;
 }
 
 int take_outputs(mc3d_ctx *ctx, MeshBuffers &mb, const Call &c, uint64_t vcap, uint64_t tcap) {
@@ -650,53 +859,12 @@ int fit_outputs(mc3d_ctx *ctx, MeshBuffers &mb, uint64_t nv, uint64_t nt) {
     return MC3D_OK;
 }
 
-void launch_emit(mc3d_ctx *ctx, const Call &c, const Upload &U, MeshBuffers &mb, uint64_t vcap, uint64_t tcap) {
-    // 2048 blocks, up to 8192 for large grids (a few listed words per wave:
-    // even tails; each block also stages the case table)
-    const unsigned grid = std::max(2048u, grid_for(c.nwords, 128, 8192));
-    if (c.nsegs > 1)
-        k_emit<true><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(),
-                                                    ctx->etable.as<uint8_t>(), ctx->offs.as<uint2>(),
-                                                    ctx->actw.as<uint32_t>(), U.ctl, ctx->vkeys.as<uint64_t>(),
-                                                    ctx->vseg.as<uint32_t>(), mb.tris->as<uint32_t>(), nullptr, vcap,
-                                                    tcap, false);
-    else
-        k_emit<false><<<grid, 256, 0, ctx->stream>>>(U.segs, c.nsegs, U.hsegs[0], U.word_seg,
-                                                     ctx->bits.as<uint64_t>(), ctx->etable.as<uint8_t>(),
-                                                     ctx->offs.as<uint2>(), ctx->actw.as<uint32_t>(), U.ctl,
-                                                     ctx->vkeys.as<uint64_t>(), nullptr, mb.tris->as<uint32_t>(), c.keep,
-                                                     vcap, tcap, c.bm);
-}
-
-// The output stages: emit (vertex keys, triangles) and bisect (vertices).
-int run_output(mc3d_ctx *ctx, const Call &c, const Upload &U, MeshBuffers &mb, uint64_t vcap, uint64_t tcap) {
-    HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
-    launch_emit(ctx, c, U, mb, vcap, tcap);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
-    double *verts = mb.verts->as<double>();
-    double *inter = c.interior ? mb.interior->as<double>() : nullptr;
-    if (c.mode == 0)
-        c.stats ? launch_bisect<0, true>(ctx, c, U, verts, inter, vcap)
-                : launch_bisect<0, false>(ctx, c, U, verts, inter, vcap);
-    else
-        c.stats ? launch_bisect<1, true>(ctx, c, U, verts, inter, vcap)
-                : launch_bisect<1, false>(ctx, c, U, verts, inter, vcap);
-    HIP_TRY(hipGetLastError());
-    if (U.conj) {
-        k_inverse_apply<<<grid_for(vcap, 256, 1u << 14), 256, 0, ctx->stream>>>(U.conj, U.conj_nops, verts, U.ctl,
-                                                                                 vcap);
-        HIP_TRY(hipGetLastError());
-    }
-    HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream));
-    return MC3D_OK;
-}
-
 // The output stages of a solid with host-evaluated atoms (a call of one):
 // the mesh size is read first (one more synchronisation), then emit, then
 // the bisection in rounds with the host answering its atoms between them.
 // Leaves the final control block in ctx->ctl_host.
-int run_output_host(mc3d_ctx *ctx, const Call &c, const Job &j, Upload &U, MeshBuffers &mb, double &ms_cb) {
+int run_output_host(mc3d_ctx *ctx, const Call &c, const Pipe &p, Bufs &b, const Job &j, Upload &U, MeshBuffers &mb,
+                    double &ms_cb) {
     Ctl *hc = (Ctl *)ctx->ctl_host.p;
     HIP_TRY(hipMemcpyAsync(hc, U.ctl, U.back_bytes, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -714,168 +882,206 @@ int run_output_host(mc3d_ctx *ctx, const Call &c, const Job &j, Upload &U, MeshB
     if ((rc = take_outputs(ctx, mb, c, vcap, tcap)) || (rc = ctx->hstate.ensure(vcap * 16)) ||
         (rc = ctx->hprobe.ensure(vcap * 24)))
         return rc;
-    HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
-    launch_emit(ctx, c, U, mb, vcap, tcap);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
+    b.vkeys = ctx->vkeys.as<uint64_t>();
+    HIP_TRY(hipEventRecord(ctx->ev[EV_EMIT0], ctx->stream));
+    HIP_TRY(launch_emit(p, b, mb.tris->as<uint32_t>(), vcap, tcap, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev[EV_EMIT1], ctx->stream));
     double *verts = mb.verts->as<double>(), *state = ctx->hstate.as<double>(), *probe = ctx->hprobe.as<double>();
     double *inter = c.interior ? mb.interior->as<double>() : nullptr;
-    const unsigned grid = grid_for(nv, 256, 1u << 14);
     for (int round = 0; round <= j.iters; round++) {
         if (round > 0 && (rc = host_round(ctx, j, U, 0, probe, nv, ms_cb))) return rc;
-        const DevProg &dp = U.hsegs[0].P;
-        if (c.mode == 1)
-            k_bisect_host<1><<<grid, 256, 0, ctx->stream>>>(dp, U.hsegs[0].g, ctx->bits.as<uint64_t>(),
-                                                            ctx->vkeys.as<uint64_t>(), state, probe, verts, inter,
-                                                            nv, round > 0);
-        else
-            k_bisect_host<0><<<grid, 256, 0, ctx->stream>>>(dp, U.hsegs[0].g, ctx->bits.as<uint64_t>(),
-                                                            ctx->vkeys.as<uint64_t>(), state, probe, verts, inter,
-                                                            nv, round > 0);
-        HIP_TRY(hipGetLastError());
+        HIP_TRY(launch_bisect_host(c.mode, U.hsegs[0].P, U.hsegs[0].g, b.bits, b.vkeys, state, probe, verts, inter,
+                                   nv, round > 0, ctx->stream));
     }
-    if (U.conj) {
-        k_inverse_apply<<<grid_for(nv, 256, 1u << 14), 256, 0, ctx->stream>>>(U.conj, U.conj_nops, verts, U.ctl, nv);
-        HIP_TRY(hipGetLastError());
-    }
-    HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream));
+    if (U.conj) HIP_TRY(launch_inverse_apply(U.conj, U.conj_nops, verts, U.ctl, nv, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev[EV_OUT], ctx->stream));
     HIP_TRY(hipMemcpyAsync(hc, U.ctl, U.back_bytes, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return MC3D_OK;
 }
 
-// Meshes `jobs` (prepared) in one pass of the pipeline; out[i] gets job i's
-// mesh.  A job with host-evaluated atoms is only ever run alone.
-double ms_since(std::chrono::steady_clock::time_point t0) {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+// Pinned host memory for a call's results: vertices, triangles, interior points.
+size_t host_result_bytes(uint64_t nv, uint64_t nt, bool interior) {
+    return align_up(nv * 24 + nt * 12, 256) + (interior ? nv * 24 : 0) + 16;
 }
 
+// Copies of the results into mb.host on the copy stream: the triangles once
+// emit is done (while the bisection runs), the vertices and interior points
+// once the output stages are.
+int enqueue_host_copy(mc3d_ctx *ctx, MeshBuffers &mb, uint64_t nv, uint64_t nt, bool interior) {
+    int rc;
+    if (!mb.host) mb.host = ctx->hpool->take(host_result_bytes(nv, nt, interior));
+    if ((rc = mb.host->ensure(host_result_bytes(nv, nt, interior)))) return rc;
+    mb.nv = nv, mb.nt = nt;
+    char *h = (char *)mb.host->p;
+    HIP_TRY(hipStreamWaitEvent(ctx->copy, ctx->ev[EV_EMIT1], 0));
+    if (nt) HIP_TRY(hipMemcpyAsync(h + nv * 24, mb.tris->p, nt * 12, hipMemcpyDeviceToHost, ctx->copy));
+    HIP_TRY(hipStreamWaitEvent(ctx->copy, ctx->ev[EV_OUT], 0));
+    if (nv) HIP_TRY(hipMemcpyAsync(h, mb.verts->p, nv * 24, hipMemcpyDeviceToHost, ctx->copy));
+    if (interior && nv)
+        HIP_TRY(hipMemcpyAsync(h + align_up(nv * 24 + nt * 12, 256), mb.interior->p, nv * 24, hipMemcpyDeviceToHost,
+                               ctx->copy));
+    return MC3D_OK;
+}
+
+// Everything a captured chain depends on: the kernels' arguments and launch
+// shapes (Pipe, Bufs, outputs and capacities) and the copies' addresses and
+// sizes.  Two calls with equal keys enqueue exactly the same work.
+struct GraphKey {
+    Pipe p;
+    Bufs b;
+    const void *verts, *tris, *interior, *staging, *blob, *ctl_host;
+    const double *conj;
+    uint64_t vcap, tcap, total, back;
+    uint32_t conj_nops, pad;
+};
+
+// The upload, every stage and the control block's copy back, as one HIP
+// graph launch: captured once per call shape and replayed by later calls of
+// the same shape (hipGraphLaunch replaces a dozen enqueues).  No memset is
+// captured: the control block arrives zeroed with the upload.
+int graph_launch(mc3d_ctx *ctx, const Pipe &p, const Bufs &b, const Upload &U, double *verts, uint32_t *tris,
+                 double *interior, uint64_t vcap, uint64_t tcap) {
+    GraphKey k;
+    memset((void *)&k, 0, sizeof k);
+    k.p = p, k.b = b, k.verts = verts, k.tris = tris, k.interior = interior;
+    k.staging = ctx->staging.p, k.blob = ctx->blob.p, k.ctl_host = ctx->ctl_host.p;
+    k.conj = U.conj, k.conj_nops = U.conj_nops;
+    k.vcap = vcap, k.tcap = tcap, k.total = U.total, k.back = U.back_bytes;
+    const uint8_t *kb = (const uint8_t *)&k;
+    hipGraphExec_t exec = nullptr;
+    for (size_t i = 0; i < ctx->graphs.size(); i++) {
+        if (memcmp(ctx->graphs[i].key.data(), kb, sizeof k) == 0) {
+            mc3d_ctx::Graph hit = std::move(ctx->graphs[i]);  // now the most recent
+            ctx->graphs.erase(ctx->graphs.begin() + i);
+            exec = hit.exec;
+            ctx->graphs.push_back(std::move(hit));
+            break;
+        }
+    }
+    if (!exec) {
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        int rc = enqueue_upload(ctx, U);
+        if (!rc) rc = enqueue_front(p, b, ctx->stream, nullptr);
+        if (!rc) rc = enqueue_output(p, b, U, verts, tris, interior, vcap, tcap, ctx->stream, nullptr);
+        if (!rc && hipMemcpyAsync(ctx->ctl_host.p, U.ctl, U.back_bytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+            rc = MC3D_E_HIP;
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+        if (!rc && e == hipSuccess && graph && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+            exec = nullptr;
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!exec) {
+            (void)hipGetLastError();
+            ctx->graphs_off = true;  // enqueue stage by stage from now on
+            return MC3D_E_UNSUPPORTED;
+        }
+        mc3d_ctx::Graph g;
+        g.key.assign(kb, kb + sizeof k);
+        g.exec = exec;
+        ctx->graphs.push_back(std::move(g));
+        while (ctx->graphs.size() > 4) {
+            (void)hipGraphExecDestroy(ctx->graphs.front().exec);
+            ctx->graphs.erase(ctx->graphs.begin());
+        }
+    }
+    HIP_TRY(hipEventRecord(ctx->ev[EV_PLAN0], ctx->stream));
+    HIP_TRY(hipGraphLaunch(exec, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev[EV_OUT], ctx->stream));
+    return MC3D_OK;
+}
+
+// Meshes `jobs` (prepared) in one pass of the pipeline; out[i] gets job i's
+// mesh.  A job with host-evaluated atoms is only ever run alone.
 int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::vector<mc3d_mesh *> &out,
              std::chrono::steady_clock::time_point t_call, double ms_prepare) {
     const auto t_enqueue = std::chrono::steady_clock::now();
-    double ms_enqueue = 0, ms_wait = 0;
+    double ms_enqueue = 0, ms_wait = 0, ms_d2h = 0;
     Call c;
-    c.nsegs = (uint32_t)jobs.size();
-    c.mode = o.precision == MC3D_PREC_FP32_GUARD ? 1 : 0;
-    c.stats = o.count_mismatch != 0;
-    c.rows_per_walk = o.rows_per_walk > 0 ? o.rows_per_walk : 1;
-    c.wpb = o.waves_per_brick > 0 ? o.waves_per_brick : 4;
-    c.cull = jobs[0].cap != 0;
-    c.interior = c.nsegs == 1 && jobs[0].interior;
-    const bool host = c.nsegs == 1 && jobs[0].host;
-    for (Job &j : jobs) {
-        j.word0 = c.nwords;
-        c.nwords += align_up(j.nwords, 64);
-        j.brick0 = c.nbricks;
-        c.nbricks += align_up(j.nbricks, 128);
-        c.cap = std::max(c.cap, j.cap);
-    }
-    if (c.nwords >= (1ull << 32)) {
-        set_error("call too large: more than 2^32 occupancy words");
-        return MC3D_E_UNSUPPORTED;
-    }
-    if (c.cull && c.nbricks * c.cap >= (1ull << 32) - 1) {
-        set_error("too many culling bricks");
-        return MC3D_E_UNSUPPORTED;
-    }
-    // One culled solid: the scan takes its words brick by brick (the
-    // positions include the rows of bricks past the grid).
-    c.nscan = c.nwords;
-    const int BY = jobs[0].B.BY, BZ = jobs[0].B.BZ;
-    if (c.cull && c.nsegs == 1 && !host && (BY & (BY - 1)) == 0 && (BZ & (BZ - 1)) == 0) {
-        const uint64_t n = c.nbricks * (uint64_t)BY * (uint64_t)BZ;
-        if (n < (1ull << 32)) c.bm = true, c.nscan = n;
-    }
-    const unsigned ctiles = (unsigned)((c.nscan + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK);  // classify blocks
     int rc;
-    if ((rc = ctx->bits.ensure(c.nwords * 8)) || (rc = ctx->counts.ensure(c.nscan * sizeof(uint32_t))) ||
-        (rc = ctx->offs.ensure(c.nscan * sizeof(uint2))) || (rc = ctx->actw.ensure(c.nwords * sizeof(uint32_t))) ||
-        (rc = ctx->tiles.ensure((uint64_t)ctiles * sizeof(uint4))))
-        return rc;
-    if (c.cull && ((rc = ctx->binfo.ensure(c.nbricks * sizeof(uint2))) ||
-                   (rc = ctx->actb.ensure(c.nbricks * sizeof(uint32_t))) ||
-                   (rc = ctx->bcode.ensure(c.nbricks * c.cap * sizeof(Insn) + 16))))
-        return rc;
+    if ((rc = plan_call(jobs, o, c))) return rc;
+    Bufs b;
+    if ((rc = ctx_buffers(ctx, c, b))) return rc;
     if (c.nsegs == 1 && !jobs[0].keep.empty()) {
         if ((rc = ctx->keep.ensure(jobs[0].keep.size() * 8))) return rc;
         HIP_TRY(hipMemcpyAsync(ctx->keep.p, jobs[0].keep.data(), jobs[0].keep.size() * 8, hipMemcpyHostToDevice,
                                ctx->stream));
-        c.keep = ctx->keep.as<uint64_t>();
+        b.keep = ctx->keep.as<uint64_t>();
     }
     Upload U;
-    if ((rc = upload(ctx, jobs, c.nwords, c.nbricks, U))) return rc;
-    if ((rc = ctx->ctl_host.ensure(U.back_bytes))) return rc;
+    if ((rc = prepare_upload(ctx, jobs, c.nwords, c.nbricks, b, U))) return rc;
+    if ((rc = ctx->ctl_host.ensure(U.back_bytes)) || (rc = ctx->ctl_early.ensure(U.back_bytes))) return rc;
 
     std::shared_ptr<MeshBuffers> mb = std::make_shared<MeshBuffers>();
     mb->pool = ctx->pool;
+    mb->hpool = ctx->hpool;
     mc3d_stats st{};
-    if (host && (rc = host_round(ctx, jobs[0], U, 0, nullptr, (uint64_t)jobs[0].g.nx * jobs[0].g.ny * jobs[0].g.nz,
-                                 st.ms_host_callbacks)))
-        return rc;
-
-    // ---- plan ----
-    HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
-    if (c.cull) {
-        // Programs that fit are staged in LDS beside the per-thread stacks.
-        size_t lds = PLAN_STACK_BYTES;
-        for (size_t s = 0; s < jobs.size(); s++)
-            if (U.hsegs[s].stage) lds = std::max(lds, plan_stage_bytes(jobs[s].P));
-        const unsigned pgrid = grid_for(c.nbricks, PLAN_BLOCK);
-        if (c.nsegs > 1)
-            k_plan<true><<<pgrid, PLAN_BLOCK, lds, ctx->stream>>>(
-                U.segs, c.nsegs, U.hsegs[0], U.brick_seg, c.nbricks, c.cap, ctx->binfo.as<uint2>(),
-                ctx->bcode.as<Insn>(), ctx->actb.as<uint32_t>(), ctx->bits.as<uint64_t>(), U.ctl, U.sout);
-        else
-            k_plan<false><<<pgrid, PLAN_BLOCK, lds, ctx->stream>>>(
-                U.segs, c.nsegs, U.hsegs[0], U.brick_seg, c.nbricks, c.cap, ctx->binfo.as<uint2>(),
-                ctx->bcode.as<Insn>(), ctx->actb.as<uint32_t>(), ctx->bits.as<uint64_t>(), U.ctl, U.sout);
-        HIP_TRY(hipGetLastError());
-    }
-    HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
-
-    // ---- corners ----
-    if (c.mode == 0)
-        c.stats ? launch_corners<0, true>(ctx, c, U) : launch_corners<0, false>(ctx, c, U);
-    else
-        c.stats ? launch_corners<1, true>(ctx, c, U) : launch_corners<1, false>(ctx, c, U);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
-
-    // ---- classify + scan (+ list of words that emit) ----
-    if (c.nsegs > 1)
-        k_classify<true><<<ctiles, CLASSIFY_BLOCK, 0, ctx->stream>>>(
-            U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
-            ctx->counts.as<uint32_t>(), ctx->tiles.as<uint4>(), nullptr, c.nwords, false);
-    else
-        k_classify<false><<<ctiles, CLASSIFY_BLOCK, 0, ctx->stream>>>(
-            U.segs, c.nsegs, U.hsegs[0], U.word_seg, ctx->bits.as<uint64_t>(), ctx->table.as<uint8_t>(),
-            ctx->counts.as<uint32_t>(), ctx->tiles.as<uint4>(), c.keep, c.nscan, c.bm);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ctx->ev[8], ctx->stream));
-    if ((rc = scan4(ctx, ctx->counts.as<uint32_t>(), ctx->offs.as<uint2>(), c.nscan, U.ctl->totals,
-                    ctx->actw.as<uint32_t>(), U.hsegs[0], c.bm)))
-        return rc;
-    if (c.nsegs > 1) {  // a single solid starts at vertex 0 and triangle 0 (zeroed by the upload)
-        k_seg_ranges<<<grid_for(c.nsegs, 256), 256, 0, ctx->stream>>>(U.segs, c.nsegs, ctx->offs.as<uint2>(), U.sout);
-        HIP_TRY(hipGetLastError());
-    }
-    HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
-
-    // ---- emit + bisect, sized from earlier calls ----
     Ctl *hc = (Ctl *)ctx->ctl_host.p;
-    const SegOut *so = (const SegOut *)((const char *)hc + ((const char *)U.sout - (const char *)U.ctl));
-    if (host) {
-        if ((rc = run_output_host(ctx, c, jobs[0], U, *mb, st.ms_host_callbacks))) return rc;
+    const SegOut *so = (const SegOut *)((const char *)hc + U.o_sout);
+    const bool to_host = !o.device_only;
+    // Results left on the device, no host-evaluated atoms, no statistics, no
+    // block filter: the whole chain is one graph launch.
+    bool graphed = o.device_only && !c.host && !c.stats && !b.keep && !o.no_graph && !ctx->graphs_off;
+    if (c.host) {
+        if ((rc = enqueue_upload(ctx, U))) return rc;
+        if ((rc = host_round(ctx, jobs[0], U, 0, nullptr, (uint64_t)jobs[0].g.nx * jobs[0].g.ny * jobs[0].g.nz,
+                             st.ms_host_callbacks)))
+            return rc;
+        // The kernels take segment 0 from their arguments: the pipe is made
+        // only now, with the host answers host_round placed in the segment.
+        const Pipe p = make_pipe(c, U);
+        if (!p.seg0.P.host_vals) {
+            set_error("internal error: host answers missing from the segment");
+            return MC3D_E_INTERNAL;
+        }
+        if ((rc = enqueue_front(p, b, ctx->stream, ctx->ev))) return rc;
+        if ((rc = run_output_host(ctx, c, p, b, jobs[0], U, *mb, st.ms_host_callbacks))) return rc;
+        if (to_host) {
+            const auto t0 = std::chrono::steady_clock::now();
+            if ((rc = enqueue_host_copy(ctx, *mb, hc->totals[0], hc->totals[1], c.interior))) return rc;
+            HIP_TRY(hipStreamSynchronize(ctx->copy));
+            ms_d2h = ms_since(t0);
+        }
     } else {
+        const Pipe p = make_pipe(c, U);
         uint64_t vcap = ctx->v_hint, tcap = ctx->t_hint;
         if ((rc = take_outputs(ctx, *mb, c, vcap, tcap))) return rc;
-        if ((rc = run_output(ctx, c, U, *mb, vcap, tcap))) return rc;
-        HIP_TRY(hipMemcpyAsync(hc, U.ctl, U.back_bytes, hipMemcpyDeviceToHost, ctx->stream));
+        b.vkeys = ctx->vkeys.as<uint64_t>();
+        b.vseg = c.nsegs > 1 ? ctx->vseg.as<uint32_t>() : nullptr;
+        double *verts = mb->verts->as<double>(), *inter = c.interior ? mb->interior->as<double>() : nullptr;
+        uint32_t *tris = mb->tris->as<uint32_t>();
+        if (graphed) {
+            rc = graph_launch(ctx, p, b, U, verts, tris, inter, vcap, tcap);
+            if (rc == MC3D_E_UNSUPPORTED) graphed = false;  // the capture failed: enqueue stage by stage
+            else if (rc) return rc;
+        }
+        // The totals after the scan, copied while the output stages run:
+        // when the meshes fit, their copies to the host are enqueued at once.
+        Ctl *he = (Ctl *)ctx->ctl_early.p;
+        if (!graphed) {
+            if ((rc = enqueue_upload(ctx, U)) || (rc = enqueue_front(p, b, ctx->stream, ctx->ev))) return rc;
+            if (to_host) {
+                HIP_TRY(hipMemcpyAsync(he, U.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
+                HIP_TRY(hipEventRecord(ctx->ev[EV_CTL], ctx->stream));
+            }
+            if ((rc = enqueue_output(p, b, U, verts, tris, inter, vcap, tcap, ctx->stream, ctx->ev))) return rc;
+            HIP_TRY(hipMemcpyAsync(hc, U.ctl, U.back_bytes, hipMemcpyDeviceToHost, ctx->stream));
+        }
+        bool copying = false;
+        if (to_host) {
+            HIP_TRY(hipEventSynchronize(ctx->ev[EV_CTL]));
+            const uint64_t nv = he->totals[0], nt = he->totals[1];
+            if (!he->inside && nv <= vcap && nt <= tcap) {
+                if ((rc = enqueue_host_copy(ctx, *mb, nv, nt, c.interior))) return rc;
+                copying = true;
+            }
+        }
         ms_enqueue = ms_since(t_enqueue);
         const auto t_wait = std::chrono::steady_clock::now();
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         ms_wait = ms_since(t_wait);
         if (hc->inside) {
+            if (copying) (void)hipStreamSynchronize(ctx->copy);
             size_t bad = 0;
             while (bad + 1 < jobs.size() && !so[bad].inside) bad++;
             set_error(c.nsegs == 1 ? std::string("solid is true outside of bounds")
@@ -884,6 +1090,7 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
         }
         const uint64_t nv = hc->totals[0], nt = hc->totals[1];
         if (nv >= (1ull << 32) || nt >= (1ull << 32)) {
+            if (copying) (void)hipStreamSynchronize(ctx->copy);
             set_error("mesh too large for 32-bit vertex indices");
             return MC3D_E_UNSUPPORTED;
         }
@@ -892,15 +1099,27 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
             ctx->t_hint = std::max(ctx->t_hint, nt + nt / 4 + 1024);
             vcap = ctx->v_hint, tcap = ctx->t_hint;
             if ((rc = take_outputs(ctx, *mb, c, vcap, tcap))) return rc;
+            b.vkeys = ctx->vkeys.as<uint64_t>();
+            b.vseg = c.nsegs > 1 ? ctx->vseg.as<uint32_t>() : nullptr;
             // the bisection's operation counts start again with the second run
             HIP_TRY(hipMemsetAsync(U.ctl->flops + 2, 0, 2 * sizeof(unsigned long long), ctx->stream));
-            if ((rc = run_output(ctx, c, U, *mb, vcap, tcap))) return rc;
+            if ((rc = enqueue_output(p, b, U, mb->verts->as<double>(), mb->tris->as<uint32_t>(),
+                                     c.interior ? mb->interior->as<double>() : nullptr, vcap, tcap, ctx->stream,
+                                     ctx->ev)))
+                return rc;
             HIP_TRY(hipMemcpyAsync(hc, U.ctl, U.back_bytes, hipMemcpyDeviceToHost, ctx->stream));
+            if (to_host && (rc = enqueue_host_copy(ctx, *mb, nv, nt, c.interior))) return rc;
+            copying = to_host;
             HIP_TRY(hipStreamSynchronize(ctx->stream));
+        }
+        if (copying) {
+            const auto t0 = std::chrono::steady_clock::now();
+            HIP_TRY(hipStreamSynchronize(ctx->copy));
+            ms_d2h = ms_since(t0);
         }
     }
     const uint64_t nv = hc->totals[0], nt = hc->totals[1];
-    if ((rc = fit_outputs(ctx, *mb, nv, nt))) return rc;
+    if (!to_host && (rc = fit_outputs(ctx, *mb, nv, nt))) return rc;
 
     st.bricks_active = hc->nact_bricks;
     st.guard_evals = hc->gstats[0];
@@ -914,14 +1133,17 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
     st.brick_code_total = hc->code_total;
     st.brick_code_max = hc->code_max;
     st.bricks_full = hc->full_bricks;
-    st.ms_plan = elapsed(ctx->ev[0], ctx->ev[1]);
     st.ms_fill = 0;  // the plan writes the constant bricks' words
-    st.ms_corners = elapsed(ctx->ev[1], ctx->ev[2]);
-    st.ms_classify = elapsed(ctx->ev[2], ctx->ev[8]);
-    st.ms_scan = elapsed(ctx->ev[8], ctx->ev[3]);
-    st.ms_emit = elapsed(ctx->ev[4], ctx->ev[5]);
-    st.ms_bisect = elapsed(ctx->ev[5], ctx->ev[6]);
-    st.ms_device = elapsed(ctx->ev[0], ctx->ev[6]);
+    if (!graphed) {  // (a graph launch has no events between its stages)
+        st.ms_plan = elapsed(ctx->ev[EV_PLAN0], ctx->ev[EV_PLAN1]);
+        st.ms_corners = elapsed(ctx->ev[EV_PLAN1], ctx->ev[EV_CORNERS]);
+        st.ms_classify = elapsed(ctx->ev[EV_CORNERS], ctx->ev[EV_CLASSIFY]);
+        st.ms_scan = elapsed(ctx->ev[EV_CLASSIFY], ctx->ev[EV_SCAN]);
+        st.ms_emit = elapsed(ctx->ev[EV_EMIT0], ctx->ev[EV_EMIT1]);
+        st.ms_bisect = elapsed(ctx->ev[EV_EMIT1], ctx->ev[EV_OUT]);
+    }
+    st.ms_device = elapsed(ctx->ev[EV_PLAN0], ctx->ev[EV_OUT]);
+    st.ms_d2h = ms_d2h;
 
     // One mesh per job: its slice of the outputs.
     std::vector<std::unique_ptr<mc3d_mesh>> meshes(jobs.size());
@@ -933,6 +1155,7 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
         m->device = ctx->device;
         m->buf = mb;
         m->has_interior = c.interior;
+        m->on_host = to_host;
         m->voff = so[s].vbase;
         m->toff = so[s].tbase;
         m->nv = (s + 1 < jobs.size() ? so[s + 1].vbase : nv) - m->voff;
@@ -947,34 +1170,10 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
         ms.bricks_total = j.nbricks;
         ms.program_length = j.P.code.size();
     }
-    if (!o.device_only) {
-        auto t0 = std::chrono::steady_clock::now();
-        // one copy of all outputs through pinned memory, then split per mesh
-        const size_t ibytes = c.interior ? nv * 24 : 0;
-        if ((rc = ctx->outstage.ensure(nv * 24 + nt * 12 + ibytes + 16))) return rc;
-        char *hs = (char *)ctx->outstage.p;
-        if (nv) HIP_TRY(hipMemcpyAsync(hs, mb->verts->p, nv * 24, hipMemcpyDeviceToHost, ctx->stream));
-        if (nt) HIP_TRY(hipMemcpyAsync(hs + nv * 24, mb->tris->p, nt * 12, hipMemcpyDeviceToHost, ctx->stream));
-        if (ibytes)
-            HIP_TRY(hipMemcpyAsync(hs + nv * 24 + nt * 12, mb->interior->p, ibytes, hipMemcpyDeviceToHost,
-                                   ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        const double *hv = (const double *)hs;
-        const uint32_t *ht = (const uint32_t *)(hs + nv * 24);
-        for (auto &m : meshes) {
-            m->hverts.reset(new double[3 * m->nv]);
-            m->htris.reset(new uint32_t[3 * m->nt]);
-            parallel_copy(m->hverts.get(), hv + 3 * m->voff, m->nv * 24);
-            parallel_copy(m->htris.get(), ht + 3 * m->toff, m->nt * 12);
-            if (ibytes) {
-                const double *hi = (const double *)(hs + nv * 24 + nt * 12);
-                m->hinterior.reset(new double[3 * m->nv]);
-                parallel_copy(m->hinterior.get(), hi + 3 * m->voff, m->nv * 24);
-            }
-            m->on_host = true;
-        }
-        const double ms_d2h = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        for (auto &m : meshes) m->stats.ms_d2h = ms_d2h;
+    if (to_host) {  // the device outputs go back to the pool now
+        ctx->pool->give(std::move(mb->verts));
+        ctx->pool->give(std::move(mb->tris));
+        ctx->pool->give(std::move(mb->interior));
     }
     const double ms_call = ms_since(t_call);
     out.resize(jobs.size());
@@ -1003,64 +1202,72 @@ extern "C" {
 int mc3d_ctx_create(int device, mc3d_ctx **out) {
     if (!out) return MC3D_E_INVALID_ARG;
     *out = nullptr;
-    int n = 0;
-    HIP_TRY(hipGetDeviceCount(&n));
-    if (device < 0 || device >= n) {
-        set_error("mc3d_ctx_create: no such HIP device");
-        return MC3D_E_INVALID_ARG;
-    }
-    HIP_TRY(hipSetDevice(device));
-    mc3d_ctx *c = new (std::nothrow) mc3d_ctx;
-    if (!c) return MC3D_E_OOM;
-    c->device = device;
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
-        delete c;
-        return MC3D_E_HIP;
-    }
-    for (auto &ev : c->ev) {
-        if (hipEventCreate(&ev) != hipSuccess) {
-            set_error("hipEventCreate failed");
-            mc3d_ctx_destroy(c);
+    return guarded([&]() -> int {
+        int n = 0;
+        HIP_TRY(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) {
+            set_error("mc3d_ctx_create: no such HIP device");
+            return MC3D_E_INVALID_ARG;
+        }
+        HIP_TRY(hipSetDevice(device));
+        std::unique_ptr<mc3d_ctx> c(new mc3d_ctx);
+        c->device = device;
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+            mc3d_ctx_destroy(c.release());
             return MC3D_E_HIP;
         }
-    }
-    // The emitter's copy of the case table: each edge e = 4 * axis + r
-    // decoded to axis | dx << 2 | dy << 3 | dz << 4 (its offset from the
-    // cube's low corner); entry 0 of a case stays its triangle count.
-    uint8_t etab[256 * 16];
-    const uint8_t *tab = case_table();
-    for (int i = 0; i < 256 * 16; i++) {
-        const int t = tab[i];
-        if ((i & 15) == 0) {
-            etab[i] = (uint8_t)t;
-            continue;
+        for (auto &ev : c->ev) {
+            if (hipEventCreate(&ev) != hipSuccess) {
+                set_error("hipEventCreate failed");
+                mc3d_ctx_destroy(c.release());
+                return MC3D_E_HIP;
+            }
         }
-        const int axis = (t >> 2) & 3, d0 = t & 1, d1 = (t >> 1) & 1;  // d0, d1: the other two axes in x, y, z order
-        const int dx = axis == 0 ? 0 : d0, dy = axis == 0 ? d0 : (axis == 1 ? 0 : d1), dz = axis == 2 ? 0 : d1;
-        etab[i] = (uint8_t)(axis | (dx << 2) | (dy << 3) | (dz << 4));
-    }
-    if (c->ctl_host.ensure(sizeof(Ctl) + 4096) || c->table.ensure(256 * 16) || c->etable.ensure(256 * 16) ||
-        hipMemcpy(c->table.p, tab, 256 * 16, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->etable.p, etab, 256 * 16, hipMemcpyHostToDevice) != hipSuccess) {
-        set_error("mc3d_ctx_create: setup failed");
-        mc3d_ctx_destroy(c);
-        return MC3D_E_HIP;
-    }
-    *out = c;
-    return MC3D_OK;
+        // The emitter's copy of the case table: each edge e = 4 * axis + r
+        // decoded to axis | dx << 2 | dy << 3 | dz << 4 (its offset from the
+        // cube's low corner); entry 0 of a case stays its triangle count.
+        uint8_t etab[256 * 16];
+        const uint8_t *tab = case_table();
+        for (int i = 0; i < 256 * 16; i++) {
+            const int t = tab[i];
+            if ((i & 15) == 0) {
+                etab[i] = (uint8_t)t;
+                continue;
+            }
+            const int axis = (t >> 2) & 3, d0 = t & 1, d1 = (t >> 1) & 1;  // d0, d1: the other two axes in x, y, z order
+            const int dx = axis == 0 ? 0 : d0, dy = axis == 0 ? d0 : (axis == 1 ? 0 : d1), dz = axis == 2 ? 0 : d1;
+            etab[i] = (uint8_t)(axis | (dx << 2) | (dy << 3) | (dz << 4));
+        }
+        if (c->ctl_host.ensure(sizeof(Ctl) + 4096) || c->ctl_early.ensure(sizeof(Ctl) + 4096) ||
+            c->table.ensure(256 * 16) || c->etable.ensure(256 * 16) ||
+            hipMemcpy(c->table.p, tab, 256 * 16, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->etable.p, etab, 256 * 16, hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("mc3d_ctx_create: setup failed");
+            mc3d_ctx_destroy(c.release());
+            return MC3D_E_HIP;
+        }
+        *out = c.release();
+        return MC3D_OK;
+    });
 }
 
 void mc3d_ctx_destroy(mc3d_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->copy) (void)hipStreamSynchronize(ctx->copy);
+    for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
     for (auto &ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->copy) (void)hipStreamDestroy(ctx->copy);
     delete ctx;
 }
+
+}  // extern "C"
 
 namespace {
 // One solid, one call: MarchingCubesSearch, with interior points
@@ -1090,35 +1297,40 @@ int search_one(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters, cons
 }
 }  // namespace
 
+struct mc3d_rect_collider {
+    RectMesh mesh;
+};
+
+extern "C" {
+
 int mc3d_marching_cubes_search_filter(mc3d_ctx *ctx, const mc3d_solid *s, mc3d_rect_filter_fn f, void *user,
                                       double delta, int iters, const mc3d_options *opts, mc3d_mesh **out) {
     if (!ctx || !s || !f || !out) {
         set_error("mc3d_marching_cubes_search_filter: null argument");
         return MC3D_E_INVALID_ARG;
     }
-    const RectFilter filter = [f, user](const double *mn, const double *mx) { return f(user, mn, mx) != 0; };
-    // a caller's filter is asked on this thread; the library's own rect
-    // collider is thread-safe and runs on 16
-    const int threads = f == mc3d_rect_collider_collides ? 16 : 1;
-    return search_one(ctx, s, delta, iters, opts, out, false, {}, &filter, threads);
+    return guarded([&]() -> int {
+        const RectFilter filter = [f, user](const double *mn, const double *mx) { return f(user, mn, mx) != 0; };
+        // a caller's filter is asked on this thread; the library's own rect
+        // collider is thread-safe and runs on 16
+        const int threads = f == mc3d_rect_collider_collides ? 16 : 1;
+        return search_one(ctx, s, delta, iters, opts, out, false, {}, &filter, threads);
+    });
 }
-
-struct mc3d_rect_collider {
-    RectMesh mesh;
-};
 
 mc3d_rect_collider *mc3d_rect_collider_create(const double *triangles, size_t n) {
     if (n && !triangles) {
         set_error("mc3d_rect_collider_create: null triangles");
         return nullptr;
     }
-    mc3d_rect_collider *c = new (std::nothrow) mc3d_rect_collider;
-    if (!c) {
-        set_error("out of memory");
-        return nullptr;
-    }
-    c->mesh.build(triangles, n);
-    return c;
+    mc3d_rect_collider *c = nullptr;
+    const int rc = guarded([&]() -> int {
+        std::unique_ptr<mc3d_rect_collider> m(new mc3d_rect_collider);
+        m->mesh.build(triangles, n);
+        c = m.release();
+        return MC3D_OK;
+    });
+    return rc ? nullptr : c;
 }
 
 int mc3d_rect_collider_collides(void *collider, const double min[3], const double max[3]) {
@@ -1143,31 +1355,33 @@ int mc3d_marching_cubes_c2f(mc3d_ctx *ctx, const mc3d_solid *s, double big_delta
         set_error("bigDelta should be >= smallDelta for efficiency");  // mc.go:162-164
         return MC3D_E_INVALID_ARG;
     }
-    extra_space += 2 * big_delta * sqrt(3.0);  // mc.go:167
-    // the coarse mesh (mc.go:169), its triangles on the host
-    mc3d_options co = default_options(opts);
-    co.device_only = 1;
-    mc3d_mesh *coarse = nullptr;
-    int rc = search_one(ctx, s, big_delta, iters, &co, &coarse, false, {});
-    if (rc) return rc;
-    std::vector<double> v(coarse->nv * 3);
-    std::vector<uint32_t> t(coarse->nt * 3);
-    rc = mc3d_mesh_copy(coarse, v.data(), t.data());
-    const size_t nt = coarse->nt;
-    mc3d_mesh_free(coarse);
-    if (rc) return rc;
-    std::vector<double> tris(nt * 9);
-    for (size_t i = 0; i < 3 * nt; i++)
-        for (int d = 0; d < 3; d++) tris[3 * i + d] = v[3 * (size_t)t[i] + d];
-    RectMesh collider;  // MeshToCollider(coarseMesh), mc.go:170
-    collider.build(tris.data(), nt);
-    // filter = collider.RectCollision(r.Expand(extraSpace)), mc.go:171-174
-    const RectFilter filter = [&collider, extra_space](const double *mn, const double *mx) {
-        const double emn[3] = {mn[0] + -extra_space, mn[1] + -extra_space, mn[2] + -extra_space};
-        const double emx[3] = {mx[0] + extra_space, mx[1] + extra_space, mx[2] + extra_space};
-        return collider.collides(emn, emx);
-    };
-    return search_one(ctx, s, small_delta, iters, opts, out, false, {}, &filter, 16);
+    return guarded([&]() -> int {
+        double extra = extra_space + 2 * big_delta * sqrt(3.0);  // mc.go:167
+        // the coarse mesh (mc.go:169), its triangles on the host
+        mc3d_options co = default_options(opts);
+        co.device_only = 1;
+        mc3d_mesh *coarse = nullptr;
+        int rc = search_one(ctx, s, big_delta, iters, &co, &coarse, false, {});
+        if (rc) return rc;
+        std::vector<double> v(coarse->nv * 3);
+        std::vector<uint32_t> t(coarse->nt * 3);
+        rc = mc3d_mesh_copy(coarse, v.data(), t.data());
+        const size_t nt = coarse->nt;
+        mc3d_mesh_free(coarse);
+        if (rc) return rc;
+        std::vector<double> tris(nt * 9);
+        for (size_t i = 0; i < 3 * nt; i++)
+            for (int d = 0; d < 3; d++) tris[3 * i + d] = v[3 * (size_t)t[i] + d];
+        RectMesh collider;  // MeshToCollider(coarseMesh), mc.go:170
+        collider.build(tris.data(), nt);
+        // filter = collider.RectCollision(r.Expand(extraSpace)), mc.go:171-174
+        const RectFilter filter = [&collider, extra](const double *mn, const double *mx) {
+            const double emn[3] = {mn[0] + -extra, mn[1] + -extra, mn[2] + -extra};
+            const double emx[3] = {mx[0] + extra, mx[1] + extra, mx[2] + extra};
+            return collider.collides(emn, emx);
+        };
+        return search_one(ctx, s, small_delta, iters, opts, out, false, {}, &filter, 16);
+    });
 }
 
 int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
@@ -1176,7 +1390,7 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
         set_error("mc3d_marching_cubes_search: null argument");
         return MC3D_E_INVALID_ARG;
     }
-    return search_one(ctx, s, delta, iters, opts, out, false, {});
+    return guarded([&]() { return search_one(ctx, s, delta, iters, opts, out, false, {}); });
 }
 
 int mc3d_marching_cubes_interior(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
@@ -1185,7 +1399,7 @@ int mc3d_marching_cubes_interior(mc3d_ctx *ctx, const mc3d_solid *s, double delt
         set_error("mc3d_marching_cubes_interior: null argument");
         return MC3D_E_INVALID_ARG;
     }
-    return search_one(ctx, s, delta, iters, opts, out, true, {});
+    return guarded([&]() { return search_one(ctx, s, delta, iters, opts, out, true, {}); });
 }
 
 int mc3d_marching_cubes_conj(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
@@ -1195,12 +1409,13 @@ int mc3d_marching_cubes_conj(mc3d_ctx *ctx, const mc3d_solid *s, double delta, i
         return MC3D_E_INVALID_ARG;
     }
     *out = nullptr;
-    // TransformSolid(JoinedTransform(xforms), s), mc.go:67-68
-    mc3d_solid *t = mc3d_transform(const_cast<mc3d_solid *>(s), xforms, n);
-    if (!t) return MC3D_E_INVALID_ARG;  // the error is set
-    const int rc = search_one(ctx, t, delta, iters, opts, out, false, inverse_records(xforms, n));
-    mc3d_solid_release(t);
-    return rc;
+    return guarded([&]() -> int {
+        // TransformSolid(JoinedTransform(xforms), s), mc.go:67-68
+        mc3d_solid *t = mc3d_transform(const_cast<mc3d_solid *>(s), xforms, n);
+        if (!t) return MC3D_E_INVALID_ARG;  // the error is set
+        std::unique_ptr<mc3d_solid, void (*)(mc3d_solid *)> hold(t, mc3d_solid_release);
+        return search_one(ctx, t, delta, iters, opts, out, false, inverse_records(xforms, n));
+    });
 }
 
 int mc3d_marching_cubes_search_batch(mc3d_ctx *ctx, const mc3d_solid *const *solids, const double *deltas,
@@ -1217,67 +1432,71 @@ int mc3d_marching_cubes_search_batch(mc3d_ctx *ctx, const mc3d_solid *const *sol
         }
     }
     if (n == 0) return MC3D_OK;
-    const auto t_call = std::chrono::steady_clock::now();
-    const mc3d_options o = default_options(opts);
-    HIP_TRY(hipSetDevice(ctx->device));
-    // Bounds, spacers and lowering of every solid, on up to 16 host threads
-    // (solid trees are immutable; errors are kept per solid).
-    std::vector<Job> all(n);
-    std::vector<int> rcs(n, MC3D_OK);
-    std::vector<std::string> errs(n);
-    std::atomic<size_t> next{0};
-    auto work = [&]() {
-        for (size_t i; (i = next.fetch_add(1)) < n;) {
-            all[i].s = solids[i], all[i].delta = deltas[i], all[i].iters = iters[i];
-            rcs[i] = prepare_job(all[i], o);
-            if (rcs[i]) errs[i] = mc3d_last_error();
-        }
-    };
-    const size_t nthreads = std::min<size_t>({16, std::max(1u, std::thread::hardware_concurrency()), (n + 7) / 8});
-    std::vector<std::thread> pool;
-    for (size_t t = 1; t < nthreads; t++) pool.emplace_back(work);
-    work();
-    for (std::thread &t : pool) t.join();
-    // Solids with host-evaluated atoms need their own callback rounds and run
-    // alone; all others share one pass of the pipeline.
-    std::vector<Job> batched, alone;
-    std::vector<size_t> batched_at, alone_at;
-    for (size_t i = 0; i < n; i++) {
-        if (rcs[i]) {
-            set_error("solid " + std::to_string(i) + " of the batch: " + errs[i]);
-            return rcs[i];
-        }
-        Job &j = all[i];
-        if (j.host) {
-            alone.push_back(std::move(j));
-            alone_at.push_back(i);
-        } else {
-            batched.push_back(std::move(j));
-            batched_at.push_back(i);
-        }
-    }
-    const double ms_prepare = ms_since(t_call);
     std::vector<mc3d_mesh *> got(n, nullptr);
-    auto fail = [&](int rc) {
+    const int rc = guarded([&]() -> int {
+        const auto t_call = std::chrono::steady_clock::now();
+        const mc3d_options o = default_options(opts);
+        HIP_TRY(hipSetDevice(ctx->device));
+        // Bounds, spacers and lowering of every solid, on up to 16 host threads
+        // (solid trees are immutable; errors are kept per solid).
+        std::vector<Job> all(n);
+        std::vector<int> rcs(n, MC3D_OK);
+        std::vector<std::string> errs(n);
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t i; (i = next.fetch_add(1)) < n;) {
+                all[i].s = solids[i], all[i].delta = deltas[i], all[i].iters = iters[i];
+                rcs[i] = guarded([&]() { return prepare_job(all[i], o); });
+                if (rcs[i]) errs[i] = mc3d_last_error();
+            }
+        };
+        const size_t nthreads =
+            std::min<size_t>({16, std::max(1u, std::thread::hardware_concurrency()), (n + 7) / 8});
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nthreads; t++) pool.emplace_back(work);
+        work();
+        for (std::thread &t : pool) t.join();
+        // Solids with host-evaluated atoms need their own callback rounds and run
+        // alone; all others share one pass of the pipeline.
+        std::vector<Job> batched, alone;
+        std::vector<size_t> batched_at, alone_at;
+        for (size_t i = 0; i < n; i++) {
+            if (rcs[i]) {
+                set_error("solid " + std::to_string(i) + " of the batch: " + errs[i]);
+                return rcs[i];
+            }
+            Job &j = all[i];
+            if (j.host) {
+                alone.push_back(std::move(j));
+                alone_at.push_back(i);
+            } else {
+                batched.push_back(std::move(j));
+                batched_at.push_back(i);
+            }
+        }
+        const double ms_prepare = ms_since(t_call);
+        if (!batched.empty()) {
+            std::vector<mc3d_mesh *> ms;
+            int r = run_call(ctx, batched, o, ms, t_call, ms_prepare);
+            if (r) return r;
+            for (size_t k = 0; k < ms.size(); k++) got[batched_at[k]] = ms[k];
+        }
+        for (size_t k = 0; k < alone.size(); k++) {
+            std::vector<Job> one;
+            one.push_back(std::move(alone[k]));
+            std::vector<mc3d_mesh *> ms;
+            int r = run_call(ctx, one, o, ms, std::chrono::steady_clock::now(), 0.0);
+            if (r) {
+                set_error("solid " + std::to_string(alone_at[k]) + " of the batch: " + mc3d_last_error());
+                return r;
+            }
+            got[alone_at[k]] = ms[0];
+        }
+        return MC3D_OK;
+    });
+    if (rc) {
         for (mc3d_mesh *m : got) delete m;
         return rc;
-    };
-    if (!batched.empty()) {
-        std::vector<mc3d_mesh *> ms;
-        int rc = run_call(ctx, batched, o, ms, t_call, ms_prepare);
-        if (rc) return fail(rc);
-        for (size_t k = 0; k < ms.size(); k++) got[batched_at[k]] = ms[k];
-    }
-    for (size_t k = 0; k < alone.size(); k++) {
-        std::vector<Job> one;
-        one.push_back(std::move(alone[k]));
-        std::vector<mc3d_mesh *> ms;
-        int rc = run_call(ctx, one, o, ms, std::chrono::steady_clock::now(), 0.0);
-        if (rc) {
-            set_error("solid " + std::to_string(alone_at[k]) + " of the batch: " + mc3d_last_error());
-            return fail(rc);
-        }
-        got[alone_at[k]] = ms[0];
     }
     for (size_t i = 0; i < n; i++) out[i] = got[i];
     return MC3D_OK;
@@ -1290,22 +1509,36 @@ int mc3d_mesh_counts(const mc3d_mesh *m, size_t *nv, size_t *nt) {
     return MC3D_OK;
 }
 
-int mc3d_mesh_copy(mc3d_mesh *m, double *verts, uint32_t *tris) {
-    if (!m) return MC3D_E_INVALID_ARG;
-    if (m->on_host) {
-        if (verts && m->nv) parallel_copy(verts, m->hverts.get(), m->nv * 24);
-        if (tris && m->nt) parallel_copy(tris, m->htris.get(), m->nt * 12);
-        return MC3D_OK;
+int mc3d_mesh_host_ptrs(const mc3d_mesh *m, const double **verts, const uint32_t **tris) {
+    if (!m || !m->on_host) {
+        set_error("mc3d_mesh_host_ptrs: not a mesh fetched to the host (device_only was set)");
+        return MC3D_E_INVALID_ARG;
     }
-    HIP_TRY(hipSetDevice(m->device));
-    if (verts && m->nv)
-        HIP_TRY(hipMemcpy(verts, m->buf->verts->as<double>() + 3 * m->voff, m->nv * 24, hipMemcpyDeviceToHost));
-    if (tris && m->nt)
-        HIP_TRY(hipMemcpy(tris, m->buf->tris->as<uint32_t>() + 3 * m->toff, m->nt * 12, hipMemcpyDeviceToHost));
+    if (verts) *verts = m->hverts();
+    if (tris) *tris = m->htris();
     return MC3D_OK;
 }
 
+int mc3d_mesh_copy(mc3d_mesh *m, double *verts, uint32_t *tris) {
+    if (!m) return MC3D_E_INVALID_ARG;
+    return guarded([&]() -> int {
+        if (m->on_host) {
+            if (verts && m->nv) parallel_copy(verts, m->hverts(), m->nv * 24);
+            if (tris && m->nt) parallel_copy(tris, m->htris(), m->nt * 12);
+            return MC3D_OK;
+        }
+        HIP_TRY(hipSetDevice(m->device));
+        if (verts && m->nv)
+            HIP_TRY(hipMemcpy(verts, m->buf->verts->as<double>() + 3 * m->voff, m->nv * 24, hipMemcpyDeviceToHost));
+        if (tris && m->nt)
+            HIP_TRY(hipMemcpy(tris, m->buf->tris->as<uint32_t>() + 3 * m->toff, m->nt * 12, hipMemcpyDeviceToHost));
+        return MC3D_OK;
+    });
+}
+
 size_t mc3d_stl_bytes(size_t n_triangles) { return 84 + 50 * n_triangles; }
+
+}  // extern "C"
 
 namespace {
 // The binary STL of nt device triangles into out (host): the body by
@@ -1328,8 +1561,7 @@ int encode_stl_device(mc3d_ctx *ctx, const double *verts, const uint32_t *tris, 
     const size_t body = 50 * nt;
     int rc;
     if ((rc = ctx->stl.ensure(body)) || (rc = ctx->outstage.ensure(body))) return rc;
-    k_encode_stl<<<grid_for(nt, 256, 1u << 16), 256, 0, ctx->stream>>>(verts, tris, nt, ctx->stl.as<uint16_t>());
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_encode_stl(verts, tris, nt, ctx->stl.as<uint16_t>(), ctx->stream));
     HIP_TRY(hipMemcpyAsync(ctx->outstage.p, ctx->stl.p, body, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     parallel_copy(out + 84, ctx->outstage.p, body);
@@ -1337,19 +1569,25 @@ int encode_stl_device(mc3d_ctx *ctx, const double *verts, const uint32_t *tris, 
 }
 }  // namespace
 
+extern "C" {
+
 int mc3d_mesh_encode_stl(mc3d_ctx *ctx, const mc3d_mesh *m, uint8_t *out, size_t out_bytes) {
     if (!ctx || !m || !out) {
         set_error("mc3d_mesh_encode_stl: null argument");
         return MC3D_E_INVALID_ARG;
     }
-    if (m->device != ctx->device || !m->buf || !m->buf->verts || !m->buf->tris) {
+    if (m->device != ctx->device) {
         set_error("mc3d_mesh_encode_stl: the mesh is not on this context's device");
         return MC3D_E_INVALID_ARG;
     }
-    HIP_TRY(hipSetDevice(ctx->device));
-    // triangle indices count from the mesh's first vertex
-    return encode_stl_device(ctx, m->buf->verts->as<double>() + 3 * m->voff,
-                             m->buf->tris->as<uint32_t>() + 3 * m->toff, m->nt, out, out_bytes);
+    return guarded([&]() -> int {
+        HIP_TRY(hipSetDevice(ctx->device));
+        if (m->on_host)  // the mesh's pinned copy goes up again
+            return mc3d_encode_stl(ctx, m->hverts(), m->nv, m->htris(), m->nt, out, out_bytes);
+        // triangle indices count from the mesh's first vertex
+        return encode_stl_device(ctx, m->buf->verts->as<double>() + 3 * m->voff,
+                                 m->buf->tris->as<uint32_t>() + 3 * m->toff, m->nt, out, out_bytes);
+    });
 }
 
 int mc3d_encode_stl(mc3d_ctx *ctx, const double *verts, size_t nv, const uint32_t *tris, size_t nt, uint8_t *out,
@@ -1364,14 +1602,16 @@ int mc3d_encode_stl(mc3d_ctx *ctx, const double *verts, size_t nv, const uint32_
                       std::to_string(tris[i]) + " of " + std::to_string(nv));
             return MC3D_E_INVALID_ARG;
         }
-    HIP_TRY(hipSetDevice(ctx->device));
-    int rc;
-    if ((rc = ctx->stl_in.ensure(nv * 24 + nt * 12 + 16))) return rc;
-    double *dv = ctx->stl_in.as<double>();
-    uint32_t *dt = (uint32_t *)(ctx->stl_in.as<char>() + align_up(nv * 24, 16));
-    if (nv) HIP_TRY(hipMemcpyAsync(dv, verts, nv * 24, hipMemcpyHostToDevice, ctx->stream));
-    if (nt) HIP_TRY(hipMemcpyAsync(dt, tris, nt * 12, hipMemcpyHostToDevice, ctx->stream));
-    return encode_stl_device(ctx, dv, dt, nt, out, out_bytes);
+    return guarded([&]() -> int {
+        HIP_TRY(hipSetDevice(ctx->device));
+        int rc;
+        if ((rc = ctx->stl_in.ensure(nv * 24 + nt * 12 + 16))) return rc;
+        double *dv = ctx->stl_in.as<double>();
+        uint32_t *dt = (uint32_t *)(ctx->stl_in.as<char>() + align_up(nv * 24, 16));
+        if (nv) HIP_TRY(hipMemcpyAsync(dv, verts, nv * 24, hipMemcpyHostToDevice, ctx->stream));
+        if (nt) HIP_TRY(hipMemcpyAsync(dt, tris, nt * 12, hipMemcpyHostToDevice, ctx->stream));
+        return encode_stl_device(ctx, dv, dt, nt, out, out_bytes);
+    });
 }
 
 int mc3d_mesh_copy_interior(mc3d_mesh *m, double *interior) {
@@ -1380,18 +1620,20 @@ int mc3d_mesh_copy_interior(mc3d_mesh *m, double *interior) {
         return MC3D_E_INVALID_ARG;
     }
     if (!interior || !m->nv) return MC3D_OK;
-    if (m->on_host) {
-        parallel_copy(interior, m->hinterior.get(), m->nv * 24);
+    return guarded([&]() -> int {
+        if (m->on_host) {
+            parallel_copy(interior, m->hinterior(), m->nv * 24);
+            return MC3D_OK;
+        }
+        HIP_TRY(hipSetDevice(m->device));
+        HIP_TRY(hipMemcpy(interior, m->buf->interior->as<double>() + 3 * m->voff, m->nv * 24, hipMemcpyDeviceToHost));
         return MC3D_OK;
-    }
-    HIP_TRY(hipSetDevice(m->device));
-    HIP_TRY(hipMemcpy(interior, m->buf->interior->as<double>() + 3 * m->voff, m->nv * 24, hipMemcpyDeviceToHost));
-    return MC3D_OK;
+    });
 }
 
 int mc3d_mesh_device_ptrs(const mc3d_mesh *m, const double **verts, const uint32_t **tris) {
     if (!m) return MC3D_E_INVALID_ARG;
-    const bool have = m->buf && m->buf->verts && m->buf->tris;
+    const bool have = !m->on_host && m->buf && m->buf->verts && m->buf->tris;
     if (verts) *verts = have ? m->buf->verts->as<double>() + 3 * m->voff : nullptr;
     if (tris) *tris = have ? m->buf->tris->as<uint32_t>() + 3 * m->toff : nullptr;
     return MC3D_OK;
@@ -1409,6 +1651,119 @@ void mc3d_mesh_free(mc3d_mesh *m) {
     delete m;
 }
 
+int mc3d_diag_stream(mc3d_ctx *ctx, size_t bytes, int32_t write, double *ms) {
+    if (!ctx || bytes < 16) {
+        set_error("mc3d_diag_stream: null context or fewer than 16 bytes");
+        return MC3D_E_INVALID_ARG;
+    }
+    return guarded([&]() -> int {
+        HIP_TRY(hipSetDevice(ctx->device));
+        const uint64_t items = bytes / 16;
+        int rc;
+        // the stream buffer reuses the STL body scratch; the read form's
+        // per-block words go to the STL input scratch
+        if ((rc = ctx->stl.ensure(items * 16)) || (rc = ctx->stl_in.ensure(8192 * 4))) return rc;
+        // one untimed pass writes the buffer (the read pass reads defined data)
+        HIP_TRY(launch_stream(true, ctx->stl.p, items, nullptr, ctx->stream));
+        HIP_TRY(hipEventRecord(ctx->ev[EV_PLAN0], ctx->stream));
+        HIP_TRY(launch_stream(write != 0, ctx->stl.p, items, ctx->stl_in.as<uint32_t>(), ctx->stream));
+        HIP_TRY(hipEventRecord(ctx->ev[EV_PLAN1], ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (ms) *ms = elapsed(ctx->ev[EV_PLAN0], ctx->ev[EV_PLAN1]);
+        return MC3D_OK;
+    });
+}
+
+int mc3d_selftest_plan_fold(uint64_t seed, uint32_t programs, uint64_t *checked, uint64_t *mismatches) {
+    if (checked) *checked = 0;
+    if (mismatches) *mismatches = 0;
+    return guarded([&]() -> int {
+        uint64_t state = seed, nchecked = 0, nbad = 0;
+        auto next = [&]() {  // splitmix64
+            uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return z ^ (z >> 31);
+        };
+        auto uni = [&](double lo, double hi) { return lo + (hi - lo) * (double)(next() >> 11) * 0x1.0p-53; };
+        for (uint32_t prog = 0; prog < programs; prog++) {
+            // a random expression over up to 24 leaves: spheres and boxes
+            // (some in a transformed frame), constants, the five operators
+            Program P;
+            std::vector<AtomBox> abox;
+            const int leaves = 1 + (int)(next() % 24);
+            int pending = 0, made = 0;
+            while (made < leaves || pending > 1) {
+                const bool leaf = made < leaves && (pending < 2 || next() % 3 != 0);
+                if (leaf) {
+                    made++, pending++;
+                    const uint32_t r = (uint32_t)(next() % 16);
+                    if (r == 0 || r == 1) {
+                        P.code.push_back(Insn{r ? OP_TRUE : OP_FALSE, 0});
+                        continue;
+                    }
+                    const uint32_t frame = next() % 5 == 0 ? 1u : 0u;
+                    const double c[3] = {uni(-1, 1), uni(-1, 1), uni(-1, 1)}, e = uni(0.01, 0.6);
+                    AtomBox b;
+                    const uint32_t at = (uint32_t)P.f64.size();
+                    if (next() & 1) {  // sphere: centre, radius, threshold
+                        P.f64.insert(P.f64.end(), {c[0], c[1], c[2], e, e * e, 0, 0});
+                        P.code.push_back(Insn{OP_ATOM | (A_SPHERE << 8) | (frame << 16), at});
+                    } else {  // box
+                        P.f64.insert(P.f64.end(), {c[0] - e, c[1] - e, c[2] - e, c[0] + e, c[1] + e, c[2] + e});
+                        P.code.push_back(Insn{OP_ATOM | (A_RECT << 8) | (frame << 16), at});
+                    }
+                    for (int k = 0; k < 3; k++) b.lo[k] = c[k] - e * 1.01, b.hi[k] = c[k] + e * 1.01;
+                    P.atom_pc.push_back((uint32_t)P.code.size() - 1);
+                    P.atom_box.push_back(b);
+                } else if (next() % 6 == 0) {
+                    P.code.push_back(Insn{OP_NOT, 0});
+                } else {
+                    P.code.push_back(Insn{OP_OR + (uint32_t)(next() % 4), 0});
+                    pending--;
+                }
+            }
+            std::vector<AtomBox> boxes(P.code.size());
+            for (size_t i = 0; i < P.atom_pc.size(); i++) boxes[P.atom_pc[i]] = P.atom_box[i];
+            std::vector<PlanNode> nodes;
+            plan_nodes(P, boxes, nodes);
+            const uint32_t ncode = (uint32_t)P.code.size();
+            // groups of bricks near each other, as a wave's tile: a subtree is
+            // skipped only when every brick of the group misses its box
+            for (int group = 0; group < 4; group++) {
+                constexpr int NB = 4;
+                double lo[NB][3], hi[NB][3];
+                const double w = group < 2 ? uni(0.001, 0.1) : uni(0.1, 1.5);
+                const double c[3] = {uni(-1.6, 1.4), uni(-1.6, 1.4), uni(-1.6, 1.4)};
+                for (int k = 0; k < NB; k++)
+                    for (int a = 0; a < 3; a++) lo[k][a] = c[a] + uni(0, 2 * w), hi[k][a] = lo[k][a] + w;
+                auto all_miss = [&](const PlanNode &t) {
+                    for (int k = 0; k < NB; k++)
+                        if (!node_misses(t, lo[k], hi[k])) return false;
+                    return true;
+                };
+                for (int k = 0; k < NB; k++) {
+                    const uint32_t cap = (group + k) % 4 == 0 ? (uint32_t)(next() % 4) : 64u;
+                    uint32_t S1[PLAN_STACK], S2[PLAN_STACK];
+                    Insn o1[64], o2[64];
+                    uint32_t n1 = 0, n2 = 0;
+                    bool over1 = false, over2 = false;
+                    plan_fold(P.code.data(), boxes.data(), P.f64.data(), ncode, S1, lo[k], hi[k], o1, cap, n1, over1);
+                    plan_fold_nodes(nodes.data(), ncode, S2, lo[k], hi[k], o2, cap, n2, over2, all_miss);
+                    bool same = S1[0] == S2[0] && n1 == n2 && over1 == over2;
+                    for (uint32_t i = 0; same && i < std::min(n1, cap); i++)
+                        same = o1[i].x == o2[i].x && o1[i].y == o2[i].y;
+                    nchecked++;
+                    nbad += same ? 0 : 1;
+                }
+            }
+        }
+        if (checked) *checked = nchecked;
+        if (mismatches) *mismatches = nbad;
+        return MC3D_OK;
+    });
+}
+
 int mc3d_contains(mc3d_ctx *ctx, const mc3d_solid *s, const double *pts, size_t n, int32_t precision,
                   uint8_t *out) {
     if (!ctx || !s || (n && (!pts || !out))) {
@@ -1416,35 +1771,33 @@ int mc3d_contains(mc3d_ctx *ctx, const mc3d_solid *s, const double *pts, size_t 
         return MC3D_E_INVALID_ARG;
     }
     if (n == 0) return MC3D_OK;
-    HIP_TRY(hipSetDevice(ctx->device));
-    std::vector<Job> jobs(1);
-    Job &j = jobs[0];
-    j.s = s;
-    std::string msg;
-    if (!lower_solid(*s->ptr, j.P, msg)) {
-        set_error(msg);
-        return MC3D_E_UNSUPPORTED;
-    }
-    j.frames = host_frames(j.P);
-    j.g = Grid{nullptr, nullptr, nullptr, 1, 1, 1, 1};
-    int rc;
-    Upload U;
-    if ((rc = upload(ctx, jobs, 0, 0, U))) return rc;
-    if ((rc = ctx->pts.ensure(n * 24)) || (rc = ctx->out8.ensure(n))) return rc;
-    HIP_TRY(hipMemcpyAsync(ctx->pts.p, pts, n * 24, hipMemcpyHostToDevice, ctx->stream));
-    if (!j.P.host_atoms.empty()) {
-        double ms_cb = 0;
-        if ((rc = host_round(ctx, j, U, 0, ctx->pts.as<double>(), n, ms_cb))) return rc;
-    }
-    const DevProg &dp = U.hsegs[0].P;
-    if (precision == MC3D_PREC_FP32_GUARD)
-        k_contains<1><<<grid_for(n, 256), 256, 0, ctx->stream>>>(dp, ctx->pts.as<double>(), ctx->out8.as<uint8_t>(), n);
-    else
-        k_contains<0><<<grid_for(n, 256), 256, 0, ctx->stream>>>(dp, ctx->pts.as<double>(), ctx->out8.as<uint8_t>(), n);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(out, ctx->out8.p, n, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    return MC3D_OK;
+    return guarded([&]() -> int {
+        HIP_TRY(hipSetDevice(ctx->device));
+        std::vector<Job> jobs(1);
+        Job &j = jobs[0];
+        j.s = s;
+        std::string msg;
+        if (!lower_solid(*s->ptr, j.P, msg)) {
+            set_error(msg);
+            return MC3D_E_UNSUPPORTED;
+        }
+        j.frames = host_frames(j.P);
+        j.g = Grid{nullptr, nullptr, nullptr, 1, 1, 1, 1};
+        int rc;
+        Upload U;
+        if ((rc = upload(ctx, jobs, 0, 0, Bufs{}, U))) return rc;
+        if ((rc = ctx->pts.ensure(n * 24)) || (rc = ctx->out8.ensure(n))) return rc;
+        HIP_TRY(hipMemcpyAsync(ctx->pts.p, pts, n * 24, hipMemcpyHostToDevice, ctx->stream));
+        if (!j.P.host_atoms.empty()) {
+            double ms_cb = 0;
+            if ((rc = host_round(ctx, j, U, 0, ctx->pts.as<double>(), n, ms_cb))) return rc;
+        }
+        HIP_TRY(launch_contains(precision == MC3D_PREC_FP32_GUARD ? 1 : 0, U.hsegs[0].P, ctx->pts.as<double>(),
+                                ctx->out8.as<uint8_t>(), n, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(out, ctx->out8.p, n, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return MC3D_OK;
+    });
 }
 
 }  // extern "C"

@@ -64,6 +64,24 @@ struct Frame {
     uint32_t nops;
 };
 
+// Node i of a program's expression tree (the subtree whose postfix code
+// ends at i) as the brick plan walks it, everything a step needs in one
+// 128-byte record: the instruction; `link`, which takes a leaf to the
+// outermost node whose subtree starts at it and an operator to its child
+// whose subtree starts where its own does; `cfalse`, the subtree's value
+// when all its atoms are false; the node's box (an atom's own, an
+// operator's union of its atoms', empty for constants); and for
+// untransformed boxes and spheres the params of the "certainly true" test
+// (rect: min, max; sphere: centre, threshold).
+struct PlanNode {
+    uint32_t x, y;  // the instruction
+    uint32_t link, cfalse;
+    double lo[3], hi[3];
+    double prm[6];
+    double pad[2];
+};
+static_assert(sizeof(PlanNode) == 128, "a plan node is one 128-byte fetch");
+
 // Conservative world-space AABB of where an atom can be true (used only to
 // cull atoms that are certainly false in a brick; never changes a result).
 struct AtomBox {

@@ -1,0 +1,82 @@
+// pipeline.h -- the data structures the host and the device stages share:
+// the control block, a solid's grid, bricks and segment, and the block
+// sizes the host needs to size its buffers (kernels.h holds the kernels).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "eval.h"
+
+namespace mc3d {
+
+constexpr uint32_t BRICK_TRUE = 0xffffffffu;  // info.y of a brick where the solid is true
+constexpr uint32_t BRICK_FULL = 0xffffffffu;  // info.x of a brick that uses the unfolded program
+
+// Per-call control block in device memory (zeroed by the upload).
+struct Ctl {
+    uint32_t inside;       // panic("solid is true outside of bounds") seen
+    uint32_t overflow;     // an output index exceeded the buffers' capacity
+    uint32_t nact_bricks;  // bricks whose folded program is not constant
+    uint32_t pad;
+    unsigned long long totals[4];  // scan totals: vertices, triangles, listed words
+    unsigned long long gstats[4];  // guard statistics (GStats order)
+    unsigned long long flops[4];   // statistics runs: float64, float32 ops of corners; the same of bisect
+    unsigned long long code_total; // instructions over all active bricks' folded programs
+    uint32_t code_max;             // longest folded program
+    uint32_t full_bricks;          // bricks whose folded program overflowed its slot
+};
+
+struct Grid {
+    const double *__restrict__ xs;
+    const double *__restrict__ ys;
+    const double *__restrict__ zs;
+    int nx, ny, nz, W;  // W = words per row = ceil(nx / 64)
+};
+
+// Per-brick programs: info[b] = {offset into code (or BRICK_FULL), length};
+// length 0: the solid is false everywhere in the brick; BRICK_TRUE: true.
+struct Bricks {
+    const uint2 *__restrict__ info;  // null: no culling, use the full program
+    const Insn *__restrict__ code;
+    int BY, BZ, nby, nbz;
+};
+
+// One solid of a call.  B.info / B.code are the call's arrays, indexed by the
+// global brick number; word and brick numbers are local to the segment
+// unless named global.  Kernels also take the first segment by value (a
+// call of one solid reads it from the kernel arguments, no load).
+struct Seg {
+    DevProg P;
+    Grid g;
+    Bricks B;
+    uint64_t word0, nwords;    // global occupancy words [word0, word0 + nwords); word0 % 64 == 0
+    uint64_t brick0, nbricks;  // global bricks; brick0 % 128 == 0
+    int32_t iters;
+    int32_t stage;  // the plan may stage this segment's program in LDS
+};
+
+// Per-solid results, copied back with the control block.
+struct SegOut {
+    unsigned long long vbase, tbase;  // first vertex / triangle of the solid in the call's outputs
+    uint32_t inside;                  // panic("solid is true outside of bounds") for this solid
+    uint32_t pad[3];
+};
+
+constexpr int SEG_WORD_SHIFT = 6;   // segment table entry per 64 words
+constexpr int SEG_BRICK_SHIFT = 7;  // and per 128 bricks
+
+// The brick plan: threads per block and the per-thread fold stack.
+constexpr int PLAN_BLOCK = 128;
+constexpr int PLAN_STACK = 33;
+constexpr size_t PLAN_STACK_BYTES = PLAN_BLOCK * PLAN_STACK * sizeof(uint32_t);
+
+// classify and the scan of its counts
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+constexpr int CLASSIFY_BLOCK = 1024;  // words per classify block and per block sum
+constexpr int SUMS_BLOCK = 1024;      // threads of the one block scanning the block sums
+static_assert(SCAN_TILE % CLASSIFY_BLOCK == 0, "a scan tile starts at a classify block");
+
+}  // namespace mc3d

@@ -10,8 +10,9 @@ HBM traffic from the FETCH_SIZE / WRITE_SIZE counters.
     python -m model3d_amd.profiling pmc FETCH_DIR WRITE_DIR --copy-bytes N [...]
         JSON: for every kernel the average FETCH_SIZE and WRITE_SIZE per
         dispatch (KiB, as the counters report them), and the same counters of
-        the plain copy kernel of N bytes the profiled program ran first, from
-        which the bytes-per-KiB calibration factors follow.
+        the library's two plain streams of N bytes the profiled program ran
+        first (one read-only, one write-only), from which the bytes-per-KiB
+        calibration factors follow.
 """
 from __future__ import annotations
 
@@ -103,10 +104,12 @@ def counter_values(path: str, counter: str) -> dict:
     return per
 
 
-def pmc_summary(fetch_dir: str, write_dir: str, copy_bytes: int, copy_kernel: str = "copyBuffer") -> dict:
+def pmc_summary(fetch_dir: str, write_dir: str, copy_bytes: int) -> dict:
     """Per-kernel mean FETCH_SIZE / WRITE_SIZE per dispatch, and the
-    calibration from the largest dispatch of the copy kernel (the program's
-    known-size copy; its smaller dispatches are the library's uploads)."""
+    calibration from the program's two streams of `copy_bytes` (the largest
+    dispatch of each: k_stream_read reads them once -- its FETCH_SIZE gives
+    the bytes per fetched KiB -- and k_stream_write writes them once -- its
+    WRITE_SIZE gives the bytes per written KiB)."""
     fetch = counter_values(fetch_dir, "FETCH_SIZE")
     write = counter_values(write_dir, "WRITE_SIZE")
     kernels = {}
@@ -115,18 +118,19 @@ def pmc_summary(fetch_dir: str, write_dir: str, copy_bytes: int, copy_kernel: st
         kernels[k] = {"fetch_kib": sum(f.values()) / len(f) if f else None,
                       "write_kib": sum(w.values()) / len(w) if w else None,
                       "dispatches": max(len(f), len(w))}
-    copies = [k for k in kernels if copy_kernel in k]
-    if not copies:
-        raise SystemExit(f"no {copy_kernel} dispatch found for the calibration")
-    cf = max(fetch.get(copies[0], {0: 0.0}).values())
-    cw = max(write.get(copies[0], {0: 0.0}).values())
+    reads = [k for k in fetch if k.startswith("k_stream_read")]
+    writes = [k for k in write if k.startswith("k_stream_write")]
+    if not reads or not writes:
+        raise SystemExit("no k_stream_read / k_stream_write dispatch found for the calibration")
+    cf = max(fetch[reads[0]].values())
+    cw = max(write[writes[0]].values())
     kib = copy_bytes / 1024.0
     return {
         "counters": "FETCH_SIZE, WRITE_SIZE (KiB per dispatch, averaged over dispatches)",
-        "copy": {"kernel": copies[0], "bytes_read": copy_bytes, "bytes_written": copy_bytes,
-                 "fetch_kib": cf, "write_kib": cw,
-                 "fetch_factor": kib / cf if cf else None,
-                 "write_factor": kib / cw if cw else None},
+        "calibration": {"read_kernel": reads[0], "write_kernel": writes[0], "bytes": copy_bytes,
+                        "fetch_kib_of_read": cf, "write_kib_of_write": cw,
+                        "fetch_factor": kib / cf if cf else None,
+                        "write_factor": kib / cw if cw else None},
         "kernels": kernels,
     }
 
@@ -134,7 +138,7 @@ def pmc_summary(fetch_dir: str, write_dir: str, copy_bytes: int, copy_kernel: st
 def traffic_bytes(summary: dict, kernel: str):
     """Calibrated HBM bytes per dispatch of `kernel` (fetch + write), or None."""
     k = summary.get("kernels", {}).get(kernel)
-    c = summary.get("copy", {})
+    c = summary.get("calibration", {})
     if not k or k.get("fetch_kib") is None or k.get("write_kib") is None:
         return None
     if not c.get("fetch_factor") or not c.get("write_factor"):

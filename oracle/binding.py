@@ -184,11 +184,27 @@ def oracle_bounds(solid):
     return tuple(mn), tuple(mx)
 
 
-def oracle_contains(solid, pts) -> np.ndarray:
+def oracle_contains(solid, pts, threads=1) -> np.ndarray:
+    """Solid.Contains at every point; with threads > 1 the points are split
+    into contiguous chunks answered concurrently (each call releases the GIL,
+    and the restated Contains only reads the solid)."""
     p = np.ascontiguousarray(np.asarray(pts, dtype=np.float64).reshape(-1, 3))
     out = np.zeros(len(p), dtype=np.uint8)
     o = OracleSolid(solid)
-    o.backend.L.or_contains_batch(o.h, p.ctypes.data_as(_dp), len(p), out.ctypes.data)
+    L = o.backend.L
+    if threads <= 1 or len(p) < 2 * threads:
+        L.or_contains_batch(o.h, p.ctypes.data_as(_dp), len(p), out.ctypes.data)
+        return out.astype(bool)
+    from concurrent.futures import ThreadPoolExecutor
+
+    bounds = np.linspace(0, len(p), threads + 1).astype(np.int64)
+
+    def chunk(i):
+        a, b = int(bounds[i]), int(bounds[i + 1])
+        L.or_contains_batch(o.h, p[a:].ctypes.data_as(_dp), b - a, out[a:].ctypes.data)
+
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(chunk, range(threads)))
     return out.astype(bool)
 
 

@@ -1,6 +1,7 @@
 """The C-ABI library loads, exports every declared symbol, and its host-side
 logic (Solid.Min/Max, case table) matches the oracle.  No GPU needed: none of
 these calls launches a kernel."""
+import json
 import os
 import re
 
@@ -68,3 +69,40 @@ def test_constructor_errors_are_reported():
     L = _lib.lib()
     assert not L.mc3d_joined(None, 0)
     assert "at least one" in _lib.last_error()
+
+
+def test_case_table_matches_mc_go_fixture():
+    """The library's edge-id table against the table derived from mc.go's
+    text (tests/golden/make_case_table_from_mc_go.py): each corner pair of the
+    reference's mcTriangle is the cube edge the library names by axis and
+    the two other corner bits."""
+    with open(os.path.join(ROOT, "tests", "golden", "mc_case_table_mc_go.json")) as f:
+        ref = [c["triangles"] for c in json.load(f)["cases"]]
+
+    def edge_id(a, b):
+        d, lo = a ^ b, a & b
+        assert d in (1, 2, 4)
+        axis = d.bit_length() - 1
+        others = [bit for bit in range(3) if bit != axis]
+        return 4 * axis + ((lo >> others[0]) & 1) + 2 * ((lo >> others[1]) & 1)
+
+    lib_t = m3.case_table()
+    for k in range(256):
+        assert int(lib_t[k, 0]) == len(ref[k]), k
+        want = [edge_id(t[2 * e], t[2 * e + 1]) for t in ref[k] for e in range(3)]
+        assert [int(x) for x in lib_t[k, 1 : 1 + 3 * len(ref[k])]] == want, k
+    # corner 0 alone: edges x, y, z at the origin corner, in that order
+    assert [int(x) for x in lib_t[1, :4]] == [1, 0, 4, 8]
+
+
+def test_plan_fold_skip_equals_serial_fold():
+    """The brick plan's subtree-skipping fold against the instruction-by-
+    instruction fold on 20,000 random programs x 16 random bricks (host
+    code, no GPU): identical folded code, root and overflow flag."""
+    import ctypes
+
+    L = _lib.lib()
+    checked, bad = ctypes.c_uint64(), ctypes.c_uint64()
+    assert L.mc3d_selftest_plan_fold(20240607, 20000, ctypes.byref(checked), ctypes.byref(bad)) == 0
+    assert checked.value == 20000 * 16
+    assert bad.value == 0

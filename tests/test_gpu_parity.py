@@ -279,3 +279,119 @@ def test_large_workload_properties():
     key.sort()
     assert np.all(key[1:] != key[:-1])  # every directed edge once
     assert np.all(np.isin(rev, key))  # and its reverse is there
+
+
+def test_go_shim_call_sequence():
+    """The calls the Go drop-in makes for MarchingCubesSearch, in order:
+    lower the solid, search with default options (results fetched to pinned
+    host memory), read the counts and the host arrays in place, free."""
+    import ctypes
+
+    from model3d_amd import _lib
+    from model3d_amd.solids import LibBackend
+
+    L = _lib.lib()
+    ctx = _lib.context(0)
+    s = workloads.csg64_solid(2)
+    print("go shim sequence: csg64 d=0.01", flush=True)
+    want, _ = oracle_search(s, 0.01, 8, threads=8)
+    b = LibBackend()
+    h = s._build(b)
+    out = ctypes.c_void_p()
+    try:
+        _lib.check(L.mc3d_marching_cubes_search(ctx, h, 0.01, 8, None, ctypes.byref(out)), "search")
+        nv, nt = ctypes.c_size_t(), ctypes.c_size_t()
+        _lib.check(L.mc3d_mesh_counts(out, ctypes.byref(nv), ctypes.byref(nt)), "counts")
+        pv, pt = ctypes.c_void_p(), ctypes.c_void_p()
+        _lib.check(L.mc3d_mesh_host_ptrs(out, ctypes.byref(pv), ctypes.byref(pt)), "host ptrs")
+        v = np.ctypeslib.as_array(ctypes.cast(pv, ctypes.POINTER(ctypes.c_double)), shape=(nv.value, 3))
+        f = np.ctypeslib.as_array(ctypes.cast(pt, ctypes.POINTER(ctypes.c_uint32)), shape=(nt.value, 3))
+        assert same_triangles(v[f.astype(np.int64)], want)
+        st = _lib.mc3d_stats()
+        L.mc3d_mesh_stats(out, ctypes.byref(st))
+        assert st.ms_d2h >= 0 and st.triangles == len(want)
+        # a mesh left on the device has no host arrays
+        dev = ctypes.c_void_p()
+        o = _lib.mc3d_options()
+        o.precision, o.cull, o.device_only = GUARD, 1, 1
+        _lib.check(L.mc3d_marching_cubes_search(ctx, h, 0.05, 2, ctypes.byref(o), ctypes.byref(dev)), "search")
+        assert L.mc3d_mesh_host_ptrs(dev, ctypes.byref(pv), ctypes.byref(pt)) == _lib.MC3D_E_INVALID_ARG
+        L.mc3d_mesh_free(dev)
+    finally:
+        if out.value:
+            L.mc3d_mesh_free(out)
+        b.release(h)
+
+
+def _spacer(lo, hi, delta):
+    """newSquareSpacer's axis (mc.go:594-608): plain float adds, as Go's."""
+    out, x = [], lo - delta
+    while x <= hi + delta:
+        out.append(x)
+        x += delta
+    return np.array(out)
+
+
+def test_collider_workload_at_bench_size():
+    """The collider workload the benchmark times: the CSG tree meshed at
+    delta 0.02 (about 50k triangles) as a ColliderSolid, 256 samples along
+    its longest axis, iters 8.  Contains is compared with the oracle's brute
+    force over every triangle on 24k points -- grid corners on and off the
+    surface, first bisection midpoints, searched vertices and points a hair
+    away from them -- and the full grid is checked through properties: float32
+    guard and float64 agree bit for bit, the surface is closed and
+    consistently oriented, and its volume matches the source mesh's to the
+    grid scale (ColliderContains, collisions.go:119-134)."""
+    c = workloads.COLLIDER_CONFIG
+    solid, delta, ntri = workloads.collider_solid(c["seed"], c["n_prims"], c["mesh_delta"])
+    print(f"collider bench size: {ntri} triangles, delta {delta}", flush=True)
+    assert ntri > 40000
+    a = m3.MarchingCubesSearch(solid, delta, c["iters"], precision=GUARD)
+    b = m3.MarchingCubesSearch(solid, delta, c["iters"], precision=FP64)
+    assert max(a.stats["grid_nx"], a.stats["grid_ny"], a.stats["grid_nz"]) == 256
+    assert a.stats["triangles"] > 50000
+    assert np.array_equal(a.vertices.view(np.uint64), b.vertices.view(np.uint64))
+    assert np.array_equal(a.faces, b.faces)
+
+    rng = np.random.default_rng(3)
+    mn, mx = solid.Min(), solid.Max()
+    ax = [_spacer(mn[i], mx[i], delta) for i in range(3)]
+    assert [len(x) for x in ax] == [a.stats["grid_nx"], a.stats["grid_ny"], a.stats["grid_nz"]]
+    # grid corners: random ones, and the two ends of the edges the vertices lie on
+    idx = np.stack([rng.integers(0, len(ax[i]), 6000) for i in range(3)], axis=1)
+    corners = np.stack([ax[i][idx[:, i]] for i in range(3)], axis=1)
+    v = b.vertices[rng.choice(len(b.vertices), 4000, replace=False)]
+    ends, mids = [], []
+    for p in v:
+        k = [int(np.searchsorted(ax[i], p[i])) for i in range(3)]
+        on = [i for i in range(3) if k[i] >= len(ax[i]) or ax[i][k[i]] != p[i]]
+        assert len(on) == 1  # a vertex lies on one grid edge: two coordinates are grid values
+        axis = on[0]
+        lo, hi = ax[axis][k[axis] - 1], ax[axis][k[axis]]
+        q0, q1, qm = p.copy(), p.copy(), p.copy()
+        q0[axis], q1[axis], qm[axis] = lo, hi, (lo + hi) / 2  # the edge's corners and first midpoint
+        ends += [q0, q1]
+        mids.append(qm)
+    near = v + rng.normal(scale=1e-9, size=v.shape)
+    pts = np.concatenate([corners, np.array(ends), np.array(mids), v, near])
+    assert len(pts) >= 20000
+    want = oracle_contains(solid, pts, threads=16)
+    assert 0 < want.sum() < len(pts)
+    for precision in (FP64, GUARD):
+        got = m3.Contains(solid, pts, precision=precision)
+        assert np.array_equal(got, want), f"{np.count_nonzero(got != want)} mismatches (precision {precision})"
+    # every edge end pair straddles the surface, as the mesh says
+    e = want[len(corners) : len(corners) + 2 * len(v)].reshape(-1, 2)
+    assert np.all(e[:, 0] != e[:, 1])
+
+    f = b.faces.astype(np.int64)
+    nv = len(b.vertices)
+    ed = np.stack([f, np.roll(f, -1, axis=1)], axis=2).reshape(-1, 2)
+    key = ed[:, 0] * nv + ed[:, 1]
+    rev = ed[:, 1] * nv + ed[:, 0]
+    key.sort()
+    assert np.all(key[1:] != key[:-1])  # every directed edge once
+    assert np.all(np.isin(rev, key))  # and its reverse is there
+    src, got_v = volume(solid.collider.triangles), volume(b.triangles())
+    print(f"collider volume {got_v} against the source mesh's {src}", flush=True)
+    assert abs(got_v - src) < 0.02 * abs(src), (got_v, src)

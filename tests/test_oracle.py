@@ -92,6 +92,40 @@ def test_case_table_fixture():
         assert [[int(c) for c in t[k, 1 + 6 * i : 7 + 6 * i]] for i in range(n)] == fix[k]["triangles"]
 
 
+def _mc_go_table():
+    """The table derived from mc.go's text by tests/golden/make_case_table_from_mc_go.py."""
+    with open(os.path.join(GOLDEN, "mc_case_table_mc_go.json")) as f:
+        fix = json.load(f)
+    assert fix["base_cases"] == 23 and fix["rotations"] == 24
+    return [c["triangles"] for c in fix["cases"]]
+
+
+def test_case_table_matches_mc_go_derivation():
+    """The oracle's table equals the one derived from mc.go's text (base
+    cases parsed as data, rotation rule restated separately), triangle by
+    triangle and corner by corner."""
+    t, _ = oracle_case_table()
+    ref = _mc_go_table()
+    for k in range(256):
+        n = int(t[k, 0])
+        assert [[int(c) for c in t[k, 1 + 6 * i : 7 + 6 * i]] for i in range(n)] == ref[k], k
+
+
+def test_case_table_hand_derived_keys():
+    """Keys whose rotation is the identity (the first in sorted order) read
+    baseTriangleTable directly (mc.go:462-586): corner 0 alone, corners 0 and
+    1, every corner but 0; the empty and full cubes have no triangles."""
+    ref = _mc_go_table()
+    assert ref[1] == [[0, 1, 0, 2, 0, 4]]
+    assert ref[3] == [[0, 4, 1, 5, 0, 2], [1, 5, 1, 3, 0, 2]]
+    assert ref[254] == [[0, 2, 0, 1, 0, 4]]
+    assert ref[0] == [] and ref[255] == []
+    t, _ = oracle_case_table()
+    for k in (1, 3, 254):
+        n = int(t[k, 0])
+        assert [[int(c) for c in t[k, 1 + 6 * i : 7 + 6 * i]] for i in range(n)] == ref[k]
+
+
 # ---------------------------------------------------------------------------
 # Spacer (mc.go:594-608)
 # ---------------------------------------------------------------------------
