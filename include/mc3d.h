@@ -44,6 +44,8 @@ enum {
     MC3D_E_CALLBACK = 8,
     /* an exception inside the library (reported, never thrown across the ABI) */
     MC3D_E_INTERNAL = 9,
+    /* the caller's output arrays are too small (the counts needed are returned) */
+    MC3D_E_CAPACITY = 10,
 };
 
 typedef struct mc3d_ctx mc3d_ctx;
@@ -175,6 +177,26 @@ typedef struct {
  * MarchingCubes, mc.go:14-37).  opts may be NULL (FP32 guard + culling). */
 int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
                                const mc3d_options *opts, mc3d_mesh **out);
+/* The same search with no allocation at all: no counterpart in the
+ * reference (MarchingCubesSearch, mc.go:45-49, returns a new *Mesh); it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * buffer the pipeline needs comes from a fixed 64 MiB device array per GPU,
+ * held by the library until the process ends; grids that need more (about
+ * 100M corners with culling at the default brick) are MC3D_E_UNSUPPORTED.
+ * The caller passes device arrays for the outputs: verts (3 * vcap doubles)
+ * and tris (3 * tcap uint32, vertex indices).  *n_vertices / *n_triangles
+ * receive the mesh's counts; a mesh larger than the arrays is
+ * MC3D_E_CAPACITY (nothing is written beyond them; zero capacities ask for
+ * the counts).  The result is bit-identical to mc3d_marching_cubes_search
+ * with the same options.  Host-evaluated and collider solids are
+ * MC3D_E_UNSUPPORTED (their tables need buffers of their own).  Calls on one
+ * GPU share its array: the library runs them one at a time (calls from
+ * other threads wait), and each call keeps the array's contents only until
+ * the next one. */
+int mc3d_marching_cubes_search_noalloc(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
+                                       const mc3d_options *opts, double *verts, size_t vcap, uint32_t *tris,
+                                       size_t tcap, size_t *n_vertices, size_t *n_triangles);
 /* MarchingCubesInterior(s, delta, iters), model3d/mc.go:55-60: the mesh of
  * MarchingCubesSearch plus, for every vertex, a nearby point known to be
  * inside the solid -- the bisection bracket's inside end (mc.go:252-255; with

@@ -23,6 +23,7 @@ MC3D_E_OOM = 6
 MC3D_E_UNSUPPORTED = 7
 MC3D_E_CALLBACK = 8
 MC3D_E_INTERNAL = 9
+MC3D_E_CAPACITY = 10
 
 MC3D_XF_TRANSLATE = 1
 MC3D_XF_SCALE = 2
@@ -145,6 +146,11 @@ SIGNATURES = {
     "mc3d_mesh_copy": (ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p]),
     "mc3d_mesh_copy_interior": (ctypes.c_int, [_P, ctypes.c_void_p]),
     "mc3d_mesh_host_ptrs": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]),
+    "mc3d_marching_cubes_search_noalloc": (
+        ctypes.c_int,
+        [_P, _P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(mc3d_options), ctypes.c_void_p, ctypes.c_size_t,
+         ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)],
+    ),
     "mc3d_stl_bytes": (ctypes.c_size_t, [ctypes.c_size_t]),
     "mc3d_marching_cubes_filter": (
         ctypes.c_int,

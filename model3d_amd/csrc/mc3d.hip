@@ -38,7 +38,45 @@
 
 using namespace mc3d;
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry point (mc3d_marching_cubes_search_noalloc):
+// every device buffer that call needs beyond its inputs and outputs -- the
+// program blob, occupancy, counts, offsets, the word and brick lists, the
+// brick programs and the vertex keys -- is carved from this array.  Each GPU
+// holds its own copy until the process ends.  In host code `scratch` is only
+// a stand-in for the symbol: its device address on the current device comes
+// from hipGetSymbolAddress (scratch_address), and only that address is used.
+// ---------------------------------------------------------------------------
+constexpr size_t SCRATCH_BYTES = 64u << 20;
+static __device__ __align__(256) char scratch[SCRATCH_BYTES];
+
 namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES];              // device address of `scratch` per device (null: not looked up)
+std::mutex g_noalloc_mu[MAX_DEVICES];          // the allocation-free calls of one device run one at a time
+
+// The device address of `scratch` on `dev`, which must be the current
+// device; looked up once per device.
+int scratch_address(int dev, void **out) {
+    if (dev < 0 || dev >= MAX_DEVICES) {
+        set_error("device index beyond the scratch table");
+        return MC3D_E_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return MC3D_E_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MC3D_OK;
+}
 
 #define HIP_TRY(expr)                                                                        \
     do {                                                                                     \
@@ -213,6 +251,7 @@ struct mc3d_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t copy = nullptr;  // device-to-host copies of results (overlapping the bisection)
+    void *scratch = nullptr;     // this device's `scratch` (allocation-free entry point)
     std::vector<std::unique_ptr<ColliderCache>> colliders;  // most recent last
     HostBuf staging, ctl_host, ctl_early, segstage, outstage;
     DevBuf blob, table, etable, bits, counts, offs, tiles, actw, actb, binfo, bcode, vkeys, vseg, pts, out8;
@@ -1195,6 +1234,107 @@ mc3d_options default_options(const mc3d_options *opts) {
     return o;
 }
 
+// ---------------------------------------------------------------------------
+// The allocation-free search: every buffer from `scratch`, outputs in the
+// caller's device arrays.
+// ---------------------------------------------------------------------------
+// Carves the call's buffers out of [base, base + cap): the blob first, then
+// the pipeline's arrays, the vertex keys last with whatever room is left
+// (up to vcap keys).  Returns the key capacity, or 0 if the rest does not fit.
+uint64_t carve_scratch(char *base, size_t cap, const Call &c, size_t blob_bytes, uint64_t vcap, Bufs &b,
+                       size_t &need) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *p = base + off;
+        off = align_up(off + bytes, 256);
+        return p;
+    };
+    (void)take(blob_bytes);  // the program blob sits at the start
+    b.bits = (uint64_t *)take(c.nwords * 8);
+    b.counts = (uint32_t *)take(c.nscan * sizeof(uint32_t));
+    b.offs = (uint2 *)take(c.nscan * sizeof(uint2));
+    b.actw = (uint32_t *)take(c.nwords * sizeof(uint32_t));
+    b.tiles = (uint4 *)take(classify_blocks(c) * sizeof(uint4));
+    if (c.cull) {
+        b.binfo = (uint2 *)take(c.nbricks * sizeof(uint2));
+        b.actb = (uint32_t *)take(c.nbricks * sizeof(uint32_t));
+        b.bcode = (Insn *)take(c.nbricks * c.cap * sizeof(Insn) + 16);
+    }
+    need = off + 8 * std::min<uint64_t>(vcap, 1);
+    if (off + 256 > cap) return 0;
+    b.vkeys = (uint64_t *)take(0);
+    const uint64_t room = (cap - off) / 8;
+    return std::min<uint64_t>(vcap, room);
+}
+
+int search_noalloc(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters, const mc3d_options *opts,
+                   double *verts, size_t vcap, uint32_t *tris, size_t tcap, size_t *nv_out, size_t *nt_out) {
+    const mc3d_options o = default_options(opts);
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard<std::mutex> lk(g_noalloc_mu[ctx->device]);
+    std::vector<Job> jobs(1);
+    jobs[0].s = s, jobs[0].delta = delta, jobs[0].iters = iters;
+    int rc = prepare_job(jobs[0], o);
+    if (rc) return rc;
+    if (jobs[0].host || !jobs[0].P.colliders.empty()) {
+        set_error("mc3d_marching_cubes_search_noalloc: host-evaluated and collider solids need buffers of their "
+                  "own; use mc3d_marching_cubes_search");
+        return MC3D_E_UNSUPPORTED;
+    }
+    Call c;
+    if ((rc = plan_call(jobs, o, c))) return rc;
+    Upload U;
+    U.cols.assign(1, {});
+    layout_upload(jobs, c.nwords, c.nbricks, U);
+    if (U.back_bytes > ctx->ctl_host.cap) {
+        set_error("control block larger than its pinned copy");
+        return MC3D_E_INTERNAL;
+    }
+    Bufs b;
+    size_t need = 0;
+    char *base = (char *)ctx->scratch;
+    const uint64_t kcap = carve_scratch(base, SCRATCH_BYTES, c, U.total, vcap, b, need);
+    if (!kcap && vcap) {
+        set_error("mc3d_marching_cubes_search_noalloc: the grid needs " + std::to_string(need) +
+                  " bytes of scratch, more than the " + std::to_string(SCRATCH_BYTES) + " the library holds");
+        return MC3D_E_UNSUPPORTED;
+    }
+    b.table = ctx->table.as<uint8_t>();
+    b.etable = ctx->etable.as<uint8_t>();
+    // The blob is built in host memory and copied from there (pageable:
+    // the copy has taken the bytes when hipMemcpyAsync returns).
+    std::vector<char> h(U.total);
+    fill_upload(jobs, c.nwords, c.nbricks, b, h.data(), base, U);
+    HIP_TRY(hipMemcpyAsync(base, h.data(), U.total, hipMemcpyHostToDevice, ctx->stream));
+    const Pipe p = make_pipe(c, U);
+    if ((rc = enqueue_front(p, b, ctx->stream, nullptr))) return rc;
+    // emit and bisect keep within the capacities (extra vertices and
+    // triangles are dropped and flagged), so a mesh too large is an error,
+    // never a write past the caller's arrays.
+    if ((rc = enqueue_output(p, b, U, verts, tris, nullptr, kcap, tcap, ctx->stream, nullptr))) return rc;
+    Ctl *hc = (Ctl *)ctx->ctl_host.p;
+    HIP_TRY(hipMemcpyAsync(hc, U.ctl, U.back_bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (hc->inside) {
+        set_error("solid is true outside of bounds");
+        return MC3D_E_INSIDE_BOUNDARY;
+    }
+    const uint64_t nv = hc->totals[0], nt = hc->totals[1];
+    if (nv_out) *nv_out = (size_t)nv;
+    if (nt_out) *nt_out = (size_t)nt;
+    if (nv > vcap || nt > tcap) {
+        set_error("mc3d_marching_cubes_search_noalloc: the mesh has " + std::to_string(nv) + " vertices and " +
+                  std::to_string(nt) + " triangles, more than the output arrays hold");
+        return MC3D_E_CAPACITY;
+    }
+    if (nv > kcap) {
+        set_error("mc3d_marching_cubes_search_noalloc: " + std::to_string(nv) +
+                  " vertex keys do not fit the scratch left after the grid");
+        return MC3D_E_UNSUPPORTED;
+    }
+    return MC3D_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1212,6 +1352,8 @@ int mc3d_ctx_create(int device, mc3d_ctx **out) {
         HIP_TRY(hipSetDevice(device));
         std::unique_ptr<mc3d_ctx> c(new mc3d_ctx);
         c->device = device;
+        int rc = scratch_address(device, &c->scratch);
+        if (rc) return rc;
         hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking);
         if (e != hipSuccess) {
@@ -1391,6 +1533,21 @@ int mc3d_marching_cubes_search(mc3d_ctx *ctx, const mc3d_solid *s, double delta,
         return MC3D_E_INVALID_ARG;
     }
     return guarded([&]() { return search_one(ctx, s, delta, iters, opts, out, false, {}); });
+}
+
+int mc3d_marching_cubes_search_noalloc(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
+                                       const mc3d_options *opts, double *verts, size_t vcap, uint32_t *tris,
+                                       size_t tcap, size_t *n_vertices, size_t *n_triangles) {
+    if (!ctx || !s || (vcap && !verts) || (tcap && !tris)) {
+        set_error("mc3d_marching_cubes_search_noalloc: null argument");
+        return MC3D_E_INVALID_ARG;
+    }
+    if (opts && opts->count_mismatch) {
+        set_error("mc3d_marching_cubes_search_noalloc: statistics runs are not supported");
+        return MC3D_E_UNSUPPORTED;
+    }
+    return guarded([&]() { return search_noalloc(ctx, s, delta, iters, opts, verts, vcap, tris, tcap, n_vertices,
+                                                 n_triangles); });
 }
 
 int mc3d_marching_cubes_interior(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters,
