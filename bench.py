@@ -110,10 +110,14 @@ def merge_stats(per_mesh):
     return st
 
 
-def run_gpu(work, run, steps, warmup, count_mismatch=False, device_only=True, each=False):
+def run_gpu(work, run, steps, warmup, count_mismatch=False, device_only=True, each=False, no_graph=False,
+            copy_out=False):
     """`steps` timed steps; a step is one library call on all the solids (the
     batch entry point for several), or with `each` one call per solid.
-    Returns (wall seconds, merged stats per step)."""
+    Results stay on the device unless device_only is False (then they arrive
+    in the library's pinned memory, and with copy_out are also copied into
+    caller arrays allocated once).  no_graph enqueues the stages one by one,
+    which times each stage.  Returns (wall seconds, merged stats per step)."""
     from model3d_amd import _lib
     from model3d_amd.solids import LibBackend
 
@@ -129,6 +133,20 @@ def run_gpu(work, run, steps, warmup, count_mismatch=False, device_only=True, ea
     opts.brick, opts.rows_per_walk, opts.waves_per_brick = run["brick"], run["rows_per_walk"], run["waves_per_brick"]
     opts.device_only = 1 if device_only else 0
     opts.count_mismatch = 1 if count_mismatch else 0
+    opts.no_graph = 1 if no_graph else 0
+    bufs = {}
+
+    def fetch(m):
+        """The mesh into caller arrays (allocated on first use, then reused)."""
+        import numpy as np
+
+        nv, nt = ctypes.c_size_t(), ctypes.c_size_t()
+        L.mc3d_mesh_counts(m, ctypes.byref(nv), ctypes.byref(nt))
+        if bufs.get("n", (-1, -1)) < (nv.value, nt.value):
+            bufs["n"] = (nv.value, nt.value)
+            bufs["v"] = np.empty((nv.value, 3))
+            bufs["f"] = np.empty((nt.value, 3), dtype=np.uint32)
+        _lib.check(L.mc3d_mesh_copy(m, bufs["v"].ctypes.data, bufs["f"].ctypes.data), "mc3d_mesh_copy")
     harr = (ctypes.c_void_p * n)(*hs)
     darr = (ctypes.c_double * n)(*work["deltas"])
     iarr = (ctypes.c_int32 * n)(*work["iters"])
@@ -147,6 +165,8 @@ def run_gpu(work, run, steps, warmup, count_mismatch=False, device_only=True, ea
                 one = ctypes.c_void_p()
                 _lib.check(L.mc3d_marching_cubes_search(ctx, hs[i], darr[i], iarr[i], ctypes.byref(opts),
                                                         ctypes.byref(one)), "mc3d_marching_cubes_search")
+                if copy_out:
+                    fetch(one)
                 if record:
                     per.append(mesh_stats(one))
                 L.mc3d_mesh_free(one)
@@ -251,8 +271,9 @@ def algorithmic(stage, st):
               per word
     scan    : reads the counts (4 B), writes the offsets (8 B) per word (the
               block sums and the 4 B entries of listed words are small)
-    emit    : reads the grid words it lists and their 8 B offsets, writes an 8 B
-              key per vertex and 12 B per triangle
+    emit    : reads each listed word (8 B) and its 8 B offsets, and the 4 B list
+              entry, writes an 8 B key per vertex and 12 B per triangle (the
+              neighbour words it also reads are listed words' own or cached)
     plan    : writes an 8 B brick record per brick, a 4 B list entry per
               active brick and the 8 B occupancy words of the constant bricks
               (their share of the words taken as their share of the bricks)
@@ -270,7 +291,7 @@ def algorithmic(stage, st):
         + 8 * words * (1 - st["bricks_active"] / max(1, st["bricks_total"])),
         "ms_classify": n / 8 + 4 * words,
         "ms_scan": 12 * words,
-        "ms_emit": n / 8 + 8 * words + 8 * v + 12 * t,
+        "ms_emit": 20 * st["words_listed"] + 8 * v + 12 * t,
     }[stage]
     return ("bytes", b)
 
@@ -281,7 +302,7 @@ def roofline_of(stage, ms, st):
         _, f64, f32 = kind
         # float64-equivalent rate: float32 runs at twice the float64 vector rate
         achieved = (f64 + f32 / 2) / (ms * 1e-3) / 1e12
-        r = {"bound": "mfma", "pipe": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        r = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
              "frac": achieved / FP64_PEAK_TFLOPS, "flops64_per_launch": f64, "flops32_per_launch": f32}
     else:
         achieved = kind[1] / (ms * 1e-3) / 1e9
@@ -421,6 +442,75 @@ def next_rows(work, run, steps=5):
     return out
 
 
+def stage_times(work, run, steps=4):
+    """Per-stage times (HIP events between the stages, graphs off) and the
+    algorithmic counts of a statistics run, merged: (stages_ms, counts)."""
+    _, st = run_gpu(work, run, steps, 1, no_graph=True)
+    stages = {k: mean(st, k) for k in STAGE_KERNEL}
+    stages["ms_device"] = mean(st, "ms_device")
+    _, sst = run_gpu(work, run, 1, 0, count_mismatch=True)
+    return stages, dict(sst[0])
+
+
+def kernel_rooflines(stages, counts, run):
+    kernels = {}
+    for stage in STAGE_KERNEL:
+        if stages[stage] > 0:
+            kernels[STAGE_KERNEL[stage]] = roofline_of(stage, stages[stage], counts)
+    if not run["cull"]:
+        kernels.pop("k_corners_act", None)
+        kernels["k_corners"] = dict(roofline_of("ms_corners", stages["ms_corners"], counts), kernel="k_corners")
+    return kernels
+
+
+def config_entry(name, steps=5, warmup=2):
+    """A compact line for one of the other configurations: ms per step in the
+    workload's configuration (results on the device, graph launches), in the
+    other precision, corner evals/s and tris/s, and the dominant kernel's
+    roofline from a stage-timed run."""
+    work = workload(name)
+    run = dict(DEFAULT_RUN[name])
+    wall, st = run_gpu(work, run, steps, warmup)
+    ms = 1e3 * wall / steps
+    other = "fp32_guard" if run["precision"] == "fp64" else "fp64"
+    wall2, _ = run_gpu(work, dict(run, precision=other), steps, 1)
+    stages, counts = stage_times(work, run, steps=3)
+    kernels = kernel_rooflines(stages, counts, run)
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
+    s0 = st[0]
+    e = {
+        "workload": work["cfg"]["name"],
+        "solids": len(work["solids"]),
+        "corners": s0["corners"],
+        "triangles": s0["triangles"],
+        "ms_per_step": ms,
+        f"ms_per_step_{other}": 1e3 * wall2 / steps,
+        "corner_evals_per_s": s0["corners"] / (ms / 1e3),
+        "tris_per_s": s0["triangles"] / (ms / 1e3),
+        "ms_device": mean(st, "ms_device"),
+        "dominant": {x: kernels[dom][x] for x in ("kernel", "avg_ms", "bound", "achieved", "unit", "frac")},
+        **run,
+    }
+    return e
+
+
+def sphere_batched(copies=64, steps=5):
+    """The north-star comparison on config 1: `copies` spheres (C1) meshed by
+    one batch call in float32 + guard, against the plain float64 GPU path
+    (every atom at every corner, one call per sphere), corner evals/s."""
+    c = dict(workload("sphere"))
+    work = dict(solids=c["solids"] * copies, deltas=c["deltas"] * copies, iters=c["iters"] * copies, cfg=c["cfg"])
+    run = dict(DEFAULT_RUN["sphere"], precision="fp32_guard", cull=True, brick=8)
+    wall, st = run_gpu(work, run, steps, 1)
+    corners = st[0]["corners"]  # summed over the copies
+    wall64, _ = run_gpu(work, PLAIN_RUN, max(1, steps // 2), 1, each=True)
+    batched = corners / (wall / steps)
+    plain = corners / (wall64 / max(1, steps // 2))
+    return {"copies": copies, "corners_per_step": corners, "ms_per_step_batched_fp32_guard": 1e3 * wall / steps,
+            "ms_per_step_plain_fp64": 1e3 * wall64 / max(1, steps // 2), "corner_evals_per_s_batched": batched,
+            "corner_evals_per_s_plain_fp64": plain, "ratio": batched / plain}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -429,9 +519,11 @@ def main():
     ap.add_argument("--workload", default="csg64", choices=sorted(DEFAULT_RUN))
     ap.add_argument("--autotune", action="store_true", help="search tile shapes and precision first")
     ap.add_argument("--profile-run", action="store_true",
-                    help="only the warm-up and timed steps (for rocprofv3), plus the copy calibration kernel")
+                    help="only the warm-up and timed steps (for rocprofv3; stages enqueued one by one), plus the "
+                         "calibration streams")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
+    ap.add_argument("--no-configs", action="store_true", help="skip the lines of the other configurations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -463,11 +555,13 @@ def main():
 
     if args.profile_run:
         streams = copy_calibration()
-        wall, st = run_gpu(work, run, args.steps, args.warmup)
+        wall, st = run_gpu(work, run, args.steps, args.warmup, no_graph=True)
         print(json.dumps({"workload": args.workload, "config": run, "ms_per_step": 1e3 * wall / args.steps,
                           "stages_ms": {k: mean(st, k) for k in STAGE_KERNEL}, "stream_GBps": streams}), flush=True)
         return
 
+    # The timed steps: one call per step, results left in HBM, the chain of
+    # stages launched as one captured HIP graph.
     wall, st = run_gpu(work, run, args.steps, args.warmup)
     s0 = st[0]
     corners, tris, verts = s0["corners"], s0["triangles"], s0["vertices"]
@@ -496,25 +590,27 @@ def main():
             **run,
         },
         "tris_per_s": tris / (wall / args.steps),
+        "ms_device": mean(st, "ms_device"),
     }
+    # The same steps in both precisions (bit-identical results).
+    other = "fp32_guard" if run["precision"] == "fp64" else "fp64"
+    wall_o, _ = run_gpu(work, dict(run, precision=other), args.steps, args.warmup)
+    result["precisions_ms_per_step"] = {run["precision"]: ms, other: 1e3 * wall_o / args.steps}
     if tuned:
         result["autotune"] = {k: {"brick": v["brick"], "rows_per_walk": v["rows_per_walk"],
                                   "waves_per_brick": v["waves_per_brick"], "best_ms": v["ms"]} for k, v in tuned.items()}
-    stages = {k: mean(st, k) for k in STAGE_KERNEL}
-    stages["ms_device"] = mean(st, "ms_device")
+
+    # Stage times from a run with the stages enqueued one by one (HIP events
+    # between them), and the algorithmic counts of a statistics run.
+    stages, counts = stage_times(work, run)
     result["stages_ms"] = stages
     result["bricks"] = {"total": s0["bricks_total"], "active": s0["bricks_active"], "full": s0["bricks_full"],
                         "code_mean": s0["brick_code_total"] / max(1, s0["bricks_active"]),
                         "code_max": s0["brick_code_max"], "program": s0["program_length"]}
 
-    # One untimed statistics run of the same configuration: the algorithmic
-    # operation counts of the evaluation stages.
-    _, sst = run_gpu(work, run, 1, 0, count_mismatch=True)
-    counts = dict(sst[0])
-
     # Guard statistics (float32 decisions checked against float64).
     if run["precision"] == "fp32_guard":
-        g = sst[0]
+        g = counts
     else:
         _, gst = run_gpu(work, dict(run, precision="fp32_guard"), 1, 0, count_mismatch=True)
         g = gst[0]
@@ -527,13 +623,7 @@ def main():
     }
 
     # Roofline of every kernel; the dominant one is the headline.
-    kernels = {}
-    for stage in STAGE_KERNEL:
-        if stages[stage] > 0:
-            kernels[STAGE_KERNEL[stage]] = roofline_of(stage, stages[stage], counts)
-    if not run["cull"]:
-        kernels.pop("k_corners_act", None)
-        kernels["k_corners"] = dict(roofline_of("ms_corners", stages["ms_corners"], counts), kernel="k_corners")
+    kernels = kernel_rooflines(stages, counts, run)
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
     traffic, source = load_traffic(args.workload, run, dom.rstrip("*").rstrip("_"))
     head = dict(kernels[dom])
@@ -543,10 +633,14 @@ def main():
     result["kernels_roofline"] = {k: {x: r[x] for x in ("bound", "achieved", "unit", "frac", "avg_ms")}
                                   for k, r in kernels.items()}
 
-    # End to end through the C-ABI with the copy of the meshes to the host.
+    # End to end through the C-ABI with the meshes fetched to the host: in
+    # the library's pinned memory (read in place, as the Go drop-in does), and
+    # copied on into caller arrays allocated once.
     n_e2e = max(1, args.steps // 2)
     wall_e2e, est = run_gpu(work, run, n_e2e, 1, device_only=False)
-    result["e2e_with_d2h"] = {"ms_per_step": 1e3 * wall_e2e / n_e2e, "ms_d2h": mean(est, "ms_d2h")}
+    wall_cp, _ = run_gpu(work, run, n_e2e, 1, device_only=False, copy_out=True)
+    result["e2e_with_d2h"] = {"ms_per_step": 1e3 * wall_e2e / n_e2e, "ms_d2h": mean(est, "ms_d2h"),
+                              "ms_per_step_into_caller_arrays": 1e3 * wall_cp / n_e2e}
 
     if not args.no_fp64:
         # the plain float64 GPU path: every atom at every corner, one call per solid
@@ -557,7 +651,6 @@ def main():
             "corner_evals_per_s": corners / (ms64 / 1e3),
             "tris_per_s": tris / (ms64 / 1e3),
             "ms_per_step": ms64,
-            "stages_ms": {k: mean(st64, k) for k in STAGE_KERNEL},
         }
         result["speedup_vs_fp64_plain"] = ms64 / ms
         if nsolids > 1:
@@ -568,6 +661,11 @@ def main():
 
     if nsolids == 1 and args.workload in ("csg64", "sphere"):
         result["next_rows"] = next_rows(work, run)
+
+    if not args.no_configs:
+        result["configs"] = {name: config_entry(name) for name in ("sphere", "collider", "batch", "large")
+                             if name != args.workload}
+        result["sphere_batched"] = sphere_batched()
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(work)

@@ -171,6 +171,7 @@ typedef struct {
     double ms_prepare;         /* host: bounds, grid axes and lowering of the call's solids */
     double ms_enqueue;         /* host: upload and kernel launches, up to the wait for the results */
     double ms_wait;            /* host: waiting for the device at the end of the call */
+    uint64_t words_listed;     /* occupancy words that emit a vertex or a triangle (the call's) */
 } mc3d_stats;
 
 /* MarchingCubesSearch(s, delta, iters), model3d/mc.go:45-49 (iters == 0 is

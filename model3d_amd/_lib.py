@@ -92,6 +92,7 @@ class mc3d_stats(ctypes.Structure):
         ("ms_prepare", ctypes.c_double),
         ("ms_enqueue", ctypes.c_double),
         ("ms_wait", ctypes.c_double),
+        ("words_listed", ctypes.c_uint64),
     ]
 
     def as_dict(self):

@@ -269,7 +269,7 @@ struct mc3d_ctx {
         hipGraphExec_t exec = nullptr;
     };
     std::vector<Graph> graphs;
-    bool graphs_off = true;  // a capture failed on this device: enqueue stage by stage
+    bool graphs_off = false;  // a capture failed on this device: enqueue stage by stage
 };
 
 // One solid's mesh: a slice [voff, voff + nv) / [toff, toff + nt) of its
@@ -821,9 +821,13 @@ Pipe make_pipe(const Call &c, const Upload &U) {
     p.rows_per_walk = c.rows_per_walk;
     p.wpb = c.wpb;
     p.plan_lds = c.plan_lds;
-    p.plan_gsz = 64;
-    p.plan_tile = false;
-    p.emit_div = 1024;
+    // Launch shapes measured on MI355X: a single solid's plan waves take
+    // compact tiles, 16 bricks for programs that stage in LDS and 64 for the
+    // large ones walked through scalar loads (csg64 39.6 -> 37.4 us, the
+    // 512-primitive tree 183 -> 160 us); emit gets a block per 512 words.
+    p.plan_tile = true;
+    p.plan_gsz = c.plan_lds > PLAN_STACK_BYTES ? 16 : 64;
+    p.emit_div = 512;
     return p;
 }
 
@@ -1169,6 +1173,7 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
     st.corner_flops32 = hc->flops[1];
     st.bisect_flops64 = hc->flops[2];
     st.bisect_flops32 = hc->flops[3];
+    st.words_listed = hc->totals[2];
     st.brick_code_total = hc->code_total;
     st.brick_code_max = hc->code_max;
     st.bricks_full = hc->full_bricks;
