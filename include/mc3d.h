@@ -105,6 +105,15 @@ mc3d_solid *mc3d_force_bounds(mc3d_solid *child, const double min[3], const doub
  * model3d/collisions.go:138-142: ray-parity Contains over the triangles
  * (9 doubles each: three xyz vertices). */
 mc3d_solid *mc3d_collider_solid(const double *triangles, size_t n_triangles);
+/* NewColliderSolidInset(MeshToCollider(mesh), inset), model3d/solid.go:260-270:
+ * Contains is ColliderContains(c, p, inset) (collisions.go:119-134) -- inside
+ * by ray parity and at least `inset` from the surface (SphereCollision,
+ * collisions.go:292-302, primitives.go:251-279); a negative inset also takes
+ * the points within -inset outside. */
+mc3d_solid *mc3d_collider_solid_inset(const double *triangles, size_t n_triangles, double inset);
+/* NewColliderSolidHollow(MeshToCollider(mesh), r), model3d/solid.go:272-279:
+ * Contains is SphereCollision(p, r), the shell within r of the surface. */
+mc3d_solid *mc3d_collider_solid_hollow(const double *triangles, size_t n_triangles, double r);
 
 /* A Solid the library cannot lower (a Go closure, a user type): Contains is
  * asked of the host in batches.  fn must write out[i] = Contains(p_i) for

@@ -121,6 +121,8 @@ SIGNATURES = {
     "mc3d_transform": (_P, [_P, ctypes.POINTER(mc3d_xform), ctypes.c_size_t]),
     "mc3d_force_bounds": (_P, [_P, _D3, _D3]),
     "mc3d_collider_solid": (_P, [_dp, ctypes.c_size_t]),
+    "mc3d_collider_solid_inset": (_P, [_dp, ctypes.c_size_t, ctypes.c_double]),
+    "mc3d_collider_solid_hollow": (_P, [_dp, ctypes.c_size_t, ctypes.c_double]),
     "mc3d_host_solid": (_P, [_D3, _D3, CONTAINS_FN, ctypes.c_void_p]),
     "mc3d_solid_retain": (_P, [_P]),
     "mc3d_solid_release": (None, [_P]),

@@ -65,38 +65,93 @@ __device__ __forceinline__ DevProg as_global(DevProg P) {
 MC3D_HD inline bool in_lo(double c, double m) { return m == -INFINITY || c >= m; }
 MC3D_HD inline bool in_hi(double c, double m) { return m == INFINITY || c <= m; }
 
-// ColliderSolid.Contains with inset 0 (solid.go:292-300): InBounds, then the
-// parity of ray hits along the fixed direction (collisions.go:119-134).  Only
-// the triangles binned in the point's projected cell are tested; each test is
+// Mesh collider SphereCollision (JoinedCollider, collisions.go:292-302): does
+// any triangle of the point's cell in the sphere grid (binned for this
+// radius) touch the sphere -- Triangle.SphereCollision (primitives.go:251-279)
+// in the reference's float64 operation order: a vertex closer than r
+// (Coord3D.Dist), an edge entering the sphere (segmentEntersSphere), or the
+// ray along the triangle's normal hitting it at |scale| < r (rayCollision,
+// its point-independent part precomputed per triangle).  A point outside
+// the grid is farther than r from every triangle.
+__device__ __forceinline__ bool sphere_collision(const DevCollider &C, double x, double y, double z, double r,
+                                                 uint32_t *candidates) {
+    const double fx = (x - C.so[0]) * C.sinv, fy = (y - C.so[1]) * C.sinv, fz = (z - C.so[2]) * C.sinv;
+    if (!(fx >= 0 && fx < C.sn[0] && fy >= 0 && fy < C.sn[1] && fz >= 0 && fz < C.sn[2])) return false;
+    const uint32_t cell = ((uint32_t)fz * (uint32_t)C.sn[1] + (uint32_t)fy) * (uint32_t)C.sn[0] + (uint32_t)fx;
+    const uint32_t *__restrict__ cs = global_ptr(C.scell_start);
+    const uint32_t *__restrict__ ct = global_ptr(C.scell_tris);
+    const double *__restrict__ rec = global_ptr(C.srec);
+    const uint32_t e = cs[cell + 1];
+    if (candidates) *candidates += e - cs[cell];
+    for (uint32_t i = cs[cell]; i < e; i++) {
+        const double *t = rec + (size_t)SPH_REC * ct[i];
+        for (int k = 0; k < 3; k++) {  // p.Dist(c) < r
+            const double d1 = t[3 * k] - x, d2 = t[3 * k + 1] - y, d3 = t[3 * k + 2] - z;
+            if (sqrt(d1 * d1 + d2 * d2 + d3 * d3) < r) return true;
+        }
+        for (int k = 0; k < 3; k++) {  // segmentEntersSphere(t[k], t[k+1], c, r)
+            const double *p1 = t + 3 * k, *p2 = t + 3 * ((k + 1) % 3);
+            const double vx = p2[0] + -p1[0], vy = p2[1] + -p1[1], vz = p2[2] + -p1[2];
+            const double frac =
+                ((x * vx + y * vy + z * vz) - (p1[0] * vx + p1[1] * vy + p1[2] * vz)) / (vx * vx + vy * vy + vz * vz);
+            const double cx = p1[0] + vx * frac, cy = p1[1] + vy * frac, cz = p1[2] + vz * frac;
+            const double d1 = cx - x, d2 = cy - y, d3 = cz - z;
+            if (frac >= 0 && frac <= 1 && sqrt(d1 * d1 + d2 * d2 + d3 * d3) < r) return true;
+        }
+        if (t[12] != 0) {  // rayCollision from c along the normal t[9..11]
+            const double inv = t[13];
+            const double ox = x + -t[0], oy = y + -t[1], oz = z + -t[2];
+            const double bary1 = inv * (ox * t[14] + oy * t[15] + oz * t[16]);
+            if (bary1 < 0 || bary1 > 1) continue;
+            const double c2x = oy * t[19] - oz * t[18], c2y = oz * t[17] - ox * t[19], c2z = ox * t[18] - oy * t[17];
+            const double bary2 = inv * (t[9] * c2x + t[10] * c2y + t[11] * c2z);
+            if (bary2 < 0 || bary1 + bary2 > 1) continue;
+            const double scale = inv * (t[20] * c2x + t[21] * c2y + t[22] * c2z);
+            if (fabs(scale) < r) return true;
+        }
+    }
+    return false;
+}
+
+// ColliderSolid.Contains (solid.go:292-300): InBounds, then with a radius
+// the hollow shell (SphereCollision), otherwise ColliderContains(c, p, inset)
+// (collisions.go:119-134): the parity of ray hits along the fixed direction,
+// and with an inset the sphere test.  Only the triangles binned in the
+// point's projected cell are tested for the ray; each test is
 // Triangle.rayCollision (primitives.go:207-249) from its point-independent
 // constants, counted by RayCollisions when `scale < 0` is false.
+// Params a: collider, min(3), max(3), inset, radius.
 __device__ __forceinline__ bool collider_contains(const DevCollider &C, const double *__restrict__ a, double x,
                                                   double y, double z, uint32_t *candidates = nullptr) {
     if (!(in_lo(x, a[1]) && in_lo(y, a[2]) && in_lo(z, a[3]) && in_hi(x, a[4]) && in_hi(y, a[5]) && in_hi(z, a[6])))
         return false;
+    const double inset = a[7], radius = a[8];
+    if (radius != 0) return sphere_collision(C, x, y, z, radius, candidates);
+    uint32_t hits = 0;
     const double fu = (x * C.u[0] + y * C.u[1] + z * C.u[2] - C.u0) * C.inv_cell;
     const double fv = (x * C.v[0] + y * C.v[1] + z * C.v[2] - C.v0) * C.inv_cell;
-    if (!(fu >= 0 && fu < C.nu && fv >= 0 && fv < C.nv)) return false;  // no triangle can be hit
-    const uint32_t cell = (uint32_t)fv * (uint32_t)C.nu + (uint32_t)fu;
-    const uint32_t *__restrict__ cs = global_ptr(C.cell_start);
-    const uint32_t *__restrict__ ct = global_ptr(C.cell_tris);
-    const double *__restrict__ rec = global_ptr(C.rec);
-    const uint32_t e = cs[cell + 1];
-    if (candidates) *candidates += e - cs[cell];
-    const double Dx = 0.5224892708603626, Dy = 0.10494477243214506, Dz = 0.43558938446126527;
-    uint32_t hits = 0;
-    for (uint32_t i = cs[cell]; i < e; i++) {
-        const double *r = rec + (size_t)RAY_REC * ct[i];
-        const double ox = x - r[0], oy = y - r[1], oz = z - r[2];
-        const double bary1 = r[6] * (ox * r[3] + oy * r[4] + oz * r[5]);
-        if (bary1 < 0 || bary1 > 1) continue;
-        const double c2x = oy * r[9] - oz * r[8], c2y = oz * r[7] - ox * r[9], c2z = ox * r[8] - oy * r[7];
-        const double bary2 = r[6] * (Dx * c2x + Dy * c2y + Dz * c2z);
-        if (bary2 < 0 || bary1 + bary2 > 1) continue;
-        const double scale = r[6] * (r[10] * c2x + r[11] * c2y + r[12] * c2z);
-        hits += !(scale < 0);
+    if (fu >= 0 && fu < C.nu && fv >= 0 && fv < C.nv) {  // else no triangle can be hit
+        const uint32_t cell = (uint32_t)fv * (uint32_t)C.nu + (uint32_t)fu;
+        const uint32_t *__restrict__ cs = global_ptr(C.cell_start);
+        const uint32_t *__restrict__ ct = global_ptr(C.cell_tris);
+        const double *__restrict__ rec = global_ptr(C.rec);
+        const uint32_t e = cs[cell + 1];
+        if (candidates) *candidates += e - cs[cell];
+        const double Dx = 0.5224892708603626, Dy = 0.10494477243214506, Dz = 0.43558938446126527;
+        for (uint32_t i = cs[cell]; i < e; i++) {
+            const double *r = rec + (size_t)RAY_REC * ct[i];
+            const double ox = x - r[0], oy = y - r[1], oz = z - r[2];
+            const double bary1 = r[6] * (ox * r[3] + oy * r[4] + oz * r[5]);
+            if (bary1 < 0 || bary1 > 1) continue;
+            const double c2x = oy * r[9] - oz * r[8], c2y = oz * r[7] - ox * r[9], c2z = ox * r[8] - oy * r[7];
+            const double bary2 = r[6] * (Dx * c2x + Dy * c2y + Dz * c2z);
+            if (bary2 < 0 || bary1 + bary2 > 1) continue;
+            const double scale = r[6] * (r[10] * c2x + r[11] * c2y + r[12] * c2z);
+            hits += !(scale < 0);
+        }
     }
-    return hits & 1u;
+    if (!(hits & 1u)) return inset < 0 && sphere_collision(C, x, y, z, -inset, candidates);
+    return inset <= 0 || !sphere_collision(C, x, y, z, inset, candidates);
 }
 
 // Guard statistics and algorithmic flop counts (statistics runs only), kept
@@ -302,7 +357,7 @@ __device__ __forceinline__ uint32_t atom_nparams(uint32_t kind) {
     case A_CONE: return 8;
     case A_CONESLICE: return 9;
     case A_TORUS: return 11;
-    case A_COLLIDER: return 7;
+    case A_COLLIDER: return 9;
     default: return 1;
     }
 }

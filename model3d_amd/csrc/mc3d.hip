@@ -241,7 +241,8 @@ struct MeshBuffers {
 // collider's id; the shared_ptr keeps the id from being reused).
 struct ColliderCache {
     std::shared_ptr<const ColliderData> keep;
-    DevBuf rec, start, tris;
+    double R = 0;  // radius of its sphere queries (0: none)
+    DevBuf rec, start, tris, srec, sstart, stris;
     DevCollider dev{};
 };
 
@@ -385,11 +386,12 @@ int prepare_job(Job &j, const mc3d_options &o) {
     return MC3D_OK;
 }
 
-// The device copy of a collider's ray grid (uploaded once per context).
-int collider_on_device(mc3d_ctx *ctx, const std::shared_ptr<const ColliderData> &cd, DevCollider &out,
+// The device copy of a collider's ray grid and, for sphere queries of
+// radius R, its sphere grid (uploaded once per context).
+int collider_on_device(mc3d_ctx *ctx, const std::shared_ptr<const ColliderData> &cd, double R, DevCollider &out,
                        size_t keep_at_least) {
     for (size_t i = 0; i < ctx->colliders.size(); i++) {
-        if (ctx->colliders[i]->keep->id == cd->id) {
+        if (ctx->colliders[i]->keep->id == cd->id && ctx->colliders[i]->R == R) {
             out = ctx->colliders[i]->dev;
             std::unique_ptr<ColliderCache> hit = std::move(ctx->colliders[i]);  // now the most recent
             ctx->colliders.erase(ctx->colliders.begin() + i);
@@ -400,6 +402,7 @@ int collider_on_device(mc3d_ctx *ctx, const std::shared_ptr<const ColliderData> 
     const RayGrid &G = cd->ray_grid();
     std::unique_ptr<ColliderCache> c(new ColliderCache);
     c->keep = cd;
+    c->R = R;
     int rc;
     if ((rc = c->rec.ensure(G.rec.size() * sizeof(double) + 8)) ||
         (rc = c->start.ensure(G.cell_start.size() * sizeof(uint32_t) + 8)) ||
@@ -416,6 +419,28 @@ int collider_on_device(mc3d_ctx *ctx, const std::shared_ptr<const ColliderData> 
     memcpy(d.u, G.u, sizeof d.u);
     memcpy(d.v, G.v, sizeof d.v);
     d.u0 = G.u0, d.v0 = G.v0, d.inv_cell = G.inv_cell, d.nu = G.nu, d.nv = G.nv;
+    d.srec = nullptr, d.scell_start = d.scell_tris = nullptr;
+    d.so[0] = d.so[1] = d.so[2] = 0, d.sinv = 1, d.sn[0] = d.sn[1] = d.sn[2] = 0, d.pad = 0;
+    if (R > 0) {
+        const SphereGrid &S = cd->sphere_grid(R);
+        if ((rc = c->srec.ensure(S.rec.size() * sizeof(double) + 8)) ||
+            (rc = c->sstart.ensure(S.cell_start.size() * sizeof(uint32_t) + 8)) ||
+            (rc = c->stris.ensure(S.cell_tris.size() * sizeof(uint32_t) + 8)))
+            return rc;
+        if (!S.rec.empty())
+            HIP_TRY(hipMemcpy(c->srec.p, S.rec.data(), S.rec.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->sstart.p, S.cell_start.data(), S.cell_start.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice));
+        if (!S.cell_tris.empty())
+            HIP_TRY(hipMemcpy(c->stris.p, S.cell_tris.data(), S.cell_tris.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice));
+        d.srec = c->srec.as<double>();
+        d.scell_start = c->sstart.as<uint32_t>();
+        d.scell_tris = c->stris.as<uint32_t>();
+        memcpy(d.so, S.o, sizeof d.so);
+        d.sinv = S.inv_cell;
+        memcpy(d.sn, S.n, sizeof d.sn);
+    }
     out = d;
     ctx->colliders.push_back(std::move(c));
     // Evict the oldest entries, never one the current call uses (those are
@@ -629,7 +654,7 @@ int upload_colliders(mc3d_ctx *ctx, const std::vector<Job> &jobs, Upload &U) {
     for (size_t s = 0; s < n; s++) {
         U.cols[s].resize(jobs[s].P.colliders.size());
         for (size_t i = 0; i < U.cols[s].size(); i++) {
-            int rc = collider_on_device(ctx, jobs[s].P.colliders[i], U.cols[s][i], ncol);
+            int rc = collider_on_device(ctx, jobs[s].P.colliders[i], jobs[s].P.collider_R[i], U.cols[s][i], ncol);
             if (rc) return rc;
         }
     }

@@ -103,6 +103,22 @@ struct RayGrid {
     int nu, nv;
 };
 
+// SphereCollision of a mesh collider (JoinedCollider, collisions.go:292-302,
+// over Triangle.SphereCollision, primitives.go:251-279) for one radius R:
+// the triangles binned in a uniform 3D grid by their boxes grown by R (far
+// beyond rounding), so that a query tests only the triangles of its cell.
+// Per triangle its vertices and the point-independent values of the ray
+// test along its own normal.
+constexpr int SPH_REC = 24;  // t0 t1 t2 (9), normal (3), hit (1: 0 never), invDet (1), cross1 (3), v1 (3), v2 (3), pad
+struct SphereGrid {
+    double R = 0;
+    std::vector<double> rec;           // SPH_REC doubles per triangle
+    std::vector<uint32_t> cell_start;  // n[0] * n[1] * n[2] + 1
+    std::vector<uint32_t> cell_tris;   // triangle indices
+    double o[3] = {0, 0, 0}, inv_cell = 1;
+    int n[3] = {1, 1, 1};
+};
+
 struct ColliderData {
     std::vector<double> tris;   // 9 doubles per triangle
     double mn[3], mx[3];
@@ -110,9 +126,12 @@ struct ColliderData {
     mutable std::once_flag grid_once;
     mutable RayGrid grid;
     const RayGrid &ray_grid() const;  // built on first use
+    mutable std::mutex sphere_mu;
+    mutable std::vector<std::unique_ptr<SphereGrid>> spheres;
+    const SphereGrid &sphere_grid(double R) const;  // built on first use per radius
 };
 
-// Device view of a RayGrid.
+// Device view of a RayGrid and (insets, hollow shells) a SphereGrid.
 struct DevCollider {
     const double *rec;
     const uint32_t *cell_start;
@@ -120,6 +139,11 @@ struct DevCollider {
     double u[3], v[3];
     double u0, v0, inv_cell;
     int nu, nv;
+    const double *srec;  // null: no sphere queries
+    const uint32_t *scell_start;
+    const uint32_t *scell_tris;
+    double so[3], sinv;
+    int sn[3], pad;
 };
 
 struct HostAtom {
@@ -136,6 +160,7 @@ struct Program {
     std::vector<uint32_t> atom_pc; // pc of each atom instruction
     std::vector<AtomBox> atom_box; // per atom, world space
     std::vector<std::shared_ptr<const ColliderData>> colliders;
+    std::vector<double> collider_R;  // per collider: the radius of its sphere queries (0: none)
     std::vector<HostAtom> host_atoms;
     int max_stack = 0;
 };

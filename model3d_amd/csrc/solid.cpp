@@ -549,6 +549,119 @@ const RayGrid &ColliderData::ray_grid() const {
 }
 
 // ---------------------------------------------------------------------------
+// Sphere grid of a mesh collider
+// ---------------------------------------------------------------------------
+const SphereGrid &ColliderData::sphere_grid(double R) const {
+    std::lock_guard<std::mutex> lk(sphere_mu);
+    for (const auto &g : spheres)
+        if (g->R == R) return *g;
+    std::unique_ptr<SphereGrid> G(new SphereGrid);
+    G->R = R;
+    const size_t nt = tris.size() / 9;
+    G->rec.resize(nt * SPH_REC, 0.0);
+    struct B3 {
+        double lo[3], hi[3];
+    };
+    std::vector<B3> boxes(nt);
+    double glo[3] = {INFINITY, INFINITY, INFINITY}, ghi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double ext_sum = 0;
+    for (size_t i = 0; i < nt; i++) {
+        const double *t = &tris[9 * i];
+        const V3 t0 = of(t), t1 = of(t + 3), t2 = of(t + 6);
+        double *r = &G->rec[SPH_REC * i];
+        memcpy(r, t, 9 * sizeof(double));
+        // Triangle.rayCollision along Triangle.Normal() (primitives.go:207-230,
+        // 263-269): the parallel test, v1, v2, cross1, det do not depend on
+        // the point.
+        const V3 nrm = normalize(cross(sub(t1, t0), sub(t2, t0)));
+        const double d = dot(nrm, normalize(nrm));
+        const V3 v1 = sub(t1, t0), v2 = sub(t2, t0);
+        const V3 c1 = cross(nrm, v2);
+        const double det = dot(c1, v1);
+        const bool hit = !(d < 1e-8 && d > -1e-8) && det != 0;
+        const double vals[15] = {nrm.x, nrm.y, nrm.z, hit ? 1.0 : 0.0, hit ? 1 / det : 0.0, c1.x, c1.y, c1.z,
+                                 v1.x,  v1.y,  v1.z,  v2.x,           v2.y,               v2.z, 0.0};
+        memcpy(r + 9, vals, sizeof vals);
+        // the box grown by R, and by far more than the rounding of a test
+        B3 b;
+        double mag = 0;
+        for (int a = 0; a < 3; a++) {
+            const double lo = std::min(t[a], std::min(t[3 + a], t[6 + a]));
+            const double hi = std::max(t[a], std::max(t[3 + a], t[6 + a]));
+            mag = std::max(mag, std::max(fabs(lo), fabs(hi)));
+            b.lo[a] = lo, b.hi[a] = hi;
+            ext_sum += hi - lo;
+        }
+        const double w = R * (1 + 1e-6) + 1e-9 * (mag + R) + 1e-300;
+        for (int a = 0; a < 3; a++) {
+            b.lo[a] -= w, b.hi[a] += w;
+            glo[a] = std::min(glo[a], b.lo[a]), ghi[a] = std::max(ghi[a], b.hi[a]);
+        }
+        boxes[i] = b;
+    }
+    if (nt) {
+        // Cubic cells about the size of a grown triangle box, at most 2^21
+        // cells (a point outside the grid is farther than R from every
+        // triangle).
+        const double mean_ext = ext_sum / (3.0 * (double)nt);
+        double cell = std::max(mean_ext + 2 * R, 1e-300);
+        const double span = std::max(ghi[0] - glo[0], std::max(ghi[1] - glo[1], ghi[2] - glo[2]));
+        if (std::isfinite(span) && std::isfinite(cell)) {
+            for (;;) {
+                double cells = 1;
+                for (int a = 0; a < 3; a++) cells *= std::floor((ghi[a] - glo[a]) / cell) + 1;
+                if (cells <= (double)(1u << 21)) break;
+                cell *= 1.25;
+            }
+            cell = std::max(cell, span / 1e6);
+            G->inv_cell = 1 / cell;
+            for (int a = 0; a < 3; a++) {
+                G->o[a] = glo[a];
+                G->n[a] = (int)((ghi[a] - glo[a]) * G->inv_cell) + 1;  // every grown box maps below n
+            }
+        } else {
+            // An infinite radius (or grown boxes beyond the float64 range):
+            // one cell with every triangle, which every finite point maps to
+            // ((p - 0) * 0 == 0); a non-finite point maps to NaN and is
+            // outside, as it is farther than any radius from every triangle.
+            G->inv_cell = 0;
+            for (int a = 0; a < 3; a++) G->o[a] = 0, G->n[a] = 1;
+        }
+        auto range = [&](double lo, double hi, int a, int &i0, int &i1) {
+            // the device's lookup: floor((p - o) * inv_cell), monotone in p
+            const double fa = (lo - G->o[a]) * G->inv_cell, fb = (hi - G->o[a]) * G->inv_cell;
+            i0 = (int)std::max(0.0, std::min((double)(G->n[a] - 1), floor(fa)));
+            i1 = (int)std::max(0.0, std::min((double)(G->n[a] - 1), floor(fb)));
+        };
+        const size_t ncell = (size_t)G->n[0] * G->n[1] * G->n[2];
+        std::vector<uint32_t> count(ncell + 1, 0);
+        for (int pass = 0; pass < 2; pass++) {
+            if (pass == 1) {
+                G->cell_start.assign(ncell + 1, 0);
+                for (size_t c = 0; c < ncell; c++) G->cell_start[c + 1] = G->cell_start[c] + count[c];
+                G->cell_tris.resize(G->cell_start.back());
+                std::copy(G->cell_start.begin(), G->cell_start.end() - 1, count.begin());
+            }
+            for (size_t i = 0; i < nt; i++) {
+                int r0[3], r1[3];
+                for (int a = 0; a < 3; a++) range(boxes[i].lo[a], boxes[i].hi[a], a, r0[a], r1[a]);
+                for (int z = r0[2]; z <= r1[2]; z++)
+                    for (int y = r0[1]; y <= r1[1]; y++)
+                        for (int x = r0[0]; x <= r1[0]; x++) {
+                            const size_t c = ((size_t)z * G->n[1] + y) * G->n[0] + x;
+                            if (pass == 0) count[c]++;
+                            else G->cell_tris[count[c]++] = (uint32_t)i;
+                        }
+            }
+        }
+    } else {
+        G->cell_start.assign(2, 0);
+    }
+    spheres.push_back(std::move(G));
+    return *spheres.back();
+}
+
+// ---------------------------------------------------------------------------
 // Lowering
 // ---------------------------------------------------------------------------
 namespace {
@@ -792,10 +905,13 @@ struct Lowerer {
             op(OP_AND);
             return true;
         }
-        case K_COLLIDER: {
+        case K_COLLIDER: {  // ColliderSolid.Contains, solid.go:292-300
+            // params: collider, the solid's bounds, inset, radius
+            const double inset = s.p[0], radius = s.p[1];
             P.colliders.push_back(s.collider);
-            const ColliderData &c = *s.collider;
-            atom(A_COLLIDER, {(double)(P.colliders.size() - 1), c.mn[0], c.mn[1], c.mn[2], c.mx[0], c.mx[1], c.mx[2]},
+            P.collider_R.push_back(radius != 0 ? radius : fabs(inset));
+            atom(A_COLLIDER, {(double)(P.colliders.size() - 1), s.mn[0], s.mn[1], s.mn[2], s.mx[0], s.mx[1], s.mx[2],
+                              inset, radius},
                  local_box(s));
             return true;
         }
@@ -1011,6 +1127,36 @@ mc3d_solid *mc3d_collider_solid(const double *tris, size_t n) {
     memcpy(s->mn, cd->mn, sizeof s->mn);
     memcpy(s->mx, cd->mx, sizeof s->mx);
     return wrap(s);
+}
+
+// NewColliderSolidInset(c, inset), solid.go:260-270: the collider's bounds
+// grown inward by inset (min + inset; max = Max(min, c.Max() - inset)).
+mc3d_solid *mc3d_collider_solid_inset(const double *tris, size_t n, double inset) {
+    if (!isfinite(inset)) return set_error("mc3d_collider_solid_inset: non-finite inset"), nullptr;
+    mc3d_solid *h = mc3d_collider_solid(tris, n);
+    if (!h) return nullptr;
+    Solid *s = const_cast<Solid *>(h->ptr.get());
+    const V3 iv = mk(inset, inset, inset);
+    const V3 mn = add(of(s->mn), iv), mx = vmax(mn, sub(of(s->mx), iv));
+    s->mn[0] = mn.x, s->mn[1] = mn.y, s->mn[2] = mn.z;
+    s->mx[0] = mx.x, s->mx[1] = mx.y, s->mx[2] = mx.z;
+    s->p[0] = inset;
+    return h;
+}
+
+// NewColliderSolidHollow(c, r), solid.go:272-279: the collider's bounds grown
+// by r; Contains is SphereCollision(p, r).
+mc3d_solid *mc3d_collider_solid_hollow(const double *tris, size_t n, double r) {
+    if (!isfinite(r)) return set_error("mc3d_collider_solid_hollow: non-finite radius"), nullptr;
+    mc3d_solid *h = mc3d_collider_solid(tris, n);
+    if (!h) return nullptr;
+    Solid *s = const_cast<Solid *>(h->ptr.get());
+    const V3 rv = mk(r, r, r);
+    const V3 mn = sub(of(s->mn), rv), mx = add(of(s->mx), rv);
+    s->mn[0] = mn.x, s->mn[1] = mn.y, s->mn[2] = mn.z;
+    s->mx[0] = mx.x, s->mx[1] = mx.y, s->mx[2] = mx.z;
+    s->p[1] = r;
+    return h;
 }
 
 mc3d_solid *mc3d_host_solid(const double min[3], const double max[3], mc3d_contains_fn fn, void *user) {

@@ -436,17 +436,37 @@ def MeshToCollider(mesh) -> MeshCollider:
 
 
 class ColliderSolid(Solid):
-    """NewColliderSolid(c) (solid.go:247-300): ray-parity containment."""
+    """NewColliderSolid(c) (solid.go:247-300): ray-parity containment; with an
+    inset, ColliderContains(c, p, inset) (collisions.go:119-134); with a
+    radius, the hollow shell SphereCollision(p, radius)."""
 
-    def __init__(self, collider: MeshCollider):
+    def __init__(self, collider: MeshCollider, inset: float = 0.0, radius: float = 0.0):
         self.collider = collider
+        self.inset = float(inset)
+        self.radius = float(radius)
 
     def _build(self, b):
+        if self.radius != 0:
+            return b.collider(self.collider.triangles, hollow=self.radius)
+        if self.inset != 0:
+            return b.collider(self.collider.triangles, inset=self.inset)
         return b.collider(self.collider.triangles)
 
 
 def NewColliderSolid(c: MeshCollider) -> ColliderSolid:
     return ColliderSolid(c)
+
+
+def NewColliderSolidInset(c: MeshCollider, inset: float) -> ColliderSolid:
+    """NewColliderSolidInset (solid.go:260-270): containment at least `inset`
+    from the surface (negative: also within -inset outside it)."""
+    return ColliderSolid(c, inset=inset)
+
+
+def NewColliderSolidHollow(c: MeshCollider, r: float) -> ColliderSolid:
+    """NewColliderSolidHollow (solid.go:272-279): the shell of points within
+    r of the surface."""
+    return ColliderSolid(c, radius=r)
 
 
 # ---------------------------------------------------------------------------
@@ -509,8 +529,14 @@ class LibBackend:
     def force_bounds(self, c, mn, mx):
         return self._chk(self.L.mc3d_force_bounds(c, _lib._D3(*mn), _lib._D3(*mx)), "mc3d_force_bounds")
 
-    def collider(self, tris):
+    def collider(self, tris, inset=None, hollow=None):
         t = np.ascontiguousarray(tris, dtype=np.float64)
+        if hollow is not None:
+            return self._chk(self.L.mc3d_collider_solid_hollow(t.ctypes.data_as(_lib._dp), len(t), hollow),
+                             "mc3d_collider_solid_hollow")
+        if inset is not None:
+            return self._chk(self.L.mc3d_collider_solid_inset(t.ctypes.data_as(_lib._dp), len(t), inset),
+                             "mc3d_collider_solid_inset")
         return self._chk(
             self.L.mc3d_collider_solid(t.ctypes.data_as(_lib._dp), len(t)), "mc3d_collider_solid"
         )

@@ -36,6 +36,9 @@ _SIG = {
     "or_transform": (_P, [_P, ctypes.POINTER(ctypes.c_int), _dp, ctypes.c_int]),
     "or_force_bounds": (_P, [_P, _D3, _D3]),
     "or_collider": (_P, [_dp, ctypes.c_long]),
+    "or_collider_inset": (_P, [_dp, ctypes.c_long, ctypes.c_double]),
+    "or_collider_hollow": (_P, [_dp, ctypes.c_long, ctypes.c_double]),
+    "or_tri_sphere_collision": (ctypes.c_int, [_dp, _D3, ctypes.c_double]),
     "or_host": (_P, [_D3, _D3, HOST_FN, ctypes.c_void_p]),
     "or_free": (None, [_P]),
     "or_bounds": (None, [_P, _D3, _D3]),
@@ -143,8 +146,12 @@ class OracleBackend:
     def force_bounds(self, c, mn, mx):
         return self.L.or_force_bounds(c, _D3(*mn), _D3(*mx))
 
-    def collider(self, tris):
+    def collider(self, tris, inset=None, hollow=None):
         t = np.ascontiguousarray(tris, dtype=np.float64)
+        if hollow is not None:
+            return self.L.or_collider_hollow(t.ctypes.data_as(_dp), len(t), hollow)
+        if inset is not None:
+            return self.L.or_collider_inset(t.ctypes.data_as(_dp), len(t), inset)
         return self.L.or_collider(t.ctypes.data_as(_dp), len(t))
 
     def host(self, mn, mx, batch_fn):
@@ -318,6 +325,13 @@ def oracle_rect_collider(tris):
         return bool(L.or_tris_rect_collision(t.ctypes.data_as(_dp), len(t), _D3(*mn), _D3(*mx)))
 
     return collides
+
+
+def oracle_tri_sphere(tri, center, r) -> bool:
+    """Triangle.SphereCollision (primitives.go:251-279) of one triangle (9
+    doubles) and a sphere."""
+    t = np.ascontiguousarray(np.asarray(tri, dtype=np.float64).reshape(9))
+    return bool(lib().or_tri_sphere_collision(t.ctypes.data_as(_dp), _D3(*center), float(r)))
 
 
 def oracle_c2f(solid, big_delta, small_delta, extra_space, iters, threads=None):

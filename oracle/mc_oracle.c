@@ -457,6 +457,29 @@ or_node *or_collider(const double *tris, long ntris) {
     return n;
 }
 
+/* NewColliderSolidInset(MeshToCollider(mesh), inset), solid.go:260-270:
+ * min = c.Min() + inset, max = min.Max(c.Max() - inset); and
+ * NewColliderSolidHollow(MeshToCollider(mesh), r), solid.go:272-279:
+ * min = c.Min() - r, max = c.Max() + r.  p[0] = inset, p[1] = radius. */
+or_node *or_collider_inset(const double *tris, long ntris, double inset) {
+    or_node *n = or_collider(tris, ntris);
+    c3 iv = v3(inset, inset, inset);
+    c3 mn = c3_add(n->mn, iv);
+    n->mx = c3_max(mn, c3_sub(n->mx, iv));
+    n->mn = mn;
+    n->p[0] = inset;
+    return n;
+}
+
+or_node *or_collider_hollow(const double *tris, long ntris, double r) {
+    or_node *n = or_collider(tris, ntris);
+    c3 rv = v3(r, r, r);
+    n->mn = c3_sub(n->mn, rv);
+    n->mx = c3_add(n->mx, rv);
+    n->p[1] = r;
+    return n;
+}
+
 or_node *or_host(const double *mn, const double *mx, or_host_fn fn, void *user) {
     or_node *n = node_new(OR_HOST);
     n->mn = c3_of(mn);
@@ -514,6 +537,56 @@ static int tri_ray_hit(const double *t, c3 origin, c3 dir) {
     /* RayCollisions: `if info == nil || scale < 0 { return 0 }` */
     return !(scale < 0);
 }
+
+/* Triangle.rayCollision (primitives.go:207-249) for Triangle.SphereCollision:
+ * 1 when it reports a collision (info != nil), its scale in *scale. */
+static int tri_ray_scale(const double *t, c3 origin, c3 dir, double *scale) {
+    double d = c3_dot(tri_normal(t), c3_normalize(dir));
+    if (d < 1e-8 && d > -1e-8) return 0;
+    c3 t0 = c3_of(t), t1 = c3_of(t + 3), t2 = c3_of(t + 6);
+    c3 v1 = c3_sub(t1, t0);
+    c3 v2 = c3_sub(t2, t0);
+    c3 cross1 = c3_cross(dir, v2);
+    double det = c3_dot(cross1, v1);
+    if (det == 0) return 0;
+    double inv_det = 1 / det;
+    c3 o = c3_sub(origin, t0);
+    double bary1 = inv_det * c3_dot(o, cross1);
+    if (bary1 < 0 || bary1 > 1) return 0;
+    c3 cross2 = c3_cross(o, v1);
+    double bary2 = inv_det * c3_dot(dir, cross2);
+    if (bary2 < 0 || bary1 + bary2 > 1) return 0;
+    *scale = inv_det * c3_dot(v2, cross2);
+    return 1;
+}
+
+/* segmentEntersSphere, primitives.go:273-279 */
+static int segment_enters_sphere(c3 p1, c3 p2, c3 c, double r) {
+    c3 v = c3_sub(p2, p1);
+    double frac = (c3_dot(c, v) - c3_dot(p1, v)) / c3_dot(v, v);
+    c3 closest = c3_add(p1, c3_scale(v, frac));
+    return frac >= 0 && frac <= 1 && c3_dist(closest, c) < r;
+}
+
+/* Triangle.SphereCollision, primitives.go:251-271 */
+static int tri_sphere(const double *t, c3 c, double r) {
+    for (int k = 0; k < 3; k++)
+        if (c3_dist(c3_of(t + 3 * k), c) < r) return 1;
+    for (int i = 0; i < 3; i++)
+        if (segment_enters_sphere(c3_of(t + 3 * i), c3_of(t + 3 * ((i + 1) % 3)), c, r)) return 1;
+    double scale;
+    return tri_ray_scale(t, c, tri_normal(t), &scale) && fabs(scale) < r;
+}
+
+/* The mesh collider's SphereCollision (JoinedCollider, collisions.go:292-302):
+ * any triangle's SphereCollision, here by brute force over all triangles. */
+static int mesh_sphere(const or_node *n, c3 c, double r) {
+    for (long i = 0; i < n->ntris; i++)
+        if (tri_sphere(n->tris + 9 * i, c, r)) return 1;
+    return 0;
+}
+
+int or_tri_sphere_collision(const double *tri, const double *c, double r) { return tri_sphere(tri, c3_of(c), r); }
 
 /* Fixed ray direction of ColliderContains, collisions.go:120-125. */
 static const double k_ray_dir[3] = {0.5224892708603626, 0.10494477243214506, 0.43558938446126527};
@@ -620,12 +693,19 @@ static int node_contains(const or_node *n, c3 c) {
     case OR_FORCEBOUNDS: /* solid.go:307-309 */
         if (!in_bounds(c, n->mn, n->mx)) return 0;
         return node_contains(n->child[0], c);
-    case OR_COLLIDER: { /* solid.go:292-300, collisions.go:119-134 (inset 0) */
+    case OR_COLLIDER: { /* ColliderSolid.Contains, solid.go:292-300 */
         if (!in_bounds(c, n->mn, n->mx)) return 0;
+        const double inset = p[0], radius = p[1];
+        if (radius != 0) return mesh_sphere(n, c, radius);
+        /* ColliderContains(c, coord, inset), collisions.go:119-134 */
         c3 dir = c3_of(k_ray_dir);
         long count = 0;
         for (long i = 0; i < n->ntris; i++) count += tri_ray_hit(n->tris + 9 * i, c, dir);
-        return (count % 2) != 0;
+        if (count % 2 == 0) {
+            if (inset < 0) return mesh_sphere(n, c, -inset);
+            return 0;
+        }
+        return inset <= 0 || !mesh_sphere(n, c, inset);
     }
     case OR_HOST: {
         double q[3] = {c.x, c.y, c.z};

@@ -55,6 +55,10 @@ or_node *or_subtracted(or_node *pos, or_node *neg);
 or_node *or_transform(or_node *child, const int *kinds, const double *params, int nops);
 or_node *or_force_bounds(or_node *child, const double *mn, const double *mx);
 or_node *or_collider(const double *tris, long ntris);
+or_node *or_collider_inset(const double *tris, long ntris, double inset);
+or_node *or_collider_hollow(const double *tris, long ntris, double r);
+/* Triangle.SphereCollision (primitives.go:251-271) of one triangle (9 doubles) */
+int or_tri_sphere_collision(const double *tri, const double *c, double r);
 or_node *or_host(const double *mn, const double *mx, or_host_fn fn, void *user);
 void or_free(or_node *n);
 

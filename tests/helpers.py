@@ -112,6 +112,12 @@ def solid_catalogue():
     sph_tris, _ = oracle_search(m3.Sphere((0.05, -0.02, 0.1), 0.7), 0.1, 4, threads=8)
     col_sph = m3.NewColliderSolid(m3.MeshCollider(sph_tris))
     col_rot = m3.RotateSolid(col_sph, (0.0, 0.6, 0.8), 0.5)
+    # ColliderContains with a margin (collisions.go:119-134) and the hollow
+    # shell (solid.go:260-300): an inset sphere mesh, the boxes grown by an
+    # outset, and a shell around the sphere mesh.
+    col_inset = m3.NewColliderSolidInset(m3.MeshCollider(sph_tris), 0.09)
+    col_outset = m3.NewColliderSolidInset(m3.MeshToCollider(boxes), -0.12)
+    col_hollow = m3.NewColliderSolidHollow(m3.MeshCollider(sph_tris), 0.07)
     # Host-evaluated solids: alone, and under transforms inside a CSG tree
     # beside native atoms (the callback sees points in its own frame).
     host = Superellipsoid((0.05, -0.1, 0.02), 0.75)
@@ -139,6 +145,9 @@ def solid_catalogue():
         "collider_boxes": col_boxes,
         "collider_sphere": col_sph,
         "collider_rotated": col_rot,
+        "collider_inset": col_inset,
+        "collider_outset": col_outset,
+        "collider_hollow": col_hollow,
         "host_superellipsoid": host,
         "host_in_csg": host_csg,
     }

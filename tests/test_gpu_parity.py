@@ -96,7 +96,7 @@ def test_culling_and_brick_sizes_do_not_change_results():
     for precision in (FP64, GUARD):
         for cull, brick, rpw, wpb in ((False, 0, 0, 0), (True, 1, 1, 4), (True, 2, 2, 1), (True, 4, 4, 2),
                                       (True, 8, 1, 1), (True, 16, 4, 4), (True, 3, 2, 2), (True, 4, 2, 1),
-                                      (True, 8, 4, 1)):
+                                      (True, 8, 4, 1), (True, 16, 1, 1), (True, 32, 2, 4)):
             got = m3.MarchingCubesSearch(s, 0.01, 8, precision=precision, cull=cull, brick=brick, rows_per_walk=rpw,
                                          waves_per_brick=wpb)
             assert np.array_equal(canonical(got.triangles()), ref), (precision, cull, brick, rpw, wpb)

@@ -444,6 +444,110 @@ def test_collider_matches_numpy_restatement():
     assert np.array_equal(oracle_contains(m3.NewColliderSolid(m3.MeshCollider(bx)), p), numpy_ray_parity(bx, p))
 
 
+# ---------------------------------------------------------------------------
+# ColliderSolid with an inset or a hollow radius (solid.go:260-300,
+# collisions.go:119-134, primitives.go:251-279)
+# ---------------------------------------------------------------------------
+def numpy_triangle_distance(tris: np.ndarray, p: np.ndarray) -> np.ndarray:
+    """Euclidean distance from p to each triangle (T, 9): the plane distance
+    where the projection falls inside the triangle, else the nearest edge
+    (the textbook decomposition, not the reference's ray formulation)."""
+    t = tris.reshape(-1, 3, 3)
+    a, b, c = t[:, 0], t[:, 1], t[:, 2]
+    n = np.cross(b - a, c - a)
+    nn = np.einsum("ij,ij->i", n, n)
+    h = np.einsum("ij,ij->i", p - a, n) / nn
+    q = p - h[:, None] * n
+    inside = np.ones(len(t), dtype=bool)
+    for u, v in ((a, b), (b, c), (c, a)):
+        inside &= np.einsum("ij,ij->i", np.cross(v - u, q - u), n) >= 0
+    d = np.where(inside, np.abs(h) * np.sqrt(nn), np.inf)
+    for u, v in ((a, b), (b, c), (c, a)):
+        e = v - u
+        s = np.clip(np.einsum("ij,ij->i", p - u, e) / np.einsum("ij,ij->i", e, e), 0, 1)
+        d = np.minimum(d, np.linalg.norm(u + s[:, None] * e - p, axis=1))
+    return d
+
+
+def test_triangle_sphere_collision_is_distance_below_radius():
+    """Triangle.SphereCollision == (distance to the triangle < r) away from
+    ties, on random triangles, centres and radii (including centres whose
+    nearest feature is a vertex, an edge or the face)."""
+    from oracle.binding import oracle_tri_sphere
+
+    rng = np.random.default_rng(21)
+    checked = {"face": 0, "other": 0}
+    for _ in range(3000):
+        tri = rng.normal(size=9)
+        c = rng.normal(size=3) * rng.choice([0.3, 1.0, 3.0])
+        d = numpy_triangle_distance(tri[None], c)[0]
+        r = d * rng.choice([0.5, 0.99, 1.01, 2.0])
+        if abs(d - r) < 1e-9 * (1 + r):
+            continue
+        assert oracle_tri_sphere(tri, c, r) == (d < r), (tri, c, r, d)
+        t = tri.reshape(3, 3)
+        n = np.cross(t[1] - t[0], t[2] - t[0])
+        plane = abs((c - t[0]) @ n) / np.linalg.norm(n)
+        checked["face" if abs(plane - d) < 1e-12 else "other"] += 1
+    assert min(checked.values()) > 500
+
+
+def _unit_box_distance(p):
+    """(inside, distance to the surface of [0, 1]^3)."""
+    inside = np.all((p > 0) & (p < 1), axis=1)
+    d_in = np.minimum(p, 1 - p).min(axis=1)
+    d_out = np.linalg.norm(np.maximum(0, np.maximum(-p, p - 1)), axis=1)
+    return inside, np.where(inside, d_in, d_out)
+
+
+def test_collider_inset_outset_hollow_on_a_box():
+    """Closed forms on the unit box mesh: inset m keeps the points at least m
+    inside; outset -m adds the points closer than m outside; the hollow shell
+    of radius r is the points closer than r to the surface."""
+    box = m3.NewMeshRect((0, 0, 0), (1, 1, 1))
+    rng = np.random.default_rng(22)
+    p = rng.uniform(-0.3, 1.3, size=(20000, 3))
+    inside, d = _unit_box_distance(p)
+    plane = np.abs(p - np.round(p)).min(axis=1)  # grazing rays along face planes
+    m = 0.15
+    cases = {
+        "inset": (m3.NewColliderSolidInset(m3.MeshToCollider(box), m), inside & (d >= m)),
+        "outset": (m3.NewColliderSolidInset(m3.MeshToCollider(box), -m), inside | (d < m)),
+        "hollow": (m3.NewColliderSolidHollow(m3.MeshToCollider(box), m), d < m),
+    }
+    far = (np.abs(d - m) > 1e-9) & (plane > 1e-9)
+    for name, (s, want) in cases.items():
+        got = oracle_contains(s, p)
+        assert np.array_equal(got[far], want[far]), (name, np.count_nonzero(got[far] != want[far]))
+        assert want[far].sum() > 1000, name
+    # bounds (solid.go:263-280): the inset box, max clamped to min when the
+    # inset passes the middle, and the box grown by the radius
+    assert oracle_bounds(cases["inset"][0]) == ((0.15,) * 3, (0.85,) * 3)
+    assert oracle_bounds(m3.NewColliderSolidInset(m3.MeshToCollider(box), 0.7)) == ((0.7,) * 3, (0.7,) * 3)
+    assert oracle_bounds(cases["outset"][0]) == ((-0.15,) * 3, (1.15,) * 3)
+    assert oracle_bounds(cases["hollow"][0]) == ((-0.15,) * 3, (1.15,) * 3)
+
+
+def test_collider_inset_on_a_sphere_mesh_matches_numpy():
+    """Ray parity and the mesh distance, both restated in numpy, on a meshed
+    sphere: inset keeps odd parity at distance >= m, outset adds even parity
+    at distance < m, hollow is distance < r."""
+    sph, _ = oracle_search(m3.Sphere((0.05, -0.02, 0.1), 0.8), 0.2, 4, threads=4)
+    rng = np.random.default_rng(23)
+    p = rng.uniform(-1, 1, size=(400, 3))
+    parity = numpy_ray_parity(sph, p)
+    d = np.array([numpy_triangle_distance(sph, q).min() for q in p])
+    m = 0.1
+    far = np.abs(d - m) > 1e-9
+    col = m3.MeshCollider(sph)
+    for s, want in ((m3.NewColliderSolidInset(col, m), parity & (d >= m)),
+                    (m3.NewColliderSolidInset(col, -m), parity | (d < m)),
+                    (m3.NewColliderSolidHollow(col, m), d < m)):
+        got = oracle_contains(s, p)
+        assert np.array_equal(got[far], want[far])
+        assert 20 < want.sum() < len(p) - 20
+
+
 def test_new_mesh_rect_is_closed_and_outward():
     m = m3.NewMeshRect((0, 0, 0), (1, 2, 3))
     t = m.triangles()
