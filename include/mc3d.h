@@ -235,19 +235,23 @@ int mc3d_rect_collider_collides(void *collider, const double min[3], const doubl
 void mc3d_rect_collider_free(mc3d_rect_collider *c);
 /* MarchingCubesFilter(s, f, delta), model3d/mc.go:73-135: marching cubes
  * only in the blocks f accepts -- the reference's block recursion (mcBlock
- * Pieces/Split, mc.go:724-797, asked on the calling thread), its triangles
- * those of the cubes of accepted blocks.  Every corner is still evaluated, so
- * "solid is true outside of bounds" covers the whole grid. */
+ * Pieces/Split, mc.go:724-797), its triangles those of the cubes of accepted
+ * blocks.  A caller's f is asked on the calling thread; with
+ * f == mc3d_rect_collider_collides the recursion and the collider's test run
+ * on the device (one wave per block, the triangles binned in a grid).  Every
+ * corner is still evaluated, so "solid is true outside of bounds" covers the
+ * whole grid. */
 int mc3d_marching_cubes_filter(mc3d_ctx *ctx, const mc3d_solid *s, mc3d_rect_filter_fn f, void *user, double delta,
                                const mc3d_options *opts, mc3d_mesh **out);
 /* MarchingCubesSearchFilter(s, f, delta, iters), mc.go:137-143 */
 int mc3d_marching_cubes_search_filter(mc3d_ctx *ctx, const mc3d_solid *s, mc3d_rect_filter_fn f, void *user,
                                       double delta, int iters, const mc3d_options *opts, mc3d_mesh **out);
 /* MarchingCubesC2F(s, bigDelta, smallDelta, extraSpace, iters), mc.go:145-180:
- * the coarse search on the GPU, its triangles as a rect collider on the host
- * (MeshToCollider(...).RectCollision, 16 threads), then the filtered fine
- * search.  bigDelta < smallDelta is MC3D_E_INVALID_ARG with the reference's
- * panic text. */
+ * the coarse search, its mesh as the rect collider (MeshToCollider(...)
+ * .RectCollision of each block's bounds expanded by extraSpace +
+ * 2 sqrt(3) bigDelta) of the block recursion, then the filtered fine search,
+ * all on the device.  bigDelta < smallDelta is MC3D_E_INVALID_ARG with the
+ * reference's panic text. */
 int mc3d_marching_cubes_c2f(mc3d_ctx *ctx, const mc3d_solid *s, double big_delta, double small_delta,
                             double extra_space, int iters, const mc3d_options *opts, mc3d_mesh **out);
 /* MarchingCubesSearch(solids[i], deltas[i], iters[i]) for i < n in one pass

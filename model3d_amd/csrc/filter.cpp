@@ -13,154 +13,29 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <thread>
 
 #include "mc3d_internal.h"
+#include "rect_collide.h"
 
 namespace mc3d {
-namespace {
 
-struct V {
-    double x, y, z;
-};
-V mk(double x, double y, double z) { return V{x, y, z}; }
-V sub(V a, V b) { return mk(a.x + -b.x, a.y + -b.y, a.z + -b.z); }  // Sub = Add(Scale(-1)), coords.go:267-269
-double dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-V cross(V a, V b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
-double norm(V a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
-V scale(V a, double s) { return mk(a.x * s, a.y * s, a.z * s); }
-V normalize(V a) { return scale(a, 1 / norm(a)); }
-double comp(V a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
-// math.Min / math.Max with Go's NaN and signed-zero rules
-double go_min(double a, double b) {
-    if (isinf(a) && a < 0) return a;
-    if (isinf(b) && b < 0) return b;
-    if (isnan(a) || isnan(b)) return NAN;
-    if (a == 0 && a == b) return signbit(a) ? a : b;
-    return a < b ? a : b;
-}
-double go_max(double a, double b) {
-    if (isinf(a) && a > 0) return a;
-    if (isinf(b) && b > 0) return b;
-    if (isnan(a) || isnan(b)) return NAN;
-    if (a == 0 && a == b) return signbit(a) ? b : a;
-    return a > b ? a : b;
-}
-
-struct Seg {
-    V a, b;
-};
-// NewSegment, primitives.go:547-554
-Seg new_segment(V p1, V p2) {
-    if (p1.x < p2.x || (p1.x == p2.x && p1.y < p2.y) || (p1.x == p2.x && p1.y == p2.y && p1.z < p2.z)) return {p1, p2};
-    return {p2, p1};
-}
-
-// rayCollisionWithBounds, bvh.go:322-351
-void ray_bounds(V origin, V dir, const double *mn, const double *mx, double &min_frac, double &max_frac) {
-    min_frac = -INFINITY, max_frac = INFINITY;
-    for (int axis = 0; axis < 3; axis++) {
-        const double o = comp(origin, axis), rate = comp(dir, axis);
-        if (rate == 0) {
-            if (o < mn[axis] || o > mx[axis]) {
-                min_frac = 0, max_frac = -1;
-                return;
-            }
-            continue;
-        }
-        double t1 = (mn[axis] - o) / rate, t2 = (mx[axis] - o) / rate;
-        if (t1 > t2) std::swap(t1, t2);
-        if (t2 < 0) {
-            min_frac = 0, max_frac = -1;
-            return;
-        }
-        if (t1 > min_frac) min_frac = t1;
-        if (t2 < max_frac) max_frac = t2;
-    }
-}
-
-// Segment.RectCollision, primitives.go:632-638
-bool segment_rect(const Seg &s, const double *mn, const double *mx) {
-    double lo, hi;
-    ray_bounds(s.a, sub(s.b, s.a), mn, mx, lo, hi);
-    return hi >= lo && hi >= 0 && lo <= 1;
-}
-
-// Triangle.rayCollision's scale (primitives.go:207-249) for FirstRayCollision
-// (181-188): false when there is no collision at scale >= 0.
-bool tri_ray(const V *t, V origin, V dir, double &scale_out) {
-    const V n = normalize(cross(sub(t[1], t[0]), sub(t[2], t[0])));  // Triangle.Normal
-    const double d = dot(n, normalize(dir));
-    if (d < 1e-8 && d > -1e-8) return false;
-    const V v1 = sub(t[1], t[0]), v2 = sub(t[2], t[0]);
-    const V cross1 = cross(dir, v2);
-    const double det = dot(cross1, v1);
-    if (det == 0) return false;
-    const double inv = 1 / det;
-    const V o = sub(origin, t[0]);
-    const double bary1 = inv * dot(o, cross1);
-    if (bary1 < 0 || bary1 > 1) return false;
-    const V cross2 = cross(o, v1);
-    const double bary2 = inv * dot(dir, cross2);
-    if (bary2 < 0 || bary1 + bary2 > 1) return false;
-    scale_out = inv * dot(v2, cross2);
-    return scale_out >= 0;
-}
-
-// Triangle.SegmentCollision, primitives.go:440-445
-bool tri_segment(const V *t, const Seg &s) {
-    double sc;
-    return tri_ray(t, s.a, sub(s.b, s.a), sc) && sc <= 1;
-}
-
-// Triangle.RectCollision, primitives.go:448-495
-bool tri_rect(const V *t, const double *mn, const double *mx) {
-    for (int i = 0; i < 3; i++)
-        if (segment_rect(new_segment(t[i], t[(i + 1) % 3]), mn, mx)) return true;
-    for (double x : {mn[0], mx[0]}) {
-        const Seg segs[4] = {
-            new_segment(mk(x, mn[1], mn[2]), mk(x, mn[1], mx[2])),
-            new_segment(mk(x, mn[1], mn[2]), mk(x, mx[1], mn[2])),
-            new_segment(mk(x, mx[1], mn[2]), mk(x, mx[1], mx[2])),
-            new_segment(mk(x, mn[1], mx[2]), mk(x, mx[1], mx[2])),
-        };
-        for (const Seg &s : segs)
-            if (tri_segment(t, s)) return true;
-    }
-    const Seg segs[4] = {
-        new_segment(mk(mn[0], mn[1], mn[2]), mk(mx[0], mn[1], mn[2])),
-        new_segment(mk(mn[0], mx[1], mn[2]), mk(mx[0], mx[1], mn[2])),
-        new_segment(mk(mn[0], mn[1], mx[2]), mk(mx[0], mn[1], mx[2])),
-        new_segment(mk(mn[0], mx[1], mx[2]), mk(mx[0], mx[1], mx[2])),
-    };
-    for (const Seg &s : segs)
-        if (tri_segment(t, s)) return true;
-    return false;
-}
-
-// The bounds test of a joined collider, collisions.go:344-350: the rect and
-// the box overlap unless max(rmin, bmin) > min(rmax, bmax) on some axis.
-bool box_overlap(const double *rmn, const double *rmx, const double *bmn, const double *bmx) {
-    for (int a = 0; a < 3; a++) {
-        const double lo = go_max(rmn[a], bmn[a]), hi = go_min(rmx[a], bmx[a]);
-        if (go_min(lo, hi) != lo) return false;
-    }
-    return true;
-}
-
-}  // namespace
+using namespace rc;
 
 // A bounding-volume tree over the coarse mesh's triangles: a rect collides
 // with the mesh if it collides with a triangle whose box it overlaps.
-void RectMesh::build(const double *tris9, size_t n) {
-    tris.assign(tris9, tris9 + 9 * n);
+void RectMesh::build(const double *tris9, size_t n) { tris.assign(tris9, tris9 + 9 * n); }
+
+void RectMesh::build_tree() const {
+    const size_t n = tris.size() / 9;
     order.resize(n);
     for (size_t i = 0; i < n; i++) order[i] = (uint32_t)i;
     nodes.clear();
     if (n) split(0, (uint32_t)n);
 }
 
-uint32_t RectMesh::split(uint32_t lo, uint32_t hi) {
+uint32_t RectMesh::split(uint32_t lo, uint32_t hi) const {
     Node nd;
     for (int a = 0; a < 3; a++) nd.mn[a] = INFINITY, nd.mx[a] = -INFINITY;
     for (uint32_t i = lo; i < hi; i++) {
@@ -188,6 +63,7 @@ uint32_t RectMesh::split(uint32_t lo, uint32_t hi) {
 }
 
 bool RectMesh::collides(const double *mn, const double *mx) const {
+    std::call_once(tree_once, [this] { build_tree(); });
     if (nodes.empty()) return false;
     uint32_t stack[128];
     int sp = 0;
@@ -197,12 +73,7 @@ bool RectMesh::collides(const double *mn, const double *mx) const {
         if (!box_overlap(mn, mx, nd.mn, nd.mx)) continue;
         if (nd.left == NONE) {
             for (uint32_t i = nd.lo; i < nd.hi; i++) {
-                const double *p = &tris[9 * (size_t)order[i]];
-                const V t[3] = {mk(p[0], p[1], p[2]), mk(p[3], p[4], p[5]), mk(p[6], p[7], p[8])};
-                double tmn[3], tmx[3];  // Triangle.Min/Max, primitives.go:35-43
-                for (int a = 0; a < 3; a++)
-                    tmn[a] = go_min(go_min(p[a], p[3 + a]), p[6 + a]), tmx[a] = go_max(go_max(p[a], p[3 + a]), p[6 + a]);
-                if (box_overlap(mn, mx, tmn, tmx) && tri_rect(t, mn, mx)) return true;
+                if (triangle_hits_rect(&tris[9 * (size_t)order[i]], mn, mx)) return true;
             }
         } else {
             stack[sp++] = nd.left;

@@ -85,6 +85,53 @@ hipError_t launch_stream(bool write, void *buf, uint64_t items, uint32_t *out, h
 hipError_t launch_encode_stl(const double *verts, const uint32_t *tris, uint64_t nt, uint16_t *body,
                              hipStream_t st);
 
+// A mesh collider's triangles (9 doubles each) binned by their boxes in a
+// uniform grid of n[0] x n[1] x n[2] cells: cell (x, y, z) holds
+// cell_tris[cell_start[c], cell_start[c + 1]), c = (z * n[1] + y) * n[0] + x;
+// point p is in cell floor((p - o) * inv), clamped to the grid.
+// Beside it, a finer grid of the vertices that are a segment's origin (all
+// but a triangle's largest, NewSegment's order) as a summed-volume table:
+// vsat[(z * vn[1] + y) * vn[0] + x] counts those in cells <= (x, y, z).
+struct RectGridDev {
+    const double *tris = nullptr;
+    uint32_t ntris = 0;
+    const uint32_t *cell_start = nullptr, *cell_tris = nullptr;
+    double o[3] = {0, 0, 0}, inv = 1;
+    int n[3] = {1, 1, 1};
+    const uint32_t *vsat = nullptr;
+    double vo[3] = {0, 0, 0}, vinv = 1;
+    int vn[3] = {1, 1, 1};
+};
+// A block of cubes [mn, mx) of MarchingCubesFilter's recursion (corner indices).
+struct FBlock {
+    int mn[3], mx[3];
+};
+// The recursion's filter: RectCollision of R with a block's bounds grown by
+// eps, then by extra; accepted leaves set their cubes in keep (occupancy
+// word layout, W words per row, ny rows per plane).
+struct BlockFilter {
+    RectGridDev R;
+    const double *xs, *ys, *zs;
+    double eps, extra;
+    uint64_t *keep;
+    int W, ny;
+    Ctl *ctl;  // filter_overflow: a level outgrew its block list
+};
+hipError_t launch_rgrid_gather(const double *verts, const uint32_t *tris, uint64_t nt, double *out, hipStream_t st);
+// counts must start at zero
+hipError_t launch_rgrid_count(const RectGridDev &R, uint32_t *counts, hipStream_t st);
+// start (ncell + 1 entries) and cursor from the counts; tiles: ceil(ncell / 1024) <= 1024 entries
+hipError_t launch_rgrid_scan(const uint32_t *counts, uint32_t ncell, uint32_t *tiles, uint32_t *start,
+                             uint32_t *cursor, hipStream_t st);
+hipError_t launch_rgrid_fill(const RectGridDev &R, uint32_t *cursor, uint32_t *cell_tris, hipStream_t st);
+// R.vsat's table from the triangles: vsat must start at zero
+hipError_t launch_vgrid(const RectGridDev &R, uint32_t *vsat, hipStream_t st);
+// level 0's list = {root}; level_counts[0] = 1, level_counts[1..nlevels] = 0
+hipError_t launch_filter_root(const FBlock &root, FBlock *front, uint32_t *level_counts, uint32_t nlevels,
+                              hipStream_t st);
+hipError_t launch_filter_level(const BlockFilter &F, const FBlock *in, const uint32_t *nin, FBlock *out,
+                               uint32_t *nout, uint32_t cap, uint64_t max_in, hipStream_t st);
+
 inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 20) {
     uint64_t b = (n + block - 1) / block;
     if (b == 0) b = 1;

@@ -22,9 +22,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <exception>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -260,6 +262,9 @@ struct mc3d_ctx {
     HostBuf hstage;                      // pinned points / answers of one callback batch
     DevBuf stl, stl_in;                  // STL encoding: body bytes; uploaded host triangles
     DevBuf keep;                         // block filter: cube mask
+    // device block filter: triangles, their grid, the recursion's block lists
+    DevBuf rf_tris, rf_counts, rf_start, rf_cursor, rf_cells, rf_tiles, rf_front[2], rf_levels, rf_vsat;
+    HostBuf rf_total;
     std::shared_ptr<Pool<DevBuf>> pool = std::make_shared<Pool<DevBuf>>();
     std::shared_ptr<Pool<HostBuf>> hpool = std::make_shared<Pool<HostBuf>>();
     uint64_t v_hint = 1u << 18, t_hint = 1u << 19;  // output capacities to start from
@@ -290,6 +295,18 @@ struct mc3d_mesh {
 
 namespace {
 
+// MarchingCubesFilter with a mesh collider's RectCollision as its filter,
+// run on the device (k_filter.hip): the collider's triangles, on the host
+// or as a mesh of this context's device, and the filter's Rect.Expand.
+struct DeviceRectFilter {
+    const double *host_tris = nullptr;   // 9 doubles per triangle, or
+    const double *dev_verts = nullptr;   // a device mesh: its vertices
+    const uint32_t *dev_tris = nullptr;  // and triangle indices
+    uint64_t ntris = 0;
+    double extra = 0;  // Rect.Expand of every block's bounds (0: none)
+    double cell = 0;   // grid cell edge: about twice a triangle's extent
+};
+
 // One solid of a call, prepared on the host.
 struct Job {
     const mc3d_solid *s = nullptr;
@@ -306,6 +323,7 @@ struct Job {
     bool interior = false;         // MarchingCubesInterior: keep each vertex's inside point
     std::vector<double> conj;      // MarchingCubesConj: inverse chain applied to the vertices
     std::vector<uint64_t> keep;    // MarchingCubesFilter: cubes of accepted blocks (occupancy layout)
+    const DeviceRectFilter *dfilter = nullptr;  // or the filter that makes that mask on the device
 };
 
 // Frame of each host-evaluated atom, by host atom index.
@@ -1061,6 +1079,135 @@ int graph_launch(mc3d_ctx *ctx, const Pipe &p, const Bufs &b, const Upload &U, d
 
 // Meshes `jobs` (prepared) in one pass of the pipeline; out[i] gets job i's
 // mesh.  A job with host-evaluated atoms is only ever run alone.
+// Blocks at each level of mcBlock.Pieces(64, ...) (mc.go:760-797) from a
+// root of l[0] x l[1] x l[2] cubes when every block is accepted: the sizes
+// the level lists need.  The halves of a split are floor and ceil of the
+// longest axis, so few distinct shapes occur per level.
+std::vector<uint64_t> filter_level_sizes(int lx, int ly, int lz) {
+    std::map<std::array<int, 3>, uint64_t> cur{{{lx, ly, lz}, 1}};
+    std::vector<uint64_t> sizes;
+    while (!cur.empty()) {
+        uint64_t total = 0;
+        std::map<std::array<int, 3>, uint64_t> next;
+        for (const auto &e : cur) {
+            total += e.second;
+            const std::array<int, 3> &d = e.first;
+            if ((int64_t)d[0] * d[1] * d[2] / 2 < 64) continue;
+            int axis = 0;
+            if (d[1] >= d[0] && d[1] >= d[2]) axis = 1;
+            else if (d[2] >= d[0] && d[2] >= d[1]) axis = 2;
+            std::array<int, 3> a = d, b = d;
+            a[axis] = d[axis] / 2, b[axis] = d[axis] - d[axis] / 2;
+            next[a] += e.second;
+            next[b] += e.second;
+        }
+        sizes.push_back(total);
+        cur.swap(next);
+    }
+    return sizes;
+}
+
+// The block filter of job j on the device, into the cube mask `keep`
+// (after the upload: it reads the grid's axes there).  The collider's
+// triangles are binned in a grid over the search grid's bounds (cells of
+// about F.cell, at most 2^20 of them; triangles outside fall in the edge
+// cells), then the recursion runs one launch per level.
+int enqueue_block_filter(mc3d_ctx *ctx, const Job &j, const Upload &U, uint64_t *keep) {
+    const DeviceRectFilter &F = *j.dfilter;
+    hipStream_t st = ctx->stream;
+    if (F.ntris >= (1ull << 32) / 9) {
+        set_error("block filter: too many collider triangles");
+        return MC3D_E_UNSUPPORTED;
+    }
+    int rc;
+    const size_t tbytes = F.ntris * 9 * sizeof(double);
+    if ((rc = ctx->rf_tris.ensure(tbytes + 8))) return rc;
+    if (F.host_tris) {
+        if (tbytes) HIP_TRY(hipMemcpyAsync(ctx->rf_tris.p, F.host_tris, tbytes, hipMemcpyHostToDevice, st));
+    } else {
+        HIP_TRY(launch_rgrid_gather(F.dev_verts, F.dev_tris, F.ntris, ctx->rf_tris.as<double>(), st));
+    }
+    RectGridDev R;
+    R.tris = ctx->rf_tris.as<double>();
+    R.ntris = (uint32_t)F.ntris;
+    double ext[3], emax = 0;
+    for (int a = 0; a < 3; a++) {
+        R.o[a] = j.ax[a].front();
+        ext[a] = j.ax[a].back() - j.ax[a].front();
+        emax = std::max(emax, ext[a]);
+    }
+    double cell = F.cell > 0 && std::isfinite(F.cell) ? F.cell : emax / 64;
+    cell = std::max(cell, emax / 1e6);
+    if (!(cell > 0) || !std::isfinite(cell)) cell = 1;
+    uint64_t ncell;
+    for (;;) {
+        ncell = 1;
+        for (int a = 0; a < 3; a++) {
+            R.n[a] = (int)std::min(ext[a] / cell + 1, 1e7);
+            ncell *= (uint64_t)R.n[a];
+        }
+        if (ncell <= (1u << 20)) break;
+        cell *= 1.25;
+    }
+    R.inv = 1 / cell;
+    if ((rc = ctx->rf_counts.ensure(ncell * 4)) || (rc = ctx->rf_start.ensure(ncell * 4 + 4)) ||
+        (rc = ctx->rf_cursor.ensure(ncell * 4)) || (rc = ctx->rf_tiles.ensure(1024 * 4)) ||
+        (rc = ctx->rf_total.ensure(4)))
+        return rc;
+    HIP_TRY(hipMemsetAsync(ctx->rf_counts.p, 0, ncell * 4, st));
+    HIP_TRY(launch_rgrid_count(R, ctx->rf_counts.as<uint32_t>(), st));
+    HIP_TRY(launch_rgrid_scan(ctx->rf_counts.as<uint32_t>(), (uint32_t)ncell, ctx->rf_tiles.as<uint32_t>(),
+                              ctx->rf_start.as<uint32_t>(), ctx->rf_cursor.as<uint32_t>(), st));
+    HIP_TRY(hipMemcpyAsync(ctx->rf_total.p, ctx->rf_start.as<uint32_t>() + ncell, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    const uint32_t total = *ctx->rf_total.as<uint32_t>();
+    if ((rc = ctx->rf_cells.ensure((size_t)total * 4 + 4))) return rc;
+    R.cell_start = ctx->rf_start.as<uint32_t>();
+    R.cell_tris = ctx->rf_cells.as<uint32_t>();
+    HIP_TRY(launch_rgrid_fill(R, ctx->rf_cursor.as<uint32_t>(), ctx->rf_cells.as<uint32_t>(), st));
+    // the vertex table: cells of half the edge, at most 2^21
+    double vcell = cell / 2;
+    uint64_t nvcell;
+    for (;;) {
+        nvcell = 1;
+        for (int a = 0; a < 3; a++) {
+            R.vn[a] = (int)std::min(ext[a] / vcell + 1, 1e7);
+            nvcell *= (uint64_t)R.vn[a];
+        }
+        if (nvcell <= (1u << 21)) break;
+        vcell *= 1.25;
+    }
+    for (int a = 0; a < 3; a++) R.vo[a] = R.o[a];
+    R.vinv = 1 / vcell;
+    if ((rc = ctx->rf_vsat.ensure(nvcell * 4))) return rc;
+    HIP_TRY(hipMemsetAsync(ctx->rf_vsat.p, 0, nvcell * 4, st));
+    HIP_TRY(launch_vgrid(R, ctx->rf_vsat.as<uint32_t>(), st));
+    R.vsat = ctx->rf_vsat.as<uint32_t>();
+
+    // the recursion from the root block (mc.go:712-716), level by level
+    const FBlock root{{0, 0, 0}, {j.g.nx - 1, j.g.ny - 1, j.g.nz - 1}};
+    const std::vector<uint64_t> sizes = filter_level_sizes(root.mx[0], root.mx[1], root.mx[2]);
+    const uint64_t cap = *std::max_element(sizes.begin(), sizes.end());
+    if (cap >= (1ull << 32)) {
+        set_error("block filter: too many blocks");
+        return MC3D_E_UNSUPPORTED;
+    }
+    const uint32_t nlevels = (uint32_t)sizes.size();
+    if ((rc = ctx->rf_front[0].ensure(cap * sizeof(FBlock))) || (rc = ctx->rf_front[1].ensure(cap * sizeof(FBlock))) ||
+        (rc = ctx->rf_levels.ensure((nlevels + 1) * 4)))
+        return rc;
+    HIP_TRY(hipMemsetAsync(keep, 0, j.nwords * 8, st));
+    uint32_t *counts = ctx->rf_levels.as<uint32_t>();
+    FBlock *front[2] = {ctx->rf_front[0].as<FBlock>(), ctx->rf_front[1].as<FBlock>()};
+    HIP_TRY(launch_filter_root(root, front[0], counts, nlevels, st));
+    const Grid &g = U.hsegs[0].g;
+    const BlockFilter BF{R, g.xs, g.ys, g.zs, j.delta * 1e-3, F.extra, keep, g.W, g.ny, U.ctl};
+    for (uint32_t L = 0; L < nlevels; L++)
+        HIP_TRY(launch_filter_level(BF, front[L & 1], counts + L, front[(L + 1) & 1], counts + L + 1, (uint32_t)cap,
+                                    sizes[L], st));
+    return MC3D_OK;
+}
+
 int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::vector<mc3d_mesh *> &out,
              std::chrono::steady_clock::time_point t_call, double ms_prepare) {
     const auto t_enqueue = std::chrono::steady_clock::now();
@@ -1070,10 +1217,12 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
     if ((rc = plan_call(jobs, o, c))) return rc;
     Bufs b;
     if ((rc = ctx_buffers(ctx, c, b))) return rc;
-    if (c.nsegs == 1 && !jobs[0].keep.empty()) {
-        if ((rc = ctx->keep.ensure(jobs[0].keep.size() * 8))) return rc;
-        HIP_TRY(hipMemcpyAsync(ctx->keep.p, jobs[0].keep.data(), jobs[0].keep.size() * 8, hipMemcpyHostToDevice,
-                               ctx->stream));
+    const bool dfilter = c.nsegs == 1 && jobs[0].dfilter;
+    if (c.nsegs == 1 && (!jobs[0].keep.empty() || dfilter)) {
+        if ((rc = ctx->keep.ensure(jobs[0].nwords * 8))) return rc;
+        if (!dfilter)
+            HIP_TRY(hipMemcpyAsync(ctx->keep.p, jobs[0].keep.data(), jobs[0].keep.size() * 8, hipMemcpyHostToDevice,
+                                   ctx->stream));
         b.keep = ctx->keep.as<uint64_t>();
     }
     Upload U;
@@ -1092,6 +1241,7 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
     bool graphed = o.device_only && !c.host && !c.stats && !b.keep && !o.no_graph && !ctx->graphs_off;
     if (c.host) {
         if ((rc = enqueue_upload(ctx, U))) return rc;
+        if (dfilter && (rc = enqueue_block_filter(ctx, jobs[0], U, ctx->keep.as<uint64_t>()))) return rc;
         if ((rc = host_round(ctx, jobs[0], U, 0, nullptr, (uint64_t)jobs[0].g.nx * jobs[0].g.ny * jobs[0].g.nz,
                              st.ms_host_callbacks)))
             return rc;
@@ -1127,7 +1277,9 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
         // when the meshes fit, their copies to the host are enqueued at once.
         Ctl *he = (Ctl *)ctx->ctl_early.p;
         if (!graphed) {
-            if ((rc = enqueue_upload(ctx, U)) || (rc = enqueue_front(p, b, ctx->stream, ctx->ev))) return rc;
+            if ((rc = enqueue_upload(ctx, U))) return rc;
+            if (dfilter && (rc = enqueue_block_filter(ctx, jobs[0], U, ctx->keep.as<uint64_t>()))) return rc;
+            if ((rc = enqueue_front(p, b, ctx->stream, ctx->ev))) return rc;
             if (to_host) {
                 HIP_TRY(hipMemcpyAsync(he, U.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
                 HIP_TRY(hipEventRecord(ctx->ev[EV_CTL], ctx->stream));
@@ -1185,6 +1337,10 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
             HIP_TRY(hipStreamSynchronize(ctx->copy));
             ms_d2h = ms_since(t0);
         }
+    }
+    if (hc->filter_overflow) {  // the level lists are sized for every block accepted
+        set_error("internal error: block filter level list overflow");
+        return MC3D_E_INTERNAL;
     }
     const uint64_t nv = hc->totals[0], nt = hc->totals[1];
     if (!to_host && (rc = fit_outputs(ctx, *mb, nv, nt))) return rc;
@@ -1449,7 +1605,7 @@ using RectFilter = std::function<bool(const double *, const double *)>;
 
 int search_one(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters, const mc3d_options *opts,
                mc3d_mesh **out, bool interior, std::vector<double> conj, const RectFilter *filter = nullptr,
-               int filter_threads = 1) {
+               int filter_threads = 1, const DeviceRectFilter *dfilter = nullptr) {
     *out = nullptr;
     const auto t_call = std::chrono::steady_clock::now();
     const mc3d_options o = default_options(opts);
@@ -1461,6 +1617,7 @@ int search_one(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters, cons
     int rc = prepare_job(jobs[0], o);
     if (rc) return rc;
     if (filter && (rc = filter_keep_mask(jobs[0].ax, delta, *filter, filter_threads, jobs[0].keep))) return rc;
+    jobs[0].dfilter = dfilter;
     const double ms_prepare = ms_since(t_call);
     std::vector<mc3d_mesh *> meshes;
     if ((rc = run_call(ctx, jobs, o, meshes, t_call, ms_prepare))) return rc;
@@ -1471,6 +1628,7 @@ int search_one(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters, cons
 
 struct mc3d_rect_collider {
     RectMesh mesh;
+    double cell = 0;  // the device filter's grid cell: twice the mean triangle extent
 };
 
 extern "C" {
@@ -1482,11 +1640,18 @@ int mc3d_marching_cubes_search_filter(mc3d_ctx *ctx, const mc3d_solid *s, mc3d_r
         return MC3D_E_INVALID_ARG;
     }
     return guarded([&]() -> int {
+        if (f == mc3d_rect_collider_collides) {
+            // the library's own rect collider: the recursion runs on the device
+            const mc3d_rect_collider *rcol = (const mc3d_rect_collider *)user;
+            DeviceRectFilter df;
+            df.host_tris = rcol->mesh.tris.data();
+            df.ntris = rcol->mesh.tris.size() / 9;
+            df.cell = rcol->cell;
+            return search_one(ctx, s, delta, iters, opts, out, false, {}, nullptr, 1, &df);
+        }
+        // a caller's filter is asked on this thread
         const RectFilter filter = [f, user](const double *mn, const double *mx) { return f(user, mn, mx) != 0; };
-        // a caller's filter is asked on this thread; the library's own rect
-        // collider is thread-safe and runs on 16
-        const int threads = f == mc3d_rect_collider_collides ? 16 : 1;
-        return search_one(ctx, s, delta, iters, opts, out, false, {}, &filter, threads);
+        return search_one(ctx, s, delta, iters, opts, out, false, {}, &filter, 1);
     });
 }
 
@@ -1499,6 +1664,16 @@ mc3d_rect_collider *mc3d_rect_collider_create(const double *triangles, size_t n)
     const int rc = guarded([&]() -> int {
         std::unique_ptr<mc3d_rect_collider> m(new mc3d_rect_collider);
         m->mesh.build(triangles, n);
+        double ext = 0;
+        size_t finite = 0;
+        for (size_t i = 0; i < n; i++) {
+            const double *t = triangles + 9 * i;
+            double e = 0;
+            for (int a = 0; a < 3; a++)
+                e = std::max(e, std::max(t[a], std::max(t[3 + a], t[6 + a])) - std::min(t[a], std::min(t[3 + a], t[6 + a])));
+            if (std::isfinite(e)) ext += e, finite++;
+        }
+        m->cell = finite ? 2 * ext / (double)finite : 0;
         c = m.release();
         return MC3D_OK;
     });
@@ -1529,30 +1704,26 @@ int mc3d_marching_cubes_c2f(mc3d_ctx *ctx, const mc3d_solid *s, double big_delta
     }
     return guarded([&]() -> int {
         double extra = extra_space + 2 * big_delta * sqrt(3.0);  // mc.go:167
-        // the coarse mesh (mc.go:169), its triangles on the host
+        // the coarse mesh (mc.go:169), left on the device
         mc3d_options co = default_options(opts);
         co.device_only = 1;
         mc3d_mesh *coarse = nullptr;
         int rc = search_one(ctx, s, big_delta, iters, &co, &coarse, false, {});
         if (rc) return rc;
-        std::vector<double> v(coarse->nv * 3);
-        std::vector<uint32_t> t(coarse->nt * 3);
-        rc = mc3d_mesh_copy(coarse, v.data(), t.data());
-        const size_t nt = coarse->nt;
-        mc3d_mesh_free(coarse);
-        if (rc) return rc;
-        std::vector<double> tris(nt * 9);
-        for (size_t i = 0; i < 3 * nt; i++)
-            for (int d = 0; d < 3; d++) tris[3 * i + d] = v[3 * (size_t)t[i] + d];
-        RectMesh collider;  // MeshToCollider(coarseMesh), mc.go:170
-        collider.build(tris.data(), nt);
-        // filter = collider.RectCollision(r.Expand(extraSpace)), mc.go:171-174
-        const RectFilter filter = [&collider, extra](const double *mn, const double *mx) {
-            const double emn[3] = {mn[0] + -extra, mn[1] + -extra, mn[2] + -extra};
-            const double emx[3] = {mx[0] + extra, mx[1] + extra, mx[2] + extra};
-            return collider.collides(emn, emx);
-        };
-        return search_one(ctx, s, small_delta, iters, opts, out, false, {}, &filter, 16);
+        std::unique_ptr<mc3d_mesh, void (*)(mc3d_mesh *)> keep_coarse(coarse, mc3d_mesh_free);
+        // filter = MeshToCollider(coarseMesh).RectCollision(r.Expand(extraSpace))
+        // (mc.go:170-174), on the device; the coarse triangles' extents are
+        // at most big_delta
+        DeviceRectFilter df;
+        if ((rc = mc3d_mesh_device_ptrs(coarse, &df.dev_verts, &df.dev_tris))) return rc;
+        df.ntris = coarse->nt;
+        df.extra = extra;
+        df.cell = 2 * big_delta;
+        if (df.ntris && (!df.dev_verts || !df.dev_tris)) {
+            set_error("internal error: coarse mesh not on the device");
+            return MC3D_E_INTERNAL;
+        }
+        return search_one(ctx, s, small_delta, iters, opts, out, false, {}, nullptr, 1, &df);
     });
 }
 

@@ -192,13 +192,15 @@ struct RectMesh {
         uint32_t lo = 0, hi = 0, left = NONE, right = NONE;
     };
     std::vector<double> tris;  // 9 doubles per triangle
-    std::vector<uint32_t> order;
-    std::vector<Node> nodes;
-    void build(const double *tris9, size_t n);
-    bool collides(const double *mn, const double *mx) const;
+    void build(const double *tris9, size_t n);  // keeps the triangles
+    bool collides(const double *mn, const double *mx) const;  // builds the tree on first use
 
   private:
-    uint32_t split(uint32_t lo, uint32_t hi);
+    mutable std::once_flag tree_once;
+    mutable std::vector<uint32_t> order;
+    mutable std::vector<Node> nodes;
+    void build_tree() const;
+    uint32_t split(uint32_t lo, uint32_t hi) const;
 };
 void solid_bounds(const Solid &s, double mn[3], double mx[3]);
 bool bounds_valid(const Solid &s);

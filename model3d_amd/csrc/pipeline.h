@@ -18,7 +18,7 @@ struct Ctl {
     uint32_t inside;       // panic("solid is true outside of bounds") seen
     uint32_t overflow;     // an output index exceeded the buffers' capacity
     uint32_t nact_bricks;  // bricks whose folded program is not constant
-    uint32_t pad;
+    uint32_t filter_overflow;  // the block filter's recursion outgrew a level's block list
     unsigned long long totals[4];  // scan totals: vertices, triangles, listed words
     unsigned long long gstats[4];  // guard statistics (GStats order)
     unsigned long long flops[4];   // statistics runs: float64, float32 ops of corners; the same of bisect

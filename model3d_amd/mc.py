@@ -296,7 +296,8 @@ def MarchingCubesFilter(s: Solid, f, delta: float, **opts) -> Mesh:
 def MarchingCubesC2F(s: Solid, bigDelta: float, smallDelta: float, extraSpace: float, iters: int, *,
                      precision: int = _lib.MC3D_PREC_FP32_GUARD, cull: bool = True, device: int = 0) -> Mesh:
     """model3d.MarchingCubesC2F (mc.go:145-180) on the GPU: a coarse search,
-    its mesh as a rect collider (host, 16 threads), the filtered fine search."""
+    its mesh (left on the device) as the rect collider of the block
+    recursion (on the device), the filtered fine search."""
     L = _lib.lib()
     ctx = _lib.context(device)
     b = LibBackend()

@@ -39,6 +39,7 @@ _SIG = {
     "or_collider_inset": (_P, [_dp, ctypes.c_long, ctypes.c_double]),
     "or_collider_hollow": (_P, [_dp, ctypes.c_long, ctypes.c_double]),
     "or_tri_sphere_collision": (ctypes.c_int, [_dp, _D3, ctypes.c_double]),
+    "or_group_triangles": (ctypes.c_int, [_dp, ctypes.c_long, ctypes.c_void_p]),
     "or_host": (_P, [_D3, _D3, HOST_FN, ctypes.c_void_p]),
     "or_free": (None, [_P]),
     "or_bounds": (None, [_P, _D3, _D3]),
@@ -332,6 +333,17 @@ def oracle_tri_sphere(tri, center, r) -> bool:
     doubles) and a sphere."""
     t = np.ascontiguousarray(np.asarray(tri, dtype=np.float64).reshape(9))
     return bool(lib().or_tri_sphere_collision(t.ctypes.data_as(_dp), _D3(*center), float(r)))
+
+
+def oracle_group_triangles(tris) -> np.ndarray:
+    """GroupTriangles (bvh.go:118-156, 219-255) of a triangle list (T, 9):
+    the grouped order as indices into it (sort ties broken by index)."""
+    t = np.ascontiguousarray(np.asarray(tris, dtype=np.float64).reshape(-1, 9))
+    perm = np.zeros(len(t), dtype=np.uint32)
+    rc = lib().or_group_triangles(t.ctypes.data_as(_dp), len(t), perm.ctypes.data)
+    if rc:
+        raise OracleError(rc)
+    return perm
 
 
 def oracle_c2f(solid, big_delta, small_delta, extra_space, iters, threads=None):

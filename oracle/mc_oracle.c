@@ -1570,6 +1570,109 @@ void or_encode_stl(const double *tris, long nt, uint8_t *out) {
     }
 }
 
+/* GroupTriangles(faces) = GroupBounders (model3d/bvh.go:118-156, 219-255):
+ * three orders of the triangles by the midpoint of their boxes (Triangle
+ * Min/Max, primitives.go:35-43; Coord3D.Mid, coords.go:196-198), then the
+ * recursive median split along the axis whose two halves' box areas
+ * (multipleBoundsArea, boundsArea, bvh.go:257-292) sum least (the first
+ * such axis), the other two orders stably partitioned by the halves.  The
+ * reference sorts with sort.Slice, which orders equal keys arbitrarily; this
+ * restatement breaks ties by the triangle's index (a stable sort), and puts
+ * a NaN key last.  perm[k] = the index of the k-th triangle of the grouped
+ * order. */
+typedef struct {
+    double mn[3], mx[3], mid[3];
+} grp_box;
+static const grp_box *grp_boxes;
+static int grp_axis;
+static int grp_cmp(const void *a, const void *b) {
+    const uint32_t i = *(const uint32_t *)a, j = *(const uint32_t *)b;
+    const double x = grp_boxes[i].mid[grp_axis], y = grp_boxes[j].mid[grp_axis];
+    const int xn = isnan(x), yn = isnan(y);
+    if (xn != yn) return xn - yn;
+    if (!xn && x < y) return -1;
+    if (!xn && y < x) return 1;
+    return i < j ? -1 : (i > j ? 1 : 0);
+}
+static double grp_area(const grp_box *bx, const uint32_t *ids, long n) {
+    double mn[3], mx[3];
+    for (int a = 0; a < 3; a++) mn[a] = bx[ids[0]].mn[a], mx[a] = bx[ids[0]].mx[a];
+    for (long i = 1; i < n; i++)
+        for (int a = 0; a < 3; a++) {
+            if (bx[ids[i]].mn[a] < mn[a]) mn[a] = bx[ids[i]].mn[a];
+            if (bx[ids[i]].mx[a] > mx[a]) mx[a] = bx[ids[i]].mx[a];
+        }
+    const double dx = mx[0] + -mn[0], dy = mx[1] + -mn[1], dz = mx[2] + -mn[2];
+    return 2 * (dx * (dy + dz) + dy * dz);
+}
+static void grp_rec(const grp_box *bx, uint32_t *sorted[3], long n, uint32_t *out, uint8_t *flag, uint32_t *tmp) {
+    if (n <= 2) {
+        for (long i = 0; i < n; i++) out[i] = sorted[0][i];
+        return;
+    }
+    const long mid = n / 2;
+    int axis = 0;
+    double best = grp_area(bx, sorted[0], mid) + grp_area(bx, sorted[0] + mid, n - mid);
+    for (int a = 1; a < 3; a++) {
+        const double v = grp_area(bx, sorted[a], mid) + grp_area(bx, sorted[a] + mid, n - mid);
+        if (v < best) best = v, axis = a;
+    }
+    for (long i = 0; i < n; i++) flag[sorted[axis][i]] = i < mid;
+    for (int a = 0; a < 3; a++) {
+        if (a == axis) continue;
+        long i0 = 0, i1 = mid;
+        for (long i = 0; i < n; i++) {
+            const uint32_t e = sorted[a][i];
+            if (flag[e]) tmp[i0++] = e;
+            else tmp[i1++] = e;
+        }
+        memcpy(sorted[a], tmp, n * sizeof(uint32_t));
+    }
+    uint32_t *lo[3] = {sorted[0], sorted[1], sorted[2]}, *hi[3] = {sorted[0] + mid, sorted[1] + mid, sorted[2] + mid};
+    grp_rec(bx, lo, mid, out, flag, tmp);
+    grp_rec(bx, hi, n - mid, out + mid, flag, tmp);
+}
+static double go_min_(double a, double b) {
+    if (isinf(a) && a < 0) return a;
+    if (isinf(b) && b < 0) return b;
+    if (isnan(a) || isnan(b)) return NAN;
+    if (a == 0 && a == b) return signbit(a) ? a : b;
+    return a < b ? a : b;
+}
+static double go_max_(double a, double b) {
+    if (isinf(a) && a > 0) return a;
+    if (isinf(b) && b > 0) return b;
+    if (isnan(a) || isnan(b)) return NAN;
+    if (a == 0 && a == b) return signbit(a) ? b : a;
+    return a > b ? a : b;
+}
+int or_group_triangles(const double *tris, long nt, uint32_t *perm) {
+    if (nt <= 0) return 0;
+    grp_box *bx = malloc(nt * sizeof(grp_box));
+    uint32_t *s[3], *tmp = malloc(nt * sizeof(uint32_t));
+    uint8_t *flag = malloc(nt);
+    for (int a = 0; a < 3; a++) s[a] = malloc(nt * sizeof(uint32_t));
+    if (!bx || !tmp || !flag || !s[0] || !s[1] || !s[2]) return 4;
+    for (long t = 0; t < nt; t++) {
+        const double *p = tris + 9 * t;
+        for (int a = 0; a < 3; a++) {
+            bx[t].mn[a] = go_min_(go_min_(p[a], p[3 + a]), p[6 + a]);
+            bx[t].mx[a] = go_max_(go_max_(p[a], p[3 + a]), p[6 + a]);
+            bx[t].mid[a] = (bx[t].mn[a] + bx[t].mx[a]) * 0.5;
+        }
+    }
+    grp_boxes = bx;
+    for (int a = 0; a < 3; a++) {
+        for (long t = 0; t < nt; t++) s[a][t] = (uint32_t)t;
+        grp_axis = a;
+        qsort(s[a], nt, sizeof(uint32_t), grp_cmp);
+    }
+    grp_rec(bx, s, nt, perm, flag, tmp);
+    free(bx), free(tmp), free(flag);
+    for (int a = 0; a < 3; a++) free(s[a]);
+    return 0;
+}
+
 long or_mesh_ntris(const or_mesh *m) { return m->ntri; }
 const double *or_mesh_tris(const or_mesh *m) { return m->tris; }
 void or_mesh_info(const or_mesh *m, long *corners, long *vertices, long *search_evals, double *ms_scan,

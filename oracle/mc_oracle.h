@@ -87,6 +87,9 @@ int or_marching_cubes_filter(const or_node *s, or_rect_fn f, void *user, double 
 int or_tris_rect_collision(const double *tris, long nt, const double *mn, const double *mx);
 /* EncodeSTL of nt triangles (9 doubles each) into out (84 + 50 nt bytes). */
 void or_encode_stl(const double *tris, long nt, uint8_t *out);
+/* GroupTriangles of nt triangles (9 doubles each): perm[k] = index of the
+ * k-th triangle in the grouped order (ties in the sorts by index).  0 or 4 (out of memory). */
+int or_group_triangles(const double *tris, long nt, uint32_t *perm);
 long or_mesh_ntris(const or_mesh *m);
 const double *or_mesh_tris(const or_mesh *m);
 void or_mesh_info(const or_mesh *m, long *corners, long *vertices, long *search_evals, double *ms_scan,

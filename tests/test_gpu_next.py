@@ -4,12 +4,13 @@ kernels -- MarchingCubesInterior (mc.go:55-60), MarchingCubesConj
 and of the binary STL encoder (export.go:20-66), through the C-ABI,
 bit-exact against the oracle."""
 import ctypes
+import math
 
 import numpy as np
 import pytest
 
 import model3d_amd as m3
-from model3d_amd import _lib
+from model3d_amd import _lib, workloads
 from oracle.binding import (
     oracle_c2f,
     oracle_conj,
@@ -232,6 +233,37 @@ def test_c2f_parity(name, precision):
     got = m3.MarchingCubesC2F(s, 0.1, 0.05, 0.0, 6, precision=precision)
     want = oracle_c2f(s, 0.1, 0.05, 0.0, 6, threads=8)
     assert same_triangles(got.triangles(), want)
+
+
+def test_c2f_extra_space_parity():
+    s = solid_catalogue()["torus"]
+    print("c2f torus extra_space=0.03", flush=True)
+    got = m3.MarchingCubesC2F(s, 0.1, 0.05, 0.03, 4)
+    want = oracle_c2f(s, 0.1, 0.05, 0.03, 4, threads=8)
+    assert same_triangles(got.triangles(), want)
+
+
+def test_device_block_filter_equals_host_filter_at_bench_size():
+    """C2F on the CSG workload at its bench size (coarse delta 0.02, fine
+    0.005): the device's block recursion over the binned coarse mesh gives
+    the mesh that the recursion on the host gives, asking the coarse mesh's
+    RectCollision (its bounding-volume tree, host code) for every block."""
+    s = workloads.csg64_solid(2)
+    big, small = 0.02, 0.005
+    extra = 0.0 + 2 * big * math.sqrt(3)  # mc.go:167
+    print("c2f csg64 0.02 -> 0.005, device filter vs host filter", flush=True)
+    got = m3.MarchingCubesC2F(s, big, small, 0.0, 8)
+    col = m3.MeshToCollider(m3.MarchingCubesSearch(s, big, 8).triangles())
+    asked = [0]
+
+    def host_filter(r):  # collider.RectCollision(r.Expand(extra)), mc.go:171-174
+        asked[0] += 1
+        return col.RectCollision(m3.Rect(tuple(v + -extra for v in r.MinVal), tuple(v + extra for v in r.MaxVal)))
+
+    want = m3.MarchingCubesSearchFilter(s, host_filter, small, 8)
+    print(f"host filter asked {asked[0]} blocks; {len(want.faces)} triangles", flush=True)
+    assert len(want.faces) > 100000
+    assert np.array_equal(canonical(got.triangles()), canonical(want.triangles()))
 
 
 def test_c2f_equals_search_and_rejects_small_big_delta():

@@ -283,8 +283,9 @@ def test_large_workload_properties():
 
 def test_go_shim_call_sequence():
     """The calls the Go drop-in makes for MarchingCubesSearch, in order:
-    lower the solid, search with default options (results fetched to pinned
-    host memory), read the counts and the host arrays in place, free."""
+    lower the solid, search (FP64, culled, results fetched to pinned host
+    memory), read the counts and the host arrays in place, free; and the
+    same with no options (the library's defaults)."""
     import ctypes
 
     from model3d_amd import _lib
@@ -298,18 +299,23 @@ def test_go_shim_call_sequence():
     b = LibBackend()
     h = s._build(b)
     out = ctypes.c_void_p()
+    shim = _lib.mc3d_options()
+    shim.precision, shim.cull, shim.device_only = FP64, 1, 0
+    pv, pt = ctypes.c_void_p(), ctypes.c_void_p()
     try:
-        _lib.check(L.mc3d_marching_cubes_search(ctx, h, 0.01, 8, None, ctypes.byref(out)), "search")
-        nv, nt = ctypes.c_size_t(), ctypes.c_size_t()
-        _lib.check(L.mc3d_mesh_counts(out, ctypes.byref(nv), ctypes.byref(nt)), "counts")
-        pv, pt = ctypes.c_void_p(), ctypes.c_void_p()
-        _lib.check(L.mc3d_mesh_host_ptrs(out, ctypes.byref(pv), ctypes.byref(pt)), "host ptrs")
-        v = np.ctypeslib.as_array(ctypes.cast(pv, ctypes.POINTER(ctypes.c_double)), shape=(nv.value, 3))
-        f = np.ctypeslib.as_array(ctypes.cast(pt, ctypes.POINTER(ctypes.c_uint32)), shape=(nt.value, 3))
-        assert same_triangles(v[f.astype(np.int64)], want)
-        st = _lib.mc3d_stats()
-        L.mc3d_mesh_stats(out, ctypes.byref(st))
-        assert st.ms_d2h >= 0 and st.triangles == len(want)
+        for opts in (ctypes.byref(shim), None):
+            _lib.check(L.mc3d_marching_cubes_search(ctx, h, 0.01, 8, opts, ctypes.byref(out)), "search")
+            nv, nt = ctypes.c_size_t(), ctypes.c_size_t()
+            _lib.check(L.mc3d_mesh_counts(out, ctypes.byref(nv), ctypes.byref(nt)), "counts")
+            _lib.check(L.mc3d_mesh_host_ptrs(out, ctypes.byref(pv), ctypes.byref(pt)), "host ptrs")
+            v = np.ctypeslib.as_array(ctypes.cast(pv, ctypes.POINTER(ctypes.c_double)), shape=(nv.value, 3))
+            f = np.ctypeslib.as_array(ctypes.cast(pt, ctypes.POINTER(ctypes.c_uint32)), shape=(nt.value, 3))
+            assert same_triangles(v[f.astype(np.int64)], want)
+            st = _lib.mc3d_stats()
+            L.mc3d_mesh_stats(out, ctypes.byref(st))
+            assert st.ms_d2h >= 0 and st.triangles == len(want)
+            L.mc3d_mesh_free(out)
+            out = ctypes.c_void_p()
         # a mesh left on the device has no host arrays
         dev = ctypes.c_void_p()
         o = _lib.mc3d_options()
