@@ -286,7 +286,7 @@ void mc3d_mesh_free(mc3d_mesh *m);
  * float64 (primitives.go:27-33), the float32 casts of its three vertices and
  * two zero bytes; little-endian.  Triangles in the given order (the
  * reference writes Mesh.TriangleSlice's map order; SaveGroupedSTL's
- * GroupTriangles reordering is not reproduced).  The encoding runs on the
+ * order is mc3d_mesh_group_triangles / mc3d_group_triangles first).  The encoding runs on the
  * device.  out must hold mc3d_stl_bytes(T) bytes; more than 2^32 - 1
  * triangles is MC3D_E_UNSUPPORTED ("too many triangles for STL format"). */
 size_t mc3d_stl_bytes(size_t n_triangles);
@@ -295,6 +295,21 @@ int mc3d_mesh_encode_stl(mc3d_ctx *ctx, const mc3d_mesh *m, uint8_t *out, size_t
 /* host arrays: verts 3 doubles per vertex, tris 3 vertex indices per triangle */
 int mc3d_encode_stl(mc3d_ctx *ctx, const double *verts, size_t nv, const uint32_t *tris, size_t nt, uint8_t *out,
                     size_t out_bytes);
+
+/* GroupTriangles(faces) (model3d/bvh.go:118-156, 219-292) on the device:
+ * perm[k] = the index of the k-th triangle of the grouped order (sorted by
+ * box midpoints per axis, recursively cut at n / 2 along the axis whose two
+ * halves' box areas sum least).  The reference's sort.Slice orders equal
+ * midpoints arbitrarily; here they keep their index order.  Triangles with
+ * NaN coordinates get an unspecified (but complete) order.  Host arrays as
+ * for mc3d_encode_stl. */
+int mc3d_group_triangles(mc3d_ctx *ctx, const double *verts, size_t nv, const uint32_t *tris, size_t nt,
+                         uint32_t *perm);
+/* The mesh's triangles reordered in place by GroupTriangles, as
+ * SaveGroupedSTL (mesh.go:721-743) writes them: then
+ * mc3d_mesh_encode_stl / mc3d_mesh_copy give the grouped order.  Not for a
+ * MarchingCubesInterior mesh. */
+int mc3d_mesh_group_triangles(mc3d_ctx *ctx, mc3d_mesh *m);
 
 /* Solid.Contains for n host points on the GPU (out[i] = 0/1). */
 int mc3d_contains(mc3d_ctx *ctx, const mc3d_solid *s, const double *pts, size_t n, int32_t precision,

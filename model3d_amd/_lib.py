@@ -177,6 +177,11 @@ SIGNATURES = {
         ctypes.c_int,
         [_P, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t],
     ),
+    "mc3d_group_triangles": (
+        ctypes.c_int,
+        [_P, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p],
+    ),
+    "mc3d_mesh_group_triangles": (ctypes.c_int, [_P, _P]),
     "mc3d_mesh_device_ptrs": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]),
     "mc3d_mesh_stats": (ctypes.c_int, [_P, ctypes.POINTER(mc3d_stats)]),
     "mc3d_mesh_free": (None, [_P]),

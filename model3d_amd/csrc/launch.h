@@ -132,6 +132,38 @@ hipError_t launch_filter_root(const FBlock &root, FBlock *front, uint32_t *level
 hipError_t launch_filter_level(const BlockFilter &F, const FBlock *in, const uint32_t *nin, FBlock *out,
                                uint32_t *nout, uint32_t cap, uint64_t max_in, hipStream_t st);
 
+// GroupTriangles (k_group.hip): a triangle's box, and one level of the
+// recursion: the three orders, the axis of each of the level's 2^L nodes,
+// the first-half flag of each triangle.
+struct GroupBox {
+    double mn[3], mx[3];
+};
+struct GroupLevel {
+    uint32_t n = 0;
+    int L = 0;
+    const GroupBox *box = nullptr;
+    const uint32_t *perm[3] = {nullptr, nullptr, nullptr};
+    uint8_t *axis = nullptr;
+    uint8_t *flag = nullptr;
+};
+// boxes and the three midpoint key arrays of triangles (verts + tris, or 9
+// doubles each when tris is null); iota[t] = t
+hipError_t launch_grp_boxes(const double *verts, const uint32_t *tris, uint32_t n, GroupBox *box, uint64_t *keys[3],
+                            uint32_t *iota, hipStream_t st);
+// stable radix sort of (key, value) pairs / exclusive sum (hipCUB; a null
+// tmp asks for the temporary bytes)
+hipError_t grp_sort(void *tmp, size_t &bytes, const uint64_t *keys_in, uint64_t *keys_out, const uint32_t *vals_in,
+                    uint32_t *vals_out, uint32_t n, hipStream_t st);
+hipError_t grp_scan(void *tmp, size_t &bytes, const uint32_t *in, uint32_t *out, uint32_t n, hipStream_t st);
+// the axis of every node (slots: 36 keys per node of a level whose nodes
+// hold more than 64 triangles)
+hipError_t launch_grp_axis(const GroupLevel &G, unsigned long long *slots, hipStream_t st);
+hipError_t launch_grp_flags(const GroupLevel &G, hipStream_t st);
+hipError_t launch_grp_gather_flags(const GroupLevel &G, int a, uint32_t *f, hipStream_t st);
+hipError_t launch_grp_scatter(const GroupLevel &G, int a, const uint32_t *scan, uint32_t *out, hipStream_t st);
+// out[k] = the three indices of triangle perm[k]
+hipError_t launch_grp_apply(const uint32_t *tris, const uint32_t *perm, uint32_t n, uint32_t *out, hipStream_t st);
+
 inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 20) {
     uint64_t b = (n + block - 1) / block;
     if (b == 0) b = 1;

@@ -261,6 +261,9 @@ struct mc3d_ctx {
     DevBuf hvals, hpts, hstate, hprobe;  // host-evaluated atoms: answers, points, bisection state
     HostBuf hstage;                      // pinned points / answers of one callback batch
     DevBuf stl, stl_in;                  // STL encoding: body bytes; uploaded host triangles
+    // GroupTriangles: boxes, keys, the three orders (and a spare), flags,
+    // scans, node axes and slots, hipCUB's temporary storage
+    DevBuf g_box, g_keys[4], g_perm[4], g_flag, g_f, g_scan, g_axis, g_slots, g_tmp, g_tris;
     DevBuf keep;                         // block filter: cube mask
     // device block filter: triangles, their grid, the recursion's block lists
     DevBuf rf_tris, rf_counts, rf_start, rf_cursor, rf_cells, rf_tiles, rf_front[2], rf_levels, rf_vsat;
@@ -1902,6 +1905,71 @@ namespace {
 // The binary STL of nt device triangles into out (host): the body by
 // k_encode_stl into device scratch, one copy through pinned memory, the
 // 84-byte header written here.
+// GroupTriangles of n triangles on this context's device (vertices and
+// triangle indices, or 9 doubles per triangle when tris is null): *perm
+// receives a device array of n triangle indices in the grouped order
+// (valid until the next grouping on this context).
+int group_order(mc3d_ctx *ctx, const double *verts, const uint32_t *tris, uint64_t n, const uint32_t **perm) {
+    *perm = nullptr;
+    if (n >= (1ull << 31)) {
+        set_error("GroupTriangles: more than 2^31 triangles");
+        return MC3D_E_UNSUPPORTED;
+    }
+    hipStream_t st = ctx->stream;
+    int rc;
+    const size_t n4 = n * 4 + 16, n8 = n * 8 + 16;
+    if ((rc = ctx->g_box.ensure(n * sizeof(GroupBox) + 16)) || (rc = ctx->g_flag.ensure(n + 16)) ||
+        (rc = ctx->g_f.ensure(n4)) || (rc = ctx->g_scan.ensure(n4)) || (rc = ctx->g_axis.ensure(n + 16)))
+        return rc;
+    for (int a = 0; a < 4; a++)
+        if ((rc = ctx->g_keys[a].ensure(n8)) || (rc = ctx->g_perm[a].ensure(n4))) return rc;
+    if (n == 0) {
+        *perm = ctx->g_perm[0].as<uint32_t>();
+        return MC3D_OK;
+    }
+    const uint32_t n32 = (uint32_t)n;
+    // the hipCUB temporary storage for the sorts and the scans
+    size_t sort_bytes = 0, scan_bytes = 0;
+    HIP_TRY(grp_sort(nullptr, sort_bytes, nullptr, nullptr, nullptr, nullptr, n32, st));
+    HIP_TRY(grp_scan(nullptr, scan_bytes, nullptr, nullptr, n32, st));
+    if ((rc = ctx->g_tmp.ensure(std::max(sort_bytes, scan_bytes) + 16))) return rc;
+    uint64_t *keys[3] = {ctx->g_keys[0].as<uint64_t>(), ctx->g_keys[1].as<uint64_t>(), ctx->g_keys[2].as<uint64_t>()};
+    uint32_t *iota = ctx->g_perm[3].as<uint32_t>();
+    HIP_TRY(launch_grp_boxes(verts, tris, n32, ctx->g_box.as<GroupBox>(), keys, iota, st));
+    uint32_t *P[4] = {ctx->g_perm[0].as<uint32_t>(), ctx->g_perm[1].as<uint32_t>(), ctx->g_perm[2].as<uint32_t>(),
+                      iota};
+    for (int a = 0; a < 3; a++) {
+        size_t bytes = ctx->g_tmp.cap;
+        HIP_TRY(grp_sort(ctx->g_tmp.p, bytes, keys[a], ctx->g_keys[3].as<uint64_t>(), iota, P[a], n32, st));
+    }
+    // the levels, until every node holds at most one triangle
+    int levels = 0;
+    while ((n32 + (1u << levels) - 1) >> levels > 1) levels++;
+    const uint32_t big_nodes = std::max<uint32_t>(1, n32 / 64 + 1);  // nodes of a level of nodes above 64
+    if ((rc = ctx->g_slots.ensure((size_t)big_nodes * 36 * 8 + 16))) return rc;
+    GroupLevel G;
+    G.n = n32;
+    G.box = ctx->g_box.as<GroupBox>();
+    G.axis = ctx->g_axis.as<uint8_t>();
+    G.flag = ctx->g_flag.as<uint8_t>();
+    uint32_t *spare = P[3];
+    for (int L = 0; L < levels; L++) {
+        G.L = L;
+        for (int a = 0; a < 3; a++) G.perm[a] = P[a];
+        HIP_TRY(launch_grp_axis(G, ctx->g_slots.as<unsigned long long>(), st));
+        HIP_TRY(launch_grp_flags(G, st));
+        for (int a = 0; a < 3; a++) {
+            HIP_TRY(launch_grp_gather_flags(G, a, ctx->g_f.as<uint32_t>(), st));
+            size_t bytes = ctx->g_tmp.cap;
+            HIP_TRY(grp_scan(ctx->g_tmp.p, bytes, ctx->g_f.as<uint32_t>(), ctx->g_scan.as<uint32_t>(), n32, st));
+            HIP_TRY(launch_grp_scatter(G, a, ctx->g_scan.as<uint32_t>(), spare, st));
+            std::swap(P[a], spare);
+        }
+    }
+    *perm = P[0];
+    return MC3D_OK;
+}
+
 int encode_stl_device(mc3d_ctx *ctx, const double *verts, const uint32_t *tris, uint64_t nt, uint8_t *out,
                       size_t out_bytes) {
     if (nt >= (1ull << 32)) {
@@ -1945,6 +2013,74 @@ int mc3d_mesh_encode_stl(mc3d_ctx *ctx, const mc3d_mesh *m, uint8_t *out, size_t
         // triangle indices count from the mesh's first vertex
         return encode_stl_device(ctx, m->buf->verts->as<double>() + 3 * m->voff,
                                  m->buf->tris->as<uint32_t>() + 3 * m->toff, m->nt, out, out_bytes);
+    });
+}
+
+int mc3d_group_triangles(mc3d_ctx *ctx, const double *verts, size_t nv, const uint32_t *tris, size_t nt,
+                         uint32_t *perm) {
+    if (!ctx || (nt && !perm) || (nv && !verts) || (nt && !tris)) {
+        set_error("mc3d_group_triangles: null argument");
+        return MC3D_E_INVALID_ARG;
+    }
+    for (size_t i = 0; i < 3 * nt; i++)
+        if (tris[i] >= nv) {
+            set_error("mc3d_group_triangles: triangle " + std::to_string(i / 3) + " refers to vertex " +
+                      std::to_string(tris[i]) + " of " + std::to_string(nv));
+            return MC3D_E_INVALID_ARG;
+        }
+    return guarded([&]() -> int {
+        HIP_TRY(hipSetDevice(ctx->device));
+        int rc;
+        if ((rc = ctx->stl_in.ensure(nv * 24 + nt * 12 + 16))) return rc;
+        double *dv = ctx->stl_in.as<double>();
+        uint32_t *dt = (uint32_t *)(ctx->stl_in.as<char>() + align_up(nv * 24, 16));
+        if (nv) HIP_TRY(hipMemcpyAsync(dv, verts, nv * 24, hipMemcpyHostToDevice, ctx->stream));
+        if (nt) HIP_TRY(hipMemcpyAsync(dt, tris, nt * 12, hipMemcpyHostToDevice, ctx->stream));
+        const uint32_t *dp = nullptr;
+        if ((rc = group_order(ctx, dv, dt, nt, &dp))) return rc;
+        if (nt) HIP_TRY(hipMemcpyAsync(perm, dp, nt * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return MC3D_OK;
+    });
+}
+
+int mc3d_mesh_group_triangles(mc3d_ctx *ctx, mc3d_mesh *m) {
+    if (!ctx || !m) {
+        set_error("mc3d_mesh_group_triangles: null argument");
+        return MC3D_E_INVALID_ARG;
+    }
+    if (m->device != ctx->device) {
+        set_error("mc3d_mesh_group_triangles: the mesh is not on this context's device");
+        return MC3D_E_INVALID_ARG;
+    }
+    if (m->has_interior) {
+        set_error("mc3d_mesh_group_triangles: a mesh with interior points keeps its order");
+        return MC3D_E_INVALID_ARG;
+    }
+    return guarded([&]() -> int {
+        HIP_TRY(hipSetDevice(ctx->device));
+        if (m->on_host) {
+            // the pinned arrays: grouped on the device, the indices permuted here
+            std::vector<uint32_t> perm(m->nt);
+            int rc = mc3d_group_triangles(ctx, m->hverts(), m->nv, m->htris(), m->nt, perm.data());
+            if (rc) return rc;
+            uint32_t *t = (uint32_t *)m->htris();
+            std::vector<uint32_t> old(t, t + 3 * m->nt);
+            for (size_t k = 0; k < m->nt; k++)
+                for (int c = 0; c < 3; c++) t[3 * k + c] = old[3 * (size_t)perm[k] + c];
+            return MC3D_OK;
+        }
+        double *dv = m->buf->verts->as<double>() + 3 * m->voff;
+        uint32_t *dt = m->buf->tris->as<uint32_t>() + 3 * m->toff;
+        const uint32_t *dp = nullptr;
+        int rc;
+        if ((rc = group_order(ctx, dv, dt, m->nt, &dp))) return rc;
+        if (!m->nt) return MC3D_OK;
+        if ((rc = ctx->g_tris.ensure(m->nt * 12 + 16))) return rc;
+        HIP_TRY(launch_grp_apply(dt, dp, (uint32_t)m->nt, ctx->g_tris.as<uint32_t>(), ctx->stream));
+        HIP_TRY(hipMemcpyAsync(dt, ctx->g_tris.p, m->nt * 12, hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return MC3D_OK;
     });
 }
 

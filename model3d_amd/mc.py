@@ -39,6 +39,12 @@ class Mesh:
         this mesh's order, encoded on the GPU (see EncodeSTL)."""
         return EncodeSTL(self, device=device)
 
+    def SaveGroupedSTL(self, path, device: int = 0) -> None:
+        """Mesh.SaveGroupedSTL (mesh.go:721-743): the triangles in
+        GroupTriangles order (grouped on the GPU), written as binary STL."""
+        g = Mesh(self.vertices, self.faces[group_order(self, device=device).astype(np.int64)])
+        WriteSTL(path, g, device=device)
+
     @classmethod
     def from_triangles(cls, tris) -> "Mesh":
         """Indexed mesh of a triangle list (vertices shared by exact bits)."""
@@ -213,6 +219,31 @@ def MarchingCubesConj(
             L.mc3d_mesh_free(out)
         b.release(h)
         b.close()
+
+
+def group_order(mesh_or_triangles, device: int = 0) -> np.ndarray:
+    """The GroupTriangles order (bvh.go:118-156, 219-292) of a Mesh's
+    triangles or of a (T, 3, 3) / (T, 9) triangle array, computed on the GPU:
+    indices into the triangles (equal box midpoints keep their index order)."""
+    if isinstance(mesh_or_triangles, Mesh):
+        v = np.ascontiguousarray(mesh_or_triangles.vertices, dtype=np.float64)
+        f = np.ascontiguousarray(mesh_or_triangles.faces, dtype=np.uint32)
+    else:
+        t = np.asarray(mesh_or_triangles, dtype=np.float64).reshape(-1, 9)
+        v = np.ascontiguousarray(t.reshape(-1, 3))
+        f = np.arange(len(v), dtype=np.uint32).reshape(-1, 3)
+    L = _lib.lib()
+    ctx = _lib.context(device)
+    perm = np.empty(len(f), dtype=np.uint32)
+    _raise_for(L.mc3d_group_triangles(ctx, v.ctypes.data, len(v), f.ctypes.data, len(f), perm.ctypes.data))
+    return perm
+
+
+def GroupTriangles(triangles, device: int = 0) -> np.ndarray:
+    """model3d.GroupTriangles (bvh.go:118-129): the triangles, (T, 3, 3), in
+    the grouped order that GroupedTrianglesToCollider and SaveGroupedSTL use."""
+    t = np.asarray(triangles, dtype=np.float64).reshape(-1, 3, 3)
+    return t[group_order(t, device=device).astype(np.int64)]
 
 
 def EncodeSTL(mesh_or_triangles, device: int = 0) -> bytes:

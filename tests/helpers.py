@@ -151,3 +151,37 @@ def solid_catalogue():
         "host_superellipsoid": host,
         "host_in_csg": host_csg,
     }
+
+
+def check_grouping(tris, perm):
+    """The grouped order's defining properties, node by node of its implicit
+    tree: a node of n > 2 triangles is the n // 2 triangles with the
+    smallest box midpoints along the axis whose two halves' box areas sum
+    least (the first such axis), then the rest; a node of two is in x order.
+    Sort ties are broken by the triangle's index."""
+    t = np.asarray(tris, dtype=np.float64).reshape(-1, 3, 3)
+    mn, mx = t.min(axis=1), t.max(axis=1)
+    mid = (mn + mx) * 0.5
+    assert sorted(perm.tolist()) == list(range(len(t)))
+
+    def area(ids):
+        d = mx[ids].max(axis=0) - mn[ids].min(axis=0)
+        return 2 * (d[0] * (d[1] + d[2]) + d[1] * d[2])
+
+    def node(ids):
+        n = len(ids)
+        if n <= 2:
+            assert list(ids) == sorted(ids, key=lambda i: (mid[i, 0], i))
+            return
+        h = n // 2
+        best, first = None, None
+        for a in range(3):
+            order = sorted(ids, key=lambda i: (mid[i, a], i))
+            v = area(np.array(order[:h])) + area(np.array(order[h:]))
+            if best is None or v < best:
+                best, first = v, set(order[:h])
+        assert set(ids[:h]) == first
+        node(ids[:h])
+        node(ids[h:])
+
+    node(list(perm))

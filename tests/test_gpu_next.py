@@ -16,11 +16,12 @@ from oracle.binding import (
     oracle_conj,
     oracle_contains,
     oracle_encode_stl,
+    oracle_group_triangles,
     oracle_rect_collider,
     oracle_search,
     oracle_search_filter,
 )
-from tests.helpers import canonical, same_triangles, solid_catalogue
+from tests.helpers import canonical, check_grouping, same_triangles, solid_catalogue
 
 pytestmark = pytest.mark.gpu
 
@@ -273,3 +274,77 @@ def test_c2f_equals_search_and_rejects_small_big_delta():
     assert np.array_equal(canonical(a.triangles()), canonical(b.triangles()))
     with pytest.raises(_lib.MC3DError, match="bigDelta should be >= smallDelta"):
         m3.MarchingCubesC2F(s, 0.05, 0.1, 0, 8)
+
+
+# ---------------------------------------------------------------------------
+# GroupTriangles (bvh.go:118-156, 219-292) and SaveGroupedSTL (mesh.go:721-743)
+# ---------------------------------------------------------------------------
+def _group_case(name):
+    if name == "random":
+        return np.random.default_rng(41).normal(size=(5000, 9))
+    if name == "sphere":  # a marching-cubes mesh: many equal midpoints
+        return m3.MarchingCubesSearch(workloads.sphere_solid(), 0.02, 8).triangles()
+    if name == "csg":
+        return m3.MarchingCubesSearch(workloads.csg64_solid(2), 0.01, 8).triangles()
+    if name == "tiny":
+        return np.random.default_rng(42).normal(size=(3, 9))
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", ["tiny", "random", "sphere", "csg"])
+def test_group_triangles_equals_oracle(name):
+    tris = _group_case(name).reshape(-1, 9)
+    print(f"group {name}: {len(tris)} triangles", flush=True)
+    got = m3.group_order(tris)
+    want = oracle_group_triangles(tris)
+    assert np.array_equal(got, want)
+
+
+def test_group_triangles_split_properties_and_grouped_stl(tmp_path):
+    mesh = m3.MarchingCubesSearch(solid_catalogue()["csg"], 0.04, 8)
+    print(f"grouped STL of {len(mesh.faces)} triangles", flush=True)
+    perm = m3.group_order(mesh)
+    check_grouping(mesh.triangles(), perm)
+    path = tmp_path / "grouped.stl"
+    mesh.SaveGroupedSTL(str(path))
+    assert path.read_bytes() == oracle_encode_stl(mesh.triangles()[perm.astype(np.int64)])
+    assert np.array_equal(m3.GroupTriangles(mesh.triangles()), mesh.triangles()[perm.astype(np.int64)])
+    assert len(m3.group_order(np.zeros((0, 9)))) == 0
+
+
+@pytest.mark.parametrize("device_only", [1, 0])
+def test_mesh_group_triangles_in_place(device_only):
+    """mc3d_mesh_group_triangles reorders a library mesh (device buffers, or
+    the pinned host arrays) into the order mc3d_group_triangles gives."""
+    from model3d_amd.solids import LibBackend
+
+    L = _lib.lib()
+    ctx = _lib.context(0)
+    b = LibBackend()
+    h = solid_catalogue()["torus"]._build(b)
+    o = _lib.mc3d_options()
+    o.precision, o.cull, o.device_only = FP64, 1, device_only
+    out = ctypes.c_void_p()
+    try:
+        _lib.check(L.mc3d_marching_cubes_search(ctx, h, 0.03, 4, ctypes.byref(o), ctypes.byref(out)), "search")
+        nv, nt = ctypes.c_size_t(), ctypes.c_size_t()
+        L.mc3d_mesh_counts(out, ctypes.byref(nv), ctypes.byref(nt))
+        v = np.empty((nv.value, 3))
+        f = np.empty((nt.value, 3), dtype=np.uint32)
+        _lib.check(L.mc3d_mesh_copy(out, v.ctypes.data, f.ctypes.data), "copy")
+        perm = m3.group_order(m3.Mesh(v, f))
+        _lib.check(L.mc3d_mesh_group_triangles(ctx, out), "group")
+        f2 = np.empty_like(f)
+        _lib.check(L.mc3d_mesh_copy(out, v.ctypes.data, f2.ctypes.data), "copy")
+        assert np.array_equal(f2, f[perm.astype(np.int64)])
+    finally:
+        if out.value:
+            L.mc3d_mesh_free(out)
+        b.release(h)
+
+
+def test_group_triangles_at_bench_size():
+    """The CSG workload's mesh at its bench size (about 812k triangles)."""
+    tris = m3.MarchingCubesSearch(workloads.csg64_solid(2), 0.005, 8).triangles().reshape(-1, 9)
+    print(f"group csg64 d=0.005: {len(tris)} triangles", flush=True)
+    assert np.array_equal(m3.group_order(tris), oracle_group_triangles(tris))
