@@ -145,6 +145,11 @@ typedef struct {
     int32_t no_graph;    /* 1: enqueue the stages one by one (per-stage timings in mc3d_stats); 0: a call
                           * left on the device runs its chain as one HIP graph launch, captured once per
                           * call shape and replayed (only ms_device is timed then) */
+    int32_t slab_planes; /* z-slab streaming of one solid's search: the grid runs through the pipeline
+                          * this many cube layers at a time (device memory for the grid bounded by one
+                          * slab; the slabs' meshes are stitched along the shared planes, so the mesh
+                          * is the whole grid's).  0: whole grid unless it passes 2^32 occupancy words
+                          * (then slabs of at most 2^30); < 0: never.  Not with a block filter. */
 } mc3d_options;
 
 typedef struct {
@@ -285,9 +290,9 @@ void mc3d_mesh_free(mc3d_mesh *m);
  * then per triangle the float32 cast of Triangle.Normal() computed in
  * float64 (primitives.go:27-33), the float32 casts of its three vertices and
  * two zero bytes; little-endian.  Triangles in the given order (the
- * reference writes Mesh.TriangleSlice's map order; SaveGroupedSTL's
- * order is mc3d_mesh_group_triangles / mc3d_group_triangles first).  The encoding runs on the
- * device.  out must hold mc3d_stl_bytes(T) bytes; more than 2^32 - 1
+ * reference writes Mesh.TriangleSlice's map order; for SaveGroupedSTL's,
+ * mc3d_mesh_group_triangles / mc3d_group_triangles first).  The encoding
+ * runs on the device.  out must hold mc3d_stl_bytes(T) bytes; more than 2^32 - 1
  * triangles is MC3D_E_UNSUPPORTED ("too many triangles for STL format"). */
 size_t mc3d_stl_bytes(size_t n_triangles);
 /* a mesh of this context's device, from its device buffers */

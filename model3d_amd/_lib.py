@@ -52,6 +52,7 @@ class mc3d_options(ctypes.Structure):
         ("rows_per_walk", ctypes.c_int32),
         ("waves_per_brick", ctypes.c_int32),
         ("no_graph", ctypes.c_int32),
+        ("slab_planes", ctypes.c_int32),
     ]
 
 

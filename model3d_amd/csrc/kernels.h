@@ -243,8 +243,8 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(const Seg *__restrict__ seg
                 uint64_t *zrow = bits_all + sg.word0 + word_index(g, bx, y0, z0);
                 for (int jz = 0; jz < jz_end; jz++, zrow += plane)
                     for (int jy = 0; jy < jy_end; jy++) zrow[(size_t)jy * W] = word;
-                if (t && (bx == 0 || x0 + nvalid == g.nx || y0 == 0 || y0 + jy_end == g.ny || z0 == 0 ||
-                          z0 + jz_end == g.nz))
+                if (t && (bx == 0 || x0 + nvalid == g.nx || y0 == 0 || y0 + jy_end == g.ny || z_face(g, z0) ||
+                          z_face(g, z0 + jz_end - 1)))
                     atomicOr(&ctl->inside, 1u), atomicOr(&sout[si].inside, 1u);
             }
             len = active ? bi.y : 0;
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256) void k_corners(const Seg *__restrict__ segs, u
         const uint64_t m = __ballot(inside);
         if (lane == 0) bits[gw] = m;
         // panic("solid is true outside of bounds"), mc.go:687-689
-        if (inside && (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1))
+        if (inside && (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z_face(g, z)))
             atomicOr(&ctl->inside, 1u), atomicOr(&sout[si].inside, 1u);
     }
     flush_stats<STATS>(gs, ctl->gstats, ctl->flops);
@@ -337,7 +337,7 @@ __device__ __forceinline__ uint64_t wave_rows(const Grid &g, const Bricks &B, in
     const bool ok = j < B.BY * B.BZ && y < g.ny && z < g.nz;
     if (ok)
         rows[lane] = RowInfo{g.ys[y], g.zs[z], (uint32_t)word_index(g, bx, y, z),
-                             (y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1) ? 1u : 0u};
+                             (y == 0 || y == g.ny - 1 || z_face(g, z)) ? 1u : 0u};
     wave_sync();
     return __ballot(ok);
 }
@@ -366,7 +366,7 @@ __device__ __forceinline__ void brick_rows(const Grid &g, const Bricks &B, uint6
     for (int jz = 0; jz < jz_end; jz++, zrow += plane) {
         const int z = z0 + jz;
         const double pz = zs[jz];
-        const bool zedge = z == 0 || z == g.nz - 1;
+        const bool zedge = z_face(g, z);
         for (int jy = ((wave - jz * B.BY) % WPB + WPB) % WPB; jy < jy_end; jy += WPB) {
             const int y = y0 + jy;
             const bool inside = eval(ys[jy], pz) && xin;
@@ -522,7 +522,7 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
             for (int q = 0; q < NP; q++)
                 if (ok[q])  // wave-uniform
                     row_out(bits + word_index(g, bx, yy[q], zz[q]), res[q] && xin, edge, xedge,
-                            yy[q] == 0 || yy[q] == g.ny - 1 || zz[q] == 0 || zz[q] == g.nz - 1, ctl, so);
+                            yy[q] == 0 || yy[q] == g.ny - 1 || z_face(g, zz[q]), ctl, so);
         }
     }
     flush_stats<STATS>(gs, ctl->gstats, ctl->flops);
@@ -588,6 +588,7 @@ struct Masks {
 // loaded from clamped addresses (no branches), so the loads issue together.
 struct Nbhd {
     uint64_t n[8];
+    bool zin;  // a plane above: the word's z edges and cubes exist
 };
 __device__ __forceinline__ Nbhd load_nbhd(const Grid &g, const uint64_t *__restrict__ bits, int wx, int y, int z) {
     const bool hx = wx + 1 < g.W, hy = y + 1 < g.ny, hz = z + 1 < g.nz;
@@ -597,18 +598,20 @@ __device__ __forceinline__ Nbhd load_nbhd(const Grid &g, const uint64_t *__restr
     for (int k = 0; k < 8; k++) r.n[k] = bits[w + ((k & 1) ? dx : 0) + ((k & 2) ? dy : 0) + ((k & 4) ? dz : 0)];
     for (int k = 0; k < 8; k++)
         if (((k & 1) && !hx) || ((k & 2) && !hy) || ((k & 4) && !hz)) r.n[k] = 0;
+    r.zin = hz;
     return r;
 }
 
 // Crossed-edge masks of word n[0]: bit i of cx is the edge from corner
 // (64wx+i, y, z) to its +x neighbour, and likewise for y and z.  Padding
-// bits (x >= nx) and boundary corners are zero, so no checks beyond the
-// neighbour words' existence are needed.
+// bits (x >= nx) and corners on the grid's faces are zero, so no checks
+// beyond the neighbour words' existence are needed -- except on the top
+// plane of a z slab, whose corners may be inside: it has no z edges.
 __device__ __forceinline__ Masks nbhd_masks(const Nbhd &h) {
     Masks m;
     m.cx = h.n[0] ^ ((h.n[0] >> 1) | (h.n[1] << 63));
     m.cy = h.n[0] ^ h.n[2];
-    m.cz = h.n[0] ^ h.n[4];
+    m.cz = h.zin ? h.n[0] ^ h.n[4] : 0;
     return m;
 }
 
@@ -637,10 +640,11 @@ __device__ __forceinline__ void keep_masks(const Grid &g, const uint64_t *__rest
 struct CubeCorners {
     uint64_t c[8];
 };
+// The top plane of the grid (of a z slab) has no cubes: no corners.
 __device__ __forceinline__ CubeCorners nbhd_corners(const Nbhd &h) {
     CubeCorners cc;
     for (int r = 0; r < 4; r++) {  // r = dy + 2 dz
-        const uint64_t b = h.n[2 * r], bn = h.n[2 * r + 1];
+        const uint64_t b = h.zin ? h.n[2 * r] : 0, bn = h.zin ? h.n[2 * r + 1] : 0;
         cc.c[2 * r] = b;
         cc.c[2 * r + 1] = (b >> 1) | (bn << 63);
     }
@@ -770,7 +774,7 @@ struct GroupWord {
     uint32_t y, z, x0;  // row, and the x of the word's first corner
     uint32_t si;        // segment
     uint32_t vsolid;    // the solid's first vertex (its triangles index from there)
-    uint32_t pad;
+    uint32_t zin;       // a plane above the word's: it has cubes
     uint64_t keep;      // block filter: the word's cubes of accepted blocks
 };
 
@@ -878,7 +882,8 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
                 const uint2 o = offs[(!MULTI && bm) ? bm_index(g_, seg0.B, cx, cy, cz) : idx];
                 if (in) {
                     r.bits = b;
-                    Masks m{b ^ ((b >> 1) | ((hx ? bn : 0) << 63)), b ^ (hy ? by : 0), b ^ (hz ? bz : 0)};
+                    // (the top plane, also of a z slab: no z edges)
+                    Masks m{b ^ ((b >> 1) | ((hx ? bn : 0) << 63)), b ^ (hy ? by : 0), hz ? b ^ bz : 0};
                     if (!MULTI && keep_all) keep_masks(g_, keep_all + word0, ex, ey, ez, m);
                     r.cx = m.cx, r.cy = m.cy, r.cz = m.cz;
                     r.base = o.x;
@@ -887,7 +892,8 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
                 if (e == 0) {
                     const uint32_t vsolid = MULTI ? offs[0].x : 0u;
                     const uint64_t kw = (!MULTI && keep_all) ? keep_all[word0 + idx] : ~0ull;
-                    gwd[g] = GroupWord{(uint32_t)y, (uint32_t)z, (uint32_t)wx * 64u, si, vsolid, 0u, kw};
+                    gwd[g] = GroupWord{(uint32_t)y, (uint32_t)z, (uint32_t)wx * 64u, si, vsolid,
+                                       z + 1 < g_.nz ? 1u : 0u, kw};
                 }
             }
             wi[g][e] = r;
@@ -935,6 +941,7 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
             if (lane < ng) {
                 Nbhd h;
                 for (int k = 0; k < 8; k++) h.n[k] = wi[lane][k].bits;
+                h.zin = gwd[lane].zin != 0;
                 const uint64_t m = mixed_cubes(nbhd_corners(h)) & gwd[lane].keep;
                 mixed[lane] = m;
                 c = (uint32_t)__popcll(m);

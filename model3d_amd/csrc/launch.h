@@ -164,6 +164,29 @@ hipError_t launch_grp_scatter(const GroupLevel &G, int a, const uint32_t *scan, 
 // out[k] = the three indices of triangle perm[k]
 hipError_t launch_grp_apply(const uint32_t *tris, const uint32_t *perm, uint32_t n, uint32_t *out, hipStream_t st);
 
+// z-slab stitching (k_slab.hip).  The x and y edges of plane z among a
+// slab's vertices (keys vkeys): keys without z and vertex indices, in
+// vertex order, *count of them (flags, scan: nv entries of scratch; tmp:
+// grp_scan's temporary storage).
+hipError_t launch_plane_list(const uint64_t *vkeys, uint32_t nv, uint32_t z, uint32_t *flags, uint32_t *scan,
+                             void *tmp, size_t tmp_bytes, uint64_t *keys, uint32_t *idx, uint32_t *count,
+                             hipStream_t st);
+// keep[i] = 1, then 0 at the ndrop indices of drop
+hipError_t launch_stitch_keep(uint32_t *keep, uint32_t nv, const uint32_t *drop, uint32_t ndrop, hipStream_t st);
+// kept vertex i (rank: exclusive sums of keep) to fv[vbase + rank[i]] (and
+// its interior point to fin); newidx[i] = its index in the mesh
+hipError_t launch_stitch_verts(const double *v, const double *in, uint32_t nv, const uint32_t *keep,
+                               const uint32_t *rank, uint32_t vbase, double *fv, double *fin, uint32_t *newidx,
+                               hipStream_t st);
+// newidx[up_idx[j]] = lo_global[j] (both lists sorted by key; a key
+// mismatch sets *bad)
+hipError_t launch_stitch_pairs(const uint64_t *lo_keys, const uint32_t *lo_global, const uint64_t *up_keys,
+                               const uint32_t *up_idx, uint32_t m, uint32_t *newidx, uint32_t *bad, hipStream_t st);
+hipError_t launch_stitch_tris(const uint32_t *t, uint64_t nt, const uint32_t *newidx, uint32_t *ft, hipStream_t st);
+// global[j] = newidx[idx[j]]
+hipError_t launch_stitch_globals(const uint32_t *idx, uint32_t m, const uint32_t *newidx, uint32_t *global,
+                                 hipStream_t st);
+
 inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 20) {
     uint64_t b = (n + block - 1) / block;
     if (b == 0) b = 1;

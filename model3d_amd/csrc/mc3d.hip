@@ -265,6 +265,8 @@ struct mc3d_ctx {
     // scans, node axes and slots, hipCUB's temporary storage
     DevBuf g_box, g_keys[4], g_perm[4], g_flag, g_f, g_scan, g_axis, g_slots, g_tmp, g_tris;
     DevBuf keep;                         // block filter: cube mask
+    // z slabs: a plane's vertex list and the stitch's per-vertex arrays
+    DevBuf s_flags, s_scan, s_keys, s_idx, s_count, s_keep, s_rank, s_newidx, s_bad;
     // device block filter: triangles, their grid, the recursion's block lists
     DevBuf rf_tris, rf_counts, rf_start, rf_cursor, rf_cells, rf_tiles, rf_front[2], rf_levels, rf_vsat;
     HostBuf rf_total;
@@ -351,7 +353,38 @@ std::string fmt_bounds(const double *mn, const double *mx) {
 // validateBounds (mc.go:15, bounder.go:48-52), the spacer (mc.go:594-608),
 // the lowering, and the culling bricks: 64 x B x B corners, B growing until
 // the fixed program slots fit.
-int prepare_job(Job &j, const mc3d_options &o) {
+// The occupancy grid of job j's axes (j.ax; a z slab keeps its open faces
+// in j.g) and its culling bricks.  With check, a grid past 2^32 occupancy
+// words is MC3D_E_UNSUPPORTED (single calls stream such grids in slabs).
+int layout_job(Job &j, const mc3d_options &o, bool check) {
+    Grid &g = j.g;
+    g.nx = (int)j.ax[0].size(), g.ny = (int)j.ax[1].size(), g.nz = (int)j.ax[2].size();
+    g.W = (g.nx + 63) / 64;
+    j.nwords = (uint64_t)g.W * g.ny * g.nz;
+    if (check && j.nwords >= (1ull << 32)) {
+        set_error("grid too large: more than 2^32 occupancy words");
+        return MC3D_E_UNSUPPORTED;
+    }
+    j.B = Bricks{nullptr, nullptr, 1, 1, 1, 1};
+    j.cap = 0, j.nbricks = 0;
+    // A host-evaluated solid may be true outside its bounds, so nothing is
+    // folded by bounds: no culling.
+    if (o.cull && !j.host) {
+        int bsz = o.brick > 0 ? o.brick : 4;
+        j.cap = std::min<uint32_t>((uint32_t)j.P.code.size(), 64u);
+        for (;;) {
+            j.B.BY = j.B.BZ = bsz;
+            j.B.nby = (g.ny + bsz - 1) / bsz;
+            j.B.nbz = (g.nz + bsz - 1) / bsz;
+            j.nbricks = (uint64_t)g.W * j.B.nby * j.B.nbz;
+            if (j.nbricks * j.cap < (1ull << 27) || bsz >= 64) break;
+            bsz *= 2;
+        }
+    }
+    return MC3D_OK;
+}
+
+int prepare_job(Job &j, const mc3d_options &o, bool check = true) {
     double mn[3], mx[3];
     solid_bounds(*j.s->ptr, mn, mx);
     if (!bounds_valid(*j.s->ptr)) {
@@ -381,30 +414,7 @@ int prepare_job(Job &j, const mc3d_options &o) {
     }
     j.host = !j.P.host_atoms.empty();
     j.frames = host_frames(j.P);
-    Grid &g = j.g;
-    g.nx = (int)j.ax[0].size(), g.ny = (int)j.ax[1].size(), g.nz = (int)j.ax[2].size();
-    g.W = (g.nx + 63) / 64;
-    j.nwords = (uint64_t)g.W * g.ny * g.nz;
-    if (j.nwords >= (1ull << 32)) {
-        set_error("grid too large: more than 2^32 occupancy words");
-        return MC3D_E_UNSUPPORTED;
-    }
-    j.B = Bricks{nullptr, nullptr, 1, 1, 1, 1};
-    // A host-evaluated solid may be true outside its bounds, so nothing is
-    // folded by bounds: no culling.
-    if (o.cull && !j.host) {
-        int bsz = o.brick > 0 ? o.brick : 4;
-        j.cap = std::min<uint32_t>((uint32_t)j.P.code.size(), 64u);
-        for (;;) {
-            j.B.BY = j.B.BZ = bsz;
-            j.B.nby = (g.ny + bsz - 1) / bsz;
-            j.B.nbz = (g.nz + bsz - 1) / bsz;
-            j.nbricks = (uint64_t)g.W * j.B.nby * j.B.nbz;
-            if (j.nbricks * j.cap < (1ull << 27) || bsz >= 64) break;
-            bsz *= 2;
-        }
-    }
-    return MC3D_OK;
+    return layout_job(j, o, check);
 }
 
 // The device copy of a collider's ray grid and, for sphere queries of
@@ -1606,6 +1616,206 @@ namespace {
 // (MarchingCubesConj).
 using RectFilter = std::function<bool(const double *, const double *)>;
 
+// The x and y edges of slab plane z among the last call's nv vertices
+// (ctx->vkeys): their keys without z and the slab's vertex indices, sorted
+// by key; `global` is left for their indices in the stitched mesh.
+struct PlaneList {
+    std::unique_ptr<DevBuf> keys{new DevBuf}, idx{new DevBuf}, global{new DevBuf};
+    uint32_t n = 0;
+};
+
+int plane_list(mc3d_ctx *ctx, uint32_t nv, uint32_t z, PlaneList &out) {
+    hipStream_t st = ctx->stream;
+    int rc;
+    size_t scan_bytes = 0, sort_bytes = 0;
+    HIP_TRY(grp_scan(nullptr, scan_bytes, nullptr, nullptr, std::max<uint32_t>(nv, 1), st));
+    HIP_TRY(grp_sort(nullptr, sort_bytes, nullptr, nullptr, nullptr, nullptr, std::max<uint32_t>(nv, 1), st));
+    if ((rc = ctx->s_flags.ensure(nv * 4 + 16)) || (rc = ctx->s_scan.ensure(nv * 4 + 16)) ||
+        (rc = ctx->s_keys.ensure(nv * 8 + 16)) || (rc = ctx->s_idx.ensure(nv * 4 + 16)) ||
+        (rc = ctx->s_count.ensure(16)) || (rc = ctx->rf_total.ensure(4)) ||
+        (rc = ctx->g_tmp.ensure(std::max(scan_bytes, sort_bytes) + 16)))
+        return rc;
+    HIP_TRY(launch_plane_list(ctx->vkeys.as<uint64_t>(), nv, z, ctx->s_flags.as<uint32_t>(),
+                              ctx->s_scan.as<uint32_t>(), ctx->g_tmp.p, ctx->g_tmp.cap, ctx->s_keys.as<uint64_t>(),
+                              ctx->s_idx.as<uint32_t>(), ctx->s_count.as<uint32_t>(), st));
+    HIP_TRY(hipMemcpyAsync(ctx->rf_total.p, ctx->s_count.p, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    const uint32_t m = *ctx->rf_total.as<uint32_t>();
+    if ((rc = out.keys->ensure((size_t)m * 8 + 16)) || (rc = out.idx->ensure((size_t)m * 4 + 16)) ||
+        (rc = out.global->ensure((size_t)m * 4 + 16)))
+        return rc;
+    if (m) {
+        size_t bytes = ctx->g_tmp.cap;
+        HIP_TRY(grp_sort(ctx->g_tmp.p, bytes, ctx->s_keys.as<uint64_t>(), out.keys->as<uint64_t>(),
+                         ctx->s_idx.as<uint32_t>(), out.idx->as<uint32_t>(), m, st));
+    }
+    out.n = m;
+    return MC3D_OK;
+}
+
+// MarchingCubesSearch of one solid in z slabs of `planes` cube layers
+// (mc.go:614-645 keeps only a few planes alive; here device memory for the
+// grid is bounded by one slab): each slab runs the pipeline on corner planes
+// [c0, c1] with its shared planes open (no boundary check there), its mesh
+// left on the device; then the slabs' meshes are stitched into one, the
+// upper copy of each shared plane's x and y edge vertices dropped for the
+// lower one.
+int search_slabs(mc3d_ctx *ctx, Job &base, const mc3d_options &o, int planes, mc3d_mesh **out,
+                 std::chrono::steady_clock::time_point t_call, double ms_prepare) {
+    const std::vector<double> zs = base.ax[2];
+    const int ncube = (int)zs.size() - 1;
+    mc3d_options so = o;
+    so.device_only = 1;
+    struct Piece {
+        std::unique_ptr<mc3d_mesh, void (*)(mc3d_mesh *)> m{nullptr, mc3d_mesh_free};
+        PlaneList top, bot;
+    };
+    std::vector<Piece> pieces;
+    pieces.reserve((size_t)(ncube + planes - 1) / planes);
+    int rc;
+    for (int c0 = 0; c0 < ncube; c0 += planes) {
+        const int c1 = std::min(ncube, c0 + planes);
+        std::vector<Job> jobs(1);
+        Job &j = jobs[0];
+        j.s = base.s, j.delta = base.delta, j.iters = base.iters, j.P = base.P;
+        j.ax[0] = base.ax[0], j.ax[1] = base.ax[1];
+        j.ax[2].assign(zs.begin() + c0, zs.begin() + c1 + 1);
+        j.host = base.host, j.frames = base.frames, j.interior = base.interior, j.conj = base.conj;
+        j.g = Grid{};
+        j.g.zlo_open = c0 > 0, j.g.zhi_open = c1 < ncube;
+        if ((rc = layout_job(j, so, true))) return rc;
+        std::vector<mc3d_mesh *> ms;
+        if ((rc = run_call(ctx, jobs, so, ms, t_call, ms_prepare))) return rc;
+        pieces.emplace_back();
+        Piece &pc = pieces.back();
+        pc.m.reset(ms[0]);
+        if (pc.m->nv >= (1ull << 32)) {
+            set_error("mesh too large for 32-bit vertex indices");
+            return MC3D_E_UNSUPPORTED;
+        }
+        const uint32_t nv = (uint32_t)pc.m->nv;
+        if (c0 > 0 && (rc = plane_list(ctx, nv, 0, pc.bot))) return rc;
+        if (c1 < ncube && (rc = plane_list(ctx, nv, (uint32_t)(c1 - c0), pc.top))) return rc;
+    }
+    // the stitched mesh's sizes and buffers
+    uint64_t nv_all = 0, nt_all = 0, nv_max = 0;
+    for (size_t k = 0; k < pieces.size(); k++) {
+        if (k && pieces[k].bot.n != pieces[k - 1].top.n) {
+            set_error("internal error: z slabs disagree on a shared plane's vertices");
+            return MC3D_E_INTERNAL;
+        }
+        nv_all += pieces[k].m->nv - pieces[k].bot.n;
+        nt_all += pieces[k].m->nt;
+        nv_max = std::max<uint64_t>(nv_max, pieces[k].m->nv);
+    }
+    if (nv_all >= (1ull << 32) || nt_all >= (1ull << 32)) {
+        set_error("mesh too large for 32-bit vertex indices");
+        return MC3D_E_UNSUPPORTED;
+    }
+    const bool interior = base.interior;
+    std::shared_ptr<MeshBuffers> mb = std::make_shared<MeshBuffers>();
+    mb->pool = ctx->pool;
+    mb->hpool = ctx->hpool;
+    mb->verts = ctx->pool->take(nv_all * 24 + 16);
+    mb->tris = ctx->pool->take(nt_all * 12 + 16);
+    if ((rc = mb->verts->ensure(nv_all * 24 + 16)) || (rc = mb->tris->ensure(nt_all * 12 + 16))) return rc;
+    if (interior) {
+        mb->interior = ctx->pool->take(nv_all * 24 + 16);
+        if ((rc = mb->interior->ensure(nv_all * 24 + 16))) return rc;
+    }
+    size_t scan_bytes = 0;
+    hipStream_t st = ctx->stream;
+    HIP_TRY(grp_scan(nullptr, scan_bytes, nullptr, nullptr, (uint32_t)std::max<uint64_t>(nv_max, 1), st));
+    if ((rc = ctx->s_keep.ensure(nv_max * 4 + 16)) || (rc = ctx->s_rank.ensure(nv_max * 4 + 16)) ||
+        (rc = ctx->s_newidx.ensure(nv_max * 4 + 16)) || (rc = ctx->s_bad.ensure(16)) ||
+        (rc = ctx->g_tmp.ensure(scan_bytes + 16)) || (rc = ctx->rf_total.ensure(4)))
+        return rc;
+    HIP_TRY(hipMemsetAsync(ctx->s_bad.p, 0, 4, st));
+    uint32_t vbase = 0;
+    uint64_t tbase = 0;
+    uint32_t *keep = ctx->s_keep.as<uint32_t>(), *rank = ctx->s_rank.as<uint32_t>();
+    uint32_t *newidx = ctx->s_newidx.as<uint32_t>();
+    for (size_t k = 0; k < pieces.size(); k++) {
+        const mc3d_mesh &m = *pieces[k].m;
+        const uint32_t nv = (uint32_t)m.nv;
+        const PlaneList &bot = pieces[k].bot;
+        const double *v = m.buf->verts->as<double>() + 3 * m.voff;
+        const double *in = interior ? m.buf->interior->as<double>() + 3 * m.voff : nullptr;
+        const uint32_t *t = m.buf->tris->as<uint32_t>() + 3 * m.toff;
+        HIP_TRY(launch_stitch_keep(keep, nv, bot.idx->as<uint32_t>(), bot.n, st));
+        if (nv) {
+            size_t bytes = ctx->g_tmp.cap;
+            HIP_TRY(grp_scan(ctx->g_tmp.p, bytes, keep, rank, nv, st));
+        }
+        HIP_TRY(launch_stitch_verts(v, in, nv, keep, rank, vbase, mb->verts->as<double>(),
+                                    interior ? mb->interior->as<double>() : nullptr, newidx, st));
+        if (k)
+            HIP_TRY(launch_stitch_pairs(pieces[k - 1].top.keys->as<uint64_t>(), pieces[k - 1].top.global->as<uint32_t>(),
+                                        bot.keys->as<uint64_t>(), bot.idx->as<uint32_t>(), bot.n, newidx,
+                                        ctx->s_bad.as<uint32_t>(), st));
+        HIP_TRY(launch_stitch_tris(t, m.nt, newidx, mb->tris->as<uint32_t>() + 3 * tbase, st));
+        const PlaneList &top = pieces[k].top;
+        HIP_TRY(launch_stitch_globals(top.idx->as<uint32_t>(), top.n, newidx, top.global->as<uint32_t>(), st));
+        vbase += nv - bot.n;
+        tbase += m.nt;
+    }
+    HIP_TRY(hipMemcpyAsync(ctx->rf_total.p, ctx->s_bad.p, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipEventRecord(ctx->ev[EV_EMIT1], st));
+    HIP_TRY(hipEventRecord(ctx->ev[EV_OUT], st));
+    const bool to_host = !o.device_only;
+    if (to_host && (rc = enqueue_host_copy(ctx, *mb, nv_all, nt_all, interior))) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    if (*ctx->rf_total.as<uint32_t>()) {
+        if (to_host) (void)hipStreamSynchronize(ctx->copy);
+        set_error("internal error: z slabs disagree on a shared plane's edges");
+        return MC3D_E_INTERNAL;
+    }
+    double ms_d2h = 0;
+    if (to_host) {
+        const auto t0 = std::chrono::steady_clock::now();
+        HIP_TRY(hipStreamSynchronize(ctx->copy));
+        ms_d2h = ms_since(t0);
+    }
+    // one mesh: the slabs' statistics summed, the whole grid's sizes
+    std::unique_ptr<mc3d_mesh> res(new (std::nothrow) mc3d_mesh);
+    if (!res) return MC3D_E_OOM;
+    res->device = ctx->device;
+    res->buf = mb;
+    res->has_interior = interior;
+    res->on_host = to_host;
+    res->nv = nv_all, res->nt = nt_all;
+    mc3d_stats &st_all = res->stats;
+    st_all = pieces[0].m->stats;
+    for (size_t k = 1; k < pieces.size(); k++) {
+        const mc3d_stats &a = pieces[k].m->stats;
+        st_all.guard_evals += a.guard_evals, st_all.guard_rechecks += a.guard_rechecks;
+        st_all.guard_mismatches += a.guard_mismatches, st_all.guard_violations += a.guard_violations;
+        st_all.bricks_total += a.bricks_total, st_all.bricks_active += a.bricks_active;
+        st_all.corner_flops64 += a.corner_flops64, st_all.corner_flops32 += a.corner_flops32;
+        st_all.bisect_flops64 += a.bisect_flops64, st_all.bisect_flops32 += a.bisect_flops32;
+        st_all.brick_code_total += a.brick_code_total, st_all.bricks_full += a.bricks_full;
+        st_all.brick_code_max = std::max(st_all.brick_code_max, a.brick_code_max);
+        st_all.ms_device += a.ms_device, st_all.ms_plan += a.ms_plan, st_all.ms_corners += a.ms_corners;
+        st_all.ms_classify += a.ms_classify, st_all.ms_scan += a.ms_scan, st_all.ms_emit += a.ms_emit;
+        st_all.ms_bisect += a.ms_bisect, st_all.ms_enqueue += a.ms_enqueue, st_all.ms_wait += a.ms_wait;
+        st_all.ms_host_callbacks += a.ms_host_callbacks, st_all.words_listed += a.words_listed;
+    }
+    st_all.corners = (uint64_t)base.ax[0].size() * base.ax[1].size() * zs.size();
+    st_all.grid_nz = zs.size();
+    st_all.vertices = nv_all, st_all.triangles = nt_all;
+    st_all.bisection_evals = base.iters > 0 ? nv_all * (uint64_t)(base.iters + 1) : 0;
+    st_all.ms_d2h = ms_d2h;
+    st_all.ms_prepare = ms_prepare;
+    st_all.ms_call = ms_since(t_call);
+    if (to_host) {  // the device outputs go back to the pool now
+        ctx->pool->give(std::move(mb->verts));
+        ctx->pool->give(std::move(mb->tris));
+        ctx->pool->give(std::move(mb->interior));
+    }
+    *out = res.release();
+    return MC3D_OK;
+}
+
 int search_one(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters, const mc3d_options *opts,
                mc3d_mesh **out, bool interior, std::vector<double> conj, const RectFilter *filter = nullptr,
                int filter_threads = 1, const DeviceRectFilter *dfilter = nullptr) {
@@ -1617,8 +1827,21 @@ int search_one(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters, cons
     jobs[0].s = s, jobs[0].delta = delta, jobs[0].iters = iters;
     jobs[0].interior = interior;
     jobs[0].conj = std::move(conj);
-    int rc = prepare_job(jobs[0], o);
+    const bool filtered = filter || dfilter;
+    int rc = prepare_job(jobs[0], o, filtered || o.slab_planes < 0);
     if (rc) return rc;
+    // z slabs: asked for, or a grid past 2^32 occupancy words
+    const Job &j0 = jobs[0];
+    int planes = 0;
+    if (!filtered && o.slab_planes > 0) planes = o.slab_planes;
+    else if (!filtered && o.slab_planes == 0 && j0.nwords >= (1ull << 32))
+        planes = (int)std::max<uint64_t>(1, (1ull << 30) / ((uint64_t)j0.g.W * j0.g.ny) - 1);
+    if (planes > 0 && planes < j0.g.nz - 1)
+        return search_slabs(ctx, jobs[0], o, planes, out, t_call, ms_since(t_call));
+    if (j0.nwords >= (1ull << 32)) {
+        set_error("grid too large: more than 2^32 occupancy words");
+        return MC3D_E_UNSUPPORTED;
+    }
     if (filter && (rc = filter_keep_mask(jobs[0].ax, delta, *filter, filter_threads, jobs[0].keep))) return rc;
     jobs[0].dfilter = dfilter;
     const double ms_prepare = ms_since(t_call);

@@ -32,7 +32,16 @@ struct Grid {
     const double *__restrict__ ys;
     const double *__restrict__ zs;
     int nx, ny, nz, W;  // W = words per row = ceil(nx / 64)
+    // a z slab of a larger grid: its first / last plane is shared with the
+    // slab below / above, not a face of the grid
+    int zlo_open, zhi_open;
 };
+
+// Plane z is a face of the whole grid (where panic("solid is true outside
+// of bounds"), mc.go:687-689, is checked).
+MC3D_HD inline bool z_face(const Grid &g, int z) {
+    return (z == 0 && !g.zlo_open) || (z == g.nz - 1 && !g.zhi_open);
+}
 
 // Per-brick programs: info[b] = {offset into code (or BRICK_FULL), length};
 // length 0: the solid is false everywhere in the brick; BRICK_TRUE: true.

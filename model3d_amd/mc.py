@@ -97,8 +97,10 @@ def NewMeshRect(min, max) -> Mesh:
     return Mesh.from_triangles(np.array(tris))
 
 
-def _options(precision, cull, device_only=False, count_mismatch=False, brick=0, rows_per_walk=0, waves_per_brick=0):
+def _options(precision, cull, device_only=False, count_mismatch=False, brick=0, rows_per_walk=0, waves_per_brick=0,
+             slab_planes=0):
     o = _lib.mc3d_options()
+    o.slab_planes = slab_planes
     o.precision = precision
     o.cull = 1 if cull else 0
     o.device_only = 1 if device_only else 0
@@ -121,14 +123,16 @@ def MarchingCubesSearch(
     brick: int = 0,
     rows_per_walk: int = 0,
     waves_per_brick: int = 0,
+    slab_planes: int = 0,
 ) -> Mesh:
     """model3d.MarchingCubesSearch (mc.go:45-49) on the GPU.
 
     precision: MC3D_PREC_FP32_GUARD (default) or MC3D_PREC_FP64; cull: per-brick
     culling (brick = its y/z edge in corners, 0 = default; rows_per_walk and
     waves_per_brick shape the corner kernel); count_mismatch:
-    also gather float32-vs-float64 statistics into ``Mesh.stats``.  Every
-    setting yields the same triangles."""
+    also gather float32-vs-float64 statistics into ``Mesh.stats``;
+    slab_planes: stream the grid in z slabs of that many cube layers (0: only
+    past 2^32 occupancy words).  Every setting yields the same triangles."""
     L = _lib.lib()
     ctx = _lib.context(device)
     b = LibBackend()
@@ -136,7 +140,7 @@ def MarchingCubesSearch(
     out = ctypes.c_void_p()
     try:
         opts = _options(precision, cull, count_mismatch=count_mismatch, brick=brick, rows_per_walk=rows_per_walk,
-                        waves_per_brick=waves_per_brick)
+                        waves_per_brick=waves_per_brick, slab_planes=slab_planes)
         _raise_for(L.mc3d_marching_cubes_search(ctx, h, float(delta), int(iters), ctypes.byref(opts),
                                                 ctypes.byref(out)))
         return _fetch(L, out)
@@ -155,6 +159,7 @@ def MarchingCubesInterior(
     precision: int = _lib.MC3D_PREC_FP32_GUARD,
     cull: bool = True,
     device: int = 0,
+    slab_planes: int = 0,
 ):
     """model3d.MarchingCubesInterior (mc.go:55-60) on the GPU: the mesh of
     MarchingCubesSearch and, per vertex, a nearby point known to be inside
@@ -167,7 +172,7 @@ def MarchingCubesInterior(
     h = s._build(b)
     out = ctypes.c_void_p()
     try:
-        opts = _options(precision, cull)
+        opts = _options(precision, cull, slab_planes=slab_planes)
         _raise_for(L.mc3d_marching_cubes_interior(ctx, h, float(delta), int(iters), ctypes.byref(opts),
                                                   ctypes.byref(out)))
         mesh = _fetch(L, out)

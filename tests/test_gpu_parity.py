@@ -401,3 +401,52 @@ def test_collider_workload_at_bench_size():
     src, got_v = volume(solid.collider.triangles), volume(b.triangles())
     print(f"collider volume {got_v} against the source mesh's {src}", flush=True)
     assert abs(got_v - src) < 0.02 * abs(src), (got_v, src)
+
+
+# ---------------------------------------------------------------------------
+# z-slab streaming (mc.go:614-645 keeps a few planes alive; the library
+# bounds the grid's device memory by one slab and stitches the meshes)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["csg64", "sphere", "torus", "collider_inset", "host_superellipsoid"])
+@pytest.mark.parametrize("planes", [1, 5, 16])
+def test_z_slabs_equal_whole_grid(name, planes):
+    """The stitched mesh is the whole grid's: the same triangles bit for bit
+    and the same number of vertices (each shared plane's vertices once)."""
+    s = workloads.csg64_solid(2) if name == "csg64" else solid_catalogue()[name]
+    delta = 0.02 if name == "csg64" else 0.04
+    print(f"z slabs {name} planes={planes}", flush=True)
+    whole = m3.MarchingCubesSearch(s, delta, 8, precision=FP64)
+    got = m3.MarchingCubesSearch(s, delta, 8, precision=FP64, slab_planes=planes)
+    assert got.stats["corners"] == whole.stats["corners"]
+    assert len(got.vertices) == len(whole.vertices)
+    assert np.array_equal(canonical(got.triangles()), canonical(whole.triangles()))
+    assert len(np.unique(got.faces)) == len(got.vertices)  # every vertex used
+    g = m3.MarchingCubesSearch(s, delta, 8, precision=GUARD, slab_planes=planes)
+    assert np.array_equal(canonical(g.triangles()), canonical(whole.triangles()))
+
+
+def test_z_slabs_interior_and_panic():
+    s = solid_catalogue()["csg"]
+    mesh, interior = m3.MarchingCubesInterior(s, 0.04, 5)
+    smesh, sinterior = m3.MarchingCubesInterior(s, 0.04, 5, slab_planes=3)
+    pairs = lambda v, p: {tuple(r) for r in np.concatenate([v, p], axis=1).view(np.uint64).tolist()}
+    assert pairs(smesh.vertices, sinterior) == pairs(mesh.vertices, interior)
+    # a solid true on the grid's top face still panics; its inner slab
+    # planes are not faces
+    for planes in (0, 4):
+        with pytest.raises(m3.Panic, match="solid is true outside of bounds"):
+            m3.MarchingCubesSearch(_TrueAbove(), 0.05, 2, slab_planes=planes)
+
+
+class _TrueAbove(m3.Solid):
+    """True for z > 0.2, also on the top face of its grid (z = 0.55) and
+    on the inner planes of every slab above 0.2."""
+
+    def Min(self):
+        return (-0.5, -0.5, -0.5)
+
+    def Max(self):
+        return (0.5, 0.5, 0.5)
+
+    def ContainsBatch(self, pts):
+        return np.asarray(pts)[:, 2] > 0.2
