@@ -30,9 +30,9 @@ void corners_act(const Pipe &p, const Bufs &b, unsigned grid, hipStream_t st) {
 
 template <int MODE, bool STATS, bool MULTI>
 void corners_m(const Pipe &p, const Bufs &b, hipStream_t st) {
-    if (!p.cull) {
-        k_corners<MODE, STATS, MULTI><<<grid_for(p.nwords, 4), 256, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg,
-                                                                             b.bits, p.ctl, p.sout, p.nwords);
+    if (!p.cull) {  // about four words per wave (fewer wave launches for a cheap program)
+        k_corners<MODE, STATS, MULTI><<<grid_for(p.nwords, 16), 256, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg,
+                                                                              b.bits, p.ctl, p.sout, p.nwords);
         return;
     }
     // (the words of constant bricks were written by the plan)

@@ -35,11 +35,15 @@ __global__ __launch_bounds__(SUMS_BLOCK) void k_scan_tiles(uint4 *__restrict__ p
 
 // Downsweep over tiles of SCAN_TILE words (the exclusive scan of the
 // classify blocks' sums gives each tile's offset); words with .z set are also
-// listed at their scanned .z.
+// listed at their scanned .z.  FUSED (few classify blocks): the sums are not
+// scanned beforehand -- each tile adds up the sums before it itself, and the
+// last tile writes the totals.
+template <bool FUSED>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *__restrict__ in, uint64_t n,
                                                           const uint4 *__restrict__ part_sums,
                                                           uint2 *__restrict__ out, uint32_t *__restrict__ list,
-                                                          const Seg seg0, bool bm) {
+                                                          const Seg seg0, bool bm,
+                                                          unsigned long long *__restrict__ totals) {
     __shared__ uint4 sh[SCAN_BLOCK / 64];
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     uint4 vals[SCAN_ITEMS];
@@ -49,8 +53,22 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *__rest
         vals[i] = make_uint4(c & 0xffu, (c >> 8) & 0x1ffu, c >> 17, 0u);
         s = add4(s, vals[i]);
     }
+    uint4 off;
+    const uint32_t first_part = blockIdx.x * (SCAN_TILE / CLASSIFY_BLOCK);
+    if (FUSED) {
+        uint4 a = make_uint4(0, 0, 0, 0);
+        for (uint32_t i = threadIdx.x; i < first_part; i += SCAN_BLOCK) a = add4(a, part_sums[i]);
+        off = block_sum<SCAN_BLOCK>(a, sh);
+        __syncthreads();  // sh is reused by the scan below
+    } else {
+        off = part_sums[first_part];
+    }
     uint4 total;
-    uint4 ex = add4(block_exclusive(s, sh, total), part_sums[(uint64_t)blockIdx.x * (SCAN_TILE / CLASSIFY_BLOCK)]);
+    uint4 ex = add4(block_exclusive(s, sh, total), off);
+    if (FUSED && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        const uint4 all = add4(off, total);
+        totals[0] = all.x, totals[1] = all.y, totals[2] = all.z;
+    }
     for (int i = 0; i < SCAN_ITEMS; i++) {
         if (base + i < n) {
             out[base + i] = make_uint2(ex.x, ex.y);
@@ -185,8 +203,15 @@ hipError_t launch_classify(const Pipe &p, const Bufs &b, hipStream_t st) {
 hipError_t launch_scan(const Pipe &p, const Bufs &b, hipStream_t st) {
     // the scan of the counts whose per-block sums classify wrote
     const uint64_t ntiles = (p.nscan + SCAN_TILE - 1) / SCAN_TILE;
-    k_scan_tiles<<<1, SUMS_BLOCK, 0, st>>>(b.tiles, (p.nscan + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK, p.ctl->totals);
-    k_scan_down<<<(unsigned)ntiles, SCAN_BLOCK, 0, st>>>(b.counts, p.nscan, b.tiles, b.offs, b.actw, p.seg0, p.bm);
+    const uint64_t nparts = (p.nscan + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK;
+    if (nparts <= SCAN_FUSE_PARTS) {  // each tile sums the block sums before it
+        k_scan_down<true><<<(unsigned)ntiles, SCAN_BLOCK, 0, st>>>(b.counts, p.nscan, b.tiles, b.offs, b.actw, p.seg0,
+                                                                  p.bm, p.ctl->totals);
+    } else {
+        k_scan_tiles<<<1, SUMS_BLOCK, 0, st>>>(b.tiles, nparts, p.ctl->totals);
+        k_scan_down<false><<<(unsigned)ntiles, SCAN_BLOCK, 0, st>>>(b.counts, p.nscan, b.tiles, b.offs, b.actw,
+                                                                   p.seg0, p.bm, nullptr);
+    }
     if (p.nsegs > 1)  // a single solid starts at vertex 0 and triangle 0 (zeroed by the upload)
         k_seg_ranges<<<grid_for(p.nsegs, 256), 256, 0, st>>>(p.segs, p.nsegs, b.offs, p.sout);
     return hipGetLastError();

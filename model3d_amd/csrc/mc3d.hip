@@ -938,12 +938,13 @@ int take_outputs(mc3d_ctx *ctx, MeshBuffers &mb, const Call &c, uint64_t vcap, u
     return MC3D_OK;
 }
 
-// Meshes keep their outputs until freed: a buffer sized for earlier, larger
-// calls is traded for one that fits (the copy is ordered on the stream), and
-// goes back to the pool for the next call.
+// Meshes keep their outputs until freed: a buffer sized for much larger
+// earlier calls (over 4x the mesh and 64 MiB more) is traded for one that
+// fits (the copy is ordered on the stream) and goes back to the pool for the
+// next call; smaller slack is kept, as trading costs a copy per call.
 int fit_outputs(mc3d_ctx *ctx, MeshBuffers &mb, uint64_t nv, uint64_t nt) {
     auto fit = [&](std::unique_ptr<DevBuf> &b, size_t need) -> int {
-        if (!b || b->cap <= 2 * need + (4u << 20)) return MC3D_OK;
+        if (!b || b->cap <= 4 * need + (64u << 20)) return MC3D_OK;
         std::unique_ptr<DevBuf> f = ctx->pool->take(need);
         int rc = f->ensure(need);
         if (rc) return rc;
@@ -1090,8 +1091,6 @@ int graph_launch(mc3d_ctx *ctx, const Pipe &p, const Bufs &b, const Upload &U, d
     return MC3D_OK;
 }
 
-// Meshes `jobs` (prepared) in one pass of the pipeline; out[i] gets job i's
-// mesh.  A job with host-evaluated atoms is only ever run alone.
 // Blocks at each level of mcBlock.Pieces(64, ...) (mc.go:760-797) from a
 // root of l[0] x l[1] x l[2] cubes when every block is accepted: the sizes
 // the level lists need.  The halves of a split are floor and ceil of the
@@ -1221,6 +1220,8 @@ int enqueue_block_filter(mc3d_ctx *ctx, const Job &j, const Upload &U, uint64_t 
     return MC3D_OK;
 }
 
+// Meshes `jobs` (prepared) in one pass of the pipeline; out[i] gets job i's
+// mesh.  A job with host-evaluated atoms is only ever run alone.
 int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::vector<mc3d_mesh *> &out,
              std::chrono::steady_clock::time_point t_call, double ms_prepare) {
     const auto t_enqueue = std::chrono::steady_clock::now();

@@ -86,6 +86,7 @@ constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
 constexpr int CLASSIFY_BLOCK = 1024;  // words per classify block and per block sum
 constexpr int SUMS_BLOCK = 1024;      // threads of the one block scanning the block sums
+constexpr int SCAN_FUSE_PARTS = 4096; // up to this many block sums, each scan tile adds its prefix itself
 static_assert(SCAN_TILE % CLASSIFY_BLOCK == 0, "a scan tile starts at a classify block");
 
 }  // namespace mc3d
