@@ -25,6 +25,8 @@
 #include <array>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <functional>
 #include <exception>
 #include <map>
 #include <memory>
@@ -198,6 +200,74 @@ struct Pool {
 
 // memcpy of large host buffers split over up to 16 threads (the first touch
 // of freshly allocated pages dominates a single-threaded copy).
+// Host threads kept for the library's parallel host loops (lowering a
+// batch's solids, filling its upload): run(n, fn) calls fn(i) for every
+// i < n on up to 16 threads, the caller's included, and returns when all
+// are done.  Runs from several threads take turns.
+class HostPool {
+  public:
+    static HostPool &get() {
+        static HostPool pool;
+        return pool;
+    }
+    void run(size_t n, const std::function<void(size_t)> &fn) {
+        if (n == 0) return;
+        if (n == 1 || workers_.empty()) {
+            for (size_t i = 0; i < n; i++) fn(i);
+            return;
+        }
+        std::lock_guard<std::mutex> one(run_mu_);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            fn_ = &fn, n_ = n, next_ = 0, busy_ = workers_.size(), gen_++;
+        }
+        cv_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return busy_ == 0; });
+        fn_ = nullptr;
+    }
+
+  private:
+    HostPool() {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        for (unsigned t = 1; t < std::min(16u, hw); t++) workers_.emplace_back([this] { loop(); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (std::thread &t : workers_) t.join();
+    }
+    void drain() {
+        for (size_t i; (i = next_.fetch_add(1)) < n_;) (*fn_)(i);
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            drain();
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--busy_ == 0) done_cv_.notify_all();
+        }
+    }
+    std::mutex run_mu_, mu_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(size_t)> *fn_ = nullptr;
+    size_t n_ = 0, busy_ = 0;
+    std::atomic<size_t> next_{0};
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+    std::vector<std::thread> workers_;
+};
+
 void parallel_copy(void *dst, const void *src, size_t bytes) {
     constexpr size_t CHUNK = 4u << 20;
     const size_t nt = std::min<size_t>(16, (bytes + CHUNK - 1) / CHUNK);
@@ -206,13 +276,10 @@ void parallel_copy(void *dst, const void *src, size_t bytes) {
         return;
     }
     const size_t per = (bytes + nt - 1) / nt;
-    std::vector<std::thread> th;
-    for (size_t i = 1; i < nt; i++) {
+    HostPool::get().run(nt, [=](size_t i) {
         const size_t a = i * per, b = std::min(bytes, a + per);
-        if (a < b) th.emplace_back([=] { memcpy((char *)dst + a, (const char *)src + a, b - a); });
-    }
-    memcpy(dst, src, std::min(bytes, per));
-    for (std::thread &t : th) t.join();
+        if (a < b) memcpy((char *)dst + a, (const char *)src + a, b - a);
+    });
 }
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -329,6 +396,7 @@ struct Job {
     std::vector<double> conj;      // MarchingCubesConj: inverse chain applied to the vertices
     std::vector<uint64_t> keep;    // MarchingCubesFilter: cubes of accepted blocks (occupancy layout)
     const DeviceRectFilter *dfilter = nullptr;  // or the filter that makes that mask on the device
+    std::vector<PlanNode> nodes;  // the brick plan's node records of P (plan_nodes)
 };
 
 // Frame of each host-evaluated atom, by host atom index.
@@ -353,6 +421,8 @@ std::string fmt_bounds(const double *mn, const double *mx) {
 // validateBounds (mc.go:15, bounder.go:48-52), the spacer (mc.go:594-608),
 // the lowering, and the culling bricks: 64 x B x B corners, B growing until
 // the fixed program slots fit.
+void job_nodes(const Program &P, std::vector<PlanNode> &nodes);
+
 // The occupancy grid of job j's axes (j.ax; a z slab keeps its open faces
 // in j.g) and its culling bricks.  With check, a grid past 2^32 occupancy
 // words is MC3D_E_UNSUPPORTED (single calls stream such grids in slabs).
@@ -414,6 +484,7 @@ int prepare_job(Job &j, const mc3d_options &o, bool check = true) {
     }
     j.host = !j.P.host_atoms.empty();
     j.frames = host_frames(j.P);
+    job_nodes(j.P, j.nodes);
     return layout_job(j, o, check);
 }
 
@@ -570,6 +641,13 @@ struct Upload {
 // The blob's layout: the control block, the per-solid results, the
 // segments, the segment tables and every job's program and grid axes.
 // `cols` holds each job's colliders as the device sees them.
+// plan_nodes of a program, its atoms' boxes placed by pc.
+void job_nodes(const Program &P, std::vector<PlanNode> &nodes) {
+    std::vector<AtomBox> boxes(P.code.size());
+    for (size_t i = 0; i < P.atom_pc.size(); i++) boxes[P.atom_pc[i]] = P.atom_box[i];
+    plan_nodes(P, boxes, nodes);
+}
+
 void layout_upload(const std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_all, Upload &U) {
     const size_t n = jobs.size();
     size_t off = 0;
@@ -591,16 +669,15 @@ void layout_upload(const std::vector<Job> &jobs, uint64_t nwords_all, uint64_t n
     U.nodes.resize(n);
     for (size_t s = 0; s < n; s++) {
         const Program &P = jobs[s].P;
-        std::vector<AtomBox> boxes(P.code.size());
-        for (size_t i = 0; i < P.atom_pc.size(); i++) boxes[P.atom_pc[i]] = P.atom_box[i];
-        plan_nodes(P, boxes, U.nodes[s]);
+        if (jobs[s].nodes.size() != P.code.size()) job_nodes(P, U.nodes[s]);
+        const std::vector<PlanNode> &nodes = U.nodes[s].empty() ? jobs[s].nodes : U.nodes[s];
         Upload::Off &po = U.po[s];
         po.code = place(P.code.size() * sizeof(Insn));
         // params padded so that staging a brick's atoms never reads past the array
         po.f64 = place((P.f64.size() + MAX_ATOM_PARAMS) * sizeof(double));
         po.f32 = place((P.f32.size() + MAX_ATOM_PARAMS) * sizeof(float));
         po.fr = place(P.frames.size() * sizeof(Frame));
-        po.nodes = place(U.nodes[s].size() * sizeof(PlanNode));
+        po.nodes = place(nodes.size() * sizeof(PlanNode));
         po.col = place(P.colliders.size() * sizeof(DevCollider));
         po.ax = place((jobs[s].ax[0].size() + jobs[s].ax[1].size() + jobs[s].ax[2].size()) * sizeof(double));
     }
@@ -615,7 +692,8 @@ void fill_upload(std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_a
     const size_t n = jobs.size();
     memset(h, 0, U.o_segs);  // control block and per-solid results start at zero
     U.hsegs.assign(n, Seg{});
-    for (size_t s = 0; s < n; s++) {
+    // the jobs' regions are disjoint: many jobs are filled in parallel
+    auto fill = [&](size_t s) {
         Job &j = jobs[s];
         const Program &P = j.P;
         const Upload::Off &po = U.po[s];
@@ -627,7 +705,8 @@ void fill_upload(std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_a
             memcpy(h + po.f32, P.f32.data(), P.f32.size() * sizeof(float));
         }
         memcpy(h + po.fr, P.frames.data(), P.frames.size() * sizeof(Frame));
-        memcpy(h + po.nodes, U.nodes[s].data(), U.nodes[s].size() * sizeof(PlanNode));
+        const std::vector<PlanNode> &nodes = U.nodes[s].empty() ? j.nodes : U.nodes[s];
+        memcpy(h + po.nodes, nodes.data(), nodes.size() * sizeof(PlanNode));
         if (!U.cols[s].empty()) memcpy(h + po.col, U.cols[s].data(), U.cols[s].size() * sizeof(DevCollider));
         double *d = (double *)(h + po.ax);
         for (int a = 0; a < 3; a++) d = std::copy(j.ax[a].begin(), j.ax[a].end(), d);
@@ -653,7 +732,10 @@ void fill_upload(std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_a
         sg.brick0 = j.brick0, sg.nbricks = j.nbricks;
         sg.iters = j.iters;
         sg.stage = j.cap && plan_stage_bytes(P) <= PLAN_STAGE_LIMIT ? 1 : 0;
-    }
+    };
+    if (n >= 16) HostPool::get().run(n, fill);
+    else
+        for (size_t s = 0; s < n; s++) fill(s);
     memcpy(h + U.o_segs, U.hsegs.data(), n * sizeof(Seg));
     if (!jobs[0].conj.empty()) {
         memcpy(h + U.o_conj, jobs[0].conj.data(), jobs[0].conj.size() * sizeof(double));
@@ -1678,7 +1760,7 @@ int search_slabs(mc3d_ctx *ctx, Job &base, const mc3d_options &o, int planes, mc
         const int c1 = std::min(ncube, c0 + planes);
         std::vector<Job> jobs(1);
         Job &j = jobs[0];
-        j.s = base.s, j.delta = base.delta, j.iters = base.iters, j.P = base.P;
+        j.s = base.s, j.delta = base.delta, j.iters = base.iters, j.P = base.P, j.nodes = base.nodes;
         j.ax[0] = base.ax[0], j.ax[1] = base.ax[1];
         j.ax[2].assign(zs.begin() + c0, zs.begin() + c1 + 1);
         j.host = base.host, j.frames = base.frames, j.interior = base.interior, j.conj = base.conj;
@@ -2027,20 +2109,11 @@ int mc3d_marching_cubes_search_batch(mc3d_ctx *ctx, const mc3d_solid *const *sol
         std::vector<Job> all(n);
         std::vector<int> rcs(n, MC3D_OK);
         std::vector<std::string> errs(n);
-        std::atomic<size_t> next{0};
-        auto work = [&]() {
-            for (size_t i; (i = next.fetch_add(1)) < n;) {
-                all[i].s = solids[i], all[i].delta = deltas[i], all[i].iters = iters[i];
-                rcs[i] = guarded([&]() { return prepare_job(all[i], o); });
-                if (rcs[i]) errs[i] = mc3d_last_error();
-            }
-        };
-        const size_t nthreads =
-            std::min<size_t>({16, std::max(1u, std::thread::hardware_concurrency()), (n + 7) / 8});
-        std::vector<std::thread> pool;
-        for (size_t t = 1; t < nthreads; t++) pool.emplace_back(work);
-        work();
-        for (std::thread &t : pool) t.join();
+        HostPool::get().run(n, [&](size_t i) {
+            all[i].s = solids[i], all[i].delta = deltas[i], all[i].iters = iters[i];
+            rcs[i] = guarded([&]() { return prepare_job(all[i], o); });
+            if (rcs[i]) errs[i] = mc3d_last_error();
+        });
         // Solids with host-evaluated atoms need their own callback rounds and run
         // alone; all others share one pass of the pipeline.
         std::vector<Job> batched, alone;
