@@ -983,23 +983,29 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
             if (t0 + tw > tcap) {
                 over = true;
             } else {
-                // corner jj belongs to triangle jj / 3 of the cube whose prefix covers it
-                for (uint32_t jj = lane; jj < 3 * tw; jj += 64) {
-                    const uint32_t ti = jj / 3, ec = jj - 3 * ti;
+                // triangle ti belongs to the cube whose prefix covers it (a
+                // lane per triangle: one search for its three corners)
+                for (uint32_t ti = lane; ti < tw; ti += 64) {
                     const int L = owner(s_pre[wave], 64, ti);
                     const int c = s_cs[wave][L], gi = s_cube[wave][L], gg = gi >> 6, ii = gi & 63;
-                    const int kk = 3 * (int)(ti - s_pre[wave][L]) + (int)ec;
-                    const int e = tab[16 * c + 1 + kk];
-                    const int axis = e & 3, ddx = (e >> 2) & 1, ddy = (e >> 3) & 1, ddz = (e >> 4) & 1;
-                    const int ex = ii + ddx;  // x within the word (64: the next word's first)
-                    const WordInfo &o = wi[gg][(ex >> 6) | (ddy << 1) | (ddz << 2)];
-                    const int bb = ex & 63;
-                    const uint64_t bl = bb ? (~0ull >> (64 - bb)) : 0ull;
-                    // vertices of the word before this edge: its x crossings, then y, then z
-                    const uint64_t m = axis == 0 ? o.cx : (axis == 1 ? o.cy : o.cz);
-                    const uint32_t idx = o.base + (axis > 0 ? (uint32_t)__popcll(o.cx) : 0u) +
-                                         (axis > 1 ? (uint32_t)__popcll(o.cy) : 0u) + (uint32_t)__popcll(m & bl);
-                    tris[3 * t0 + jj] = idx - gwd[gg].vsolid;
+                    const int k0 = 3 * (int)(ti - s_pre[wave][L]);
+                    const uint32_t vsolid = gwd[gg].vsolid;
+                    uint32_t corner[3];
+                    for (int ec = 0; ec < 3; ec++) {
+                        const int e = tab[16 * c + 1 + k0 + ec];
+                        const int axis = e & 3, ddx = (e >> 2) & 1, ddy = (e >> 3) & 1, ddz = (e >> 4) & 1;
+                        const int ex = ii + ddx;  // x within the word (64: the next word's first)
+                        const WordInfo &o = wi[gg][(ex >> 6) | (ddy << 1) | (ddz << 2)];
+                        const int bb = ex & 63;
+                        const uint64_t bl = bb ? (~0ull >> (64 - bb)) : 0ull;
+                        // vertices of the word before this edge: its x crossings, then y, then z
+                        const uint64_t m = axis == 0 ? o.cx : (axis == 1 ? o.cy : o.cz);
+                        const uint32_t idx = o.base + (axis > 0 ? (uint32_t)__popcll(o.cx) : 0u) +
+                                             (axis > 1 ? (uint32_t)__popcll(o.cy) : 0u) + (uint32_t)__popcll(m & bl);
+                        corner[ec] = idx - vsolid;
+                    }
+                    uint32_t *dst = tris + 3 * (t0 + ti);
+                    dst[0] = corner[0], dst[1] = corner[1], dst[2] = corner[2];
                 }
             }
             tdone += tw;
