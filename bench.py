@@ -135,6 +135,7 @@ def run_gpu(work, run, steps, warmup, count_mismatch=False, device_only=True, ea
     opts.device_only = 1 if device_only else 0
     opts.count_mismatch = 1 if count_mismatch else 0
     opts.no_graph = 1 if no_graph else 0
+    opts.slab_planes = run.get("slab_planes", 0)
     bufs = {}
 
     def fetch(m):
@@ -496,6 +497,14 @@ def config_entry(name, steps=5, warmup=2):
         "dominant": {x: kernels[dom][x] for x in ("kernel", "avg_ms", "bound", "achieved", "unit", "frac")},
         **run,
     }
+    if name == "large":
+        # the same search streamed in z slabs of 125 cube layers (the grid's
+        # device memory bounded by one slab; the meshes stitched)
+        planes = 125
+        wall3, st3 = run_gpu(work, dict(run, slab_planes=planes), max(2, steps // 2), 1)
+        assert st3[0]["triangles"] == s0["triangles"] and st3[0]["vertices"] == s0["vertices"]
+        e["z_slabs"] = {"slab_planes": planes, "slabs": -(-(s0["grid_nz"] - 1) // planes),
+                        "ms_per_step": 1e3 * wall3 / max(2, steps // 2)}
     return e
 
 

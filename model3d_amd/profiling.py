@@ -7,6 +7,12 @@ HBM traffic from the FETCH_SIZE / WRITE_SIZE counters.
         (the rocpd `*_results.db`, or `*_kernel_trace.csv` with
         `--output-format csv`).
 
+    python -m model3d_amd.profiling sq DIR [--title T] [--command C]
+        Markdown table per kernel of the SQ counters of a `rocprofv3 --pmc
+        SQ_...` run (per dispatch, averaged over dispatches), with per-wave
+        figures: VALU / SALU / LDS instructions per wave, wave cycles per
+        wave, and the share of wave cycles spent waiting (SQ_WAIT_INST_ANY).
+
     python -m model3d_amd.profiling pmc FETCH_DIR WRITE_DIR --copy-bytes N [...]
         JSON: for every kernel the average FETCH_SIZE and WRITE_SIZE per
         dispatch (KiB, as the counters report them), and the same counters of
@@ -135,6 +141,38 @@ def pmc_summary(fetch_dir: str, write_dir: str, copy_bytes: int) -> dict:
     }
 
 
+SQ_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_SMEM", "SQ_WAVE_CYCLES",
+               "SQ_BUSY_CYCLES", "SQ_WAIT_INST_ANY")
+
+
+def sq_markdown(path: str, title: str = "", command: str = "") -> str:
+    per = {c: counter_values(path, c) for c in SQ_COUNTERS}
+    kernels = sorted({k for c in per.values() for k in c}, key=lambda k: -sum(per["SQ_WAVE_CYCLES"].get(k, {}).values()))
+    lines = []
+    if title:
+        lines += [f"# {title}", ""]
+    if command:
+        lines += ["Command:", "", "```", command, "```", ""]
+    lines += ["Per dispatch, averaged over the kernel's dispatches.  Per-wave columns divide by SQ_WAVES; "
+              "wait % is SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES.", "",
+              "| kernel | dispatches | " + " | ".join(c[3:] for c in SQ_COUNTERS) +
+              " | VALU/wave | SALU/wave | LDS/wave | cycles/wave | wait % |",
+              "|---|---:|" + "---:|" * (len(SQ_COUNTERS) + 5)]
+    for k in kernels:
+        avg = {}
+        n = 0
+        for c in SQ_COUNTERS:
+            d = per[c].get(k, {})
+            n = max(n, len(d))
+            avg[c] = sum(d.values()) / len(d) if d else 0.0
+        w = avg["SQ_WAVES"] or 1.0
+        wait = 100.0 * avg["SQ_WAIT_INST_ANY"] / avg["SQ_WAVE_CYCLES"] if avg["SQ_WAVE_CYCLES"] else 0.0
+        lines.append(f"| `{k}` | {n} | " + " | ".join(f"{avg[c]:.0f}" for c in SQ_COUNTERS) +
+                     f" | {avg['SQ_INSTS_VALU'] / w:.0f} | {avg['SQ_INSTS_SALU'] / w:.0f} | "
+                     f"{avg['SQ_INSTS_LDS'] / w:.0f} | {avg['SQ_WAVE_CYCLES'] / w:.0f} | {wait:.1f} |")
+    return "\n".join(lines) + "\n"
+
+
 def traffic_bytes(summary: dict, kernel: str):
     """Calibrated HBM bytes per dispatch of `kernel` (fetch + write), or None."""
     k = summary.get("kernels", {}).get(kernel)
@@ -153,6 +191,10 @@ def main(argv=None):
     t.add_argument("dir")
     t.add_argument("--title", default="")
     t.add_argument("--command", default="")
+    q = sub.add_parser("sq")
+    q.add_argument("dir")
+    q.add_argument("--title", default="")
+    q.add_argument("--command", default="")
     p = sub.add_parser("pmc")
     p.add_argument("fetch_dir")
     p.add_argument("write_dir")
@@ -163,6 +205,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "trace":
         sys.stdout.write(trace_markdown(a.dir, a.title, a.command))
+    elif a.cmd == "sq":
+        sys.stdout.write(sq_markdown(a.dir, a.title, a.command))
     else:
         s = pmc_summary(a.fetch_dir, a.write_dir, a.copy_bytes)
         s = {"workload": a.workload, "config": json.loads(a.config), "commands": a.command, **s}
