@@ -268,7 +268,10 @@ int mc3d_marching_cubes_c2f(mc3d_ctx *ctx, const mc3d_solid *s, double big_delta
 int mc3d_marching_cubes_search_batch(mc3d_ctx *ctx, const mc3d_solid *const *solids, const double *deltas,
                                      const int32_t *iters, size_t n, const mc3d_options *opts, mc3d_mesh **out);
 int mc3d_mesh_counts(const mc3d_mesh *m, size_t *n_vertices, size_t *n_triangles);
-/* Copies the results to host arrays of 3*V doubles and 3*T uint32. */
+/* Copies the results to host arrays of 3*V doubles and 3*T uint32 (either
+ * may be NULL).  For a mesh left on the device (device_only == 1) the copy
+ * runs in 4 MiB chunks through pinned buffers of the library, each chunk's
+ * host copy overlapping the next chunks' DMA. */
 int mc3d_mesh_copy(mc3d_mesh *m, double *verts, uint32_t *tris);
 /* The results of a mesh searched with device_only == 0, in the library's
  * pinned host memory (3*V doubles, 3*T uint32; valid until mc3d_mesh_free):

@@ -329,6 +329,52 @@ def test_go_shim_call_sequence():
         b.release(h)
 
 
+@pytest.mark.gpu
+def test_device_mesh_copy_into_caller_arrays():
+    """mc3d_mesh_copy of a mesh left on the device (chunked through pinned
+    buffers, more chunks than the ring holds) and of its interior points
+    equals the same call's mesh fetched to the host, byte for byte, at the
+    benchmark's size (19.5 MB of mesh)."""
+    import ctypes
+
+    from model3d_amd import _lib
+    from model3d_amd.solids import LibBackend
+
+    L = _lib.lib()
+    ctx = _lib.context(0)
+    s = workloads.csg64_solid(2)
+    print("device mesh copy: csg64 d=0.005", flush=True)
+    b = LibBackend()
+    h = s._build(b)
+    got = {}
+    try:
+        for dev in (0, 1):
+            o = _lib.mc3d_options()
+            o.precision, o.cull, o.device_only = FP64, 1, dev
+            out = ctypes.c_void_p()
+            _lib.check(L.mc3d_marching_cubes_interior(ctx, h, 0.005, 8, ctypes.byref(o), ctypes.byref(out)), "interior")
+            try:
+                nv, nt = ctypes.c_size_t(), ctypes.c_size_t()
+                _lib.check(L.mc3d_mesh_counts(out, ctypes.byref(nv), ctypes.byref(nt)), "counts")
+                v = np.full((nv.value, 3), np.nan)
+                f = np.full((nt.value, 3), 0xFFFFFFFF, dtype=np.uint32)
+                inner = np.full((nv.value, 3), np.nan)
+                _lib.check(L.mc3d_mesh_copy(out, v.ctypes.data, f.ctypes.data), "copy")
+                _lib.check(L.mc3d_mesh_copy_interior(out, inner.ctypes.data), "copy interior")
+                # either array alone
+                f2 = np.zeros_like(f)
+                _lib.check(L.mc3d_mesh_copy(out, None, f2.ctypes.data), "copy tris")
+                assert np.array_equal(f, f2)
+                got[dev] = (v, f, inner)
+            finally:
+                L.mc3d_mesh_free(out)
+        assert got[0][0].nbytes + got[0][1].nbytes > 16 << 20
+        for a, c in zip(got[0], got[1]):
+            assert a.tobytes() == c.tobytes()
+    finally:
+        b.release(h)
+
+
 def _spacer(lo, hi, delta):
     """newSquareSpacer's axis (mc.go:594-608): plain float adds, as Go's."""
     out, x = [], lo - delta
