@@ -429,7 +429,7 @@ def next_rows(work, run, steps=5):
         timed("c2f", lambda m: L.mc3d_marching_cubes_c2f(ctx, h, 4 * delta, delta, 0.0, iters, ctypes.byref(o),
                                                          ctypes.byref(m)))
 
-        def stl(m, st):
+        def stl_and_group(m, st):
             n = L.mc3d_stl_bytes(st["triangles"])
             buf = np.empty(n, dtype=np.uint8)
             t = []
@@ -439,9 +439,17 @@ def next_rows(work, run, steps=5):
                 t.append(1e3 * (time.perf_counter() - t0))
             out["stl"] = {"ms_per_encode": sum(t) / len(t), "bytes": n,
                           "GB_per_s_to_host": n / (sum(t) / len(t) / 1e3) / 1e9}
+            # GroupTriangles (SaveGroupedSTL's order) of the mesh in place on
+            # the device
+            t = []
+            for _ in range(steps):
+                t0 = time.perf_counter()
+                _lib.check(L.mc3d_mesh_group_triangles(ctx, m), "group")
+                t.append(1e3 * (time.perf_counter() - t0))
+            out["group_triangles"] = {"ms_per_call": sum(t) / len(t), "triangles": st["triangles"]}
 
         timed("search", lambda m: L.mc3d_marching_cubes_search(ctx, h, delta, iters, ctypes.byref(o), ctypes.byref(m)),
-              extra=stl)
+              extra=stl_and_group)
     finally:
         b.release(h)
         b.close()
@@ -535,14 +543,16 @@ def main():
     ap.add_argument("--profile-run", action="store_true",
                     help="only the warm-up and timed steps (for rocprofv3; stages enqueued one by one), plus the "
                          "calibration streams")
+    ap.add_argument("--next-rows-only", action="store_true",
+                    help="only the entry points next to the search (for rocprofv3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
     ap.add_argument("--no-configs", action="store_true", help="skip the lines of the other configurations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
-            "bench.py measures one GPU only: this build runs on single-GPU boxes and has no multi-GPU path "
-            "(z-slab / per-solid sharding is a design only)",
+            "bench.py measures one GPU only: the library runs on one GPU and has no multi-GPU path "
+            "(sharding solids or z slabs over GPUs is a design only)",
             file=sys.stderr,
         )
         raise SystemExit(2)
@@ -566,6 +576,11 @@ def main():
         best = min(tuned, key=lambda k: tuned[k]["ms"])
         run = dict(precision=best, cull=True, brick=tuned[best]["brick"], rows_per_walk=tuned[best]["rows_per_walk"],
                    waves_per_brick=tuned[best]["waves_per_brick"])
+
+    if args.next_rows_only:
+        print(json.dumps({"workload": args.workload, "config": run, "next_rows": next_rows(work, run, args.steps)}),
+              flush=True)
+        return
 
     if args.profile_run:
         streams = copy_calibration()
