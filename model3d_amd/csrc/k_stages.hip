@@ -166,6 +166,19 @@ __global__ __launch_bounds__(256) void k_stream_write(uint4 *__restrict__ out, u
         out[i] = make_uint4(v, v + 1, v + 2, (uint32_t)i);
 }
 
+// Zeroes `bytes` (a multiple of 16) with plain stores: a fill kernel, not a
+// memset, so that it may sit in a captured graph.
+__global__ __launch_bounds__(256) void k_zero(uint4 *__restrict__ p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4(0, 0, 0, 0);
+}
+
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t st) {
+    const uint64_t n = bytes / 16;
+    if (n) k_zero<<<grid_for(n, 256, 4096), 256, 0, st>>>((uint4 *)p, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_stream(bool write, void *buf, uint64_t items, uint32_t *out, hipStream_t st) {
     const unsigned grid = 8192;
     if (write)
@@ -191,20 +204,32 @@ hipError_t launch_plan(const Pipe &p, const Bufs &b, hipStream_t st) {
 
 hipError_t launch_classify(const Pipe &p, const Bufs &b, hipStream_t st) {
     const unsigned ctiles = (unsigned)((p.nscan + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK);
-    if (p.nsegs > 1)
+    if (p.scan_one_pass) {  // small grids: the scan too
+        if (p.nsegs > 1)
+            k_classify_scan<true><<<ctiles, CLASSIFY_BLOCK, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits,
+                                                                     b.table, nullptr, p.nwords, false, b.agg, b.offs,
+                                                                     b.actw, p.ctl);
+        else
+            k_classify_scan<false><<<ctiles, CLASSIFY_BLOCK, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits,
+                                                                      b.table, b.keep, p.nscan, p.bm, b.agg, b.offs,
+                                                                      b.actw, p.ctl);
+    } else if (p.nsegs > 1) {
         k_classify<true><<<ctiles, CLASSIFY_BLOCK, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits, b.table,
                                                             b.counts, b.tiles, nullptr, p.nwords, false);
-    else
+    } else {
         k_classify<false><<<ctiles, CLASSIFY_BLOCK, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits, b.table,
                                                              b.counts, b.tiles, b.keep, p.nscan, p.bm);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_scan(const Pipe &p, const Bufs &b, hipStream_t st) {
-    // the scan of the counts whose per-block sums classify wrote
+    // the scan of the counts whose per-block sums classify wrote (done by
+    // classify itself on small grids)
     const uint64_t ntiles = (p.nscan + SCAN_TILE - 1) / SCAN_TILE;
     const uint64_t nparts = (p.nscan + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK;
-    if (nparts <= SCAN_FUSE_PARTS) {  // each tile sums the block sums before it
+    if (p.scan_one_pass) {
+    } else if (nparts <= SCAN_FUSE_PARTS) {  // each tile sums the block sums before it
         k_scan_down<true><<<(unsigned)ntiles, SCAN_BLOCK, 0, st>>>(b.counts, p.nscan, b.tiles, b.offs, b.actw, p.seg0,
                                                                   p.bm, p.ctl->totals);
     } else {

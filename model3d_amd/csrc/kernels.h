@@ -687,68 +687,159 @@ __device__ __forceinline__ bool bm_word(const Grid &g, const Bricks &B, uint32_t
     return y < g.ny && z < g.nz;
 }
 
-// counts[w] = crossed edges | triangles << 8 | (1 if the word emits anything)
-// << 17 (at most 192 edges and 320 triangles per word); a word per thread,
-// and per block (CLASSIFY_BLOCK words) the block's sums {edges, triangles,
-// emitting words, 0} for the scan: part_sums[blockIdx.x].  With `bm` (one
-// solid, culled) thread i takes the word at brick-major position i and
-// counts[i] is that word's (nwords_all: the positions).
+// One scan position classified: {crossed edges (its vertices), triangles
+// of its mixed cubes, 1 if it emits anything, 0} (at most 192 edges and 320
+// triangles per word), and in `word` the position's word as the list of
+// emitting words names it.  Position gw is an occupancy word, or with `bm`
+// (one solid, culled) the word at brick-major position gw.
+template <bool MULTI>
+__device__ __forceinline__ uint4 classify_position(const Seg *__restrict__ segs, const Seg &seg0,
+                                                   const uint32_t *__restrict__ word_seg,
+                                                   const uint64_t *__restrict__ bits_all, const uint8_t *ntri,
+                                                   const uint64_t *__restrict__ keep_all, bool bm, uint64_t gw,
+                                                   uint32_t &word) {
+    const uint32_t si = wave_seg<MULTI>(word_seg, gw, SEG_WORD_SHIFT);
+    const Seg sg = pick_seg<MULTI>(segs, seg0, si);
+    const Grid &g = sg.g;
+    const uint64_t *bits = bits_all + sg.word0;
+    uint64_t w = gw - sg.word0;
+    int wx = 0, y = 0, z = 0;
+    bool has;
+    if (MULTI || !bm) {
+        has = w < sg.nwords;  // else alignment padding
+        if (has) word_coords(g, w, wx, y, z);
+        word = (uint32_t)gw;
+    } else {
+        has = bm_word(g, sg.B, (uint32_t)gw, wx, y, z);
+        w = word_index(g, wx, y, z);
+        word = (uint32_t)w;
+    }
+    if (!has) return make_uint4(0, 0, 0, 0);
+    const Nbhd h = load_nbhd(g, bits, wx, y, z);
+    Masks m = nbhd_masks(h);
+    const CubeCorners cc = nbhd_corners(h);
+    uint64_t mixed = mixed_cubes(cc);
+    if (keep_all) {  // a block filter: only the cubes of accepted blocks
+        const uint64_t *keep = keep_all + sg.word0;
+        keep_masks(g, keep, wx, y, z, m);
+        mixed &= keep[w];
+    }
+    const uint32_t nv = __popcll(m.cx) + __popcll(m.cy) + __popcll(m.cz);
+    uint32_t nt = 0;
+    while (mixed) {
+        const int i = __ffsll((unsigned long long)mixed) - 1;
+        mixed &= mixed - 1;
+        nt += ntri[cube_case(cc, i)];
+    }
+    return make_uint4(nv, nt, (nv | nt) != 0 ? 1u : 0u, 0u);
+}
+
+// counts[pos] = crossed edges | triangles << 8 | (1 if the word emits
+// anything) << 17, a position per thread, and per block (CLASSIFY_BLOCK
+// positions) the block's sums {edges, triangles, emitting words, 0} for the
+// scan: part_sums[blockIdx.x] (npos: the positions).
 template <bool MULTI>
 __global__ __launch_bounds__(CLASSIFY_BLOCK) void k_classify(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
                                                          const uint32_t *__restrict__ word_seg,
                                                          const uint64_t *__restrict__ bits_all,
                                                          const uint8_t *__restrict__ table, uint32_t *__restrict__ counts,
                                                          uint4 *__restrict__ part_sums,
-                                                         const uint64_t *__restrict__ keep_all, uint64_t nwords_all,
+                                                         const uint64_t *__restrict__ keep_all, uint64_t npos,
                                                          bool bm) {
     __shared__ uint8_t ntri[256];
     __shared__ uint4 sh[CLASSIFY_BLOCK / 64];
     if (threadIdx.x < 256) ntri[threadIdx.x] = table[16 * threadIdx.x];
     __syncthreads();
     uint4 c = make_uint4(0, 0, 0, 0);
-    // consecutive words per wave (segment ranges are aligned to 64)
+    // consecutive positions per wave (segment ranges are aligned to 64)
     const uint64_t gw = (uint64_t)blockIdx.x * CLASSIFY_BLOCK + threadIdx.x;
-    if (gw < nwords_all) {  // wave-uniform: nwords_all is a multiple of 64
-        const uint32_t si = wave_seg<MULTI>(word_seg, gw, SEG_WORD_SHIFT);
-        const Seg sg = pick_seg<MULTI>(segs, seg0, si);
-        const Grid &g = sg.g;
-        const uint64_t *bits = bits_all + sg.word0;
-        uint64_t w = gw - sg.word0;
-        int wx = 0, y = 0, z = 0;
-        bool has;
-        if (MULTI || !bm) {
-            has = w < sg.nwords;  // else alignment padding
-            if (has) word_coords(g, w, wx, y, z);
-        } else {
-            has = bm_word(g, sg.B, (uint32_t)gw, wx, y, z);
-            w = word_index(g, wx, y, z);
-        }
-        if (has) {
-            const Nbhd h = load_nbhd(g, bits, wx, y, z);
-            Masks m = nbhd_masks(h);
-            const CubeCorners cc = nbhd_corners(h);
-            uint64_t mixed = mixed_cubes(cc);
-            if (keep_all) {  // a block filter: only the cubes of accepted blocks
-                const uint64_t *keep = keep_all + sg.word0;
-                keep_masks(g, keep, wx, y, z, m);
-                mixed &= keep[w];
-            }
-            const uint32_t nv = __popcll(m.cx) + __popcll(m.cy) + __popcll(m.cz);
-            uint32_t nt = 0;
-            while (mixed) {
-                const int i = __ffsll((unsigned long long)mixed) - 1;
-                mixed &= mixed - 1;
-                nt += ntri[cube_case(cc, i)];
-            }
-            c = make_uint4(nv, nt, (nv | nt) != 0 ? 1u : 0u, 0u);
-        }
+    if (gw < npos) {  // wave-uniform: npos is a multiple of 64
+        uint32_t word;
+        c = classify_position<MULTI>(segs, seg0, word_seg, bits_all, ntri, keep_all, bm, gw, word);
         counts[gw] = c.x | (c.y << 8) | (c.z << 17);
     }
     const uint4 t = block_sum<CLASSIFY_BLOCK>(c, sh);
     if (threadIdx.x == 0) part_sums[blockIdx.x] = t;
 }
 
-
+// Classify and the scan of its counts in one pass, for at most
+// SCAN_ONE_PASS blocks: a block takes tile t in the order blocks start (a
+// ticket), classifies its CLASSIFY_BLOCK positions, scans them, publishes
+// its sums in agg[t], waits until every tile before it has published (one
+// lane per tile), adds theirs up and writes offs[pos] = {first vertex,
+// first triangle} and the list of emitting words itself; the last tile
+// writes the totals.  A block that has waited SCAN_SPIN_LIMIT rounds gives
+// up (scan_error) rather than spin on; every tile before it has a running
+// or finished block (tickets follow the start order), so the wait ends.
+template <bool MULTI>
+__global__ __launch_bounds__(CLASSIFY_BLOCK) void k_classify_scan(const Seg *__restrict__ segs, uint32_t nsegs,
+                                                              const Seg seg0, const uint32_t *__restrict__ word_seg,
+                                                              const uint64_t *__restrict__ bits_all,
+                                                              const uint8_t *__restrict__ table,
+                                                              const uint64_t *__restrict__ keep_all, uint64_t npos,
+                                                              bool bm, ScanAgg *__restrict__ agg,
+                                                              uint2 *__restrict__ offs, uint32_t *__restrict__ list,
+                                                              Ctl *__restrict__ ctl) {
+    __shared__ uint8_t ntri[256];
+    __shared__ uint4 sh[CLASSIFY_BLOCK / 64];
+    __shared__ uint32_t s_tile;
+    __shared__ uint4 s_pre;
+    if (threadIdx.x < 256) ntri[threadIdx.x] = table[16 * threadIdx.x];
+    if (threadIdx.x == 0) s_tile = atomicAdd(&ctl->scan_ticket, 1u);
+    __syncthreads();
+    const uint32_t t = s_tile;
+    const unsigned long long epoch = ctl->epoch;
+    uint4 c = make_uint4(0, 0, 0, 0);
+    uint32_t word = 0;
+    const uint64_t gw = (uint64_t)t * CLASSIFY_BLOCK + threadIdx.x;
+    if (gw < npos) c = classify_position<MULTI>(segs, seg0, word_seg, bits_all, ntri, keep_all, bm, gw, word);
+    uint4 total;
+    const uint4 ex = block_exclusive<CLASSIFY_BLOCK>(c, sh, total);
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        if (lane == 0) {  // this tile's sums (atomics: seen by every block at once)
+            __hip_atomic_exchange(&agg[t].w[0], ((unsigned long long)total.x << 32) | epoch, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_exchange(&agg[t].w[1], ((unsigned long long)((total.y << 12) | total.z) << 32) | epoch,
+                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        uint4 pre = make_uint4(0, 0, 0, 0);
+        for (uint32_t spins = 0;; spins++) {
+            uint4 v = make_uint4(0, 0, 0, 0);
+            bool ready = true;
+            if ((uint32_t)lane < t) {
+                const unsigned long long a =
+                    __hip_atomic_fetch_or(&agg[lane].w[0], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long b =
+                    __hip_atomic_fetch_or(&agg[lane].w[1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ready = (a & 0xffffffffull) == epoch && (b & 0xffffffffull) == epoch;
+                const uint32_t tl = (uint32_t)(b >> 32);
+                v = make_uint4((uint32_t)(a >> 32), tl >> 12, tl & 0xfffu, 0u);
+            }
+            if (!__ballot(!ready)) {
+                for (int d = 32; d; d >>= 1)
+                    v = add4(v, make_uint4(__shfl_xor(v.x, d), __shfl_xor(v.y, d), __shfl_xor(v.z, d), 0u));
+                pre = v;
+                break;
+            }
+            if (spins >= SCAN_SPIN_LIMIT) {
+                if (lane == 0) atomicOr(&ctl->scan_error, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        if (lane == 0) {
+            s_pre = pre;
+            if (t == gridDim.x - 1)
+                ctl->totals[0] = pre.x + total.x, ctl->totals[1] = pre.y + total.y, ctl->totals[2] = pre.z + total.z;
+        }
+    }
+    __syncthreads();
+    if (gw < npos) {
+        offs[gw] = make_uint2(s_pre.x + ex.x, s_pre.y + ex.y);
+        if (c.z) list[s_pre.z + ex.z] = word;
+    }
+}
 
 // ---------------------------------------------------------------------------
 // emit

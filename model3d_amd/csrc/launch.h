@@ -20,6 +20,7 @@ struct Bufs {
     uint2 *offs = nullptr;       // scanned {first vertex, first triangle}
     uint32_t *actw = nullptr;    // listed (emitting) words
     uint4 *tiles = nullptr;      // classify block sums
+    ScanAgg *agg = nullptr;      // classify block sums of the one-pass scan (SCAN_ONE_PASS entries)
     uint2 *binfo = nullptr;      // per brick {code offset, length}
     uint32_t *actb = nullptr;    // active bricks
     Insn *bcode = nullptr;       // folded brick programs
@@ -45,6 +46,7 @@ struct Pipe {
     size_t plan_lds = 0;  // dynamic LDS of the plan (its stacks, the staged program)
     uint32_t plan_gsz = 64;  // bricks per plan wave
     bool plan_tile = false;  // plan waves take compact tiles of bricks (one solid)
+    bool scan_one_pass = false;  // classify scans its counts itself (at most SCAN_ONE_PASS blocks)
     uint32_t emit_div = 1024;  // emit blocks: occupancy words / emit_div, in [512, 4096]
 };
 
@@ -64,6 +66,8 @@ inline hipError_t launch_corners(const Pipe &p, const Bufs &b, hipStream_t st) {
     if (p.mode == 0) return p.stats ? launch_corners_ms<0, true>(p, b, st) : launch_corners_ms<0, false>(p, b, st);
     return p.stats ? launch_corners_ms<1, true>(p, b, st) : launch_corners_ms<1, false>(p, b, st);
 }
+// zeroes bytes (a multiple of 16) of device memory with a kernel
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t st);
 hipError_t launch_classify(const Pipe &p, const Bufs &b, hipStream_t st);
 // the scan of classify's counts, the list of emitting words and (calls of
 // several solids) each solid's output ranges

@@ -46,12 +46,15 @@ using namespace mc3d;
 // Scratch of the allocation-free entry point (mc3d_marching_cubes_search_noalloc):
 // every device buffer that call needs beyond its inputs and outputs -- the
 // program blob, occupancy, counts, offsets, the word and brick lists, the
-// brick programs and the vertex keys -- is carved from this array.  Each GPU
+// brick programs and the vertex keys -- is carved from this array, except its
+// last SCRATCH_AGG_BYTES, which hold classify's one-pass scan sums (zeroed
+// once).  Each GPU
 // holds its own copy until the process ends.  In host code `scratch` is only
 // a stand-in for the symbol: its device address on the current device comes
 // from hipGetSymbolAddress (scratch_address), and only that address is used.
 // ---------------------------------------------------------------------------
 constexpr size_t SCRATCH_BYTES = 64u << 20;
+constexpr size_t SCRATCH_AGG_BYTES = 4096;
 static __device__ __align__(256) char scratch[SCRATCH_BYTES];
 
 namespace {
@@ -59,6 +62,7 @@ namespace {
 constexpr int MAX_DEVICES = 64;
 std::mutex g_scratch_mu;
 void *g_scratch_ptr[MAX_DEVICES];              // device address of `scratch` per device (null: not looked up)
+uint64_t g_scratch_agg_gen[MAX_DEVICES];       // epoch generation its scan sums were zeroed for
 std::mutex g_noalloc_mu[MAX_DEVICES];          // the allocation-free calls of one device run one at a time
 
 // The device address of `scratch` on `dev`, which must be the current
@@ -71,15 +75,32 @@ int scratch_address(int dev, void **out) {
     std::lock_guard<std::mutex> lk(g_scratch_mu);
     if (!g_scratch_ptr[dev]) {
         void *p = nullptr;
-        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
         if (e != hipSuccess || !p) {
             set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return MC3D_E_HIP;
+        }
+        // the one-pass scan sums start zeroed (epoch generation 0)
+        if ((e = hipMemset((char *)p + SCRATCH_BYTES - SCRATCH_AGG_BYTES, 0, SCRATCH_AGG_BYTES)) != hipSuccess ||
+            (e = hipDeviceSynchronize()) != hipSuccess) {
+            set_error(std::string("hipMemset(scratch): ") + hipGetErrorString(e));
             return MC3D_E_HIP;
         }
         g_scratch_ptr[dev] = p;
     }
     *out = g_scratch_ptr[dev];
     return MC3D_OK;
+}
+
+// Scan epochs: every call tags its classify blocks' one-pass sums with an
+// epoch no other call of the process uses (1 .. SCAN_EPOCH_MAX - 1).  When
+// the sequence starts over its generation grows, and a sums array is zeroed
+// again before its first use in the new generation.
+std::atomic<uint64_t> g_epoch_count{0};
+void next_epoch(uint32_t &epoch, uint64_t &gen) {
+    const uint64_t n = g_epoch_count.fetch_add(1);
+    epoch = 1 + (uint32_t)(n % (SCAN_EPOCH_MAX - 1));
+    gen = n / (SCAN_EPOCH_MAX - 1);
 }
 
 #define HIP_TRY(expr)                                                                        \
@@ -394,7 +415,8 @@ struct mc3d_ctx {
     void *scratch = nullptr;     // this device's `scratch` (allocation-free entry point)
     std::vector<std::unique_ptr<ColliderCache>> colliders;  // most recent last
     HostBuf staging, ctl_host, ctl_early, segstage, outstage;
-    DevBuf blob, table, etable, bits, counts, offs, tiles, actw, actb, binfo, bcode, vkeys, vseg, pts, out8;
+    DevBuf blob, table, etable, bits, counts, offs, tiles, agg, actw, actb, binfo, bcode, vkeys, vseg, pts, out8;
+    uint64_t agg_gen = ~0ull;  // epoch generation `agg` was zeroed for
     DevBuf hvals, hpts, hstate, hprobe;  // host-evaluated atoms: answers, points, bisection state
     HostBuf hstage;                      // pinned points / answers of one callback batch
     DevBuf stl, stl_in;                  // STL encoding: body bytes; uploaded host triangles
@@ -706,6 +728,7 @@ struct Upload {
     size_t back_bytes = 0;   // control block + per-solid results, copied back at the end
     const double *conj = nullptr;  // job 0's MarchingCubesConj inverse chain (records of 10 doubles)
     uint32_t conj_nops = 0;
+    uint32_t epoch = 0;  // this call's scan epoch (written into the control block)
 };
 
 // The blob's layout: the control block, the per-solid results, the
@@ -761,6 +784,7 @@ void fill_upload(std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_a
                  Upload &U) {
     const size_t n = jobs.size();
     memset(h, 0, U.o_segs);  // control block and per-solid results start at zero
+    ((Ctl *)h)->epoch = U.epoch;
     U.hsegs.assign(n, Seg{});
     // the jobs' regions are disjoint: many jobs are filled in parallel
     auto fill = [&](size_t s) {
@@ -985,8 +1009,17 @@ int plan_call(std::vector<Job> &jobs, const mc3d_options &o, Call &c) {
 uint64_t classify_blocks(const Call &c) { return (c.nscan + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK; }
 
 // The context's buffers for a call (grown as needed).
-int ctx_buffers(mc3d_ctx *ctx, const Call &c, Bufs &b) {
+int ctx_buffers(mc3d_ctx *ctx, const Call &c, Bufs &b, uint32_t &epoch) {
     int rc;
+    const void *agg_was = ctx->agg.p;
+    if ((rc = ctx->agg.ensure(SCAN_ONE_PASS * sizeof(ScanAgg)))) return rc;
+    uint64_t gen;
+    next_epoch(epoch, gen);
+    if (ctx->agg.p != agg_was || ctx->agg_gen != gen) {  // new memory, or the epochs started over
+        HIP_TRY(launch_zero(ctx->agg.p, ctx->agg.cap, ctx->stream));
+        ctx->agg_gen = gen;
+    }
+    b.agg = ctx->agg.as<ScanAgg>();
     if ((rc = ctx->bits.ensure(c.nwords * 8)) || (rc = ctx->counts.ensure(c.nscan * sizeof(uint32_t))) ||
         (rc = ctx->offs.ensure(c.nscan * sizeof(uint2))) || (rc = ctx->actw.ensure(c.nwords * sizeof(uint32_t))) ||
         (rc = ctx->tiles.ensure(classify_blocks(c) * sizeof(uint4))))
@@ -1036,6 +1069,7 @@ Pipe make_pipe(const Call &c, const Upload &U) {
     p.plan_tile = true;
     p.plan_gsz = c.plan_lds > PLAN_STACK_BYTES ? 16 : 64;
     p.emit_div = 512;
+    p.scan_one_pass = classify_blocks(c) <= SCAN_ONE_PASS;
     return p;
 }
 
@@ -1120,6 +1154,10 @@ int run_output_host(mc3d_ctx *ctx, const Call &c, const Pipe &p, Bufs &b, const 
     Ctl *hc = (Ctl *)ctx->ctl_host.p;
     HIP_TRY(hipMemcpyAsync(hc, U.ctl, U.back_bytes, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (hc->scan_error) {
+        set_error("internal error: classify gave up waiting for the blocks before it");
+        return MC3D_E_INTERNAL;
+    }
     if (hc->inside) {
         set_error("solid is true outside of bounds");
         return MC3D_E_INSIDE_BOUNDARY;
@@ -1382,7 +1420,8 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
     int rc;
     if ((rc = plan_call(jobs, o, c))) return rc;
     Bufs b;
-    if ((rc = ctx_buffers(ctx, c, b))) return rc;
+    uint32_t epoch = 0;
+    if ((rc = ctx_buffers(ctx, c, b, epoch))) return rc;
     const bool dfilter = c.nsegs == 1 && jobs[0].dfilter;
     if (c.nsegs == 1 && (!jobs[0].keep.empty() || dfilter)) {
         if ((rc = ctx->keep.ensure(jobs[0].nwords * 8))) return rc;
@@ -1392,6 +1431,7 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
         b.keep = ctx->keep.as<uint64_t>();
     }
     Upload U;
+    U.epoch = epoch;
     if ((rc = prepare_upload(ctx, jobs, c.nwords, c.nbricks, b, U))) return rc;
     if ((rc = ctx->ctl_host.ensure(U.back_bytes)) || (rc = ctx->ctl_early.ensure(U.back_bytes))) return rc;
 
@@ -1466,6 +1506,11 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
         const auto t_wait = std::chrono::steady_clock::now();
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         ms_wait = ms_since(t_wait);
+        if (hc->scan_error) {
+            if (copying) (void)hipStreamSynchronize(ctx->copy);
+            set_error("internal error: classify gave up waiting for the blocks before it");
+            return MC3D_E_INTERNAL;
+        }
         if (hc->inside) {
             if (copying) (void)hipStreamSynchronize(ctx->copy);
             size_t bad = 0;
@@ -1645,7 +1690,7 @@ int search_noalloc(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters, 
     Bufs b;
     size_t need = 0;
     char *base = (char *)ctx->scratch;
-    const uint64_t kcap = carve_scratch(base, SCRATCH_BYTES, c, U.total, vcap, b, need);
+    const uint64_t kcap = carve_scratch(base, SCRATCH_BYTES - SCRATCH_AGG_BYTES, c, U.total, vcap, b, need);
     if (!kcap && vcap) {
         set_error("mc3d_marching_cubes_search_noalloc: the grid needs " + std::to_string(need) +
                   " bytes of scratch, more than the " + std::to_string(SCRATCH_BYTES) + " the library holds");
@@ -1653,6 +1698,13 @@ int search_noalloc(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters, 
     }
     b.table = ctx->table.as<uint8_t>();
     b.etable = ctx->etable.as<uint8_t>();
+    b.agg = (ScanAgg *)(base + SCRATCH_BYTES - SCRATCH_AGG_BYTES);
+    uint64_t gen;
+    next_epoch(U.epoch, gen);
+    if (g_scratch_agg_gen[ctx->device] != gen) {  // the epochs started over: zero the sums again
+        HIP_TRY(launch_zero(b.agg, SCRATCH_AGG_BYTES, ctx->stream));
+        g_scratch_agg_gen[ctx->device] = gen;
+    }
     // The blob is built in host memory and copied from there (pageable:
     // the copy has taken the bytes when hipMemcpyAsync returns).
     std::vector<char> h(U.total);
@@ -1667,6 +1719,10 @@ int search_noalloc(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters, 
     Ctl *hc = (Ctl *)ctx->ctl_host.p;
     HIP_TRY(hipMemcpyAsync(hc, U.ctl, U.back_bytes, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (hc->scan_error) {
+        set_error("internal error: classify gave up waiting for the blocks before it");
+        return MC3D_E_INTERNAL;
+    }
     if (hc->inside) {
         set_error("solid is true outside of bounds");
         return MC3D_E_INSIDE_BOUNDARY;

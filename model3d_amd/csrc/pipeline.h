@@ -25,7 +25,23 @@ struct Ctl {
     unsigned long long code_total; // instructions over all active bricks' folded programs
     uint32_t code_max;             // longest folded program
     uint32_t full_bricks;          // bricks whose folded program overflowed its slot
+    uint32_t epoch;                // this call's tag for the classify blocks' sums (set by the host)
+    uint32_t scan_ticket;          // classify blocks in the order they started
+    uint32_t scan_error;           // a classify block gave up waiting for the blocks before it
+    uint32_t pad;
 };
+
+// A classify block's sums for the one-pass scan of small grids: two words,
+// w[0] = vertices << 32 | epoch, w[1] = (triangles << 12 | emitting words)
+// << 32 | epoch (a block of CLASSIFY_BLOCK words has fewer than 2^20
+// triangles and at most 2^10 emitting words).  Each word is written and read
+// whole by an atomic, so a reader takes the two only when both carry this
+// call's epoch.  A word of another epoch is an earlier call's (the array
+// starts zeroed and epochs are never reused).
+struct ScanAgg {
+    unsigned long long w[2];
+};
+constexpr uint32_t SCAN_EPOCH_MAX = 1u << 30;
 
 struct Grid {
     const double *__restrict__ xs;
@@ -87,6 +103,8 @@ constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
 constexpr int CLASSIFY_BLOCK = 1024;  // words per classify block and per block sum
 constexpr int SUMS_BLOCK = 1024;      // threads of the one block scanning the block sums
 constexpr int SCAN_FUSE_PARTS = 4096; // up to this many block sums, each scan tile adds its prefix itself
+constexpr int SCAN_ONE_PASS = 64;     // up to this many classify blocks, classify scans its counts itself
+constexpr uint32_t SCAN_SPIN_LIMIT = 1u << 22;  // rounds of waiting after which a block gives up
 static_assert(SCAN_TILE % CLASSIFY_BLOCK == 0, "a scan tile starts at a classify block");
 
 }  // namespace mc3d
