@@ -194,7 +194,9 @@ struct HostBuf {
 };
 
 // Buffers released by freed meshes, reused by later calls (device buffers,
-// or pinned host buffers).
+// or pinned host buffers): the most recent 32, up to 8 GiB together (a
+// z-slab search holds a mesh per slab, and freeing device memory
+// synchronises the device).
 template <class B>
 struct Pool {
     std::mutex mu;  // meshes may be freed on any thread
@@ -215,7 +217,12 @@ struct Pool {
         if (!b || !b->p) return;
         std::lock_guard<std::mutex> lk(mu);
         free.push_back(std::move(b));
-        while (free.size() > 8) free.erase(free.begin());
+        size_t bytes = 0;
+        for (const auto &f : free) bytes += f->cap;
+        while (free.size() > 32 || (free.size() > 1 && bytes > (8ull << 30))) {
+            bytes -= free.front()->cap;
+            free.erase(free.begin());
+        }
     }
 };
 
@@ -1875,6 +1882,9 @@ int search_slabs(mc3d_ctx *ctx, Job &base, const mc3d_options &o, int planes, mc
     const int ncube = (int)zs.size() - 1;
     mc3d_options so = o;
     so.device_only = 1;
+    // each slab's outputs are buffers of their own, so its chain would be
+    // captured anew: enqueue the stages instead
+    so.no_graph = 1;
     struct Piece {
         std::unique_ptr<mc3d_mesh, void (*)(mc3d_mesh *)> m{nullptr, mc3d_mesh_free};
         PlaneList top, bot;
