@@ -68,6 +68,20 @@ def test_sphere_config_bit_exact(iters):
         assert same_triangles(got.triangles(), want)
 
 
+@pytest.mark.parametrize("delta,tiles", [(0.013885, 64), (0.01379, 65)])
+def test_scan_one_pass_boundary(delta, tiles):
+    """Grids of 64 classify blocks (classify scans its counts itself) and 65
+    (the separate scan), without culling: the meshes equal the oracle's."""
+    s = m3.Sphere((0.0, 0.0, 0.0), 1.0)
+    print(f"sphere d={delta} ({tiles} classify blocks)", flush=True)
+    want, _ = oracle_search(s, delta, 4, threads=8)
+    got = m3.MarchingCubesSearch(s, delta, 4, cull=False)
+    st = got.stats
+    words = (st["grid_nx"] + 63) // 64 * st["grid_ny"] * st["grid_nz"]
+    assert ((words + 63) // 64 * 64 + 1023) // 1024 == tiles
+    assert same_triangles(got.triangles(), want)
+
+
 def test_csg64_coarse_bit_exact():
     s = workloads.csg64_solid(2)
     print("csg64 d=0.02", flush=True)
