@@ -189,8 +189,15 @@ hipError_t launch_stream(bool write, void *buf, uint64_t items, uint32_t *out, h
 }
 
 hipError_t launch_plan(const Pipe &p, const Bufs &b, hipStream_t st) {
-    // about one wave per unit of bricks (a grid-stride loop takes the rest)
-    const unsigned pgrid = grid_for((p.nbricks + p.plan_gsz - 1) / p.plan_gsz, PLAN_BLOCK / 64, 1u << 16);
+    // one wave per unit of bricks (plan_units; past 2^17 units a grid-stride
+    // loop takes the rest): tiles at the grid's edges are partial, so a
+    // one-solid call has more units than nbricks / gsz
+    uint64_t units = (p.nbricks + p.plan_gsz - 1) / p.plan_gsz;
+    if (p.nsegs == 1 && p.plan_tile) {
+        const uint32_t ty = 1u << ((31 - __builtin_clz(p.plan_gsz)) / 2), tz = p.plan_gsz / ty;
+        units = (uint64_t)p.seg0.g.W * ((p.seg0.B.nby + ty - 1) / ty) * ((p.seg0.B.nbz + tz - 1) / tz);
+    }
+    const unsigned pgrid = grid_for(units, PLAN_BLOCK / 64, 1u << 16);
     if (p.nsegs > 1)
         k_plan<true><<<pgrid, PLAN_BLOCK, p.plan_lds, st>>>(p.segs, p.nsegs, p.seg0, p.brick_seg, p.nbricks, p.cap,
                                                             b.binfo, b.bcode, b.actb, b.bits, p.ctl, p.sout,
