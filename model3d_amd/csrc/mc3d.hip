@@ -483,19 +483,16 @@ struct Job {
     const mc3d_solid *s = nullptr;
     double delta = 0;
     int iters = 0;
-    Program P;
+    std::shared_ptr<const Lowered> low;  // the solid's (shared with its handle)
     std::vector<double> ax[3];
     Grid g{};    // sizes; axis pointers set by the upload
     Bricks B{};  // shape; array pointers set by the upload
     uint64_t nwords = 0, nbricks = 0, word0 = 0, brick0 = 0;
     uint32_t cap = 0;  // folded-program slot of its culling bricks (0: no culling)
-    bool host = false;
-    std::vector<uint32_t> frames;  // frame of each host-evaluated atom
     bool interior = false;         // MarchingCubesInterior: keep each vertex's inside point
     std::vector<double> conj;      // MarchingCubesConj: inverse chain applied to the vertices
     std::vector<uint64_t> keep;    // MarchingCubesFilter: cubes of accepted blocks (occupancy layout)
     const DeviceRectFilter *dfilter = nullptr;  // or the filter that makes that mask on the device
-    std::vector<PlanNode> nodes;  // the brick plan's node records of P (plan_nodes)
 };
 
 // Frame of each host-evaluated atom, by host atom index.
@@ -522,6 +519,26 @@ std::string fmt_bounds(const double *mn, const double *mx) {
 // the fixed program slots fit.
 void job_nodes(const Program &P, std::vector<PlanNode> &nodes);
 
+// The solid's lowered program: made by the first call on the handle (under
+// its lock), shared by every later call.  A failed lowering is not kept.
+int lowered(const mc3d_solid *s, std::shared_ptr<const Lowered> &out) {
+    std::lock_guard<std::mutex> g(s->low_mu);
+    if (!s->low) {
+        auto L = std::make_shared<Lowered>();
+        std::string msg;
+        if (!lower_solid(*s->ptr, L->P, msg)) {
+            set_error(msg);
+            return MC3D_E_UNSUPPORTED;
+        }
+        L->host = !L->P.host_atoms.empty();
+        L->frames = host_frames(L->P);
+        job_nodes(L->P, L->nodes);
+        s->low = std::move(L);
+    }
+    out = s->low;
+    return MC3D_OK;
+}
+
 // The occupancy grid of job j's axes (j.ax; a z slab keeps its open faces
 // in j.g) and its culling bricks.  With check, a grid past 2^32 occupancy
 // words is MC3D_E_UNSUPPORTED (single calls stream such grids in slabs).
@@ -538,9 +555,9 @@ int layout_job(Job &j, const mc3d_options &o, bool check) {
     j.cap = 0, j.nbricks = 0;
     // A host-evaluated solid may be true outside its bounds, so nothing is
     // folded by bounds: no culling.
-    if (o.cull && !j.host) {
+    if (o.cull && !j.low->host) {
         int bsz = o.brick > 0 ? o.brick : 4;
-        j.cap = std::min<uint32_t>((uint32_t)j.P.code.size(), 64u);
+        j.cap = std::min<uint32_t>((uint32_t)j.low->P.code.size(), 64u);
         for (;;) {
             j.B.BY = j.B.BZ = bsz;
             j.B.nby = (g.ny + bsz - 1) / bsz;
@@ -576,14 +593,8 @@ int prepare_job(Job &j, const mc3d_options &o, bool check = true) {
             return MC3D_E_INVALID_ARG;
         }
     }
-    std::string msg;
-    if (!lower_solid(*j.s->ptr, j.P, msg)) {
-        set_error(msg);
-        return MC3D_E_UNSUPPORTED;
-    }
-    j.host = !j.P.host_atoms.empty();
-    j.frames = host_frames(j.P);
-    job_nodes(j.P, j.nodes);
+    int rc = lowered(j.s, j.low);
+    if (rc) return rc;
     return layout_job(j, o, check);
 }
 
@@ -768,9 +779,9 @@ void layout_upload(const std::vector<Job> &jobs, uint64_t nwords_all, uint64_t n
     U.po.resize(n);
     U.nodes.resize(n);
     for (size_t s = 0; s < n; s++) {
-        const Program &P = jobs[s].P;
-        if (jobs[s].nodes.size() != P.code.size()) job_nodes(P, U.nodes[s]);
-        const std::vector<PlanNode> &nodes = U.nodes[s].empty() ? jobs[s].nodes : U.nodes[s];
+        const Program &P = jobs[s].low->P;
+        if (jobs[s].low->nodes.size() != P.code.size()) job_nodes(P, U.nodes[s]);
+        const std::vector<PlanNode> &nodes = U.nodes[s].empty() ? jobs[s].low->nodes : U.nodes[s];
         Upload::Off &po = U.po[s];
         po.code = place(P.code.size() * sizeof(Insn));
         // params padded so that staging a brick's atoms never reads past the array
@@ -796,7 +807,7 @@ void fill_upload(std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_a
     // the jobs' regions are disjoint: many jobs are filled in parallel
     auto fill = [&](size_t s) {
         Job &j = jobs[s];
-        const Program &P = j.P;
+        const Program &P = j.low->P;
         const Upload::Off &po = U.po[s];
         memcpy(h + po.code, P.code.data(), P.code.size() * sizeof(Insn));
         memset(h + po.f64, 0, (P.f64.size() + MAX_ATOM_PARAMS) * sizeof(double));
@@ -806,7 +817,7 @@ void fill_upload(std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_a
             memcpy(h + po.f32, P.f32.data(), P.f32.size() * sizeof(float));
         }
         memcpy(h + po.fr, P.frames.data(), P.frames.size() * sizeof(Frame));
-        const std::vector<PlanNode> &nodes = U.nodes[s].empty() ? j.nodes : U.nodes[s];
+        const std::vector<PlanNode> &nodes = U.nodes[s].empty() ? j.low->nodes : U.nodes[s];
         memcpy(h + po.nodes, nodes.data(), nodes.size() * sizeof(PlanNode));
         if (!U.cols[s].empty()) memcpy(h + po.col, U.cols[s].data(), U.cols[s].size() * sizeof(DevCollider));
         double *d = (double *)(h + po.ax);
@@ -863,12 +874,13 @@ void fill_upload(std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_a
 int upload_colliders(mc3d_ctx *ctx, const std::vector<Job> &jobs, Upload &U) {
     const size_t n = jobs.size();
     size_t ncol = 0;
-    for (const Job &j : jobs) ncol += j.P.colliders.size();
+    for (const Job &j : jobs) ncol += j.low->P.colliders.size();
     U.cols.assign(n, {});
     for (size_t s = 0; s < n; s++) {
-        U.cols[s].resize(jobs[s].P.colliders.size());
+        const Program &P = jobs[s].low->P;
+        U.cols[s].resize(P.colliders.size());
         for (size_t i = 0; i < U.cols[s].size(); i++) {
-            int rc = collider_on_device(ctx, jobs[s].P.colliders[i], jobs[s].P.collider_R[i], U.cols[s][i], ncol);
+            int rc = collider_on_device(ctx, P.colliders[i], P.collider_R[i], U.cols[s][i], ncol);
             if (rc) return rc;
         }
     }
@@ -917,7 +929,7 @@ constexpr uint64_t HOST_BATCH = 1u << 20;  // points per callback
 // them.  The GPU computes each atom's local points; the host only runs the
 // callbacks.
 int host_round(mc3d_ctx *ctx, const Job &j, Upload &U, size_t s, const double *pts, uint64_t n, double &ms_cb) {
-    const Program &P = j.P;
+    const Program &P = j.low->P;
     DevProg &dp = U.hsegs[s].P;
     const size_t nh = P.host_atoms.size();
     int rc;
@@ -929,7 +941,8 @@ int host_round(mc3d_ctx *ctx, const Job &j, Upload &U, size_t s, const double *p
     for (size_t h = 0; h < nh; h++) {
         for (uint64_t s0 = 0; s0 < n; s0 += HOST_BATCH) {
             const uint64_t c = std::min(HOST_BATCH, n - s0);
-            HIP_TRY(launch_host_points(dp, U.hsegs[s].g, pts, j.frames[h], s0, c, ctx->hpts.as<double>(), ctx->stream));
+            HIP_TRY(launch_host_points(dp, U.hsegs[s].g, pts, j.low->frames[h], s0, c, ctx->hpts.as<double>(),
+                                       ctx->stream));
             HIP_TRY(hipMemcpyAsync(hp, ctx->hpts.p, c * 24, hipMemcpyDeviceToHost, ctx->stream));
             // Also orders the previous batch's upload of `ho` before it is rewritten.
             HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -982,7 +995,7 @@ int plan_call(std::vector<Job> &jobs, const mc3d_options &o, Call &c) {
     c.wpb = o.waves_per_brick > 0 ? o.waves_per_brick : 4;
     c.cull = jobs[0].cap != 0;
     c.interior = c.nsegs == 1 && jobs[0].interior;
-    c.host = c.nsegs == 1 && jobs[0].host;
+    c.host = c.nsegs == 1 && jobs[0].low->host;
     for (Job &j : jobs) {
         j.word0 = c.nwords;
         c.nwords += align_up(j.nwords, 64);
@@ -1008,8 +1021,10 @@ int plan_call(std::vector<Job> &jobs, const mc3d_options &o, Call &c) {
     }
     c.plan_lds = PLAN_STACK_BYTES;
     if (c.cull)
-        for (const Job &j : jobs)
-            if (plan_stage_bytes(j.P) <= PLAN_STAGE_LIMIT) c.plan_lds = std::max(c.plan_lds, plan_stage_bytes(j.P));
+        for (const Job &j : jobs) {
+            const size_t lds = plan_stage_bytes(j.low->P);
+            if (lds <= PLAN_STAGE_LIMIT) c.plan_lds = std::max(c.plan_lds, lds);
+        }
     return MC3D_OK;
 }
 
@@ -1611,7 +1626,7 @@ int run_call(mc3d_ctx *ctx, std::vector<Job> &jobs, const mc3d_options &o, std::
         ms.triangles = m->nt;
         ms.bisection_evals = j.iters > 0 ? m->nv * (uint64_t)(j.iters + 1) : 0;
         ms.bricks_total = j.nbricks;
-        ms.program_length = j.P.code.size();
+        ms.program_length = j.low->P.code.size();
     }
     if (to_host) {  // the device outputs go back to the pool now
         ctx->pool->give(std::move(mb->verts));
@@ -1680,7 +1695,7 @@ int search_noalloc(mc3d_ctx *ctx, const mc3d_solid *s, double delta, int iters, 
     jobs[0].s = s, jobs[0].delta = delta, jobs[0].iters = iters;
     int rc = prepare_job(jobs[0], o);
     if (rc) return rc;
-    if (jobs[0].host || !jobs[0].P.colliders.empty()) {
+    if (jobs[0].low->host || !jobs[0].low->P.colliders.empty()) {
         set_error("mc3d_marching_cubes_search_noalloc: host-evaluated and collider solids need buffers of their "
                   "own; use mc3d_marching_cubes_search");
         return MC3D_E_UNSUPPORTED;
@@ -1896,10 +1911,10 @@ int search_slabs(mc3d_ctx *ctx, Job &base, const mc3d_options &o, int planes, mc
         const int c1 = std::min(ncube, c0 + planes);
         std::vector<Job> jobs(1);
         Job &j = jobs[0];
-        j.s = base.s, j.delta = base.delta, j.iters = base.iters, j.P = base.P, j.nodes = base.nodes;
+        j.s = base.s, j.delta = base.delta, j.iters = base.iters, j.low = base.low;
         j.ax[0] = base.ax[0], j.ax[1] = base.ax[1];
         j.ax[2].assign(zs.begin() + c0, zs.begin() + c1 + 1);
-        j.host = base.host, j.frames = base.frames, j.interior = base.interior, j.conj = base.conj;
+        j.interior = base.interior, j.conj = base.conj;
         j.g = Grid{};
         j.g.zlo_open = c0 > 0, j.g.zhi_open = c1 < ncube;
         if ((rc = layout_job(j, so, true))) return rc;
@@ -2260,7 +2275,7 @@ int mc3d_marching_cubes_search_batch(mc3d_ctx *ctx, const mc3d_solid *const *sol
                 return rcs[i];
             }
             Job &j = all[i];
-            if (j.host) {
+            if (j.low->host) {
                 alone.push_back(std::move(j));
                 alone_at.push_back(i);
             } else {
@@ -2699,19 +2714,14 @@ int mc3d_contains(mc3d_ctx *ctx, const mc3d_solid *s, const double *pts, size_t 
         std::vector<Job> jobs(1);
         Job &j = jobs[0];
         j.s = s;
-        std::string msg;
-        if (!lower_solid(*s->ptr, j.P, msg)) {
-            set_error(msg);
-            return MC3D_E_UNSUPPORTED;
-        }
-        j.frames = host_frames(j.P);
+        int rc = lowered(s, j.low);
+        if (rc) return rc;
         j.g = Grid{nullptr, nullptr, nullptr, 1, 1, 1, 1};
-        int rc;
         Upload U;
         if ((rc = upload(ctx, jobs, 0, 0, Bufs{}, U))) return rc;
         if ((rc = ctx->pts.ensure(n * 24)) || (rc = ctx->out8.ensure(n))) return rc;
         HIP_TRY(hipMemcpyAsync(ctx->pts.p, pts, n * 24, hipMemcpyHostToDevice, ctx->stream));
-        if (!j.P.host_atoms.empty()) {
+        if (!j.low->P.host_atoms.empty()) {
             double ms_cb = 0;
             if ((rc = host_round(ctx, j, U, 0, ctx->pts.as<double>(), n, ms_cb))) return rc;
         }

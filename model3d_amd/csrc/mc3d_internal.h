@@ -216,7 +216,22 @@ void set_error(const std::string &msg);
 
 }  // namespace mc3d
 
+namespace mc3d {
+// A solid's program as the pipeline takes it: the lowered postfix program,
+// the frame of each host-evaluated atom and the brick plan's node records.
+struct Lowered {
+    Program P;
+    std::vector<uint32_t> frames;  // frame of each host-evaluated atom
+    std::vector<PlanNode> nodes;   // the brick plan's node records of P (plan_nodes)
+    bool host = false;             // some atom is evaluated on the host
+};
+}  // namespace mc3d
+
 struct mc3d_solid {
     mc3d::SolidPtr ptr;
     int refs;
+    // The solid's lowered program, made by its first search and shared by
+    // the later ones (a solid does not change once made).
+    mutable std::mutex low_mu;
+    mutable std::shared_ptr<const mc3d::Lowered> low;
 };
