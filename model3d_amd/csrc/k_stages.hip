@@ -44,20 +44,38 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *__rest
                                                           uint2 *__restrict__ out, uint32_t *__restrict__ list,
                                                           const Seg seg0, bool bm,
                                                           unsigned long long *__restrict__ totals) {
+    static_assert(SCAN_ITEMS == 8, "a thread loads its counts as two 16-byte vectors");
     __shared__ uint4 sh[SCAN_BLOCK / 64];
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    // a thread's counts: two 16-byte loads (the buffers are 256-byte aligned)
+    // unless its items run past n
+    const bool whole = base + SCAN_ITEMS <= n;
+    uint32_t c[SCAN_ITEMS];
+    if (whole) {
+        const uint4 c0 = ((const uint4 *)(in + base))[0], c1 = ((const uint4 *)(in + base))[1];
+        c[0] = c0.x, c[1] = c0.y, c[2] = c0.z, c[3] = c0.w, c[4] = c1.x, c[5] = c1.y, c[6] = c1.z, c[7] = c1.w;
+    } else {
+        for (int i = 0; i < SCAN_ITEMS; i++) c[i] = base + i < n ? in[base + i] : 0u;
+    }
     uint4 vals[SCAN_ITEMS];
     uint4 s = make_uint4(0, 0, 0, 0);
     for (int i = 0; i < SCAN_ITEMS; i++) {
-        const uint32_t c = base + i < n ? in[base + i] : 0u;
-        vals[i] = make_uint4(c & 0xffu, (c >> 8) & 0x1ffu, c >> 17, 0u);
+        vals[i] = make_uint4(c[i] & 0xffu, (c[i] >> 8) & 0x1ffu, c[i] >> 17, 0u);
         s = add4(s, vals[i]);
     }
     uint4 off;
     const uint32_t first_part = blockIdx.x * (SCAN_TILE / CLASSIFY_BLOCK);
     if (FUSED) {
+        // the block sums before this tile, four loads in flight per thread
         uint4 a = make_uint4(0, 0, 0, 0);
-        for (uint32_t i = threadIdx.x; i < first_part; i += SCAN_BLOCK) a = add4(a, part_sums[i]);
+        for (uint32_t i0 = threadIdx.x; i0 < first_part; i0 += 4 * SCAN_BLOCK) {
+            uint4 v[4];
+            for (int k = 0; k < 4; k++) {
+                const uint32_t i = i0 + (uint32_t)k * SCAN_BLOCK;
+                v[k] = i < first_part ? part_sums[i] : make_uint4(0, 0, 0, 0);
+            }
+            for (int k = 0; k < 4; k++) a = add4(a, v[k]);
+        }
         off = block_sum<SCAN_BLOCK>(a, sh);
         __syncthreads();  // sh is reused by the scan below
     } else {
@@ -69,9 +87,18 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *__rest
         const uint4 all = add4(off, total);
         totals[0] = all.x, totals[1] = all.y, totals[2] = all.z;
     }
+    if (whole) {  // the offsets as four 16-byte stores
+        uint4 *q = (uint4 *)(out + base);
+        uint4 e = ex;
+        for (int i = 0; i < SCAN_ITEMS; i += 2) {
+            const uint4 e1 = add4(e, vals[i]);
+            q[i / 2] = make_uint4(e.x, e.y, e1.x, e1.y);
+            e = add4(e1, vals[i + 1]);
+        }
+    }
     for (int i = 0; i < SCAN_ITEMS; i++) {
         if (base + i < n) {
-            out[base + i] = make_uint2(ex.x, ex.y);
+            if (!whole) out[base + i] = make_uint2(ex.x, ex.y);
             if (list && vals[i].z) {
                 uint32_t w = (uint32_t)(base + i);
                 if (bm) {  // brick-major position -> word
