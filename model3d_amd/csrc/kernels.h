@@ -187,24 +187,13 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(const Seg *__restrict__ seg
         const DevProg &P = sg.P;
         const Grid &g = sg.g;
         const Bricks &B = sg.B;
-        // a staged program's node records in LDS
-        PlanNode *sn = (PlanNode *)(plan_smem + PLAN_STACK_BYTES);
-        if (sg.stage) {  // block-uniform
-            if (staged != si) {
-                __syncthreads();  // the previous segment's walks are done with the LDS copy
-                const uint4 *src = (const uint4 *)P.plan;
-                uint4 *dst = (uint4 *)sn;
-                for (uint32_t i = threadIdx.x; i < P.ncode * (sizeof(PlanNode) / 16); i += blockDim.x) dst[i] = src[i];
-                __syncthreads();
-                staged = si;
-            }
-        }
         const uint64_t b = gb - sg.brick0;  // local brick
         valid = valid && b < sg.nbricks;
         bool active = false, full = false;
         uint32_t len = 0;
         // the brick's box (a lane without a brick takes the first brick's
-        // and keeps its output slot empty)
+        // and keeps its output slot empty), loaded before the program is
+        // staged so that the two loads overlap
         const uint64_t bl = valid ? b : 0;
         const uint32_t r = (uint32_t)bl / (uint32_t)g.W;
         const int bx = (int)((uint32_t)bl - r * (uint32_t)g.W);
@@ -213,6 +202,26 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_plan(const Seg *__restrict__ seg
         const int x1 = min(x0 + 64, g.nx - 1), y1 = min(y0 + B.BY, g.ny - 1), z1 = min(z0 + B.BZ, g.nz - 1);
         const double lo[3] = {g.xs[x0], g.ys[y0], g.zs[z0]};
         const double hi[3] = {g.xs[x1], g.ys[y1], g.zs[z1]};
+        // a staged program's node records in LDS
+        PlanNode *sn = (PlanNode *)(plan_smem + PLAN_STACK_BYTES);
+        if (sg.stage) {  // block-uniform
+            if (staged != si) {
+                __syncthreads();  // the previous segment's walks are done with the LDS copy
+                const uint4 *src = (const uint4 *)P.plan;
+                uint4 *dst = (uint4 *)sn;
+                // copied by the LDS DMA (16 bytes a lane, 1 KiB a wave per
+                // load, no registers), all loads in flight before the wait
+                const uint32_t nq = P.ncode * (uint32_t)(sizeof(PlanNode) / 16);
+                for (uint32_t c = (uint32_t)wave; c * 64 < nq; c += WPB)
+                    if (c * 64 + (uint32_t)lane < nq)
+                        __builtin_amdgcn_global_load_lds(
+                            (const void __attribute__((address_space(1))) *)(src + c * 64 + lane),
+                            (void __attribute__((address_space(3))) *)(dst + c * 64), 16, 0, 0);
+                __builtin_amdgcn_s_waitcnt(VMCNT0);
+                __syncthreads();
+                staged = si;
+            }
+        }
         Insn *o = out + (valid ? gb * cap : 0);
         const uint32_t ocap = valid ? cap : 0u;
         uint32_t n = 0;
