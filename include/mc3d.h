@@ -123,6 +123,9 @@ mc3d_solid *mc3d_collider_solid_hollow(const double *triangles, size_t n_triangl
 typedef int (*mc3d_contains_fn)(void *user, const double *pts, size_t n, uint8_t *out);
 mc3d_solid *mc3d_host_solid(const double min[3], const double max[3], mc3d_contains_fn fn, void *user);
 
+/* A solid does not change once made: its first search lowers it to the
+ * device program, which the handle keeps (and shares with later calls, from
+ * any thread) until it is released. */
 mc3d_solid *mc3d_solid_retain(mc3d_solid *s);
 void mc3d_solid_release(mc3d_solid *s);
 /* Solid.Min() / Solid.Max() with the reference's float64 arithmetic. */
