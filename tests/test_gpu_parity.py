@@ -154,6 +154,70 @@ def _search_on(ctx, solid, delta, iters):
         b.release(h)
 
 
+def test_solid_handle_reused_across_calls_and_threads():
+    """A handle keeps its lowered program after the first search: later calls
+    on it, from one thread or from four at once (each on its own context),
+    give the oracle's mesh bit for bit."""
+    import ctypes
+    import threading
+
+    from model3d_amd import _lib
+    from model3d_amd.solids import LibBackend
+
+    s = workloads.csg64_solid(2)
+    delta, iters = 0.03, 8
+    print("csg64 d=0.03, one handle, serial then 4 threads", flush=True)
+    want, _ = oracle_search(s, delta, iters, threads=8)
+    L = _lib.lib()
+    b = LibBackend()
+    h = s._build(b)
+
+    def search(ctx):
+        out = ctypes.c_void_p()
+        opts = _lib.mc3d_options()
+        opts.precision = FP64
+        opts.cull = 1
+        try:
+            _lib.check(L.mc3d_marching_cubes_search(ctx, h, delta, iters, ctypes.byref(opts), ctypes.byref(out)))
+            nv, nt = ctypes.c_size_t(), ctypes.c_size_t()
+            L.mc3d_mesh_counts(out, ctypes.byref(nv), ctypes.byref(nt))
+            v, t = np.empty((nv.value, 3)), np.empty((nt.value, 3), dtype=np.uint32)
+            _lib.check(L.mc3d_mesh_copy(out, v.ctypes.data, t.ctypes.data))
+            return v[t.astype(np.int64)]
+        finally:
+            if out.value:
+                L.mc3d_mesh_free(out)
+
+    ctxs = []
+    try:
+        for _ in range(3):
+            assert same_triangles(search(_lib.context(0)), want)
+        for _ in range(4):
+            c = ctypes.c_void_p()
+            _lib.check(L.mc3d_ctx_create(0, ctypes.byref(c)))
+            ctxs.append(c)
+        got, errs = [None] * 4, []
+
+        def run(i):
+            try:
+                got[i] = search(ctxs[i])
+            except Exception as e:  # reported below, on the test's thread
+                errs.append(e)
+
+        threads = [threading.Thread(target=run, args=(i,)) for i in range(4)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(timeout=120)
+        assert not errs, errs
+        for g in got:
+            assert g is not None and same_triangles(g, want)
+    finally:
+        for c in ctxs:
+            L.mc3d_ctx_destroy(c)
+        b.release(h)
+
+
 def test_output_capacity_growth_and_reuse():
     """A fresh context starts with small output buffers: the first large mesh
     overflows them and the output stages run again; later calls reuse them."""
