@@ -250,6 +250,9 @@ hipError_t launch_classify(const Pipe &p, const Bufs &b, hipStream_t st) {
     } else if (p.nsegs > 1) {
         k_classify<true><<<ctiles, CLASSIFY_BLOCK, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits, b.table,
                                                             b.counts, b.tiles, nullptr, p.nwords, false);
+    } else if (p.bm && p.nscan >= CLASSIFY_SHFL_MIN) {  // large brick-major grids: +y / +z words by shuffles
+        k_classify<false, true><<<ctiles, CLASSIFY_BLOCK, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits,
+                                                                   b.table, b.counts, b.tiles, b.keep, p.nscan, p.bm);
     } else {
         k_classify<false><<<ctiles, CLASSIFY_BLOCK, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits, b.table,
                                                              b.counts, b.tiles, b.keep, p.nscan, p.bm);

@@ -611,6 +611,50 @@ __device__ __forceinline__ Nbhd load_nbhd(const Grid &g, const uint64_t *__restr
     return r;
 }
 
+// load_nbhd for a one-solid call in brick-major order: a wave's 64 lanes
+// hold consecutive positions, so the rows y + 1 and z + 1 of a brick column
+// are lanes + 1 and + BY of the same wave.  Each lane loads its word and
+// its +x neighbour; the +y / +z words come from those lanes by shuffles,
+// and only lanes on a brick's last row or plane (or the wave's end) load
+// them.  Every lane of the wave takes part (has: the position holds a word).
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src), hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ Nbhd load_nbhd_bm(const Grid &g, const Bricks &B, const uint64_t *__restrict__ bits,
+                                             int wx, int y, int z, bool has, uint64_t pos) {
+    const int lane = threadIdx.x & 63;
+    const bool hx = wx + 1 < g.W, hy = y + 1 < g.ny, hz = z + 1 < g.nz;
+    const size_t W = (size_t)g.W, plane = (size_t)g.ny * g.W;
+    const size_t w = has ? word_index(g, wx, y, z) : 0;
+    // lanes + 1 / + BY hold rows y + 1 / z + 1 of this brick column
+    const uint32_t jy = (uint32_t)pos & (uint32_t)(B.BY - 1), jz = (uint32_t)(pos / (uint32_t)B.BY) & (uint32_t)(B.BZ - 1);
+    const bool sy = jy + 1 < (uint32_t)B.BY && lane + 1 < 64, sz = jz + 1 < (uint32_t)B.BZ && lane + B.BY < 64;
+    Nbhd r;
+    for (int k = 0; k < 8; k++) r.n[k] = 0;
+    if (has) {
+        r.n[0] = bits[w];
+        if (hx) r.n[1] = bits[w + 1];
+        if (hy && !sy) r.n[2] = bits[w + W];
+        if (hy && hx && !sy) r.n[3] = bits[w + W + 1];
+        if (hz && !sz) r.n[4] = bits[w + plane];
+        if (hz && hx && !sz) r.n[5] = bits[w + plane + 1];
+        if (hy && hz && !(sy && sz)) r.n[6] = bits[w + W + plane];
+        if (hx && hy && hz && !(sy && sz)) r.n[7] = bits[w + W + plane + 1];
+    }
+    const int ly = min(lane + 1, 63), lz = min(lane + B.BY, 63), lyz = min(lane + B.BY + 1, 63);
+    const uint64_t a2 = shfl64(r.n[0], ly), a3 = shfl64(r.n[1], ly), a4 = shfl64(r.n[0], lz);
+    const uint64_t a5 = shfl64(r.n[1], lz), a6 = shfl64(r.n[0], lyz), a7 = shfl64(r.n[1], lyz);
+    if (hy && sy) r.n[2] = a2;
+    if (hy && hx && sy) r.n[3] = a3;
+    if (hz && sz) r.n[4] = a4;
+    if (hz && hx && sz) r.n[5] = a5;
+    if (hy && hz && sy && sz) r.n[6] = a6;
+    if (hx && hy && hz && sy && sz) r.n[7] = a7;
+    r.zin = hz;
+    return r;
+}
+
 // Crossed-edge masks of word n[0]: bit i of cx is the edge from corner
 // (64wx+i, y, z) to its +x neighbour, and likewise for y and z.  Padding
 // bits (x >= nx) and corners on the grid's faces are zero, so no checks
@@ -701,7 +745,7 @@ __device__ __forceinline__ bool bm_word(const Grid &g, const Bricks &B, uint32_t
 // triangles per word), and in `word` the position's word as the list of
 // emitting words names it.  Position gw is an occupancy word, or with `bm`
 // (one solid, culled) the word at brick-major position gw.
-template <bool MULTI>
+template <bool MULTI, bool SHFL = false>
 __device__ __forceinline__ uint4 classify_position(const Seg *__restrict__ segs, const Seg &seg0,
                                                    const uint32_t *__restrict__ word_seg,
                                                    const uint64_t *__restrict__ bits_all, const uint8_t *ntri,
@@ -714,17 +758,25 @@ __device__ __forceinline__ uint4 classify_position(const Seg *__restrict__ segs,
     uint64_t w = gw - sg.word0;
     int wx = 0, y = 0, z = 0;
     bool has;
+    Nbhd h;
     if (MULTI || !bm) {
         has = w < sg.nwords;  // else alignment padding
-        if (has) word_coords(g, w, wx, y, z);
+        if (!has) return make_uint4(0, 0, 0, 0);
+        word_coords(g, w, wx, y, z);
         word = (uint32_t)gw;
+        h = load_nbhd(g, bits, wx, y, z);
     } else {
         has = bm_word(g, sg.B, (uint32_t)gw, wx, y, z);
         w = word_index(g, wx, y, z);
         word = (uint32_t)w;
+        if (SHFL) {  // (positions are padded to whole waves)
+            h = load_nbhd_bm(g, sg.B, bits, wx, y, z, has, gw);
+            if (!has) return make_uint4(0, 0, 0, 0);
+        } else {
+            if (!has) return make_uint4(0, 0, 0, 0);
+            h = load_nbhd(g, bits, wx, y, z);
+        }
     }
-    if (!has) return make_uint4(0, 0, 0, 0);
-    const Nbhd h = load_nbhd(g, bits, wx, y, z);
     Masks m = nbhd_masks(h);
     const CubeCorners cc = nbhd_corners(h);
     uint64_t mixed = mixed_cubes(cc);
@@ -747,7 +799,7 @@ __device__ __forceinline__ uint4 classify_position(const Seg *__restrict__ segs,
 // anything) << 17, a position per thread, and per block (CLASSIFY_BLOCK
 // positions) the block's sums {edges, triangles, emitting words, 0} for the
 // scan: part_sums[blockIdx.x] (npos: the positions).
-template <bool MULTI>
+template <bool MULTI, bool SHFL = false>
 __global__ __launch_bounds__(CLASSIFY_BLOCK) void k_classify(const Seg *__restrict__ segs, uint32_t nsegs, const Seg seg0,
                                                          const uint32_t *__restrict__ word_seg,
                                                          const uint64_t *__restrict__ bits_all,
@@ -764,7 +816,7 @@ __global__ __launch_bounds__(CLASSIFY_BLOCK) void k_classify(const Seg *__restri
     const uint64_t gw = (uint64_t)blockIdx.x * CLASSIFY_BLOCK + threadIdx.x;
     if (gw < npos) {  // wave-uniform: npos is a multiple of 64
         uint32_t word;
-        c = classify_position<MULTI>(segs, seg0, word_seg, bits_all, ntri, keep_all, bm, gw, word);
+        c = classify_position<MULTI, SHFL>(segs, seg0, word_seg, bits_all, ntri, keep_all, bm, gw, word);
         counts[gw] = c.x | (c.y << 8) | (c.z << 17);
     }
     const uint4 t = block_sum<CLASSIFY_BLOCK>(c, sh);

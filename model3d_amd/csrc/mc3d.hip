@@ -1017,7 +1017,8 @@ int plan_call(std::vector<Job> &jobs, const mc3d_options &o, Call &c) {
     const int BY = jobs[0].B.BY, BZ = jobs[0].B.BZ;
     if (c.cull && c.nsegs == 1 && !c.host && (BY & (BY - 1)) == 0 && (BZ & (BZ - 1)) == 0) {
         const uint64_t n = c.nbricks * (uint64_t)BY * (uint64_t)BZ;
-        if (n < (1ull << 32)) c.bm = true, c.nscan = n;
+        // padded to whole waves: classify reads a wave's rows from its own lanes
+        if (n < (1ull << 32) - 64) c.bm = true, c.nscan = align_up(n, 64);
     }
     c.plan_lds = PLAN_STACK_BYTES;
     if (c.cull)

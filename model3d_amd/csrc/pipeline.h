@@ -103,6 +103,10 @@ constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
 constexpr int CLASSIFY_BLOCK = 1024;  // words per classify block and per block sum
 constexpr int SUMS_BLOCK = 1024;      // threads of the one block scanning the block sums
 constexpr int SCAN_FUSE_PARTS = 4096; // up to this many block sums, each scan tile adds its prefix itself
+// From this many brick-major positions classify takes the +y / +z words
+// from neighbouring lanes (the 1G-corner tree 298 -> 223 us; below it the
+// shuffles cost more than the loads they save: csg64 21.7 -> 22.6 us).
+constexpr uint64_t CLASSIFY_SHFL_MIN = 1ull << 22;
 constexpr int SCAN_ONE_PASS = 64;     // up to this many classify blocks, classify scans its counts itself
 constexpr uint32_t SCAN_SPIN_LIMIT = 1u << 22;  // rounds of waiting after which a block gives up
 static_assert(SCAN_TILE % CLASSIFY_BLOCK == 0, "a scan tile starts at a classify block");
