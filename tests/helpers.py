@@ -17,6 +17,18 @@ def canonical(tris) -> np.ndarray:
     return a[np.lexsort(a.T[::-1])]
 
 
+def triangle_checksum(tris) -> int:
+    """An order-independent checksum of a triangle multiset (each triangle's
+    9 float64 bit patterns, in their order, mixed into 64 bits; the sum of
+    those over the triangles): for meshes too large to sort."""
+    a = np.ascontiguousarray(np.asarray(tris, dtype=np.float64).reshape(-1, 9)).view(np.uint64)
+    h = np.zeros(len(a), dtype=np.uint64)
+    for k in range(9):
+        h = (h ^ a[:, k]) * np.uint64(0x9E3779B97F4A7C15 + 2 * k)
+        h ^= h >> np.uint64(31)
+    return int(h.sum(dtype=np.uint64))
+
+
 def same_triangles(a, b) -> bool:
     ca, cb = canonical(a), canonical(b)
     return ca.shape == cb.shape and bool(np.array_equal(ca, cb))

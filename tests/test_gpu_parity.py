@@ -11,7 +11,7 @@ import pytest
 import model3d_amd as m3
 from model3d_amd import workloads
 from oracle.binding import oracle_contains, oracle_search
-from tests.helpers import canonical, edge_topology, same_triangles, solid_catalogue, volume
+from tests.helpers import canonical, edge_topology, same_triangles, solid_catalogue, triangle_checksum, volume
 
 pytestmark = pytest.mark.gpu
 
@@ -357,6 +357,14 @@ def test_large_workload_properties():
     key.sort()
     assert np.all(key[1:] != key[:-1])  # every directed edge once
     assert np.all(np.isin(rev, key))  # and its reverse is there
+    # The same triangles through the other classify paths: z slabs (each
+    # slab small enough for per-lane loads) and the unculled word order.
+    want = triangle_checksum(b.triangles())
+    print("large: z slabs of 125 layers, then unculled", flush=True)
+    slabs = m3.MarchingCubesSearch(s, c["delta"], c["iters"], precision=FP64, slab_planes=125)
+    assert len(slabs.faces) == len(b.faces) and triangle_checksum(slabs.triangles()) == want
+    whole = m3.MarchingCubesSearch(s, c["delta"], c["iters"], precision=FP64, cull=False)
+    assert len(whole.faces) == len(b.faces) and triangle_checksum(whole.triangles()) == want
 
 
 def test_go_shim_call_sequence():
