@@ -413,11 +413,14 @@ __global__ __launch_bounds__(64 * WPB) void k_corners_act(const Seg *__restrict_
                                                           uint64_t *__restrict__ bits_all, Ctl *__restrict__ ctl,
                                                           SegOut *__restrict__ sout) {
     const int lane = threadIdx.x & 63, wave = wave_id();
+    // the block's first list entry loaded beside the count (the list holds
+    // one entry per brick and the grid has at most that many blocks)
+    const uint32_t gb0 = scalar_ptr(act)[blockIdx.x];
     const uint32_t nact = ctl->nact_bricks;
     __shared__ RowInfo row_info[WPB][64];  // each wave's rows of the current brick
     GStats gs;
     for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x) {
-        const uint32_t gb = scalar_ptr(act)[k];
+        const uint32_t gb = k == blockIdx.x ? gb0 : scalar_ptr(act)[k];
         const uint32_t si = wave_seg<MULTI>(brick_seg, gb, SEG_BRICK_SHIFT);
         const Seg sg = pick_seg<MULTI>(segs, seg0, si);
         const DevProg &P = sg.P;
@@ -980,6 +983,8 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
     __shared__ uint32_t s_pre[4][64];
     __shared__ uint16_t s_cube[4][64];
     __shared__ uint8_t s_cs[4][64];
+    // the listed words' count, loaded beside the table
+    const uint32_t nact = (uint32_t)ctl->totals[2];
     // (table: the case table with each edge decoded to axis | dx << 2 |
     // dy << 3 | dz << 4, its offset from the cube's low corner)
     for (int i = threadIdx.x; i < 256; i += blockDim.x)  // 16 bytes a thread
@@ -990,7 +995,6 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
     GroupWord *gwd = gw_all[wave];
     uint64_t *mixed = mixed_all[wave];
     uint32_t *vpre = vpre_all[wave], *mpre = mpre_all[wave];
-    const uint32_t nact = (uint32_t)ctl->totals[2];
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
     bool over = false;
     const uint32_t kstep = gridDim.x * 4 * EMIT_G;
@@ -1181,7 +1185,7 @@ __device__ __forceinline__ bool corner_bit(const Grid &g, const uint64_t *__rest
 // that brick's program decides every midpoint.
 template <int MODE, bool STATS>
 __device__ __forceinline__ void bisect_vertex(const Seg &sg, const uint64_t *__restrict__ bits_all,
-                                              const uint64_t *__restrict__ vkeys, double *__restrict__ verts,
+                                              const uint64_t key, double *__restrict__ verts,
                                               double *__restrict__ interior, uint64_t v, GStats &gs) {
     const DevProg &P = sg.P;
     const Grid &g = sg.g;
@@ -1189,7 +1193,6 @@ __device__ __forceinline__ void bisect_vertex(const Seg &sg, const uint64_t *__r
     const uint64_t *bits = bits_all + sg.word0;
     const int iters = sg.iters;
     {
-        const uint64_t key = vkeys[v];
         const int axis = (int)(key & 3);
         const int x = (int)((key >> 2) & 0xfffff), y = (int)((key >> 22) & 0xfffff), z = (int)((key >> 42) & 0xfffff);
         double p[3] = {g.xs[x], g.ys[y], g.zs[z]};
@@ -1266,15 +1269,20 @@ __global__ __launch_bounds__(256) void k_bisect(const Seg *__restrict__ segs, ui
                                                 const uint64_t *__restrict__ vkeys, double *__restrict__ verts,
                                                 double *__restrict__ interior, Ctl *__restrict__ ctl, uint64_t vcap) {
     GStats gs;
+    // the first vertex's key (and solid) loaded beside the vertex count: the
+    // arrays hold vcap entries, so the loads stay in bounds
+    const uint64_t v0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t key0 = v0 < vcap ? vkeys[v0] : 0;
+    const uint32_t seg0v = MULTI && v0 < vcap ? vseg[v0] : 0u;
     const uint64_t nv = min((uint64_t)ctl->totals[0], vcap);
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
-         v += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint64_t v = v0; v < nv; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t key = v == v0 ? key0 : vkeys[v];
         // Lanes may belong to different solids: take them one solid at a time.
-        const uint32_t mine = MULTI ? vseg[v] : 0u;
+        const uint32_t mine = MULTI ? (v == v0 ? seg0v : vseg[v]) : 0u;
         for (;;) {
             const uint32_t si = (uint32_t)__builtin_amdgcn_readfirstlane((int)mine);
             if (mine != si) continue;
-            bisect_vertex<MODE, STATS>(pick_seg<MULTI>(segs, seg0, si), bits_all, vkeys, verts, interior, v, gs);
+            bisect_vertex<MODE, STATS>(pick_seg<MULTI>(segs, seg0, si), bits_all, key, verts, interior, v, gs);
             break;
         }
     }
