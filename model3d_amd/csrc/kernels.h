@@ -631,7 +631,8 @@ __device__ __forceinline__ Nbhd load_nbhd_bm(const Grid &g, const Bricks &B, con
     const size_t W = (size_t)g.W, plane = (size_t)g.ny * g.W;
     const size_t w = has ? word_index(g, wx, y, z) : 0;
     // lanes + 1 / + BY hold rows y + 1 / z + 1 of this brick column
-    const uint32_t jy = (uint32_t)pos & (uint32_t)(B.BY - 1), jz = (uint32_t)(pos / (uint32_t)B.BY) & (uint32_t)(B.BZ - 1);
+    const uint32_t jy = (uint32_t)pos & (uint32_t)(B.BY - 1);
+    const uint32_t jz = (uint32_t)(pos / (uint32_t)B.BY) & (uint32_t)(B.BZ - 1);
     const bool sy = jy + 1 < (uint32_t)B.BY && lane + 1 < 64, sz = jz + 1 < (uint32_t)B.BZ && lane + B.BY < 64;
     Nbhd r;
     for (int k = 0; k < 8; k++) r.n[k] = 0;
