@@ -42,7 +42,8 @@ template <bool FUSED>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *__restrict__ in, uint64_t n,
                                                           const uint4 *__restrict__ part_sums,
                                                           uint2 *__restrict__ out, uint32_t *__restrict__ list,
-                                                          const Seg seg0, bool bm,
+                                                          const Seg seg0, const Seg *__restrict__ segs,
+                                                          const uint32_t *__restrict__ pos_seg, bool bm,
                                                           unsigned long long *__restrict__ totals) {
     static_assert(SCAN_ITEMS == 8, "a thread loads its counts as two 16-byte vectors");
     __shared__ uint4 sh[SCAN_BLOCK / 64];
@@ -101,10 +102,11 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *__rest
             if (!whole) out[base + i] = make_uint2(ex.x, ex.y);
             if (list && vals[i].z) {
                 uint32_t w = (uint32_t)(base + i);
-                if (bm) {  // brick-major position -> word
+                if (bm) {  // brick-major position -> word (of its solid, pos_seg with several)
+                    const Seg &sg = pos_seg ? segs[pos_seg[w >> SEG_WORD_SHIFT]] : seg0;
                     int wx, y, z;
-                    bm_word(seg0.g, seg0.B, w, wx, y, z);
-                    w = (uint32_t)word_index(seg0.g, wx, y, z);
+                    bm_word(sg.g, sg.B, (uint32_t)(w - sg.pos0), wx, y, z);
+                    w = (uint32_t)(sg.word0 + word_index(sg.g, wx, y, z));
                 }
                 list[ex.z] = w;
             }
@@ -117,7 +119,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *__rest
 __global__ __launch_bounds__(256) void k_seg_ranges(const Seg *__restrict__ segs, uint32_t nsegs,
                                                     const uint2 *__restrict__ offs, SegOut *__restrict__ sout) {
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nsegs; s += gridDim.x * blockDim.x) {
-        const uint2 o = offs[segs[s].word0];
+        const uint2 o = offs[segs[s].pos0];
         sout[s].vbase = o.x;
         sout[s].tbase = o.y;
     }
@@ -240,16 +242,21 @@ hipError_t launch_classify(const Pipe &p, const Bufs &b, hipStream_t st) {
     const unsigned ctiles = (unsigned)((p.nscan + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK);
     if (p.scan_one_pass) {  // small grids: the scan too
         if (p.nsegs > 1)
-            k_classify_scan<true><<<ctiles, CLASSIFY_BLOCK, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits,
-                                                                     b.table, nullptr, p.nwords, false, b.agg, b.offs,
+            k_classify_scan<true><<<ctiles, CLASSIFY_BLOCK, 0, st>>>(p.segs, p.nsegs, p.seg0, p.pos_seg, b.bits,
+                                                                     b.table, nullptr, p.nscan, p.bm, b.agg, b.offs,
                                                                      b.actw, p.ctl);
         else
             k_classify_scan<false><<<ctiles, CLASSIFY_BLOCK, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits,
                                                                       b.table, b.keep, p.nscan, p.bm, b.agg, b.offs,
                                                                       b.actw, p.ctl);
     } else if (p.nsegs > 1) {
-        k_classify<true><<<ctiles, CLASSIFY_BLOCK, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits, b.table,
-                                                            b.counts, b.tiles, nullptr, p.nwords, false);
+        if (p.bm && p.nscan >= CLASSIFY_SHFL_MIN)
+            k_classify<true, true><<<ctiles, CLASSIFY_BLOCK, 0, st>>>(p.segs, p.nsegs, p.seg0, p.pos_seg, b.bits,
+                                                                      b.table, b.counts, b.tiles, nullptr, p.nscan,
+                                                                      p.bm);
+        else
+            k_classify<true><<<ctiles, CLASSIFY_BLOCK, 0, st>>>(p.segs, p.nsegs, p.seg0, p.pos_seg, b.bits,
+                                                                b.table, b.counts, b.tiles, nullptr, p.nscan, p.bm);
     } else if (p.bm && p.nscan >= CLASSIFY_SHFL_MIN) {  // large brick-major grids: +y / +z words by shuffles
         k_classify<false, true><<<ctiles, CLASSIFY_BLOCK, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits,
                                                                    b.table, b.counts, b.tiles, b.keep, p.nscan, p.bm);
@@ -265,14 +272,15 @@ hipError_t launch_scan(const Pipe &p, const Bufs &b, hipStream_t st) {
     // classify itself on small grids)
     const uint64_t ntiles = (p.nscan + SCAN_TILE - 1) / SCAN_TILE;
     const uint64_t nparts = (p.nscan + CLASSIFY_BLOCK - 1) / CLASSIFY_BLOCK;
+    const uint32_t *multi_pos = p.nsegs > 1 ? p.pos_seg : nullptr;  // positions -> solids
     if (p.scan_one_pass) {
     } else if (nparts <= SCAN_FUSE_PARTS) {  // each tile sums the block sums before it
         k_scan_down<true><<<(unsigned)ntiles, SCAN_BLOCK, 0, st>>>(b.counts, p.nscan, b.tiles, b.offs, b.actw, p.seg0,
-                                                                  p.bm, p.ctl->totals);
+                                                                  p.segs, multi_pos, p.bm, p.ctl->totals);
     } else {
         k_scan_tiles<<<1, SUMS_BLOCK, 0, st>>>(b.tiles, nparts, p.ctl->totals);
         k_scan_down<false><<<(unsigned)ntiles, SCAN_BLOCK, 0, st>>>(b.counts, p.nscan, b.tiles, b.offs, b.actw,
-                                                                   p.seg0, p.bm, nullptr);
+                                                                   p.seg0, p.segs, multi_pos, p.bm, nullptr);
     }
     if (p.nsegs > 1)  // a single solid starts at vertex 0 and triangle 0 (zeroed by the upload)
         k_seg_ranges<<<grid_for(p.nsegs, 256), 256, 0, st>>>(p.segs, p.nsegs, b.offs, p.sout);
@@ -285,7 +293,7 @@ hipError_t launch_emit(const Pipe &p, const Bufs &b, uint32_t *tris, uint64_t vc
     const unsigned grid = std::max(512u, grid_for(p.nwords, p.emit_div, 4096));
     if (p.nsegs > 1)
         k_emit<true><<<grid, 256, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits, b.etable, b.offs, b.actw,
-                                           p.ctl, b.vkeys, b.vseg, tris, nullptr, vcap, tcap, false);
+                                           p.ctl, b.vkeys, b.vseg, tris, nullptr, vcap, tcap, p.bm);
     else
         k_emit<false><<<grid, 256, 0, st>>>(p.segs, p.nsegs, p.seg0, p.word_seg, b.bits, b.etable, b.offs, b.actw,
                                             p.ctl, b.vkeys, nullptr, tris, b.keep, vcap, tcap, p.bm);

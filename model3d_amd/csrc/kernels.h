@@ -625,7 +625,7 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
     return ((uint64_t)hi << 32) | lo;
 }
 __device__ __forceinline__ Nbhd load_nbhd_bm(const Grid &g, const Bricks &B, const uint64_t *__restrict__ bits,
-                                             int wx, int y, int z, bool has, uint64_t pos) {
+                                             int wx, int y, int z, bool has, uint64_t pos) {  // pos: within the solid
     const int lane = threadIdx.x & 63;
     const bool hx = wx + 1 < g.W, hy = y + 1 < g.ny, hz = z + 1 < g.nz;
     const size_t W = (size_t)g.W, plane = (size_t)g.ny * g.W;
@@ -755,6 +755,7 @@ __device__ __forceinline__ uint4 classify_position(const Seg *__restrict__ segs,
                                                    const uint64_t *__restrict__ bits_all, const uint8_t *ntri,
                                                    const uint64_t *__restrict__ keep_all, bool bm, uint64_t gw,
                                                    uint32_t &word) {
+    // (with bm, word_seg is the table of scan positions)
     const uint32_t si = wave_seg<MULTI>(word_seg, gw, SEG_WORD_SHIFT);
     const Seg sg = pick_seg<MULTI>(segs, seg0, si);
     const Grid &g = sg.g;
@@ -763,18 +764,18 @@ __device__ __forceinline__ uint4 classify_position(const Seg *__restrict__ segs,
     int wx = 0, y = 0, z = 0;
     bool has;
     Nbhd h;
-    if (MULTI || !bm) {
+    if (!bm) {
         has = w < sg.nwords;  // else alignment padding
         if (!has) return make_uint4(0, 0, 0, 0);
         word_coords(g, w, wx, y, z);
         word = (uint32_t)gw;
         h = load_nbhd(g, bits, wx, y, z);
-    } else {
-        has = bm_word(g, sg.B, (uint32_t)gw, wx, y, z);
+    } else {  // the solid's brick-major position gw - pos0
+        has = bm_word(g, sg.B, (uint32_t)(gw - sg.pos0), wx, y, z);
         w = word_index(g, wx, y, z);
-        word = (uint32_t)w;
+        word = (uint32_t)(sg.word0 + w);
         if (SHFL) {  // (positions are padded to whole waves)
-            h = load_nbhd_bm(g, sg.B, bits, wx, y, z, has, gw);
+            h = load_nbhd_bm(g, sg.B, bits, wx, y, z, has, gw - sg.pos0);
             if (!has) return make_uint4(0, 0, 0, 0);
         } else {
             if (!has) return make_uint4(0, 0, 0, 0);
@@ -1011,17 +1012,20 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
                 const uint32_t gw = active[k0 + g];
                 const uint32_t si = MULTI ? word_seg[gw >> SEG_WORD_SHIFT] : 0u;
                 Grid g_;
-                uint64_t word0;
+                Bricks B_{};
+                uint64_t word0, pos0;
                 if (MULTI) {
                     const Seg *sp = &segs[si];
                     g_.nx = sp->g.nx, g_.ny = sp->g.ny, g_.nz = sp->g.nz, g_.W = sp->g.W;
-                    word0 = sp->word0;
+                    if (bm) B_.BY = sp->B.BY, B_.BZ = sp->B.BZ, B_.nby = sp->B.nby;
+                    word0 = sp->word0, pos0 = sp->pos0;
                 } else {
                     g_ = seg0.g;
-                    word0 = seg0.word0;
+                    B_ = seg0.B;
+                    word0 = seg0.word0, pos0 = seg0.pos0;
                 }
                 const uint64_t *bits = bits_all + word0;
-                const uint2 *offs = offs_all + word0;
+                const uint2 *offs = offs_all + pos0;  // the solid's scan positions
                 int wx, y, z;
                 word_coords(g_, gw - word0, wx, y, z);
                 const int ex = wx + (e & 1), ey = y + ((e >> 1) & 1), ez = z + (e >> 2);
@@ -1036,7 +1040,7 @@ __global__ __launch_bounds__(256) void k_emit(const Seg *__restrict__ segs, uint
                 const uint64_t b = bits[idx];
                 const uint64_t bn = bits[idx + (hx ? 1u : 0u)], by = bits[idx + (hy ? W : 0u)];
                 const uint64_t bz = bits[idx + (hz ? plane : 0u)];
-                const uint2 o = offs[(!MULTI && bm) ? bm_index(g_, seg0.B, cx, cy, cz) : idx];
+                const uint2 o = offs[bm ? bm_index(g_, B_, cx, cy, cz) : idx];
                 if (in) {
                     r.bits = b;
                     // (the top plane, also of a z slab: no z edges)

@@ -37,6 +37,7 @@ struct Pipe {
     uint32_t nsegs = 0;
     Seg seg0{};
     const uint32_t *word_seg = nullptr, *brick_seg = nullptr;  // segment tables (calls of several)
+    const uint32_t *pos_seg = nullptr;  // segment per 64 scan positions (word_seg unless brick-major)
     Ctl *ctl = nullptr;
     SegOut *sout = nullptr;
     uint64_t nwords = 0, nbricks = 0, nscan = 0;

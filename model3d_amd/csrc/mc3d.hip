@@ -488,6 +488,7 @@ struct Job {
     Grid g{};    // sizes; axis pointers set by the upload
     Bricks B{};  // shape; array pointers set by the upload
     uint64_t nwords = 0, nbricks = 0, word0 = 0, brick0 = 0;
+    uint64_t pos0 = 0, npos = 0;  // scan positions (npos 0: the scan takes the words, pos0 = word0)
     uint32_t cap = 0;  // folded-program slot of its culling bricks (0: no culling)
     bool interior = false;         // MarchingCubesInterior: keep each vertex's inside point
     std::vector<double> conj;      // MarchingCubesConj: inverse chain applied to the vertices
@@ -729,7 +730,8 @@ void plan_nodes(const Program &P, const std::vector<AtomBox> &boxes, std::vector
 // device address once placed.
 struct Upload {
     // layout (bytes from the blob's start)
-    size_t total = 0, o_sout = 0, o_segs = 0, o_wseg = 0, o_bseg = 0, o_conj = 0;
+    size_t total = 0, o_sout = 0, o_segs = 0, o_wseg = 0, o_bseg = 0, o_pseg = 0, o_conj = 0;
+    uint64_t npos_all = 0;  // brick-major scan positions of a call of several (0: none)
     struct Off {
         size_t code, f64, f32, fr, nodes, col, ax;
     };
@@ -741,7 +743,7 @@ struct Upload {
     Ctl *ctl = nullptr;
     SegOut *sout = nullptr;
     Seg *segs = nullptr;
-    uint32_t *word_seg = nullptr, *brick_seg = nullptr;
+    uint32_t *word_seg = nullptr, *brick_seg = nullptr, *pos_seg = nullptr;
     std::vector<Seg> hsegs;  // host copies of the segments
     size_t back_bytes = 0;   // control block + per-solid results, copied back at the end
     const double *conj = nullptr;  // job 0's MarchingCubesConj inverse chain (records of 10 doubles)
@@ -775,6 +777,9 @@ void layout_upload(const std::vector<Job> &jobs, uint64_t nwords_all, uint64_t n
     U.tables = n > 1;
     U.o_wseg = place(U.tables ? nwords_all / 64 * sizeof(uint32_t) : 0);
     U.o_bseg = place(U.tables ? nbricks_all / 128 * sizeof(uint32_t) : 0);
+    // brick-major positions of several solids: a table of their own
+    U.npos_all = U.tables && jobs[0].npos ? jobs.back().pos0 + jobs.back().npos : 0;
+    U.o_pseg = place(U.npos_all / 64 * sizeof(uint32_t));
     U.o_conj = place(jobs[0].conj.size() * sizeof(double));
     U.po.resize(n);
     U.nodes.resize(n);
@@ -842,6 +847,7 @@ void fill_upload(std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_a
         if (j.cap) sg.B.info = b.binfo, sg.B.code = b.bcode;
         sg.word0 = j.word0, sg.nwords = j.nwords;
         sg.brick0 = j.brick0, sg.nbricks = j.nbricks;
+        sg.pos0 = j.pos0;
         sg.iters = j.iters;
         sg.stage = j.cap && plan_stage_bytes(P) <= PLAN_STAGE_LIMIT ? 1 : 0;
     };
@@ -862,12 +868,18 @@ void fill_upload(std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_a
             const uint64_t b1 = s + 1 < n ? jobs[s + 1].brick0 : nbricks_all;
             for (uint64_t bb = jobs[s].brick0; bb < b1; bb += 128) bs[bb / 128] = (uint32_t)s;
         }
+        if (U.npos_all) {
+            uint32_t *ps = (uint32_t *)(h + U.o_pseg);
+            for (size_t s = 0; s < n; s++)
+                for (uint64_t q = jobs[s].pos0; q < jobs[s].pos0 + jobs[s].npos; q += 64) ps[q / 64] = (uint32_t)s;
+        }
     }
     U.ctl = (Ctl *)dv;
     U.sout = (SegOut *)(dv + U.o_sout);
     U.segs = (Seg *)(dv + U.o_segs);
     U.word_seg = U.tables ? (uint32_t *)(dv + U.o_wseg) : nullptr;
     U.brick_seg = U.tables ? (uint32_t *)(dv + U.o_bseg) : nullptr;
+    U.pos_seg = U.npos_all ? (uint32_t *)(dv + U.o_pseg) : U.word_seg;
 }
 
 // The colliders of every job on this context's device.
@@ -1011,14 +1023,27 @@ int plan_call(std::vector<Job> &jobs, const mc3d_options &o, Call &c) {
         set_error("too many culling bricks");
         return MC3D_E_UNSUPPORTED;
     }
-    // One culled solid: the scan takes its words brick by brick (the
-    // positions include the rows of bricks past the grid).
+    // Culled solids: the scan takes each solid's words brick by brick (the
+    // positions include the rows of bricks past the grid), each solid's
+    // positions padded to whole waves (classify reads a wave's rows from its
+    // own lanes, and a wave stays in one solid).
     c.nscan = c.nwords;
-    const int BY = jobs[0].B.BY, BZ = jobs[0].B.BZ;
-    if (c.cull && c.nsegs == 1 && !c.host && (BY & (BY - 1)) == 0 && (BZ & (BZ - 1)) == 0) {
-        const uint64_t n = c.nbricks * (uint64_t)BY * (uint64_t)BZ;
-        // padded to whole waves: classify reads a wave's rows from its own lanes
-        if (n < (1ull << 32) - 64) c.bm = true, c.nscan = align_up(n, 64);
+    bool bm = c.cull && !c.host;
+    uint64_t np = 0;
+    for (const Job &j : jobs) {
+        const int BY = j.B.BY, BZ = j.B.BZ;
+        bm = bm && j.cap && (BY & (BY - 1)) == 0 && (BZ & (BZ - 1)) == 0;
+        if (bm) np += align_up(j.nbricks * (uint64_t)BY * (uint64_t)BZ, 64);
+    }
+    if (bm && np < (1ull << 32) - 64) {
+        c.bm = true, c.nscan = np;
+        np = 0;
+        for (Job &j : jobs) {
+            j.pos0 = np, j.npos = align_up(j.nbricks * (uint64_t)j.B.BY * (uint64_t)j.B.BZ, 64);
+            np += j.npos;
+        }
+    } else {
+        for (Job &j : jobs) j.pos0 = j.word0, j.npos = 0;
     }
     c.plan_lds = PLAN_STACK_BYTES;
     if (c.cull)
@@ -1072,6 +1097,7 @@ Pipe make_pipe(const Call &c, const Upload &U) {
     p.seg0 = U.hsegs[0];
     p.word_seg = U.word_seg;
     p.brick_seg = U.brick_seg;
+    p.pos_seg = U.pos_seg;
     p.ctl = U.ctl;
     p.sout = U.sout;
     p.nwords = c.nwords;

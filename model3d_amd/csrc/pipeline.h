@@ -77,6 +77,7 @@ struct Seg {
     Bricks B;
     uint64_t word0, nwords;    // global occupancy words [word0, word0 + nwords); word0 % 64 == 0
     uint64_t brick0, nbricks;  // global bricks; brick0 % 128 == 0
+    uint64_t pos0;             // first scan position (brick-major calls), else word0; pos0 % 64 == 0
     int32_t iters;
     int32_t stage;  // the plan may stage this segment's program in LDS
 };
