@@ -814,6 +814,16 @@ void fill_upload(std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_a
         Job &j = jobs[s];
         const Program &P = j.low->P;
         const Upload::Off &po = U.po[s];
+        if (U.tables) {  // this solid's entries of the segment tables (disjoint per solid)
+            uint32_t *ws = (uint32_t *)(h + U.o_wseg), *bs = (uint32_t *)(h + U.o_bseg);
+            const uint64_t w1 = s + 1 < n ? jobs[s + 1].word0 : nwords_all;
+            for (uint64_t w = j.word0; w < w1; w += 64) ws[w / 64] = (uint32_t)s;
+            const uint64_t b1 = s + 1 < n ? jobs[s + 1].brick0 : nbricks_all;
+            for (uint64_t bb = j.brick0; bb < b1; bb += 128) bs[bb / 128] = (uint32_t)s;
+            uint32_t *ps = (uint32_t *)(h + U.o_pseg);
+            if (U.npos_all)
+                for (uint64_t q = j.pos0; q < j.pos0 + j.npos; q += 64) ps[q / 64] = (uint32_t)s;
+        }
         memcpy(h + po.code, P.code.data(), P.code.size() * sizeof(Insn));
         memset(h + po.f64, 0, (P.f64.size() + MAX_ATOM_PARAMS) * sizeof(double));
         memset(h + po.f32, 0, (P.f32.size() + MAX_ATOM_PARAMS) * sizeof(float));
@@ -859,20 +869,6 @@ void fill_upload(std::vector<Job> &jobs, uint64_t nwords_all, uint64_t nbricks_a
         memcpy(h + U.o_conj, jobs[0].conj.data(), jobs[0].conj.size() * sizeof(double));
         U.conj = (const double *)(dv + U.o_conj);
         U.conj_nops = (uint32_t)(jobs[0].conj.size() / 10);
-    }
-    if (U.tables) {
-        uint32_t *ws = (uint32_t *)(h + U.o_wseg), *bs = (uint32_t *)(h + U.o_bseg);
-        for (size_t s = 0; s < n; s++) {
-            const uint64_t w1 = s + 1 < n ? jobs[s + 1].word0 : nwords_all;
-            for (uint64_t w = jobs[s].word0; w < w1; w += 64) ws[w / 64] = (uint32_t)s;
-            const uint64_t b1 = s + 1 < n ? jobs[s + 1].brick0 : nbricks_all;
-            for (uint64_t bb = jobs[s].brick0; bb < b1; bb += 128) bs[bb / 128] = (uint32_t)s;
-        }
-        if (U.npos_all) {
-            uint32_t *ps = (uint32_t *)(h + U.o_pseg);
-            for (size_t s = 0; s < n; s++)
-                for (uint64_t q = jobs[s].pos0; q < jobs[s].pos0 + jobs[s].npos; q += 64) ps[q / 64] = (uint32_t)s;
-        }
     }
     U.ctl = (Ctl *)dv;
     U.sout = (SegOut *)(dv + U.o_sout);
