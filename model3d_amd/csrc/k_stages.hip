@@ -102,11 +102,15 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *__rest
             if (!whole) out[base + i] = make_uint2(ex.x, ex.y);
             if (list && vals[i].z) {
                 uint32_t w = (uint32_t)(base + i);
-                if (bm) {  // brick-major position -> word (of its solid, pos_seg with several)
-                    const Seg &sg = pos_seg ? segs[pos_seg[w >> SEG_WORD_SHIFT]] : seg0;
+                if (bm && pos_seg) {  // brick-major position -> word of its solid
+                    const Seg &sg = segs[pos_seg[w >> SEG_WORD_SHIFT]];
                     int wx, y, z;
                     bm_word(sg.g, sg.B, (uint32_t)(w - sg.pos0), wx, y, z);
                     w = (uint32_t)(sg.word0 + word_index(sg.g, wx, y, z));
+                } else if (bm) {  // (one solid: position 0 is its first)
+                    int wx, y, z;
+                    bm_word(seg0.g, seg0.B, w, wx, y, z);
+                    w = (uint32_t)word_index(seg0.g, wx, y, z);
                 }
                 list[ex.z] = w;
             }
